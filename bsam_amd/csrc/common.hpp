@@ -1,0 +1,72 @@
+// common.hpp -- data structures shared by the host engine and the gfx950
+// kernels of the FAS multigrid engine.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace bsam {
+
+// Device record of one patch. Field storage keeps the reference layout
+// (column-major, variable slowest, ghost-extended q with lower bound 0 for
+// mbc = 1), so uploads and downloads are plain copies; offsets index the
+// per-field pools.
+struct DPatch {
+  int mx, my, mz;          // interior extents (mz = 1 in 2D)
+  int px, py;              // q row pitch (mx+2) and rows per plane (my+2)
+  int cmx, cmy, cmz;       // coarse extents (mx/2 ...), qc pitch cmx+2
+  int level, parent;       // parent patch index, -1 if none
+  int mb[3][2];            // mbounds in the parent's local coarse indices
+  int pad0;
+  long long q;             // offset of q(0,0,k0,1) in the q and qold pools
+  long long f;             // offset of f(1,1,1,1) in the f/rf/ftmp pools
+  long long qc;            // offset of qc(0,0,k0,1) in the qc pool
+  long long qvs, fvs, cvs; // variable strides of q, f and qc
+  double dx;
+};
+
+// One ghost-cell update of a SetGhost program. Every source is a value that
+// the same program never writes, so all updates of a program run
+// concurrently. kind: 0 copy, 1 and 2 two fine + coarse quadratic
+// interpolation with a single coarse value or a 3-point coarse interpolant,
+// 3 and 4 the 3D face forms (two fine + bilinear coarse, two fine + four
+// coarse with 3-point rows). nflip counts trailing "2 - x" boundary maps.
+struct GhostOp {
+  int dst, dvs;            // q-pool offset (variable 1) and variable stride
+  int kind, nflip;
+  int f0, f1, fvs;         // fine (or copy) sources and their variable stride
+  int cvs;                 // parent variable stride
+  int c[8];                // parent sources
+};
+
+// A tile of cells of one patch: origin (i0, j0, k0) in the patch's index
+// space; the tile extent is fixed per kernel.
+struct Tile {
+  int patch, i0, j0, k0;
+};
+
+struct Problem {
+  double dt, eps2;
+  double fc_alpha, fc_shift, fe_num, fe_den;
+};
+
+// Kernel argument bundle: pool base pointers plus the patch table.
+struct Dev {
+  double *q, *qold, *qc, *f, *rf, *ftmp;
+  const DPatch *P;
+  Problem prm;
+  int nrvars;
+};
+
+// Coarse-fine interpolation weights (lib/boundary.f90:45-54).
+#define BSAM_A1 (-1.0 / 5.0)
+#define BSAM_A2 (2.0 / 3.0)
+#define BSAM_A3 (8.0 / 15.0)
+#define BSAM_B1 (5.0 / 32.0)
+#define BSAM_B2 (15.0 / 16.0)
+#define BSAM_B3 (-3.0 / 32.0)
+#define BSAM_C1 (9.0 / 16.0)
+#define BSAM_C2 (3.0 / 16.0)
+#define BSAM_C3 (3.0 / 16.0)
+#define BSAM_C4 (1.0 / 16.0)
+
+}  // namespace bsam

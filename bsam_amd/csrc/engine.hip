@@ -1,0 +1,678 @@
+// engine.hip -- host side of the FAS multigrid engine: hierarchy commit,
+// ghost-fill programs, level orchestration and the C ABI of bsam_gpu.h.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+#include <unordered_set>
+#include <vector>
+
+#include "bsam_gpu.h"
+#include "ghost.hpp"
+#include "kernels.hpp"
+
+using namespace bsam;
+
+namespace {
+
+struct Error : std::runtime_error {
+  int code;
+  Error(const std::string &m, int c = -1) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCK(x)                                                                    \
+  do {                                                                              \
+    hipError_t e_ = (x);                                                            \
+    if (e_ != hipSuccess)                                                           \
+      throw Error(std::string(#x) + ": " + hipGetErrorString(e_), -2);              \
+  } while (0)
+
+template <class T> T *dev_upload(const std::vector<T> &v) {
+  if (v.empty()) return nullptr;
+  T *p = nullptr;
+  HIPCK(hipMalloc(&p, sizeof(T) * v.size()));
+  HIPCK(hipMemcpy(p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+  return p;
+}
+
+struct LevelData {
+  std::vector<int> patches;  // patch indices in level-list order
+  std::vector<Tile> t_int, t_ext, t_coarse, t_cext;
+  Tile *d_int = nullptr, *d_ext = nullptr, *d_coarse = nullptr, *d_cext = nullptr;
+  std::vector<GhostOp> gp[3];
+  GhostOp *d_gp[3] = {nullptr, nullptr, nullptr};
+  long long cells = 0;
+};
+
+}  // namespace
+
+struct bsam_ctx {
+  bsam_cfg cfg{};
+  std::string err;
+  bool host_only = true;
+  hipStream_t stream = nullptr;
+  bool building = false, committed = false;
+  std::vector<HPatch> hp;
+  std::vector<DPatch> dp;
+  std::vector<LevelData> lev;
+  int finest = 0;
+  long long nq = 0, nc = 0, nf = 0;
+  double *q = nullptr, *qold = nullptr, *qc = nullptr, *f = nullptr, *rf = nullptr,
+         *ftmp = nullptr;
+  DPatch *d_P = nullptr;
+  double *d_partial = nullptr;
+  int partial_cap = 0;
+  double *d_hist = nullptr;
+  int *d_state = nullptr;
+  int *h_state = nullptr;
+
+  int nd() const { return cfg.ndims; }
+  LevelData &L(int level) {
+    if (level < cfg.minlevel || level > finest) throw Error("level out of range");
+    return lev[level - cfg.minlevel];
+  }
+  Dev dev() const {
+    Dev d;
+    d.q = q; d.qold = qold; d.qc = qc; d.f = f; d.rf = rf; d.ftmp = ftmp;
+    d.P = d_P;
+    d.prm = Problem{cfg.dt, cfg.eps2, cfg.fc_alpha, cfg.fc_shift, cfg.fe_num, cfg.fe_den};
+    d.nrvars = cfg.nrvars;
+    return d;
+  }
+  void need_device() const {
+    if (host_only) throw Error("host-only context: no device");
+    if (!committed) throw Error("hierarchy not committed");
+  }
+  void free_device();
+  ~bsam_ctx() { free_device(); }
+};
+
+void bsam_ctx::free_device() {
+  if (host_only) return;
+  auto F = [](void *p) {
+    if (p) (void)hipFree(p);
+  };
+  F(q); F(qold); F(qc); F(f); F(rf); F(ftmp); F(d_P); F(d_partial); F(d_hist);
+  F(d_state);
+  q = qold = qc = f = rf = ftmp = nullptr;
+  d_P = nullptr; d_partial = nullptr; d_hist = nullptr; d_state = nullptr;
+  for (auto &l : lev) {
+    F(l.d_int); F(l.d_ext); F(l.d_coarse); F(l.d_cext);
+    for (int p = 0; p < 3; p++) F(l.d_gp[p]);
+  }
+  if (h_state) (void)hipHostFree(h_state);
+  h_state = nullptr;
+}
+
+static thread_local std::string g_err;
+
+#define API_BEGIN(ctx)                     \
+  if (!(ctx)) {                            \
+    g_err = "null context";                \
+    return -1;                             \
+  }                                        \
+  try {
+#define API_END(ctx)                       \
+  }                                        \
+  catch (const Error &e) {                 \
+    (ctx)->err = e.what();                 \
+    return e.code;                         \
+  }                                        \
+  catch (const std::exception &e) {        \
+    (ctx)->err = e.what();                 \
+    return -1;                             \
+  }                                        \
+  return 0;
+
+// ---- hierarchy commit --------------------------------------------------------
+static void build_tiles(bsam_ctx *c, LevelData &L, int level) {
+  int nd = c->nd();
+  int TX = tile_x(nd), TY = tile_y(nd), TZ = tile_z(nd);
+  auto cover = [&](std::vector<Tile> &out, int p, int lo, int ex, int ey, int ez) {
+    // tiles over [lo, lo+ex) x [lo, lo+ey) x [lo, lo+ez) (z fixed 0 in 2D)
+    for (int k = (nd == 3 ? lo : 0); k < (nd == 3 ? lo + ez : 1); k += TZ)
+      for (int j = lo; j < lo + ey; j += TY)
+        for (int i = lo; i < lo + ex; i += TX) out.push_back(Tile{p, i, j, k});
+  };
+  for (int p : L.patches) {
+    const HPatch &h = c->hp[p];
+    cover(L.t_int, p, 1, h.mx[0], h.mx[1], h.mx[2]);
+    cover(L.t_ext, p, 0, h.mx[0] + 2, h.mx[1] + 2, h.mx[2] + 2);
+    if (level > c->cfg.minlevel) {
+      cover(L.t_coarse, p, 1, h.cmx[0], h.cmx[1], h.cmx[2]);
+      cover(L.t_cext, p, 0, h.cmx[0] + 2, h.cmx[1] + 2, h.cmx[2] + 2);
+    }
+    L.cells += (long long)h.mx[0] * h.mx[1] * (nd == 3 ? h.mx[2] : 1);
+  }
+}
+
+static void commit(bsam_ctx *c) {
+  const int nd = c->nd(), nv = c->cfg.nrvars;
+  int n = (int)c->hp.size();
+  if (n == 0) throw Error("empty hierarchy");
+  int minl = c->cfg.minlevel;
+  c->finest = minl;
+  for (auto &h : c->hp) c->finest = std::max(c->finest, h.level);
+  c->lev.assign(c->finest - minl + 1, LevelData());
+  int prev = minl - 1;
+  for (int p = 0; p < n; p++) {
+    HPatch &h = c->hp[p];
+    if (h.level < prev) throw Error("patches must be added with levels ascending");
+    prev = h.level;
+    if (h.level < minl) throw Error("patch level below minlevel");
+    c->lev[h.level - minl].patches.push_back(p);
+    if (h.level > minl) {
+      if (h.parent < 0 || h.parent >= p || c->hp[h.parent].level != h.level - 1)
+        throw Error("patch " + std::to_string(p) + ": parent must be an earlier patch one level coarser");
+      const HPatch &pa = c->hp[h.parent];
+      for (int d = 0; d < nd; d++) {
+        if (h.mx[d] != 2 * (h.mb[d][1] - h.mb[d][0] + 1))
+          throw Error("patch " + std::to_string(p) + ": mx != 2*(mbounds extent)");
+        if (h.mb[d][0] < 1 || h.mb[d][1] > pa.mx[d])
+          throw Error("patch " + std::to_string(p) + ": mbounds outside the parent");
+      }
+    } else if (h.parent != -1) {
+      throw Error("coarsest patch must have parent -1");
+    }
+    for (int d = 0; d < nd; d++)
+      if (h.mx[d] < 1) throw Error("mx must be positive");
+    for (int fc = 0; fc < 2 * nd; fc++) {
+      int t = h.mthbc[fc];
+      if (t == 2) throw Error("periodic boundaries (mthbc = 2) are not supported");
+      if (!(t == 1 || t == 999 || (t >= 10 && t <= 99)))
+        throw Error("unsupported boundary code " + std::to_string(t));
+    }
+  }
+  for (int l = minl; l <= c->finest; l++) {
+    auto &P = c->lev[l - minl].patches;
+    if (P.empty()) throw Error("level " + std::to_string(l) + " has no patches");
+    if (l <= 0 && P.size() != 1) throw Error("levels <= 0 hold one patch");
+    // same-level interiors must not overlap (GhostOverlap copies assume it)
+    for (size_t a = 0; a < P.size(); a++)
+      for (size_t b = a + 1; b < P.size(); b++) {
+        const HPatch &A = c->hp[P[a]], &B = c->hp[P[b]];
+        bool ov = true;
+        for (int d = 0; d < nd; d++)
+          ov = ov && A.mg[d][0] <= B.mg[d][1] && B.mg[d][0] <= A.mg[d][1];
+        if (ov) throw Error("overlapping patches on level " + std::to_string(l));
+      }
+  }
+  // pool offsets
+  c->nq = c->nc = c->nf = 0;
+  c->dp.resize(n);
+  for (int p = 0; p < n; p++) {
+    HPatch &h = c->hp[p];
+    DPatch &D = c->dp[p];
+    memset(&D, 0, sizeof D);
+    long long ex = h.mx[0] + 2, ey = h.mx[1] + 2, ez = nd == 3 ? h.mx[2] + 2 : 1;
+    long long cx = h.cmx[0] + 2, cy = h.cmx[1] + 2, cz = nd == 3 ? h.cmx[2] + 2 : 1;
+    long long fn = (long long)h.mx[0] * h.mx[1] * (nd == 3 ? h.mx[2] : 1);
+    D.mx = h.mx[0]; D.my = h.mx[1]; D.mz = nd == 3 ? h.mx[2] : 1;
+    D.px = (int)ex; D.py = (int)ey;
+    D.cmx = h.cmx[0]; D.cmy = h.cmx[1]; D.cmz = nd == 3 ? h.cmx[2] : 1;
+    D.level = h.level; D.parent = h.parent;
+    for (int d = 0; d < 3; d++) {
+      D.mb[d][0] = h.mb[d][0];
+      D.mb[d][1] = h.mb[d][1];
+    }
+    D.q = c->nq; D.qvs = ex * ey * ez;
+    D.qc = c->nc; D.cvs = cx * cy * cz;
+    D.f = c->nf; D.fvs = fn;
+    D.dx = h.dx;
+    h.qoff = D.q; h.qvs = D.qvs; h.px = ex; h.py = ey;
+    c->nq += D.qvs * nv;
+    c->nc += D.cvs * nv;
+    c->nf += D.fvs * nv;
+  }
+  if (c->nq >= (1LL << 31)) throw Error("q pool exceeds 2^31 elements");
+  for (int l = minl; l <= c->finest; l++) {
+    LevelData &L = c->lev[l - minl];
+    build_tiles(c, L, l);
+    for (int ip = 0; ip < 3; ip++) {
+      std::string msg;
+      if (build_ghost_program(c->hp, L.patches, nd, l, ip, L.gp[ip], msg))
+        throw Error("ghost program level " + std::to_string(l) + ": " + msg);
+    }
+  }
+  if (c->host_only) return;
+  // device allocation
+  c->free_device();
+  HIPCK(hipMalloc(&c->q, sizeof(double) * c->nq));
+  HIPCK(hipMalloc(&c->qold, sizeof(double) * c->nq));
+  HIPCK(hipMalloc(&c->qc, sizeof(double) * c->nc));
+  HIPCK(hipMalloc(&c->f, sizeof(double) * c->nf));
+  HIPCK(hipMalloc(&c->rf, sizeof(double) * c->nf));
+  HIPCK(hipMalloc(&c->ftmp, sizeof(double) * c->nf));
+  HIPCK(hipMemset(c->q, 0, sizeof(double) * c->nq));
+  HIPCK(hipMemset(c->qold, 0, sizeof(double) * c->nq));
+  HIPCK(hipMemset(c->qc, 0, sizeof(double) * c->nc));
+  HIPCK(hipMemset(c->f, 0, sizeof(double) * c->nf));
+  HIPCK(hipMemset(c->rf, 0, sizeof(double) * c->nf));
+  HIPCK(hipMemset(c->ftmp, 0, sizeof(double) * c->nf));
+  c->d_P = dev_upload(c->dp);
+  size_t maxt = 1;
+  for (auto &L : c->lev) {
+    L.d_int = dev_upload(L.t_int);
+    L.d_ext = dev_upload(L.t_ext);
+    L.d_coarse = dev_upload(L.t_coarse);
+    L.d_cext = dev_upload(L.t_cext);
+    for (int ip = 0; ip < 3; ip++) L.d_gp[ip] = dev_upload(L.gp[ip]);
+    maxt = std::max(maxt, L.t_int.size());
+  }
+  c->partial_cap = (int)maxt;
+  HIPCK(hipMalloc(&c->d_partial, sizeof(double) * maxt));
+  int hmax = std::max(c->cfg.maxvcycles, 1) + 1;
+  HIPCK(hipMalloc(&c->d_hist, sizeof(double) * hmax));
+  HIPCK(hipMalloc(&c->d_state, sizeof(int) * 4));
+  HIPCK(hipHostMalloc(&c->h_state, sizeof(int) * 4));
+  HIPCK(hipDeviceSynchronize());
+}
+
+// ---- level operations ------------------------------------------------------------
+static void op_set_ghost(bsam_ctx *c, int level, int ipass) {
+  LevelData &L = c->L(level);
+  if (ipass < 0 || ipass > 2) throw Error("ipass must be 0, 1 or 2");
+  HIPCK(launch_ghost(c->dev(), L.d_gp[ipass], (int)L.gp[ipass].size(), c->stream));
+}
+static void op_relax(bsam_ctx *c, int level, int color) {
+  LevelData &L = c->L(level);
+  HIPCK(launch_relax(c->nd(), c->dev(), L.d_int, (int)L.t_int.size(), color, c->stream));
+}
+static LevelData &child_level(bsam_ctx *c, int level) {
+  if (level <= c->cfg.minlevel) throw Error("level has no parent level");
+  return c->L(level);
+}
+static void op_restrict(bsam_ctx *c, int level) {
+  LevelData &L = child_level(c, level);
+  HIPCK(launch_restrict(c->nd(), c->dev(), L.d_coarse, (int)L.t_coarse.size(), c->stream));
+}
+static void op_coarse_ghost(bsam_ctx *c, int level) {
+  LevelData &L = child_level(c, level);
+  HIPCK(launch_coarse_ghost(c->nd(), c->dev(), L.d_cext, (int)L.t_cext.size(), c->stream));
+}
+static void op_coarse_load(bsam_ctx *c, int level) {
+  LevelData &L = child_level(c, level);
+  HIPCK(launch_coarse_load(c->nd(), c->dev(), L.d_coarse, (int)L.t_coarse.size(), c->stream));
+}
+static void op_correct(bsam_ctx *c, int level) {
+  LevelData &L = child_level(c, level);
+  HIPCK(launch_correct(c->nd(), c->dev(), L.d_ext, (int)L.t_ext.size(), c->stream));
+}
+// FillDown, afasroutines.f90:223-239
+static void op_fill_down(bsam_ctx *c, int field) {
+  if (field != BSAM_FIELD_F && field != BSAM_FIELD_Q) throw Error("fill_down: field must be F or Q");
+  for (int level = c->finest; level >= c->cfg.minlevel + 1; level--) {
+    LevelData &L = c->L(level);
+    HIPCK(launch_filldown(c->nd(), c->dev(), L.d_coarse, (int)L.t_coarse.size(),
+                          field == BSAM_FIELD_F ? 1 : 2, c->stream));
+    if (field == BSAM_FIELD_Q) op_set_ghost(c, level - 1, 0);
+  }
+}
+static void op_source(bsam_ctx *c, int full) {
+  for (int level = c->finest; level >= 0; level--) {
+    LevelData &L = c->L(level);
+    HIPCK(launch_source(c->nd(), c->dev(), L.d_int, (int)L.t_int.size(), full, c->stream));
+  }
+}
+static void level_relax(bsam_ctx *c, int level) {
+  for (int pass = 1; pass <= c->cfg.nsmoothingpasses; pass++)
+    for (int color = 1; color <= 2; color++) {
+      op_relax(c, level, color);
+      op_set_ghost(c, level, color);
+    }
+}
+// AFASVcycle, afasroutines.f90:488-521
+static void op_vcycle(bsam_ctx *c, int level) {
+  level_relax(c, level);
+  if (level > c->cfg.minlevel) {
+    op_restrict(c, level);
+    op_set_ghost(c, level - 1, 0);
+    op_coarse_ghost(c, level);
+    op_coarse_load(c, level);
+    op_vcycle(c, level - 1);
+    op_correct(c, level);
+    op_set_ghost(c, level, 0);
+    level_relax(c, level);
+  }
+}
+static void reset_state(bsam_ctx *c) {
+  HIPCK(hipMemsetAsync(c->d_state, 0, sizeof(int) * 4, c->stream));
+}
+// queue ErrorAFAS(level): residual norm into d_hist[state[0]]
+static void queue_error(bsam_ctx *c, int level, double tol) {
+  LevelData &L = c->L(level);
+  int nt = (int)L.t_int.size();
+  HIPCK(launch_resnorm(c->nd(), c->dev(), L.d_int, nt, c->d_partial, 0, c->stream));
+  double count = (double)(c->cfg.nrvars * L.cells);
+  HIPCK(launch_finalize(c->d_partial, nt, count, tol, c->d_hist, c->d_state, c->stream));
+}
+
+static void multigrid_iterations(bsam_ctx *c, int *ncycles, double *hist, int maxhist,
+                                 int *converged) {
+  op_fill_down(c, BSAM_FIELD_Q);
+  op_source(c, 1);
+  op_fill_down(c, BSAM_FIELD_F);
+  reset_state(c);
+  int it = 0, done = 0;
+  for (it = 1; it <= c->cfg.maxvcycles; it++) {
+    op_vcycle(c, c->finest);
+    op_fill_down(c, BSAM_FIELD_Q);
+    op_source(c, 0);
+    op_fill_down(c, BSAM_FIELD_F);
+    queue_error(c, c->finest, c->cfg.qerrortol);
+    HIPCK(hipMemcpyAsync(c->h_state, c->d_state, sizeof(int) * 4, hipMemcpyDeviceToHost,
+                         c->stream));
+    HIPCK(hipStreamSynchronize(c->stream));
+    if (c->h_state[1]) {
+      done = 1;
+      break;
+    }
+  }
+  int n = done ? it : c->cfg.maxvcycles;
+  if (ncycles) *ncycles = n;
+  if (converged) *converged = done;
+  if (hist && maxhist > 0) {
+    int m = std::min(n, maxhist);
+    HIPCK(hipMemcpy(hist, c->d_hist, sizeof(double) * m, hipMemcpyDeviceToHost));
+  }
+}
+
+// ---- C ABI ---------------------------------------------------------------------------
+extern "C" {
+
+bsam_ctx *bsam_ctx_create(const bsam_cfg *cfg) {
+  if (!cfg) {
+    g_err = "null config";
+    return nullptr;
+  }
+  if (cfg->ndims != 2 && cfg->ndims != 3) {
+    g_err = "ndims must be 2 or 3";
+    return nullptr;
+  }
+  if (cfg->nrvars != 2) {
+    g_err = "the Cahn-Hilliard engine needs nrvars = 2";
+    return nullptr;
+  }
+  if (cfg->mbc != 1) {
+    g_err = "only mbc = 1 is supported";
+    return nullptr;
+  }
+  if (cfg->minlevel > 0) {
+    g_err = "minlevel must be <= 0";
+    return nullptr;
+  }
+  if (cfg->precision != BSAM_PREC_FP64 && cfg->precision != BSAM_PREC_MIXED) {
+    g_err = "unknown precision";
+    return nullptr;
+  }
+  bsam_ctx *c = new (std::nothrow) bsam_ctx();
+  if (!c) {
+    g_err = "out of memory";
+    return nullptr;
+  }
+  c->cfg = *cfg;
+  c->host_only = cfg->device < 0;
+  if (!c->host_only) {
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || cfg->device >= ndev) {
+      g_err = "no such HIP device";
+      delete c;
+      return nullptr;
+    }
+    if (hipSetDevice(cfg->device) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+      g_err = "cannot create a stream on the device";
+      delete c;
+      return nullptr;
+    }
+  }
+  return c;
+}
+
+int bsam_ctx_destroy(bsam_ctx *c) {
+  if (!c) return 0;
+  if (!c->host_only) {
+    (void)hipSetDevice(c->cfg.device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    c->free_device();
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+  }
+  delete c;
+  return 0;
+}
+
+const char *bsam_last_error(const bsam_ctx *c) { return c ? c->err.c_str() : g_err.c_str(); }
+
+int bsam_hierarchy_begin(bsam_ctx *c) {
+  API_BEGIN(c)
+  if (!c->host_only) HIPCK(hipStreamSynchronize(c->stream));
+  c->free_device();
+  c->hp.clear();
+  c->dp.clear();
+  c->lev.clear();
+  c->committed = false;
+  c->building = true;
+  API_END(c)
+}
+
+int bsam_add_patch(bsam_ctx *c, int id, int level, int parent, const int *mx,
+                   const int *mbounds, const int *mglobal, const int *mthbc, double dx,
+                   const double *xlower) {
+  API_BEGIN(c)
+  if (!c->building) throw Error("call bsam_hierarchy_begin first");
+  if (!mx || !mbounds || !mglobal || !mthbc) throw Error("null geometry");
+  HPatch h;
+  int nd = c->nd();
+  h.id = id;
+  h.level = level;
+  h.parent = parent;
+  for (int d = 0; d < 3; d++) {
+    h.mx[d] = d < nd ? mx[d] : 1;
+    h.mb[d][0] = d < nd ? mbounds[2 * d] : 1;
+    h.mb[d][1] = d < nd ? mbounds[2 * d + 1] : 1;
+    h.mg[d][0] = d < nd ? mglobal[2 * d] : 1;
+    h.mg[d][1] = d < nd ? mglobal[2 * d + 1] : 1;
+    h.cmx[d] = d < nd ? (h.mx[d] % 2 == 1 ? 1 : h.mx[d] / 2) : 1;
+    h.xlower[d] = (xlower && d < nd) ? xlower[d] : 0.0;
+  }
+  for (int fc = 0; fc < 6; fc++) h.mthbc[fc] = fc < 2 * nd ? mthbc[fc] : 1;
+  h.dx = dx;
+  c->hp.push_back(h);
+  API_END(c)
+}
+
+int bsam_hierarchy_commit(bsam_ctx *c) {
+  API_BEGIN(c)
+  if (!c->building) throw Error("call bsam_hierarchy_begin first");
+  if (!c->host_only) HIPCK(hipSetDevice(c->cfg.device));
+  commit(c);
+  c->building = false;
+  c->committed = true;
+  API_END(c)
+}
+
+int bsam_num_patches(const bsam_ctx *c) { return c ? (int)c->hp.size() : -1; }
+
+static void field_ptr(bsam_ctx *c, int patch, int field, double **base, long long *off,
+                      long long *n) {
+  if (patch < 0 || patch >= (int)c->hp.size()) throw Error("patch index out of range");
+  const DPatch &D = c->dp[patch];
+  int nv = c->cfg.nrvars;
+  switch (field) {
+    case BSAM_FIELD_Q: *base = c->q; *off = D.q; *n = D.qvs * nv; break;
+    case BSAM_FIELD_QOLD: *base = c->qold; *off = D.q; *n = D.qvs * nv; break;
+    case BSAM_FIELD_QC: *base = c->qc; *off = D.qc; *n = D.cvs * nv; break;
+    case BSAM_FIELD_F: *base = c->f; *off = D.f; *n = D.fvs * nv; break;
+    case BSAM_FIELD_FTMP: *base = c->ftmp; *off = D.f; *n = D.fvs * nv; break;
+    case BSAM_FIELD_RF: *base = c->rf; *off = D.f; *n = D.fvs * nv; break;
+    default: throw Error("unknown field");
+  }
+}
+
+long bsam_field_len(const bsam_ctx *cc, int patch, int field) {
+  bsam_ctx *c = const_cast<bsam_ctx *>(cc);
+  if (!c || !c->committed) return -1;
+  try {
+    double *b;
+    long long off, n;
+    field_ptr(c, patch, field, &b, &off, &n);
+    return (long)n;
+  } catch (...) {
+    return -1;
+  }
+}
+
+int bsam_upload(bsam_ctx *c, int patch, int field, const double *host, long n) {
+  API_BEGIN(c)
+  c->need_device();
+  double *b;
+  long long off, len;
+  field_ptr(c, patch, field, &b, &off, &len);
+  if (n != len) throw Error("upload: size mismatch");
+  HIPCK(hipMemcpyAsync(b + off, host, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+  HIPCK(hipStreamSynchronize(c->stream));
+  API_END(c)
+}
+
+int bsam_download(bsam_ctx *c, int patch, int field, double *host, long n) {
+  API_BEGIN(c)
+  c->need_device();
+  double *b;
+  long long off, len;
+  field_ptr(c, patch, field, &b, &off, &len);
+  if (n != len) throw Error("download: size mismatch");
+  HIPCK(hipMemcpyAsync(host, b + off, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  HIPCK(hipStreamSynchronize(c->stream));
+  API_END(c)
+}
+
+int bsam_set_ghost(bsam_ctx *c, int level, int ipass) {
+  API_BEGIN(c)
+  c->need_device();
+  op_set_ghost(c, level, ipass);
+  API_END(c)
+}
+int bsam_relax(bsam_ctx *c, int level, int color) {
+  API_BEGIN(c)
+  c->need_device();
+  if (color != 1 && color != 2) throw Error("color must be 1 or 2");
+  op_relax(c, level, color);
+  API_END(c)
+}
+int bsam_restrict_solution(bsam_ctx *c, int level) {
+  API_BEGIN(c)
+  c->need_device();
+  op_restrict(c, level);
+  API_END(c)
+}
+int bsam_coarse_ghost_points(bsam_ctx *c, int level) {
+  API_BEGIN(c)
+  c->need_device();
+  op_coarse_ghost(c, level);
+  API_END(c)
+}
+int bsam_coarse_load(bsam_ctx *c, int level) {
+  API_BEGIN(c)
+  c->need_device();
+  op_coarse_load(c, level);
+  API_END(c)
+}
+int bsam_correct_fine(bsam_ctx *c, int level) {
+  API_BEGIN(c)
+  c->need_device();
+  op_correct(c, level);
+  API_END(c)
+}
+int bsam_fill_down(bsam_ctx *c, int field) {
+  API_BEGIN(c)
+  c->need_device();
+  op_fill_down(c, field);
+  API_END(c)
+}
+int bsam_source(bsam_ctx *c) {
+  API_BEGIN(c)
+  c->need_device();
+  op_source(c, 1);
+  API_END(c)
+}
+int bsam_vcycle(bsam_ctx *c, int level) {
+  API_BEGIN(c)
+  c->need_device();
+  c->L(level);
+  op_vcycle(c, level);
+  API_END(c)
+}
+int bsam_error_afas(bsam_ctx *c, int level, double *residual) {
+  API_BEGIN(c)
+  c->need_device();
+  reset_state(c);
+  queue_error(c, level, -1.0);
+  double r = 0.0;
+  HIPCK(hipMemcpyAsync(&r, c->d_hist, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCK(hipStreamSynchronize(c->stream));
+  if (residual) *residual = r;
+  API_END(c)
+}
+int bsam_copy_qold(bsam_ctx *c) {
+  API_BEGIN(c)
+  c->need_device();
+  HIPCK(launch_copy(c->qold, c->q, c->nq, c->stream));
+  API_END(c)
+}
+int bsam_multigrid_iterations(bsam_ctx *c, int *ncycles, double *hist, int maxhist,
+                              int *converged) {
+  API_BEGIN(c)
+  c->need_device();
+  multigrid_iterations(c, ncycles, hist, maxhist, converged);
+  API_END(c)
+}
+int bsam_time_step(bsam_ctx *c, int *ncycles, double *hist, int maxhist, int *converged) {
+  API_BEGIN(c)
+  c->need_device();
+  for (int level = 0; level <= c->finest; level++) op_set_ghost(c, level, 0);
+  op_fill_down(c, BSAM_FIELD_Q);
+  HIPCK(launch_copy(c->qold, c->q, c->nq, c->stream));
+  multigrid_iterations(c, ncycles, hist, maxhist, converged);
+  API_END(c)
+}
+int bsam_sync(bsam_ctx *c) {
+  API_BEGIN(c)
+  if (!c->host_only) HIPCK(hipStreamSynchronize(c->stream));
+  API_END(c)
+}
+
+int bsam_ghost_program_size(const bsam_ctx *cc, int level, int ipass) {
+  bsam_ctx *c = const_cast<bsam_ctx *>(cc);
+  if (!c || !c->committed || ipass < 0 || ipass > 2) return -1;
+  if (level < c->cfg.minlevel || level > c->finest) return -1;
+  return (int)c->L(level).gp[ipass].size();
+}
+
+int bsam_ghost_program_dst(const bsam_ctx *cc, int level, int ipass, int *out, int maxn) {
+  bsam_ctx *c = const_cast<bsam_ctx *>(cc);
+  API_BEGIN(c)
+  if (!c->committed) throw Error("hierarchy not committed");
+  if (ipass < 0 || ipass > 2) throw Error("bad ipass");
+  auto &g = c->L(level).gp[ipass];
+  int n = std::min((int)g.size(), maxn);
+  for (int e = 0; e < n; e++) {
+    int p, i, j, k;
+    locate_q_cell(c->hp, g[e].dst, p, i, j, k);
+    out[4 * e] = p; out[4 * e + 1] = i; out[4 * e + 2] = j; out[4 * e + 3] = k;
+  }
+  API_END(c)
+}
+
+int bsam_ghost_program_apply_host(const bsam_ctx *cc, int level, int ipass,
+                                  double *const *q_patches) {
+  bsam_ctx *c = const_cast<bsam_ctx *>(cc);
+  API_BEGIN(c)
+  if (!c->committed) throw Error("hierarchy not committed");
+  if (ipass < 0 || ipass > 2) throw Error("bad ipass");
+  apply_ghost_program_host(c->hp, c->L(level).gp[ipass], c->cfg.nrvars, q_patches);
+  API_END(c)
+}
+
+}  // extern "C"

@@ -1,0 +1,660 @@
+// ghost.cpp -- ghost-fill programs.
+//
+// SetGhost(level, ipass) (boundary.f90:58-89) runs four ordered phases per
+// level: coarse-fine interpolation of the colour-selected ghosts
+// (Interpolate2D/3D), same-level sibling copies (GhostOverlap over
+// ApplyOnLevelPairs), periodic copies, then the physical boundary faces in
+// order 1..2*ndims. Later phases overwrite earlier ones and boundary faces read
+// ghosts written before them, so the order decides the result.
+//
+// Here the phases are executed symbolically, once per hierarchy, on the host:
+// each ghost cell ends with an expression over values that the program never
+// writes (interior cells, parent cells, sibling interiors, or ghosts this pass
+// leaves untouched). The device then applies all updates of a program in one
+// launch with no ordering between them; the builder checks the no-overlap
+// property and refuses programs that violate it.
+#include "ghost.hpp"
+
+#include <algorithm>
+#include <array>
+#include <cstring>
+#include <initializer_list>
+#include <map>
+#include <unordered_set>
+
+namespace bsam {
+namespace {
+
+struct Cell {
+  int p = -1, i = 0, j = 0, k = 0;
+};
+
+struct Expr {
+  int kind = 0;   // GhostOp kinds
+  Cell f0, f1;
+  Cell c[4];
+  int nflip = 0;
+};
+
+// A Fortran array-section subscript: start, stop, stride (scalars have
+// start == stop, stride 1).
+struct R {
+  int s, e, st;
+  int count() const { return e < s ? 0 : (e - s) / st + 1; }
+};
+inline R S(int x) { return R{x, x, 1}; }
+inline R G(int s, int e, int st) { return R{s, e, st}; }
+using Sec = std::array<R, 3>;
+
+struct PatchMap {
+  int p = -1;
+  std::map<long long, Expr> m;  // ext-linear index -> expression
+};
+
+struct Builder {
+  const std::vector<HPatch> &hp;
+  int nd;
+  std::string &msg;
+  bool ok = true;
+
+  Builder(const std::vector<HPatch> &h, int n, std::string &m) : hp(h), nd(n), msg(m) {}
+
+  long long key(const HPatch &h, int i, int j, int k) const {
+    long long ex = h.mx[0] + 2, ey = h.mx[1] + 2;
+    return i + ex * (j + ey * (long long)k);
+  }
+  Cell cell_of(const HPatch &h, int p, long long key) const {
+    long long ex = h.mx[0] + 2, ey = h.mx[1] + 2;
+    Cell c;
+    c.p = p;
+    c.i = (int)(key % ex);
+    c.j = (int)((key / ex) % ey);
+    c.k = (int)(key / (ex * ey));
+    return c;
+  }
+  bool interior(const HPatch &h, int i, int j, int k) const {
+    return i >= 1 && i <= h.mx[0] && j >= 1 && j <= h.mx[1] &&
+           (nd == 2 || (k >= 1 && k <= h.mx[2]));
+  }
+
+  // One array-section assignment of Interpolate2D/3D: dst(d) = f(a, b, cs...)
+  // with the subscripts of every operand advancing together.
+  void put(PatchMap &pm, int kind, const Sec &d, const Sec &a, const Sec &b,
+           std::initializer_list<Sec> cs) {
+    const HPatch &h = hp[pm.p];
+    int par = h.parent;
+    int n[3];
+    for (int x = 0; x < 3; x++) {
+      n[x] = d[x].count();
+      auto chk = [&](const Sec &s) {
+        if (s[x].count() != n[x]) {
+          ok = false;
+          msg = "non-conforming interpolation section";
+        }
+      };
+      chk(a);
+      chk(b);
+      for (const Sec &s : cs) chk(s);
+    }
+    if (!ok) return;
+    for (int z = 0; z < n[2]; z++)
+      for (int y = 0; y < n[1]; y++)
+        for (int x = 0; x < n[0]; x++) {
+          int pos[3] = {x, y, z};
+          auto at = [&](const Sec &s, int p) {
+            Cell c;
+            c.p = p;
+            c.i = s[0].s + s[0].st * pos[0];
+            c.j = s[1].s + s[1].st * pos[1];
+            c.k = s[2].s + s[2].st * pos[2];
+            return c;
+          };
+          Cell dc = at(d, pm.p);
+          Expr e;
+          e.kind = kind;
+          e.f0 = at(a, pm.p);
+          e.f1 = at(b, pm.p);
+          int q = 0;
+          for (const Sec &s : cs) e.c[q++] = at(s, par);
+          pm.m[key(h, dc.i, dc.j, dc.k)] = e;
+        }
+  }
+
+  // Interpolate2D, boundary.f90:135-323.
+  void interp2d(PatchMap &pm, int ipass) {
+    const HPatch &h = hp[pm.p];
+    int mx = h.mx[0], my = h.mx[1];
+    int mb11 = h.mb[0][0], mb12 = h.mb[0][1], mb21 = h.mb[1][0], mb22 = h.mb[1][1];
+    bool red = ipass == 1 || ipass == 0, black = ipass == 2 || ipass == 0;
+    auto P2 = [](R a, R b) { return Sec{a, b, S(0)}; };
+    if (red) {
+      put(pm, 1, P2(S(mx + 1), S(my + 1)), P2(S(mx - 1), S(my - 1)), P2(S(mx), S(my)),
+          {P2(S(mb12 + 1), S(mb22 + 1))});
+      put(pm, 1, P2(S(0), S(0)), P2(S(2), S(2)), P2(S(1), S(1)),
+          {P2(S(mb11 - 1), S(mb21 - 1))});
+    }
+    if (black) {
+      put(pm, 1, P2(S(mx + 1), S(0)), P2(S(mx - 1), S(2)), P2(S(mx), S(1)),
+          {P2(S(mb12 + 1), S(mb21 - 1))});
+      put(pm, 1, P2(S(0), S(my + 1)), P2(S(2), S(my - 1)), P2(S(1), S(my)),
+          {P2(S(mb11 - 1), S(mb22 + 1))});
+    }
+    auto internal = [&](int f) { return h.mthbc[f] == 2 || h.mthbc[f] == 999; };
+    if (internal(0)) {
+      if (red) {
+        put(pm, 1, P2(S(0), G(2, my - 2, 2)), P2(S(2), G(4, my, 2)), P2(S(1), G(3, my - 1, 2)),
+            {P2(S(mb11 - 1), G(mb21, mb22 - 1, 1))});
+        put(pm, 2, P2(S(0), S(my)), P2(S(2), S(my)), P2(S(1), S(my)),
+            {P2(S(mb11 - 1), S(mb22 + 1)), P2(S(mb11 - 1), S(mb22)),
+             P2(S(mb11 - 1), S(mb22 - 1))});
+      }
+      if (black) {
+        put(pm, 1, P2(S(0), G(3, my - 1, 2)), P2(S(2), G(1, my - 3, 2)),
+            P2(S(1), G(2, my - 2, 2)), {P2(S(mb11 - 1), G(mb21 + 1, mb22, 1))});
+        put(pm, 2, P2(S(0), S(1)), P2(S(2), S(1)), P2(S(1), S(1)),
+            {P2(S(mb11 - 1), S(mb21 - 1)), P2(S(mb11 - 1), S(mb21)),
+             P2(S(mb11 - 1), S(mb21 + 1))});
+      }
+    }
+    if (internal(1)) {
+      if (red) {
+        put(pm, 1, P2(S(mx + 1), G(3, my - 1, 2)), P2(S(mx - 1), G(1, my - 3, 2)),
+            P2(S(mx), G(2, my - 2, 2)), {P2(S(mb12 + 1), G(mb21 + 1, mb22, 1))});
+        put(pm, 2, P2(S(mx + 1), S(1)), P2(S(mx - 1), S(1)), P2(S(mx), S(1)),
+            {P2(S(mb12 + 1), S(mb21 - 1)), P2(S(mb12 + 1), S(mb21)),
+             P2(S(mb12 + 1), S(mb21 + 1))});
+      }
+      if (black) {
+        put(pm, 1, P2(S(mx + 1), G(2, my - 2, 2)), P2(S(mx - 1), G(4, my, 2)),
+            P2(S(mx), G(3, my - 1, 2)), {P2(S(mb12 + 1), G(mb21, mb22 - 1, 1))});
+        put(pm, 2, P2(S(mx + 1), S(my)), P2(S(mx - 1), S(my)), P2(S(mx), S(my)),
+            {P2(S(mb12 + 1), S(mb22 + 1)), P2(S(mb12 + 1), S(mb22)),
+             P2(S(mb12 + 1), S(mb22 - 1))});
+      }
+    }
+    if (internal(2)) {
+      if (red) {
+        put(pm, 1, P2(G(2, mx - 2, 2), S(0)), P2(G(4, mx, 2), S(2)), P2(G(3, mx - 1, 2), S(1)),
+            {P2(G(mb11, mb12 - 1, 1), S(mb21 - 1))});
+        put(pm, 2, P2(S(mx), S(0)), P2(S(mx), S(2)), P2(S(mx), S(1)),
+            {P2(S(mb12 + 1), S(mb21 - 1)), P2(S(mb12), S(mb21 - 1)),
+             P2(S(mb12 - 1), S(mb21 - 1))});
+      }
+      if (black) {
+        put(pm, 1, P2(G(3, mx - 1, 2), S(0)), P2(G(1, mx - 3, 2), S(2)),
+            P2(G(2, mx - 2, 2), S(1)), {P2(G(mb11 + 1, mb12, 1), S(mb21 - 1))});
+        put(pm, 2, P2(S(1), S(0)), P2(S(1), S(2)), P2(S(1), S(1)),
+            {P2(S(mb11 - 1), S(mb21 - 1)), P2(S(mb11), S(mb21 - 1)),
+             P2(S(mb11 + 1), S(mb21 - 1))});
+      }
+    }
+    if (internal(3)) {
+      if (red) {
+        put(pm, 1, P2(G(3, mx - 1, 2), S(my + 1)), P2(G(1, mx - 3, 2), S(my - 1)),
+            P2(G(2, mx - 2, 2), S(my)), {P2(G(mb11 + 1, mb12, 1), S(mb22 + 1))});
+        put(pm, 2, P2(S(1), S(my + 1)), P2(S(1), S(my - 1)), P2(S(1), S(my)),
+            {P2(S(mb11 - 1), S(mb22 + 1)), P2(S(mb11), S(mb22 + 1)),
+             P2(S(mb11 + 1), S(mb22 + 1))});
+      }
+      if (black) {
+        put(pm, 1, P2(G(2, mx - 2, 2), S(my + 1)), P2(G(4, mx, 2), S(my - 1)),
+            P2(G(3, mx - 1, 2), S(my)), {P2(G(mb11, mb12 - 1, 1), S(mb22 + 1))});
+        put(pm, 2, P2(S(mx), S(my + 1)), P2(S(mx), S(my - 1)), P2(S(mx), S(my)),
+            {P2(S(mb12 + 1), S(mb22 + 1)), P2(S(mb12), S(mb22 + 1)),
+             P2(S(mb12 - 1), S(mb22 + 1))});
+      }
+    }
+  }
+
+  // GetFaceIndex, boundary.f90:1271-1297: {ghost, 2nd fine, 1st fine,
+  // coarse outside, coarse inside} along dimension d for side parity.
+  std::array<int, 5> face_index(const HPatch &h, int d, int parity) const {
+    if (parity == 1) return {0, 2, 1, h.mb[d][0] - 1, h.mb[d][0]};
+    return {h.mx[d] + 1, h.mx[d] - 1, h.mx[d], h.mb[d][1] + 1, h.mb[d][1]};
+  }
+
+  // Interpolate3D, boundary.f90:1299-1799, including the edge test of :1354
+  // that compares the z-face code with 99 rather than 999.
+  void interp3d(PatchMap &pm, int ipass) {
+    const HPatch &h = hp[pm.p];
+    int mx = h.mx[0], my = h.mx[1], mz = h.mx[2];
+    int mb11 = h.mb[0][0], mb12 = h.mb[0][1], mb21 = h.mb[1][0], mb22 = h.mb[1][1],
+        mb31 = h.mb[2][0], mb32 = h.mb[2][1];
+    auto phys = [&](int f) { return h.mthbc[f] != 2 && h.mthbc[f] != 999; };
+    // (1) corners
+    for (int p1 = 1; p1 <= 2; p1++) {
+      if (phys(p1 - 1) && ipass == 1) continue;
+      auto f1 = face_index(h, 0, p1);
+      for (int p2 = 1; p2 <= 2; p2++) {
+        if (phys(p2 + 1) && ipass == 1) continue;
+        auto f2 = face_index(h, 1, p2);
+        for (int p3 = 1; p3 <= 2; p3++) {
+          if (phys(p3 + 3) && ipass == 1) continue;
+          auto f3 = face_index(h, 2, p3);
+          put(pm, 1, Sec{S(f1[0]), S(f2[0]), S(f3[0])}, Sec{S(f1[1]), S(f2[1]), S(f3[1])},
+              Sec{S(f1[2]), S(f2[2]), S(f3[2])}, {Sec{S(f1[3]), S(f2[3]), S(f3[3])}});
+        }
+      }
+    }
+    // (2) edges 1-4: coordinates 1 and 3 fixed
+    for (int p1 = 1; p1 <= 2; p1++) {
+      if (phys(p1 - 1) && ipass == 1) continue;
+      auto f1 = face_index(h, 0, p1);
+      for (int p3 = 1; p3 <= 2; p3++) {
+        if (h.mthbc[p3 + 3] != 2 && h.mthbc[p3 + 3] != 99 && ipass == 1) continue;
+        auto f3 = face_index(h, 2, p3);
+        put(pm, 1, Sec{S(f1[0]), G(3, my - 1, 2), S(f3[0])},
+            Sec{S(f1[1]), G(1, my - 3, 2), S(f3[1])}, Sec{S(f1[2]), G(2, my - 2, 2), S(f3[2])},
+            {Sec{S(f1[3]), G(mb21 + 1, mb22, 1), S(f3[3])}});
+        put(pm, 2, Sec{S(f1[0]), S(1), S(f3[0])}, Sec{S(f1[1]), S(1), S(f3[1])},
+            Sec{S(f1[2]), S(1), S(f3[2])},
+            {Sec{S(f1[3]), S(mb21 - 1), S(f3[3])}, Sec{S(f1[3]), S(mb21), S(f3[3])},
+             Sec{S(f1[3]), S(mb21 + 1), S(f3[3])}});
+        put(pm, 1, Sec{S(f1[0]), G(2, my - 2, 2), S(f3[0])},
+            Sec{S(f1[1]), G(4, my, 2), S(f3[1])}, Sec{S(f1[2]), G(3, my - 1, 2), S(f3[2])},
+            {Sec{S(f1[3]), G(mb21, mb22 - 1, 1), S(f3[3])}});
+        put(pm, 2, Sec{S(f1[0]), S(my), S(f3[0])}, Sec{S(f1[1]), S(my), S(f3[1])},
+            Sec{S(f1[2]), S(my), S(f3[2])},
+            {Sec{S(f1[3]), S(mb22 + 1), S(f3[3])}, Sec{S(f1[3]), S(mb22), S(f3[3])},
+             Sec{S(f1[3]), S(mb22 - 1), S(f3[3])}});
+      }
+    }
+    // edges 5-8: coordinates 1 and 2 fixed
+    for (int p1 = 1; p1 <= 2; p1++) {
+      if (phys(p1 - 1) && ipass == 1) continue;
+      auto f1 = face_index(h, 0, p1);
+      for (int p2 = 1; p2 <= 2; p2++) {
+        if (phys(p2 + 1) && ipass == 1) continue;
+        auto f2 = face_index(h, 1, p2);
+        put(pm, 1, Sec{S(f1[0]), S(f2[0]), G(3, mz - 1, 2)},
+            Sec{S(f1[1]), S(f2[1]), G(1, mz - 3, 2)}, Sec{S(f1[2]), S(f2[2]), G(2, mz - 2, 2)},
+            {Sec{S(f1[3]), S(f2[3]), G(mb31 + 1, mb32, 1)}});
+        put(pm, 2, Sec{S(f1[0]), S(f2[0]), S(1)}, Sec{S(f1[1]), S(f2[1]), S(1)},
+            Sec{S(f1[2]), S(f2[2]), S(1)},
+            {Sec{S(f1[3]), S(f2[3]), S(mb31 - 1)}, Sec{S(f1[3]), S(f2[3]), S(mb31)},
+             Sec{S(f1[3]), S(f2[3]), S(mb31 + 1)}});
+        put(pm, 1, Sec{S(f1[0]), S(f2[0]), G(2, mz - 2, 2)},
+            Sec{S(f1[1]), S(f2[1]), G(4, mz, 2)}, Sec{S(f1[2]), S(f2[2]), G(3, mz - 1, 2)},
+            {Sec{S(f1[3]), S(f2[3]), G(mb31, mb32 - 1, 1)}});
+        put(pm, 2, Sec{S(f1[0]), S(f2[0]), S(mz)}, Sec{S(f1[1]), S(f2[1]), S(mz)},
+            Sec{S(f1[2]), S(f2[2]), S(mz)},
+            {Sec{S(f1[3]), S(f2[3]), S(mb32 + 1)}, Sec{S(f1[3]), S(f2[3]), S(mb32)},
+             Sec{S(f1[3]), S(f2[3]), S(mb32 - 1)}});
+      }
+    }
+    // edges 9-12: coordinates 2 and 3 fixed
+    for (int p2 = 1; p2 <= 2; p2++) {
+      if (phys(p2 + 1) && ipass == 1) continue;
+      auto f2 = face_index(h, 1, p2);
+      for (int p3 = 1; p3 <= 2; p3++) {
+        if (phys(p3 + 3) && ipass == 1) continue;
+        auto f3 = face_index(h, 2, p3);
+        put(pm, 1, Sec{G(3, mx - 1, 2), S(f2[0]), S(f3[0])},
+            Sec{G(1, mx - 3, 2), S(f2[1]), S(f3[1])}, Sec{G(2, mx - 2, 2), S(f2[2]), S(f3[2])},
+            {Sec{G(mb11 + 1, mb12, 1), S(f2[3]), S(f3[3])}});
+        put(pm, 2, Sec{S(1), S(f2[0]), S(f3[0])}, Sec{S(1), S(f2[1]), S(f3[1])},
+            Sec{S(1), S(f2[2]), S(f3[2])},
+            {Sec{S(mb11 - 1), S(f2[3]), S(f3[3])}, Sec{S(mb11), S(f2[3]), S(f3[3])},
+             Sec{S(mb11 + 1), S(f2[3]), S(f3[3])}});
+        put(pm, 1, Sec{G(2, mx - 2, 2), S(f2[0]), S(f3[0])},
+            Sec{G(4, mx, 2), S(f2[1]), S(f3[1])}, Sec{G(3, mx - 1, 2), S(f2[2]), S(f3[2])},
+            {Sec{G(mb11, mb12 - 1, 1), S(f2[3]), S(f3[3])}});
+        put(pm, 2, Sec{S(mx), S(f2[0]), S(f3[0])}, Sec{S(mx), S(f2[1]), S(f3[1])},
+            Sec{S(mx), S(f2[2]), S(f3[2])},
+            {Sec{S(mb12 + 1), S(f2[3]), S(f3[3])}, Sec{S(mb12), S(f2[3]), S(f3[3])},
+             Sec{S(mb12 - 1), S(f2[3]), S(f3[3])}});
+      }
+    }
+    // (3) faces, periodic or internal only
+    const R Y0 = G(2, my - 2, 2), Y1 = G(3, my - 1, 2), YA = G(4, my, 2), YB = G(1, my - 3, 2);
+    const R Z0 = G(2, mz - 2, 2), Z1 = G(3, mz - 1, 2), ZA = G(4, mz, 2), ZB = G(1, mz - 3, 2);
+    const R X0 = G(2, mx - 2, 2), X1 = G(3, mx - 1, 2), XA = G(4, mx, 2), XB = G(1, mx - 3, 2);
+    const R CY0 = G(mb21, mb22 - 1, 1), CY1 = G(mb21 + 1, mb22, 1);
+    const R CZ0 = G(mb31, mb32 - 1, 1), CZ1 = G(mb31 + 1, mb32, 1);
+    const R CX0 = G(mb11, mb12 - 1, 1), CX1 = G(mb11 + 1, mb12, 1);
+    // faces 1 and 2 (coordinate 1 fixed)
+    for (int p1 = 1; p1 <= 2; p1++) {
+      if (phys(p1 - 1)) continue;
+      auto f = face_index(h, 0, p1);
+      R g = S(f[0]), a = S(f[1]), b = S(f[2]), cc = S(f[3]);
+      if (ipass == 0 || (p1 + ipass) % 2 == 0) {
+        // yellow
+        put(pm, 1, Sec{g, Y0, Z0}, Sec{a, YA, ZA}, Sec{b, Y1, Z1}, {Sec{cc, CY0, CZ0}});
+        put(pm, 2, Sec{g, S(my), Z0}, Sec{a, S(my), ZA}, Sec{b, S(my), Z1},
+            {Sec{cc, S(mb22 + 1), CZ0}, Sec{cc, S(mb22), CZ0}, Sec{cc, S(mb22 - 1), CZ0}});
+        put(pm, 2, Sec{g, Y0, S(mz)}, Sec{a, YA, S(mz)}, Sec{b, Y1, S(mz)},
+            {Sec{cc, CY0, S(mb32 + 1)}, Sec{cc, CY0, S(mb32)}, Sec{cc, CY0, S(mb32 - 1)}});
+        // blue
+        put(pm, 1, Sec{g, Y1, Z1}, Sec{a, YB, ZB}, Sec{b, Y0, Z0}, {Sec{cc, CY1, CZ1}});
+        put(pm, 2, Sec{g, Y1, S(1)}, Sec{a, YB, S(1)}, Sec{b, Y0, S(1)},
+            {Sec{cc, CY1, S(mb31 - 1)}, Sec{cc, CY1, S(mb31)}, Sec{cc, CY1, S(mb31 + 1)}});
+        put(pm, 2, Sec{g, S(1), Z1}, Sec{a, S(1), ZB}, Sec{b, S(1), Z0},
+            {Sec{cc, S(mb21 - 1), CZ1}, Sec{cc, S(mb21), CZ1}, Sec{cc, S(mb21 + 1), CZ1}});
+      }
+      if (ipass == 0 || (p1 + ipass) % 2 == 1) {
+        // green
+        put(pm, 1, Sec{g, Y1, Z0}, Sec{a, YB, ZA}, Sec{b, Y0, Z1}, {Sec{cc, CY1, CZ0}});
+        put(pm, 2, Sec{g, S(1), Z0}, Sec{a, S(1), ZA}, Sec{b, S(1), Z1},
+            {Sec{cc, S(mb21 - 1), CZ0}, Sec{cc, S(mb21), CZ0}, Sec{cc, S(mb21 + 1), CZ0}});
+        put(pm, 2, Sec{g, Y1, S(mz)}, Sec{a, YB, S(mz)}, Sec{b, Y0, S(mz)},
+            {Sec{cc, CY1, S(mb32 + 1)}, Sec{cc, CY1, S(mb32)}, Sec{cc, CY1, S(mb32 - 1)}});
+        // red
+        put(pm, 1, Sec{g, Y0, Z1}, Sec{a, YA, ZB}, Sec{b, Y1, Z0}, {Sec{cc, CY0, CZ1}});
+        put(pm, 2, Sec{g, Y0, S(1)}, Sec{a, YA, S(1)}, Sec{b, Y1, S(1)},
+            {Sec{cc, CY0, S(mb31 - 1)}, Sec{cc, CY0, S(mb31)}, Sec{cc, CY0, S(mb31 + 1)}});
+        put(pm, 2, Sec{g, S(my), Z1}, Sec{a, S(my), ZB}, Sec{b, S(my), Z0},
+            {Sec{cc, S(mb22 + 1), CZ1}, Sec{cc, S(mb22), CZ1}, Sec{cc, S(mb22 - 1), CZ1}});
+      }
+      for (int p2 = 1; p2 <= 2; p2++) {
+        auto f2 = face_index(h, 1, p2);
+        for (int p3 = 1; p3 <= 2; p3++) {
+          auto f3 = face_index(h, 2, p3);
+          put(pm, 3, Sec{g, S(f2[2]), S(f3[2])}, Sec{a, S(f2[2]), S(f3[2])},
+              Sec{b, S(f2[2]), S(f3[2])},
+              {Sec{cc, S(f2[4]), S(f3[4])}, Sec{cc, S(f2[3]), S(f3[4])},
+               Sec{cc, S(f2[4]), S(f3[3])}, Sec{cc, S(f2[3]), S(f3[3])}});
+        }
+      }
+    }
+    // faces 3 and 4 (coordinate 2 fixed)
+    for (int p2 = 1; p2 <= 2; p2++) {
+      if (phys(p2 + 1)) continue;
+      auto f = face_index(h, 1, p2);
+      R g = S(f[0]), a = S(f[1]), b = S(f[2]), cc = S(f[3]);
+      if (ipass == 0 || (p2 + ipass) % 2 == 0) {
+        // yellow
+        put(pm, 1, Sec{X0, g, Z0}, Sec{XA, a, ZA}, Sec{X1, b, Z1}, {Sec{CX0, cc, CZ0}});
+        put(pm, 2, Sec{X0, g, S(mz)}, Sec{XA, a, S(mz)}, Sec{X1, b, S(mz)},
+            {Sec{CX0, cc, S(mb32 + 1)}, Sec{CX0, cc, S(mb32)}, Sec{CX0, cc, S(mb32 - 1)}});
+        put(pm, 2, Sec{S(mx), g, Z0}, Sec{S(mx), a, ZA}, Sec{S(mx), b, Z1},
+            {Sec{S(mb12 + 1), cc, CZ0}, Sec{S(mb12), cc, CZ0}, Sec{S(mb12 - 1), cc, CZ0}});
+        // blue
+        put(pm, 1, Sec{X1, g, Z1}, Sec{XB, a, ZB}, Sec{X0, b, Z0}, {Sec{CX1, cc, CZ1}});
+        put(pm, 2, Sec{S(1), g, Z1}, Sec{S(1), a, ZB}, Sec{S(1), b, Z0},
+            {Sec{S(mb11 - 1), cc, CZ1}, Sec{S(mb11), cc, CZ1}, Sec{S(mb11 + 1), cc, CZ1}});
+        put(pm, 2, Sec{X1, g, S(1)}, Sec{XB, a, S(1)}, Sec{X0, b, S(1)},
+            {Sec{CX1, cc, S(mb31 - 1)}, Sec{CX1, cc, S(mb31)}, Sec{CX1, cc, S(mb31 + 1)}});
+      }
+      if (ipass == 0 || (p2 + ipass) % 2 == 1) {
+        // green
+        put(pm, 1, Sec{X1, g, Z0}, Sec{XB, a, ZA}, Sec{X0, b, Z1}, {Sec{CX1, cc, CZ0}});
+        put(pm, 2, Sec{X0, g, S(1)}, Sec{XA, a, S(1)}, Sec{X1, b, S(1)},
+            {Sec{CX0, cc, S(mb31 - 1)}, Sec{CX0, cc, S(mb31)}, Sec{CX0, cc, S(mb31 + 1)}});
+        put(pm, 2, Sec{S(mx), g, Z1}, Sec{S(mx), a, ZB}, Sec{S(mx), b, Z0},
+            {Sec{S(mb12 + 1), cc, CZ1}, Sec{S(mb12), cc, CZ1}, Sec{S(mb12 - 1), cc, CZ1}});
+        // red
+        put(pm, 1, Sec{X0, g, Z1}, Sec{XA, a, ZB}, Sec{X1, b, Z0}, {Sec{CX0, cc, CZ1}});
+        put(pm, 2, Sec{S(1), g, Z0}, Sec{S(1), a, ZA}, Sec{S(1), b, Z1},
+            {Sec{S(mb11 - 1), cc, CZ0}, Sec{S(mb11), cc, CZ0}, Sec{S(mb11 + 1), cc, CZ0}});
+        put(pm, 2, Sec{X1, g, S(mz)}, Sec{XB, a, S(mz)}, Sec{X0, b, S(mz)},
+            {Sec{CX1, cc, S(mb32 + 1)}, Sec{CX1, cc, S(mb32)}, Sec{CX1, cc, S(mb32 - 1)}});
+      }
+      for (int p1 = 1; p1 <= 2; p1++) {
+        auto f1 = face_index(h, 0, p1);
+        for (int p3 = 1; p3 <= 2; p3++) {
+          auto f3 = face_index(h, 2, p3);
+          put(pm, 3, Sec{S(f1[2]), g, S(f3[2])}, Sec{S(f1[2]), a, S(f3[2])},
+              Sec{S(f1[2]), b, S(f3[2])},
+              {Sec{S(f1[4]), cc, S(f3[4])}, Sec{S(f1[3]), cc, S(f3[4])},
+               Sec{S(f1[4]), cc, S(f3[3])}, Sec{S(f1[3]), cc, S(f3[3])}});
+        }
+      }
+    }
+    // faces 5 and 6 (coordinate 3 fixed)
+    for (int p3 = 1; p3 <= 2; p3++) {
+      if (phys(p3 + 3)) continue;
+      auto f = face_index(h, 2, p3);
+      R g = S(f[0]), a = S(f[1]), b = S(f[2]), cc = S(f[3]);
+      if (ipass == 0 || (p3 + ipass) % 2 == 0) {
+        // yellow
+        put(pm, 1, Sec{X0, Y0, g}, Sec{XA, YA, a}, Sec{X1, Y1, b}, {Sec{CX0, CY0, cc}});
+        put(pm, 2, Sec{S(mx), Y0, g}, Sec{S(mx), YA, a}, Sec{S(mx), Y1, b},
+            {Sec{S(mb12 + 1), CY0, cc}, Sec{S(mb12), CY0, cc}, Sec{S(mb12 - 1), CY0, cc}});
+        put(pm, 2, Sec{X0, S(my), g}, Sec{XA, S(my), a}, Sec{X1, S(my), b},
+            {Sec{CX0, S(mb22 + 1), cc}, Sec{CX0, S(mb22), cc}, Sec{CX0, S(mb22 - 1), cc}});
+        // blue
+        put(pm, 1, Sec{X1, Y1, g}, Sec{XB, YB, a}, Sec{X0, Y0, b}, {Sec{CX1, CY1, cc}});
+        put(pm, 2, Sec{X1, S(1), g}, Sec{XB, S(1), a}, Sec{X0, S(1), b},
+            {Sec{CX1, S(mb21 - 1), cc}, Sec{CX1, S(mb21), cc}, Sec{CX1, S(mb21 + 1), cc}});
+        put(pm, 2, Sec{S(1), Y1, g}, Sec{S(1), YB, a}, Sec{S(1), Y0, b},
+            {Sec{S(mb11 - 1), CY1, cc}, Sec{S(mb11), CY1, cc}, Sec{S(mb11 + 1), CY1, cc}});
+      }
+      if (ipass == 0 || (p3 + ipass) % 2 == 1) {
+        // green
+        put(pm, 1, Sec{X1, Y0, g}, Sec{XB, YA, a}, Sec{X0, Y1, b}, {Sec{CX1, CY0, cc}});
+        put(pm, 2, Sec{S(1), Y0, g}, Sec{S(1), YA, a}, Sec{S(1), Y1, b},
+            {Sec{S(mb11 - 1), CY0, cc}, Sec{S(mb11), CY0, cc}, Sec{S(mb11 + 1), CY0, cc}});
+        put(pm, 2, Sec{X1, S(my), g}, Sec{XB, S(my), a}, Sec{X0, S(my), b},
+            {Sec{CX1, S(mb22 + 1), cc}, Sec{CX1, S(mb22), cc}, Sec{CX1, S(mb22 - 1), cc}});
+        // red
+        put(pm, 1, Sec{X0, Y1, g}, Sec{XA, YB, a}, Sec{X1, Y0, b}, {Sec{CX0, CY1, cc}});
+        put(pm, 2, Sec{X0, S(1), g}, Sec{XA, S(1), a}, Sec{X1, S(1), b},
+            {Sec{CX0, S(mb21 - 1), cc}, Sec{CX0, S(mb21), cc}, Sec{CX0, S(mb21 + 1), cc}});
+        put(pm, 2, Sec{S(mx), Y1, g}, Sec{S(mx), YB, a}, Sec{S(mx), Y0, b},
+            {Sec{S(mb12 + 1), CY1, cc}, Sec{S(mb12), CY1, cc}, Sec{S(mb12 - 1), CY1, cc}});
+      }
+      for (int p1 = 1; p1 <= 2; p1++) {
+        auto f1 = face_index(h, 0, p1);
+        for (int p2 = 1; p2 <= 2; p2++) {
+          auto f2 = face_index(h, 1, p2);
+          put(pm, 3, Sec{S(f1[2]), S(f2[2]), g}, Sec{S(f1[2]), S(f2[2]), a},
+              Sec{S(f1[2]), S(f2[2]), b},
+              {Sec{S(f1[4]), S(f2[4]), cc}, Sec{S(f1[3]), S(f2[4]), cc},
+               Sec{S(f1[4]), S(f2[3]), cc}, Sec{S(f1[3]), S(f2[3]), cc}});
+        }
+      }
+    }
+  }
+
+  // GhostOverlap, boundary.f90:346-415, for one ordered pair: cells of the
+  // ghost-extended overlap that lie in g2's interior are copied into g1.
+  void overlap_into(PatchMap &m1, int p2) {
+    const HPatch &g1 = hp[m1.p], &g2 = hp[p2];
+    int lo[3], hi[3];
+    for (int n = 0; n < 3; n++) {
+      int a1 = n < nd ? g1.mg[n][0] : 1, a2 = n < nd ? g1.mg[n][1] : 1;
+      int b1 = n < nd ? g2.mg[n][0] : 1, b2 = n < nd ? g2.mg[n][1] : 1;
+      int o0 = std::max(a1 - 1, b1 - 1), o1 = std::min(a2 + 1, b2 + 1);
+      if (n < nd && o1 - o0 < 0) return;
+      lo[n] = std::max(o0, b1);
+      hi[n] = std::min(o1, b2);
+    }
+    int g1lo[3], g2lo[3];
+    for (int n = 0; n < 3; n++) {
+      g1lo[n] = n < nd ? g1.mg[n][0] : 1;
+      g2lo[n] = n < nd ? g2.mg[n][0] : 1;
+    }
+    for (int k = lo[2]; k <= hi[2]; k++)
+      for (int j = lo[1]; j <= hi[1]; j++)
+        for (int i = lo[0]; i <= hi[0]; i++) {
+          int di = i - g1lo[0] + 1, dj = j - g1lo[1] + 1;
+          int dk = nd == 3 ? k - g1lo[2] + 1 : 0;
+          Expr e;
+          e.kind = 0;
+          e.f0 = Cell{p2, i - g2lo[0] + 1, j - g2lo[1] + 1, nd == 3 ? k - g2lo[2] + 1 : 0};
+          m1.m[key(g1, di, dj, dk)] = e;
+        }
+  }
+
+  // SetBC2D/SetBC3D (boundary.f90:643-891) with the CH module's UserBC2D/3D
+  // (problem.f90:646-756): faces in order over the full tangential range.
+  void physical_bc(PatchMap &pm) {
+    const HPatch &h = hp[pm.p];
+    int lo[3], hi[3];
+    for (int d = 0; d < 3; d++) {
+      lo[d] = d < nd ? 0 : 0;
+      hi[d] = d < nd ? h.mx[d] + 1 : 0;
+    }
+    for (int face = 0; face < 2 * nd; face++) {
+      int t = h.mthbc[face];
+      if (t == 2 || t == 999) continue;
+      int flip = (t == 1) ? 0 : ((nd == 3 || face < 2) ? 1 : 0);
+      int d = face / 2, right = face % 2;
+      int ta = d == 0 ? 1 : 0, tb = d == 2 ? 1 : 2;
+      std::vector<std::pair<long long, Expr>> upd;
+      for (int b = lo[tb]; b <= hi[tb]; b++)
+        for (int a = lo[ta]; a <= hi[ta]; a++) {
+          int di[3], si[3];
+          di[d] = right ? h.mx[d] + 1 : 0;
+          si[d] = right ? h.mx[d] : 1;
+          di[ta] = si[ta] = a;
+          di[tb] = si[tb] = b;
+          Expr e;
+          if (interior(h, si[0], si[1], si[2])) {
+            e.kind = 0;
+            e.f0 = Cell{pm.p, si[0], si[1], si[2]};
+            e.nflip = flip;
+          } else {
+            auto it = pm.m.find(key(h, si[0], si[1], si[2]));
+            if (it != pm.m.end()) {
+              e = it->second;
+              e.nflip += flip;
+            } else {
+              e.kind = 0;
+              e.f0 = Cell{pm.p, si[0], si[1], si[2]};
+              e.nflip = flip;
+            }
+          }
+          upd.push_back({key(h, di[0], di[1], di[2]), e});
+        }
+      for (auto &u : upd) pm.m[u.first] = u.second;
+    }
+  }
+};
+
+long long qoff(const std::vector<HPatch> &hp, const Cell &c) {
+  const HPatch &h = hp[c.p];
+  return h.qoff + c.i + h.px * (c.j + h.py * (long long)c.k);
+}
+
+}  // namespace
+
+int build_ghost_program(const std::vector<HPatch> &hp, const std::vector<int> &patches,
+                        int ndims, int level, int ipass, std::vector<GhostOp> &out,
+                        std::string &msg) {
+  out.clear();
+  Builder B(hp, ndims, msg);
+  std::vector<PatchMap> maps(patches.size());
+  for (size_t a = 0; a < patches.size(); a++) maps[a].p = patches[a];
+  if (level > 0) {
+    for (auto &pm : maps) {
+      if (ndims == 2) B.interp2d(pm, ipass);
+      else B.interp3d(pm, ipass);
+      if (!B.ok) return -1;
+    }
+    // ApplyOnLevelPairs order (treeops.f90:574-610); both directions per pair
+    for (size_t a = 0; a < maps.size(); a++)
+      for (size_t b = a + 1; b < maps.size(); b++) {
+        B.overlap_into(maps[a], maps[b].p);
+        B.overlap_into(maps[b], maps[a].p);
+      }
+  }
+  for (auto &pm : maps) B.physical_bc(pm);
+  for (auto &pm : maps) {
+    const HPatch &h = hp[pm.p];
+    for (auto &kv : pm.m) {
+      const Expr &e = kv.second;
+      Cell dc = B.cell_of(h, pm.p, kv.first);
+      if (B.interior(h, dc.i, dc.j, dc.k)) {
+        msg = "ghost program writes an interior cell";
+        return -1;
+      }
+      GhostOp g;
+      memset(&g, 0, sizeof g);
+      g.dst = (int)qoff(hp, dc);
+      g.dvs = (int)h.qvs;
+      g.kind = e.kind;
+      g.nflip = e.nflip;
+      g.f0 = (int)qoff(hp, e.f0);
+      g.fvs = (int)hp[e.f0.p].qvs;
+      if (e.kind > 0) {
+        g.f1 = (int)qoff(hp, e.f1);
+        const HPatch &pa = hp[h.parent];
+        g.cvs = (int)pa.qvs;
+        int nc = e.kind == 1 ? 1 : (e.kind == 2 ? 3 : 4);
+        for (int q = 0; q < nc; q++) g.c[q] = (int)qoff(hp, e.c[q]);
+      }
+      out.push_back(g);
+    }
+  }
+  // every source must be a value the program leaves untouched
+  std::unordered_set<int> written;
+  for (auto &g : out) written.insert(g.dst);
+  for (auto &g : out) {
+    bool bad = written.count(g.f0) > 0;
+    if (g.kind > 0) {
+      bad = bad || written.count(g.f1) > 0;
+      int nc = g.kind == 1 ? 1 : (g.kind == 2 ? 3 : 4);
+      for (int q = 0; q < nc; q++) bad = bad || written.count(g.c[q]) > 0;
+    }
+    if (bad) {
+      msg = "ghost program reads a cell it writes";
+      return -1;
+    }
+  }
+  return 0;
+}
+
+void locate_q_cell(const std::vector<HPatch> &hp, long long off, int &p, int &i, int &j,
+                   int &k) {
+  p = -1;
+  for (size_t n = 0; n < hp.size(); n++)
+    if (off >= hp[n].qoff && off < hp[n].qoff + hp[n].qvs) {
+      p = (int)n;
+      break;
+    }
+  if (p < 0) {
+    i = j = k = -1;
+    return;
+  }
+  const HPatch &h = hp[p];
+  long long r = off - h.qoff;
+  i = (int)(r % h.px);
+  j = (int)((r / h.px) % h.py);
+  k = (int)(r / (h.px * h.py));
+}
+
+void apply_ghost_program_host(const std::vector<HPatch> &hp, const std::vector<GhostOp> &ops,
+                              int nrvars, double *const *q_patches) {
+  auto val = [&](long long off, int v, long long vs) {
+    int p, i, j, k;
+    locate_q_cell(hp, off, p, i, j, k);
+    const HPatch &h = hp[p];
+    return q_patches[p][off - h.qoff + (long long)v * vs];
+  };
+  std::vector<double> res(ops.size() * nrvars);
+  for (size_t e = 0; e < ops.size(); e++) {
+    const GhostOp &g = ops[e];
+    for (int v = 0; v < nrvars; v++) {
+      double x;
+      switch (g.kind) {
+        case 0: x = val(g.f0, v, g.fvs); break;
+        case 1:
+          x = BSAM_A1 * val(g.f0, v, g.fvs) + BSAM_A2 * val(g.f1, v, g.fvs) +
+              BSAM_A3 * val(g.c[0], v, g.cvs);
+          break;
+        case 2:
+          x = BSAM_A1 * val(g.f0, v, g.fvs) + BSAM_A2 * val(g.f1, v, g.fvs) +
+              BSAM_A3 * (BSAM_B1 * val(g.c[0], v, g.cvs) + BSAM_B2 * val(g.c[1], v, g.cvs) +
+                         BSAM_B3 * val(g.c[2], v, g.cvs));
+          break;
+        default:
+          x = BSAM_A1 * val(g.f0, v, g.fvs) + BSAM_A2 * val(g.f1, v, g.fvs) +
+              BSAM_A3 * (BSAM_C1 * val(g.c[0], v, g.cvs) + BSAM_C2 * val(g.c[1], v, g.cvs) +
+                         BSAM_C3 * val(g.c[2], v, g.cvs) + BSAM_C4 * val(g.c[3], v, g.cvs));
+          break;
+      }
+      for (int r = 0; r < g.nflip; r++) x = 2.0 - x;
+      res[e * nrvars + v] = x;
+    }
+  }
+  for (size_t e = 0; e < ops.size(); e++) {
+    const GhostOp &g = ops[e];
+    int p, i, j, k;
+    locate_q_cell(hp, g.dst, p, i, j, k);
+    for (int v = 0; v < nrvars; v++)
+      q_patches[p][g.dst - hp[p].qoff + (long long)v * g.dvs] = res[e * nrvars + v];
+  }
+}
+
+}  // namespace bsam

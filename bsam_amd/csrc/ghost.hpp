@@ -1,0 +1,38 @@
+// ghost.hpp -- host-side integer work of the ghost fill: builds, per level and
+// colour pass, the list of ghost-cell updates that SetGhost performs.
+#pragma once
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+
+namespace bsam {
+
+// Host record of one patch (the geometry of nodeinfo, nodeinfodef.f90:124-219)
+// plus its q-pool placement, filled at commit.
+struct HPatch {
+  int id = 0, level = 0, parent = -1;
+  int mx[3] = {1, 1, 1}, cmx[3] = {1, 1, 1};
+  int mb[3][2] = {{1, 1}, {1, 1}, {1, 1}};
+  int mg[3][2] = {{1, 1}, {1, 1}, {1, 1}};
+  int mthbc[6] = {1, 1, 1, 1, 1, 1};
+  double dx = 0.0, xlower[3] = {0.0, 0.0, 0.0};
+  long long qoff = 0, qvs = 0, px = 0, py = 0;
+};
+
+// Builds the ghost-fill program of SetGhost(level, ipass) (boundary.f90:58-89)
+// for the patches of one level. Returns 0 on success, else fills msg.
+int build_ghost_program(const std::vector<HPatch> &hp, const std::vector<int> &patches,
+                        int ndims, int level, int ipass, std::vector<GhostOp> &out,
+                        std::string &msg);
+
+// Patch and (i,j,k) (lower bound 0; k = 0 in 2D) of a q-pool offset.
+void locate_q_cell(const std::vector<HPatch> &hp, long long off, int &p, int &i, int &j,
+                   int &k);
+
+// Evaluates a program on host arrays (one reference-layout q array per patch),
+// reading the state before the program as the device kernel does.
+void apply_ghost_program_host(const std::vector<HPatch> &hp, const std::vector<GhostOp> &ops,
+                              int nrvars, double *const *q_patches);
+
+}  // namespace bsam

@@ -1,0 +1,30 @@
+// kernels.hpp -- launchers of the gfx950 kernels (kernels.hip).
+#pragma once
+#include "common.hpp"
+
+namespace bsam {
+
+hipError_t launch_relax(int nd, const Dev &d, const Tile *tiles, int nt, int ipass,
+                        hipStream_t s);
+hipError_t launch_ghost(const Dev &d, const GhostOp *ops, int n, hipStream_t s);
+hipError_t launch_restrict(int nd, const Dev &d, const Tile *tiles, int nt, hipStream_t s);
+hipError_t launch_filldown(int nd, const Dev &d, const Tile *tiles, int nt, int field,
+                           hipStream_t s);
+hipError_t launch_coarse_ghost(int nd, const Dev &d, const Tile *tiles, int nt,
+                               hipStream_t s);
+hipError_t launch_coarse_load(int nd, const Dev &d, const Tile *tiles, int nt,
+                              hipStream_t s);
+hipError_t launch_correct(int nd, const Dev &d, const Tile *tiles, int nt, hipStream_t s);
+hipError_t launch_source(int nd, const Dev &d, const Tile *tiles, int nt, int full,
+                         hipStream_t s);
+hipError_t launch_resnorm(int nd, const Dev &d, const Tile *tiles, int nt, double *partial,
+                          int write_rf, hipStream_t s);
+hipError_t launch_finalize(const double *partial, int n, double count, double tol,
+                           double *hist, int *state, hipStream_t s);
+hipError_t launch_copy(double *dst, const double *src, long long n, hipStream_t s);
+
+int tile_x(int nd);
+int tile_y(int nd);
+int tile_z(int nd);
+
+}  // namespace bsam

@@ -1,0 +1,231 @@
+"""Python binding of the engine's C ABI (include/bsam_gpu.h) via ctypes.
+
+:class:`Engine` mirrors the reference's solver interface for one patch
+hierarchy: the method names follow the Fortran routines they replace
+(``set_ghost`` for SetGhost, ``vcycle`` for AFASVcycle,
+``multigrid_iterations`` for MultigridIterations, ...). There is no
+fallback: if ``libbsamgpu.so`` is missing or no HIP device is present, the
+constructor raises.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Dict, List, Optional
+
+import numpy as np
+
+from .layout import Layout
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libbsamgpu.so")
+
+FIELD_F, FIELD_Q, FIELD_QOLD, FIELD_QC, FIELD_FTMP, FIELD_RF = 1, 2, 3, 4, 5, 6
+FIELDS = {"f": FIELD_F, "q": FIELD_Q, "qold": FIELD_QOLD, "qc": FIELD_QC,
+          "ftmp": FIELD_FTMP, "rf": FIELD_RF}
+PREC_FP64, PREC_MIXED = 0, 1
+
+# Double-well constants: the reference module (problem.f90:781-813) with
+# alpha, and the +-1 well F = (phi^2-1)^2/4.
+WELLS = {
+    "reference": lambda alpha: (alpha, 0.5, alpha, 4.0),
+    "plusminus": lambda alpha: (1.0, 0.0, 1.0, 1.0),
+}
+
+
+class BsamError(RuntimeError):
+    pass
+
+
+class Cfg(C.Structure):
+    _fields_ = [("ndims", C.c_int), ("nrvars", C.c_int), ("mbc", C.c_int),
+                ("minlevel", C.c_int), ("dt", C.c_double), ("eps2", C.c_double),
+                ("fc_alpha", C.c_double), ("fc_shift", C.c_double),
+                ("fe_num", C.c_double), ("fe_den", C.c_double),
+                ("nsmoothingpasses", C.c_int), ("maxvcycles", C.c_int),
+                ("qerrortol", C.c_double), ("precision", C.c_int), ("device", C.c_int)]
+
+
+_lib = None
+
+
+def lib():
+    """Load libbsamgpu.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise BsamError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        vp, ip, dp = C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_double)
+        L.bsam_ctx_create.restype = vp
+        L.bsam_ctx_create.argtypes = [C.POINTER(Cfg)]
+        L.bsam_ctx_destroy.argtypes = [vp]
+        L.bsam_last_error.restype = C.c_char_p
+        L.bsam_last_error.argtypes = [vp]
+        L.bsam_hierarchy_begin.argtypes = [vp]
+        L.bsam_add_patch.argtypes = [vp, C.c_int, C.c_int, C.c_int, ip, ip, ip, ip,
+                                     C.c_double, dp]
+        L.bsam_hierarchy_commit.argtypes = [vp]
+        L.bsam_num_patches.argtypes = [vp]
+        L.bsam_field_len.restype = C.c_long
+        L.bsam_field_len.argtypes = [vp, C.c_int, C.c_int]
+        L.bsam_upload.argtypes = [vp, C.c_int, C.c_int, dp, C.c_long]
+        L.bsam_download.argtypes = [vp, C.c_int, C.c_int, dp, C.c_long]
+        for n in ("bsam_set_ghost", "bsam_relax"):
+            getattr(L, n).argtypes = [vp, C.c_int, C.c_int]
+        for n in ("bsam_restrict_solution", "bsam_coarse_ghost_points", "bsam_coarse_load",
+                  "bsam_correct_fine", "bsam_fill_down", "bsam_vcycle"):
+            getattr(L, n).argtypes = [vp, C.c_int]
+        for n in ("bsam_source", "bsam_copy_qold", "bsam_sync"):
+            getattr(L, n).argtypes = [vp]
+        L.bsam_error_afas.argtypes = [vp, C.c_int, dp]
+        for n in ("bsam_multigrid_iterations", "bsam_time_step"):
+            getattr(L, n).argtypes = [vp, ip, dp, C.c_int, ip]
+        L.bsam_ghost_program_size.argtypes = [vp, C.c_int, C.c_int]
+        L.bsam_ghost_program_dst.argtypes = [vp, C.c_int, C.c_int, ip, C.c_int]
+        L.bsam_ghost_program_apply_host.argtypes = [vp, C.c_int, C.c_int,
+                                                    C.POINTER(C.POINTER(C.c_double))]
+        _lib = L
+    return _lib
+
+
+def _ia(vals):
+    return (C.c_int * len(vals))(*[int(v) for v in vals])
+
+
+def _dptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Engine:
+    """A device-resident patch hierarchy and its FAS solver.
+
+    ``device=-1`` gives a host-only context (geometry and ghost programs,
+    no GPU), used to check the integer maps on a CPU.
+    """
+
+    def __init__(self, layout: Layout, dt: float, eps: float, alpha: float = 1.0,
+                 well: str = "reference", nsmoothingpasses: int = 2,
+                 maxvcycles: int = 20, qerrortol: float = 1e-8,
+                 precision: int = PREC_FP64, device: int = 0):
+        L = lib()
+        fa, fs, fn, fd = WELLS[well](alpha)
+        cfg = Cfg(layout.ndims, layout.nrvars, layout.mbc, layout.minlevel, dt, eps * eps,
+                  fa, fs, fn, fd, nsmoothingpasses, maxvcycles, qerrortol, precision, device)
+        self.layout = layout
+        self.maxvcycles = maxvcycles
+        self.ctx = L.bsam_ctx_create(C.byref(cfg))
+        if not self.ctx:
+            raise BsamError(L.bsam_last_error(None).decode())
+        self.patches = layout.ordered()
+        idx = layout.index_map()
+        self._ck(L.bsam_hierarchy_begin(self.ctx))
+        for p in self.patches:
+            mb = [v for d in range(3) for v in p.mbounds[d]]
+            mg = [v for d in range(3) for v in p.mglobal[d]]
+            par = idx[p.parent_id] if p.parent_id is not None else -1
+            self._ck(L.bsam_add_patch(self.ctx, p.id, p.level, par, _ia(p.mx), _ia(mb),
+                                      _ia(mg), _ia(p.mthbc), p.dx,
+                                      (C.c_double * 3)(*p.xlower)))
+        self._ck(L.bsam_hierarchy_commit(self.ctx))
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            lib().bsam_ctx_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        self.close()
+
+    def _ck(self, rc):
+        if rc != 0:
+            raise BsamError(lib().bsam_last_error(self.ctx).decode())
+        return rc
+
+    # -- fields -----------------------------------------------------------
+    def upload(self, n: int, name: str, arr: np.ndarray):
+        a = np.asfortranarray(arr, dtype=np.float64)
+        shape = self.layout.field_shape(self.patches[n], name)
+        if a.shape != shape:
+            raise BsamError(f"shape {a.shape} != {shape}")
+        flat = a.reshape(-1, order="F")
+        self._ck(lib().bsam_upload(self.ctx, n, FIELDS[name], _dptr(flat), flat.size))
+
+    def download(self, n: int, name: str) -> np.ndarray:
+        shape = self.layout.field_shape(self.patches[n], name)
+        flat = np.empty(int(np.prod(shape)), dtype=np.float64)
+        self._ck(lib().bsam_download(self.ctx, n, FIELDS[name], _dptr(flat), flat.size))
+        return flat.reshape(shape, order="F")
+
+    # -- reference operations ------------------------------------------------
+    def set_ghost(self, level: int, ipass: int):
+        self._ck(lib().bsam_set_ghost(self.ctx, level, ipass))
+
+    def relax(self, level: int, color: int):
+        self._ck(lib().bsam_relax(self.ctx, level, color))
+
+    def restrict(self, level: int):
+        self._ck(lib().bsam_restrict_solution(self.ctx, level))
+
+    def coarse_ghost(self, level: int):
+        self._ck(lib().bsam_coarse_ghost_points(self.ctx, level))
+
+    def coarse_load(self, level: int):
+        self._ck(lib().bsam_coarse_load(self.ctx, level))
+
+    def correct(self, level: int):
+        self._ck(lib().bsam_correct_fine(self.ctx, level))
+
+    def fill_down(self, field: int):
+        """field 1 = f (sourcefield), 2 = q (solutionfield)."""
+        self._ck(lib().bsam_fill_down(self.ctx, field))
+
+    def source(self):
+        self._ck(lib().bsam_source(self.ctx))
+
+    def vcycle(self, level: int):
+        self._ck(lib().bsam_vcycle(self.ctx, level))
+
+    def error_afas(self, level: int) -> float:
+        r = C.c_double()
+        self._ck(lib().bsam_error_afas(self.ctx, level, C.byref(r)))
+        return r.value
+
+    def copy_qold(self):
+        self._ck(lib().bsam_copy_qold(self.ctx))
+
+    def sync(self):
+        self._ck(lib().bsam_sync(self.ctx))
+
+    def _solve(self, fn):
+        n, conv = C.c_int(), C.c_int()
+        hist = (C.c_double * self.maxvcycles)()
+        self._ck(fn(self.ctx, C.byref(n), hist, self.maxvcycles, C.byref(conv)))
+        return list(hist[:n.value]), bool(conv.value)
+
+    def multigrid_iterations(self):
+        """Returns (per-cycle residuals, converged)."""
+        return self._solve(lib().bsam_multigrid_iterations)
+
+    def time_step(self):
+        return self._solve(lib().bsam_time_step)
+
+    # -- integer maps ---------------------------------------------------------
+    def ghost_program_dst(self, level: int, ipass: int) -> np.ndarray:
+        L = lib()
+        n = L.bsam_ghost_program_size(self.ctx, level, ipass)
+        if n < 0:
+            raise BsamError("bad level or ipass")
+        out = np.zeros(4 * max(n, 1), dtype=np.int32)
+        self._ck(L.bsam_ghost_program_dst(self.ctx, level, ipass,
+                                          out.ctypes.data_as(C.POINTER(C.c_int)), n))
+        return out[:4 * n].reshape(n, 4)
+
+    def ghost_apply_host(self, level: int, ipass: int, qs: List[np.ndarray]):
+        """Apply SetGhost(level, ipass)'s program to host q arrays in place."""
+        ptrs = (C.POINTER(C.c_double) * len(qs))()
+        for n, a in enumerate(qs):
+            assert a.flags.f_contiguous and a.dtype == np.float64
+            ptrs[n] = _dptr(a)
+        self._ck(lib().bsam_ghost_program_apply_host(self.ctx, level, ipass, ptrs))

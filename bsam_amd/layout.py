@@ -131,11 +131,10 @@ class Layout:
             g1 = par.mglobal[d][0] + mb[d][0] - 1
             g2 = g1 + mb[d][1] - mb[d][0]
             mg.append(((g1 - 1) * 2 + 1, g2 * 2))
+        # all six codes start internal, as in MakeNewGrid; only the first
+        # 2*ndims can inherit a physical code
         bc = [INTERNAL_BC] * 6
-        for d in range(3):
-            if d >= nd:
-                bc[2 * d] = bc[2 * d + 1] = 1
-                continue
+        for d in range(nd):
             if par.mthbc[2 * d] < INTERNAL_BC and mb[d][0] == 1:
                 bc[2 * d] = par.mthbc[2 * d]
             if par.mthbc[2 * d + 1] < INTERNAL_BC and mb[d][1] == par.mx[d]:

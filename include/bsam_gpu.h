@@ -1,0 +1,149 @@
+/*
+ * bsam_gpu.h -- C ABI of the MI355X FAS multigrid engine for block-structured
+ * AMR Cahn-Hilliard solves.
+ *
+ * The engine replaces the reference's adaptive FAS solve and ghost fill. The
+ * host (Fortran via ISO_C_BINDING, C, C++ or Python ctypes) keeps the patch
+ * tree and its arrays; it describes the hierarchy once per regrid, uploads
+ * fields, and calls the entry points below, which run hand-written gfx950
+ * kernels on device-resident mirrors of the fields.
+ *
+ * Conventions
+ *  - Every function returns 0 on success and a negative code on failure;
+ *    bsam_last_error(ctx) then holds a message. Nothing exits the process.
+ *  - Arrays cross the ABI in the reference layout (lib/bsamstorage.f90:46-123):
+ *    column-major, variable index slowest, q/qold/qc ghost-extended with lower
+ *    bound 1-mbc, f/rf/ftmp interior only. 2D arrays have a third extent 1.
+ *  - Patches are identified by their index in the order they were added:
+ *    levels ascending, each level in ApplyOnLevel order (youngest first,
+ *    lib/treeops.f90:536-572).
+ *  - One host thread per context. Contexts are independent (own device
+ *    memory and stream); several may share one GPU.
+ */
+#ifndef BSAM_GPU_H
+#define BSAM_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct bsam_ctx bsam_ctx;
+
+/* Field selectors; values 1 and 2 match sourcefield/solutionfield
+ * (lib/nodeinfodef.f90:64-66). */
+enum {
+  BSAM_FIELD_F = 1,     /* source/load function f (interior) */
+  BSAM_FIELD_Q = 2,     /* solution q (ghost-extended) */
+  BSAM_FIELD_QOLD = 3,  /* previous time level (ghost-extended) */
+  BSAM_FIELD_QC = 4,    /* coarse-under-fine restriction (ghost-extended) */
+  BSAM_FIELD_FTMP = 5,  /* time-step source part (interior) */
+  BSAM_FIELD_RF = 6     /* residual scratch (interior) */
+};
+
+/* Smoothing precision. */
+enum {
+  BSAM_PREC_FP64 = 0,   /* plain FP64 everywhere: parity path */
+  BSAM_PREC_MIXED = 1   /* FP32 smoothing inside FP64 defect correction */
+};
+
+typedef struct bsam_cfg {
+  int ndims;              /* 2 or 3 */
+  int nrvars;             /* 2: (phi, mu) */
+  int mbc;                /* ghost layers; 1 */
+  int minlevel;           /* coarsest level (<= 0) */
+  /* Cahn-Hilliard problem constants (problem module hooks):
+   *   Dfc(c) = fc_alpha*(c-fc_shift)^3, D2fc = 3*fc_alpha*(c-fc_shift)^2,
+   *   Dfe(c) = ((c-fc_shift)*fe_num)/fe_den, Mob = 1.
+   * The reference module is fc_shift=0.5, fc_alpha=fe_num=alpha, fe_den=4. */
+  double dt, eps2;
+  double fc_alpha, fc_shift, fe_num, fe_den;
+  /* &rundata solver keys (lib/bsamroutines.f90:68-90) */
+  int nsmoothingpasses;
+  int maxvcycles;
+  double qerrortol;
+  int precision;          /* BSAM_PREC_* */
+  int device;             /* HIP device ordinal; -1 = host-only context
+                             (geometry and ghost programs, no device memory) */
+} bsam_cfg;
+
+/* Context lifetime. Replaces the module globals of NodeInfoDef
+ * (lib/nodeinfodef.f90:47-104) with an explicit configuration. */
+bsam_ctx *bsam_ctx_create(const bsam_cfg *cfg);
+int bsam_ctx_destroy(bsam_ctx *ctx);
+const char *bsam_last_error(const bsam_ctx *ctx);
+
+/* Hierarchy description, fed from an ApplyOnLevel walk of the reference tree
+ * (treeops.f90:536-572), one patch per call, levels ascending, each level in
+ * level-list order. Geometry is the nodeinfo record (nodeinfodef.f90:124-219):
+ * mx[3], mbounds[3][2] and mglobal[3][2] flattened as {lo1,hi1,lo2,hi2,lo3,hi3},
+ * mthbc[6], dx, xlower[3]; parent is the parent's patch index (-1 for the
+ * coarsest patch). Commit builds the device patch table, the ghost-fill
+ * programs and the tile lists, and allocates field storage (q, qold zeroed as
+ * in AllocFields, bsamstorage.f90:116-117). */
+int bsam_hierarchy_begin(bsam_ctx *ctx);
+int bsam_add_patch(bsam_ctx *ctx, int id, int level, int parent, const int *mx,
+                   const int *mbounds, const int *mglobal, const int *mthbc,
+                   double dx, const double *xlower);
+int bsam_hierarchy_commit(bsam_ctx *ctx);
+int bsam_num_patches(const bsam_ctx *ctx);
+
+/* Host <-> device field transfer for one patch, n doubles in the reference
+ * layout. Synchronous. */
+long bsam_field_len(const bsam_ctx *ctx, int patch, int field);
+int bsam_upload(bsam_ctx *ctx, int patch, int field, const double *host, long n);
+int bsam_download(bsam_ctx *ctx, int patch, int field, double *host, long n);
+
+/* Solver entry points. Each replaces the reference routine it names and runs
+ * on the context's stream. */
+/* SetGhost(level, ipass), lib/boundary.f90:58-89; ipass 0 all, 1 red, 2 black */
+int bsam_set_ghost(bsam_ctx *ctx, int level, int ipass);
+/* ApplyOnLevel(level, RelaxPatch) for one colour, lib/afasroutines.f90:549-598
+ * with the problem's RelaxGrid2D/3D (problem.f90:164-215, 453-518) */
+int bsam_relax(bsam_ctx *ctx, int level, int color);
+/* ApplyOnLevel(level, RestrictSolution), lib/afasroutines.f90:600-644 */
+int bsam_restrict_solution(bsam_ctx *ctx, int level);
+/* ApplyOnLevel(level, GetCoarseGhostPoints), lib/boundary.f90:1203-1269 */
+int bsam_coarse_ghost_points(bsam_ctx *ctx, int level);
+/* ApplyOnLevel(level, CoarseLoadingFunction), lib/afasroutines.f90:646-714 */
+int bsam_coarse_load(bsam_ctx *ctx, int level);
+/* ApplyOnLevel(level, CorrectFine), lib/afasroutines.f90:842-963 */
+int bsam_correct_fine(bsam_ctx *ctx, int level);
+/* FillDown(field), lib/afasroutines.f90:223-239; field BSAM_FIELD_F or _Q */
+int bsam_fill_down(bsam_ctx *ctx, int field);
+/* GetSourceFunction then UpdateSourceFunction on levels finest..0,
+ * lib/afasroutines.f90:57-64 */
+int bsam_source(bsam_ctx *ctx);
+/* AFASVcycle(level), lib/afasroutines.f90:488-521 */
+int bsam_vcycle(bsam_ctx *ctx, int level);
+/* ErrorAFAS(level) with errortype 1, lib/afasroutines.f90:108-179 */
+int bsam_error_afas(bsam_ctx *ctx, int level, double *residual);
+/* qold = q on every level (CopyQToQold, lib/bsamroutines.f90:568-580) */
+int bsam_copy_qold(bsam_ctx *ctx);
+/* MultigridIterations, lib/afasroutines.f90:38-106. Writes the number of
+ * V-cycles run, up to maxhist per-cycle residuals, and whether the last
+ * residual fell below qerrortol (the reference sets lconverged, :98-101). */
+int bsam_multigrid_iterations(bsam_ctx *ctx, int *ncycles, double *hist,
+                              int maxhist, int *converged);
+/* One time step on a fixed hierarchy: SetGhost on levels 0..finest, FillDown
+ * of q, qold = q, then MultigridIterations (TakeTimeSteps,
+ * lib/bsamroutines.f90:648-723, without the regrid). */
+int bsam_time_step(bsam_ctx *ctx, int *ncycles, double *hist, int maxhist,
+                   int *converged);
+/* Wait for all work queued on the context's stream. */
+int bsam_sync(bsam_ctx *ctx);
+
+/* Introspection of the integer maps (valid on host-only contexts too).
+ * Ghost-fill program of (level, ipass): number of ghost-cell updates, and
+ * their destinations as q-field (patch, i, j, k) tuples, 4 ints each. */
+int bsam_ghost_program_size(const bsam_ctx *ctx, int level, int ipass);
+int bsam_ghost_program_dst(const bsam_ctx *ctx, int level, int ipass, int *out,
+                           int maxn);
+/* Apply a ghost-fill program on the host to per-patch q arrays (reference
+ * layout, one pointer per patch): used to check the integer maps on a CPU. */
+int bsam_ghost_program_apply_host(const bsam_ctx *ctx, int level, int ipass,
+                                  double *const *q_patches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

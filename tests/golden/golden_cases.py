@@ -1,0 +1,99 @@
+"""Layouts and inputs of the golden fixtures (shared by make_golden.py and
+the tests that read the fixtures back)."""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from bsam_amd.layout import Layout
+
+
+def _amr2d():
+    L = Layout(2, (16, 16), 1.0 / 16)
+    c1 = L.add_child(0, ((3, 10), (5, 12)))
+    L.add_child(0, ((11, 14), (5, 8)))
+    L.add_child(c1, ((2, 5), (3, 8)))
+    L.add_child(c1, ((6, 9), (3, 6)))
+    c5 = L.add_child(0, ((1, 2), (13, 16)))
+    L.add_child(c5, ((1, 2), (1, 4)))
+    return L
+
+
+def _amr2d_userbc():
+    L = Layout(2, (16, 16), 1.0 / 16, mthbc=(10, 10, 10, 10))
+    a = L.add_child(0, ((1, 8), (1, 8)))
+    b = L.add_child(0, ((9, 16), (1, 8)))
+    L.add_child(0, ((1, 16), (9, 16)))
+    L.add_child(a, ((1, 8), (1, 8)))
+    L.add_child(b, ((1, 8), (3, 8)))
+    return L
+
+
+def _rect2d():
+    # non-square root (the shipped example is 128 x 64) and a non-unit domain
+    L = Layout(2, (24, 12), 0.5, xlower=(1.0, -2.0))
+    c = L.add_child(0, ((4, 15), (2, 9)))
+    L.add_child(c, ((3, 6), (3, 12)))
+    L.add_child(c, ((7, 12), (1, 4)))
+    return L
+
+
+LAYOUTS = {
+    "single32": lambda: Layout(2, (32, 32), 1.0 / 32),
+    "c1_128": lambda: Layout(2, (128, 128), 1.0 / 128),
+    "amr2d": _amr2d,
+    "amr2d_userbc": _amr2d_userbc,
+    "rect2d": _rect2d,
+}
+
+OPS = [("setghost", 1, 0), ("setghost", 2, 1), ("setghost", 2, 2), ("relax", 2, 1),
+       ("relax", 1, 2), ("restrict", 2), ("setghost", 1, 0), ("coarseghost", 2),
+       ("coarseload", 2), ("correct", 2), ("filldown", 2), ("filldown", 1)]
+
+# case -> (layout, seed, op sequence): one fixture per case with the
+# inputs and the fields after the whole sequence
+OPS_CASES = {
+    "amr2d_seq": ("amr2d", 1, OPS),
+    "amr2d_vcycle": ("amr2d", 2, [("vcycle", 2)]),
+    "userbc_seq": ("amr2d_userbc", 3, OPS),
+    "rect2d_vcycle": ("rect2d", 4, [("vcycle", 2)]),
+}
+
+
+def noise_init(mean, amp, seed):
+    def init(lay):
+        rng = np.random.default_rng(seed)
+        out = {}
+        for p in lay.ordered():
+            q = np.zeros(lay.field_shape(p, "q"), order="F")
+            q[..., 0] = mean + rng.uniform(-amp, amp, q.shape[:3])
+            out[p.id] = {"q": q, "f": np.zeros(lay.field_shape(p, "f"), order="F"),
+                         "qc": np.zeros(lay.field_shape(p, "qc"), order="F")}
+        return out
+    return init
+
+
+# The +-1 double well F = (phi^2-1)^2/4 with (eps, dt) is the reference
+# module's well F = alpha c^2 (c-1)^2/4 under c = (phi+1)/2 with alpha = 8,
+# eps_ref = sqrt(2) eps and dt_ref = dt/2 (mu unchanged). PM_* hold that map.
+PM_EPS, PM_DT = 0.03, 1e-3
+
+STEP_CASES = {
+    # C1 geometry (128^2, [0,1]^2, Neumann, eps 0.03, dt 1e-3), reference
+    # double well around its spinodal point c = 0.5
+    "c1_128": dict(layout="c1_128", init=noise_init(0.5, 0.05, 1), dt=1e-3, alpha=1.0,
+                   eps=0.03, maxv=40, tol=1e-8, steps=2, keep_levels=(0,)),
+    "single32": dict(layout="single32", init=noise_init(0.5, 0.05, 2), dt=1e-3, alpha=1.0,
+                     eps=0.03, maxv=40, tol=1e-8, steps=3),
+    "amr2d": dict(layout="amr2d", init=noise_init(0.5, 0.05, 3), dt=1e-3, alpha=1.0,
+                  eps=0.03, maxv=40, tol=1e-8, steps=3),
+    "amr2d_userbc": dict(layout="amr2d_userbc", init=noise_init(0.6, 0.05, 4), dt=2e-3,
+                         alpha=2.0, eps=0.05, maxv=40, tol=1e-9, steps=2),
+    "rect2d": dict(layout="rect2d", init=noise_init(0.5, 0.1, 5), dt=5e-2, alpha=1.0,
+                   eps=0.3, maxv=40, tol=1e-8, steps=2),
+    # fixed 6 V-cycles (tol 0) of the mapped +-1 problem, phi ~ 0.1 + U(-.05,.05)
+    "plusminus_map": dict(layout="amr2d", init=noise_init(0.55, 0.025, 6), dt=PM_DT / 2,
+                          alpha=8.0, eps=math.sqrt(2.0) * PM_EPS, maxv=6, tol=0.0, steps=1),
+}

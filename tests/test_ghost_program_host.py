@@ -1,0 +1,70 @@
+"""Ghost-fill programs (the engine's integer maps) checked on the CPU.
+
+A host-only engine context builds the SetGhost programs exactly as a device
+context does; applying them on host arrays must reproduce the oracle's
+SetGhost bit for bit, for every level and colour pass.
+"""
+
+import numpy as np
+import pytest
+
+import orc
+from bsam_amd.engine import Engine
+from helpers import layout_amr2d, layout_amr2d_tiles, layout_single
+
+LAYOUTS = {
+    "single16": lambda: layout_single(16),
+    "single16_user_bc": lambda: layout_single(16, mthbc=(10, 10, 10, 10)),
+    "amr2d": layout_amr2d,
+    "amr2d_user_bc": lambda: layout_amr2d(mthbc=(10, 10, 10, 10)),
+    "amr2d_tiles": layout_amr2d_tiles,
+    "amr2d_tiles_mixed_bc": lambda: layout_amr2d_tiles(mthbc=(1, 10, 10, 1)),
+}
+
+
+@pytest.mark.parametrize("name", sorted(LAYOUTS))
+def test_ghost_program_matches_oracle(name):
+    lay = LAYOUTS[name]()
+    eng = Engine(lay, dt=1e-3, eps=0.03, device=-1)
+    rng = np.random.default_rng(0)
+    for level in range(lay.minlevel, lay.finestlevel + 1):
+        for ipass in (0, 1, 2):
+            ora = orc.Oracle(lay, dt=1e-3, eps=0.03)
+            qs = []
+            for n, p in enumerate(ora.patches):
+                a = rng.uniform(-1, 1, lay.field_shape(p, "q"))
+                ora.field(n, "q")[...] = a
+                qs.append(np.asfortranarray(a.copy()))
+            ora.set_ghost(level, ipass)
+            eng.ghost_apply_host(level, ipass, qs)
+            for n, p in enumerate(ora.patches):
+                assert np.array_equal(ora.field(n, "q"), qs[n]), (level, ipass, p.id)
+
+
+def test_ghost_program_destinations_are_ghosts():
+    lay = layout_amr2d_tiles()
+    eng = Engine(lay, dt=1e-3, eps=0.03, device=-1)
+    pats = lay.ordered()
+    for level in range(lay.minlevel, lay.finestlevel + 1):
+        for ipass in (0, 1, 2):
+            dst = eng.ghost_program_dst(level, ipass)
+            assert len(np.unique(dst, axis=0)) == len(dst)
+            for p, i, j, k in dst:
+                pt = pats[p]
+                assert pt.level == level
+                assert i in (0, pt.mx[0] + 1) or j in (0, pt.mx[1] + 1)
+
+
+def test_red_and_black_programs_partition_interpolated_faces():
+    """On a patch with four internal faces, the red and black passes refresh
+    disjoint sets of coarse-fine ghosts that together equal the full pass."""
+    from bsam_amd.layout import Layout
+    lay = Layout(2, (16, 16), 1.0 / 16)
+    lay.add_child(0, ((5, 12), (5, 12)))
+    eng = Engine(lay, dt=1e-3, eps=0.03, device=-1)
+    red = {tuple(r) for r in eng.ghost_program_dst(1, 1)}
+    black = {tuple(r) for r in eng.ghost_program_dst(1, 2)}
+    full = {tuple(r) for r in eng.ghost_program_dst(1, 0)}
+    assert not red & black
+    assert red | black == full
+    assert len(full) == 4 * 16 + 4
