@@ -1,0 +1,159 @@
+#!/usr/bin/env python3
+"""Benchmark of the FAS multigrid engine on one MI355X.
+
+Workload (one "step" = one implicit time step, i.e. one MultigridIterations
+solve to the 1e-8 residual on a fixed hierarchy): 2D AMR Cahn-Hilliard,
+root 256^2 on [0,1]^2 with homogeneous Neumann boundaries plus two refined
+levels tagged where |grad phi| > 0.5 max (blocks of 16..64 cells per side),
+eps = 0.03, dt = 1e-3, +-1 double well, phi0 = tanh((0.25-r)/(sqrt(2) eps))
+with r the distance to 5 seeded centres; plain FP64 V(2,2) cycles.
+
+value = relaxation cell-updates per second over the timed steps (a
+cell-update is one full red-black relaxation of one cell's (phi, mu); per
+V-cycle sum over levels of 2*nu*N_L, nu*N at the coarsest level), inputs
+resident in HBM. The roofline object is for the relaxation kernel, timed with
+HIP events on the engine's stream during the timed steps; cpu_baseline is the
+FP64 oracle (single-threaded C restatement of the reference) on the same
+workload.
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "V-cycle cell-updates/s + HBM roofline %; solves/s & p99 latency at 1e-8 resid"
+HBM_PEAK_GBS = 8000.0
+
+
+def percentile(xs, p):
+    xs = sorted(xs)
+    if not xs:
+        return None
+    k = (len(xs) - 1) * p / 100.0
+    lo, hi = int(k), min(int(k) + 1, len(xs) - 1)
+    return xs[lo] + (xs[hi] - xs[lo]) * (k - lo)
+
+
+def cpu_baseline(layout, q0, params, budget_s):
+    """Oracle time steps from the same initial state until budget_s elapses."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import orc
+    o = orc.Oracle(layout, dt=params["dt"], eps=params["eps"], well=params["well"],
+                   nsmoothingpasses=params["nsmoothingpasses"], maxvcycles=50,
+                   qerrortol=params["qerrortol"])
+    for n, p in enumerate(o.patches):
+        o.field(n, "q")[...] = q0[p.id]
+    per_v = layout.relax_cell_updates_per_vcycle(params["nsmoothingpasses"])
+    t0 = time.perf_counter()
+    steps = cycles = 0
+    while True:
+        cycles += len(o.time_step())
+        steps += 1
+        if time.perf_counter() - t0 >= budget_s:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": cycles * per_v / dt, "unit": "cell-updates/s", "cores": 1,
+            "kind": "port",
+            "sample": f"{steps} time steps ({cycles} V-cycles) of the same workload from "
+                      f"t=0, single-threaded FP64 C oracle, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--cpu-budget", type=float, default=15.0,
+                    help="seconds of CPU-baseline work (0 to skip)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the solver path here is single-GPU "
+              "(run with --gpus 1).", file=sys.stderr)
+        sys.exit(2)
+
+    from bsam_amd import engine as E
+    from bsam_amd.workloads import C2_PARAMS, config_c2
+
+    layout, q0 = config_c2()
+    params = dict(C2_PARAMS)
+    eng = E.Engine(layout, dt=params["dt"], eps=params["eps"], well=params["well"],
+                   nsmoothingpasses=params["nsmoothingpasses"], maxvcycles=50,
+                   qerrortol=params["qerrortol"], device=0)
+    for n, p in enumerate(eng.patches):
+        eng.upload(n, "q", q0[p.id])
+    per_v = layout.relax_cell_updates_per_vcycle(params["nsmoothingpasses"])
+
+    for _ in range(args.warmup):
+        eng.time_step()
+    eng.sync()
+
+    lat, cycles, conv_all = [], 0, True
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        s0 = time.perf_counter()
+        hist, conv = eng.time_step()   # returns after the convergence test
+        lat.append(time.perf_counter() - s0)
+        cycles += len(hist)
+        conv_all = conv_all and conv
+    eng.sync()
+    elapsed = time.perf_counter() - t0
+
+    # Roofline of the relaxation kernel: every launch bracketed by HIP events
+    # on the engine's stream, over a second pass of the same number of steps
+    # (launched one by one: events are not recorded inside graph replays).
+    eng.profile_begin(("relax",))
+    for _ in range(args.steps):
+        eng.time_step()
+    prof = eng.profile_end()["relax"]
+    avg_s = prof["ms"] / 1e3 / prof["launches"]
+    bytes_per_launch = prof["bytes"] / prof["launches"]
+    achieved = bytes_per_launch / avg_s / 1e9
+
+    value = cycles * per_v / elapsed
+    levels = layout.finestlevel
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "2D AMR Cahn-Hilliard, root 256^2 + 2 refined levels (ratio 2), "
+                        "FP64 FAS V(2,2) to residual 1e-8, one time step per step",
+            "root": 256, "refined_levels": levels,
+            "patches": len(layout.ordered()),
+            "cells": sum(layout.cells_on_level(l) for l in range(0, levels + 1)),
+            "cell_updates_per_vcycle": per_v,
+            "vcycles": cycles, "all_converged": conv_all,
+            "solves_per_s": args.steps / elapsed,
+            "p50_ms": 1e3 * percentile(lat, 50), "p99_ms": 1e3 * percentile(lat, 99),
+        },
+        "roofline": {
+            "bound": "hbm", "kernel": "k_relax<2> (red-black half-sweep)",
+            "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "launches": prof["launches"], "avg_launch_us": avg_s * 1e6,
+            "bytes_per_launch": bytes_per_launch,
+        },
+        "cpu_baseline": None,
+    }
+    eng.close()
+    if args.cpu_budget > 0:
+        out["cpu_baseline"] = cpu_baseline(layout, q0, params, args.cpu_budget)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -44,6 +44,14 @@ struct Tile {
   int patch, i0, j0, k0;
 };
 
+// One level of the single-patch coarse chain, for the tail kernels: its patch
+// and its three SetGhost programs (ipass 0, 1, 2).
+struct TailLevel {
+  int patch;
+  int ngp[3];
+  const GhostOp *gp[3];
+};
+
 struct Problem {
   double dt, eps2;
   double fc_alpha, fc_shift, fe_num, fe_den;

@@ -67,8 +67,32 @@ struct bsam_ctx {
   double *d_hist = nullptr;
   int *d_state = nullptr;
   int *h_state = nullptr;
+  // kernel timing with HIP events on the context's stream (bsam_profile_*)
+  unsigned prof_mask = 0;
+  std::vector<hipEvent_t> ev_pool;
+  size_t ev_used = 0;
+  struct ProfRec {
+    int kind;
+    double bytes;
+    size_t ev;
+  };
+  std::vector<ProfRec> prof;
+  double prof_ms[BSAM_PROF_NKINDS] = {0};
+  double prof_bytes[BSAM_PROF_NKINDS] = {0};
+  long long prof_n[BSAM_PROF_NKINDS] = {0};
+
+  // tuning (bsam_tune): single-workgroup tail below tail_cells cells per level,
+  // hipGraph replay of the multigrid iteration
+  long long tail_cells = 4096;
+  int tail_threads = 1024;
+  int use_graph = 1;
+  int tail_top = 0;                 // highest level run by the tail kernels
+  TailLevel *d_tail = nullptr;      // levels minlevel..tail_top
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
 
   int nd() const { return cfg.ndims; }
+  int ntail() const { return tail_top - cfg.minlevel + 1; }
   LevelData &L(int level) {
     if (level < cfg.minlevel || level > finest) throw Error("level out of range");
     return lev[level - cfg.minlevel];
@@ -96,6 +120,12 @@ void bsam_ctx::free_device() {
   };
   F(q); F(qold); F(qc); F(f); F(rf); F(ftmp); F(d_P); F(d_partial); F(d_hist);
   F(d_state);
+  F(d_tail);
+  d_tail = nullptr;
+  if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  graph_exec = nullptr;
+  graph = nullptr;
   q = qold = qc = f = rf = ftmp = nullptr;
   d_P = nullptr; d_partial = nullptr; d_hist = nullptr; d_state = nullptr;
   for (auto &l : lev) {
@@ -104,7 +134,34 @@ void bsam_ctx::free_device() {
   }
   if (h_state) (void)hipHostFree(h_state);
   h_state = nullptr;
+  for (auto e : ev_pool) (void)hipEventDestroy(e);
+  ev_pool.clear();
+  ev_used = 0;
+  prof.clear();
 }
+
+// Brackets one launch of `kind` with events when that kind is profiled.
+struct ProfScope {
+  bsam_ctx *c;
+  size_t ev = (size_t)-1;
+  ProfScope(bsam_ctx *cc, int kind, double bytes) : c(cc) {
+    if (!(c->prof_mask & (1u << kind))) return;
+    if (c->ev_used + 2 > c->ev_pool.size()) {
+      for (int n = 0; n < 256; n++) {
+        hipEvent_t e;
+        HIPCK(hipEventCreate(&e));
+        c->ev_pool.push_back(e);
+      }
+    }
+    ev = c->ev_used;
+    c->ev_used += 2;
+    HIPCK(hipEventRecord(c->ev_pool[ev], c->stream));
+    c->prof.push_back({kind, bytes, ev});
+  }
+  ~ProfScope() {
+    if (ev != (size_t)-1) (void)hipEventRecord(c->ev_pool[ev + 1], c->stream);
+  }
+};
 
 static thread_local std::string g_err;
 
@@ -261,6 +318,26 @@ static void commit(bsam_ctx *c) {
     for (int ip = 0; ip < 3; ip++) L.d_gp[ip] = dev_upload(L.gp[ip]);
     maxt = std::max(maxt, L.t_int.size());
   }
+  // tail: the longest run of single-patch levels from minlevel whose cell
+  // counts stay within tail_cells (levels <= 0 hold one patch each)
+  c->tail_top = c->cfg.minlevel - 1;
+  for (int l = c->cfg.minlevel; l <= std::min(0, c->finest); l++)
+    if (c->lev[l - minl].cells <= c->tail_cells) c->tail_top = l;
+    else break;
+  if (c->ntail() > 0) {
+    std::vector<TailLevel> tl;
+    for (int l = c->cfg.minlevel; l <= c->tail_top; l++) {
+      LevelData &L = c->lev[l - minl];
+      TailLevel t;
+      t.patch = L.patches[0];
+      for (int ip = 0; ip < 3; ip++) {
+        t.ngp[ip] = (int)L.gp[ip].size();
+        t.gp[ip] = L.d_gp[ip];
+      }
+      tl.push_back(t);
+    }
+    c->d_tail = dev_upload(tl);
+  }
   c->partial_cap = (int)maxt;
   HIPCK(hipMalloc(&c->d_partial, sizeof(double) * maxt));
   int hmax = std::max(c->cfg.maxvcycles, 1) + 1;
@@ -274,10 +351,12 @@ static void commit(bsam_ctx *c) {
 static void op_set_ghost(bsam_ctx *c, int level, int ipass) {
   LevelData &L = c->L(level);
   if (ipass < 0 || ipass > 2) throw Error("ipass must be 0, 1 or 2");
+  ProfScope ps(c, BSAM_PROF_GHOST, 32.0 * L.gp[ipass].size());
   HIPCK(launch_ghost(c->dev(), L.d_gp[ipass], (int)L.gp[ipass].size(), c->stream));
 }
 static void op_relax(bsam_ctx *c, int level, int color) {
   LevelData &L = c->L(level);
+  ProfScope ps(c, BSAM_PROF_RELAX, 24.0 * L.cells);
   HIPCK(launch_relax(c->nd(), c->dev(), L.d_int, (int)L.t_int.size(), color, c->stream));
 }
 static LevelData &child_level(bsam_ctx *c, int level) {
@@ -286,33 +365,47 @@ static LevelData &child_level(bsam_ctx *c, int level) {
 }
 static void op_restrict(bsam_ctx *c, int level) {
   LevelData &L = child_level(c, level);
+  ProfScope ps(c, BSAM_PROF_RESTRICT, 24.0 * L.cells);
   HIPCK(launch_restrict(c->nd(), c->dev(), L.d_coarse, (int)L.t_coarse.size(), c->stream));
 }
 static void op_coarse_ghost(bsam_ctx *c, int level) {
   LevelData &L = child_level(c, level);
+  ProfScope ps(c, BSAM_PROF_COARSE_GHOST, 0.0);
   HIPCK(launch_coarse_ghost(c->nd(), c->dev(), L.d_cext, (int)L.t_cext.size(), c->stream));
 }
 static void op_coarse_load(bsam_ctx *c, int level) {
   LevelData &L = child_level(c, level);
+  ProfScope ps(c, BSAM_PROF_COARSE_LOAD, 40.0 * L.cells);
   HIPCK(launch_coarse_load(c->nd(), c->dev(), L.d_coarse, (int)L.t_coarse.size(), c->stream));
 }
 static void op_correct(bsam_ctx *c, int level) {
   LevelData &L = child_level(c, level);
+  ProfScope ps(c, BSAM_PROF_CORRECT, 40.0 * L.cells);
   HIPCK(launch_correct(c->nd(), c->dev(), L.d_ext, (int)L.t_ext.size(), c->stream));
 }
 // FillDown, afasroutines.f90:223-239
 static void op_fill_down(bsam_ctx *c, int field) {
   if (field != BSAM_FIELD_F && field != BSAM_FIELD_Q) throw Error("fill_down: field must be F or Q");
-  for (int level = c->finest; level >= c->cfg.minlevel + 1; level--) {
+  int low = std::max(c->cfg.minlevel + 1, c->tail_top + 1);
+  for (int level = c->finest; level >= low; level--) {
     LevelData &L = c->L(level);
-    HIPCK(launch_filldown(c->nd(), c->dev(), L.d_coarse, (int)L.t_coarse.size(),
-                          field == BSAM_FIELD_F ? 1 : 2, c->stream));
+    {
+      ProfScope ps(c, BSAM_PROF_FILLDOWN, 20.0 * L.cells);
+      HIPCK(launch_filldown(c->nd(), c->dev(), L.d_coarse, (int)L.t_coarse.size(),
+                            field == BSAM_FIELD_F ? 1 : 2, c->stream));
+    }
     if (field == BSAM_FIELD_Q) op_set_ghost(c, level - 1, 0);
+  }
+  if (c->ntail() > 1) {
+    ProfScope ps(c, BSAM_PROF_TAIL, 0.0);
+    HIPCK(launch_tail_filldown(c->nd(), c->dev(), c->d_tail, c->ntail(),
+                               field == BSAM_FIELD_F ? 1 : 2, c->tail_threads, c->stream));
   }
 }
 static void op_source(bsam_ctx *c, int full) {
   for (int level = c->finest; level >= 0; level--) {
     LevelData &L = c->L(level);
+    ProfScope ps(c, BSAM_PROF_SOURCE, 32.0 * L.cells);
     HIPCK(launch_source(c->nd(), c->dev(), L.d_int, (int)L.t_int.size(), full, c->stream));
   }
 }
@@ -325,6 +418,12 @@ static void level_relax(bsam_ctx *c, int level) {
 }
 // AFASVcycle, afasroutines.f90:488-521
 static void op_vcycle(bsam_ctx *c, int level) {
+  if (level <= c->tail_top) {
+    ProfScope ps(c, BSAM_PROF_TAIL, 0.0);
+    HIPCK(launch_tail_vcycle(c->nd(), c->dev(), c->d_tail, level - c->cfg.minlevel + 1,
+                             c->cfg.nsmoothingpasses, c->tail_threads, c->stream));
+    return;
+  }
   level_relax(c, level);
   if (level > c->cfg.minlevel) {
     op_restrict(c, level);
@@ -344,7 +443,10 @@ static void reset_state(bsam_ctx *c) {
 static void queue_error(bsam_ctx *c, int level, double tol) {
   LevelData &L = c->L(level);
   int nt = (int)L.t_int.size();
-  HIPCK(launch_resnorm(c->nd(), c->dev(), L.d_int, nt, c->d_partial, 0, c->stream));
+  {
+    ProfScope ps(c, BSAM_PROF_RESNORM, 32.0 * L.cells);
+    HIPCK(launch_resnorm(c->nd(), c->dev(), L.d_int, nt, c->d_partial, 0, c->stream));
+  }
   double count = (double)(c->cfg.nrvars * L.cells);
   HIPCK(launch_finalize(c->d_partial, nt, count, tol, c->d_hist, c->d_state, c->stream));
 }
@@ -355,13 +457,33 @@ static void multigrid_iterations(bsam_ctx *c, int *ncycles, double *hist, int ma
   op_source(c, 1);
   op_fill_down(c, BSAM_FIELD_F);
   reset_state(c);
-  int it = 0, done = 0;
-  for (it = 1; it <= c->cfg.maxvcycles; it++) {
+  // one iteration of the vcycleloop (afasroutines.f90:69-104): the V-cycle,
+  // FillDown(q), source restore, FillDown(f) and the residual norm
+  auto body = [&]() {
     op_vcycle(c, c->finest);
     op_fill_down(c, BSAM_FIELD_Q);
     op_source(c, 0);
     op_fill_down(c, BSAM_FIELD_F);
     queue_error(c, c->finest, c->cfg.qerrortol);
+  };
+  bool graph = c->use_graph && c->prof_mask == 0;
+  if (graph && !c->graph_exec) {
+    HIPCK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    try {
+      body();
+    } catch (...) {
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(c->stream, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    HIPCK(hipStreamEndCapture(c->stream, &c->graph));
+    HIPCK(hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0));
+  }
+  int it = 0, done = 0;
+  for (it = 1; it <= c->cfg.maxvcycles; it++) {
+    if (graph) HIPCK(hipGraphLaunch(c->graph_exec, c->stream));
+    else body();
     HIPCK(hipMemcpyAsync(c->h_state, c->d_state, sizeof(int) * 4, hipMemcpyDeviceToHost,
                          c->stream));
     HIPCK(hipStreamSynchronize(c->stream));
@@ -640,6 +762,72 @@ int bsam_time_step(bsam_ctx *c, int *ncycles, double *hist, int maxhist, int *co
 int bsam_sync(bsam_ctx *c) {
   API_BEGIN(c)
   if (!c->host_only) HIPCK(hipStreamSynchronize(c->stream));
+  API_END(c)
+}
+
+int bsam_tune(bsam_ctx *c, const char *key, double value) {
+  API_BEGIN(c)
+  std::string k = key ? key : "";
+  if (k == "tail_cells") {
+    if (c->committed || c->building) throw Error("tail_cells must be set before the hierarchy");
+    c->tail_cells = (long long)value;
+  } else if (k == "tail_threads") {
+    int t = (int)value;
+    if (t < 64 || t > 1024 || t % 64) throw Error("tail_threads must be 64..1024, multiple of 64");
+    c->tail_threads = t;
+    if (c->graph_exec) {
+      (void)hipGraphExecDestroy(c->graph_exec);
+      (void)hipGraphDestroy(c->graph);
+      c->graph_exec = nullptr;
+      c->graph = nullptr;
+    }
+  } else if (k == "graph") {
+    c->use_graph = value != 0.0;
+  } else {
+    throw Error("unknown tuning key " + k);
+  }
+  API_END(c)
+}
+
+int bsam_profile_begin(bsam_ctx *c, unsigned kind_mask) {
+  API_BEGIN(c)
+  c->need_device();
+  HIPCK(hipStreamSynchronize(c->stream));
+  c->prof_mask = kind_mask;
+  c->prof.clear();
+  c->ev_used = 0;
+  for (int k = 0; k < BSAM_PROF_NKINDS; k++) {
+    c->prof_ms[k] = c->prof_bytes[k] = 0.0;
+    c->prof_n[k] = 0;
+  }
+  API_END(c)
+}
+
+int bsam_profile_end(bsam_ctx *c) {
+  API_BEGIN(c)
+  c->need_device();
+  HIPCK(hipStreamSynchronize(c->stream));
+  for (auto &r : c->prof) {
+    float ms = 0.f;
+    HIPCK(hipEventElapsedTime(&ms, c->ev_pool[r.ev], c->ev_pool[r.ev + 1]));
+    c->prof_ms[r.kind] += ms;
+    c->prof_bytes[r.kind] += r.bytes;
+    c->prof_n[r.kind] += 1;
+  }
+  c->prof.clear();
+  c->ev_used = 0;
+  c->prof_mask = 0;
+  API_END(c)
+}
+
+int bsam_profile_get(const bsam_ctx *cc, int kind, long long *launches, double *ms,
+                     double *bytes) {
+  bsam_ctx *c = const_cast<bsam_ctx *>(cc);
+  API_BEGIN(c)
+  if (kind < 0 || kind >= BSAM_PROF_NKINDS) throw Error("bad profile kind");
+  if (launches) *launches = c->prof_n[kind];
+  if (ms) *ms = c->prof_ms[kind];
+  if (bytes) *bytes = c->prof_bytes[kind];
   API_END(c)
 }
 

@@ -148,35 +148,13 @@ __device__ __forceinline__ double restrict_f(const double *x, const DPatch &P, i
            x[fat(P, i2, j2 - 1, k2 - 1)]) + x[fat(P, i2 - 1, j2 - 1, k2 - 1)]);
 }
 
-// ---- tile decode -----------------------------------------------------------
-template <int ND>
-__device__ __forceinline__ void tile_cell(const Tile &t, int &i, int &j, int &k) {
-  using S = TileShape<ND>;
-  int tid = threadIdx.x;
-  i = t.i0 + tid % S::X;
-  j = t.j0 + (tid / S::X) % S::Y;
-  k = ND == 3 ? t.k0 + tid / (S::X * S::Y) : 0;
-}
+// ---- per-cell operations ----------------------------------------------------
+// Each function performs the reference operation for one cell; the level
+// kernels below map one tile of cells to a workgroup, and the tail kernels
+// loop the same functions over whole single-patch levels in one workgroup.
 
-// ---- kernels ---------------------------------------------------------------
-template <int ND>
-__global__ __launch_bounds__(256) void k_relax(Dev d, const Tile *tiles, int ipass) {
-  Tile t = tiles[blockIdx.x];
-  const DPatch &P = d.P[t.patch];
-  int i, j, k;
-  tile_cell<ND>(t, i, j, k);
-  if (i > P.mx || j > P.my || (ND == 3 && k > P.mz)) return;
-  // RelaxGrid's loop start 1+mod(j+ipass,2) (and +k in 3D) selects the
-  // cells with i+j(+k)+ipass odd.
-  if (((i + j + (ND == 3 ? k : 0) + ipass) & 1) == 0) return;
-  relax_cell<ND>(d, P, i, j, k);
-}
-
-// One SetGhost(level, ipass) program: every ghost-cell update is independent.
-__global__ __launch_bounds__(256) void k_ghost(Dev d, const GhostOp *ops, int n) {
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= n) return;
-  GhostOp g = ops[e];
+// One ghost-cell update of a SetGhost program.
+__device__ __forceinline__ void ghost_op(const Dev &d, const GhostOp &g) {
   double *q = d.q;
   for (int v = 0; v < d.nrvars; v++) {
     const double *f = q + (long long)v * g.fvs;
@@ -204,15 +182,11 @@ __global__ __launch_bounds__(256) void k_ghost(Dev d, const GhostOp *ops, int n)
   }
 }
 
-// RestrictSolution (afasroutines.f90:600-644): qc = R(q) and the parent's
-// q under the child takes the same values.
+// RestrictSolution (afasroutines.f90:600-644) for coarse cell (i,j,k): qc =
+// R(q) and the parent's q under the child takes the same value.
 template <int ND>
-__global__ __launch_bounds__(256) void k_restrict(Dev d, const Tile *tiles) {
-  Tile t = tiles[blockIdx.x];
-  const DPatch &P = d.P[t.patch];
-  int i, j, k;
-  tile_cell<ND>(t, i, j, k);
-  if (i > P.cmx || j > P.cmy || (ND == 3 && k > P.cmz)) return;
+__device__ __forceinline__ void restrict_cell(const Dev &d, const DPatch &P, int i, int j,
+                                              int k) {
   const DPatch &PA = d.P[P.parent];
   View a = qview(d, P), cq = qcview(d, P), pq = qview(d, PA);
   long long pc = pq.at(P.mb[0][0] + i - 1, P.mb[1][0] + j - 1,
@@ -228,12 +202,8 @@ __global__ __launch_bounds__(256) void k_restrict(Dev d, const Tile *tiles) {
 // FillDownLevel (afasroutines.f90:241-310): parent X(mbounds) = R(child X)
 // for X = f (field 1) or q (field 2).
 template <int ND>
-__global__ __launch_bounds__(256) void k_filldown(Dev d, const Tile *tiles, int field) {
-  Tile t = tiles[blockIdx.x];
-  const DPatch &P = d.P[t.patch];
-  int i, j, k;
-  tile_cell<ND>(t, i, j, k);
-  if (i > P.cmx || j > P.cmy || (ND == 3 && k > P.cmz)) return;
+__device__ __forceinline__ void filldown_cell(const Dev &d, const DPatch &P, int i, int j,
+                                              int k, int field) {
   const DPatch &PA = d.P[P.parent];
   int pi = P.mb[0][0] + i - 1, pj = P.mb[1][0] + j - 1,
       pk = ND == 3 ? P.mb[2][0] + k - 1 : 1;
@@ -248,16 +218,11 @@ __global__ __launch_bounds__(256) void k_filldown(Dev d, const Tile *tiles, int 
   }
 }
 
-// GetCoarseGhostPoints (boundary.f90:1203-1269): the ring of qc takes the
-// parent's q around the child footprint (ring cell (ci,cj,ck) <- parent
-// (mb1-1+ci, mb2-1+cj, mb3-1+ck)). Tiles cover 0..cmx+1.
+// GetCoarseGhostPoints (boundary.f90:1203-1269): ring cell (ci,cj,ck) of qc
+// <- parent (mb1-1+ci, mb2-1+cj, mb3-1+ck); (i,j,k) ranges over 0..cmx+1.
 template <int ND>
-__global__ __launch_bounds__(256) void k_coarse_ghost(Dev d, const Tile *tiles) {
-  Tile t = tiles[blockIdx.x];
-  const DPatch &P = d.P[t.patch];
-  int i, j, k;
-  tile_cell<ND>(t, i, j, k);
-  if (i > P.cmx + 1 || j > P.cmy + 1 || (ND == 3 && k > P.cmz + 1)) return;
+__device__ __forceinline__ void coarse_ghost_cell(const Dev &d, const DPatch &P, int i, int j,
+                                                  int k) {
   bool ring = i == 0 || i == P.cmx + 1 || j == 0 || j == P.cmy + 1 ||
               (ND == 3 && (k == 0 || k == P.cmz + 1));
   if (!ring) return;
@@ -271,20 +236,16 @@ __global__ __launch_bounds__(256) void k_coarse_ghost(Dev d, const Tile *tiles) 
 
 // CoarseLoadingFunction (afasroutines.f90:646-714): parent f(mbounds) =
 // R(f - A_h q) + A_2h(qc), the FAS right-hand side, with the fine residual
-// recomputed in registers for the 2^d children of each coarse cell.
+// recomputed in registers for the 2^d children of coarse cell (i,j,k).
 template <int ND>
-__global__ __launch_bounds__(256) void k_coarse_load(Dev d, const Tile *tiles) {
-  Tile t = tiles[blockIdx.x];
-  const DPatch &P = d.P[t.patch];
-  int i, j, k;
-  tile_cell<ND>(t, i, j, k);
-  if (i > P.cmx || j > P.cmy || (ND == 3 && k > P.cmz)) return;
+__device__ __forceinline__ void coarse_load_cell(const Dev &d, const DPatch &P, int i, int j,
+                                                 int k) {
   const DPatch &PA = d.P[P.parent];
   View a = qview(d, P), cq = qcview(d, P);
   const double *f1 = d.f + P.f, *f2 = d.f + P.f + P.fvs;
-  // residual of the children in the order Restriction sums them
   constexpr int NC = ND == 2 ? 4 : 8;
   double r1[NC], r2[NC];
+#pragma unroll
   for (int s = 0; s < NC; s++) {
     int fi = 2 * i - (s & 1), fj = 2 * j - ((s >> 1) & 1);
     int fk = ND == 3 ? 2 * k - ((s >> 2) & 1) : 0;
@@ -311,17 +272,13 @@ __global__ __launch_bounds__(256) void k_coarse_load(Dev d, const Tile *tiles) {
   d.f[po + PA.fvs] = R2 + c2;
 }
 
-// CorrectFine (afasroutines.f90:842-963, mbc = 1): q(0:mx+1) += P1(parent q
-// - qc), with the separable (3,1)/4 prolongation of BiLinProlongationP1 /
-// TriLinProlongationP1 (gridutilities.f90:137-175, :474-529) evaluated per
-// fine cell in the reference's x, y(, z) pass order. Tiles cover 0..mx+1.
+// CorrectFine (afasroutines.f90:842-963, mbc = 1): q += P1(parent q - qc) at
+// fine cell (i,j,k) of 0..mx+1, the separable (3,1)/4 prolongation of
+// BiLinProlongationP1 / TriLinProlongationP1 (gridutilities.f90:137-175,
+// :474-529) evaluated in the reference's x, y(, z) pass order.
 template <int ND>
-__global__ __launch_bounds__(256) void k_correct(Dev d, const Tile *tiles) {
-  Tile t = tiles[blockIdx.x];
-  const DPatch &P = d.P[t.patch];
-  int i, j, k;
-  tile_cell<ND>(t, i, j, k);
-  if (i > P.mx + 1 || j > P.my + 1 || (ND == 3 && k > P.mz + 1)) return;
+__device__ __forceinline__ void correct_cell(const Dev &d, const DPatch &P, int i, int j,
+                                             int k) {
   const DPatch &PA = d.P[P.parent];
   View a = qview(d, P), cq = qcview(d, P), pq = qview(d, PA);
   int kx = i >> 1, ky = j >> 1, kz = ND == 3 ? k >> 1 : 0;
@@ -358,12 +315,8 @@ __global__ __launch_bounds__(256) void k_correct(Dev d, const Tile *tiles) {
 // SourceUpdate = 0; problem.f90:258-303): ftmp = (qold_1, -Dfe(qold_1)) and
 // f = ftmp + 0 when full != 0; only f = ftmp + 0 otherwise.
 template <int ND>
-__global__ __launch_bounds__(256) void k_source(Dev d, const Tile *tiles, int full) {
-  Tile t = tiles[blockIdx.x];
-  const DPatch &P = d.P[t.patch];
-  int i, j, k;
-  tile_cell<ND>(t, i, j, k);
-  if (i > P.mx || j > P.my || (ND == 3 && k > P.mz)) return;
+__device__ __forceinline__ void source_cell(const Dev &d, const DPatch &P, int i, int j,
+                                            int k, int full) {
   long long fo = P.f + fat(P, i, j, ND == 3 ? k : 1);
   if (full) {
     double c = d.qold[P.q + i + (long long)P.px * (j + (long long)P.py * k)];
@@ -373,10 +326,182 @@ __global__ __launch_bounds__(256) void k_source(Dev d, const Tile *tiles, int fu
   for (int v = 0; v < d.nrvars; v++) d.f[fo + v * P.fvs] = d.ftmp[fo + v * P.fvs] + 0.0;
 }
 
+// ---- tile decode -----------------------------------------------------------
+template <int ND>
+__device__ __forceinline__ void tile_cell(const Tile &t, int &i, int &j, int &k) {
+  using S = TileShape<ND>;
+  int tid = threadIdx.x;
+  i = t.i0 + tid % S::X;
+  j = t.j0 + (tid / S::X) % S::Y;
+  k = ND == 3 ? t.k0 + tid / (S::X * S::Y) : 0;
+}
+
+// ---- level kernels ---------------------------------------------------------
+template <int ND>
+__global__ __launch_bounds__(256) void k_relax(Dev d, const Tile *tiles, int ipass) {
+  Tile t = tiles[blockIdx.x];
+  const DPatch &P = d.P[t.patch];
+  int i, j, k;
+  tile_cell<ND>(t, i, j, k);
+  if (i > P.mx || j > P.my || (ND == 3 && k > P.mz)) return;
+  // RelaxGrid's loop start 1+mod(j+ipass,2) (and +k in 3D) selects the
+  // cells with i+j(+k)+ipass odd.
+  if (((i + j + (ND == 3 ? k : 0) + ipass) & 1) == 0) return;
+  relax_cell<ND>(d, P, i, j, k);
+}
+
+// One SetGhost(level, ipass) program: every ghost-cell update is independent.
+__global__ __launch_bounds__(256) void k_ghost(Dev d, const GhostOp *ops, int n) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  ghost_op(d, ops[e]);
+}
+
+template <int ND>
+__global__ __launch_bounds__(256) void k_restrict(Dev d, const Tile *tiles) {
+  Tile t = tiles[blockIdx.x];
+  const DPatch &P = d.P[t.patch];
+  int i, j, k;
+  tile_cell<ND>(t, i, j, k);
+  if (i > P.cmx || j > P.cmy || (ND == 3 && k > P.cmz)) return;
+  restrict_cell<ND>(d, P, i, j, k);
+}
+
+template <int ND>
+__global__ __launch_bounds__(256) void k_filldown(Dev d, const Tile *tiles, int field) {
+  Tile t = tiles[blockIdx.x];
+  const DPatch &P = d.P[t.patch];
+  int i, j, k;
+  tile_cell<ND>(t, i, j, k);
+  if (i > P.cmx || j > P.cmy || (ND == 3 && k > P.cmz)) return;
+  filldown_cell<ND>(d, P, i, j, k, field);
+}
+
+// Tiles cover the ghost-extended coarse box 0..cmx+1.
+template <int ND>
+__global__ __launch_bounds__(256) void k_coarse_ghost(Dev d, const Tile *tiles) {
+  Tile t = tiles[blockIdx.x];
+  const DPatch &P = d.P[t.patch];
+  int i, j, k;
+  tile_cell<ND>(t, i, j, k);
+  if (i > P.cmx + 1 || j > P.cmy + 1 || (ND == 3 && k > P.cmz + 1)) return;
+  coarse_ghost_cell<ND>(d, P, i, j, k);
+}
+
+template <int ND>
+__global__ __launch_bounds__(256) void k_coarse_load(Dev d, const Tile *tiles) {
+  Tile t = tiles[blockIdx.x];
+  const DPatch &P = d.P[t.patch];
+  int i, j, k;
+  tile_cell<ND>(t, i, j, k);
+  if (i > P.cmx || j > P.cmy || (ND == 3 && k > P.cmz)) return;
+  coarse_load_cell<ND>(d, P, i, j, k);
+}
+
+// Tiles cover the ghost-extended fine box 0..mx+1.
+template <int ND>
+__global__ __launch_bounds__(256) void k_correct(Dev d, const Tile *tiles) {
+  Tile t = tiles[blockIdx.x];
+  const DPatch &P = d.P[t.patch];
+  int i, j, k;
+  tile_cell<ND>(t, i, j, k);
+  if (i > P.mx + 1 || j > P.my + 1 || (ND == 3 && k > P.mz + 1)) return;
+  correct_cell<ND>(d, P, i, j, k);
+}
+
+template <int ND>
+__global__ __launch_bounds__(256) void k_source(Dev d, const Tile *tiles, int full) {
+  Tile t = tiles[blockIdx.x];
+  const DPatch &P = d.P[t.patch];
+  int i, j, k;
+  tile_cell<ND>(t, i, j, k);
+  if (i > P.mx || j > P.my || (ND == 3 && k > P.mz)) return;
+  source_cell<ND>(d, P, i, j, k, full);
+}
+
+// ---- single-workgroup tail: the single-patch coarse levels ------------------
+// Below the root every level is one patch with physical boundaries only. The
+// tail kernels run the part of the V-cycle (or of FillDown) on levels
+// lo..hi of that chain in one workgroup, synchronising with barriers instead
+// of launches.
+template <int ND, class F>
+__device__ __forceinline__ void for_cells(int ex, int ey, int ez, int base, F f) {
+  int n = ex * ey * ez;
+  for (int e = threadIdx.x; e < n; e += blockDim.x) {
+    int i = e % ex, j = (e / ex) % ey, k = e / (ex * ey);
+    f(i + base, j + base, ND == 3 ? k + base : 0);
+  }
+}
+
+template <int ND>
+__device__ void tail_ghost(const Dev &d, const TailLevel &L, int ipass) {
+  const GhostOp *ops = L.gp[ipass];
+  for (int e = threadIdx.x; e < L.ngp[ipass]; e += blockDim.x) ghost_op(d, ops[e]);
+  __syncthreads();
+}
+
+template <int ND>
+__device__ void tail_relax(const Dev &d, const TailLevel &L, int nsmooth) {
+  const DPatch &P = d.P[L.patch];
+  for (int pass = 0; pass < nsmooth; pass++)
+    for (int color = 1; color <= 2; color++) {
+      for_cells<ND>(P.mx, P.my, ND == 3 ? P.mz : 1, 1, [&](int i, int j, int k) {
+        if (((i + j + (ND == 3 ? k : 0) + color) & 1) == 1) relax_cell<ND>(d, P, i, j, k);
+      });
+      __syncthreads();
+      tail_ghost<ND>(d, L, color);
+    }
+}
+
+// AFASVcycle(hi) (afasroutines.f90:488-521) restricted to levels lo..hi of
+// the single-patch chain; levels are indexed from lo.
+template <int ND>
+__global__ __launch_bounds__(1024) void k_tail_vcycle(Dev d, const TailLevel *lv, int nlev,
+                                                       int nsmooth) {
+  for (int l = nlev - 1; l >= 1; l--) {
+    const DPatch &P = d.P[lv[l].patch];
+    int cz = ND == 3 ? P.cmz : 1;
+    tail_relax<ND>(d, lv[l], nsmooth);
+    for_cells<ND>(P.cmx, P.cmy, cz, 1,
+                  [&](int i, int j, int k) { restrict_cell<ND>(d, P, i, j, k); });
+    __syncthreads();
+    tail_ghost<ND>(d, lv[l - 1], 0);
+    for_cells<ND>(P.cmx + 2, P.cmy + 2, ND == 3 ? P.cmz + 2 : 1, 0,
+                  [&](int i, int j, int k) { coarse_ghost_cell<ND>(d, P, i, j, k); });
+    __syncthreads();
+    for_cells<ND>(P.cmx, P.cmy, cz, 1,
+                  [&](int i, int j, int k) { coarse_load_cell<ND>(d, P, i, j, k); });
+    __syncthreads();
+  }
+  tail_relax<ND>(d, lv[0], nsmooth);
+  for (int l = 1; l < nlev; l++) {
+    const DPatch &P = d.P[lv[l].patch];
+    for_cells<ND>(P.mx + 2, P.my + 2, ND == 3 ? P.mz + 2 : 1, 0,
+                  [&](int i, int j, int k) { correct_cell<ND>(d, P, i, j, k); });
+    __syncthreads();
+    tail_ghost<ND>(d, lv[l], 0);
+    tail_relax<ND>(d, lv[l], nsmooth);
+  }
+}
+
+// FillDown (afasroutines.f90:223-239) from level hi down to lo of the chain:
+// restriction into the parent, and for q a SetGhost(parent, 0).
+template <int ND>
+__global__ __launch_bounds__(1024) void k_tail_filldown(Dev d, const TailLevel *lv, int nlev,
+                                                         int field) {
+  for (int l = nlev - 1; l >= 1; l--) {
+    const DPatch &P = d.P[lv[l].patch];
+    for_cells<ND>(P.cmx, P.cmy, ND == 3 ? P.cmz : 1, 1,
+                  [&](int i, int j, int k) { filldown_cell<ND>(d, P, i, j, k, field); });
+    __syncthreads();
+    if (field == 2) tail_ghost<ND>(d, lv[l - 1], 0);
+  }
+}
+
 // Residual of the finest level (Residual, afasroutines.f90:799-840) reduced
 // to one sum of squares per tile; deterministic (fixed order) reduction.
 __device__ __forceinline__ double block_sum(double s) {
-  __shared__ double red[8];
+  __shared__ double red[16];
   for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
   int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   if (lane == 0) red[w] = s;
@@ -413,9 +538,9 @@ __global__ __launch_bounds__(256) void k_resnorm(Dev d, const Tile *tiles, doubl
   if (threadIdx.x == 0) partial[blockIdx.x] = tot;
 }
 
-// Sum the per-tile partials in index order (256 threads, strided then
-// reduced in a fixed tree) and finish ErrorAFAS: sqrt(sum / count).
-// out[0] = residual; flags: out_it[0] counts calls, done set below tol.
+// Sum the per-tile partials in index order (strided, then a fixed tree) and
+// finish ErrorAFAS: sqrt(sum / count) into hist[state[0]]; state[0] counts
+// the calls and state[1] is set once a residual falls below tol.
 __global__ __launch_bounds__(256) void k_finalize(const double *partial, int n, double count,
                                                    double tol, double *hist, int *state) {
   double s = 0.0;
@@ -518,6 +643,22 @@ hipError_t launch_copy(double *dst, const double *src, long long n, hipStream_t 
   long long blocks = (n + 255) / 256;
   if (blocks > 4096) blocks = 4096;
   k_copy<<<dim3((unsigned)blocks), 256, 0, s>>>(dst, src, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_tail_vcycle(int nd, const Dev &d, const TailLevel *lv, int nlev, int nsmooth,
+                              int threads, hipStream_t s) {
+  if (nlev <= 0) return hipSuccess;
+  if (nd == 2) k_tail_vcycle<2><<<1, threads, 0, s>>>(d, lv, nlev, nsmooth);
+  else k_tail_vcycle<3><<<1, threads, 0, s>>>(d, lv, nlev, nsmooth);
+  return hipGetLastError();
+}
+
+hipError_t launch_tail_filldown(int nd, const Dev &d, const TailLevel *lv, int nlev, int field,
+                                int threads, hipStream_t s) {
+  if (nlev <= 1) return hipSuccess;
+  if (nd == 2) k_tail_filldown<2><<<1, threads, 0, s>>>(d, lv, nlev, field);
+  else k_tail_filldown<3><<<1, threads, 0, s>>>(d, lv, nlev, field);
   return hipGetLastError();
 }
 

@@ -23,6 +23,11 @@ hipError_t launch_finalize(const double *partial, int n, double count, double to
                            double *hist, int *state, hipStream_t s);
 hipError_t launch_copy(double *dst, const double *src, long long n, hipStream_t s);
 
+hipError_t launch_tail_vcycle(int nd, const Dev &d, const TailLevel *lv, int nlev, int nsmooth,
+                              int threads, hipStream_t s);
+hipError_t launch_tail_filldown(int nd, const Dev &d, const TailLevel *lv, int nlev, int field,
+                                int threads, hipStream_t s);
+
 int tile_x(int nd);
 int tile_y(int nd);
 int tile_z(int nd);

@@ -82,6 +82,11 @@ def lib():
         L.bsam_error_afas.argtypes = [vp, C.c_int, dp]
         for n in ("bsam_multigrid_iterations", "bsam_time_step"):
             getattr(L, n).argtypes = [vp, ip, dp, C.c_int, ip]
+        L.bsam_tune.argtypes = [vp, C.c_char_p, C.c_double]
+        L.bsam_profile_begin.argtypes = [vp, C.c_uint]
+        L.bsam_profile_end.argtypes = [vp]
+        L.bsam_profile_get.argtypes = [vp, C.c_int, C.POINTER(C.c_longlong),
+                                       C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.bsam_ghost_program_size.argtypes = [vp, C.c_int, C.c_int]
         L.bsam_ghost_program_dst.argtypes = [vp, C.c_int, C.c_int, ip, C.c_int]
         L.bsam_ghost_program_apply_host.argtypes = [vp, C.c_int, C.c_int,
@@ -108,7 +113,8 @@ class Engine:
     def __init__(self, layout: Layout, dt: float, eps: float, alpha: float = 1.0,
                  well: str = "reference", nsmoothingpasses: int = 2,
                  maxvcycles: int = 20, qerrortol: float = 1e-8,
-                 precision: int = PREC_FP64, device: int = 0):
+                 precision: int = PREC_FP64, device: int = 0,
+                 tune: Optional[Dict[str, float]] = None):
         L = lib()
         fa, fs, fn, fd = WELLS[well](alpha)
         cfg = Cfg(layout.ndims, layout.nrvars, layout.mbc, layout.minlevel, dt, eps * eps,
@@ -118,6 +124,8 @@ class Engine:
         self.ctx = L.bsam_ctx_create(C.byref(cfg))
         if not self.ctx:
             raise BsamError(L.bsam_last_error(None).decode())
+        for key, val in (tune or {}).items():
+            self._ck(L.bsam_tune(self.ctx, key.encode(), float(val)))
         self.patches = layout.ordered()
         idx = layout.index_map()
         self._ck(L.bsam_hierarchy_begin(self.ctx))
@@ -210,6 +218,31 @@ class Engine:
 
     def time_step(self):
         return self._solve(lib().bsam_time_step)
+
+    def tune(self, key: str, value: float):
+        self._ck(lib().bsam_tune(self.ctx, key.encode(), float(value)))
+
+    # -- kernel timing ----------------------------------------------------------
+    PROF_KINDS = ("relax", "ghost", "restrict", "coarse_ghost", "coarse_load", "correct",
+                  "filldown", "source", "resnorm", "tail")
+
+    def profile_begin(self, kinds=("relax",)):
+        mask = 0
+        for k in kinds:
+            mask |= 1 << self.PROF_KINDS.index(k)
+        self._ck(lib().bsam_profile_begin(self.ctx, mask))
+
+    def profile_end(self) -> Dict[str, dict]:
+        """Per kernel kind: launches, summed ms and summed algorithmic bytes."""
+        self._ck(lib().bsam_profile_end(self.ctx))
+        out = {}
+        for n, k in enumerate(self.PROF_KINDS):
+            cnt, ms, by = C.c_longlong(), C.c_double(), C.c_double()
+            self._ck(lib().bsam_profile_get(self.ctx, n, C.byref(cnt), C.byref(ms),
+                                            C.byref(by)))
+            if cnt.value:
+                out[k] = {"launches": cnt.value, "ms": ms.value, "bytes": by.value}
+        return out
 
     # -- integer maps ---------------------------------------------------------
     def ghost_program_dst(self, level: int, ipass: int) -> np.ndarray:

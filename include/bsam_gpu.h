@@ -132,6 +132,32 @@ int bsam_time_step(bsam_ctx *ctx, int *ncycles, double *hist, int maxhist,
 /* Wait for all work queued on the context's stream. */
 int bsam_sync(bsam_ctx *ctx);
 
+/* Tuning knobs (no counterpart in the reference):
+ *   "tail_cells"   levels from minlevel up whose single patch has at most
+ *                  this many cells run inside one workgroup (default 4096;
+ *                  0 disables). Set before bsam_hierarchy_begin.
+ *   "tail_threads" workgroup size of that kernel (64..1024, default 1024).
+ *   "graph"        1 (default): replay each multigrid iteration as a
+ *                  captured hipGraph; 0: launch kernels one by one. */
+int bsam_tune(bsam_ctx *ctx, const char *key, double value);
+
+/* Kernel timing. Between bsam_profile_begin and bsam_profile_end, every
+ * launch of a kind whose bit is set in kind_mask is bracketed by HIP events
+ * on the context's stream; bsam_profile_get then returns, per kind, the
+ * number of launches, their summed duration (ms) and their summed
+ * algorithmic HBM bytes (FP64, ghosts excluded: relax half-sweep 24 B,
+ * residual norm 32 B, restriction 24 B, coarse loading 40 B, correction
+ * 40 B, fill-down 20 B, source 32 B per cell of the level operated on). */
+enum {
+  BSAM_PROF_RELAX = 0, BSAM_PROF_GHOST, BSAM_PROF_RESTRICT, BSAM_PROF_COARSE_GHOST,
+  BSAM_PROF_COARSE_LOAD, BSAM_PROF_CORRECT, BSAM_PROF_FILLDOWN, BSAM_PROF_SOURCE,
+  BSAM_PROF_RESNORM, BSAM_PROF_TAIL, BSAM_PROF_NKINDS
+};
+int bsam_profile_begin(bsam_ctx *ctx, unsigned kind_mask);
+int bsam_profile_end(bsam_ctx *ctx);
+int bsam_profile_get(const bsam_ctx *ctx, int kind, long long *launches, double *ms,
+                     double *bytes);
+
 /* Introspection of the integer maps (valid on host-only contexts too).
  * Ghost-fill program of (level, ipass): number of ghost-cell updates, and
  * their destinations as q-field (patch, i, j, k) tuples, 4 ints each. */

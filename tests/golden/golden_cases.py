@@ -40,12 +40,24 @@ def _rect2d():
     return L
 
 
+def _amr3d(mthbc=(1, 1, 1, 1, 1, 1)):
+    L = Layout(3, (8, 8, 8), 1.0 / 8, mthbc=mthbc)
+    c1 = L.add_child(0, ((2, 5), (3, 6), (1, 4)))
+    L.add_child(0, ((6, 8), (3, 6), (1, 4)))
+    L.add_child(c1, ((2, 5), (3, 6), (2, 5)))
+    L.add_child(c1, ((6, 8), (1, 4), (3, 6)))
+    return L
+
+
 LAYOUTS = {
     "single32": lambda: Layout(2, (32, 32), 1.0 / 32),
     "c1_128": lambda: Layout(2, (128, 128), 1.0 / 128),
     "amr2d": _amr2d,
     "amr2d_userbc": _amr2d_userbc,
     "rect2d": _rect2d,
+    "amr3d": _amr3d,
+    "amr3d_userbc": lambda: _amr3d((1, 10, 1, 10, 10, 1)),
+    "single16_3d": lambda: Layout(3, (16, 16, 16), 1.0 / 16),
 }
 
 OPS = [("setghost", 1, 0), ("setghost", 2, 1), ("setghost", 2, 2), ("relax", 2, 1),
@@ -59,6 +71,8 @@ OPS_CASES = {
     "amr2d_vcycle": ("amr2d", 2, [("vcycle", 2)]),
     "userbc_seq": ("amr2d_userbc", 3, OPS),
     "rect2d_vcycle": ("rect2d", 4, [("vcycle", 2)]),
+    "amr3d_seq": ("amr3d", 7, OPS),
+    "amr3d_userbc_vcycle": ("amr3d_userbc", 8, [("vcycle", 2)]),
 }
 
 
@@ -93,6 +107,10 @@ STEP_CASES = {
                          alpha=2.0, eps=0.05, maxv=40, tol=1e-9, steps=2),
     "rect2d": dict(layout="rect2d", init=noise_init(0.5, 0.1, 5), dt=5e-2, alpha=1.0,
                    eps=0.3, maxv=40, tol=1e-8, steps=2),
+    "amr3d": dict(layout="amr3d", init=noise_init(0.5, 0.05, 7), dt=5e-4, alpha=1.0,
+                  eps=0.02 * 8, maxv=40, tol=1e-8, steps=2),
+    "single16_3d": dict(layout="single16_3d", init=noise_init(0.5, 0.05, 8), dt=5e-4,
+                        alpha=1.0, eps=0.1, maxv=40, tol=1e-8, steps=2),
     # fixed 6 V-cycles (tol 0) of the mapped +-1 problem, phi ~ 0.1 + U(-.05,.05)
     "plusminus_map": dict(layout="amr2d", init=noise_init(0.55, 0.025, 6), dt=PM_DT / 2,
                           alpha=8.0, eps=math.sqrt(2.0) * PM_EPS, maxv=6, tol=0.0, steps=1),

@@ -52,3 +52,14 @@ def noise_q(layout, seed=0, mean=0.5, amp=0.05):
         q[..., 0] = mean + rng.uniform(-amp, amp, q.shape[:3])
         out[p.id] = q
     return out
+
+
+def layout_amr3d(mthbc=(1, 1, 1, 1, 1, 1)):
+    """8^3 root, two touching level-1 blocks, level-2 blocks (one touching a
+    physical face of its parent)."""
+    L = Layout(3, (8, 8, 8), 1.0 / 8, mthbc=mthbc)
+    c1 = L.add_child(0, ((2, 5), (3, 6), (1, 4)))
+    L.add_child(0, ((6, 8), (3, 6), (1, 4)))
+    L.add_child(c1, ((2, 5), (3, 6), (2, 5)))
+    L.add_child(c1, ((6, 8), (1, 4), (3, 6)))
+    return L

@@ -10,7 +10,7 @@ import pytest
 
 import orc
 from bsam_amd.engine import Engine
-from helpers import layout_amr2d, layout_amr2d_tiles, layout_single
+from helpers import layout_amr2d, layout_amr2d_tiles, layout_amr3d, layout_single
 
 LAYOUTS = {
     "single16": lambda: layout_single(16),
@@ -19,6 +19,10 @@ LAYOUTS = {
     "amr2d_user_bc": lambda: layout_amr2d(mthbc=(10, 10, 10, 10)),
     "amr2d_tiles": layout_amr2d_tiles,
     "amr2d_tiles_mixed_bc": lambda: layout_amr2d_tiles(mthbc=(1, 10, 10, 1)),
+    "single8_3d": lambda: layout_single(8, ndims=3),
+    "amr3d": layout_amr3d,
+    "amr3d_user_bc": lambda: layout_amr3d(mthbc=(10, 10, 10, 10, 10, 10)),
+    "amr3d_mixed_bc": lambda: layout_amr3d(mthbc=(1, 10, 1, 10, 10, 1)),
 }
 
 

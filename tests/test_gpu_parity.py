@@ -11,8 +11,8 @@ import pytest
 
 import orc
 from bsam_amd.engine import Engine
-from helpers import (layout_amr2d, layout_amr2d_tiles, layout_single, noise_q,
-                     random_fields)
+from helpers import (layout_amr2d, layout_amr2d_tiles, layout_amr3d, layout_single,
+                     noise_q, random_fields)
 
 pytestmark = pytest.mark.gpu
 
@@ -22,11 +22,23 @@ LAYOUTS = {
     "amr2d": layout_amr2d,
     "amr2d_tiles": layout_amr2d_tiles,
     "amr2d_user_bc": lambda: layout_amr2d(mthbc=(10, 10, 10, 10)),
+    "single8_3d": lambda: layout_single(8, ndims=3),
+    "amr3d": layout_amr3d,
+    "amr3d_mixed_bc": lambda: layout_amr3d(mthbc=(1, 10, 1, 10, 10, 1)),
 }
 
 
-def pair(layout, fields, names=("q", "f", "qc")):
-    eng = Engine(layout, device=0, **KW)
+# engine schedules: per-level launches only; default (single-workgroup tail
+# for small coarse levels, hipGraph replay); every single-patch level in the tail
+SCHEDULES = {
+    "launch": {"tail_cells": 0, "graph": 0},
+    "default": {},
+    "all_tail": {"tail_cells": 1 << 30, "graph": 1},
+}
+
+
+def pair(layout, fields, names=("q", "f", "qc"), tune=None):
+    eng = Engine(layout, device=0, tune=tune, **KW)
     ora = orc.Oracle(layout, **KW)
     for n, p in enumerate(ora.patches):
         for name in names:
@@ -96,10 +108,11 @@ def test_fill_down_and_source(name):
     eng.close()
 
 
+@pytest.mark.parametrize("sched", sorted(SCHEDULES))
 @pytest.mark.parametrize("name", sorted(LAYOUTS))
-def test_vcycle_bitwise(name):
+def test_vcycle_bitwise(name, sched):
     lay = LAYOUTS[name]()
-    eng, ora = pair(lay, random_fields(lay, seed=3))
+    eng, ora = pair(lay, random_fields(lay, seed=3), tune=SCHEDULES[sched])
     eng.vcycle(lay.finestlevel)
     ora.vcycle(lay.finestlevel)
     assert_same(eng, ora, names=("q", "f"))
@@ -115,11 +128,13 @@ def test_error_afas(name):
         assert abs(r - want) <= 1e-12 * want
 
 
+@pytest.mark.parametrize("sched", sorted(SCHEDULES))
 @pytest.mark.parametrize("name", sorted(LAYOUTS))
-def test_time_steps(name):
+def test_time_steps(name, sched):
     lay = LAYOUTS[name]()
     q0 = noise_q(lay, seed=11)
-    eng, ora = pair(lay, {k: {"q": v} for k, v in q0.items()}, names=("q",))
+    eng, ora = pair(lay, {k: {"q": v} for k, v in q0.items()}, names=("q",),
+                    tune=SCHEDULES[sched])
     for step in range(3):
         hist, conv = eng.time_step()
         ref = ora.time_step()
