@@ -57,13 +57,21 @@ struct Problem {
   double fc_alpha, fc_shift, fe_num, fe_den;
 };
 
-// Kernel argument bundle: pool base pointers plus the patch table.
+// Kernel argument bundle: pool base pointers plus the patch table. The FP64
+// pools hold the solution; the FP32 pools (mixed precision only) hold the
+// correction e, its coarse restriction qce, its right-hand side fe and the
+// FP32 copy qb of the solution at which the defect operator is linearised.
+// FP32 pools share the offsets of their FP64 counterparts.
 struct Dev {
   double *q, *qold, *qc, *f, *rf, *ftmp;
+  float *e, *qce, *fe, *qb;
   const DPatch *P;
   Problem prm;
   int nrvars;
 };
+
+// Which field set a level operation acts on.
+enum { PREC_SOLUTION_FP64 = 0, PREC_DEFECT_FP32 = 1 };
 
 // Coarse-fine interpolation weights (lib/boundary.f90:45-54).
 #define BSAM_A1 (-1.0 / 5.0)

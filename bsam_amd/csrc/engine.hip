@@ -61,6 +61,9 @@ struct bsam_ctx {
   long long nq = 0, nc = 0, nf = 0;
   double *q = nullptr, *qold = nullptr, *qc = nullptr, *f = nullptr, *rf = nullptr,
          *ftmp = nullptr;
+  float *e = nullptr, *qce = nullptr, *fe = nullptr, *qb = nullptr;  // mixed precision
+  int inner_cycles = 1;      // FP32 V-cycles per defect-correction step
+  double fnorm = 0.0;        // RMS of f on the finest level (relative residual)
   DPatch *d_P = nullptr;
   double *d_partial = nullptr;
   int partial_cap = 0;
@@ -88,8 +91,8 @@ struct bsam_ctx {
   int use_graph = 1;
   int tail_top = 0;                 // highest level run by the tail kernels
   TailLevel *d_tail = nullptr;      // levels minlevel..tail_top
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t graph_exec = nullptr;
+  hipGraph_t graph = nullptr, graph_mixed = nullptr;
+  hipGraphExec_t graph_exec = nullptr, graph_mixed_exec = nullptr;
 
   int nd() const { return cfg.ndims; }
   int ntail() const { return tail_top - cfg.minlevel + 1; }
@@ -100,6 +103,7 @@ struct bsam_ctx {
   Dev dev() const {
     Dev d;
     d.q = q; d.qold = qold; d.qc = qc; d.f = f; d.rf = rf; d.ftmp = ftmp;
+    d.e = e; d.qce = qce; d.fe = fe; d.qb = qb;
     d.P = d_P;
     d.prm = Problem{cfg.dt, cfg.eps2, cfg.fc_alpha, cfg.fc_shift, cfg.fe_num, cfg.fe_den};
     d.nrvars = cfg.nrvars;
@@ -110,6 +114,14 @@ struct bsam_ctx {
     if (!committed) throw Error("hierarchy not committed");
   }
   void free_device();
+  void drop_graphs() {
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (graph_mixed_exec) (void)hipGraphExecDestroy(graph_mixed_exec);
+    if (graph_mixed) (void)hipGraphDestroy(graph_mixed);
+    graph_exec = graph_mixed_exec = nullptr;
+    graph = graph_mixed = nullptr;
+  }
   ~bsam_ctx() { free_device(); }
 };
 
@@ -119,13 +131,12 @@ void bsam_ctx::free_device() {
     if (p) (void)hipFree(p);
   };
   F(q); F(qold); F(qc); F(f); F(rf); F(ftmp); F(d_P); F(d_partial); F(d_hist);
+  F(e); F(qce); F(fe); F(qb);
+  e = qce = fe = qb = nullptr;
   F(d_state);
   F(d_tail);
   d_tail = nullptr;
-  if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
-  if (graph) (void)hipGraphDestroy(graph);
-  graph_exec = nullptr;
-  graph = nullptr;
+  drop_graphs();
   q = qold = qc = f = rf = ftmp = nullptr;
   d_P = nullptr; d_partial = nullptr; d_hist = nullptr; d_state = nullptr;
   for (auto &l : lev) {
@@ -308,6 +319,16 @@ static void commit(bsam_ctx *c) {
   HIPCK(hipMemset(c->f, 0, sizeof(double) * c->nf));
   HIPCK(hipMemset(c->rf, 0, sizeof(double) * c->nf));
   HIPCK(hipMemset(c->ftmp, 0, sizeof(double) * c->nf));
+  if (c->cfg.precision == BSAM_PREC_MIXED) {
+    HIPCK(hipMalloc(&c->e, sizeof(float) * c->nq));
+    HIPCK(hipMalloc(&c->qb, sizeof(float) * c->nq));
+    HIPCK(hipMalloc(&c->qce, sizeof(float) * c->nc));
+    HIPCK(hipMalloc(&c->fe, sizeof(float) * c->nf));
+    HIPCK(hipMemset(c->e, 0, sizeof(float) * c->nq));
+    HIPCK(hipMemset(c->qb, 0, sizeof(float) * c->nq));
+    HIPCK(hipMemset(c->qce, 0, sizeof(float) * c->nc));
+    HIPCK(hipMemset(c->fe, 0, sizeof(float) * c->nf));
+  }
   c->d_P = dev_upload(c->dp);
   size_t maxt = 1;
   for (auto &L : c->lev) {
@@ -348,40 +369,40 @@ static void commit(bsam_ctx *c) {
 }
 
 // ---- level operations ------------------------------------------------------------
-static void op_set_ghost(bsam_ctx *c, int level, int ipass) {
+static void op_set_ghost(bsam_ctx *c, int level, int ipass, int prec = PREC_SOLUTION_FP64) {
   LevelData &L = c->L(level);
   if (ipass < 0 || ipass > 2) throw Error("ipass must be 0, 1 or 2");
   ProfScope ps(c, BSAM_PROF_GHOST, 32.0 * L.gp[ipass].size());
-  HIPCK(launch_ghost(c->dev(), L.d_gp[ipass], (int)L.gp[ipass].size(), c->stream));
+  HIPCK(launch_ghost(prec, c->dev(), L.d_gp[ipass], (int)L.gp[ipass].size(), c->stream));
 }
-static void op_relax(bsam_ctx *c, int level, int color) {
+static void op_relax(bsam_ctx *c, int level, int color, int prec = PREC_SOLUTION_FP64) {
   LevelData &L = c->L(level);
-  ProfScope ps(c, BSAM_PROF_RELAX, 24.0 * L.cells);
-  HIPCK(launch_relax(c->nd(), c->dev(), L.d_int, (int)L.t_int.size(), color, c->stream));
+  ProfScope ps(c, BSAM_PROF_RELAX, (prec == PREC_SOLUTION_FP64 ? 24.0 : 12.0) * L.cells);
+  HIPCK(launch_relax(c->nd(), prec, c->dev(), L.d_int, (int)L.t_int.size(), color, c->stream));
 }
 static LevelData &child_level(bsam_ctx *c, int level) {
   if (level <= c->cfg.minlevel) throw Error("level has no parent level");
   return c->L(level);
 }
-static void op_restrict(bsam_ctx *c, int level) {
+static void op_restrict(bsam_ctx *c, int level, int prec = PREC_SOLUTION_FP64) {
   LevelData &L = child_level(c, level);
   ProfScope ps(c, BSAM_PROF_RESTRICT, 24.0 * L.cells);
-  HIPCK(launch_restrict(c->nd(), c->dev(), L.d_coarse, (int)L.t_coarse.size(), c->stream));
+  HIPCK(launch_restrict(c->nd(), prec, c->dev(), L.d_coarse, (int)L.t_coarse.size(), c->stream));
 }
-static void op_coarse_ghost(bsam_ctx *c, int level) {
+static void op_coarse_ghost(bsam_ctx *c, int level, int prec = PREC_SOLUTION_FP64) {
   LevelData &L = child_level(c, level);
   ProfScope ps(c, BSAM_PROF_COARSE_GHOST, 0.0);
-  HIPCK(launch_coarse_ghost(c->nd(), c->dev(), L.d_cext, (int)L.t_cext.size(), c->stream));
+  HIPCK(launch_coarse_ghost(c->nd(), prec, c->dev(), L.d_cext, (int)L.t_cext.size(), c->stream));
 }
-static void op_coarse_load(bsam_ctx *c, int level) {
+static void op_coarse_load(bsam_ctx *c, int level, int prec = PREC_SOLUTION_FP64) {
   LevelData &L = child_level(c, level);
   ProfScope ps(c, BSAM_PROF_COARSE_LOAD, 40.0 * L.cells);
-  HIPCK(launch_coarse_load(c->nd(), c->dev(), L.d_coarse, (int)L.t_coarse.size(), c->stream));
+  HIPCK(launch_coarse_load(c->nd(), prec, c->dev(), L.d_coarse, (int)L.t_coarse.size(), c->stream));
 }
-static void op_correct(bsam_ctx *c, int level) {
+static void op_correct(bsam_ctx *c, int level, int prec = PREC_SOLUTION_FP64) {
   LevelData &L = child_level(c, level);
   ProfScope ps(c, BSAM_PROF_CORRECT, 40.0 * L.cells);
-  HIPCK(launch_correct(c->nd(), c->dev(), L.d_ext, (int)L.t_ext.size(), c->stream));
+  HIPCK(launch_correct(c->nd(), prec, c->dev(), L.d_ext, (int)L.t_ext.size(), c->stream));
 }
 // FillDown, afasroutines.f90:223-239
 static void op_fill_down(bsam_ctx *c, int field) {
@@ -409,31 +430,31 @@ static void op_source(bsam_ctx *c, int full) {
     HIPCK(launch_source(c->nd(), c->dev(), L.d_int, (int)L.t_int.size(), full, c->stream));
   }
 }
-static void level_relax(bsam_ctx *c, int level) {
+static void level_relax(bsam_ctx *c, int level, int prec = PREC_SOLUTION_FP64) {
   for (int pass = 1; pass <= c->cfg.nsmoothingpasses; pass++)
     for (int color = 1; color <= 2; color++) {
-      op_relax(c, level, color);
-      op_set_ghost(c, level, color);
+      op_relax(c, level, color, prec);
+      op_set_ghost(c, level, color, prec);
     }
 }
 // AFASVcycle, afasroutines.f90:488-521
-static void op_vcycle(bsam_ctx *c, int level) {
+static void op_vcycle(bsam_ctx *c, int level, int prec = PREC_SOLUTION_FP64) {
   if (level <= c->tail_top) {
     ProfScope ps(c, BSAM_PROF_TAIL, 0.0);
-    HIPCK(launch_tail_vcycle(c->nd(), c->dev(), c->d_tail, level - c->cfg.minlevel + 1,
+    HIPCK(launch_tail_vcycle(c->nd(), prec, c->dev(), c->d_tail, level - c->cfg.minlevel + 1,
                              c->cfg.nsmoothingpasses, c->tail_threads, c->stream));
     return;
   }
-  level_relax(c, level);
+  level_relax(c, level, prec);
   if (level > c->cfg.minlevel) {
-    op_restrict(c, level);
-    op_set_ghost(c, level - 1, 0);
-    op_coarse_ghost(c, level);
-    op_coarse_load(c, level);
-    op_vcycle(c, level - 1);
-    op_correct(c, level);
-    op_set_ghost(c, level, 0);
-    level_relax(c, level);
+    op_restrict(c, level, prec);
+    op_set_ghost(c, level - 1, 0, prec);
+    op_coarse_ghost(c, level, prec);
+    op_coarse_load(c, level, prec);
+    op_vcycle(c, level - 1, prec);
+    op_correct(c, level, prec);
+    op_set_ghost(c, level, 0, prec);
+    level_relax(c, level, prec);
   }
 }
 static void reset_state(bsam_ctx *c) {
@@ -451,38 +472,31 @@ static void queue_error(bsam_ctx *c, int level, double tol) {
   HIPCK(launch_finalize(c->d_partial, nt, count, tol, c->d_hist, c->d_state, c->stream));
 }
 
-static void multigrid_iterations(bsam_ctx *c, int *ncycles, double *hist, int maxhist,
-                                 int *converged) {
-  op_fill_down(c, BSAM_FIELD_Q);
-  op_source(c, 1);
-  op_fill_down(c, BSAM_FIELD_F);
-  reset_state(c);
-  // one iteration of the vcycleloop (afasroutines.f90:69-104): the V-cycle,
-  // FillDown(q), source restore, FillDown(f) and the residual norm
-  auto body = [&]() {
-    op_vcycle(c, c->finest);
-    op_fill_down(c, BSAM_FIELD_Q);
-    op_source(c, 0);
-    op_fill_down(c, BSAM_FIELD_F);
-    queue_error(c, c->finest, c->cfg.qerrortol);
-  };
-  bool graph = c->use_graph && c->prof_mask == 0;
-  if (graph && !c->graph_exec) {
-    HIPCK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    try {
-      body();
-    } catch (...) {
-      hipGraph_t g = nullptr;
-      (void)hipStreamEndCapture(c->stream, &g);
-      if (g) (void)hipGraphDestroy(g);
-      throw;
-    }
-    HIPCK(hipStreamEndCapture(c->stream, &c->graph));
-    HIPCK(hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0));
+// Captures `body` on the context's stream into *exec (once per hierarchy).
+template <class F> static void capture(bsam_ctx *c, hipGraph_t *g, hipGraphExec_t *exec, F body) {
+  if (*exec) return;
+  HIPCK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  try {
+    body();
+  } catch (...) {
+    hipGraph_t tmp = nullptr;
+    (void)hipStreamEndCapture(c->stream, &tmp);
+    if (tmp) (void)hipGraphDestroy(tmp);
+    throw;
   }
+  HIPCK(hipStreamEndCapture(c->stream, g));
+  HIPCK(hipGraphInstantiate(exec, *g, nullptr, nullptr, 0));
+}
+
+// Runs `body` (one iteration ending in queue_error) until the residual test
+// passes or maxvcycles iterations are done; returns the iteration count.
+template <class F>
+static int iterate(bsam_ctx *c, hipGraph_t *g, hipGraphExec_t *exec, F body, int *done_out) {
+  bool graph = c->use_graph && c->prof_mask == 0;
+  if (graph) capture(c, g, exec, body);
   int it = 0, done = 0;
   for (it = 1; it <= c->cfg.maxvcycles; it++) {
-    if (graph) HIPCK(hipGraphLaunch(c->graph_exec, c->stream));
+    if (graph) HIPCK(hipGraphLaunch(*exec, c->stream));
     else body();
     HIPCK(hipMemcpyAsync(c->h_state, c->d_state, sizeof(int) * 4, hipMemcpyDeviceToHost,
                          c->stream));
@@ -492,13 +506,88 @@ static void multigrid_iterations(bsam_ctx *c, int *ncycles, double *hist, int ma
       break;
     }
   }
-  int n = done ? it : c->cfg.maxvcycles;
+  *done_out = done;
+  return done ? it : c->cfg.maxvcycles;
+}
+
+static void report(bsam_ctx *c, int n, int done, int *ncycles, double *hist, int maxhist,
+                   int *converged) {
   if (ncycles) *ncycles = n;
   if (converged) *converged = done;
   if (hist && maxhist > 0) {
     int m = std::min(n, maxhist);
     HIPCK(hipMemcpy(hist, c->d_hist, sizeof(double) * m, hipMemcpyDeviceToHost));
   }
+}
+
+// MultigridIterations (afasroutines.f90:38-106) in FP64.
+static void multigrid_iterations(bsam_ctx *c, int *ncycles, double *hist, int maxhist,
+                                 int *converged) {
+  op_fill_down(c, BSAM_FIELD_Q);
+  op_source(c, 1);
+  op_fill_down(c, BSAM_FIELD_F);
+  reset_state(c);
+  // one iteration of the vcycleloop (afasroutines.f90:69-104): the V-cycle,
+  // FillDown(q), source restore, FillDown(f) and the residual norm
+  int done = 0;
+  int n = iterate(c, &c->graph, &c->graph_exec, [&]() {
+    op_vcycle(c, c->finest);
+    op_fill_down(c, BSAM_FIELD_Q);
+    op_source(c, 0);
+    op_fill_down(c, BSAM_FIELD_F);
+    queue_error(c, c->finest, c->cfg.qerrortol);
+  }, &done);
+  report(c, n, done, ncycles, hist, maxhist, converged);
+}
+
+// Mixed precision: FP64 defect correction around FP32 FAS V-cycles. Each
+// step computes the FP64 residual r = f - A(q) on every level, solves
+// A(qb + e) - A(qb) = r for the correction e with inner FP32 V-cycles (the
+// same level schedule and ghost programs as the FP64 solve), adds e to q in
+// FP64 and restores FP64 consistency (FillDown, ghost fill) before the FP64
+// residual test. The test is relative: RMS(f - A(q)) < qerrortol * RMS(f)
+// on the finest level; hist holds RMS(f - A(q)) / RMS(f).
+static void mixed_iterations(bsam_ctx *c, int *ncycles, double *hist, int maxhist,
+                             int *converged) {
+  op_fill_down(c, BSAM_FIELD_Q);
+  op_source(c, 1);
+  op_fill_down(c, BSAM_FIELD_F);
+  LevelData &LF = c->L(c->finest);
+  int nt = (int)LF.t_int.size();
+  double count = (double)(c->cfg.nrvars * LF.cells);
+  reset_state(c);
+  HIPCK(launch_resnorm(c->nd(), c->dev(), LF.d_int, nt, c->d_partial, 1, c->stream));
+  HIPCK(launch_finalize(c->d_partial, nt, count, -1.0, c->d_hist, c->d_state, c->stream));
+  HIPCK(hipMemcpyAsync(&c->fnorm, c->d_hist, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCK(hipStreamSynchronize(c->stream));
+  double fn = c->fnorm > 0.0 ? c->fnorm : 1.0;
+  reset_state(c);
+  int done = 0;
+  int n = iterate(c, &c->graph_mixed, &c->graph_mixed_exec, [&]() {
+    for (int l = c->cfg.minlevel; l <= c->finest; l++) {
+      LevelData &L = c->L(l);
+      HIPCK(launch_defect_setup(c->nd(), c->dev(), L.d_ext, (int)L.t_ext.size(), c->stream));
+    }
+    for (int k = 0; k < c->inner_cycles; k++) op_vcycle(c, c->finest, PREC_DEFECT_FP32);
+    for (int l = c->cfg.minlevel; l <= c->finest; l++) {
+      LevelData &L = c->L(l);
+      HIPCK(launch_apply_correction(c->nd(), c->dev(), L.d_int, (int)L.t_int.size(),
+                                    c->stream));
+    }
+    op_fill_down(c, BSAM_FIELD_Q);
+    for (int l = 0; l <= c->finest; l++) op_set_ghost(c, l, 0);
+    queue_error(c, c->finest, c->cfg.qerrortol * fn);
+  }, &done);
+  report(c, n, done, ncycles, hist, maxhist, converged);
+  if (hist)
+    for (int k = 0; k < std::min(n, maxhist); k++) hist[k] /= fn;
+}
+
+static void solve(bsam_ctx *c, int *ncycles, double *hist, int maxhist, int *converged) {
+  if (c->cfg.precision == BSAM_PREC_MIXED)
+    mixed_iterations(c, ncycles, hist, maxhist, converged);
+  else
+    multigrid_iterations(c, ncycles, hist, maxhist, converged);
 }
 
 // ---- C ABI ---------------------------------------------------------------------------
@@ -747,7 +836,7 @@ int bsam_multigrid_iterations(bsam_ctx *c, int *ncycles, double *hist, int maxhi
                               int *converged) {
   API_BEGIN(c)
   c->need_device();
-  multigrid_iterations(c, ncycles, hist, maxhist, converged);
+  solve(c, ncycles, hist, maxhist, converged);
   API_END(c)
 }
 int bsam_time_step(bsam_ctx *c, int *ncycles, double *hist, int maxhist, int *converged) {
@@ -756,7 +845,7 @@ int bsam_time_step(bsam_ctx *c, int *ncycles, double *hist, int maxhist, int *co
   for (int level = 0; level <= c->finest; level++) op_set_ghost(c, level, 0);
   op_fill_down(c, BSAM_FIELD_Q);
   HIPCK(launch_copy(c->qold, c->q, c->nq, c->stream));
-  multigrid_iterations(c, ncycles, hist, maxhist, converged);
+  solve(c, ncycles, hist, maxhist, converged);
   API_END(c)
 }
 int bsam_sync(bsam_ctx *c) {
@@ -775,12 +864,12 @@ int bsam_tune(bsam_ctx *c, const char *key, double value) {
     int t = (int)value;
     if (t < 64 || t > 1024 || t % 64) throw Error("tail_threads must be 64..1024, multiple of 64");
     c->tail_threads = t;
-    if (c->graph_exec) {
-      (void)hipGraphExecDestroy(c->graph_exec);
-      (void)hipGraphDestroy(c->graph);
-      c->graph_exec = nullptr;
-      c->graph = nullptr;
-    }
+    c->drop_graphs();
+  } else if (k == "inner_cycles") {
+    int n = (int)value;
+    if (n < 1 || n > 8) throw Error("inner_cycles must be 1..8");
+    c->inner_cycles = n;
+    c->drop_graphs();
   } else if (k == "graph") {
     c->use_graph = value != 0.0;
   } else {

@@ -1,13 +1,18 @@
-// kernels.hip -- gfx950 kernels of the FAS multigrid engine (FP64 path).
+// kernels.hip -- gfx950 kernels of the FAS multigrid engine.
 //
-// Every kernel works on one level of the hierarchy at a time: a launch covers
-// all patches of the level through a tile list (one 256-thread workgroup per
-// tile, tiles of uneven patches mixed in one launch). Field storage is the
-// reference layout, so a wavefront reads whole rows of q, f and qc.
+// Every level kernel covers all patches of one level through a tile list (one
+// 256-thread workgroup per tile; tiles of uneven patches mixed in one launch).
+// Field storage keeps the reference layout, so a wavefront reads whole rows.
 //
-// Floating-point expressions keep the reference's evaluation order and the
-// library is built without multiply-add contraction, so field results equal
-// the FP64 CPU restatement bit for bit.
+// Each operation is instantiated twice:
+//  - FP64 on the solution (T = double, DEF = false): floating-point
+//    expressions keep the reference's evaluation order and the library is
+//    built without multiply-add contraction, so fields equal the FP64 CPU
+//    restatement bit for bit;
+//  - FP32 on the correction of a defect-correction step (T = float,
+//    DEF = true): the operator becomes A(qb + e) - A(qb), whose cubic term is
+//    evaluated as alpha*e*(3a^2 + 3ae + e^2) (a = qb - shift) so that FP32
+//    keeps its relative accuracy in e however small e gets.
 #include "kernels.hpp"
 
 namespace bsam {
@@ -17,7 +22,7 @@ template <int ND> struct TileShape;
 template <> struct TileShape<2> { static constexpr int X = 32, Y = 8, Z = 1; };
 template <> struct TileShape<3> { static constexpr int X = 8, Y = 8, Z = 4; };
 
-// ---- CH elemental terms (problem.f90:760-813) ----------------------------
+// ---- CH elemental terms (problem.f90:760-813; Mob = 1) -------------------
 __device__ __forceinline__ double dfc(const Problem &p, double c) {
   double cmh = c - p.fc_shift;
   return cmh * cmh * cmh * p.fc_alpha;
@@ -29,23 +34,45 @@ __device__ __forceinline__ double d2fc(const Problem &p, double c) {
 __device__ __forceinline__ double dfe(const Problem &p, double c) {
   return (c - p.fc_shift) * p.fe_num / p.fe_den;
 }
+// Dfc(b + e) - Dfc(b) and D2fc(b + e) in the precision of e.
+template <class T>
+__device__ __forceinline__ T dfc_delta(const Problem &p, T b, T e) {
+  T a = b - (T)p.fc_shift;
+  return (T)p.fc_alpha * e * ((T)3 * a * a + (T)3 * a * e + e * e);
+}
+template <class T>
+__device__ __forceinline__ T d2fc_at(const Problem &p, T b, T e) {
+  T x = b - (T)p.fc_shift + e;
+  return (T)3 * x * x * (T)p.fc_alpha;
+}
 
-// ---- array views -----------------------------------------------------------
+// ---- field sets and array views ------------------------------------------------
+template <class T> struct Fld {
+  T *q, *qc, *f;     // solution-layout, qc-layout and f-layout pools
+  const T *qb;       // FP32 base point (defect form), q layout
+};
+template <class T> __device__ __forceinline__ Fld<T> fields(const Dev &d);
+template <> __device__ __forceinline__ Fld<double> fields<double>(const Dev &d) {
+  return Fld<double>{d.q, d.qc, d.f, nullptr};
+}
+template <> __device__ __forceinline__ Fld<float> fields<float>(const Dev &d) {
+  return Fld<float>{d.e, d.qce, d.fe, d.qb};
+}
+
 // A ghost-extended array with lower bound 0: element (i,j,k,v) at
 // base + i + px*(j + py*k) + v*vs (k = 0 in 2D).
-struct View {
-  double *p;
+template <class T> struct View {
+  T *p;
   long long px, pl, vs;
   __device__ __forceinline__ long long at(int i, int j, int k) const {
     return i + px * j + pl * k;
   }
 };
-
-__device__ __forceinline__ View qview(const Dev &d, const DPatch &P) {
-  return View{d.q + P.q, P.px, (long long)P.px * P.py, P.qvs};
+template <class T> __device__ __forceinline__ View<T> qview(T *base, const DPatch &P) {
+  return View<T>{base + P.q, P.px, (long long)P.px * P.py, P.qvs};
 }
-__device__ __forceinline__ View qcview(const Dev &d, const DPatch &P) {
-  return View{d.qc + P.qc, P.cmx + 2, (long long)(P.cmx + 2) * (P.cmy + 2), P.cvs};
+template <class T> __device__ __forceinline__ View<T> qcview(T *base, const DPatch &P) {
+  return View<T>{base + P.qc, P.cmx + 2, (long long)(P.cmx + 2) * (P.cmy + 2), P.cvs};
 }
 // Interior f-type arrays (lower bound 1).
 __device__ __forceinline__ long long fat(const DPatch &P, int i, int j, int k) {
@@ -55,93 +82,103 @@ __device__ __forceinline__ long long fat(const DPatch &P, int i, int j, int k) {
 // Operator2D/3D (problem.f90:231-257, :534-561) at cell c of view a with
 // spacing hh: r1 = phi - dt/h^2 ULapMob(phi,mu), r2 = mu - Dfc(phi) +
 // eps^2/h^2 ULap(phi), with the undivided stencils of gridutilities.f90.
-template <int ND>
-__device__ __forceinline__ void op_at(const Problem &prm, const View &a, double hh,
-                                      long long c, double &r1, double &r2) {
-  double h2 = hh * hh, tmp1 = prm.dt / h2, tmp2 = prm.eps2 / h2;
-  const double *q1 = a.p, *q2 = a.p + a.vs;
-  double phi = q1[c], mu = q2[c];
+// Defect form: Dfc(phi) becomes Dfc(base + phi) - Dfc(base).
+template <int ND, class T, bool DEF>
+__device__ __forceinline__ void op_at(const Problem &prm, const View<T> &a, T base, double hh,
+                                      long long c, T &r1, T &r2) {
+  double h2 = hh * hh;
+  T tmp1 = (T)(prm.dt / h2), tmp2 = (T)(prm.eps2 / h2);
+  const T *q1 = a.p, *q2 = a.p + a.vs;
+  T phi = q1[c], mu = q2[c];
   long long sx = 1, sy = a.px, sz = a.pl;
-  double lapmob, lap;
+  T lapmob, lap;
   if (ND == 2) {
     lapmob = (((q2[c + sx] - mu) - (mu - q2[c - sx])) + (q2[c + sy] - mu)) -
              (mu - q2[c - sy]);
-    lap = (((q1[c + sx] + q1[c - sx]) + q1[c + sy]) + q1[c - sy]) - 4.0 * phi;
+    lap = (((q1[c + sx] + q1[c - sx]) + q1[c + sy]) + q1[c - sy]) - (T)4 * phi;
   } else {
     lapmob = (((((q2[c + sx] - mu) - (mu - q2[c - sx])) + (q2[c + sy] - mu)) -
                (mu - q2[c - sy])) + (q2[c + sz] - mu)) - (mu - q2[c - sz]);
     lap = (((((q1[c + sx] + q1[c - sx]) + q1[c + sy]) + q1[c - sy]) + q1[c + sz]) +
-           q1[c - sz]) - 6.0 * phi;
+           q1[c - sz]) - (T)6 * phi;
   }
   r1 = phi - tmp1 * lapmob;
-  r2 = (mu - dfc(prm, phi)) + tmp2 * lap;
+  if (DEF) r2 = (mu - dfc_delta<T>(prm, base, phi)) + tmp2 * lap;
+  else r2 = (mu - (T)dfc(prm, (double)phi)) + tmp2 * lap;
 }
 
 // Pointwise nonlinear Gauss-Seidel update of one cell: one Newton step on the
 // 2x2 (phi, mu) system, Cramer's rule (RelaxGrid2D/3D, problem.f90:164-215,
-// :453-518; Mob = 1).
-template <int ND>
-__device__ __forceinline__ void relax_cell(const Dev &d, const DPatch &P, int i, int j,
-                                           int k) {
+// :453-518; Mob = 1). Defect form: the Newton step linearises
+// Dfc(base + e) about the current e.
+template <int ND, class T, bool DEF>
+__device__ __forceinline__ void relax_cell(const Dev &d, const Fld<T> &F, const DPatch &P,
+                                           int i, int j, int k) {
   const Problem &prm = d.prm;
-  View a = qview(d, P);
+  View<T> a = qview(F.q, P);
   long long c = a.at(i, j, k);
-  double *q1 = a.p, *q2 = a.p + a.vs;
+  T *q1 = a.p, *q2 = a.p + a.vs;
   long long fo = P.f + fat(P, i, j, ND == 3 ? k : 1);
   double h2 = P.dx * P.dx;
-  double tmp1 = prm.dt / h2, tmp2 = prm.eps2 / h2;
-  double phi = q1[c];
-  double a12, a21, b1, b2;
+  T tmp1 = (T)(prm.dt / h2), tmp2 = (T)(prm.eps2 / h2);
+  T tmp3 = (T)((ND == 2 ? 4.0 : 6.0) * prm.eps2 / h2);
+  T phi = q1[c];
+  T dc, d2;
+  if (DEF) {
+    T base = F.qb[P.q + c];
+    dc = dfc_delta<T>(prm, base, phi);
+    d2 = d2fc_at<T>(prm, base, phi);
+  } else {
+    dc = (T)dfc(prm, (double)phi);
+    d2 = (T)d2fc(prm, (double)phi);
+  }
+  T a12, a21, b1, b2;
   long long sx = 1, sy = a.px, sz = a.pl;
   if (ND == 2) {
-    double tmp3 = 4.0 * prm.eps2 / h2;
-    a12 = tmp1 * 4.0;
-    a21 = -d2fc(prm, phi) - tmp3;
-    b1 = d.f[fo] + tmp1 * (((q2[c - sx] + q2[c + sx]) + q2[c - sy]) + q2[c + sy]);
-    b2 = ((d.f[fo + P.fvs] + dfc(prm, phi)) - d2fc(prm, phi) * phi) -
+    a12 = tmp1 * (T)4;
+    a21 = -d2 - tmp3;
+    b1 = F.f[fo] + tmp1 * (((q2[c - sx] + q2[c + sx]) + q2[c - sy]) + q2[c + sy]);
+    b2 = ((F.f[fo + P.fvs] + dc) - d2 * phi) -
          tmp2 * (((q1[c + sx] + q1[c - sx]) + q1[c + sy]) + q1[c - sy]);
   } else {
-    double tmp3 = 6.0 * prm.eps2 / h2;
-    a12 = tmp1 * 6.0;
-    a21 = -d2fc(prm, phi) - tmp3;
-    b1 = d.f[fo] + tmp1 * (((((q2[c - sx] + q2[c + sx]) + q2[c - sy]) + q2[c + sy]) +
+    a12 = tmp1 * (T)6;
+    a21 = -d2 - tmp3;
+    b1 = F.f[fo] + tmp1 * (((((q2[c - sx] + q2[c + sx]) + q2[c - sy]) + q2[c + sy]) +
                             q2[c - sz]) + q2[c + sz]);
-    b2 = ((d.f[fo + P.fvs] + dfc(prm, phi)) - d2fc(prm, phi) * phi) -
+    b2 = ((F.f[fo + P.fvs] + dc) - d2 * phi) -
          tmp2 * (((((q1[c + sx] + q1[c - sx]) + q1[c + sy]) + q1[c - sy]) + q1[c + sz]) +
                  q1[c - sz]);
   }
-  double det = 1.0 - a12 * a21;
+  T det = (T)1 - a12 * a21;
   q1[c] = (b1 - a12 * b2) / det;
   q2[c] = (b2 - a21 * b1) / det;
 }
 
 // Restriction2D/3D (gridutilities.f90:89-110, :406-433) of the child cells
 // under coarse cell (ci,cj,ck) of a ghost-extended (lower bound 0) view.
-template <int ND>
-__device__ __forceinline__ double restrict_view(const View &a, int v, int ci, int cj,
-                                                int ck) {
-  const double *x = a.p + v * a.vs;
+template <int ND, class T>
+__device__ __forceinline__ T restrict_view(const View<T> &a, int v, int ci, int cj, int ck) {
+  const T *x = a.p + v * a.vs;
   int i2 = 2 * ci, j2 = 2 * cj;
   if (ND == 2)
-    return 0.25 * (((x[a.at(i2, j2, 0)] + x[a.at(i2 - 1, j2, 0)]) +
-                    x[a.at(i2, j2 - 1, 0)]) + x[a.at(i2 - 1, j2 - 1, 0)]);
+    return (T)0.25 * (((x[a.at(i2, j2, 0)] + x[a.at(i2 - 1, j2, 0)]) +
+                       x[a.at(i2, j2 - 1, 0)]) + x[a.at(i2 - 1, j2 - 1, 0)]);
   int k2 = 2 * ck;
-  return 0.125 *
+  return (T)0.125 *
          (((((((x[a.at(i2, j2, k2)] + x[a.at(i2 - 1, j2, k2)]) + x[a.at(i2, j2 - 1, k2)]) +
               x[a.at(i2 - 1, j2 - 1, k2)]) + x[a.at(i2, j2, k2 - 1)]) +
             x[a.at(i2 - 1, j2, k2 - 1)]) + x[a.at(i2, j2 - 1, k2 - 1)]) +
           x[a.at(i2 - 1, j2 - 1, k2 - 1)]);
 }
 // Same for an interior (lower bound 1) f-type array.
-template <int ND>
-__device__ __forceinline__ double restrict_f(const double *x, const DPatch &P, int ci,
-                                             int cj, int ck) {
+template <int ND, class T>
+__device__ __forceinline__ T restrict_f(const T *x, const DPatch &P, int ci, int cj, int ck) {
   int i2 = 2 * ci, j2 = 2 * cj;
   if (ND == 2)
-    return 0.25 * (((x[fat(P, i2, j2, 1)] + x[fat(P, i2 - 1, j2, 1)]) +
-                    x[fat(P, i2, j2 - 1, 1)]) + x[fat(P, i2 - 1, j2 - 1, 1)]);
+    return (T)0.25 * (((x[fat(P, i2, j2, 1)] + x[fat(P, i2 - 1, j2, 1)]) +
+                       x[fat(P, i2, j2 - 1, 1)]) + x[fat(P, i2 - 1, j2 - 1, 1)]);
   int k2 = 2 * ck;
-  return 0.125 *
+  return (T)0.125 *
          (((((((x[fat(P, i2, j2, k2)] + x[fat(P, i2 - 1, j2, k2)]) +
                x[fat(P, i2, j2 - 1, k2)]) + x[fat(P, i2 - 1, j2 - 1, k2)]) +
              x[fat(P, i2, j2, k2 - 1)]) + x[fat(P, i2 - 1, j2, k2 - 1)]) +
@@ -153,47 +190,49 @@ __device__ __forceinline__ double restrict_f(const double *x, const DPatch &P, i
 // kernels below map one tile of cells to a workgroup, and the tail kernels
 // loop the same functions over whole single-patch levels in one workgroup.
 
-// One ghost-cell update of a SetGhost program.
-__device__ __forceinline__ void ghost_op(const Dev &d, const GhostOp &g) {
-  double *q = d.q;
+// One ghost-cell update of a SetGhost program. A boundary "2 - x" map is
+// affine; in the defect form only its linear part, -x, applies.
+template <class T, bool DEF>
+__device__ __forceinline__ void ghost_op(const Dev &d, T *q, const GhostOp &g) {
   for (int v = 0; v < d.nrvars; v++) {
-    const double *f = q + (long long)v * g.fvs;
-    const double *c = q + (long long)v * g.cvs;
-    double val;
+    const T *f = q + (long long)v * g.fvs;
+    const T *c = q + (long long)v * g.cvs;
+    T val;
     switch (g.kind) {
       case 0:
         val = f[g.f0];
         break;
       case 1:
-        val = BSAM_A1 * f[g.f0] + BSAM_A2 * f[g.f1] + BSAM_A3 * c[g.c[0]];
+        val = (T)BSAM_A1 * f[g.f0] + (T)BSAM_A2 * f[g.f1] + (T)BSAM_A3 * c[g.c[0]];
         break;
       case 2:
-        val = BSAM_A1 * f[g.f0] + BSAM_A2 * f[g.f1] +
-              BSAM_A3 * (BSAM_B1 * c[g.c[0]] + BSAM_B2 * c[g.c[1]] + BSAM_B3 * c[g.c[2]]);
+        val = (T)BSAM_A1 * f[g.f0] + (T)BSAM_A2 * f[g.f1] +
+              (T)BSAM_A3 * ((T)BSAM_B1 * c[g.c[0]] + (T)BSAM_B2 * c[g.c[1]] +
+                            (T)BSAM_B3 * c[g.c[2]]);
         break;
       default:
-        val = BSAM_A1 * f[g.f0] + BSAM_A2 * f[g.f1] +
-              BSAM_A3 * (BSAM_C1 * c[g.c[0]] + BSAM_C2 * c[g.c[1]] + BSAM_C3 * c[g.c[2]] +
-                         BSAM_C4 * c[g.c[3]]);
+        val = (T)BSAM_A1 * f[g.f0] + (T)BSAM_A2 * f[g.f1] +
+              (T)BSAM_A3 * ((T)BSAM_C1 * c[g.c[0]] + (T)BSAM_C2 * c[g.c[1]] +
+                            (T)BSAM_C3 * c[g.c[2]] + (T)BSAM_C4 * c[g.c[3]]);
         break;
     }
-    for (int r = 0; r < g.nflip; r++) val = 2.0 - val;
+    for (int r = 0; r < g.nflip; r++) val = DEF ? -val : (T)2 - val;
     q[g.dst + (long long)v * g.dvs] = val;
   }
 }
 
 // RestrictSolution (afasroutines.f90:600-644) for coarse cell (i,j,k): qc =
 // R(q) and the parent's q under the child takes the same value.
-template <int ND>
-__device__ __forceinline__ void restrict_cell(const Dev &d, const DPatch &P, int i, int j,
-                                              int k) {
+template <int ND, class T>
+__device__ __forceinline__ void restrict_cell(const Dev &d, const Fld<T> &F, const DPatch &P,
+                                              int i, int j, int k) {
   const DPatch &PA = d.P[P.parent];
-  View a = qview(d, P), cq = qcview(d, P), pq = qview(d, PA);
+  View<T> a = qview(F.q, P), cq = qcview(F.qc, P), pq = qview(F.q, PA);
   long long pc = pq.at(P.mb[0][0] + i - 1, P.mb[1][0] + j - 1,
                        ND == 3 ? P.mb[2][0] + k - 1 : 0);
   long long cc = cq.at(i, j, k);
   for (int v = 0; v < d.nrvars; v++) {
-    double r = restrict_view<ND>(a, v, i, j, k);
+    T r = restrict_view<ND, T>(a, v, i, j, k);
     cq.p[cc + v * cq.vs] = r;
     pq.p[pc + v * pq.vs] = r;
   }
@@ -201,33 +240,34 @@ __device__ __forceinline__ void restrict_cell(const Dev &d, const DPatch &P, int
 
 // FillDownLevel (afasroutines.f90:241-310): parent X(mbounds) = R(child X)
 // for X = f (field 1) or q (field 2).
-template <int ND>
-__device__ __forceinline__ void filldown_cell(const Dev &d, const DPatch &P, int i, int j,
-                                              int k, int field) {
+template <int ND, class T>
+__device__ __forceinline__ void filldown_cell(const Dev &d, const Fld<T> &F, const DPatch &P,
+                                              int i, int j, int k, int field) {
   const DPatch &PA = d.P[P.parent];
   int pi = P.mb[0][0] + i - 1, pj = P.mb[1][0] + j - 1,
       pk = ND == 3 ? P.mb[2][0] + k - 1 : 1;
   if (field == 1) {
     for (int v = 0; v < d.nrvars; v++)
-      d.f[PA.f + fat(PA, pi, pj, pk) + v * PA.fvs] =
-          restrict_f<ND>(d.f + P.f + v * P.fvs, P, i, j, k);
+      F.f[PA.f + fat(PA, pi, pj, pk) + v * PA.fvs] =
+          restrict_f<ND, T>(F.f + P.f + v * P.fvs, P, i, j, k);
   } else {
-    View a = qview(d, P), pq = qview(d, PA);
+    View<T> a = qview(F.q, P), pq = qview(F.q, PA);
     long long pc = pq.at(pi, pj, ND == 3 ? pk : 0);
-    for (int v = 0; v < d.nrvars; v++) pq.p[pc + v * pq.vs] = restrict_view<ND>(a, v, i, j, k);
+    for (int v = 0; v < d.nrvars; v++)
+      pq.p[pc + v * pq.vs] = restrict_view<ND, T>(a, v, i, j, k);
   }
 }
 
 // GetCoarseGhostPoints (boundary.f90:1203-1269): ring cell (ci,cj,ck) of qc
 // <- parent (mb1-1+ci, mb2-1+cj, mb3-1+ck); (i,j,k) ranges over 0..cmx+1.
-template <int ND>
-__device__ __forceinline__ void coarse_ghost_cell(const Dev &d, const DPatch &P, int i, int j,
-                                                  int k) {
+template <int ND, class T>
+__device__ __forceinline__ void coarse_ghost_cell(const Dev &d, const Fld<T> &F,
+                                                  const DPatch &P, int i, int j, int k) {
   bool ring = i == 0 || i == P.cmx + 1 || j == 0 || j == P.cmy + 1 ||
               (ND == 3 && (k == 0 || k == P.cmz + 1));
   if (!ring) return;
   const DPatch &PA = d.P[P.parent];
-  View cq = qcview(d, P), pq = qview(d, PA);
+  View<T> cq = qcview(F.qc, P), pq = qview(F.q, PA);
   long long pc = pq.at(P.mb[0][0] - 1 + i, P.mb[1][0] - 1 + j,
                        ND == 3 ? P.mb[2][0] - 1 + k : 0);
   long long cc = cq.at(i, j, k);
@@ -237,76 +277,81 @@ __device__ __forceinline__ void coarse_ghost_cell(const Dev &d, const DPatch &P,
 // CoarseLoadingFunction (afasroutines.f90:646-714): parent f(mbounds) =
 // R(f - A_h q) + A_2h(qc), the FAS right-hand side, with the fine residual
 // recomputed in registers for the 2^d children of coarse cell (i,j,k).
-template <int ND>
-__device__ __forceinline__ void coarse_load_cell(const Dev &d, const DPatch &P, int i, int j,
-                                                 int k) {
+template <int ND, class T, bool DEF>
+__device__ __forceinline__ void coarse_load_cell(const Dev &d, const Fld<T> &F,
+                                                 const DPatch &P, int i, int j, int k) {
   const DPatch &PA = d.P[P.parent];
-  View a = qview(d, P), cq = qcview(d, P);
-  const double *f1 = d.f + P.f, *f2 = d.f + P.f + P.fvs;
+  View<T> a = qview(F.q, P), cq = qcview(F.qc, P);
+  const T *f1 = F.f + P.f, *f2 = F.f + P.f + P.fvs;
   constexpr int NC = ND == 2 ? 4 : 8;
-  double r1[NC], r2[NC];
+  T r1[NC], r2[NC];
 #pragma unroll
   for (int s = 0; s < NC; s++) {
     int fi = 2 * i - (s & 1), fj = 2 * j - ((s >> 1) & 1);
     int fk = ND == 3 ? 2 * k - ((s >> 2) & 1) : 0;
-    double o1, o2;
-    op_at<ND>(d.prm, a, P.dx, a.at(fi, fj, fk), o1, o2);
+    long long cidx = a.at(fi, fj, fk);
+    T base = DEF ? F.qb[P.q + cidx] : (T)0;
+    T o1, o2;
+    op_at<ND, T, DEF>(d.prm, a, base, P.dx, cidx, o1, o2);
     long long fo = fat(P, fi, fj, ND == 3 ? fk : 1);
     r1[s] = f1[fo] - o1;
     r2[s] = f2[fo] - o2;
   }
-  double R1, R2;
+  T R1, R2;
   if (ND == 2) {
-    R1 = 0.25 * (((r1[0] + r1[1]) + r1[2]) + r1[3]);
-    R2 = 0.25 * (((r2[0] + r2[1]) + r2[2]) + r2[3]);
+    R1 = (T)0.25 * (((r1[0] + r1[1]) + r1[2]) + r1[3]);
+    R2 = (T)0.25 * (((r2[0] + r2[1]) + r2[2]) + r2[3]);
   } else {
-    R1 = 0.125 * (((((((r1[0] + r1[1]) + r1[2]) + r1[3]) + r1[4]) + r1[5]) + r1[6]) + r1[7]);
-    R2 = 0.125 * (((((((r2[0] + r2[1]) + r2[2]) + r2[3]) + r2[4]) + r2[5]) + r2[6]) + r2[7]);
+    R1 = (T)0.125 * (((((((r1[0] + r1[1]) + r1[2]) + r1[3]) + r1[4]) + r1[5]) + r1[6]) + r1[7]);
+    R2 = (T)0.125 * (((((((r2[0] + r2[1]) + r2[2]) + r2[3]) + r2[4]) + r2[5]) + r2[6]) + r2[7]);
   }
-  double c1, c2;
-  op_at<ND>(d.prm, cq, PA.dx, cq.at(i, j, k), c1, c2);
   int pi = P.mb[0][0] + i - 1, pj = P.mb[1][0] + j - 1,
       pk = ND == 3 ? P.mb[2][0] + k - 1 : 1;
+  T base = (T)0;
+  if (DEF)
+    base = F.qb[PA.q + pi + (long long)PA.px * (pj + (long long)PA.py * (ND == 3 ? pk : 0))];
+  T c1, c2;
+  op_at<ND, T, DEF>(d.prm, cq, base, PA.dx, cq.at(i, j, k), c1, c2);
   long long po = PA.f + fat(PA, pi, pj, pk);
-  d.f[po] = R1 + c1;
-  d.f[po + PA.fvs] = R2 + c2;
+  F.f[po] = R1 + c1;
+  F.f[po + PA.fvs] = R2 + c2;
 }
 
 // CorrectFine (afasroutines.f90:842-963, mbc = 1): q += P1(parent q - qc) at
 // fine cell (i,j,k) of 0..mx+1, the separable (3,1)/4 prolongation of
 // BiLinProlongationP1 / TriLinProlongationP1 (gridutilities.f90:137-175,
 // :474-529) evaluated in the reference's x, y(, z) pass order.
-template <int ND>
-__device__ __forceinline__ void correct_cell(const Dev &d, const DPatch &P, int i, int j,
-                                             int k) {
+template <int ND, class T>
+__device__ __forceinline__ void correct_cell(const Dev &d, const Fld<T> &F, const DPatch &P,
+                                             int i, int j, int k) {
   const DPatch &PA = d.P[P.parent];
-  View a = qview(d, P), cq = qcview(d, P), pq = qview(d, PA);
+  View<T> a = qview(F.q, P), cq = qcview(F.qc, P), pq = qview(F.q, PA);
   int kx = i >> 1, ky = j >> 1, kz = ND == 3 ? k >> 1 : 0;
   bool ex = (i & 1) == 0, ey = (j & 1) == 0, ez = (k & 1) == 0;
   int o0 = P.mb[0][0] - 1, o1 = P.mb[1][0] - 1, o2 = ND == 3 ? P.mb[2][0] - 1 : 0;
   long long qc0 = a.at(i, j, k);
   for (int v = 0; v < d.nrvars; v++) {
-    const double *cp = cq.p + v * cq.vs, *pp = pq.p + v * pq.vs;
+    const T *cp = cq.p + v * cq.vs, *pp = pq.p + v * pq.vs;
     auto corr = [&](int ci, int cj, int ck) {
       return pp[pq.at(o0 + ci, o1 + cj, ND == 3 ? o2 + ck : 0)] - cp[cq.at(ci, cj, ck)];
     };
     auto xpass = [&](int cj, int ck) {
-      double a0 = corr(kx, cj, ck), a1 = corr(kx + 1, cj, ck);
-      return ex ? 3.0 * a0 + a1 : a0 + 3.0 * a1;
+      T a0 = corr(kx, cj, ck), a1 = corr(kx + 1, cj, ck);
+      return ex ? (T)3 * a0 + a1 : a0 + (T)3 * a1;
     };
-    double p;
+    T p;
     if (ND == 2) {
-      double b0 = xpass(ky, 0), b1 = xpass(ky + 1, 0);
-      p = ey ? (3.0 * b0 + b1) / 16.0 : (b0 + 3.0 * b1) / 16.0;
+      T b0 = xpass(ky, 0), b1 = xpass(ky + 1, 0);
+      p = ey ? ((T)3 * b0 + b1) / (T)16 : (b0 + (T)3 * b1) / (T)16;
     } else {
       auto ypass = [&](int ck) {
-        double b0 = xpass(ky, ck), b1 = xpass(ky + 1, ck);
-        return ey ? 3.0 * b0 + b1 : b0 + 3.0 * b1;
+        T b0 = xpass(ky, ck), b1 = xpass(ky + 1, ck);
+        return ey ? (T)3 * b0 + b1 : b0 + (T)3 * b1;
       };
-      double c0 = ypass(kz), c1 = ypass(kz + 1);
-      p = ez ? (3.0 * c0 + c1) / 64.0 : (c0 + 3.0 * c1) / 64.0;
+      T c0 = ypass(kz), c1 = ypass(kz + 1);
+      p = ez ? ((T)3 * c0 + c1) / (T)64 : (c0 + (T)3 * c1) / (T)64;
     }
-    double *qv = a.p + v * a.vs;
+    T *qv = a.p + v * a.vs;
     qv[qc0] = qv[qc0] + p;
   }
 }
@@ -326,6 +371,36 @@ __device__ __forceinline__ void source_cell(const Dev &d, const DPatch &P, int i
   for (int v = 0; v < d.nrvars; v++) d.f[fo + v * P.fvs] = d.ftmp[fo + v * P.fvs] + 0.0;
 }
 
+// Start of a defect-correction step on cell (i,j,k) of 0..mx+1: the FP32
+// right-hand side fe = f - A(q) (FP64 residual, interior cells), the base
+// point qb = q_1 and a zero correction e.
+template <int ND>
+__device__ __forceinline__ void defect_setup_cell(const Dev &d, const DPatch &P, int i, int j,
+                                                  int k) {
+  View<double> a = qview(d.q, P);
+  long long c = a.at(i, j, k);
+  d.qb[P.q + c] = (float)d.q[P.q + c];
+  d.e[P.q + c] = 0.f;
+  d.e[P.q + c + P.qvs] = 0.f;
+  bool inner = i >= 1 && i <= P.mx && j >= 1 && j <= P.my &&
+               (ND == 2 || (k >= 1 && k <= P.mz));
+  if (!inner) return;
+  double o1, o2;
+  op_at<ND, double, false>(d.prm, a, 0.0, P.dx, c, o1, o2);
+  long long fo = P.f + fat(P, i, j, ND == 3 ? k : 1);
+  d.fe[fo] = (float)(d.f[fo] - o1);
+  d.fe[fo + P.fvs] = (float)(d.f[fo + P.fvs] - o2);
+}
+
+// End of a defect-correction step: q += e on interior cell (i,j,k).
+template <int ND>
+__device__ __forceinline__ void apply_correction_cell(const Dev &d, const DPatch &P, int i,
+                                                      int j, int k) {
+  long long c = P.q + i + (long long)P.px * (j + (long long)P.py * k);
+  d.q[c] = d.q[c] + (double)d.e[c];
+  d.q[c + P.qvs] = d.q[c + P.qvs] + (double)d.e[c + P.qvs];
+}
+
 // ---- tile decode -----------------------------------------------------------
 template <int ND>
 __device__ __forceinline__ void tile_cell(const Tile &t, int &i, int &j, int &k) {
@@ -335,78 +410,83 @@ __device__ __forceinline__ void tile_cell(const Tile &t, int &i, int &j, int &k)
   j = t.j0 + (tid / S::X) % S::Y;
   k = ND == 3 ? t.k0 + tid / (S::X * S::Y) : 0;
 }
+template <int ND>
+__device__ __forceinline__ bool in_box(int i, int j, int k, int ex, int ey, int ez) {
+  return i <= ex && j <= ey && (ND == 2 || k <= ez);
+}
 
 // ---- level kernels ---------------------------------------------------------
-template <int ND>
+template <int ND, class T, bool DEF>
 __global__ __launch_bounds__(256) void k_relax(Dev d, const Tile *tiles, int ipass) {
   Tile t = tiles[blockIdx.x];
   const DPatch &P = d.P[t.patch];
   int i, j, k;
   tile_cell<ND>(t, i, j, k);
-  if (i > P.mx || j > P.my || (ND == 3 && k > P.mz)) return;
+  if (!in_box<ND>(i, j, k, P.mx, P.my, P.mz)) return;
   // RelaxGrid's loop start 1+mod(j+ipass,2) (and +k in 3D) selects the
   // cells with i+j(+k)+ipass odd.
   if (((i + j + (ND == 3 ? k : 0) + ipass) & 1) == 0) return;
-  relax_cell<ND>(d, P, i, j, k);
+  relax_cell<ND, T, DEF>(d, fields<T>(d), P, i, j, k);
 }
 
 // One SetGhost(level, ipass) program: every ghost-cell update is independent.
+template <class T, bool DEF>
 __global__ __launch_bounds__(256) void k_ghost(Dev d, const GhostOp *ops, int n) {
   int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n) return;
-  ghost_op(d, ops[e]);
+  ghost_op<T, DEF>(d, fields<T>(d).q, ops[e]);
 }
 
-template <int ND>
+template <int ND, class T>
 __global__ __launch_bounds__(256) void k_restrict(Dev d, const Tile *tiles) {
   Tile t = tiles[blockIdx.x];
   const DPatch &P = d.P[t.patch];
   int i, j, k;
   tile_cell<ND>(t, i, j, k);
-  if (i > P.cmx || j > P.cmy || (ND == 3 && k > P.cmz)) return;
-  restrict_cell<ND>(d, P, i, j, k);
+  if (!in_box<ND>(i, j, k, P.cmx, P.cmy, P.cmz)) return;
+  restrict_cell<ND, T>(d, fields<T>(d), P, i, j, k);
 }
 
-template <int ND>
+template <int ND, class T>
 __global__ __launch_bounds__(256) void k_filldown(Dev d, const Tile *tiles, int field) {
   Tile t = tiles[blockIdx.x];
   const DPatch &P = d.P[t.patch];
   int i, j, k;
   tile_cell<ND>(t, i, j, k);
-  if (i > P.cmx || j > P.cmy || (ND == 3 && k > P.cmz)) return;
-  filldown_cell<ND>(d, P, i, j, k, field);
+  if (!in_box<ND>(i, j, k, P.cmx, P.cmy, P.cmz)) return;
+  filldown_cell<ND, T>(d, fields<T>(d), P, i, j, k, field);
 }
 
 // Tiles cover the ghost-extended coarse box 0..cmx+1.
-template <int ND>
+template <int ND, class T>
 __global__ __launch_bounds__(256) void k_coarse_ghost(Dev d, const Tile *tiles) {
   Tile t = tiles[blockIdx.x];
   const DPatch &P = d.P[t.patch];
   int i, j, k;
   tile_cell<ND>(t, i, j, k);
-  if (i > P.cmx + 1 || j > P.cmy + 1 || (ND == 3 && k > P.cmz + 1)) return;
-  coarse_ghost_cell<ND>(d, P, i, j, k);
+  if (!in_box<ND>(i, j, k, P.cmx + 1, P.cmy + 1, P.cmz + 1)) return;
+  coarse_ghost_cell<ND, T>(d, fields<T>(d), P, i, j, k);
 }
 
-template <int ND>
+template <int ND, class T, bool DEF>
 __global__ __launch_bounds__(256) void k_coarse_load(Dev d, const Tile *tiles) {
   Tile t = tiles[blockIdx.x];
   const DPatch &P = d.P[t.patch];
   int i, j, k;
   tile_cell<ND>(t, i, j, k);
-  if (i > P.cmx || j > P.cmy || (ND == 3 && k > P.cmz)) return;
-  coarse_load_cell<ND>(d, P, i, j, k);
+  if (!in_box<ND>(i, j, k, P.cmx, P.cmy, P.cmz)) return;
+  coarse_load_cell<ND, T, DEF>(d, fields<T>(d), P, i, j, k);
 }
 
 // Tiles cover the ghost-extended fine box 0..mx+1.
-template <int ND>
+template <int ND, class T>
 __global__ __launch_bounds__(256) void k_correct(Dev d, const Tile *tiles) {
   Tile t = tiles[blockIdx.x];
   const DPatch &P = d.P[t.patch];
   int i, j, k;
   tile_cell<ND>(t, i, j, k);
-  if (i > P.mx + 1 || j > P.my + 1 || (ND == 3 && k > P.mz + 1)) return;
-  correct_cell<ND>(d, P, i, j, k);
+  if (!in_box<ND>(i, j, k, P.mx + 1, P.my + 1, P.mz + 1)) return;
+  correct_cell<ND, T>(d, fields<T>(d), P, i, j, k);
 }
 
 template <int ND>
@@ -415,8 +495,29 @@ __global__ __launch_bounds__(256) void k_source(Dev d, const Tile *tiles, int fu
   const DPatch &P = d.P[t.patch];
   int i, j, k;
   tile_cell<ND>(t, i, j, k);
-  if (i > P.mx || j > P.my || (ND == 3 && k > P.mz)) return;
+  if (!in_box<ND>(i, j, k, P.mx, P.my, P.mz)) return;
   source_cell<ND>(d, P, i, j, k, full);
+}
+
+// Tiles cover the ghost-extended fine box 0..mx+1.
+template <int ND>
+__global__ __launch_bounds__(256) void k_defect_setup(Dev d, const Tile *tiles) {
+  Tile t = tiles[blockIdx.x];
+  const DPatch &P = d.P[t.patch];
+  int i, j, k;
+  tile_cell<ND>(t, i, j, k);
+  if (!in_box<ND>(i, j, k, P.mx + 1, P.my + 1, P.mz + 1)) return;
+  defect_setup_cell<ND>(d, P, i, j, k);
+}
+
+template <int ND>
+__global__ __launch_bounds__(256) void k_apply_correction(Dev d, const Tile *tiles) {
+  Tile t = tiles[blockIdx.x];
+  const DPatch &P = d.P[t.patch];
+  int i, j, k;
+  tile_cell<ND>(t, i, j, k);
+  if (!in_box<ND>(i, j, k, P.mx, P.my, P.mz)) return;
+  apply_correction_cell<ND>(d, P, i, j, k);
 }
 
 // ---- single-workgroup tail: the single-patch coarse levels ------------------
@@ -433,54 +534,59 @@ __device__ __forceinline__ void for_cells(int ex, int ey, int ez, int base, F f)
   }
 }
 
-template <int ND>
+template <class T, bool DEF>
 __device__ void tail_ghost(const Dev &d, const TailLevel &L, int ipass) {
   const GhostOp *ops = L.gp[ipass];
-  for (int e = threadIdx.x; e < L.ngp[ipass]; e += blockDim.x) ghost_op(d, ops[e]);
+  T *q = fields<T>(d).q;
+  for (int e = threadIdx.x; e < L.ngp[ipass]; e += blockDim.x) ghost_op<T, DEF>(d, q, ops[e]);
   __syncthreads();
 }
 
-template <int ND>
+template <int ND, class T, bool DEF>
 __device__ void tail_relax(const Dev &d, const TailLevel &L, int nsmooth) {
   const DPatch &P = d.P[L.patch];
+  Fld<T> F = fields<T>(d);
   for (int pass = 0; pass < nsmooth; pass++)
     for (int color = 1; color <= 2; color++) {
       for_cells<ND>(P.mx, P.my, ND == 3 ? P.mz : 1, 1, [&](int i, int j, int k) {
-        if (((i + j + (ND == 3 ? k : 0) + color) & 1) == 1) relax_cell<ND>(d, P, i, j, k);
+        if (((i + j + (ND == 3 ? k : 0) + color) & 1) == 1)
+          relax_cell<ND, T, DEF>(d, F, P, i, j, k);
       });
       __syncthreads();
-      tail_ghost<ND>(d, L, color);
+      tail_ghost<T, DEF>(d, L, color);
     }
 }
 
 // AFASVcycle(hi) (afasroutines.f90:488-521) restricted to levels lo..hi of
 // the single-patch chain; levels are indexed from lo.
-template <int ND>
+template <int ND, class T, bool DEF>
 __global__ __launch_bounds__(1024) void k_tail_vcycle(Dev d, const TailLevel *lv, int nlev,
                                                        int nsmooth) {
+  Fld<T> F = fields<T>(d);
   for (int l = nlev - 1; l >= 1; l--) {
     const DPatch &P = d.P[lv[l].patch];
     int cz = ND == 3 ? P.cmz : 1;
-    tail_relax<ND>(d, lv[l], nsmooth);
+    tail_relax<ND, T, DEF>(d, lv[l], nsmooth);
     for_cells<ND>(P.cmx, P.cmy, cz, 1,
-                  [&](int i, int j, int k) { restrict_cell<ND>(d, P, i, j, k); });
+                  [&](int i, int j, int k) { restrict_cell<ND, T>(d, F, P, i, j, k); });
     __syncthreads();
-    tail_ghost<ND>(d, lv[l - 1], 0);
+    tail_ghost<T, DEF>(d, lv[l - 1], 0);
     for_cells<ND>(P.cmx + 2, P.cmy + 2, ND == 3 ? P.cmz + 2 : 1, 0,
-                  [&](int i, int j, int k) { coarse_ghost_cell<ND>(d, P, i, j, k); });
+                  [&](int i, int j, int k) { coarse_ghost_cell<ND, T>(d, F, P, i, j, k); });
     __syncthreads();
-    for_cells<ND>(P.cmx, P.cmy, cz, 1,
-                  [&](int i, int j, int k) { coarse_load_cell<ND>(d, P, i, j, k); });
+    for_cells<ND>(P.cmx, P.cmy, cz, 1, [&](int i, int j, int k) {
+      coarse_load_cell<ND, T, DEF>(d, F, P, i, j, k);
+    });
     __syncthreads();
   }
-  tail_relax<ND>(d, lv[0], nsmooth);
+  tail_relax<ND, T, DEF>(d, lv[0], nsmooth);
   for (int l = 1; l < nlev; l++) {
     const DPatch &P = d.P[lv[l].patch];
     for_cells<ND>(P.mx + 2, P.my + 2, ND == 3 ? P.mz + 2 : 1, 0,
-                  [&](int i, int j, int k) { correct_cell<ND>(d, P, i, j, k); });
+                  [&](int i, int j, int k) { correct_cell<ND, T>(d, F, P, i, j, k); });
     __syncthreads();
-    tail_ghost<ND>(d, lv[l], 0);
-    tail_relax<ND>(d, lv[l], nsmooth);
+    tail_ghost<T, DEF>(d, lv[l], 0);
+    tail_relax<ND, T, DEF>(d, lv[l], nsmooth);
   }
 }
 
@@ -489,17 +595,19 @@ __global__ __launch_bounds__(1024) void k_tail_vcycle(Dev d, const TailLevel *lv
 template <int ND>
 __global__ __launch_bounds__(1024) void k_tail_filldown(Dev d, const TailLevel *lv, int nlev,
                                                          int field) {
+  Fld<double> F = fields<double>(d);
   for (int l = nlev - 1; l >= 1; l--) {
     const DPatch &P = d.P[lv[l].patch];
-    for_cells<ND>(P.cmx, P.cmy, ND == 3 ? P.cmz : 1, 1,
-                  [&](int i, int j, int k) { filldown_cell<ND>(d, P, i, j, k, field); });
+    for_cells<ND>(P.cmx, P.cmy, ND == 3 ? P.cmz : 1, 1, [&](int i, int j, int k) {
+      filldown_cell<ND, double>(d, F, P, i, j, k, field);
+    });
     __syncthreads();
-    if (field == 2) tail_ghost<ND>(d, lv[l - 1], 0);
+    if (field == 2) tail_ghost<double, false>(d, lv[l - 1], 0);
   }
 }
 
-// Residual of the finest level (Residual, afasroutines.f90:799-840) reduced
-// to one sum of squares per tile; deterministic (fixed order) reduction.
+// ---- norms -------------------------------------------------------------------
+// Fixed-order (deterministic) workgroup sum; the total is valid in thread 0.
 __device__ __forceinline__ double block_sum(double s) {
   __shared__ double red[16];
   for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
@@ -514,23 +622,25 @@ __device__ __forceinline__ double block_sum(double s) {
   return tot;
 }
 
+// Per-tile sum of squares of the residual f - A(q) (Residual,
+// afasroutines.f90:799-840; what = 0) or of f itself (what = 1).
 template <int ND>
 __global__ __launch_bounds__(256) void k_resnorm(Dev d, const Tile *tiles, double *partial,
-                                                  int write_rf) {
+                                                  int what) {
   Tile t = tiles[blockIdx.x];
   const DPatch &P = d.P[t.patch];
   int i, j, k;
   tile_cell<ND>(t, i, j, k);
   double s = 0.0;
-  if (i <= P.mx && j <= P.my && (ND == 2 || k <= P.mz)) {
-    View a = qview(d, P);
-    double o1, o2;
-    op_at<ND>(d.prm, a, P.dx, a.at(i, j, k), o1, o2);
+  if (in_box<ND>(i, j, k, P.mx, P.my, P.mz)) {
     long long fo = P.f + fat(P, i, j, ND == 3 ? k : 1);
-    double r1 = d.f[fo] - o1, r2 = d.f[fo + P.fvs] - o2;
-    if (write_rf) {
-      d.rf[fo] = r1;
-      d.rf[fo + P.fvs] = r2;
+    double r1 = d.f[fo], r2 = d.f[fo + P.fvs];
+    if (what == 0) {
+      View<double> a = qview(d.q, P);
+      double o1, o2;
+      op_at<ND, double, false>(d.prm, a, 0.0, P.dx, a.at(i, j, k), o1, o2);
+      r1 = r1 - o1;
+      r2 = r2 - o2;
     }
     s = r1 * r1 + r2 * r2;
   }
@@ -564,71 +674,108 @@ __global__ __launch_bounds__(256) void k_copy(double *dst, const double *src, lo
 // ---- launchers -------------------------------------------------------------
 static inline dim3 grid1(int n) { return dim3((unsigned)n); }
 
-hipError_t launch_relax(int nd, const Dev &d, const Tile *tiles, int nt, int ipass,
+#define BSAM_LAUNCH_ND_PREC(KERNEL, nd, prec, grid, block, ...)                        \
+  do {                                                                                 \
+    if (prec == PREC_SOLUTION_FP64) {                                                  \
+      if (nd == 2) KERNEL<2, double, false><<<grid, block, 0, s>>>(__VA_ARGS__);       \
+      else KERNEL<3, double, false><<<grid, block, 0, s>>>(__VA_ARGS__);               \
+    } else {                                                                           \
+      if (nd == 2) KERNEL<2, float, true><<<grid, block, 0, s>>>(__VA_ARGS__);         \
+      else KERNEL<3, float, true><<<grid, block, 0, s>>>(__VA_ARGS__);                 \
+    }                                                                                  \
+  } while (0)
+#define BSAM_LAUNCH_ND_T(KERNEL, nd, prec, grid, block, ...)                           \
+  do {                                                                                 \
+    if (prec == PREC_SOLUTION_FP64) {                                                  \
+      if (nd == 2) KERNEL<2, double><<<grid, block, 0, s>>>(__VA_ARGS__);              \
+      else KERNEL<3, double><<<grid, block, 0, s>>>(__VA_ARGS__);                      \
+    } else {                                                                           \
+      if (nd == 2) KERNEL<2, float><<<grid, block, 0, s>>>(__VA_ARGS__);               \
+      else KERNEL<3, float><<<grid, block, 0, s>>>(__VA_ARGS__);                       \
+    }                                                                                  \
+  } while (0)
+#define BSAM_LAUNCH_ND(KERNEL, nd, grid, block, ...)                                   \
+  do {                                                                                 \
+    if (nd == 2) KERNEL<2><<<grid, block, 0, s>>>(__VA_ARGS__);                        \
+    else KERNEL<3><<<grid, block, 0, s>>>(__VA_ARGS__);                                \
+  } while (0)
+
+hipError_t launch_relax(int nd, int prec, const Dev &d, const Tile *tiles, int nt, int ipass,
                         hipStream_t s) {
   if (nt <= 0) return hipSuccess;
-  if (nd == 2) k_relax<2><<<grid1(nt), 256, 0, s>>>(d, tiles, ipass);
-  else k_relax<3><<<grid1(nt), 256, 0, s>>>(d, tiles, ipass);
+  BSAM_LAUNCH_ND_PREC(k_relax, nd, prec, grid1(nt), 256, d, tiles, ipass);
   return hipGetLastError();
 }
 
-hipError_t launch_ghost(const Dev &d, const GhostOp *ops, int n, hipStream_t s) {
+hipError_t launch_ghost(int prec, const Dev &d, const GhostOp *ops, int n, hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  k_ghost<<<dim3((n + 255) / 256), 256, 0, s>>>(d, ops, n);
+  dim3 g((n + 255) / 256);
+  if (prec == PREC_SOLUTION_FP64) k_ghost<double, false><<<g, 256, 0, s>>>(d, ops, n);
+  else k_ghost<float, true><<<g, 256, 0, s>>>(d, ops, n);
   return hipGetLastError();
 }
 
-hipError_t launch_restrict(int nd, const Dev &d, const Tile *tiles, int nt, hipStream_t s) {
+hipError_t launch_restrict(int nd, int prec, const Dev &d, const Tile *tiles, int nt,
+                           hipStream_t s) {
   if (nt <= 0) return hipSuccess;
-  if (nd == 2) k_restrict<2><<<grid1(nt), 256, 0, s>>>(d, tiles);
-  else k_restrict<3><<<grid1(nt), 256, 0, s>>>(d, tiles);
+  BSAM_LAUNCH_ND_T(k_restrict, nd, prec, grid1(nt), 256, d, tiles);
   return hipGetLastError();
 }
 
 hipError_t launch_filldown(int nd, const Dev &d, const Tile *tiles, int nt, int field,
                            hipStream_t s) {
   if (nt <= 0) return hipSuccess;
-  if (nd == 2) k_filldown<2><<<grid1(nt), 256, 0, s>>>(d, tiles, field);
-  else k_filldown<3><<<grid1(nt), 256, 0, s>>>(d, tiles, field);
+  if (nd == 2) k_filldown<2, double><<<grid1(nt), 256, 0, s>>>(d, tiles, field);
+  else k_filldown<3, double><<<grid1(nt), 256, 0, s>>>(d, tiles, field);
   return hipGetLastError();
 }
 
-hipError_t launch_coarse_ghost(int nd, const Dev &d, const Tile *tiles, int nt,
+hipError_t launch_coarse_ghost(int nd, int prec, const Dev &d, const Tile *tiles, int nt,
                                hipStream_t s) {
   if (nt <= 0) return hipSuccess;
-  if (nd == 2) k_coarse_ghost<2><<<grid1(nt), 256, 0, s>>>(d, tiles);
-  else k_coarse_ghost<3><<<grid1(nt), 256, 0, s>>>(d, tiles);
+  BSAM_LAUNCH_ND_T(k_coarse_ghost, nd, prec, grid1(nt), 256, d, tiles);
   return hipGetLastError();
 }
 
-hipError_t launch_coarse_load(int nd, const Dev &d, const Tile *tiles, int nt,
+hipError_t launch_coarse_load(int nd, int prec, const Dev &d, const Tile *tiles, int nt,
                               hipStream_t s) {
   if (nt <= 0) return hipSuccess;
-  if (nd == 2) k_coarse_load<2><<<grid1(nt), 256, 0, s>>>(d, tiles);
-  else k_coarse_load<3><<<grid1(nt), 256, 0, s>>>(d, tiles);
+  BSAM_LAUNCH_ND_PREC(k_coarse_load, nd, prec, grid1(nt), 256, d, tiles);
   return hipGetLastError();
 }
 
-hipError_t launch_correct(int nd, const Dev &d, const Tile *tiles, int nt, hipStream_t s) {
+hipError_t launch_correct(int nd, int prec, const Dev &d, const Tile *tiles, int nt,
+                          hipStream_t s) {
   if (nt <= 0) return hipSuccess;
-  if (nd == 2) k_correct<2><<<grid1(nt), 256, 0, s>>>(d, tiles);
-  else k_correct<3><<<grid1(nt), 256, 0, s>>>(d, tiles);
+  BSAM_LAUNCH_ND_T(k_correct, nd, prec, grid1(nt), 256, d, tiles);
   return hipGetLastError();
 }
 
 hipError_t launch_source(int nd, const Dev &d, const Tile *tiles, int nt, int full,
                          hipStream_t s) {
   if (nt <= 0) return hipSuccess;
-  if (nd == 2) k_source<2><<<grid1(nt), 256, 0, s>>>(d, tiles, full);
-  else k_source<3><<<grid1(nt), 256, 0, s>>>(d, tiles, full);
+  BSAM_LAUNCH_ND(k_source, nd, grid1(nt), 256, d, tiles, full);
+  return hipGetLastError();
+}
+
+hipError_t launch_defect_setup(int nd, const Dev &d, const Tile *tiles, int nt,
+                               hipStream_t s) {
+  if (nt <= 0) return hipSuccess;
+  BSAM_LAUNCH_ND(k_defect_setup, nd, grid1(nt), 256, d, tiles);
+  return hipGetLastError();
+}
+
+hipError_t launch_apply_correction(int nd, const Dev &d, const Tile *tiles, int nt,
+                                   hipStream_t s) {
+  if (nt <= 0) return hipSuccess;
+  BSAM_LAUNCH_ND(k_apply_correction, nd, grid1(nt), 256, d, tiles);
   return hipGetLastError();
 }
 
 hipError_t launch_resnorm(int nd, const Dev &d, const Tile *tiles, int nt, double *partial,
-                          int write_rf, hipStream_t s) {
+                          int what, hipStream_t s) {
   if (nt <= 0) return hipSuccess;
-  if (nd == 2) k_resnorm<2><<<grid1(nt), 256, 0, s>>>(d, tiles, partial, write_rf);
-  else k_resnorm<3><<<grid1(nt), 256, 0, s>>>(d, tiles, partial, write_rf);
+  BSAM_LAUNCH_ND(k_resnorm, nd, grid1(nt), 256, d, tiles, partial, what);
   return hipGetLastError();
 }
 
@@ -646,19 +793,17 @@ hipError_t launch_copy(double *dst, const double *src, long long n, hipStream_t 
   return hipGetLastError();
 }
 
-hipError_t launch_tail_vcycle(int nd, const Dev &d, const TailLevel *lv, int nlev, int nsmooth,
-                              int threads, hipStream_t s) {
+hipError_t launch_tail_vcycle(int nd, int prec, const Dev &d, const TailLevel *lv, int nlev,
+                              int nsmooth, int threads, hipStream_t s) {
   if (nlev <= 0) return hipSuccess;
-  if (nd == 2) k_tail_vcycle<2><<<1, threads, 0, s>>>(d, lv, nlev, nsmooth);
-  else k_tail_vcycle<3><<<1, threads, 0, s>>>(d, lv, nlev, nsmooth);
+  BSAM_LAUNCH_ND_PREC(k_tail_vcycle, nd, prec, 1, threads, d, lv, nlev, nsmooth);
   return hipGetLastError();
 }
 
 hipError_t launch_tail_filldown(int nd, const Dev &d, const TailLevel *lv, int nlev, int field,
                                 int threads, hipStream_t s) {
   if (nlev <= 1) return hipSuccess;
-  if (nd == 2) k_tail_filldown<2><<<1, threads, 0, s>>>(d, lv, nlev, field);
-  else k_tail_filldown<3><<<1, threads, 0, s>>>(d, lv, nlev, field);
+  BSAM_LAUNCH_ND(k_tail_filldown, nd, 1, threads, d, lv, nlev, field);
   return hipGetLastError();
 }
 

@@ -1,30 +1,36 @@
-// kernels.hpp -- launchers of the gfx950 kernels (kernels.hip).
+// kernels.hpp -- launchers of the gfx950 kernels (kernels.hip). `prec` selects
+// the field set: PREC_SOLUTION_FP64 (the solution, reference arithmetic) or
+// PREC_DEFECT_FP32 (the FP32 correction of a defect-correction step).
 #pragma once
 #include "common.hpp"
 
 namespace bsam {
 
-hipError_t launch_relax(int nd, const Dev &d, const Tile *tiles, int nt, int ipass,
+hipError_t launch_relax(int nd, int prec, const Dev &d, const Tile *tiles, int nt, int ipass,
                         hipStream_t s);
-hipError_t launch_ghost(const Dev &d, const GhostOp *ops, int n, hipStream_t s);
-hipError_t launch_restrict(int nd, const Dev &d, const Tile *tiles, int nt, hipStream_t s);
+hipError_t launch_ghost(int prec, const Dev &d, const GhostOp *ops, int n, hipStream_t s);
+hipError_t launch_restrict(int nd, int prec, const Dev &d, const Tile *tiles, int nt,
+                           hipStream_t s);
 hipError_t launch_filldown(int nd, const Dev &d, const Tile *tiles, int nt, int field,
                            hipStream_t s);
-hipError_t launch_coarse_ghost(int nd, const Dev &d, const Tile *tiles, int nt,
+hipError_t launch_coarse_ghost(int nd, int prec, const Dev &d, const Tile *tiles, int nt,
                                hipStream_t s);
-hipError_t launch_coarse_load(int nd, const Dev &d, const Tile *tiles, int nt,
+hipError_t launch_coarse_load(int nd, int prec, const Dev &d, const Tile *tiles, int nt,
                               hipStream_t s);
-hipError_t launch_correct(int nd, const Dev &d, const Tile *tiles, int nt, hipStream_t s);
+hipError_t launch_correct(int nd, int prec, const Dev &d, const Tile *tiles, int nt,
+                          hipStream_t s);
 hipError_t launch_source(int nd, const Dev &d, const Tile *tiles, int nt, int full,
                          hipStream_t s);
+hipError_t launch_defect_setup(int nd, const Dev &d, const Tile *tiles, int nt, hipStream_t s);
+hipError_t launch_apply_correction(int nd, const Dev &d, const Tile *tiles, int nt,
+                                   hipStream_t s);
 hipError_t launch_resnorm(int nd, const Dev &d, const Tile *tiles, int nt, double *partial,
-                          int write_rf, hipStream_t s);
+                          int what, hipStream_t s);
 hipError_t launch_finalize(const double *partial, int n, double count, double tol,
                            double *hist, int *state, hipStream_t s);
 hipError_t launch_copy(double *dst, const double *src, long long n, hipStream_t s);
-
-hipError_t launch_tail_vcycle(int nd, const Dev &d, const TailLevel *lv, int nlev, int nsmooth,
-                              int threads, hipStream_t s);
+hipError_t launch_tail_vcycle(int nd, int prec, const Dev &d, const TailLevel *lv, int nlev,
+                              int nsmooth, int threads, hipStream_t s);
 hipError_t launch_tail_filldown(int nd, const Dev &d, const TailLevel *lv, int nlev, int field,
                                 int threads, hipStream_t s);
 

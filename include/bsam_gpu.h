@@ -121,7 +121,10 @@ int bsam_error_afas(bsam_ctx *ctx, int level, double *residual);
 int bsam_copy_qold(bsam_ctx *ctx);
 /* MultigridIterations, lib/afasroutines.f90:38-106. Writes the number of
  * V-cycles run, up to maxhist per-cycle residuals, and whether the last
- * residual fell below qerrortol (the reference sets lconverged, :98-101). */
+ * residual fell below qerrortol (the reference sets lconverged, :98-101).
+ * With BSAM_PREC_MIXED the iterations are FP64 defect-correction steps around
+ * FP32 V-cycles, the test is relative (RMS(f - A q) < qerrortol * RMS(f) on
+ * the finest level) and hist holds the relative residuals. */
 int bsam_multigrid_iterations(bsam_ctx *ctx, int *ncycles, double *hist,
                               int maxhist, int *converged);
 /* One time step on a fixed hierarchy: SetGhost on levels 0..finest, FillDown
@@ -138,7 +141,9 @@ int bsam_sync(bsam_ctx *ctx);
  *                  0 disables). Set before bsam_hierarchy_begin.
  *   "tail_threads" workgroup size of that kernel (64..1024, default 1024).
  *   "graph"        1 (default): replay each multigrid iteration as a
- *                  captured hipGraph; 0: launch kernels one by one. */
+ *                  captured hipGraph; 0: launch kernels one by one.
+ *   "inner_cycles" FP32 V-cycles per defect-correction step (mixed
+ *                  precision; 1..8, default 1). */
 int bsam_tune(bsam_ctx *ctx, const char *key, double value);
 
 /* Kernel timing. Between bsam_profile_begin and bsam_profile_end, every

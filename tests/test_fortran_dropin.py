@@ -1,0 +1,75 @@
+"""The reference Fortran host code drives the engine through its Fortran
+binding (bsam_amd/fortran/bsam_gpu.f90, afas_gpu.f90).
+
+tests/fortran/dropin_check is linked from the reference's own modules
+(oracle/_ref, compiled from the reference sources), the binding and
+libbsamgpu.so. On a CPU it checks the engine's ghost-fill programs applied to
+the reference's own arrays against the reference SetGhost (host-only context);
+on a GPU it checks GpuMultigridIterations against the reference
+MultigridIterations time step by time step. Both must agree bit for bit.
+Skipped where the reference build (oracle/_ref) is absent.
+"""
+
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import refrun
+from golden_cases import LAYOUTS
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "tests", "fortran", "_build", "dropin_check")
+REF_OBJ = os.path.join(ROOT, "oracle", "_ref", "treeops.o")
+
+
+def ensure_built():
+    if not os.path.exists(REF_OBJ):
+        pytest.skip("reference objects (oracle/_ref) not built")
+    if not os.path.exists(BIN):
+        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "tests", "fortran")], check=True)
+
+
+def write_case(tmp_path, lname, seed, dt=1e-3, alpha=1.0, eps=0.03, maxv=40, tol=1e-8):
+    lay = LAYOUTS[lname]()
+    spec = os.path.join(tmp_path, "spec.txt")
+    inp = os.path.join(tmp_path, "in.bin")
+    refrun.write_spec(spec, lay, dt, alpha, eps, 2, maxv, tol, [])
+    rng = np.random.default_rng(seed)
+    with open(inp, "wb") as fh:
+        for p in lay.ordered():
+            q = np.zeros(lay.field_shape(p, "q"), order="F")
+            q[..., 0] = 0.5 + rng.uniform(-0.05, 0.05, q.shape[:-1])
+            for name, arr in (("q", q), ("f", np.zeros(lay.field_shape(p, "f"))),
+                              ("qc", np.zeros(lay.field_shape(p, "qc")))):
+                fh.write(np.asarray(arr, dtype=np.float64).tobytes(order="F"))
+    return spec, inp
+
+
+def run(mode, spec, inp, *extra):
+    out = subprocess.run([BIN, mode, spec, inp, *extra], capture_output=True, text=True,
+                         timeout=300)
+    assert out.returncode == 0, out.stdout + out.stderr
+    diff = [l for l in out.stdout.splitlines() if l.startswith("max_abs_diff")]
+    assert diff, out.stdout
+    return float(diff[-1].split()[1]), out.stdout
+
+
+@pytest.mark.parametrize("lname", ["amr2d", "amr2d_userbc", "rect2d", "amr3d", "amr3d_userbc"])
+def test_ghost_programs_on_reference_arrays(tmp_path, lname):
+    ensure_built()
+    spec, inp = write_case(str(tmp_path), lname, seed=1)
+    diff, _ = run("host", spec, inp)
+    assert diff == 0.0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("lname", ["amr2d", "rect2d", "amr3d"])
+def test_gpu_multigrid_iterations_replace_reference(tmp_path, lname):
+    if not os.path.exists(BIN):
+        pytest.skip("dropin_check not built (needs the reference build)")
+    spec, inp = write_case(str(tmp_path), lname, seed=2)
+    diff, out = run("gpu", spec, inp, "2")
+    assert "converged 1" in out, out
+    assert diff == 0.0, out
