@@ -248,8 +248,7 @@ static void commit(bsam_ctx *c) {
       if (h.mx[d] < 1) throw Error("mx must be positive");
     for (int fc = 0; fc < 2 * nd; fc++) {
       int t = h.mthbc[fc];
-      if (t == 2) throw Error("periodic boundaries (mthbc = 2) are not supported");
-      if (!(t == 1 || t == 999 || (t >= 10 && t <= 99)))
+      if (!(t == 1 || t == 2 || t == 999 || (t >= 10 && t <= 99)))
         throw Error("unsupported boundary code " + std::to_string(t));
     }
   }
@@ -295,12 +294,20 @@ static void commit(bsam_ctx *c) {
     c->nf += D.fvs * nv;
   }
   if (c->nq >= (1LL << 31)) throw Error("q pool exceeds 2^31 elements");
+  Periodic per;
+  if (c->finest >= 0) {
+    const HPatch &root = c->hp[c->lev[0 - minl].patches[0]];
+    per = periodic_setup(root, nd);
+    for (int d = 0; d < nd; d++)
+      if ((root.mthbc[2 * d] == 2) != (root.mthbc[2 * d + 1] == 2))
+        throw Error("periodic boundaries must pair opposite faces");
+  }
   for (int l = minl; l <= c->finest; l++) {
     LevelData &L = c->lev[l - minl];
     build_tiles(c, L, l);
     for (int ip = 0; ip < 3; ip++) {
       std::string msg;
-      if (build_ghost_program(c->hp, L.patches, nd, l, ip, L.gp[ip], msg))
+      if (build_ghost_program(c->hp, L.patches, nd, l, ip, per, L.gp[ip], msg))
         throw Error("ghost program level " + std::to_string(l) + ": " + msg);
     }
   }
@@ -705,6 +712,15 @@ int bsam_hierarchy_commit(bsam_ctx *c) {
 }
 
 int bsam_num_patches(const bsam_ctx *c) { return c ? (int)c->hp.size() : -1; }
+
+// Device bytes of a committed hierarchy's field pools (q, qold, qc, f, rf,
+// ftmp, plus the FP32 pools in mixed precision), excluding small tables.
+long long bsam_footprint_bytes(const bsam_ctx *c) {
+  if (!c || !c->committed) return -1;
+  long long b = 8 * (2 * c->nq + c->nc + 3 * c->nf);
+  if (c->cfg.precision == BSAM_PREC_MIXED) b += 4 * (2 * c->nq + c->nc + c->nf);
+  return b;
+}
 
 static void field_ptr(bsam_ctx *c, int patch, int field, double **base, long long *off,
                       long long *n) {

@@ -476,6 +476,85 @@ struct Builder {
         }
   }
 
+  // Resolves the current expression of a cell read by a later phase: an
+  // interior cell or an untouched ghost is a plain copy of its old value.
+  Expr source(PatchMap &pm, const HPatch &h, int i, int j, int k) {
+    if (!interior(h, i, j, k)) {
+      auto it = pm.m.find(key(h, i, j, k));
+      if (it != pm.m.end()) return it->second;
+    }
+    Expr e;
+    e.kind = 0;
+    e.f0 = Cell{pm.p, i, j, k};
+    return e;
+  }
+
+  // PeriodicBC (boundary.f90:417-486, qpo = 0): a patch periodic on both
+  // sides of a dimension wraps itself, dimensions in order, over the full
+  // ghost-extended tangential range.
+  void periodic_self(PatchMap &pm) {
+    const HPatch &h = hp[pm.p];
+    for (int d = 0; d < nd; d++) {
+      if (!(h.mthbc[2 * d] == 2 && h.mthbc[2 * d + 1] == 2)) continue;
+      int ta = d == 0 ? 1 : 0, tb = d == 2 ? 1 : 2;
+      int ahi = ta < nd ? h.mx[ta] + 1 : 0, bhi = tb < nd ? h.mx[tb] + 1 : 0;
+      std::vector<std::pair<long long, Expr>> upd;
+      for (int b = 0; b <= bhi; b++)
+        for (int a = 0; a <= ahi; a++) {
+          int hi[3], shi[3], lo[3], slo[3];
+          hi[d] = h.mx[d] + 1; shi[d] = 1;
+          lo[d] = 0; slo[d] = h.mx[d];
+          hi[ta] = shi[ta] = lo[ta] = slo[ta] = a;
+          hi[tb] = shi[tb] = lo[tb] = slo[tb] = b;
+          upd.push_back({key(h, hi[0], hi[1], hi[2]), source(pm, h, shi[0], shi[1], shi[2])});
+          upd.push_back({key(h, lo[0], lo[1], lo[2]), source(pm, h, slo[0], slo[1], slo[2])});
+        }
+      for (auto &u : upd) pm.m[u.first] = u.second;
+    }
+  }
+
+  // PeriodicGhostOverlap (boundary.f90:510-596) for the ordered pair
+  // (g1, g2): every periodic image of g2 exchanges its ghost-extended overlap
+  // with g1 both ways; all sources are interior cells.
+  void periodic_pair(PatchMap &m1, PatchMap &m2, int level, const Periodic &per) {
+    const HPatch &g1 = hp[m1.p], &g2 = hp[m2.p];
+    static const int tab[13][3] = {{1, 0, 0},  {0, 1, 0},  {0, 0, 1},  {1, 1, 0},  {-1, 1, 0},
+                                   {0, 1, 1},  {0, 1, -1}, {1, 0, 1},  {-1, 0, 1}, {1, 1, 1},
+                                   {1, 1, -1}, {-1, 1, 1}, {1, -1, 1}};
+    for (int o = 0; o < per.noff; o++) {
+      int off[3];
+      for (int d = 0; d < 3; d++) off[d] = tab[per.idx[o] - 1][d] * (per.rootmx[d] << level);
+      int a1[3], a2[3], b1[3], b2[3], lo1[3], hi1[3], lo2[3], hi2[3];
+      bool empty = false;
+      for (int n = 0; n < 3; n++) {
+        a1[n] = n < nd ? g1.mg[n][0] : 1; a2[n] = n < nd ? g1.mg[n][1] : 1;
+        b1[n] = n < nd ? g2.mg[n][0] + off[n] : 1; b2[n] = n < nd ? g2.mg[n][1] + off[n] : 1;
+        int o0 = std::max(a1[n] - 1, b1[n] - 1), o1 = std::min(a2[n] + 1, b2[n] + 1);
+        if (n < nd && o1 - o0 < 0) empty = true;
+        lo1[n] = std::max(o0, b1[n]); hi1[n] = std::min(o1, b2[n]);
+        lo2[n] = std::max(o0, a1[n]); hi2[n] = std::min(o1, a2[n]);
+      }
+      if (empty) continue;
+      auto put_copy = [&](PatchMap &dm, const HPatch &dh, int di, int dj, int dk, int sp,
+                          int si, int sj, int sk) {
+        Expr e;
+        e.kind = 0;
+        e.f0 = Cell{sp, si, sj, sk};
+        dm.m[key(dh, di, dj, dk)] = e;
+      };
+      for (int k = lo1[2]; k <= hi1[2]; k++)
+        for (int j = lo1[1]; j <= hi1[1]; j++)
+          for (int i = lo1[0]; i <= hi1[0]; i++)
+            put_copy(m1, g1, i - a1[0] + 1, j - a1[1] + 1, nd == 3 ? k - a1[2] + 1 : 0, m2.p,
+                     i - b1[0] + 1, j - b1[1] + 1, nd == 3 ? k - b1[2] + 1 : 0);
+      for (int k = lo2[2]; k <= hi2[2]; k++)
+        for (int j = lo2[1]; j <= hi2[1]; j++)
+          for (int i = lo2[0]; i <= hi2[0]; i++)
+            put_copy(m2, g2, i - b1[0] + 1, j - b1[1] + 1, nd == 3 ? k - b1[2] + 1 : 0, m1.p,
+                     i - a1[0] + 1, j - a1[1] + 1, nd == 3 ? k - a1[2] + 1 : 0);
+    }
+  }
+
   // SetBC2D/SetBC3D (boundary.f90:643-891) with the CH module's UserBC2D/3D
   // (problem.f90:646-756): faces in order over the full tangential range.
   void physical_bc(PatchMap &pm) {
@@ -529,9 +608,26 @@ long long qoff(const std::vector<HPatch> &hp, const Cell &c) {
 
 }  // namespace
 
+Periodic periodic_setup(const HPatch &root, int ndims) {
+  Periodic per;
+  bool px = root.mthbc[0] == 2, py = root.mthbc[2] == 2, pz = ndims == 3 && root.mthbc[4] == 2;
+  std::vector<int> list;
+  if (px && py && pz) list = {1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13};
+  else if (px && py) list = {1, 2, 4, 5};
+  else if (px && pz) list = {1, 3, 8, 9};
+  else if (px) list = {1};
+  else if (py && pz) list = {2, 3, 6, 7};
+  else if (py) list = {2};
+  else if (pz) list = {3};
+  per.noff = (int)list.size();
+  for (int k = 0; k < per.noff; k++) per.idx[k] = list[k];
+  for (int d = 0; d < 3; d++) per.rootmx[d] = root.mx[d];
+  return per;
+}
+
 int build_ghost_program(const std::vector<HPatch> &hp, const std::vector<int> &patches,
-                        int ndims, int level, int ipass, std::vector<GhostOp> &out,
-                        std::string &msg) {
+                        int ndims, int level, int ipass, const Periodic &per,
+                        std::vector<GhostOp> &out, std::string &msg) {
   out.clear();
   Builder B(hp, ndims, msg);
   std::vector<PatchMap> maps(patches.size());
@@ -548,6 +644,15 @@ int build_ghost_program(const std::vector<HPatch> &hp, const std::vector<int> &p
         B.overlap_into(maps[a], maps[b].p);
         B.overlap_into(maps[b], maps[a].p);
       }
+  }
+  if (per.noff > 0) {
+    for (auto &pm : maps) B.periodic_self(pm);
+    if (level > 0)
+      for (size_t a = 0; a < maps.size(); a++)
+        for (size_t b = a + 1; b < maps.size(); b++) {
+          B.periodic_pair(maps[a], maps[b], level, per);
+          B.periodic_pair(maps[b], maps[a], level, per);
+        }
   }
   for (auto &pm : maps) B.physical_bc(pm);
   for (auto &pm : maps) {

@@ -20,11 +20,21 @@ struct HPatch {
   long long qoff = 0, qvs = 0, px = 0, py = 0;
 };
 
+// Periodic set-up of the hierarchy (PeriodicSetup, boundary.f90:893-963):
+// the offset vectors in use (GetPeriodicOffsets numbering 1..13) and the root
+// extents that scale them (mxmax = root mx * 2^level).
+struct Periodic {
+  int noff = 0;
+  int idx[13] = {0};
+  int rootmx[3] = {1, 1, 1};
+};
+Periodic periodic_setup(const HPatch &root, int ndims);
+
 // Builds the ghost-fill program of SetGhost(level, ipass) (boundary.f90:58-89)
 // for the patches of one level. Returns 0 on success, else fills msg.
 int build_ghost_program(const std::vector<HPatch> &hp, const std::vector<int> &patches,
-                        int ndims, int level, int ipass, std::vector<GhostOp> &out,
-                        std::string &msg);
+                        int ndims, int level, int ipass, const Periodic &per,
+                        std::vector<GhostOp> &out, std::string &msg);
 
 // Patch and (i,j,k) (lower bound 0; k = 0 in 2D) of a q-pool offset.
 void locate_q_cell(const std::vector<HPatch> &hp, long long off, int &p, int &i, int &j,

@@ -68,6 +68,8 @@ def lib():
                                      C.c_double, dp]
         L.bsam_hierarchy_commit.argtypes = [vp]
         L.bsam_num_patches.argtypes = [vp]
+        L.bsam_footprint_bytes.restype = C.c_longlong
+        L.bsam_footprint_bytes.argtypes = [vp]
         L.bsam_field_len.restype = C.c_long
         L.bsam_field_len.argtypes = [vp, C.c_int, C.c_int]
         L.bsam_upload.argtypes = [vp, C.c_int, C.c_int, dp, C.c_long]
@@ -150,6 +152,9 @@ class Engine:
         if rc != 0:
             raise BsamError(lib().bsam_last_error(self.ctx).decode())
         return rc
+
+    def footprint_bytes(self) -> int:
+        return int(lib().bsam_footprint_bytes(self.ctx))
 
     # -- fields -----------------------------------------------------------
     def upload(self, n: int, name: str, arr: np.ndarray):

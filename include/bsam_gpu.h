@@ -86,6 +86,9 @@ int bsam_add_patch(bsam_ctx *ctx, int id, int level, int parent, const int *mx,
                    double dx, const double *xlower);
 int bsam_hierarchy_commit(bsam_ctx *ctx);
 int bsam_num_patches(const bsam_ctx *ctx);
+/* Device bytes held by the committed hierarchy's field pools (the quantity a
+ * server admits requests by; no counterpart in the reference). */
+long long bsam_footprint_bytes(const bsam_ctx *ctx);
 
 /* Host <-> device field transfer for one patch, n doubles in the reference
  * layout. Synchronous. */

@@ -148,6 +148,29 @@ int orc_commit(orc_hier *h) {
                                    h->p[p->parent].level != p->level - 1))
       return fail(h, "patch %d: bad parent", n);
   }
+  /* PeriodicSetup (lib/boundary.f90:893-963): offset vectors by the root's
+   * periodic faces; mxmax(level) = root mx * 2^level (bsamroutines.f90:373-377) */
+  h->periodic = 0;
+  h->noff = 0;
+  for (int n = 0; n < h->npatches; n++) {
+    orc_patch *r = &h->p[n];
+    if (r->level != 0) continue;
+    int px = r->mthbc[0] == 2, py = r->mthbc[2] == 2, pz = h->ndims == 3 && r->mthbc[4] == 2;
+    static const int t13[13] = {1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13};
+    const int *list = NULL;
+    static const int l124[4] = {1, 2, 4, 5}, l1[1] = {1}, l1389[4] = {1, 3, 8, 9},
+                     l2367[4] = {2, 3, 6, 7}, l2[1] = {2}, l3[1] = {3};
+    if (px && py && pz) { list = t13; h->noff = 13; }
+    else if (px && py) { list = l124; h->noff = 4; }
+    else if (px && pz) { list = l1389; h->noff = 4; }
+    else if (px) { list = l1; h->noff = 1; }
+    else if (py && pz) { list = l2367; h->noff = 4; }
+    else if (py) { list = l2; h->noff = 1; }
+    else if (pz) { list = l3; h->noff = 1; }
+    for (int k = 0; k < h->noff; k++) h->offidx[k] = list[k];
+    h->periodic = h->noff > 0;
+    for (int d = 0; d < 3; d++) h->rootmx[d] = r->mx[d];
+  }
   return 0;
 }
 
@@ -682,10 +705,82 @@ static int set_bc_patch(orc_hier *h, orc_patch *p) {
   return 0;
 }
 
+/* PeriodicBC, lib/boundary.f90:417-486 (quasi-periodic offset qpo = 0): a
+ * patch periodic on both sides of a dimension wraps its own interior into its
+ * ghosts, dimensions in order, over the full ghost-extended range. */
+static void periodic_bc(orc_hier *h, orc_patch *p) {
+  int nd = h->ndims;
+  for (int d = 0; d < nd; d++) {
+    if (!(p->mthbc[2 * d] == 2 && p->mthbc[2 * d + 1] == 2)) continue;
+    int ta = d == 0 ? 1 : 0, tb = d == 2 ? 1 : 2;
+    int alo = p->ql[ta], ahi = ta < nd ? p->mx[ta] + h->mbc : 1;
+    int blo = p->ql[tb], bhi = tb < nd ? p->mx[tb] + h->mbc : 1;
+    for (int v = 1; v <= h->nrvars; v++)
+      for (int ibc = 1; ibc <= h->mbc; ibc++)
+        for (int b = blo; b <= bhi; b++)
+          for (int a = alo; a <= ahi; a++) {
+            int hi[3], lo[3], shi[3], slo[3];
+            hi[d] = p->mx[d] + ibc; shi[d] = ibc;
+            lo[d] = 1 - ibc; slo[d] = p->mx[d] - ibc + 1;
+            hi[ta] = lo[ta] = shi[ta] = slo[ta] = a;
+            hi[tb] = lo[tb] = shi[tb] = slo[tb] = b;
+            Q(p, hi[0], hi[1], hi[2], v) = Q(p, shi[0], shi[1], shi[2], v);
+            Q(p, lo[0], lo[1], lo[2], v) = Q(p, slo[0], slo[1], slo[2], v);
+          }
+  }
+}
+
+/* GetPeriodicOffsets, lib/boundary.f90:965-1044 (qoffset = 0 when qpo = 0). */
+static void periodic_offset(const orc_hier *h, int level, int idx, int off[3]) {
+  int W = h->rootmx[0] << level, H = h->rootmx[1] << level, D = h->rootmx[2] << level;
+  static const int tab[13][3] = {{1, 0, 0},  {0, 1, 0},  {0, 0, 1},  {1, 1, 0},  {-1, 1, 0},
+                                 {0, 1, 1},  {0, 1, -1}, {1, 0, 1},  {-1, 0, 1}, {1, 1, 1},
+                                 {1, 1, -1}, {-1, 1, 1}, {1, -1, 1}};
+  off[0] = tab[idx - 1][0] * W;
+  off[1] = tab[idx - 1][1] * H;
+  off[2] = tab[idx - 1][2] * D;
+}
+
+/* PeriodicGhostOverlap, lib/boundary.f90:510-596: for every periodic image
+ * of g2 (g2 shifted by an offset), the ghost-extended overlap with g1 is
+ * filled both ways as in GhostOverlap. */
+static void periodic_overlap(orc_hier *h, int level, orc_patch *g1, orc_patch *g2) {
+  int nd = h->ndims, mbc = h->mbc;
+  for (int o = 0; o < h->noff; o++) {
+    int off[3];
+    periodic_offset(h, level, h->offidx[o], off);
+    int a1[3], a2[3], b1[3], b2[3], ov[3][2], o1[3][2], o2[3][2];
+    int empty = 0;
+    for (int n = 0; n < 3; n++) {
+      a1[n] = n < nd ? g1->mg[n][0] : 1; a2[n] = n < nd ? g1->mg[n][1] : 1;
+      b1[n] = n < nd ? g2->mg[n][0] + off[n] : 1; b2[n] = n < nd ? g2->mg[n][1] + off[n] : 1;
+      ov[n][0] = (a1[n] - mbc > b1[n] - mbc) ? a1[n] - mbc : b1[n] - mbc;
+      ov[n][1] = (a2[n] + mbc < b2[n] + mbc) ? a2[n] + mbc : b2[n] + mbc;
+      if (n < nd && ov[n][1] - ov[n][0] < 0) empty = 1;
+      o1[n][0] = ov[n][0] > b1[n] ? ov[n][0] : b1[n];
+      o1[n][1] = ov[n][1] < b2[n] ? ov[n][1] : b2[n];
+      o2[n][0] = ov[n][0] > a1[n] ? ov[n][0] : a1[n];
+      o2[n][1] = ov[n][1] < a2[n] ? ov[n][1] : a2[n];
+    }
+    if (empty) continue;
+    for (int v = 1; v <= h->nrvars; v++)
+      for (int k = o1[2][0]; k <= o1[2][1]; k++)
+        for (int j = o1[1][0]; j <= o1[1][1]; j++)
+          for (int i = o1[0][0]; i <= o1[0][1]; i++)
+            Q(g1, i - a1[0] + 1, j - a1[1] + 1, k - a1[2] + 1, v) =
+                Q(g2, i - b1[0] + 1, j - b1[1] + 1, k - b1[2] + 1, v);
+    for (int v = 1; v <= h->nrvars; v++)
+      for (int k = o2[2][0]; k <= o2[2][1]; k++)
+        for (int j = o2[1][0]; j <= o2[1][1]; j++)
+          for (int i = o2[0][0]; i <= o2[0][1]; i++)
+            Q(g2, i - b1[0] + 1, j - b1[1] + 1, k - b1[2] + 1, v) =
+                Q(g1, i - a1[0] + 1, j - a1[1] + 1, k - a1[2] + 1, v);
+  }
+}
+
 /* SetGhost, lib/boundary.f90:58-89. */
 int orc_set_ghost(orc_hier *h, int level, int ipass) {
   if (!level_ok(h, level)) return fail(h, "set_ghost: bad level %d", level);
-  if (h->periodic) return fail(h, "periodic transfers are not restated");
   int n = nlev(h, level);
   if (level > 0) {
     for (int a = 0; a < n; a++) {
@@ -697,6 +792,15 @@ int orc_set_ghost(orc_hier *h, int level, int ipass) {
     for (int a = 0; a < n; a++)
       for (int b = a + 1; b < n; b++)
         ghost_overlap(h, lev_patch(h, level, a), lev_patch(h, level, b));
+  }
+  if (h->periodic) {
+    for (int a = 0; a < n; a++) periodic_bc(h, lev_patch(h, level, a));
+    if (level > 0)
+      for (int a = 0; a < n; a++)
+        for (int b = a + 1; b < n; b++) {
+          periodic_overlap(h, level, lev_patch(h, level, a), lev_patch(h, level, b));
+          periodic_overlap(h, level, lev_patch(h, level, b), lev_patch(h, level, a));
+        }
   }
   for (int a = 0; a < n; a++)
     if (set_bc_patch(h, lev_patch(h, level, a))) return -1;

@@ -55,7 +55,9 @@ typedef struct orc_hier {
   double fe_num, fe_den;       /* Dfe = ((c-fc_shift)*fe_num)/fe_den */
   int nsmoothingpasses, maxvcycles;
   double qerrortol;
-  int periodic;                /* periodic transfers not restated: must be 0 */
+  int periodic;                /* root has periodic faces (set at commit) */
+  int noff, offidx[13];        /* PeriodicSetup's offset list */
+  int rootmx[3];
   char err[256];
 } orc_hier;
 

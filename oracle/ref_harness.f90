@@ -28,7 +28,8 @@
 ! Input stream: for every level minlevel..finestlevel in level-list order
 ! (ApplyOnLevel order), the arrays q, f, qc of each patch.
 ! Output stream: the same order, arrays q, f, qc, rf.
-! Residual history (mgiter/step ops): one line per V-cycle in res.txt.
+! Residual history (mgiter/step ops): MultigridIterations prints one line per
+! V-cycle to stdout, after a 'harness step N' line per time step.
 program RefHarness
   use NodeInfoDef
   use ProblemDef
@@ -108,20 +109,16 @@ program RefHarness
         call ApplyOnLevel(lev, CopyToOld, dummy)
       end do
     case('mgiter')
-      ! MultigridIterations (lib/afasroutines.f90:38-106) called once per
-      ! V-cycle so that every cycle's residual is recorded; repeating the
-      ! prologue between cycles is idempotent on a fixed hierarchy.
-      maxvcycles = 1
-      do it = 1, maxv_user
-        call MultigridIterations
-        r = ErrorAFAS(finestlevel)
-        write(14,'(i6,1x,i6,1x,es25.17)') 1, it, r
-        if (r < qerrortol) exit
-      end do
+      ! MultigridIterations (lib/afasroutines.f90:38-106) itself, with
+      ! maxvcycles from the spec; it prints every cycle's residual to stdout,
+      ! which tests/golden/refrun.py records.
+      write(*,'(a,i6)') 'harness step ', 1
+      maxvcycles = maxv_user
+      call MultigridIterations
     case('step')
       ! a1 time steps on a fixed hierarchy, following TakeTimeSteps
       ! (lib/bsamroutines.f90:648-723) without regridding.
-      maxvcycles = 1
+      maxvcycles = maxv_user
       do istep = 1, a1
         do lev = rootlevel, finestlevel
           call SetGhost(lev, 0)
@@ -130,12 +127,8 @@ program RefHarness
         do lev = minlevel, finestlevel
           call ApplyOnLevel(lev, CopyToOld, dummy)
         end do
-        do it = 1, maxv_user
-          call MultigridIterations
-          r = ErrorAFAS(finestlevel)
-          write(14,'(i6,1x,i6,1x,es25.17)') istep, it, r
-          if (r < qerrortol) exit
-        end do
+        write(*,'(a,i6)') 'harness step ', istep
+        call MultigridIterations
       end do
     case default
       print *, 'harness: unknown op ', trim(opname)

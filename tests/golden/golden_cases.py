@@ -49,6 +49,27 @@ def _amr3d(mthbc=(1, 1, 1, 1, 1, 1)):
     return L
 
 
+def _periodic2d(mthbc=(2, 2, 2, 2)):
+    # children on the periodic faces, images of one another across them
+    L = Layout(2, (16, 16), 1.0 / 16, mthbc=mthbc)
+    a = L.add_child(0, ((1, 4), (3, 10)))
+    b = L.add_child(0, ((13, 16), (5, 12)))
+    L.add_child(0, ((6, 10), (13, 16)))
+    L.add_child(0, ((6, 9), (1, 2)))
+    L.add_child(a, ((1, 4), (1, 8)))
+    L.add_child(b, ((5, 8), (3, 10)))
+    return L
+
+
+def _periodic3d(mthbc=(2, 2, 2, 2, 2, 2)):
+    L = Layout(3, (8, 8, 8), 1.0 / 8, mthbc=mthbc)
+    a = L.add_child(0, ((1, 3), (2, 5), (1, 4)))
+    b = L.add_child(0, ((6, 8), (3, 6), (5, 8)))
+    L.add_child(a, ((1, 3), (1, 4), (1, 4)))
+    L.add_child(b, ((4, 6), (3, 6), (5, 8)))
+    return L
+
+
 LAYOUTS = {
     "single32": lambda: Layout(2, (32, 32), 1.0 / 32),
     "c1_128": lambda: Layout(2, (128, 128), 1.0 / 128),
@@ -58,6 +79,10 @@ LAYOUTS = {
     "amr3d": _amr3d,
     "amr3d_userbc": lambda: _amr3d((1, 10, 1, 10, 10, 1)),
     "single16_3d": lambda: Layout(3, (16, 16, 16), 1.0 / 16),
+    "periodic2d": _periodic2d,
+    "periodic2d_x_userbc": lambda: _periodic2d((2, 2, 10, 10)),
+    "periodic3d": _periodic3d,
+    "periodic3d_xz": lambda: _periodic3d((2, 2, 1, 1, 2, 2)),
 }
 
 OPS = [("setghost", 1, 0), ("setghost", 2, 1), ("setghost", 2, 2), ("relax", 2, 1),
@@ -73,6 +98,10 @@ OPS_CASES = {
     "rect2d_vcycle": ("rect2d", 4, [("vcycle", 2)]),
     "amr3d_seq": ("amr3d", 7, OPS),
     "amr3d_userbc_vcycle": ("amr3d_userbc", 8, [("vcycle", 2)]),
+    "periodic2d_seq": ("periodic2d", 9, OPS),
+    "periodic2d_x_userbc_vcycle": ("periodic2d_x_userbc", 10, [("vcycle", 2)]),
+    "periodic3d_seq": ("periodic3d", 11, OPS),
+    "periodic3d_xz_vcycle": ("periodic3d_xz", 12, [("vcycle", 2)]),
 }
 
 
@@ -111,6 +140,10 @@ STEP_CASES = {
                   eps=0.02 * 8, maxv=40, tol=1e-8, steps=2),
     "single16_3d": dict(layout="single16_3d", init=noise_init(0.5, 0.05, 8), dt=5e-4,
                         alpha=1.0, eps=0.1, maxv=40, tol=1e-8, steps=2),
+    "periodic2d": dict(layout="periodic2d", init=noise_init(0.5, 0.05, 13), dt=1e-3,
+                       alpha=1.0, eps=0.03, maxv=40, tol=1e-8, steps=3),
+    "periodic3d": dict(layout="periodic3d", init=noise_init(0.5, 0.05, 14), dt=5e-4,
+                       alpha=1.0, eps=0.16, maxv=40, tol=1e-8, steps=2),
     # fixed 6 V-cycles (tol 0) of the mapped +-1 problem, phi ~ 0.1 + U(-.05,.05)
     "plusminus_map": dict(layout="amr2d", init=noise_init(0.55, 0.025, 6), dt=PM_DT / 2,
                           alpha=8.0, eps=math.sqrt(2.0) * PM_EPS, maxv=6, tol=0.0, steps=1),

@@ -85,6 +85,16 @@ def run(layout, arrays, ops, dt=1e-3, alpha=1.0, eps=0.03, nsmooth=2, maxv=20,
         for line in fh:
             s, it, r = line.split()
             res.append((int(s), int(it), float(r)))
+    # MultigridIterations prints "it = N  residual = R" per V-cycle
+    # (lib/afasroutines.f90:96); the harness precedes each solve with
+    # "harness step S".
+    step = 0
+    for line in proc.stdout.splitlines():
+        tok = line.split()
+        if tok[:2] == ["harness", "step"]:
+            step = int(tok[2])
+        elif len(tok) == 6 and tok[0] == "it" and tok[3] == "residual":
+            res.append((step, int(tok[2]), float(tok[5])))
     listing = []
     with open(fout + ".patches") as fh:
         for line in fh:

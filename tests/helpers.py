@@ -63,3 +63,24 @@ def layout_amr3d(mthbc=(1, 1, 1, 1, 1, 1)):
     L.add_child(c1, ((2, 5), (3, 6), (2, 5)))
     L.add_child(c1, ((6, 8), (1, 4), (3, 6)))
     return L
+
+
+def layout_periodic2d(mthbc=(2, 2, 1, 1)):
+    """Children touching periodic faces, with periodic images of each other."""
+    L = Layout(2, (16, 16), 1.0 / 16, mthbc=mthbc)
+    a = L.add_child(0, ((1, 4), (3, 10)))
+    b = L.add_child(0, ((13, 16), (5, 12)))
+    L.add_child(0, ((6, 10), (13, 16)))
+    L.add_child(0, ((6, 9), (1, 2)))
+    L.add_child(a, ((1, 4), (1, 8)))
+    L.add_child(b, ((5, 8), (3, 10)))
+    return L
+
+
+def layout_periodic3d(mthbc=(2, 2, 2, 2, 2, 2)):
+    L = Layout(3, (8, 8, 8), 1.0 / 8, mthbc=mthbc)
+    a = L.add_child(0, ((1, 3), (2, 5), (1, 4)))
+    b = L.add_child(0, ((6, 8), (3, 6), (5, 8)))
+    L.add_child(a, ((1, 3), (1, 4), (1, 4)))
+    L.add_child(b, ((4, 6), (3, 6), (5, 8)))
+    return L

@@ -56,7 +56,9 @@ def run(mode, spec, inp, *extra):
     return float(diff[-1].split()[1]), out.stdout
 
 
-@pytest.mark.parametrize("lname", ["amr2d", "amr2d_userbc", "rect2d", "amr3d", "amr3d_userbc"])
+@pytest.mark.parametrize("lname", ["amr2d", "amr2d_userbc", "rect2d", "amr3d", "amr3d_userbc",
+                                   "periodic2d", "periodic2d_x_userbc", "periodic3d",
+                                   "periodic3d_xz"])
 def test_ghost_programs_on_reference_arrays(tmp_path, lname):
     ensure_built()
     spec, inp = write_case(str(tmp_path), lname, seed=1)
@@ -65,7 +67,7 @@ def test_ghost_programs_on_reference_arrays(tmp_path, lname):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("lname", ["amr2d", "rect2d", "amr3d"])
+@pytest.mark.parametrize("lname", ["amr2d", "rect2d", "amr3d", "periodic2d", "periodic3d"])
 def test_gpu_multigrid_iterations_replace_reference(tmp_path, lname):
     if not os.path.exists(BIN):
         pytest.skip("dropin_check not built (needs the reference build)")

@@ -10,7 +10,8 @@ import pytest
 
 import orc
 from bsam_amd.engine import Engine
-from helpers import layout_amr2d, layout_amr2d_tiles, layout_amr3d, layout_single
+from helpers import (layout_amr2d, layout_amr2d_tiles, layout_amr3d, layout_periodic2d,
+                     layout_periodic3d, layout_single)
 
 LAYOUTS = {
     "single16": lambda: layout_single(16),
@@ -23,6 +24,12 @@ LAYOUTS = {
     "amr3d": layout_amr3d,
     "amr3d_user_bc": lambda: layout_amr3d(mthbc=(10, 10, 10, 10, 10, 10)),
     "amr3d_mixed_bc": lambda: layout_amr3d(mthbc=(1, 10, 1, 10, 10, 1)),
+    "periodic_x": layout_periodic2d,
+    "periodic_xy": lambda: layout_periodic2d((2, 2, 2, 2)),
+    "periodic_x_user_y": lambda: layout_periodic2d((2, 2, 10, 10)),
+    "periodic_y": lambda: layout_periodic2d((1, 1, 2, 2)),
+    "periodic3d_xyz": layout_periodic3d,
+    "periodic3d_xz": lambda: layout_periodic3d((2, 2, 1, 1, 2, 2)),
 }
 
 

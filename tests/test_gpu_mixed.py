@@ -5,7 +5,15 @@ the bar stated here is on the result, not on bits: after each time step the
 relative FP64 residual is below qerrortol, phi and mu differ from the FP64
 oracle's by less than TOL_FIELD (a multiple of the solver tolerance), and
 the composite mass h^d sum(phi) over the root (which FillDown makes the
-composite integral) differs from the oracle's by less than TOL_MASS.
+composite integral) differs from the oracle's by less than TOL_MASS per step
+taken.
+
+Mass bound: summing the phi equation phi - dt Lap(mu) = f over the unit
+domain, two solutions whose residuals r1, r2 both meet the stop test differ in
+mass by |mean(r1 - r2)| <= 2 max RMS(r) (the Laplacian sums to zero up to the
+coarse-fine flux mismatch, second order in the field difference). The stop
+test bounds RMS(r) by qerrortol * RMS(f) and RMS(f) <= 1 here (|phi| < 1), so
+each step adds at most 2 * qerrortol.
 """
 
 import numpy as np
@@ -13,17 +21,20 @@ import pytest
 
 import orc
 from bsam_amd.engine import PREC_MIXED, Engine
-from helpers import layout_amr2d, layout_amr2d_tiles, layout_amr3d, layout_single, noise_q
+from helpers import (layout_amr2d, layout_amr2d_tiles, layout_amr3d, layout_periodic2d,
+                     layout_single, noise_q)
 
 pytestmark = pytest.mark.gpu
 
 TOL_FIELD = 2e-6     # max |phi - phi_fp64|, |mu - mu_fp64| after a step
-TOL_MASS = 1e-10     # |mass - mass_fp64| (domain area 1)
+QERRORTOL = 1e-8
+TOL_MASS = 2 * QERRORTOL   # |mass - mass_fp64| added per step (domain area 1)
 LAYOUTS = {
     "single32": lambda: layout_single(32),
     "amr2d": layout_amr2d,
     "amr2d_tiles": layout_amr2d_tiles,
     "amr3d": layout_amr3d,
+    "periodic2d": lambda: layout_periodic2d((2, 2, 2, 2)),
 }
 
 
@@ -38,7 +49,7 @@ def root_mass(lay, q_root):
 def test_mixed_matches_fp64(name, inner):
     lay = LAYOUTS[name]()
     kw = dict(dt=1e-3, eps=0.05 if lay.ndims == 3 else 0.03, well="plusminus",
-              maxvcycles=40, qerrortol=1e-8)
+              maxvcycles=40, qerrortol=QERRORTOL)
     q0 = noise_q(lay, seed=21, mean=0.1, amp=0.05)
     eng = Engine(lay, device=0, precision=PREC_MIXED, tune={"inner_cycles": inner}, **kw)
     ora = orc.Oracle(lay, **kw)
@@ -61,5 +72,5 @@ def test_mixed_matches_fp64(name, inner):
         assert worst < TOL_FIELD, (step, worst)
         m_got = root_mass(lay, eng.download(root, "q"))
         m_want = root_mass(lay, ora.field(root, "q"))
-        assert abs(m_got - m_want) < TOL_MASS, (m_got, m_want)
+        assert abs(m_got - m_want) < TOL_MASS * (step + 1), (m_got, m_want)
     eng.close()

@@ -11,8 +11,8 @@ import pytest
 
 import orc
 from bsam_amd.engine import Engine
-from helpers import (layout_amr2d, layout_amr2d_tiles, layout_amr3d, layout_single,
-                     noise_q, random_fields)
+from helpers import (layout_amr2d, layout_amr2d_tiles, layout_amr3d, layout_periodic2d,
+                     layout_periodic3d, layout_single, noise_q, random_fields)
 
 pytestmark = pytest.mark.gpu
 
@@ -25,6 +25,9 @@ LAYOUTS = {
     "single8_3d": lambda: layout_single(8, ndims=3),
     "amr3d": layout_amr3d,
     "amr3d_mixed_bc": lambda: layout_amr3d(mthbc=(1, 10, 1, 10, 10, 1)),
+    "periodic2d_xy": lambda: layout_periodic2d((2, 2, 2, 2)),
+    "periodic2d_x_user_y": lambda: layout_periodic2d((2, 2, 10, 10)),
+    "periodic3d_xyz": layout_periodic3d,
 }
 
 
