@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstring>
 #include <stdexcept>
+#include <mutex>
 #include <string>
 #include <unordered_map>
 #include <unordered_set>
@@ -29,11 +30,20 @@ struct Error : std::runtime_error {
       throw Error(std::string(#x) + ": " + hipGetErrorString(e_), -2);              \
   } while (0)
 
-template <class T> T *dev_upload(const std::vector<T> &v) {
+// Contexts may be driven from several host threads at once (one per stream).
+// Stream capture forbids device-wide synchronisation and legacy-stream work
+// while any stream is capturing, and hipFree / hipMalloc / stream creation may
+// imply such work, so those calls and every capture hold this lock. All
+// copies and fills go on the context's own (non-blocking) stream.
+static std::recursive_mutex g_device_lock;
+using DeviceLock = std::lock_guard<std::recursive_mutex>;
+
+template <class T> T *dev_upload(const std::vector<T> &v, hipStream_t s) {
   if (v.empty()) return nullptr;
   T *p = nullptr;
   HIPCK(hipMalloc(&p, sizeof(T) * v.size()));
-  HIPCK(hipMemcpy(p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+  HIPCK(hipMemcpyAsync(p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, s));
+  HIPCK(hipStreamSynchronize(s));
   return p;
 }
 
@@ -69,6 +79,7 @@ struct bsam_ctx {
   int partial_cap = 0;
   double *d_hist = nullptr;
   int *d_state = nullptr;
+  double *d_scale = nullptr;   // residual-threshold scale of the mixed solve
   int *h_state = nullptr;
   // kernel timing with HIP events on the context's stream (bsam_profile_*)
   unsigned prof_mask = 0;
@@ -127,13 +138,19 @@ struct bsam_ctx {
 
 void bsam_ctx::free_device() {
   if (host_only) return;
-  auto F = [](void *p) {
-    if (p) (void)hipFree(p);
+  DeviceLock lock(g_device_lock);
+  // Frees and forgets: free_device runs again at hierarchy_begin and in the
+  // destructor, and a second hipFree of a stale pointer would fail (leaving
+  // a sticky error) or free another context's reuse of the address.
+  auto F = [](auto *&p) {
+    if (p) (void)hipFree((void *)p);
+    p = nullptr;
   };
   F(q); F(qold); F(qc); F(f); F(rf); F(ftmp); F(d_P); F(d_partial); F(d_hist);
   F(e); F(qce); F(fe); F(qb);
   e = qce = fe = qb = nullptr;
   F(d_state);
+  F(d_scale);
   F(d_tail);
   d_tail = nullptr;
   drop_graphs();
@@ -158,6 +175,7 @@ struct ProfScope {
   ProfScope(bsam_ctx *cc, int kind, double bytes) : c(cc) {
     if (!(c->prof_mask & (1u << kind))) return;
     if (c->ev_used + 2 > c->ev_pool.size()) {
+      DeviceLock lock(g_device_lock);
       for (int n = 0; n < 256; n++) {
         hipEvent_t e;
         HIPCK(hipEventCreate(&e));
@@ -313,37 +331,39 @@ static void commit(bsam_ctx *c) {
   }
   if (c->host_only) return;
   // device allocation
+  DeviceLock lock(g_device_lock);
   c->free_device();
+  hipStream_t s = c->stream;
   HIPCK(hipMalloc(&c->q, sizeof(double) * c->nq));
   HIPCK(hipMalloc(&c->qold, sizeof(double) * c->nq));
   HIPCK(hipMalloc(&c->qc, sizeof(double) * c->nc));
   HIPCK(hipMalloc(&c->f, sizeof(double) * c->nf));
   HIPCK(hipMalloc(&c->rf, sizeof(double) * c->nf));
   HIPCK(hipMalloc(&c->ftmp, sizeof(double) * c->nf));
-  HIPCK(hipMemset(c->q, 0, sizeof(double) * c->nq));
-  HIPCK(hipMemset(c->qold, 0, sizeof(double) * c->nq));
-  HIPCK(hipMemset(c->qc, 0, sizeof(double) * c->nc));
-  HIPCK(hipMemset(c->f, 0, sizeof(double) * c->nf));
-  HIPCK(hipMemset(c->rf, 0, sizeof(double) * c->nf));
-  HIPCK(hipMemset(c->ftmp, 0, sizeof(double) * c->nf));
+  HIPCK(hipMemsetAsync(c->q, 0, sizeof(double) * c->nq, s));
+  HIPCK(hipMemsetAsync(c->qold, 0, sizeof(double) * c->nq, s));
+  HIPCK(hipMemsetAsync(c->qc, 0, sizeof(double) * c->nc, s));
+  HIPCK(hipMemsetAsync(c->f, 0, sizeof(double) * c->nf, s));
+  HIPCK(hipMemsetAsync(c->rf, 0, sizeof(double) * c->nf, s));
+  HIPCK(hipMemsetAsync(c->ftmp, 0, sizeof(double) * c->nf, s));
   if (c->cfg.precision == BSAM_PREC_MIXED) {
     HIPCK(hipMalloc(&c->e, sizeof(float) * c->nq));
     HIPCK(hipMalloc(&c->qb, sizeof(float) * c->nq));
     HIPCK(hipMalloc(&c->qce, sizeof(float) * c->nc));
     HIPCK(hipMalloc(&c->fe, sizeof(float) * c->nf));
-    HIPCK(hipMemset(c->e, 0, sizeof(float) * c->nq));
-    HIPCK(hipMemset(c->qb, 0, sizeof(float) * c->nq));
-    HIPCK(hipMemset(c->qce, 0, sizeof(float) * c->nc));
-    HIPCK(hipMemset(c->fe, 0, sizeof(float) * c->nf));
+    HIPCK(hipMemsetAsync(c->e, 0, sizeof(float) * c->nq, s));
+    HIPCK(hipMemsetAsync(c->qb, 0, sizeof(float) * c->nq, s));
+    HIPCK(hipMemsetAsync(c->qce, 0, sizeof(float) * c->nc, s));
+    HIPCK(hipMemsetAsync(c->fe, 0, sizeof(float) * c->nf, s));
   }
-  c->d_P = dev_upload(c->dp);
+  c->d_P = dev_upload(c->dp, s);
   size_t maxt = 1;
   for (auto &L : c->lev) {
-    L.d_int = dev_upload(L.t_int);
-    L.d_ext = dev_upload(L.t_ext);
-    L.d_coarse = dev_upload(L.t_coarse);
-    L.d_cext = dev_upload(L.t_cext);
-    for (int ip = 0; ip < 3; ip++) L.d_gp[ip] = dev_upload(L.gp[ip]);
+    L.d_int = dev_upload(L.t_int, s);
+    L.d_ext = dev_upload(L.t_ext, s);
+    L.d_coarse = dev_upload(L.t_coarse, s);
+    L.d_cext = dev_upload(L.t_cext, s);
+    for (int ip = 0; ip < 3; ip++) L.d_gp[ip] = dev_upload(L.gp[ip], s);
     maxt = std::max(maxt, L.t_int.size());
   }
   // tail: the longest run of single-patch levels from minlevel whose cell
@@ -364,15 +384,16 @@ static void commit(bsam_ctx *c) {
       }
       tl.push_back(t);
     }
-    c->d_tail = dev_upload(tl);
+    c->d_tail = dev_upload(tl, s);
   }
   c->partial_cap = (int)maxt;
   HIPCK(hipMalloc(&c->d_partial, sizeof(double) * maxt));
   int hmax = std::max(c->cfg.maxvcycles, 1) + 1;
   HIPCK(hipMalloc(&c->d_hist, sizeof(double) * hmax));
   HIPCK(hipMalloc(&c->d_state, sizeof(int) * 4));
+  HIPCK(hipMalloc(&c->d_scale, sizeof(double)));
   HIPCK(hipHostMalloc(&c->h_state, sizeof(int) * 4));
-  HIPCK(hipDeviceSynchronize());
+  HIPCK(hipStreamSynchronize(s));
 }
 
 // ---- level operations ------------------------------------------------------------
@@ -468,7 +489,7 @@ static void reset_state(bsam_ctx *c) {
   HIPCK(hipMemsetAsync(c->d_state, 0, sizeof(int) * 4, c->stream));
 }
 // queue ErrorAFAS(level): residual norm into d_hist[state[0]]
-static void queue_error(bsam_ctx *c, int level, double tol) {
+static void queue_error(bsam_ctx *c, int level, double tol, const double *scale = nullptr) {
   LevelData &L = c->L(level);
   int nt = (int)L.t_int.size();
   {
@@ -476,12 +497,13 @@ static void queue_error(bsam_ctx *c, int level, double tol) {
     HIPCK(launch_resnorm(c->nd(), c->dev(), L.d_int, nt, c->d_partial, 0, c->stream));
   }
   double count = (double)(c->cfg.nrvars * L.cells);
-  HIPCK(launch_finalize(c->d_partial, nt, count, tol, c->d_hist, c->d_state, c->stream));
+  HIPCK(launch_finalize(c->d_partial, nt, count, tol, scale, c->d_hist, c->d_state, c->stream));
 }
 
 // Captures `body` on the context's stream into *exec (once per hierarchy).
 template <class F> static void capture(bsam_ctx *c, hipGraph_t *g, hipGraphExec_t *exec, F body) {
   if (*exec) return;
+  DeviceLock lock(g_device_lock);
   HIPCK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
   try {
     body();
@@ -523,7 +545,9 @@ static void report(bsam_ctx *c, int n, int done, int *ncycles, double *hist, int
   if (converged) *converged = done;
   if (hist && maxhist > 0) {
     int m = std::min(n, maxhist);
-    HIPCK(hipMemcpy(hist, c->d_hist, sizeof(double) * m, hipMemcpyDeviceToHost));
+    HIPCK(hipMemcpyAsync(hist, c->d_hist, sizeof(double) * m, hipMemcpyDeviceToHost,
+                         c->stream));
+    HIPCK(hipStreamSynchronize(c->stream));
   }
 }
 
@@ -564,10 +588,14 @@ static void mixed_iterations(bsam_ctx *c, int *ncycles, double *hist, int maxhis
   double count = (double)(c->cfg.nrvars * LF.cells);
   reset_state(c);
   HIPCK(launch_resnorm(c->nd(), c->dev(), LF.d_int, nt, c->d_partial, 1, c->stream));
-  HIPCK(launch_finalize(c->d_partial, nt, count, -1.0, c->d_hist, c->d_state, c->stream));
+  HIPCK(launch_finalize(c->d_partial, nt, count, -1.0, nullptr, c->d_hist, c->d_state,
+                        c->stream));
   HIPCK(hipMemcpyAsync(&c->fnorm, c->d_hist, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCK(hipStreamSynchronize(c->stream));
   double fn = c->fnorm > 0.0 ? c->fnorm : 1.0;
+  // the captured iteration reads the threshold scale from d_scale
+  HIPCK(hipMemcpyAsync(c->d_scale, &fn, sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCK(hipStreamSynchronize(c->stream));
   reset_state(c);
   int done = 0;
   int n = iterate(c, &c->graph_mixed, &c->graph_mixed_exec, [&]() {
@@ -583,7 +611,7 @@ static void mixed_iterations(bsam_ctx *c, int *ncycles, double *hist, int maxhis
     }
     op_fill_down(c, BSAM_FIELD_Q);
     for (int l = 0; l <= c->finest; l++) op_set_ghost(c, l, 0);
-    queue_error(c, c->finest, c->cfg.qerrortol * fn);
+    queue_error(c, c->finest, c->cfg.qerrortol, c->d_scale);
   }, &done);
   report(c, n, done, ncycles, hist, maxhist, converged);
   if (hist)
@@ -639,6 +667,7 @@ bsam_ctx *bsam_ctx_create(const bsam_cfg *cfg) {
       delete c;
       return nullptr;
     }
+    DeviceLock lock(g_device_lock);
     if (hipSetDevice(cfg->device) != hipSuccess ||
         hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
       g_err = "cannot create a stream on the device";
@@ -654,8 +683,10 @@ int bsam_ctx_destroy(bsam_ctx *c) {
   if (!c->host_only) {
     (void)hipSetDevice(c->cfg.device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    DeviceLock lock(g_device_lock);
     c->free_device();
     if (c->stream) (void)hipStreamDestroy(c->stream);
+    c->stream = nullptr;
   }
   delete c;
   return 0;

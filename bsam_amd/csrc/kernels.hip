@@ -651,8 +651,12 @@ __global__ __launch_bounds__(256) void k_resnorm(Dev d, const Tile *tiles, doubl
 // Sum the per-tile partials in index order (strided, then a fixed tree) and
 // finish ErrorAFAS: sqrt(sum / count) into hist[state[0]]; state[0] counts
 // the calls and state[1] is set once a residual falls below tol.
+// Residual norm sqrt(sum / count) into hist[state[0]]; sets state[1] when it
+// is below tol (times *scale when scale is given: a threshold that changes
+// between calls stays out of the captured kernel arguments).
 __global__ __launch_bounds__(256) void k_finalize(const double *partial, int n, double count,
-                                                   double tol, double *hist, int *state) {
+                                                   double tol, const double *scale,
+                                                   double *hist, int *state) {
   double s = 0.0;
   for (int e = threadIdx.x; e < n; e += blockDim.x) s += partial[e];
   double tot = block_sum(s);
@@ -661,7 +665,8 @@ __global__ __launch_bounds__(256) void k_finalize(const double *partial, int n, 
     int it = state[0];
     hist[it] = r;
     state[0] = it + 1;
-    if (r < tol) state[1] = 1;
+    double thr = scale ? tol * scale[0] : tol;
+    if (r < thr) state[1] = 1;
   }
 }
 
@@ -780,8 +785,8 @@ hipError_t launch_resnorm(int nd, const Dev &d, const Tile *tiles, int nt, doubl
 }
 
 hipError_t launch_finalize(const double *partial, int n, double count, double tol,
-                           double *hist, int *state, hipStream_t s) {
-  k_finalize<<<1, 256, 0, s>>>(partial, n, count, tol, hist, state);
+                           const double *scale, double *hist, int *state, hipStream_t s) {
+  k_finalize<<<1, 256, 0, s>>>(partial, n, count, tol, scale, hist, state);
   return hipGetLastError();
 }
 

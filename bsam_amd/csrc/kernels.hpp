@@ -27,7 +27,7 @@ hipError_t launch_apply_correction(int nd, const Dev &d, const Tile *tiles, int 
 hipError_t launch_resnorm(int nd, const Dev &d, const Tile *tiles, int nt, double *partial,
                           int what, hipStream_t s);
 hipError_t launch_finalize(const double *partial, int n, double count, double tol,
-                           double *hist, int *state, hipStream_t s);
+                           const double *scale, double *hist, int *state, hipStream_t s);
 hipError_t launch_copy(double *dst, const double *src, long long n, hipStream_t s);
 hipError_t launch_tail_vcycle(int nd, int prec, const Dev &d, const TailLevel *lv, int nlev,
                               int nsmooth, int threads, hipStream_t s);

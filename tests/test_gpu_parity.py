@@ -67,7 +67,7 @@ def test_set_ghost_all_levels(name):
     lay = LAYOUTS[name]()
     for level in range(lay.minlevel, lay.finestlevel + 1):
         for ipass in (0, 1, 2):
-            eng, ora = pair(lay, random_fields(lay, seed=level * 3 + ipass))
+            eng, ora = pair(lay, random_fields(lay, seed=(level - lay.minlevel) * 3 + ipass))
             eng.set_ghost(level, ipass)
             ora.set_ghost(level, ipass)
             assert_same(eng, ora)
@@ -84,7 +84,7 @@ def test_level_operations(name):
                ("restrict", (level,), ("q", "qc")), ("coarse_ghost", (level,), ("q", "qc")),
                ("coarse_load", (level,), ("q", "f")), ("correct", (level,), ("q",))]
         for op, args, names in ops:
-            eng, ora = pair(lay, random_fields(lay, seed=level + 17))
+            eng, ora = pair(lay, random_fields(lay, seed=(level - lay.minlevel) + 17))
             getattr(eng, op)(*args)
             getattr(ora, op)(*args)
             assert_same(eng, ora, names=names)
