@@ -44,12 +44,16 @@ struct Tile {
   int patch, i0, j0, k0;
 };
 
-// One level of the single-patch coarse chain, for the tail kernels: its patch
-// and its three SetGhost programs (ipass 0, 1, 2).
+// One level of the single-patch coarse chain, for the tail kernels: its patch,
+// its three SetGhost programs (ipass 0, 1, 2; offsets relative to the patch's
+// q origin) and where its arrays live: lds[set][a] is a byte offset into the
+// tail kernel's dynamic LDS or -1 for the global pool, for a = q, qc, f, qb and
+// set 0 (FP64 solution) or 1 (FP32 correction).
 struct TailLevel {
   int patch;
   int ngp[3];
   const GhostOp *gp[3];
+  int lds[2][4];
 };
 
 struct Problem {

@@ -53,8 +53,33 @@ struct LevelData {
   Tile *d_int = nullptr, *d_ext = nullptr, *d_coarse = nullptr, *d_cext = nullptr;
   std::vector<GhostOp> gp[3];
   GhostOp *d_gp[3] = {nullptr, nullptr, nullptr};
+  GhostOp *d_tgp[3] = {nullptr, nullptr, nullptr};  // patch-local copies (tail levels)
   long long cells = 0;
 };
+
+// The SetGhost programs of a single-patch level with offsets relative to the
+// patch's q origin, for the tail kernels. Returns false unless every update is
+// a copy within the patch (physical and periodic boundaries of a level that
+// covers the domain).
+bool local_programs(const HPatch &h, const std::vector<GhostOp> (&gp)[3], int nv,
+                    std::vector<GhostOp> (&out)[3]) {
+  long long n = h.qvs;
+  for (int ip = 0; ip < 3; ip++) {
+    out[ip].clear();
+    for (GhostOp g : gp[ip]) {
+      long long dst = g.dst - h.qoff, src = g.f0 - h.qoff;
+      if (g.kind != 0 || dst < 0 || dst >= n || src < 0 || src >= n || g.dvs != n ||
+          g.fvs != n)
+        return false;
+      (void)nv;
+      g.dst = (int)dst;
+      g.f0 = (int)src;
+      g.f1 = 0;
+      out[ip].push_back(g);
+    }
+  }
+  return true;
+}
 
 }  // namespace
 
@@ -99,6 +124,8 @@ struct bsam_ctx {
   // hipGraph replay of the multigrid iteration
   long long tail_cells = 4096;
   int tail_threads = 1024;
+  long long tail_lds_cap = -1;      // LDS bytes the tail may use (-1: device limit)
+  size_t tail_lds[2] = {0, 0};      // dynamic LDS of the tail kernels per field set
   int use_graph = 1;
   int tail_top = 0;                 // highest level run by the tail kernels
   TailLevel *d_tail = nullptr;      // levels minlevel..tail_top
@@ -158,7 +185,10 @@ void bsam_ctx::free_device() {
   d_P = nullptr; d_partial = nullptr; d_hist = nullptr; d_state = nullptr;
   for (auto &l : lev) {
     F(l.d_int); F(l.d_ext); F(l.d_coarse); F(l.d_cext);
-    for (int p = 0; p < 3; p++) F(l.d_gp[p]);
+    for (int p = 0; p < 3; p++) {
+      F(l.d_gp[p]);
+      F(l.d_tgp[p]);
+    }
   }
   if (h_state) (void)hipHostFree(h_state);
   h_state = nullptr;
@@ -232,6 +262,55 @@ static void build_tiles(bsam_ctx *c, LevelData &L, int level) {
     }
     L.cells += (long long)h.mx[0] * h.mx[1] * (nd == 3 ? h.mx[2] : 1);
   }
+}
+
+// Places the tail levels' arrays in LDS within `budget` bytes (coarsest level
+// first, per level q, qc, qb, f; what does not fit stays in the global pools)
+// and uploads the tail level table. Returns false if the runtime refuses the
+// resulting dynamic LDS size.
+static bool plan_tail(bsam_ctx *c, long long budget, hipStream_t s) {
+  const int minl = c->cfg.minlevel, nv = c->cfg.nrvars;
+  std::vector<TailLevel> tl;
+  long long used[2] = {0, 0};
+  bool open[2][4] = {{true, true, true, true}, {true, true, true, true}};
+  for (int l = c->cfg.minlevel; l <= c->tail_top; l++) {
+    LevelData &L = c->lev[l - minl];
+    const DPatch &D = c->dp[L.patches[0]];
+    TailLevel t;
+    t.patch = L.patches[0];
+    for (int ip = 0; ip < 3; ip++) {
+      t.ngp[ip] = (int)L.gp[ip].size();
+      t.gp[ip] = L.d_tgp[ip];
+    }
+    for (int set = 0; set < 2; set++) {
+      long long sz = set == 0 ? 8 : 4;
+      long long bytes[4] = {D.qvs * nv * sz, D.cvs * nv * sz, D.fvs * nv * sz,
+                            set == 1 ? D.qvs * sz : 0};
+      for (int a = 0; a < 4; a++) t.lds[set][a] = -1;
+      bool usable = set == 0 || c->cfg.precision == BSAM_PREC_MIXED;
+      for (int a : {0, 1, 3, 2}) {
+        long long b = (bytes[a] + 15) & ~15LL;
+        if (!usable || b == 0 || !open[set][a]) continue;
+        if (used[set] + b > budget) {
+          open[set][a] = false;   // larger levels would not fit either
+          continue;
+        }
+        t.lds[set][a] = (int)used[set];
+        used[set] += b;
+      }
+    }
+    tl.push_back(t);
+  }
+  size_t need = (size_t)std::max(used[0], used[1]);
+  if (need > 0 && tail_allow_lds(need) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  c->tail_lds[0] = (size_t)used[0];
+  c->tail_lds[1] = (size_t)used[1];
+  if (c->d_tail) (void)hipFree(c->d_tail);
+  c->d_tail = dev_upload(tl, s);
+  return true;
 }
 
 static void commit(bsam_ctx *c) {
@@ -367,24 +446,32 @@ static void commit(bsam_ctx *c) {
     maxt = std::max(maxt, L.t_int.size());
   }
   // tail: the longest run of single-patch levels from minlevel whose cell
-  // counts stay within tail_cells (levels <= 0 hold one patch each)
+  // counts stay within tail_cells (levels <= 0 hold one patch each) and whose
+  // ghost programs are patch-local copies
   c->tail_top = c->cfg.minlevel - 1;
-  for (int l = c->cfg.minlevel; l <= std::min(0, c->finest); l++)
-    if (c->lev[l - minl].cells <= c->tail_cells) c->tail_top = l;
-    else break;
+  std::vector<std::vector<GhostOp>> tprog;
+  for (int l = c->cfg.minlevel; l <= std::min(0, c->finest); l++) {
+    LevelData &L = c->lev[l - minl];
+    std::vector<GhostOp> loc[3];
+    if (L.cells > c->tail_cells || L.patches.size() != 1 ||
+        !local_programs(c->hp[L.patches[0]], L.gp, nv, loc))
+      break;
+    c->tail_top = l;
+    for (int ip = 0; ip < 3; ip++) L.d_tgp[ip] = dev_upload(loc[ip], s);
+  }
+  c->tail_lds[0] = c->tail_lds[1] = 0;
   if (c->ntail() > 0) {
-    std::vector<TailLevel> tl;
-    for (int l = c->cfg.minlevel; l <= c->tail_top; l++) {
-      LevelData &L = c->lev[l - minl];
-      TailLevel t;
-      t.patch = L.patches[0];
-      for (int ip = 0; ip < 3; ip++) {
-        t.ngp[ip] = (int)L.gp[ip].size();
-        t.gp[ip] = L.d_gp[ip];
-      }
-      tl.push_back(t);
+    // LDS placement, coarsest level first and per level q, qc, qb, f, while
+    // the budget lasts; the rest stays in the global pools
+    int maxlds = 0;
+    HIPCK(hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock,
+                                c->cfg.device));
+    long long budget = c->tail_lds_cap < 0 ? maxlds : std::min<long long>(c->tail_lds_cap, maxlds);
+    // if the runtime refuses the planned dynamic LDS, plan again within the
+    // default 64 KiB, then without LDS
+    for (long long b : {budget, std::min<long long>(budget, 65536), 0LL}) {
+      if (plan_tail(c, b, s)) break;
     }
-    c->d_tail = dev_upload(tl, s);
   }
   c->partial_cap = (int)maxt;
   HIPCK(hipMalloc(&c->d_partial, sizeof(double) * maxt));
@@ -470,7 +557,8 @@ static void op_vcycle(bsam_ctx *c, int level, int prec = PREC_SOLUTION_FP64) {
   if (level <= c->tail_top) {
     ProfScope ps(c, BSAM_PROF_TAIL, 0.0);
     HIPCK(launch_tail_vcycle(c->nd(), prec, c->dev(), c->d_tail, level - c->cfg.minlevel + 1,
-                             c->cfg.nsmoothingpasses, c->tail_threads, c->stream));
+                             c->cfg.nsmoothingpasses, c->tail_threads,
+                             c->tail_lds[prec == PREC_SOLUTION_FP64 ? 0 : 1], c->stream));
     return;
   }
   level_relax(c, level, prec);
@@ -907,6 +995,9 @@ int bsam_tune(bsam_ctx *c, const char *key, double value) {
   if (k == "tail_cells") {
     if (c->committed || c->building) throw Error("tail_cells must be set before the hierarchy");
     c->tail_cells = (long long)value;
+  } else if (k == "tail_lds") {
+    if (c->committed || c->building) throw Error("tail_lds must be set before the hierarchy");
+    c->tail_lds_cap = (long long)value;
   } else if (k == "tail_threads") {
     int t = (int)value;
     if (t < 64 || t > 1024 || t % 64) throw Error("tail_threads must be 64..1024, multiple of 64");
@@ -922,6 +1013,23 @@ int bsam_tune(bsam_ctx *c, const char *key, double value) {
   } else {
     throw Error("unknown tuning key " + k);
   }
+  API_END(c)
+}
+
+int bsam_tune_get(const bsam_ctx *cc, const char *key, double *value) {
+  bsam_ctx *c = const_cast<bsam_ctx *>(cc);
+  API_BEGIN(c)
+  std::string k = key ? key : "";
+  if (!value) throw Error("null value pointer");
+  if (k == "tail_cells") *value = (double)c->tail_cells;
+  else if (k == "tail_threads") *value = c->tail_threads;
+  else if (k == "tail_lds") *value = (double)c->tail_lds_cap;
+  else if (k == "inner_cycles") *value = c->inner_cycles;
+  else if (k == "graph") *value = c->use_graph;
+  else if (k == "tail_top") *value = c->tail_top;
+  else if (k == "tail_lds_fp64") *value = (double)c->tail_lds[0];
+  else if (k == "tail_lds_fp32") *value = (double)c->tail_lds[1];
+  else throw Error("unknown tuning key " + k);
   API_END(c)
 }
 

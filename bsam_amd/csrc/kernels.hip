@@ -46,7 +46,7 @@ __device__ __forceinline__ T d2fc_at(const Problem &p, T b, T e) {
   return (T)3 * x * x * (T)p.fc_alpha;
 }
 
-// ---- field sets and array views ------------------------------------------------
+// ---- field sets and level references ------------------------------------------
 template <class T> struct Fld {
   T *q, *qc, *f;     // solution-layout, qc-layout and f-layout pools
   const T *qb;       // FP32 base point (defect form), q layout
@@ -59,8 +59,24 @@ template <> __device__ __forceinline__ Fld<float> fields<float>(const Dev &d) {
   return Fld<float>{d.e, d.qce, d.fe, d.qb};
 }
 
+// One patch's arrays, each addressed from its own origin: q(0,0,0,1),
+// qc(0,0,0,1), f(1,1,1,1) and the defect base qb(0,0,0). The arrays may live in
+// the global pools or, in the tail kernels, in LDS; the layout (pitches and
+// variable strides of the patch record P) is the same in both. P stays a
+// reference into the patch table (scalar loads) rather than a register copy.
+template <class T> struct Lvl {
+  T *q, *qc, *f;
+  const T *qb;
+  const DPatch &P;
+};
+template <class T> __device__ __forceinline__ Lvl<T> level_ref(const Dev &d, int patch) {
+  Fld<T> F = fields<T>(d);
+  const DPatch &P = d.P[patch];
+  return Lvl<T>{F.q + P.q, F.qc + P.qc, F.f + P.f, F.qb ? F.qb + P.q : nullptr, P};
+}
+
 // A ghost-extended array with lower bound 0: element (i,j,k,v) at
-// base + i + px*(j + py*k) + v*vs (k = 0 in 2D).
+// p + i + px*(j + py*k) + v*vs (k = 0 in 2D).
 template <class T> struct View {
   T *p;
   long long px, pl, vs;
@@ -68,11 +84,11 @@ template <class T> struct View {
     return i + px * j + pl * k;
   }
 };
-template <class T> __device__ __forceinline__ View<T> qview(T *base, const DPatch &P) {
-  return View<T>{base + P.q, P.px, (long long)P.px * P.py, P.qvs};
+template <class T> __device__ __forceinline__ View<T> qview(const Lvl<T> &L) {
+  return View<T>{L.q, L.P.px, (long long)L.P.px * L.P.py, L.P.qvs};
 }
-template <class T> __device__ __forceinline__ View<T> qcview(T *base, const DPatch &P) {
-  return View<T>{base + P.qc, P.cmx + 2, (long long)(P.cmx + 2) * (P.cmy + 2), P.cvs};
+template <class T> __device__ __forceinline__ View<T> qcview(const Lvl<T> &L) {
+  return View<T>{L.qc, L.P.cmx + 2, (long long)(L.P.cmx + 2) * (L.P.cmy + 2), L.P.cvs};
 }
 // Interior f-type arrays (lower bound 1).
 __device__ __forceinline__ long long fat(const DPatch &P, int i, int j, int k) {
@@ -112,20 +128,19 @@ __device__ __forceinline__ void op_at(const Problem &prm, const View<T> &a, T ba
 // :453-518; Mob = 1). Defect form: the Newton step linearises
 // Dfc(base + e) about the current e.
 template <int ND, class T, bool DEF>
-__device__ __forceinline__ void relax_cell(const Dev &d, const Fld<T> &F, const DPatch &P,
-                                           int i, int j, int k) {
-  const Problem &prm = d.prm;
-  View<T> a = qview(F.q, P);
+__device__ __forceinline__ void relax_cell_f(const Problem &prm, const Lvl<T> &L, int i, int j,
+                                             int k, T fv1, T fv2) {
+  const DPatch &P = L.P;
+  View<T> a = qview(L);
   long long c = a.at(i, j, k);
   T *q1 = a.p, *q2 = a.p + a.vs;
-  long long fo = P.f + fat(P, i, j, ND == 3 ? k : 1);
   double h2 = P.dx * P.dx;
   T tmp1 = (T)(prm.dt / h2), tmp2 = (T)(prm.eps2 / h2);
   T tmp3 = (T)((ND == 2 ? 4.0 : 6.0) * prm.eps2 / h2);
   T phi = q1[c];
   T dc, d2;
   if (DEF) {
-    T base = F.qb[P.q + c];
+    T base = L.qb[c];
     dc = dfc_delta<T>(prm, base, phi);
     d2 = d2fc_at<T>(prm, base, phi);
   } else {
@@ -137,21 +152,27 @@ __device__ __forceinline__ void relax_cell(const Dev &d, const Fld<T> &F, const 
   if (ND == 2) {
     a12 = tmp1 * (T)4;
     a21 = -d2 - tmp3;
-    b1 = F.f[fo] + tmp1 * (((q2[c - sx] + q2[c + sx]) + q2[c - sy]) + q2[c + sy]);
-    b2 = ((F.f[fo + P.fvs] + dc) - d2 * phi) -
+    b1 = fv1 + tmp1 * (((q2[c - sx] + q2[c + sx]) + q2[c - sy]) + q2[c + sy]);
+    b2 = ((fv2 + dc) - d2 * phi) -
          tmp2 * (((q1[c + sx] + q1[c - sx]) + q1[c + sy]) + q1[c - sy]);
   } else {
     a12 = tmp1 * (T)6;
     a21 = -d2 - tmp3;
-    b1 = F.f[fo] + tmp1 * (((((q2[c - sx] + q2[c + sx]) + q2[c - sy]) + q2[c + sy]) +
-                            q2[c - sz]) + q2[c + sz]);
-    b2 = ((F.f[fo + P.fvs] + dc) - d2 * phi) -
+    b1 = fv1 + tmp1 * (((((q2[c - sx] + q2[c + sx]) + q2[c - sy]) + q2[c + sy]) +
+                         q2[c - sz]) + q2[c + sz]);
+    b2 = ((fv2 + dc) - d2 * phi) -
          tmp2 * (((((q1[c + sx] + q1[c - sx]) + q1[c + sy]) + q1[c - sy]) + q1[c + sz]) +
                  q1[c - sz]);
   }
   T det = (T)1 - a12 * a21;
   q1[c] = (b1 - a12 * b2) / det;
   q2[c] = (b2 - a21 * b1) / det;
+}
+template <int ND, class T, bool DEF>
+__device__ __forceinline__ void relax_cell(const Problem &prm, const Lvl<T> &L, int i, int j,
+                                           int k) {
+  long long fo = fat(L.P, i, j, ND == 3 ? k : 1);
+  relax_cell_f<ND, T, DEF>(prm, L, i, j, k, L.f[fo], L.f[fo + L.P.fvs]);
 }
 
 // Restriction2D/3D (gridutilities.f90:89-110, :406-433) of the child cells
@@ -186,15 +207,17 @@ __device__ __forceinline__ T restrict_f(const T *x, const DPatch &P, int ci, int
 }
 
 // ---- per-cell operations ----------------------------------------------------
-// Each function performs the reference operation for one cell; the level
+// Each function performs the reference operation for one cell of a child
+// patch C (and its parent PA where the operation couples the two); the level
 // kernels below map one tile of cells to a workgroup, and the tail kernels
 // loop the same functions over whole single-patch levels in one workgroup.
 
-// One ghost-cell update of a SetGhost program. A boundary "2 - x" map is
-// affine; in the defect form only its linear part, -x, applies.
+// One ghost-cell update of a SetGhost program, offsets relative to q. A
+// boundary "2 - x" map is affine; in the defect form only its linear part,
+// -x, applies.
 template <class T, bool DEF>
-__device__ __forceinline__ void ghost_op(const Dev &d, T *q, const GhostOp &g) {
-  for (int v = 0; v < d.nrvars; v++) {
+__device__ __forceinline__ void ghost_op(int nrvars, T *q, const GhostOp &g) {
+  for (int v = 0; v < nrvars; v++) {
     const T *f = q + (long long)v * g.fvs;
     const T *c = q + (long long)v * g.cvs;
     T val;
@@ -224,14 +247,14 @@ __device__ __forceinline__ void ghost_op(const Dev &d, T *q, const GhostOp &g) {
 // RestrictSolution (afasroutines.f90:600-644) for coarse cell (i,j,k): qc =
 // R(q) and the parent's q under the child takes the same value.
 template <int ND, class T>
-__device__ __forceinline__ void restrict_cell(const Dev &d, const Fld<T> &F, const DPatch &P,
+__device__ __forceinline__ void restrict_cell(int nrvars, const Lvl<T> &C, const Lvl<T> &PA,
                                               int i, int j, int k) {
-  const DPatch &PA = d.P[P.parent];
-  View<T> a = qview(F.q, P), cq = qcview(F.qc, P), pq = qview(F.q, PA);
+  const DPatch &P = C.P;
+  View<T> a = qview(C), cq = qcview(C), pq = qview(PA);
   long long pc = pq.at(P.mb[0][0] + i - 1, P.mb[1][0] + j - 1,
                        ND == 3 ? P.mb[2][0] + k - 1 : 0);
   long long cc = cq.at(i, j, k);
-  for (int v = 0; v < d.nrvars; v++) {
+  for (int v = 0; v < nrvars; v++) {
     T r = restrict_view<ND, T>(a, v, i, j, k);
     cq.p[cc + v * cq.vs] = r;
     pq.p[pc + v * pq.vs] = r;
@@ -241,48 +264,47 @@ __device__ __forceinline__ void restrict_cell(const Dev &d, const Fld<T> &F, con
 // FillDownLevel (afasroutines.f90:241-310): parent X(mbounds) = R(child X)
 // for X = f (field 1) or q (field 2).
 template <int ND, class T>
-__device__ __forceinline__ void filldown_cell(const Dev &d, const Fld<T> &F, const DPatch &P,
+__device__ __forceinline__ void filldown_cell(int nrvars, const Lvl<T> &C, const Lvl<T> &PA,
                                               int i, int j, int k, int field) {
-  const DPatch &PA = d.P[P.parent];
+  const DPatch &P = C.P;
   int pi = P.mb[0][0] + i - 1, pj = P.mb[1][0] + j - 1,
       pk = ND == 3 ? P.mb[2][0] + k - 1 : 1;
   if (field == 1) {
-    for (int v = 0; v < d.nrvars; v++)
-      F.f[PA.f + fat(PA, pi, pj, pk) + v * PA.fvs] =
-          restrict_f<ND, T>(F.f + P.f + v * P.fvs, P, i, j, k);
+    for (int v = 0; v < nrvars; v++)
+      PA.f[fat(PA.P, pi, pj, pk) + v * PA.P.fvs] =
+          restrict_f<ND, T>(C.f + v * P.fvs, P, i, j, k);
   } else {
-    View<T> a = qview(F.q, P), pq = qview(F.q, PA);
+    View<T> a = qview(C), pq = qview(PA);
     long long pc = pq.at(pi, pj, ND == 3 ? pk : 0);
-    for (int v = 0; v < d.nrvars; v++)
-      pq.p[pc + v * pq.vs] = restrict_view<ND, T>(a, v, i, j, k);
+    for (int v = 0; v < nrvars; v++) pq.p[pc + v * pq.vs] = restrict_view<ND, T>(a, v, i, j, k);
   }
 }
 
 // GetCoarseGhostPoints (boundary.f90:1203-1269): ring cell (ci,cj,ck) of qc
 // <- parent (mb1-1+ci, mb2-1+cj, mb3-1+ck); (i,j,k) ranges over 0..cmx+1.
 template <int ND, class T>
-__device__ __forceinline__ void coarse_ghost_cell(const Dev &d, const Fld<T> &F,
-                                                  const DPatch &P, int i, int j, int k) {
+__device__ __forceinline__ void coarse_ghost_cell(int nrvars, const Lvl<T> &C,
+                                                  const Lvl<T> &PA, int i, int j, int k) {
+  const DPatch &P = C.P;
   bool ring = i == 0 || i == P.cmx + 1 || j == 0 || j == P.cmy + 1 ||
               (ND == 3 && (k == 0 || k == P.cmz + 1));
   if (!ring) return;
-  const DPatch &PA = d.P[P.parent];
-  View<T> cq = qcview(F.qc, P), pq = qview(F.q, PA);
+  View<T> cq = qcview(C), pq = qview(PA);
   long long pc = pq.at(P.mb[0][0] - 1 + i, P.mb[1][0] - 1 + j,
                        ND == 3 ? P.mb[2][0] - 1 + k : 0);
   long long cc = cq.at(i, j, k);
-  for (int v = 0; v < d.nrvars; v++) cq.p[cc + v * cq.vs] = pq.p[pc + v * pq.vs];
+  for (int v = 0; v < nrvars; v++) cq.p[cc + v * cq.vs] = pq.p[pc + v * pq.vs];
 }
 
 // CoarseLoadingFunction (afasroutines.f90:646-714): parent f(mbounds) =
 // R(f - A_h q) + A_2h(qc), the FAS right-hand side, with the fine residual
 // recomputed in registers for the 2^d children of coarse cell (i,j,k).
 template <int ND, class T, bool DEF>
-__device__ __forceinline__ void coarse_load_cell(const Dev &d, const Fld<T> &F,
-                                                 const DPatch &P, int i, int j, int k) {
-  const DPatch &PA = d.P[P.parent];
-  View<T> a = qview(F.q, P), cq = qcview(F.qc, P);
-  const T *f1 = F.f + P.f, *f2 = F.f + P.f + P.fvs;
+__device__ __forceinline__ void coarse_load_cell(const Problem &prm, const Lvl<T> &C,
+                                                 const Lvl<T> &PA, int i, int j, int k) {
+  const DPatch &P = C.P;
+  View<T> a = qview(C), cq = qcview(C);
+  const T *f1 = C.f, *f2 = C.f + P.fvs;
   constexpr int NC = ND == 2 ? 4 : 8;
   T r1[NC], r2[NC];
 #pragma unroll
@@ -290,15 +312,15 @@ __device__ __forceinline__ void coarse_load_cell(const Dev &d, const Fld<T> &F,
     int fi = 2 * i - (s & 1), fj = 2 * j - ((s >> 1) & 1);
     int fk = ND == 3 ? 2 * k - ((s >> 2) & 1) : 0;
     long long cidx = a.at(fi, fj, fk);
-    T base = DEF ? F.qb[P.q + cidx] : (T)0;
+    T base = DEF ? C.qb[cidx] : (T)0;
     T o1, o2;
-    op_at<ND, T, DEF>(d.prm, a, base, P.dx, cidx, o1, o2);
+    op_at<ND, T, DEF>(prm, a, base, P.dx, cidx, o1, o2);
     long long fo = fat(P, fi, fj, ND == 3 ? fk : 1);
     r1[s] = f1[fo] - o1;
     r2[s] = f2[fo] - o2;
   }
   T R1, R2;
-  if (ND == 2) {
+  if constexpr (ND == 2) {
     R1 = (T)0.25 * (((r1[0] + r1[1]) + r1[2]) + r1[3]);
     R2 = (T)0.25 * (((r2[0] + r2[1]) + r2[2]) + r2[3]);
   } else {
@@ -309,12 +331,12 @@ __device__ __forceinline__ void coarse_load_cell(const Dev &d, const Fld<T> &F,
       pk = ND == 3 ? P.mb[2][0] + k - 1 : 1;
   T base = (T)0;
   if (DEF)
-    base = F.qb[PA.q + pi + (long long)PA.px * (pj + (long long)PA.py * (ND == 3 ? pk : 0))];
+    base = PA.qb[pi + (long long)PA.P.px * (pj + (long long)PA.P.py * (ND == 3 ? pk : 0))];
   T c1, c2;
-  op_at<ND, T, DEF>(d.prm, cq, base, PA.dx, cq.at(i, j, k), c1, c2);
-  long long po = PA.f + fat(PA, pi, pj, pk);
-  F.f[po] = R1 + c1;
-  F.f[po + PA.fvs] = R2 + c2;
+  op_at<ND, T, DEF>(prm, cq, base, PA.P.dx, cq.at(i, j, k), c1, c2);
+  long long po = fat(PA.P, pi, pj, pk);
+  PA.f[po] = R1 + c1;
+  PA.f[po + PA.P.fvs] = R2 + c2;
 }
 
 // CorrectFine (afasroutines.f90:842-963, mbc = 1): q += P1(parent q - qc) at
@@ -322,15 +344,15 @@ __device__ __forceinline__ void coarse_load_cell(const Dev &d, const Fld<T> &F,
 // BiLinProlongationP1 / TriLinProlongationP1 (gridutilities.f90:137-175,
 // :474-529) evaluated in the reference's x, y(, z) pass order.
 template <int ND, class T>
-__device__ __forceinline__ void correct_cell(const Dev &d, const Fld<T> &F, const DPatch &P,
+__device__ __forceinline__ void correct_cell(int nrvars, const Lvl<T> &C, const Lvl<T> &PA,
                                              int i, int j, int k) {
-  const DPatch &PA = d.P[P.parent];
-  View<T> a = qview(F.q, P), cq = qcview(F.qc, P), pq = qview(F.q, PA);
+  const DPatch &P = C.P;
+  View<T> a = qview(C), cq = qcview(C), pq = qview(PA);
   int kx = i >> 1, ky = j >> 1, kz = ND == 3 ? k >> 1 : 0;
   bool ex = (i & 1) == 0, ey = (j & 1) == 0, ez = (k & 1) == 0;
   int o0 = P.mb[0][0] - 1, o1 = P.mb[1][0] - 1, o2 = ND == 3 ? P.mb[2][0] - 1 : 0;
   long long qc0 = a.at(i, j, k);
-  for (int v = 0; v < d.nrvars; v++) {
+  for (int v = 0; v < nrvars; v++) {
     const T *cp = cq.p + v * cq.vs, *pp = pq.p + v * pq.vs;
     auto corr = [&](int ci, int cj, int ck) {
       return pp[pq.at(o0 + ci, o1 + cj, ND == 3 ? o2 + ck : 0)] - cp[cq.at(ci, cj, ck)];
@@ -377,7 +399,8 @@ __device__ __forceinline__ void source_cell(const Dev &d, const DPatch &P, int i
 template <int ND>
 __device__ __forceinline__ void defect_setup_cell(const Dev &d, const DPatch &P, int i, int j,
                                                   int k) {
-  View<double> a = qview(d.q, P);
+  Lvl<double> L{d.q + P.q, nullptr, nullptr, nullptr, P};
+  View<double> a = qview(L);
   long long c = a.at(i, j, k);
   d.qb[P.q + c] = (float)d.q[P.q + c];
   d.e[P.q + c] = 0.f;
@@ -419,14 +442,14 @@ __device__ __forceinline__ bool in_box(int i, int j, int k, int ex, int ey, int 
 template <int ND, class T, bool DEF>
 __global__ __launch_bounds__(256) void k_relax(Dev d, const Tile *tiles, int ipass) {
   Tile t = tiles[blockIdx.x];
-  const DPatch &P = d.P[t.patch];
+  Lvl<T> L = level_ref<T>(d, t.patch);
   int i, j, k;
   tile_cell<ND>(t, i, j, k);
-  if (!in_box<ND>(i, j, k, P.mx, P.my, P.mz)) return;
+  if (!in_box<ND>(i, j, k, L.P.mx, L.P.my, L.P.mz)) return;
   // RelaxGrid's loop start 1+mod(j+ipass,2) (and +k in 3D) selects the
   // cells with i+j(+k)+ipass odd.
   if (((i + j + (ND == 3 ? k : 0) + ipass) & 1) == 0) return;
-  relax_cell<ND, T, DEF>(d, fields<T>(d), P, i, j, k);
+  relax_cell<ND, T, DEF>(d.prm, L, i, j, k);
 }
 
 // One SetGhost(level, ipass) program: every ghost-cell update is independent.
@@ -434,59 +457,59 @@ template <class T, bool DEF>
 __global__ __launch_bounds__(256) void k_ghost(Dev d, const GhostOp *ops, int n) {
   int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n) return;
-  ghost_op<T, DEF>(d, fields<T>(d).q, ops[e]);
+  ghost_op<T, DEF>(d.nrvars, fields<T>(d).q, ops[e]);
 }
 
 template <int ND, class T>
 __global__ __launch_bounds__(256) void k_restrict(Dev d, const Tile *tiles) {
   Tile t = tiles[blockIdx.x];
-  const DPatch &P = d.P[t.patch];
+  Lvl<T> C = level_ref<T>(d, t.patch);
   int i, j, k;
   tile_cell<ND>(t, i, j, k);
-  if (!in_box<ND>(i, j, k, P.cmx, P.cmy, P.cmz)) return;
-  restrict_cell<ND, T>(d, fields<T>(d), P, i, j, k);
+  if (!in_box<ND>(i, j, k, C.P.cmx, C.P.cmy, C.P.cmz)) return;
+  restrict_cell<ND, T>(d.nrvars, C, level_ref<T>(d, C.P.parent), i, j, k);
 }
 
 template <int ND, class T>
 __global__ __launch_bounds__(256) void k_filldown(Dev d, const Tile *tiles, int field) {
   Tile t = tiles[blockIdx.x];
-  const DPatch &P = d.P[t.patch];
+  Lvl<T> C = level_ref<T>(d, t.patch);
   int i, j, k;
   tile_cell<ND>(t, i, j, k);
-  if (!in_box<ND>(i, j, k, P.cmx, P.cmy, P.cmz)) return;
-  filldown_cell<ND, T>(d, fields<T>(d), P, i, j, k, field);
+  if (!in_box<ND>(i, j, k, C.P.cmx, C.P.cmy, C.P.cmz)) return;
+  filldown_cell<ND, T>(d.nrvars, C, level_ref<T>(d, C.P.parent), i, j, k, field);
 }
 
 // Tiles cover the ghost-extended coarse box 0..cmx+1.
 template <int ND, class T>
 __global__ __launch_bounds__(256) void k_coarse_ghost(Dev d, const Tile *tiles) {
   Tile t = tiles[blockIdx.x];
-  const DPatch &P = d.P[t.patch];
+  Lvl<T> C = level_ref<T>(d, t.patch);
   int i, j, k;
   tile_cell<ND>(t, i, j, k);
-  if (!in_box<ND>(i, j, k, P.cmx + 1, P.cmy + 1, P.cmz + 1)) return;
-  coarse_ghost_cell<ND, T>(d, fields<T>(d), P, i, j, k);
+  if (!in_box<ND>(i, j, k, C.P.cmx + 1, C.P.cmy + 1, C.P.cmz + 1)) return;
+  coarse_ghost_cell<ND, T>(d.nrvars, C, level_ref<T>(d, C.P.parent), i, j, k);
 }
 
 template <int ND, class T, bool DEF>
 __global__ __launch_bounds__(256) void k_coarse_load(Dev d, const Tile *tiles) {
   Tile t = tiles[blockIdx.x];
-  const DPatch &P = d.P[t.patch];
+  Lvl<T> C = level_ref<T>(d, t.patch);
   int i, j, k;
   tile_cell<ND>(t, i, j, k);
-  if (!in_box<ND>(i, j, k, P.cmx, P.cmy, P.cmz)) return;
-  coarse_load_cell<ND, T, DEF>(d, fields<T>(d), P, i, j, k);
+  if (!in_box<ND>(i, j, k, C.P.cmx, C.P.cmy, C.P.cmz)) return;
+  coarse_load_cell<ND, T, DEF>(d.prm, C, level_ref<T>(d, C.P.parent), i, j, k);
 }
 
 // Tiles cover the ghost-extended fine box 0..mx+1.
 template <int ND, class T>
 __global__ __launch_bounds__(256) void k_correct(Dev d, const Tile *tiles) {
   Tile t = tiles[blockIdx.x];
-  const DPatch &P = d.P[t.patch];
+  Lvl<T> C = level_ref<T>(d, t.patch);
   int i, j, k;
   tile_cell<ND>(t, i, j, k);
-  if (!in_box<ND>(i, j, k, P.mx + 1, P.my + 1, P.mz + 1)) return;
-  correct_cell<ND, T>(d, fields<T>(d), P, i, j, k);
+  if (!in_box<ND>(i, j, k, C.P.mx + 1, C.P.my + 1, C.P.mz + 1)) return;
+  correct_cell<ND, T>(d.nrvars, C, level_ref<T>(d, C.P.parent), i, j, k);
 }
 
 template <int ND>
@@ -523,8 +546,20 @@ __global__ __launch_bounds__(256) void k_apply_correction(Dev d, const Tile *til
 // ---- single-workgroup tail: the single-patch coarse levels ------------------
 // Below the root every level is one patch with physical boundaries only. The
 // tail kernels run the part of the V-cycle (or of FillDown) on levels
-// lo..hi of that chain in one workgroup, synchronising with barriers instead
-// of launches.
+// 0..nlev-1 of that chain (coarsest first) in one workgroup, synchronising
+// with barriers instead of launches. The V-cycle kernel keeps the arrays a
+// level places in LDS (TailLevel::lds, per field set) there for the whole
+// cycle: copied in at the start and out at the end; every other array is used
+// in place in the global pools. The ghost programs of these levels copy
+// within their own patch (patch-local offsets). Per visit of a level, each
+// thread loads its ghost updates and its cells' right-hand sides into
+// registers once, so the barrier-separated phases touch only LDS, registers
+// and the level arrays.
+extern __shared__ __align__(16) unsigned char tail_lds[];
+
+constexpr int TAIL_GR = 1;   // ghost updates per thread and pass held in registers
+constexpr int TAIL_FR = 2;   // cells per thread and colour whose f is held in registers
+
 template <int ND, class F>
 __device__ __forceinline__ void for_cells(int ex, int ey, int ez, int base, F f) {
   int n = ex * ey * ez;
@@ -534,75 +569,217 @@ __device__ __forceinline__ void for_cells(int ex, int ey, int ez, int base, F f)
   }
 }
 
-template <class T, bool DEF>
-__device__ void tail_ghost(const Dev &d, const TailLevel &L, int ipass) {
-  const GhostOp *ops = L.gp[ipass];
-  T *q = fields<T>(d).q;
-  for (int e = threadIdx.x; e < L.ngp[ipass]; e += blockDim.x) ghost_op<T, DEF>(d, q, ops[e]);
+template <class T>
+__device__ __forceinline__ Lvl<T> tail_ref(const Dev &d, const TailLevel &t, int set,
+                                           bool lds) {
+  Lvl<T> L = level_ref<T>(d, t.patch);
+  if (!lds) return L;
+  const int *o = t.lds[set];
+  if (o[0] >= 0) L.q = (T *)(tail_lds + o[0]);
+  if (o[1] >= 0) L.qc = (T *)(tail_lds + o[1]);
+  if (o[2] >= 0) L.f = (T *)(tail_lds + o[2]);
+  if (o[3] >= 0) L.qb = (const T *)(tail_lds + o[3]);
+  return L;
+}
+
+template <class T>
+__device__ __forceinline__ void copy_n(T *dst, const T *src, long long n) {
+  for (long long e = threadIdx.x; e < n; e += blockDim.x) dst[e] = src[e];
+}
+
+// Moves the LDS-resident arrays of levels 0..nlev-1 between LDS and the
+// global pools (in: q, qc, f, qb; out: q, qc, f). Ends with a barrier.
+template <class T>
+__device__ void tail_stage(const Dev &d, const TailLevel *lv, int nlev, int set, bool in,
+                           int nrvars) {
+  for (int l = 0; l < nlev; l++) {
+    Lvl<T> G = level_ref<T>(d, lv[l].patch);
+    const int *o = lv[l].lds[set];
+    const DPatch &P = G.P;
+    T *ptr[3] = {G.q, G.qc, G.f};
+    long long n[3] = {P.qvs * nrvars, P.cvs * nrvars, P.fvs * nrvars};
+    for (int a = 0; a < 3; a++) {
+      if (o[a] < 0) continue;
+      T *sm = (T *)(tail_lds + o[a]);
+      if (in) copy_n(sm, ptr[a], n[a]);
+      else copy_n(ptr[a], sm, n[a]);
+    }
+    if (in && o[3] >= 0) copy_n((T *)(tail_lds + o[3]), G.qb, P.qvs);
+  }
   __syncthreads();
 }
 
-template <int ND, class T, bool DEF>
-__device__ void tail_relax(const Dev &d, const TailLevel &L, int nsmooth) {
-  const DPatch &P = d.P[L.patch];
-  Fld<T> F = fields<T>(d);
-  for (int pass = 0; pass < nsmooth; pass++)
-    for (int color = 1; color <= 2; color++) {
-      for_cells<ND>(P.mx, P.my, ND == 3 ? P.mz : 1, 1, [&](int i, int j, int k) {
-        if (((i + j + (ND == 3 ? k : 0) + color) & 1) == 1)
-          relax_cell<ND, T, DEF>(d, F, P, i, j, k);
-      });
-      __syncthreads();
-      tail_ghost<T, DEF>(d, L, color);
+// One SetGhost program of a tail level, this thread's share of its updates
+// held in registers: destination, and source | nflip << 28.
+struct TailPass {
+  int n;
+  const GhostOp *ops;
+  int dst[TAIL_GR], srcf[TAIL_GR];
+};
+struct TailGhost {
+  TailPass p0, p1, p2;   // ipass 0, 1, 2
+};
+__device__ __forceinline__ void tail_pass_load(int n, const GhostOp *ops, TailPass &g) {
+  g.n = n;
+  g.ops = ops;
+#pragma unroll
+  for (int r = 0; r < TAIL_GR; r++) {
+    int e = threadIdx.x + r * blockDim.x;
+    g.dst[r] = g.srcf[r] = 0;
+    if (e < n) {
+      g.dst[r] = ops[e].dst;
+      g.srcf[r] = ops[e].f0 | (ops[e].nflip << 28);
     }
+  }
+}
+__device__ __forceinline__ void tail_ghost_load(const TailLevel &t, TailGhost &g) {
+  tail_pass_load(t.ngp[0], t.gp[0], g.p0);
+  tail_pass_load(t.ngp[1], t.gp[1], g.p1);
+  tail_pass_load(t.ngp[2], t.gp[2], g.p2);
+}
+// SetGhost on a tail level: patch-local copies with "2 - x" maps (the
+// ghost_op kind 0 update).
+template <class T, bool DEF>
+__device__ __forceinline__ void tail_ghost(int nrvars, const Lvl<T> &L, const TailPass &g) {
+  const long long vs = L.P.qvs;
+#pragma unroll
+  for (int r = 0; r < TAIL_GR; r++) {
+    if ((int)(threadIdx.x + r * blockDim.x) < g.n) {
+      int src = g.srcf[r] & 0x0fffffff, nflip = (unsigned)g.srcf[r] >> 28;
+      for (int v = 0; v < nrvars; v++) {
+        T val = L.q[src + v * vs];
+        for (int x = 0; x < nflip; x++) val = DEF ? -val : (T)2 - val;
+        L.q[g.dst[r] + v * vs] = val;
+      }
+    }
+  }
+  for (int e = threadIdx.x + TAIL_GR * blockDim.x; e < g.n; e += blockDim.x)
+    ghost_op<T, DEF>(nrvars, L.q, g.ops[e]);
+  __syncthreads();
 }
 
-// AFASVcycle(hi) (afasroutines.f90:488-521) restricted to levels lo..hi of
-// the single-patch chain; levels are indexed from lo.
+// Cell e of colour `color` on a level, colour-major: hx = ceil(mx/2) cells
+// per row, i = i0 + 2t with i0 chosen so that i + j (+ k) + color is odd (the
+// cells RelaxGrid updates in that pass). Returns false past the row's end.
+template <int ND>
+__device__ __forceinline__ bool color_cell(const DPatch &P, int color, int e, int &i, int &j,
+                                           int &k) {
+  int hx = (P.mx + 1) >> 1;
+  int t = e % hx;
+  j = (e / hx) % P.my + 1;
+  k = ND == 3 ? e / (hx * P.my) + 1 : 0;
+  i = 1 + ((j + k + color) & 1) + 2 * t;
+  return i <= P.mx;
+}
+
+// nsmooth red-black passes on a tail level. Threads walk the cells of each
+// colour; the first TAIL_FR cells per thread and colour keep their f in
+// registers for the whole visit.
+template <int ND, class T, bool DEF>
+__device__ void tail_relax(const Problem &prm, int nrvars, const Lvl<T> &L, const TailGhost &g,
+                           int nsmooth) {
+  const DPatch &P = L.P;
+  const int n = ((P.mx + 1) >> 1) * P.my * (ND == 3 ? P.mz : 1);
+  T f1[2][TAIL_FR], f2[2][TAIL_FR];
+#pragma unroll
+  for (int c = 0; c < 2; c++)
+#pragma unroll
+    for (int r = 0; r < TAIL_FR; r++) {
+      int e = threadIdx.x + r * blockDim.x, i, j, k;
+      f1[c][r] = f2[c][r] = (T)0;
+      if (e < n && color_cell<ND>(P, c + 1, e, i, j, k)) {
+        long long fo = fat(P, i, j, ND == 3 ? k : 1);
+        f1[c][r] = L.f[fo];
+        f2[c][r] = L.f[fo + P.fvs];
+      }
+    }
+  auto half = [&](int c) {
+#pragma unroll
+    for (int r = 0; r < TAIL_FR; r++) {
+      int e = threadIdx.x + r * blockDim.x, i, j, k;
+      if (e < n && color_cell<ND>(P, c + 1, e, i, j, k))
+        relax_cell_f<ND, T, DEF>(prm, L, i, j, k, f1[c][r], f2[c][r]);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    for (int e = threadIdx.x + TAIL_FR * blockDim.x; e < n; e += blockDim.x) {
+      int i, j, k;
+      if (color_cell<ND>(P, c + 1, e, i, j, k)) relax_cell<ND, T, DEF>(prm, L, i, j, k);
+    }
+    __syncthreads();
+  };
+  for (int pass = 0; pass < nsmooth; pass++) {
+    half(0);
+    tail_ghost<T, DEF>(nrvars, L, g.p1);
+    half(1);
+    tail_ghost<T, DEF>(nrvars, L, g.p2);
+  }
+}
+
+// AFASVcycle(top) (afasroutines.f90:488-521) on levels 0..nlev-1 of the
+// single-patch chain.
 template <int ND, class T, bool DEF>
 __global__ __launch_bounds__(1024) void k_tail_vcycle(Dev d, const TailLevel *lv, int nlev,
-                                                       int nsmooth) {
-  Fld<T> F = fields<T>(d);
+                                                       int nsmooth, int lds) {
+  const int set = DEF ? 1 : 0, nv = d.nrvars;
+  const Problem prm = d.prm;
+  if (lds) tail_stage<T>(d, lv, nlev, set, true, nv);
   for (int l = nlev - 1; l >= 1; l--) {
-    const DPatch &P = d.P[lv[l].patch];
+    Lvl<T> C = tail_ref<T>(d, lv[l], set, lds), PA = tail_ref<T>(d, lv[l - 1], set, lds);
+    TailGhost gc;
+    TailPass gp;
+    tail_ghost_load(lv[l], gc);
+    tail_pass_load(lv[l - 1].ngp[0], lv[l - 1].gp[0], gp);
+    const DPatch &P = C.P;
     int cz = ND == 3 ? P.cmz : 1;
-    tail_relax<ND, T, DEF>(d, lv[l], nsmooth);
+    tail_relax<ND, T, DEF>(prm, nv, C, gc, nsmooth);
     for_cells<ND>(P.cmx, P.cmy, cz, 1,
-                  [&](int i, int j, int k) { restrict_cell<ND, T>(d, F, P, i, j, k); });
+                  [&](int i, int j, int k) { restrict_cell<ND, T>(nv, C, PA, i, j, k); });
     __syncthreads();
-    tail_ghost<T, DEF>(d, lv[l - 1], 0);
+    tail_ghost<T, DEF>(nv, PA, gp);
     for_cells<ND>(P.cmx + 2, P.cmy + 2, ND == 3 ? P.cmz + 2 : 1, 0,
-                  [&](int i, int j, int k) { coarse_ghost_cell<ND, T>(d, F, P, i, j, k); });
+                  [&](int i, int j, int k) { coarse_ghost_cell<ND, T>(nv, C, PA, i, j, k); });
     __syncthreads();
-    for_cells<ND>(P.cmx, P.cmy, cz, 1, [&](int i, int j, int k) {
-      coarse_load_cell<ND, T, DEF>(d, F, P, i, j, k);
-    });
+    for_cells<ND>(P.cmx, P.cmy, cz, 1,
+                  [&](int i, int j, int k) { coarse_load_cell<ND, T, DEF>(prm, C, PA, i, j, k); });
     __syncthreads();
   }
-  tail_relax<ND, T, DEF>(d, lv[0], nsmooth);
+  {
+    Lvl<T> B = tail_ref<T>(d, lv[0], set, lds);
+    TailGhost gb;
+    tail_ghost_load(lv[0], gb);
+    tail_relax<ND, T, DEF>(prm, nv, B, gb, nsmooth);
+  }
   for (int l = 1; l < nlev; l++) {
-    const DPatch &P = d.P[lv[l].patch];
+    Lvl<T> C = tail_ref<T>(d, lv[l], set, lds), PA = tail_ref<T>(d, lv[l - 1], set, lds);
+    TailGhost gc;
+    tail_ghost_load(lv[l], gc);
+    const DPatch &P = C.P;
     for_cells<ND>(P.mx + 2, P.my + 2, ND == 3 ? P.mz + 2 : 1, 0,
-                  [&](int i, int j, int k) { correct_cell<ND, T>(d, F, P, i, j, k); });
+                  [&](int i, int j, int k) { correct_cell<ND, T>(nv, C, PA, i, j, k); });
     __syncthreads();
-    tail_ghost<T, DEF>(d, lv[l], 0);
-    tail_relax<ND, T, DEF>(d, lv[l], nsmooth);
+    tail_ghost<T, DEF>(nv, C, gc.p0);
+    tail_relax<ND, T, DEF>(prm, nv, C, gc, nsmooth);
   }
+  if (lds) tail_stage<T>(d, lv, nlev, set, false, nv);
 }
 
-// FillDown (afasroutines.f90:223-239) from level hi down to lo of the chain:
-// restriction into the parent, and for q a SetGhost(parent, 0).
+// FillDown (afasroutines.f90:223-239) from level nlev-1 down to 0 of the
+// chain, in the global pools: restriction into the parent, and for q a
+// SetGhost(parent, 0).
 template <int ND>
 __global__ __launch_bounds__(1024) void k_tail_filldown(Dev d, const TailLevel *lv, int nlev,
                                                          int field) {
-  Fld<double> F = fields<double>(d);
+  const int nv = d.nrvars;
   for (int l = nlev - 1; l >= 1; l--) {
-    const DPatch &P = d.P[lv[l].patch];
+    Lvl<double> C = level_ref<double>(d, lv[l].patch), PA = level_ref<double>(d, lv[l - 1].patch);
+    TailPass gp;
+    tail_pass_load(lv[l - 1].ngp[0], lv[l - 1].gp[0], gp);
+    const DPatch &P = C.P;
     for_cells<ND>(P.cmx, P.cmy, ND == 3 ? P.cmz : 1, 1, [&](int i, int j, int k) {
-      filldown_cell<ND, double>(d, F, P, i, j, k, field);
+      filldown_cell<ND, double>(nv, C, PA, i, j, k, field);
     });
     __syncthreads();
-    if (field == 2) tail_ghost<double, false>(d, lv[l - 1], 0);
+    if (field == 2) tail_ghost<double, false>(nv, PA, gp);
   }
 }
 
@@ -636,7 +813,8 @@ __global__ __launch_bounds__(256) void k_resnorm(Dev d, const Tile *tiles, doubl
     long long fo = P.f + fat(P, i, j, ND == 3 ? k : 1);
     double r1 = d.f[fo], r2 = d.f[fo + P.fvs];
     if (what == 0) {
-      View<double> a = qview(d.q, P);
+      Lvl<double> L{d.q + P.q, nullptr, nullptr, nullptr, P};
+      View<double> a = qview(L);
       double o1, o2;
       op_at<ND, double, false>(d.prm, a, 0.0, P.dx, a.at(i, j, k), o1, o2);
       r1 = r1 - o1;
@@ -799,17 +977,36 @@ hipError_t launch_copy(double *dst, const double *src, long long n, hipStream_t 
 }
 
 hipError_t launch_tail_vcycle(int nd, int prec, const Dev &d, const TailLevel *lv, int nlev,
-                              int nsmooth, int threads, hipStream_t s) {
+                              int nsmooth, int threads, size_t lds, hipStream_t s) {
   if (nlev <= 0) return hipSuccess;
-  BSAM_LAUNCH_ND_PREC(k_tail_vcycle, nd, prec, 1, threads, d, lv, nlev, nsmooth);
+  int use = lds > 0;
+  if (prec == PREC_SOLUTION_FP64) {
+    if (nd == 2) k_tail_vcycle<2, double, false><<<1, threads, lds, s>>>(d, lv, nlev, nsmooth, use);
+    else k_tail_vcycle<3, double, false><<<1, threads, lds, s>>>(d, lv, nlev, nsmooth, use);
+  } else {
+    if (nd == 2) k_tail_vcycle<2, float, true><<<1, threads, lds, s>>>(d, lv, nlev, nsmooth, use);
+    else k_tail_vcycle<3, float, true><<<1, threads, lds, s>>>(d, lv, nlev, nsmooth, use);
+  }
   return hipGetLastError();
 }
 
 hipError_t launch_tail_filldown(int nd, const Dev &d, const TailLevel *lv, int nlev, int field,
                                 int threads, hipStream_t s) {
   if (nlev <= 1) return hipSuccess;
-  BSAM_LAUNCH_ND(k_tail_filldown, nd, 1, threads, d, lv, nlev, field);
+  if (nd == 2) k_tail_filldown<2><<<1, threads, 0, s>>>(d, lv, nlev, field);
+  else k_tail_filldown<3><<<1, threads, 0, s>>>(d, lv, nlev, field);
   return hipGetLastError();
+}
+
+hipError_t tail_allow_lds(size_t lds) {
+  const void *fns[] = {
+      (const void *)k_tail_vcycle<2, double, false>, (const void *)k_tail_vcycle<3, double, false>,
+      (const void *)k_tail_vcycle<2, float, true>, (const void *)k_tail_vcycle<3, float, true>};
+  for (const void *f : fns) {
+    hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 int tile_x(int nd) { return nd == 2 ? TileShape<2>::X : TileShape<3>::X; }

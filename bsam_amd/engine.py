@@ -70,6 +70,7 @@ def lib():
         L.bsam_num_patches.argtypes = [vp]
         L.bsam_footprint_bytes.restype = C.c_longlong
         L.bsam_footprint_bytes.argtypes = [vp]
+        L.bsam_tune_get.argtypes = [vp, C.c_char_p, dp]
         L.bsam_field_len.restype = C.c_long
         L.bsam_field_len.argtypes = [vp, C.c_int, C.c_int]
         L.bsam_upload.argtypes = [vp, C.c_int, C.c_int, dp, C.c_long]
@@ -226,6 +227,13 @@ class Engine:
 
     def tune(self, key: str, value: float):
         self._ck(lib().bsam_tune(self.ctx, key.encode(), float(value)))
+
+    def tune_get(self, key: str) -> float:
+        """A tuning value, or a schedule property: tail_top, tail_lds_fp64,
+        tail_lds_fp32."""
+        v = C.c_double()
+        self._ck(lib().bsam_tune_get(self.ctx, key.encode(), C.byref(v)))
+        return v.value
 
     # -- kernel timing ----------------------------------------------------------
     PROF_KINDS = ("relax", "ghost", "restrict", "coarse_ghost", "coarse_load", "correct",

@@ -146,8 +146,16 @@ int bsam_sync(bsam_ctx *ctx);
  *   "graph"        1 (default): replay each multigrid iteration as a
  *                  captured hipGraph; 0: launch kernels one by one.
  *   "inner_cycles" FP32 V-cycles per defect-correction step (mixed
- *                  precision; 1..8, default 1). */
+ *                  precision; 1..8, default 1).
+ *   "tail_lds"     bytes of LDS the tail kernel may hold its levels' arrays
+ *                  in (-1, default: the device limit; 0: none); before the
+ *                  hierarchy. */
 int bsam_tune(bsam_ctx *ctx, const char *key, double value);
+/* Reads a tuning value back, or a property of the committed schedule:
+ * "tail_top" (highest level run by the tail kernel; below minlevel when
+ * none), "tail_lds_fp64" and "tail_lds_fp32" (dynamic LDS bytes of the tail
+ * kernels). No counterpart in the reference. */
+int bsam_tune_get(const bsam_ctx *ctx, const char *key, double *value);
 
 /* Kernel timing. Between bsam_profile_begin and bsam_profile_end, every
  * launch of a kind whose bit is set in kind_mask is bracketed by HIP events

@@ -32,11 +32,14 @@ LAYOUTS = {
 
 
 # engine schedules: per-level launches only; default (single-workgroup tail
-# for small coarse levels, hipGraph replay); every single-patch level in the tail
+# for small coarse levels with its arrays in LDS, hipGraph replay); every
+# single-patch level in the tail, in LDS as far as it fits or all in global
+# memory
 SCHEDULES = {
     "launch": {"tail_cells": 0, "graph": 0},
     "default": {},
     "all_tail": {"tail_cells": 1 << 30, "graph": 1},
+    "all_tail_global": {"tail_cells": 1 << 30, "tail_lds": 0, "graph": 0},
 }
 
 
@@ -145,4 +148,17 @@ def test_time_steps(name, sched):
         assert len(hist) == len(ref), (step, hist, ref)
         np.testing.assert_allclose(hist, ref, rtol=1e-12, atol=0)
         assert_same(eng, ora, names=("q", "qold"))
+    eng.close()
+
+
+def test_tail_schedule_places_levels_in_lds():
+    """The default schedule runs the small single-patch levels in the tail
+    kernel with their arrays in LDS; tail_lds = 0 keeps them in global memory."""
+    lay = layout_amr2d()
+    eng = Engine(lay, device=0, **KW)
+    assert eng.tune_get("tail_top") >= lay.minlevel
+    assert eng.tune_get("tail_lds_fp64") > 0
+    eng.close()
+    eng = Engine(lay, device=0, tune={"tail_lds": 0}, **KW)
+    assert eng.tune_get("tail_lds_fp64") == 0
     eng.close()
