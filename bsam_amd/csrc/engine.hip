@@ -492,7 +492,7 @@ static void op_set_ghost(bsam_ctx *c, int level, int ipass, int prec = PREC_SOLU
 }
 static void op_relax(bsam_ctx *c, int level, int color, int prec = PREC_SOLUTION_FP64) {
   LevelData &L = c->L(level);
-  ProfScope ps(c, BSAM_PROF_RELAX, (prec == PREC_SOLUTION_FP64 ? 24.0 : 12.0) * L.cells);
+  ProfScope ps(c, BSAM_PROF_RELAX, (prec == PREC_SOLUTION_FP64 ? 32.0 : 16.0) * L.cells);
   HIPCK(launch_relax(c->nd(), prec, c->dev(), L.d_int, (int)L.t_int.size(), color, c->stream));
 }
 static LevelData &child_level(bsam_ctx *c, int level) {
@@ -1061,6 +1061,48 @@ int bsam_profile_end(bsam_ctx *c) {
   c->prof.clear();
   c->ev_used = 0;
   c->prof_mask = 0;
+  API_END(c)
+}
+
+int bsam_profile_kernel(bsam_ctx *c, int kind, int reps, double *us_per_launch,
+                        double *bytes_per_launch) {
+  API_BEGIN(c)
+  c->need_device();
+  if (kind != BSAM_PROF_RELAX) throw Error("bsam_profile_kernel: only the relax kernel");
+  if (reps < 1) throw Error("reps must be positive");
+  int low = std::max(c->cfg.minlevel, c->tail_top + 1);
+  if (low > c->finest) throw Error("no level above the tail");
+  hipEvent_t e[2];
+  {
+    DeviceLock lock(g_device_lock);
+    HIPCK(hipEventCreate(&e[0]));
+    HIPCK(hipEventCreate(&e[1]));
+  }
+  double bytes = 0.0;
+  long long n = 0;
+  hipError_t err = hipEventRecord(e[0], c->stream);
+  for (int r = 0; r < reps && err == hipSuccess; r++)
+    for (int l = c->finest; l >= low && err == hipSuccess; l--) {
+      LevelData &L = c->L(l);
+      for (int color = 1; color <= 2 && err == hipSuccess; color++) {
+        err = launch_relax(c->nd(), PREC_SOLUTION_FP64, c->dev(), L.d_int, (int)L.t_int.size(),
+                           color, c->stream);
+        bytes += 32.0 * L.cells;
+        n++;
+      }
+    }
+  if (err == hipSuccess) err = hipEventRecord(e[1], c->stream);
+  if (err == hipSuccess) err = hipEventSynchronize(e[1]);
+  float ms = 0.f;
+  if (err == hipSuccess) err = hipEventElapsedTime(&ms, e[0], e[1]);
+  {
+    DeviceLock lock(g_device_lock);
+    (void)hipEventDestroy(e[0]);
+    (void)hipEventDestroy(e[1]);
+  }
+  HIPCK(err);
+  if (us_per_launch) *us_per_launch = 1e3 * ms / (double)n;
+  if (bytes_per_launch) *bytes_per_launch = bytes / (double)n;
   API_END(c)
 }
 

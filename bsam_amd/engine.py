@@ -71,6 +71,7 @@ def lib():
         L.bsam_footprint_bytes.restype = C.c_longlong
         L.bsam_footprint_bytes.argtypes = [vp]
         L.bsam_tune_get.argtypes = [vp, C.c_char_p, dp]
+        L.bsam_profile_kernel.argtypes = [vp, C.c_int, C.c_int, dp, dp]
         L.bsam_field_len.restype = C.c_long
         L.bsam_field_len.argtypes = [vp, C.c_int, C.c_int]
         L.bsam_upload.argtypes = [vp, C.c_int, C.c_int, dp, C.c_long]
@@ -227,6 +228,14 @@ class Engine:
 
     def tune(self, key: str, value: float):
         self._ck(lib().bsam_tune(self.ctx, key.encode(), float(value)))
+
+    def profile_kernel(self, kind: str = "relax", reps: int = 20):
+        """Back-to-back timing of one kernel over the V-cycle's launch mix:
+        (us per launch, algorithmic bytes per launch)."""
+        us, by = C.c_double(), C.c_double()
+        self._ck(lib().bsam_profile_kernel(self.ctx, self.PROF_KINDS.index(kind), reps,
+                                           C.byref(us), C.byref(by)))
+        return us.value, by.value
 
     def tune_get(self, key: str) -> float:
         """A tuning value, or a schedule property: tail_top, tail_lds_fp64,

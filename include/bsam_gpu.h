@@ -161,9 +161,11 @@ int bsam_tune_get(const bsam_ctx *ctx, const char *key, double *value);
  * launch of a kind whose bit is set in kind_mask is bracketed by HIP events
  * on the context's stream; bsam_profile_get then returns, per kind, the
  * number of launches, their summed duration (ms) and their summed
- * algorithmic HBM bytes (FP64, ghosts excluded: relax half-sweep 24 B,
+ * algorithmic HBM bytes (FP64, ghosts excluded: relax half-sweep 32 B -- q
+ * read 16, f of the updated colour 8, q of the updated colour written 8 --,
  * residual norm 32 B, restriction 24 B, coarse loading 40 B, correction
- * 40 B, fill-down 20 B, source 32 B per cell of the level operated on). */
+ * 40 B, fill-down 20 B, source 32 B per cell of the level operated on; FP32
+ * half of these). */
 enum {
   BSAM_PROF_RELAX = 0, BSAM_PROF_GHOST, BSAM_PROF_RESTRICT, BSAM_PROF_COARSE_GHOST,
   BSAM_PROF_COARSE_LOAD, BSAM_PROF_CORRECT, BSAM_PROF_FILLDOWN, BSAM_PROF_SOURCE,
@@ -173,6 +175,13 @@ int bsam_profile_begin(bsam_ctx *ctx, unsigned kind_mask);
 int bsam_profile_end(bsam_ctx *ctx);
 int bsam_profile_get(const bsam_ctx *ctx, int kind, long long *launches, double *ms,
                      double *bytes);
+/* Back-to-back timing of one kernel kind (BSAM_PROF_RELAX only): `reps`
+ * rounds of the V-cycle's launch mix of that kernel (both colours on every
+ * level above the tail) between two HIP events on the context's stream;
+ * returns the mean duration per launch (us) and the mean algorithmic bytes
+ * per launch. Changes the fields as relaxation does. */
+int bsam_profile_kernel(bsam_ctx *ctx, int kind, int reps, double *us_per_launch,
+                        double *bytes_per_launch);
 
 /* Introspection of the integer maps (valid on host-only contexts too).
  * Ghost-fill program of (level, ipass): number of ghost-cell updates, and
