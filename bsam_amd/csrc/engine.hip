@@ -54,8 +54,22 @@ struct LevelData {
   std::vector<GhostOp> gp[3];
   GhostOp *d_gp[3] = {nullptr, nullptr, nullptr};
   GhostOp *d_tgp[3] = {nullptr, nullptr, nullptr};  // patch-local copies (tail levels)
+  // SetGhost(ipass) fused into the next relaxation, per transition
+  // (ipass -> colour): 0: 0 -> 1, 1: 1 -> 2, 2: 2 -> 1
+  FusedGhost fused[3];
+  int *d_fstart[3] = {nullptr, nullptr, nullptr};
+  GhostOp *d_fops[3] = {nullptr, nullptr, nullptr};
   long long cells = 0;
 };
+
+// Transition index of SetGhost(ipass) followed by relaxation of `color`.
+int fused_index(int ipass, int color) {
+  if (ipass == 0 && color == 1) return 0;
+  if (ipass == 1 && color == 2) return 1;
+  if (ipass == 2 && color == 1) return 2;
+  return -1;
+}
+const int kFusedPass[3] = {0, 1, 2}, kFusedColor[3] = {1, 2, 1};
 
 // The SetGhost programs of a single-patch level with offsets relative to the
 // patch's q origin, for the tail kernels. Returns false unless every update is
@@ -127,6 +141,7 @@ struct bsam_ctx {
   long long tail_lds_cap = -1;      // LDS bytes the tail may use (-1: device limit)
   size_t tail_lds[2] = {0, 0};      // dynamic LDS of the tail kernels per field set
   int use_graph = 1;
+  int fuse_ghost = 1;               // apply SetGhost inside the next relaxation
   int tail_top = 0;                 // highest level run by the tail kernels
   TailLevel *d_tail = nullptr;      // levels minlevel..tail_top
   hipGraph_t graph = nullptr, graph_mixed = nullptr;
@@ -188,6 +203,8 @@ void bsam_ctx::free_device() {
     for (int p = 0; p < 3; p++) {
       F(l.d_gp[p]);
       F(l.d_tgp[p]);
+      F(l.d_fstart[p]);
+      F(l.d_fops[p]);
     }
   }
   if (h_state) (void)hipHostFree(h_state);
@@ -407,6 +424,12 @@ static void commit(bsam_ctx *c) {
       if (build_ghost_program(c->hp, L.patches, nd, l, ip, per, L.gp[ip], msg))
         throw Error("ghost program level " + std::to_string(l) + ": " + msg);
     }
+    for (int f = 0; f < 3; f++) {
+      std::string why;
+      L.fused[f] = plan_fused_ghost(c->hp, L.t_int, nd, tile_x(nd), tile_y(nd), tile_z(nd),
+                                    L.gp[kFusedPass[f]], L.gp[kFusedColor[f]], kFusedColor[f],
+                                    why);
+    }
   }
   if (c->host_only) return;
   // device allocation
@@ -443,6 +466,11 @@ static void commit(bsam_ctx *c) {
     L.d_coarse = dev_upload(L.t_coarse, s);
     L.d_cext = dev_upload(L.t_cext, s);
     for (int ip = 0; ip < 3; ip++) L.d_gp[ip] = dev_upload(L.gp[ip], s);
+    for (int f = 0; f < 3; f++)
+      if (L.fused[f].ok) {
+        L.d_fstart[f] = dev_upload(L.fused[f].start, s);
+        L.d_fops[f] = dev_upload(L.fused[f].ops, s);
+      }
     maxt = std::max(maxt, L.t_int.size());
   }
   // tail: the longest run of single-patch levels from minlevel whose cell
@@ -545,12 +573,29 @@ static void op_source(bsam_ctx *c, int full) {
     HIPCK(launch_source(c->nd(), c->dev(), L.d_int, (int)L.t_int.size(), full, c->stream));
   }
 }
-static void level_relax(bsam_ctx *c, int level, int prec = PREC_SOLUTION_FP64) {
+// The smoothing passes of AFASVcycle (afasroutines.f90:493-497, 512-516): per
+// pass, relax colour 1, SetGhost(1), relax colour 2, SetGhost(2). With
+// ghost0_first, a SetGhost(level, 0) precedes them. Each SetGhost is applied
+// inside the following relaxation launch where the level's fused plan allows.
+static void level_relax(bsam_ctx *c, int level, int prec = PREC_SOLUTION_FP64,
+                        bool ghost0_first = false) {
+  LevelData &L = c->L(level);
+  int pending = ghost0_first ? 0 : -1;   // SetGhost program not yet applied
   for (int pass = 1; pass <= c->cfg.nsmoothingpasses; pass++)
     for (int color = 1; color <= 2; color++) {
-      op_relax(c, level, color, prec);
-      op_set_ghost(c, level, color, prec);
+      int f = pending >= 0 ? fused_index(pending, color) : -1;
+      if (f >= 0 && c->fuse_ghost && L.fused[f].ok) {
+        ProfScope ps(c, BSAM_PROF_RELAX,
+                     (prec == PREC_SOLUTION_FP64 ? 32.0 : 16.0) * L.cells);
+        HIPCK(launch_relax_fused(c->nd(), prec, c->dev(), L.d_int, (int)L.t_int.size(), color,
+                                 L.d_fstart[f], L.d_fops[f], c->stream));
+      } else {
+        if (pending >= 0) op_set_ghost(c, level, pending, prec);
+        op_relax(c, level, color, prec);
+      }
+      pending = color;
     }
+  if (pending >= 0) op_set_ghost(c, level, pending, prec);
 }
 // AFASVcycle, afasroutines.f90:488-521
 static void op_vcycle(bsam_ctx *c, int level, int prec = PREC_SOLUTION_FP64) {
@@ -569,8 +614,7 @@ static void op_vcycle(bsam_ctx *c, int level, int prec = PREC_SOLUTION_FP64) {
     op_coarse_load(c, level, prec);
     op_vcycle(c, level - 1, prec);
     op_correct(c, level, prec);
-    op_set_ghost(c, level, 0, prec);
-    level_relax(c, level, prec);
+    level_relax(c, level, prec, true);
   }
 }
 static void reset_state(bsam_ctx *c) {
@@ -1010,6 +1054,9 @@ int bsam_tune(bsam_ctx *c, const char *key, double value) {
     c->drop_graphs();
   } else if (k == "graph") {
     c->use_graph = value != 0.0;
+  } else if (k == "fuse_ghost") {
+    c->fuse_ghost = value != 0.0;
+    c->drop_graphs();
   } else {
     throw Error("unknown tuning key " + k);
   }
@@ -1026,6 +1073,7 @@ int bsam_tune_get(const bsam_ctx *cc, const char *key, double *value) {
   else if (k == "tail_lds") *value = (double)c->tail_lds_cap;
   else if (k == "inner_cycles") *value = c->inner_cycles;
   else if (k == "graph") *value = c->use_graph;
+  else if (k == "fuse_ghost") *value = c->fuse_ghost;
   else if (k == "tail_top") *value = c->tail_top;
   else if (k == "tail_lds_fp64") *value = (double)c->tail_lds[0];
   else if (k == "tail_lds_fp32") *value = (double)c->tail_lds[1];
@@ -1146,6 +1194,28 @@ int bsam_ghost_program_apply_host(const bsam_ctx *cc, int level, int ipass,
   if (!c->committed) throw Error("hierarchy not committed");
   if (ipass < 0 || ipass > 2) throw Error("bad ipass");
   apply_ghost_program_host(c->hp, c->L(level).gp[ipass], c->cfg.nrvars, q_patches);
+  API_END(c)
+}
+
+int bsam_fused_ghost_size(const bsam_ctx *cc, int level, int ipass, int color) {
+  bsam_ctx *c = const_cast<bsam_ctx *>(cc);
+  if (!c || !c->committed) return -1;
+  if (level < c->cfg.minlevel || level > c->finest) return -1;
+  int f = fused_index(ipass, color);
+  if (f < 0 || !c->L(level).fused[f].ok) return -1;
+  return c->L(level).fused[f].kept;
+}
+
+int bsam_fused_ghost_apply_host(const bsam_ctx *cc, int level, int ipass, int color,
+                                double *const *q_patches) {
+  bsam_ctx *c = const_cast<bsam_ctx *>(cc);
+  API_BEGIN(c)
+  if (!c->committed) throw Error("hierarchy not committed");
+  int f = fused_index(ipass, color);
+  if (f < 0) throw Error("no fused form of this ipass and colour");
+  const FusedGhost &g = c->L(level).fused[f];
+  if (!g.ok) throw Error("this transition is not fused on this level");
+  apply_ghost_program_host(c->hp, g.ops, c->cfg.nrvars, q_patches);
   API_END(c)
 }
 

@@ -20,6 +20,7 @@
 #include <cstring>
 #include <initializer_list>
 #include <map>
+#include <unordered_map>
 #include <unordered_set>
 
 namespace bsam {
@@ -698,6 +699,93 @@ int build_ghost_program(const std::vector<HPatch> &hp, const std::vector<int> &p
     }
   }
   return 0;
+}
+
+FusedGhost plan_fused_ghost(const std::vector<HPatch> &hp, const std::vector<Tile> &tiles,
+                            int nd, int tx, int ty, int tz,
+                            const std::vector<GhostOp> &prog, const std::vector<GhostOp> &next,
+                            int color, std::string &why) {
+  FusedGhost out;
+  // patches by q offset (pools are laid out in patch order)
+  std::vector<long long> qoffs(hp.size());
+  for (size_t n = 0; n < hp.size(); n++) qoffs[n] = hp[n].qoff;
+  auto locate = [&](long long off, int &p, int &i, int &j, int &k) {
+    auto it = std::upper_bound(qoffs.begin(), qoffs.end(), off);
+    p = (int)(it - qoffs.begin()) - 1;
+    if (p < 0 || off >= hp[p].qoff + hp[p].qvs) {
+      p = -1;
+      return;
+    }
+    const HPatch &h = hp[p];
+    long long r = off - h.qoff;
+    i = (int)(r % h.px);
+    j = (int)((r / h.px) % h.py);
+    k = (int)(r / (h.px * h.py));
+  };
+  // first interior tile of every patch; tiles of a patch are k, j, i nested
+  std::unordered_map<int, int> first;
+  for (size_t t = 0; t < tiles.size(); t++)
+    if (!first.count(tiles[t].patch)) first[tiles[t].patch] = (int)t;
+  auto inside = [&](int p, int i, int j, int k) {
+    const HPatch &h = hp[p];
+    return i >= 1 && i <= h.mx[0] && j >= 1 && j <= h.mx[1] &&
+           (nd == 2 || (k >= 1 && k <= h.mx[2]));
+  };
+  auto tile_of = [&](int p, int i, int j, int k) {
+    const HPatch &h = hp[p];
+    int nx = (h.mx[0] + tx - 1) / tx, ny = (h.mx[1] + ty - 1) / ty;
+    int a = (i - 1) / tx, b = (j - 1) / ty, cz = nd == 3 ? (k - 1) / tz : 0;
+    return first.at(p) + (cz * ny + b) * nx + a;
+  };
+  auto relaxed = [&](int i, int j, int k) { return ((i + j + (nd == 3 ? k : 0) + color) & 1) == 1; };
+  std::unordered_set<int> next_dst;
+  for (const GhostOp &g : next) next_dst.insert(g.dst);
+  std::vector<std::vector<GhostOp>> per(tiles.size());
+  for (const GhostOp &g : prog) {
+    int p, i, j, k;
+    locate(g.dst, p, i, j, k);
+    if (p < 0 || !first.count(p)) {
+      why = "ghost destination outside the level's patches";
+      return out;
+    }
+    const HPatch &h = hp[p];
+    int m[3] = {h.mx[0], h.mx[1], nd == 3 ? h.mx[2] : 1};
+    int c[3] = {i, j, nd == 3 ? k : 1};
+    int outside = 0;
+    for (int d = 0; d < 3; d++) {
+      if (c[d] < 1 || c[d] > m[d]) outside++;
+      c[d] = std::min(std::max(c[d], 1), m[d]);
+    }
+    // the interior cell a face ghost borders is the only reader of the ghost
+    bool read = outside == 1 && relaxed(c[0], c[1], nd == 3 ? c[2] : 0);
+    if (!read && next_dst.count(g.dst)) continue;
+    int owner = tile_of(p, c[0], c[1], c[2]);
+    long long srcs[2] = {g.f0, g.f1};
+    int nsrc = g.kind == 0 ? 1 : 2;
+    for (int s2 = 0; s2 < nsrc; s2++) {
+      int sp, si, sj, sk;
+      locate(srcs[s2], sp, si, sj, sk);
+      if (sp < 0) {
+        why = "ghost source outside the q pool";
+        return out;
+      }
+      if (!first.count(sp) || !inside(sp, si, sj, sk)) continue;   // not relaxed here
+      if (!relaxed(si, sj, nd == 3 ? sk : 0)) continue;
+      if (tile_of(sp, si, sj, sk) != owner) {
+        why = "an update reads a cell relaxed by another tile";
+        return out;
+      }
+    }
+    per[owner].push_back(g);
+  }
+  out.start.assign(tiles.size() + 1, 0);
+  for (size_t t = 0; t < tiles.size(); t++) {
+    out.start[t + 1] = out.start[t] + (int)per[t].size();
+    out.ops.insert(out.ops.end(), per[t].begin(), per[t].end());
+  }
+  out.kept = (int)out.ops.size();
+  out.ok = true;
+  return out;
 }
 
 void locate_q_cell(const std::vector<HPatch> &hp, long long off, int &p, int &i, int &j,

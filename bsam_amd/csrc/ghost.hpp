@@ -36,6 +36,27 @@ int build_ghost_program(const std::vector<HPatch> &hp, const std::vector<int> &p
                         int ndims, int level, int ipass, const Periodic &per,
                         std::vector<GhostOp> &out, std::string &msg);
 
+// The ghost-cell updates of SetGhost(level, ipass) that a relaxation kernel
+// for colour `color` applies before relaxing, grouped by the tile that
+// applies them (ops[start[t] .. start[t+1]) for tile t of the level's
+// interior tile list). Updates whose destination the relaxation does not read
+// and that the next program, SetGhost(level, color), rewrites are dropped
+// (their value is never observed). Built only if every kept update reads no
+// cell of colour `color` outside its own tile, so the tiles of one launch
+// never race; `why` says otherwise.
+struct FusedGhost {
+  bool ok = false;
+  int kept = 0;
+  std::vector<int> start;
+  std::vector<GhostOp> ops;
+};
+// tiles: interior tiles of the level (origin i0/j0/k0 = 1 + multiples of the
+// tile extents tx, ty, tz; k0 = 0 in 2D), in patch order, k, j, i nested.
+FusedGhost plan_fused_ghost(const std::vector<HPatch> &hp, const std::vector<Tile> &tiles,
+                            int ndims, int tx, int ty, int tz,
+                            const std::vector<GhostOp> &prog, const std::vector<GhostOp> &next,
+                            int color, std::string &why);
+
 // Patch and (i,j,k) (lower bound 0; k = 0 in 2D) of a q-pool offset.
 void locate_q_cell(const std::vector<HPatch> &hp, long long off, int &p, int &i, int &j,
                    int &k);

@@ -452,6 +452,29 @@ __global__ __launch_bounds__(256) void k_relax(Dev d, const Tile *tiles, int ipa
   relax_cell<ND, T, DEF>(d.prm, L, i, j, k);
 }
 
+// SetGhost(level, ipass) followed by the relaxation of colour `color`, in one
+// launch: each workgroup first applies the ghost updates its tile's cells read
+// (ops[start[b] .. start[b+1]), chosen on the host so that no update reads a
+// cell another tile relaxes), then relaxes its tile.
+template <int ND, class T, bool DEF>
+__global__ __launch_bounds__(256) void k_relax_fused(Dev d, const Tile *tiles, int color,
+                                                      const int *start, const GhostOp *ops) {
+  int s0 = start[blockIdx.x], s1 = start[blockIdx.x + 1];
+  if (s1 > s0) {
+    T *q = fields<T>(d).q;
+    for (int e = s0 + (int)threadIdx.x; e < s1; e += blockDim.x)
+      ghost_op<T, DEF>(d.nrvars, q, ops[e]);
+    __syncthreads();
+  }
+  Tile t = tiles[blockIdx.x];
+  Lvl<T> L = level_ref<T>(d, t.patch);
+  int i, j, k;
+  tile_cell<ND>(t, i, j, k);
+  if (!in_box<ND>(i, j, k, L.P.mx, L.P.my, L.P.mz)) return;
+  if (((i + j + (ND == 3 ? k : 0) + color) & 1) == 0) return;
+  relax_cell<ND, T, DEF>(d.prm, L, i, j, k);
+}
+
 // One SetGhost(level, ipass) program: every ghost-cell update is independent.
 template <class T, bool DEF>
 __global__ __launch_bounds__(256) void k_ghost(Dev d, const GhostOp *ops, int n) {
@@ -887,6 +910,13 @@ hipError_t launch_relax(int nd, int prec, const Dev &d, const Tile *tiles, int n
                         hipStream_t s) {
   if (nt <= 0) return hipSuccess;
   BSAM_LAUNCH_ND_PREC(k_relax, nd, prec, grid1(nt), 256, d, tiles, ipass);
+  return hipGetLastError();
+}
+
+hipError_t launch_relax_fused(int nd, int prec, const Dev &d, const Tile *tiles, int nt,
+                              int color, const int *start, const GhostOp *ops, hipStream_t s) {
+  if (nt <= 0) return hipSuccess;
+  BSAM_LAUNCH_ND_PREC(k_relax_fused, nd, prec, grid1(nt), 256, d, tiles, color, start, ops);
   return hipGetLastError();
 }
 

@@ -8,6 +8,10 @@ namespace bsam {
 
 hipError_t launch_relax(int nd, int prec, const Dev &d, const Tile *tiles, int nt, int ipass,
                         hipStream_t s);
+// A SetGhost program's updates, grouped per tile (start has nt+1 entries),
+// then the relaxation of one colour, in one launch.
+hipError_t launch_relax_fused(int nd, int prec, const Dev &d, const Tile *tiles, int nt,
+                              int color, const int *start, const GhostOp *ops, hipStream_t s);
 hipError_t launch_ghost(int prec, const Dev &d, const GhostOp *ops, int n, hipStream_t s);
 hipError_t launch_restrict(int nd, int prec, const Dev &d, const Tile *tiles, int nt,
                            hipStream_t s);

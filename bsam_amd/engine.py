@@ -95,6 +95,9 @@ def lib():
         L.bsam_ghost_program_dst.argtypes = [vp, C.c_int, C.c_int, ip, C.c_int]
         L.bsam_ghost_program_apply_host.argtypes = [vp, C.c_int, C.c_int,
                                                     C.POINTER(C.POINTER(C.c_double))]
+        L.bsam_fused_ghost_size.argtypes = [vp, C.c_int, C.c_int, C.c_int]
+        L.bsam_fused_ghost_apply_host.argtypes = [vp, C.c_int, C.c_int, C.c_int,
+                                                  C.POINTER(C.POINTER(C.c_double))]
         _lib = L
     return _lib
 
@@ -276,6 +279,19 @@ class Engine:
         self._ck(L.bsam_ghost_program_dst(self.ctx, level, ipass,
                                           out.ctypes.data_as(C.POINTER(C.c_int)), n))
         return out[:4 * n].reshape(n, 4)
+
+    def fused_ghost_size(self, level: int, ipass: int, color: int) -> int:
+        """Updates of SetGhost(level, ipass) applied inside the relaxation of
+        `color`; -1 when that transition is not fused."""
+        return int(lib().bsam_fused_ghost_size(self.ctx, level, ipass, color))
+
+    def fused_ghost_apply_host(self, level: int, ipass: int, color: int,
+                               qs: List[np.ndarray]):
+        ptrs = (C.POINTER(C.c_double) * len(qs))()
+        for n, a in enumerate(qs):
+            assert a.flags.f_contiguous and a.dtype == np.float64
+            ptrs[n] = _dptr(a)
+        self._ck(lib().bsam_fused_ghost_apply_host(self.ctx, level, ipass, color, ptrs))
 
     def ghost_apply_host(self, level: int, ipass: int, qs: List[np.ndarray]):
         """Apply SetGhost(level, ipass)'s program to host q arrays in place."""

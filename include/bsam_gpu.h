@@ -147,6 +147,8 @@ int bsam_sync(bsam_ctx *ctx);
  *                  captured hipGraph; 0: launch kernels one by one.
  *   "inner_cycles" FP32 V-cycles per defect-correction step (mixed
  *                  precision; 1..8, default 1).
+ *   "fuse_ghost"   1 (default): apply each SetGhost inside the following
+ *                  relaxation launch where the level allows; 0: separately.
  *   "tail_lds"     bytes of LDS the tail kernel may hold its levels' arrays
  *                  in (-1, default: the device limit; 0: none); before the
  *                  hierarchy. */
@@ -189,6 +191,14 @@ int bsam_profile_kernel(bsam_ctx *ctx, int kind, int reps, double *us_per_launch
 int bsam_ghost_program_size(const bsam_ctx *ctx, int level, int ipass);
 int bsam_ghost_program_dst(const bsam_ctx *ctx, int level, int ipass, int *out,
                            int maxn);
+/* The updates of SetGhost(level, ipass) that the engine applies inside the
+ * following relaxation of `color` ((ipass, color) = (0, 1), (1, 2) or (2, 1)):
+ * their number (-1 when that transition runs as two launches), and their
+ * application to host arrays. Updates the relaxation does not read and the
+ * next SetGhost(level, color) rewrites are left out. */
+int bsam_fused_ghost_size(const bsam_ctx *ctx, int level, int ipass, int color);
+int bsam_fused_ghost_apply_host(const bsam_ctx *ctx, int level, int ipass, int color,
+                                double *const *q_patches);
 /* Apply a ghost-fill program on the host to per-patch q arrays (reference
  * layout, one pointer per patch): used to check the integer maps on a CPU. */
 int bsam_ghost_program_apply_host(const bsam_ctx *ctx, int level, int ipass,

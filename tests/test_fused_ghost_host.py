@@ -1,0 +1,76 @@
+"""SetGhost fused into the following relaxation, checked on the CPU.
+
+The engine applies a SetGhost program inside the next relaxation launch:
+each tile first applies the ghost updates its cells read, and updates whose
+value nobody reads before the next SetGhost rewrites them are left out. Here
+the fused sequence is replayed on the CPU -- kept updates applied to the
+oracle's arrays, then the oracle's relaxation -- and must leave every array
+bit-identical to the reference sequence (relax, SetGhost, relax, SetGhost,
+... with the oracle's full SetGhost), for every level, for smoothing with and
+without a leading SetGhost(level, 0).
+"""
+
+import numpy as np
+import pytest
+
+import orc
+from bsam_amd.engine import Engine
+from test_ghost_program_host import LAYOUTS
+
+KW = dict(dt=1e-3, eps=0.03)
+
+
+def run(lay, eng, level, ghost0_first, fused, seed, nsmooth=2):
+    ora = orc.Oracle(lay, nsmoothingpasses=nsmooth, **KW)
+    rng = np.random.default_rng(seed)
+    for n, p in enumerate(ora.patches):
+        for name in ("q", "f"):
+            ora.field(n, name)[...] = rng.uniform(-1, 1, lay.field_shape(p, name))
+    qs = [ora.field(n, "q") for n in range(len(ora.patches))]
+    pending = 0 if ghost0_first else -1
+    nfused = 0
+    for _ in range(nsmooth):
+        for color in (1, 2):
+            if pending >= 0 and fused and eng.fused_ghost_size(level, pending, color) >= 0:
+                eng.fused_ghost_apply_host(level, pending, color, qs)
+                nfused += 1
+            elif pending >= 0:
+                ora.set_ghost(level, pending)
+            ora.relax(level, color)
+            pending = color
+    ora.set_ghost(level, pending)
+    return [ora.field(n, "q").copy() for n in range(len(ora.patches))], nfused
+
+
+@pytest.mark.parametrize("name", sorted(LAYOUTS))
+def test_fused_smoothing_matches_reference_sequence(name):
+    lay = LAYOUTS[name]()
+    eng = Engine(lay, device=-1, **KW)
+    total = 0
+    for level in range(lay.minlevel, lay.finestlevel + 1):
+        for g0 in (False, True):
+            seed = 7 * (level - lay.minlevel) + g0
+            want, _ = run(lay, eng, level, g0, False, seed)
+            got, nf = run(lay, eng, level, g0, True, seed)
+            total += nf
+            for n, (a, b) in enumerate(zip(got, want)):
+                assert np.array_equal(a, b), (level, g0, n)
+    assert total > 0   # the fused path was exercised
+
+
+def test_fused_plans_cover_the_2d_levels():
+    """On the 2D layouts every transition of every level fuses, and each
+    level drops part of its programs. (On refined 3D levels the face
+    interpolation reads tangentially diagonal fine cells that can sit in the
+    neighbouring tile, so those transitions stay two launches.)"""
+    for name in ("amr2d_tiles", "amr2d", "periodic_xy", "amr2d_user_bc"):
+        lay = LAYOUTS[name]()
+        eng = Engine(lay, device=-1, **KW)
+        for level in range(lay.minlevel, lay.finestlevel + 1):
+            dropped = 0
+            for ipass, color in ((0, 1), (1, 2), (2, 1)):
+                n = eng.fused_ghost_size(level, ipass, color)
+                full = len(eng.ghost_program_dst(level, ipass))
+                assert 0 <= n <= full, (name, level, ipass, color, n, full)
+                dropped += full - n
+            assert dropped > 0, (name, level)

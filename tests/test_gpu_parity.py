@@ -32,14 +32,16 @@ LAYOUTS = {
 
 
 # engine schedules: per-level launches only; default (single-workgroup tail
-# for small coarse levels with its arrays in LDS, hipGraph replay); every
-# single-patch level in the tail, in LDS as far as it fits or all in global
-# memory
+# for small coarse levels with its arrays in LDS, SetGhost applied inside the
+# next relaxation launch, hipGraph replay); every single-patch level in the
+# tail, in LDS as far as it fits or all in global memory; SetGhost and
+# relaxation as separate launches
 SCHEDULES = {
     "launch": {"tail_cells": 0, "graph": 0},
     "default": {},
     "all_tail": {"tail_cells": 1 << 30, "graph": 1},
     "all_tail_global": {"tail_cells": 1 << 30, "tail_lds": 0, "graph": 0},
+    "unfused": {"fuse_ghost": 0, "graph": 0},
 }
 
 
