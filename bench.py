@@ -28,6 +28,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -81,6 +83,34 @@ def cpu_baseline(layout, q0, params, budget_s):
                       f"t=0, single-threaded FP64 C oracle, {dt:.1f} s"}
 
 
+WORKLOADS = {
+    "c2": ("2D AMR Cahn-Hilliard, root 256^2 + 2 refined levels (ratio 2), "
+           "FAS V(2,2) to residual 1e-8, one time step per step"),
+    "c3": ("2D AMR Cahn-Hilliard, root 512^2 + 3 refined levels, ~2000 uneven patches "
+           "8^2..128^2, FAS V(2,2) to residual 1e-8, one time step per step"),
+    "c4": ("3D AMR Cahn-Hilliard, root 128^3 + 2 refined levels of 16^3/32^3 blocks, "
+           "FAS V(2,2) to residual 1e-8, one time step per step"),
+}
+
+
+def load_workload(name):
+    from bsam_amd import workloads as W
+    if name == "c2":
+        lay, q0 = W.config_c2()
+        return lay, q0, dict(W.C2_PARAMS)
+    if name == "c3":
+        lay, q0 = W.config_c3()
+        return lay, q0, dict(W.C3_PARAMS)
+    lay, q0 = W.config_c4()
+    return lay, q0, dict(W.C4_PARAMS)
+
+
+def root_mass(layout, q_root):
+    p = layout.patches[0]
+    inner = q_root[(slice(1, -1),) * layout.ndims + ((0,) if layout.ndims == 2 else ()) + (0,)]
+    return float(inner.sum()) * p.dx ** layout.ndims
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -88,6 +118,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--cpu-budget", type=float, default=15.0,
                     help="seconds of CPU-baseline work (0 to skip)")
+    ap.add_argument("--config", choices=sorted(WORKLOADS), default="c2",
+                    help="workload (c2 is the headline configuration)")
+    ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64")
+    ap.add_argument("--check", action="store_true",
+                    help="mixed precision: also run the FP64 path and report the "
+                         "difference (max |phi|, |mu| and root mass) after the steps")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the solver path here is single-GPU "
@@ -95,16 +131,22 @@ def main():
         sys.exit(2)
 
     from bsam_amd import engine as E
-    from bsam_amd.workloads import C2_PARAMS, config_c2
 
-    layout, q0 = config_c2()
-    params = dict(C2_PARAMS)
-    eng = E.Engine(layout, dt=params["dt"], eps=params["eps"], well=params["well"],
-                   nsmoothingpasses=params["nsmoothingpasses"], maxvcycles=50,
-                   qerrortol=params["qerrortol"], device=0)
-    for n, p in enumerate(eng.patches):
-        eng.upload(n, "q", q0[p.id])
+    layout, q0, params = load_workload(args.config)
+    prec = E.PREC_MIXED if args.precision == "mixed" else E.PREC_FP64
+
+    def make_engine(precision):
+        eng = E.Engine(layout, dt=params["dt"], eps=params["eps"], well=params["well"],
+                       nsmoothingpasses=params["nsmoothingpasses"], maxvcycles=50,
+                       qerrortol=params["qerrortol"], precision=precision, device=0)
+        for n, p in enumerate(eng.patches):
+            eng.upload(n, "q", q0[p.id])
+        return eng
+
+    eng = make_engine(prec)
     per_v = layout.relax_cell_updates_per_vcycle(params["nsmoothingpasses"])
+    if prec == E.PREC_MIXED:
+        per_v *= eng.tune_get("inner_cycles")
 
     for _ in range(args.warmup):
         eng.time_step()
@@ -121,13 +163,33 @@ def main():
     eng.sync()
     elapsed = time.perf_counter() - t0
 
+    check = None
+    if args.check and prec == E.PREC_MIXED:
+        ref = make_engine(E.PREC_FP64)
+        for _ in range(args.warmup + args.steps):
+            ref.time_step()
+        worst = 0.0
+        for n, p in enumerate(layout.ordered()):
+            if p.level < 0:
+                continue
+            a, b = eng.download(n, "q"), ref.download(n, "q")
+            inner = (slice(1, -1),) * layout.ndims
+            worst = max(worst, float(np.abs(a[inner] - b[inner]).max()))
+        root = layout.index_map()[0]
+        check = {"vs": "FP64 GPU path (bit-identical to the FP64 oracle)",
+                 "max_abs_field_diff": worst,
+                 "root_mass_diff": abs(root_mass(layout, eng.download(root, "q")) -
+                                       root_mass(layout, ref.download(root, "q"))),
+                 "steps": args.warmup + args.steps}
+        ref.close()
+
     # Roofline of the relaxation kernel: the V-cycle's relaxation launch mix,
     # back to back between two HIP events on the engine's stream.
     reps = 50
     avg_us, bytes_per_launch = eng.profile_kernel("relax", reps)
     avg_s = avg_us * 1e-6
     achieved = bytes_per_launch / avg_s / 1e9
-    traffic = measured_traffic()
+    traffic = measured_traffic() if args.config == "c2" else None
 
     value = cycles * per_v / elapsed
     levels = layout.finestlevel
@@ -142,27 +204,29 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if prec == E.PREC_FP64 else "f32 smoothing in f64 defect correction",
         "data": "synthetic",
         "config": {
-            "workload": "2D AMR Cahn-Hilliard, root 256^2 + 2 refined levels (ratio 2), "
-                        "FP64 FAS V(2,2) to residual 1e-8, one time step per step",
-            "root": 256, "refined_levels": levels,
+            "workload": WORKLOADS[args.config] + (
+                "" if prec == E.PREC_FP64 else " (mixed precision)"),
             "patches": len(layout.ordered()),
             "cells": sum(layout.cells_on_level(l) for l in range(0, levels + 1)),
+            "refined_levels": levels,
             "cell_updates_per_vcycle": per_v,
             "vcycles": cycles, "all_converged": conv_all,
             "solves_per_s": args.steps / elapsed,
             "p50_ms": 1e3 * percentile(lat, 50), "p99_ms": 1e3 * percentile(lat, 99),
         },
         "roofline": {
-            "bound": "hbm", "kernel": "k_relax<2> (red-black half-sweep)",
+            "bound": "hbm", "kernel": f"k_relax<{layout.ndims}> (red-black half-sweep, FP64)",
             "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
             "avg_launch_us": avg_us, "bytes_per_launch": bytes_per_launch,
         },
         "cpu_baseline": None,
     }
+    if check:
+        out["check"] = check
     eng.close()
     if args.cpu_budget > 0:
         out["cpu_baseline"] = cpu_baseline(layout, q0, params, args.cpu_budget)

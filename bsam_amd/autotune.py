@@ -1,14 +1,15 @@
 """Schedule autotuner: times one workload under each combination of the
 engine's schedule knobs and reports the fastest.
 
-Knobs (bsam_tune): tail_cells (which single-patch coarse levels run inside
-the one-workgroup tail kernel), tail_threads (its workgroup size) and
-tail_lds (whether the tail keeps its levels in LDS). Every candidate solves
-the same time steps from the same initial state; results are bit-identical
-across candidates (the schedule changes where work runs, never the
-arithmetic), which the sweep checks on the final root field.
+Knobs (bsam_tune): the level kernels' tile shape (tile_x x tile_y, in 3D
+times the extent that fills 256 threads), tail_cells (which single-patch
+coarse levels run inside the one-workgroup tail kernel), tail_threads (its
+workgroup size) and tail_lds (whether the tail keeps its levels in LDS).
+Every candidate solves the same time steps from the same initial state;
+results are bit-identical across candidates (the schedule changes where work
+runs, never the arithmetic), which the sweep checks on the final root field.
 
-    python -m bsam_amd.autotune [--steps 5]
+    python -m bsam_amd.autotune [--config c2|c3|c4] [--steps 5] [--knobs tiles|tail]
 """
 
 from __future__ import annotations
@@ -22,13 +23,14 @@ from typing import Dict, List
 import numpy as np
 
 from . import engine as E
-from .workloads import C2_PARAMS, config_c2
 
-GRID = {
+TAIL_GRID = {
     "tail_cells": [0, 256, 1024, 4096, 16384],
     "tail_threads": [256, 512, 1024],
     "tail_lds": [-1, 0],
 }
+TILES_2D = [(32, 8), (64, 4), (16, 16), (128, 2), (256, 1)]
+TILES_3D = [(8, 8), (16, 4), (32, 4), (16, 8), (32, 2), (64, 2), (16, 16), (32, 8)]
 
 
 def time_schedule(layout, q0, params, tune: Dict[str, float], steps: int, warmup: int = 1):
@@ -52,17 +54,35 @@ def time_schedule(layout, q0, params, tune: Dict[str, float], steps: int, warmup
     return dt / cycles, cycles, root
 
 
-def sweep(steps: int = 5, grid: Dict[str, List[float]] = None):
-    layout, q0 = config_c2()
-    params = dict(C2_PARAMS)
-    grid = grid or GRID
-    keys = list(grid)
-    out, ref = [], None
-    for combo in itertools.product(*(grid[k] for k in keys)):
+def workload(name: str):
+    from . import workloads as W
+    if name == "c2":
+        lay, q0 = W.config_c2()
+        return lay, q0, dict(W.C2_PARAMS)
+    if name == "c3":
+        lay, q0 = W.config_c3()
+        return lay, q0, dict(W.C3_PARAMS)
+    lay, q0 = W.config_c4()
+    return lay, q0, dict(W.C4_PARAMS)
+
+
+def candidates(knobs: str, ndims: int):
+    if knobs == "tiles":
+        for tx, ty in (TILES_2D if ndims == 2 else TILES_3D):
+            yield {"tile_x": tx, "tile_y": ty}
+        return
+    keys = list(TAIL_GRID)
+    for combo in itertools.product(*(TAIL_GRID[k] for k in keys)):
         tune = dict(zip(keys, combo))
-        if tune.get("tail_cells", 1) == 0 and (tune.get("tail_threads", 1024) != 1024
-                                              or tune.get("tail_lds", -1) != -1):
+        if tune["tail_cells"] == 0 and (tune["tail_threads"] != 1024 or tune["tail_lds"] != -1):
             continue   # no tail: the other tail knobs do nothing
+        yield tune
+
+
+def sweep(config: str = "c2", steps: int = 5, knobs: str = "tail"):
+    layout, q0, params = workload(config)
+    out, ref = [], None
+    for tune in candidates(knobs, layout.ndims):
         per_v, cycles, root = time_schedule(layout, q0, params, tune, steps)
         same = True
         if ref is None:
@@ -78,9 +98,11 @@ def sweep(steps: int = 5, grid: Dict[str, List[float]] = None):
 
 def main():
     ap = argparse.ArgumentParser(description="sweep the engine's schedule knobs")
+    ap.add_argument("--config", choices=("c2", "c3", "c4"), default="c2")
     ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--knobs", choices=("tiles", "tail"), default="tiles")
     args = ap.parse_args()
-    best, _ = sweep(args.steps)
+    best, _ = sweep(args.config, args.steps, args.knobs)
     print(json.dumps({"best": best}))
 
 

@@ -72,6 +72,7 @@ struct Dev {
   const DPatch *P;
   Problem prm;
   int nrvars;
+  int tile_lx, tile_ly;   // level-kernel tiles: 2^tile_lx x 2^tile_ly (x the rest of 256)
 };
 
 // Which field set a level operation acts on.

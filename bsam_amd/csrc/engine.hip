@@ -112,13 +112,11 @@ struct bsam_ctx {
          *ftmp = nullptr;
   float *e = nullptr, *qce = nullptr, *fe = nullptr, *qb = nullptr;  // mixed precision
   int inner_cycles = 1;      // FP32 V-cycles per defect-correction step
-  double fnorm = 0.0;        // RMS of f on the finest level (relative residual)
   DPatch *d_P = nullptr;
   double *d_partial = nullptr;
   int partial_cap = 0;
   double *d_hist = nullptr;
   int *d_state = nullptr;
-  double *d_scale = nullptr;   // residual-threshold scale of the mixed solve
   int *h_state = nullptr;
   // kernel timing with HIP events on the context's stream (bsam_profile_*)
   unsigned prof_mask = 0;
@@ -136,6 +134,11 @@ struct bsam_ctx {
 
   // tuning (bsam_tune): single-workgroup tail below tail_cells cells per level,
   // hipGraph replay of the multigrid iteration
+  // tile lists spanning levels: interior and ghost-extended tiles of all
+  // levels, interior tiles of levels 0..finest (source)
+  std::vector<Tile> all_int, all_ext, src_int;
+  Tile *d_all_int = nullptr, *d_all_ext = nullptr, *d_src_int = nullptr;
+  long long src_cells = 0;
   long long tail_cells = 4096;
   int tail_threads = 1024;
   long long tail_lds_cap = -1;      // LDS bytes the tail may use (-1: device limit)
@@ -147,6 +150,10 @@ struct bsam_ctx {
   hipGraph_t graph = nullptr, graph_mixed = nullptr;
   hipGraphExec_t graph_exec = nullptr, graph_mixed_exec = nullptr;
 
+  int tile_lx = -1, tile_ly = -1;   // tile extents (log2); -1: per-dimension default
+  int tx() const { return 1 << tile_lx; }
+  int ty() const { return 1 << tile_ly; }
+  int tz() const { return nd() == 3 ? 256 >> (tile_lx + tile_ly) : 1; }
   int nd() const { return cfg.ndims; }
   int ntail() const { return tail_top - cfg.minlevel + 1; }
   LevelData &L(int level) {
@@ -160,6 +167,8 @@ struct bsam_ctx {
     d.P = d_P;
     d.prm = Problem{cfg.dt, cfg.eps2, cfg.fc_alpha, cfg.fc_shift, cfg.fe_num, cfg.fe_den};
     d.nrvars = cfg.nrvars;
+    d.tile_lx = tile_lx;
+    d.tile_ly = tile_ly;
     return d;
   }
   void need_device() const {
@@ -192,7 +201,9 @@ void bsam_ctx::free_device() {
   F(e); F(qce); F(fe); F(qb);
   e = qce = fe = qb = nullptr;
   F(d_state);
-  F(d_scale);
+  F(d_all_int);
+  F(d_all_ext);
+  F(d_src_int);
   F(d_tail);
   d_tail = nullptr;
   drop_graphs();
@@ -262,7 +273,7 @@ static thread_local std::string g_err;
 // ---- hierarchy commit --------------------------------------------------------
 static void build_tiles(bsam_ctx *c, LevelData &L, int level) {
   int nd = c->nd();
-  int TX = tile_x(nd), TY = tile_y(nd), TZ = tile_z(nd);
+  int TX = c->tx(), TY = c->ty(), TZ = c->tz();
   auto cover = [&](std::vector<Tile> &out, int p, int lo, int ex, int ey, int ez) {
     // tiles over [lo, lo+ex) x [lo, lo+ey) x [lo, lo+ez) (z fixed 0 in 2D)
     for (int k = (nd == 3 ? lo : 0); k < (nd == 3 ? lo + ez : 1); k += TZ)
@@ -334,6 +345,20 @@ static void commit(bsam_ctx *c) {
   const int nd = c->nd(), nv = c->cfg.nrvars;
   int n = (int)c->hp.size();
   if (n == 0) throw Error("empty hierarchy");
+  // tile shape: 32 x 8 in 2D and 16 x 8 x 2 in 3D unless tuned (the fastest
+  // of the tile sweep on the 2D and 3D AMR workloads)
+  int &lx = c->tile_lx, &ly = c->tile_ly;
+  if (nd == 2) {
+    if (lx < 0 && ly < 0) lx = 5, ly = 3;
+    else if (lx < 0) lx = 8 - ly;
+    else if (ly < 0) ly = 8 - lx;
+    if (lx + ly != 8) throw Error("2D tiles must hold 256 cells (tile_x * tile_y = 256)");
+  } else {
+    if (lx < 0 && ly < 0) lx = 4, ly = 3;
+    else if (lx < 0) lx = std::min(4, 8 - ly);
+    else if (ly < 0) ly = std::min(3, 8 - lx);
+    if (lx + ly > 8) throw Error("3D tiles: tile_x * tile_y must be at most 256");
+  }
   int minl = c->cfg.minlevel;
   c->finest = minl;
   for (auto &h : c->hp) c->finest = std::max(c->finest, h.level);
@@ -426,7 +451,7 @@ static void commit(bsam_ctx *c) {
     }
     for (int f = 0; f < 3; f++) {
       std::string why;
-      L.fused[f] = plan_fused_ghost(c->hp, L.t_int, nd, tile_x(nd), tile_y(nd), tile_z(nd),
+      L.fused[f] = plan_fused_ghost(c->hp, L.t_int, nd, c->tx(), c->ty(), c->tz(),
                                     L.gp[kFusedPass[f]], L.gp[kFusedColor[f]], kFusedColor[f],
                                     why);
     }
@@ -501,12 +526,27 @@ static void commit(bsam_ctx *c) {
       if (plan_tail(c, b, s)) break;
     }
   }
+  c->all_int.clear();
+  c->all_ext.clear();
+  c->src_int.clear();
+  c->src_cells = 0;
+  for (int l = c->cfg.minlevel; l <= c->finest; l++) {
+    LevelData &L = c->lev[l - minl];
+    c->all_int.insert(c->all_int.end(), L.t_int.begin(), L.t_int.end());
+    c->all_ext.insert(c->all_ext.end(), L.t_ext.begin(), L.t_ext.end());
+    if (l >= 0) {
+      c->src_int.insert(c->src_int.end(), L.t_int.begin(), L.t_int.end());
+      c->src_cells += L.cells;
+    }
+  }
+  c->d_all_int = dev_upload(c->all_int, s);
+  c->d_all_ext = dev_upload(c->all_ext, s);
+  c->d_src_int = dev_upload(c->src_int, s);
   c->partial_cap = (int)maxt;
   HIPCK(hipMalloc(&c->d_partial, sizeof(double) * maxt));
   int hmax = std::max(c->cfg.maxvcycles, 1) + 1;
   HIPCK(hipMalloc(&c->d_hist, sizeof(double) * hmax));
   HIPCK(hipMalloc(&c->d_state, sizeof(int) * 4));
-  HIPCK(hipMalloc(&c->d_scale, sizeof(double)));
   HIPCK(hipHostMalloc(&c->h_state, sizeof(int) * 4));
   HIPCK(hipStreamSynchronize(s));
 }
@@ -566,12 +606,12 @@ static void op_fill_down(bsam_ctx *c, int field) {
                                field == BSAM_FIELD_F ? 1 : 2, c->tail_threads, c->stream));
   }
 }
+// GetSourceFunction / UpdateSourceFunction on levels finest..0: independent
+// per cell, so one launch over the interior tiles of all those levels.
 static void op_source(bsam_ctx *c, int full) {
-  for (int level = c->finest; level >= 0; level--) {
-    LevelData &L = c->L(level);
-    ProfScope ps(c, BSAM_PROF_SOURCE, 32.0 * L.cells);
-    HIPCK(launch_source(c->nd(), c->dev(), L.d_int, (int)L.t_int.size(), full, c->stream));
-  }
+  ProfScope ps(c, BSAM_PROF_SOURCE, 32.0 * c->src_cells);
+  HIPCK(launch_source(c->nd(), c->dev(), c->d_src_int, (int)c->src_int.size(), full,
+                      c->stream));
 }
 // The smoothing passes of AFASVcycle (afasroutines.f90:493-497, 512-516): per
 // pass, relax colour 1, SetGhost(1), relax colour 2, SetGhost(2). With
@@ -621,7 +661,7 @@ static void reset_state(bsam_ctx *c) {
   HIPCK(hipMemsetAsync(c->d_state, 0, sizeof(int) * 4, c->stream));
 }
 // queue ErrorAFAS(level): residual norm into d_hist[state[0]]
-static void queue_error(bsam_ctx *c, int level, double tol, const double *scale = nullptr) {
+static void queue_error(bsam_ctx *c, int level, double tol) {
   LevelData &L = c->L(level);
   int nt = (int)L.t_int.size();
   {
@@ -629,7 +669,7 @@ static void queue_error(bsam_ctx *c, int level, double tol, const double *scale 
     HIPCK(launch_resnorm(c->nd(), c->dev(), L.d_int, nt, c->d_partial, 0, c->stream));
   }
   double count = (double)(c->cfg.nrvars * L.cells);
-  HIPCK(launch_finalize(c->d_partial, nt, count, tol, scale, c->d_hist, c->d_state, c->stream));
+  HIPCK(launch_finalize(c->d_partial, nt, count, tol, c->d_hist, c->d_state, c->stream));
 }
 
 // Captures `body` on the context's stream into *exec (once per hierarchy).
@@ -708,46 +748,27 @@ static void multigrid_iterations(bsam_ctx *c, int *ncycles, double *hist, int ma
 // A(qb + e) - A(qb) = r for the correction e with inner FP32 V-cycles (the
 // same level schedule and ghost programs as the FP64 solve), adds e to q in
 // FP64 and restores FP64 consistency (FillDown, ghost fill) before the FP64
-// residual test. The test is relative: RMS(f - A(q)) < qerrortol * RMS(f)
-// on the finest level; hist holds RMS(f - A(q)) / RMS(f).
+// residual test, which is the FP64 path's: ErrorAFAS(finest) < qerrortol.
+// The per-level set-up and correction run as one launch over the tiles of
+// all levels.
 static void mixed_iterations(bsam_ctx *c, int *ncycles, double *hist, int maxhist,
                              int *converged) {
   op_fill_down(c, BSAM_FIELD_Q);
   op_source(c, 1);
   op_fill_down(c, BSAM_FIELD_F);
-  LevelData &LF = c->L(c->finest);
-  int nt = (int)LF.t_int.size();
-  double count = (double)(c->cfg.nrvars * LF.cells);
-  reset_state(c);
-  HIPCK(launch_resnorm(c->nd(), c->dev(), LF.d_int, nt, c->d_partial, 1, c->stream));
-  HIPCK(launch_finalize(c->d_partial, nt, count, -1.0, nullptr, c->d_hist, c->d_state,
-                        c->stream));
-  HIPCK(hipMemcpyAsync(&c->fnorm, c->d_hist, sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  HIPCK(hipStreamSynchronize(c->stream));
-  double fn = c->fnorm > 0.0 ? c->fnorm : 1.0;
-  // the captured iteration reads the threshold scale from d_scale
-  HIPCK(hipMemcpyAsync(c->d_scale, &fn, sizeof(double), hipMemcpyHostToDevice, c->stream));
-  HIPCK(hipStreamSynchronize(c->stream));
   reset_state(c);
   int done = 0;
   int n = iterate(c, &c->graph_mixed, &c->graph_mixed_exec, [&]() {
-    for (int l = c->cfg.minlevel; l <= c->finest; l++) {
-      LevelData &L = c->L(l);
-      HIPCK(launch_defect_setup(c->nd(), c->dev(), L.d_ext, (int)L.t_ext.size(), c->stream));
-    }
+    HIPCK(launch_defect_setup(c->nd(), c->dev(), c->d_all_ext, (int)c->all_ext.size(),
+                              c->stream));
     for (int k = 0; k < c->inner_cycles; k++) op_vcycle(c, c->finest, PREC_DEFECT_FP32);
-    for (int l = c->cfg.minlevel; l <= c->finest; l++) {
-      LevelData &L = c->L(l);
-      HIPCK(launch_apply_correction(c->nd(), c->dev(), L.d_int, (int)L.t_int.size(),
-                                    c->stream));
-    }
+    HIPCK(launch_apply_correction(c->nd(), c->dev(), c->d_all_int, (int)c->all_int.size(),
+                                  c->stream));
     op_fill_down(c, BSAM_FIELD_Q);
     for (int l = 0; l <= c->finest; l++) op_set_ghost(c, l, 0);
-    queue_error(c, c->finest, c->cfg.qerrortol, c->d_scale);
+    queue_error(c, c->finest, c->cfg.qerrortol);
   }, &done);
   report(c, n, done, ncycles, hist, maxhist, converged);
-  if (hist)
-    for (int k = 0; k < std::min(n, maxhist); k++) hist[k] /= fn;
 }
 
 static void solve(bsam_ctx *c, int *ncycles, double *hist, int maxhist, int *converged) {
@@ -1039,6 +1060,12 @@ int bsam_tune(bsam_ctx *c, const char *key, double value) {
   if (k == "tail_cells") {
     if (c->committed || c->building) throw Error("tail_cells must be set before the hierarchy");
     c->tail_cells = (long long)value;
+  } else if (k == "tile_x" || k == "tile_y") {
+    if (c->committed || c->building) throw Error(k + " must be set before the hierarchy");
+    int v = (int)value, lg = 0;
+    while ((1 << lg) < v) lg++;
+    if (v < 1 || (1 << lg) != v || v > 256) throw Error(k + " must be a power of two <= 256");
+    (k == "tile_x" ? c->tile_lx : c->tile_ly) = lg;
   } else if (k == "tail_lds") {
     if (c->committed || c->building) throw Error("tail_lds must be set before the hierarchy");
     c->tail_lds_cap = (long long)value;
@@ -1074,6 +1101,9 @@ int bsam_tune_get(const bsam_ctx *cc, const char *key, double *value) {
   else if (k == "inner_cycles") *value = c->inner_cycles;
   else if (k == "graph") *value = c->use_graph;
   else if (k == "fuse_ghost") *value = c->fuse_ghost;
+  else if (k == "tile_x") *value = c->tx();
+  else if (k == "tile_y") *value = c->ty();
+  else if (k == "tile_z") *value = c->tz();
   else if (k == "tail_top") *value = c->tail_top;
   else if (k == "tail_lds_fp64") *value = (double)c->tail_lds[0];
   else if (k == "tail_lds_fp32") *value = (double)c->tail_lds[1];

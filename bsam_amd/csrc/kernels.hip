@@ -17,10 +17,9 @@
 
 namespace bsam {
 
-// Tile extents of the level kernels (256 threads).
-template <int ND> struct TileShape;
-template <> struct TileShape<2> { static constexpr int X = 32, Y = 8, Z = 1; };
-template <> struct TileShape<3> { static constexpr int X = 8, Y = 8, Z = 4; };
+// Level kernels run 256-thread workgroups over tiles of 2^tile_lx x 2^tile_ly
+// (x 256 / 2^(tile_lx + tile_ly) in 3D) cells; the shape is chosen per
+// context (Dev::tile_lx, tile_ly).
 
 // ---- CH elemental terms (problem.f90:760-813; Mob = 1) -------------------
 __device__ __forceinline__ double dfc(const Problem &p, double c) {
@@ -426,12 +425,11 @@ __device__ __forceinline__ void apply_correction_cell(const Dev &d, const DPatch
 
 // ---- tile decode -----------------------------------------------------------
 template <int ND>
-__device__ __forceinline__ void tile_cell(const Tile &t, int &i, int &j, int &k) {
-  using S = TileShape<ND>;
-  int tid = threadIdx.x;
-  i = t.i0 + tid % S::X;
-  j = t.j0 + (tid / S::X) % S::Y;
-  k = ND == 3 ? t.k0 + tid / (S::X * S::Y) : 0;
+__device__ __forceinline__ void tile_cell(const Dev &d, const Tile &t, int &i, int &j, int &k) {
+  int tid = threadIdx.x, lx = d.tile_lx, ly = d.tile_ly;
+  i = t.i0 + (tid & ((1 << lx) - 1));
+  j = t.j0 + ((tid >> lx) & ((1 << ly) - 1));
+  k = ND == 3 ? t.k0 + (tid >> (lx + ly)) : 0;
 }
 template <int ND>
 __device__ __forceinline__ bool in_box(int i, int j, int k, int ex, int ey, int ez) {
@@ -444,7 +442,7 @@ __global__ __launch_bounds__(256) void k_relax(Dev d, const Tile *tiles, int ipa
   Tile t = tiles[blockIdx.x];
   Lvl<T> L = level_ref<T>(d, t.patch);
   int i, j, k;
-  tile_cell<ND>(t, i, j, k);
+  tile_cell<ND>(d, t, i, j, k);
   if (!in_box<ND>(i, j, k, L.P.mx, L.P.my, L.P.mz)) return;
   // RelaxGrid's loop start 1+mod(j+ipass,2) (and +k in 3D) selects the
   // cells with i+j(+k)+ipass odd.
@@ -469,7 +467,7 @@ __global__ __launch_bounds__(256) void k_relax_fused(Dev d, const Tile *tiles, i
   Tile t = tiles[blockIdx.x];
   Lvl<T> L = level_ref<T>(d, t.patch);
   int i, j, k;
-  tile_cell<ND>(t, i, j, k);
+  tile_cell<ND>(d, t, i, j, k);
   if (!in_box<ND>(i, j, k, L.P.mx, L.P.my, L.P.mz)) return;
   if (((i + j + (ND == 3 ? k : 0) + color) & 1) == 0) return;
   relax_cell<ND, T, DEF>(d.prm, L, i, j, k);
@@ -488,7 +486,7 @@ __global__ __launch_bounds__(256) void k_restrict(Dev d, const Tile *tiles) {
   Tile t = tiles[blockIdx.x];
   Lvl<T> C = level_ref<T>(d, t.patch);
   int i, j, k;
-  tile_cell<ND>(t, i, j, k);
+  tile_cell<ND>(d, t, i, j, k);
   if (!in_box<ND>(i, j, k, C.P.cmx, C.P.cmy, C.P.cmz)) return;
   restrict_cell<ND, T>(d.nrvars, C, level_ref<T>(d, C.P.parent), i, j, k);
 }
@@ -498,7 +496,7 @@ __global__ __launch_bounds__(256) void k_filldown(Dev d, const Tile *tiles, int 
   Tile t = tiles[blockIdx.x];
   Lvl<T> C = level_ref<T>(d, t.patch);
   int i, j, k;
-  tile_cell<ND>(t, i, j, k);
+  tile_cell<ND>(d, t, i, j, k);
   if (!in_box<ND>(i, j, k, C.P.cmx, C.P.cmy, C.P.cmz)) return;
   filldown_cell<ND, T>(d.nrvars, C, level_ref<T>(d, C.P.parent), i, j, k, field);
 }
@@ -509,7 +507,7 @@ __global__ __launch_bounds__(256) void k_coarse_ghost(Dev d, const Tile *tiles) 
   Tile t = tiles[blockIdx.x];
   Lvl<T> C = level_ref<T>(d, t.patch);
   int i, j, k;
-  tile_cell<ND>(t, i, j, k);
+  tile_cell<ND>(d, t, i, j, k);
   if (!in_box<ND>(i, j, k, C.P.cmx + 1, C.P.cmy + 1, C.P.cmz + 1)) return;
   coarse_ghost_cell<ND, T>(d.nrvars, C, level_ref<T>(d, C.P.parent), i, j, k);
 }
@@ -519,7 +517,7 @@ __global__ __launch_bounds__(256) void k_coarse_load(Dev d, const Tile *tiles) {
   Tile t = tiles[blockIdx.x];
   Lvl<T> C = level_ref<T>(d, t.patch);
   int i, j, k;
-  tile_cell<ND>(t, i, j, k);
+  tile_cell<ND>(d, t, i, j, k);
   if (!in_box<ND>(i, j, k, C.P.cmx, C.P.cmy, C.P.cmz)) return;
   coarse_load_cell<ND, T, DEF>(d.prm, C, level_ref<T>(d, C.P.parent), i, j, k);
 }
@@ -530,7 +528,7 @@ __global__ __launch_bounds__(256) void k_correct(Dev d, const Tile *tiles) {
   Tile t = tiles[blockIdx.x];
   Lvl<T> C = level_ref<T>(d, t.patch);
   int i, j, k;
-  tile_cell<ND>(t, i, j, k);
+  tile_cell<ND>(d, t, i, j, k);
   if (!in_box<ND>(i, j, k, C.P.mx + 1, C.P.my + 1, C.P.mz + 1)) return;
   correct_cell<ND, T>(d.nrvars, C, level_ref<T>(d, C.P.parent), i, j, k);
 }
@@ -540,7 +538,7 @@ __global__ __launch_bounds__(256) void k_source(Dev d, const Tile *tiles, int fu
   Tile t = tiles[blockIdx.x];
   const DPatch &P = d.P[t.patch];
   int i, j, k;
-  tile_cell<ND>(t, i, j, k);
+  tile_cell<ND>(d, t, i, j, k);
   if (!in_box<ND>(i, j, k, P.mx, P.my, P.mz)) return;
   source_cell<ND>(d, P, i, j, k, full);
 }
@@ -551,7 +549,7 @@ __global__ __launch_bounds__(256) void k_defect_setup(Dev d, const Tile *tiles) 
   Tile t = tiles[blockIdx.x];
   const DPatch &P = d.P[t.patch];
   int i, j, k;
-  tile_cell<ND>(t, i, j, k);
+  tile_cell<ND>(d, t, i, j, k);
   if (!in_box<ND>(i, j, k, P.mx + 1, P.my + 1, P.mz + 1)) return;
   defect_setup_cell<ND>(d, P, i, j, k);
 }
@@ -561,7 +559,7 @@ __global__ __launch_bounds__(256) void k_apply_correction(Dev d, const Tile *til
   Tile t = tiles[blockIdx.x];
   const DPatch &P = d.P[t.patch];
   int i, j, k;
-  tile_cell<ND>(t, i, j, k);
+  tile_cell<ND>(d, t, i, j, k);
   if (!in_box<ND>(i, j, k, P.mx, P.my, P.mz)) return;
   apply_correction_cell<ND>(d, P, i, j, k);
 }
@@ -830,7 +828,7 @@ __global__ __launch_bounds__(256) void k_resnorm(Dev d, const Tile *tiles, doubl
   Tile t = tiles[blockIdx.x];
   const DPatch &P = d.P[t.patch];
   int i, j, k;
-  tile_cell<ND>(t, i, j, k);
+  tile_cell<ND>(d, t, i, j, k);
   double s = 0.0;
   if (in_box<ND>(i, j, k, P.mx, P.my, P.mz)) {
     long long fo = P.f + fat(P, i, j, ND == 3 ? k : 1);
@@ -852,12 +850,8 @@ __global__ __launch_bounds__(256) void k_resnorm(Dev d, const Tile *tiles, doubl
 // Sum the per-tile partials in index order (strided, then a fixed tree) and
 // finish ErrorAFAS: sqrt(sum / count) into hist[state[0]]; state[0] counts
 // the calls and state[1] is set once a residual falls below tol.
-// Residual norm sqrt(sum / count) into hist[state[0]]; sets state[1] when it
-// is below tol (times *scale when scale is given: a threshold that changes
-// between calls stays out of the captured kernel arguments).
 __global__ __launch_bounds__(256) void k_finalize(const double *partial, int n, double count,
-                                                   double tol, const double *scale,
-                                                   double *hist, int *state) {
+                                                   double tol, double *hist, int *state) {
   double s = 0.0;
   for (int e = threadIdx.x; e < n; e += blockDim.x) s += partial[e];
   double tot = block_sum(s);
@@ -866,8 +860,7 @@ __global__ __launch_bounds__(256) void k_finalize(const double *partial, int n, 
     int it = state[0];
     hist[it] = r;
     state[0] = it + 1;
-    double thr = scale ? tol * scale[0] : tol;
-    if (r < thr) state[1] = 1;
+    if (r < tol) state[1] = 1;
   }
 }
 
@@ -993,8 +986,8 @@ hipError_t launch_resnorm(int nd, const Dev &d, const Tile *tiles, int nt, doubl
 }
 
 hipError_t launch_finalize(const double *partial, int n, double count, double tol,
-                           const double *scale, double *hist, int *state, hipStream_t s) {
-  k_finalize<<<1, 256, 0, s>>>(partial, n, count, tol, scale, hist, state);
+                           double *hist, int *state, hipStream_t s) {
+  k_finalize<<<1, 256, 0, s>>>(partial, n, count, tol, hist, state);
   return hipGetLastError();
 }
 
@@ -1039,8 +1032,5 @@ hipError_t tail_allow_lds(size_t lds) {
   return hipSuccess;
 }
 
-int tile_x(int nd) { return nd == 2 ? TileShape<2>::X : TileShape<3>::X; }
-int tile_y(int nd) { return nd == 2 ? TileShape<2>::Y : TileShape<3>::Y; }
-int tile_z(int nd) { return nd == 2 ? TileShape<2>::Z : TileShape<3>::Z; }
 
 }  // namespace bsam

@@ -31,7 +31,7 @@ hipError_t launch_apply_correction(int nd, const Dev &d, const Tile *tiles, int 
 hipError_t launch_resnorm(int nd, const Dev &d, const Tile *tiles, int nt, double *partial,
                           int what, hipStream_t s);
 hipError_t launch_finalize(const double *partial, int n, double count, double tol,
-                           const double *scale, double *hist, int *state, hipStream_t s);
+                           double *hist, int *state, hipStream_t s);
 hipError_t launch_copy(double *dst, const double *src, long long n, hipStream_t s);
 // Tail kernels: one workgroup of `threads` over levels 0..nlev-1 of the
 // single-patch chain; the V-cycle with `lds` bytes of dynamic LDS
@@ -42,9 +42,5 @@ hipError_t launch_tail_filldown(int nd, const Dev &d, const TailLevel *lv, int n
                                 int threads, hipStream_t s);
 // Raises the dynamic-LDS limit of the tail kernels to `lds` bytes.
 hipError_t tail_allow_lds(size_t lds);
-
-int tile_x(int nd);
-int tile_y(int nd);
-int tile_z(int nd);
 
 }  // namespace bsam

@@ -249,3 +249,48 @@ def config_c3(root: int = 512, seed: int = 3, counts=(220, 640, 1140)):
     8..128, log-uniform), phi0 = 0.1 + U(-0.05, 0.05)."""
     lay = nested_random_layout(root, 3, counts, seed)
     return lay, noise_q(lay, seed, 0.1, 0.05)
+
+
+C4_PARAMS = dict(eps=0.02, dt=5e-4, well="plusminus", nsmoothingpasses=2, qerrortol=1e-8)
+
+
+def nested_block_layout_3d(root: int, nlevels: int, fill, seed: int, sides=(16, 32),
+                           margin: int = 1, max_tries: int = 100000) -> Layout:
+    """A properly nested synthetic 3D hierarchy: on each level, cubic patches
+    of side (fine cells) drawn from `sides`, placed at random inside a parent
+    (at least `margin` coarse cells from its faces, not overlapping siblings)
+    until they cover `fill[l]` of the parent level's volume."""
+    rng = np.random.default_rng(seed)
+    lay = Layout(3, (root,) * 3, 1.0 / root)
+    occ = {0: np.zeros((root,) * 3, bool)}
+    for lev in range(1, nlevels + 1):
+        parents = [lay.patches[i] for i in lay.level_ids(lev - 1)]
+        vol = np.array([np.prod(p.mx) for p in parents], float)
+        target = fill[lev - 1] * vol.sum()
+        covered = 0.0
+        tries = 0
+        while covered < target and tries < max_tries:
+            tries += 1
+            par = parents[rng.choice(len(parents), p=vol / vol.sum())]
+            cs = int(rng.choice(sides)) // 2          # coarse cells per side
+            room = [par.mx[d] - 2 * margin - cs for d in range(3)]
+            if min(room) < 0:
+                continue
+            lo = [margin + 1 + int(rng.integers(0, room[d] + 1)) for d in range(3)]
+            sl = tuple(slice(lo[d] - 1, lo[d] - 1 + cs) for d in range(3))
+            o = occ[par.id]
+            if o[sl].any():
+                continue
+            o[sl] = True
+            cid = lay.add_child(par.id, tuple((lo[d], lo[d] + cs - 1) for d in range(3)))
+            occ[cid] = np.zeros((2 * cs,) * 3, bool)
+            covered += cs ** 3
+    return lay
+
+
+def config_c4(root: int = 128, seed: int = 4, fill=(0.25, 0.25)):
+    """3D root 128^3 plus 2 refined levels of 16^3 and 32^3 blocks (each
+    level covering about a quarter of its parent level), phi0 = U(-0.05, 0.05)."""
+    lay = nested_block_layout_3d(root, 2, fill, seed)
+    return lay, noise_q(lay, seed, 0.0, 0.05)
+

@@ -126,8 +126,7 @@ int bsam_copy_qold(bsam_ctx *ctx);
  * V-cycles run, up to maxhist per-cycle residuals, and whether the last
  * residual fell below qerrortol (the reference sets lconverged, :98-101).
  * With BSAM_PREC_MIXED the iterations are FP64 defect-correction steps around
- * FP32 V-cycles, the test is relative (RMS(f - A q) < qerrortol * RMS(f) on
- * the finest level) and hist holds the relative residuals. */
+ * FP32 V-cycles, with the same FP64 residual test and history. */
 int bsam_multigrid_iterations(bsam_ctx *ctx, int *ncycles, double *hist,
                               int maxhist, int *converged);
 /* One time step on a fixed hierarchy: SetGhost on levels 0..finest, FillDown
@@ -147,6 +146,9 @@ int bsam_sync(bsam_ctx *ctx);
  *                  captured hipGraph; 0: launch kernels one by one.
  *   "inner_cycles" FP32 V-cycles per defect-correction step (mixed
  *                  precision; 1..8, default 1).
+ *   "tile_x", "tile_y"  level-kernel tile extents (powers of two; 2D
+ *                  tile_x * tile_y = 256, default 32 x 8; 3D the third extent
+ *                  fills 256 threads, default 16 x 8 x 2). Before the hierarchy.
  *   "fuse_ghost"   1 (default): apply each SetGhost inside the following
  *                  relaxation launch where the level allows; 0: separately.
  *   "tail_lds"     bytes of LDS the tail kernel may hold its levels' arrays

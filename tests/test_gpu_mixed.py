@@ -2,18 +2,18 @@
 
 The mixed solve converges to the same discrete solution as the FP64 path;
 the bar stated here is on the result, not on bits: after each time step the
-relative FP64 residual is below qerrortol, phi and mu differ from the FP64
-oracle's by less than TOL_FIELD (a multiple of the solver tolerance), and
-the composite mass h^d sum(phi) over the root (which FillDown makes the
-composite integral) differs from the oracle's by less than TOL_MASS per step
-taken.
+FP64 residual (ErrorAFAS on the finest level, the FP64 path's stop test) is
+below qerrortol, phi and mu differ from the FP64 oracle's by less than
+TOL_FIELD (a multiple of the solver tolerance), and the composite mass
+h^d sum(phi) over the root (which FillDown makes the composite integral)
+differs from the oracle's by less than TOL_MASS per step taken.
 
 Mass bound: summing the phi equation phi - dt Lap(mu) = f over the unit
 domain, two solutions whose residuals r1, r2 both meet the stop test differ in
 mass by |mean(r1 - r2)| <= 2 max RMS(r) (the Laplacian sums to zero up to the
-coarse-fine flux mismatch, second order in the field difference). The stop
-test bounds RMS(r) by qerrortol * RMS(f) and RMS(f) <= 1 here (|phi| < 1), so
-each step adds at most 2 * qerrortol.
+coarse-fine flux mismatch, second order in the field difference), and the
+stop test bounds RMS(r) by qerrortol, so each step adds at most
+2 * qerrortol.
 """
 
 import numpy as np

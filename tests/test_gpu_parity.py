@@ -35,13 +35,16 @@ LAYOUTS = {
 # for small coarse levels with its arrays in LDS, SetGhost applied inside the
 # next relaxation launch, hipGraph replay); every single-patch level in the
 # tail, in LDS as far as it fits or all in global memory; SetGhost and
-# relaxation as separate launches
+# relaxation as separate launches; other tile shapes (in 3D the third extent
+# fills the 256 threads)
 SCHEDULES = {
     "launch": {"tail_cells": 0, "graph": 0},
     "default": {},
     "all_tail": {"tail_cells": 1 << 30, "graph": 1},
     "all_tail_global": {"tail_cells": 1 << 30, "tail_lds": 0, "graph": 0},
     "unfused": {"fuse_ghost": 0, "graph": 0},
+    "tiles_64x4": {"tile_x": 64, "tile_y": 4},
+    "tiles_16x16": {"tile_x": 16, "tile_y": 16, "fuse_ghost": 1},
 }
 
 
