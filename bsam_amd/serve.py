@@ -241,11 +241,15 @@ def main():
     ap.add_argument("--budget-gb", type=float, default=64.0)
     ap.add_argument("--precision", choices=("fp64", "mixed"), default="mixed")
     ap.add_argument("--baseline", action="store_true", help="also run the FP64 serial baseline")
+    ap.add_argument("--burst", action="store_true",
+                    help="all requests waiting at t = 0 (capacity rather than arrival-bound)")
     args = ap.parse_args()
     reqs = make_requests(args.requests, args.rate, steps=args.steps)
     prec = E.PREC_MIXED if args.precision == "mixed" else E.PREC_FP64
     srv = Server(int(args.budget_gb * (1 << 30)), args.max_active, prec)
-    out = srv.run(reqs)
+    out = srv.run(reqs, realtime=not args.burst)
+    out["peak_active"] = srv.peak_active
+    out["peak_bytes"] = srv.peak_used
     out.pop("results")
     line = {"served": out}
     if args.baseline:
