@@ -57,6 +57,9 @@ struct LevelData {
   // SetGhost(ipass) fused into the next relaxation, per transition
   // (ipass -> colour): 0: 0 -> 1, 1: 1 -> 2, 2: 2 -> 1
   FusedGhost fused[3];
+  // SetGhost(ipass) of this level may run inside this level's restriction or
+  // fill-down launch (its reads avoid the parent cells those launches write)
+  bool down_ok[3] = {false, false, false};
   int *d_fstart[3] = {nullptr, nullptr, nullptr};
   GhostOp *d_fops[3] = {nullptr, nullptr, nullptr};
   long long cells = 0;
@@ -455,6 +458,20 @@ static void commit(bsam_ctx *c) {
                                     L.gp[kFusedPass[f]], L.gp[kFusedColor[f]], kFusedColor[f],
                                     why);
     }
+    if (l > minl) {
+      // parent cells under this level's patches: what its restriction and
+      // fill-down write
+      std::vector<unsigned char> written((size_t)c->nq, 0);
+      for (int p : L.patches) {
+        const HPatch &h = c->hp[p];
+        const HPatch &pa = c->hp[h.parent];
+        for (int k = (nd == 3 ? h.mb[2][0] : 0); k <= (nd == 3 ? h.mb[2][1] : 0); k++)
+          for (int j = h.mb[1][0]; j <= h.mb[1][1]; j++)
+            for (int i = h.mb[0][0]; i <= h.mb[0][1]; i++)
+              written[(size_t)(pa.qoff + i + pa.px * (j + (long long)pa.py * k))] = 1;
+      }
+      for (int ip = 0; ip < 3; ip++) L.down_ok[ip] = program_reads_avoid(L.gp[ip], written);
+    }
   }
   if (c->host_only) return;
   // device allocation
@@ -567,10 +584,23 @@ static LevelData &child_level(bsam_ctx *c, int level) {
   if (level <= c->cfg.minlevel) throw Error("level has no parent level");
   return c->L(level);
 }
-static void op_restrict(bsam_ctx *c, int level, int prec = PREC_SOLUTION_FP64) {
+// RestrictSolution of `level`; a pending SetGhost(level, ghost) (ghost >= 0)
+// runs in the same launch when its reads allow, else just before.
+static void op_restrict(bsam_ctx *c, int level, int prec = PREC_SOLUTION_FP64, int ghost = -1) {
   LevelData &L = child_level(c, level);
+  const GhostOp *gops = nullptr;
+  int ng = 0;
+  if (ghost >= 0) {
+    if (c->fuse_ghost && L.down_ok[ghost]) {
+      gops = L.d_gp[ghost];
+      ng = (int)L.gp[ghost].size();
+    } else {
+      op_set_ghost(c, level, ghost, prec);
+    }
+  }
   ProfScope ps(c, BSAM_PROF_RESTRICT, 24.0 * L.cells);
-  HIPCK(launch_restrict(c->nd(), prec, c->dev(), L.d_coarse, (int)L.t_coarse.size(), c->stream));
+  HIPCK(launch_restrict(c->nd(), prec, c->dev(), L.d_coarse, (int)L.t_coarse.size(), gops, ng,
+                        c->stream));
 }
 static void op_coarse_ghost(bsam_ctx *c, int level, int prec = PREC_SOLUTION_FP64) {
   LevelData &L = child_level(c, level);
@@ -582,24 +612,46 @@ static void op_coarse_load(bsam_ctx *c, int level, int prec = PREC_SOLUTION_FP64
   ProfScope ps(c, BSAM_PROF_COARSE_LOAD, 40.0 * L.cells);
   HIPCK(launch_coarse_load(c->nd(), prec, c->dev(), L.d_coarse, (int)L.t_coarse.size(), c->stream));
 }
+static void op_coarse_ghost_load(bsam_ctx *c, int level, int prec = PREC_SOLUTION_FP64) {
+  LevelData &L = child_level(c, level);
+  ProfScope ps(c, BSAM_PROF_COARSE_LOAD, 40.0 * L.cells);
+  HIPCK(launch_coarse_ghost_load(c->nd(), prec, c->dev(), L.d_cext, (int)L.t_cext.size(),
+                                 c->stream));
+}
 static void op_correct(bsam_ctx *c, int level, int prec = PREC_SOLUTION_FP64) {
   LevelData &L = child_level(c, level);
   ProfScope ps(c, BSAM_PROF_CORRECT, 40.0 * L.cells);
   HIPCK(launch_correct(c->nd(), prec, c->dev(), L.d_ext, (int)L.t_ext.size(), c->stream));
 }
-// FillDown, afasroutines.f90:223-239
-static void op_fill_down(bsam_ctx *c, int field) {
+// FillDown, afasroutines.f90:223-239. For q, each SetGhost(level - 1, 0) runs
+// inside the next level's fill-down launch when its reads allow; `ghost` >= 0
+// is a SetGhost(finest, ghost) still pending from the caller, handled the
+// same way.
+static void op_fill_down(bsam_ctx *c, int field, int ghost = -1) {
   if (field != BSAM_FIELD_F && field != BSAM_FIELD_Q) throw Error("fill_down: field must be F or Q");
   int low = std::max(c->cfg.minlevel + 1, c->tail_top + 1);
-  for (int level = c->finest; level >= low; level--) {
+  int carry = ghost;   // SetGhost program of `level` not yet applied
+  int level = c->finest;
+  for (; level >= low; level--) {
     LevelData &L = c->L(level);
+    const GhostOp *gops = nullptr;
+    int ng = 0;
+    if (carry >= 0) {
+      if (c->fuse_ghost && L.down_ok[carry]) {
+        gops = L.d_gp[carry];
+        ng = (int)L.gp[carry].size();
+      } else {
+        op_set_ghost(c, level, carry);
+      }
+    }
     {
       ProfScope ps(c, BSAM_PROF_FILLDOWN, 20.0 * L.cells);
       HIPCK(launch_filldown(c->nd(), c->dev(), L.d_coarse, (int)L.t_coarse.size(),
-                            field == BSAM_FIELD_F ? 1 : 2, c->stream));
+                            field == BSAM_FIELD_F ? 1 : 2, gops, ng, c->stream));
     }
-    if (field == BSAM_FIELD_Q) op_set_ghost(c, level - 1, 0);
+    carry = field == BSAM_FIELD_Q ? 0 : -1;
   }
+  if (carry >= 0) op_set_ghost(c, level, carry);
   if (c->ntail() > 1) {
     ProfScope ps(c, BSAM_PROF_TAIL, 0.0);
     HIPCK(launch_tail_filldown(c->nd(), c->dev(), c->d_tail, c->ntail(),
@@ -617,8 +669,10 @@ static void op_source(bsam_ctx *c, int full) {
 // pass, relax colour 1, SetGhost(1), relax colour 2, SetGhost(2). With
 // ghost0_first, a SetGhost(level, 0) precedes them. Each SetGhost is applied
 // inside the following relaxation launch where the level's fused plan allows.
-static void level_relax(bsam_ctx *c, int level, int prec = PREC_SOLUTION_FP64,
-                        bool ghost0_first = false) {
+// With defer_last the final SetGhost is not launched but returned (its ipass)
+// for the caller to apply.
+static int level_relax(bsam_ctx *c, int level, int prec = PREC_SOLUTION_FP64,
+                       bool ghost0_first = false, bool defer_last = false) {
   LevelData &L = c->L(level);
   int pending = ghost0_first ? 0 : -1;   // SetGhost program not yet applied
   for (int pass = 1; pass <= c->cfg.nsmoothingpasses; pass++)
@@ -635,27 +689,30 @@ static void level_relax(bsam_ctx *c, int level, int prec = PREC_SOLUTION_FP64,
       }
       pending = color;
     }
+  if (defer_last) return pending;
   if (pending >= 0) op_set_ghost(c, level, pending, prec);
+  return -1;
 }
-// AFASVcycle, afasroutines.f90:488-521
-static void op_vcycle(bsam_ctx *c, int level, int prec = PREC_SOLUTION_FP64) {
+// AFASVcycle, afasroutines.f90:488-521. With defer_final, the last
+// SetGhost of the level's post-smoothing is returned (its ipass, or -1)
+// instead of launched.
+static int op_vcycle(bsam_ctx *c, int level, int prec = PREC_SOLUTION_FP64,
+                     bool defer_final = false) {
   if (level <= c->tail_top) {
     ProfScope ps(c, BSAM_PROF_TAIL, 0.0);
     HIPCK(launch_tail_vcycle(c->nd(), prec, c->dev(), c->d_tail, level - c->cfg.minlevel + 1,
                              c->cfg.nsmoothingpasses, c->tail_threads,
                              c->tail_lds[prec == PREC_SOLUTION_FP64 ? 0 : 1], c->stream));
-    return;
+    return -1;
   }
-  level_relax(c, level, prec);
-  if (level > c->cfg.minlevel) {
-    op_restrict(c, level, prec);
-    op_set_ghost(c, level - 1, 0, prec);
-    op_coarse_ghost(c, level, prec);
-    op_coarse_load(c, level, prec);
-    op_vcycle(c, level - 1, prec);
-    op_correct(c, level, prec);
-    level_relax(c, level, prec, true);
-  }
+  if (level == c->cfg.minlevel) return level_relax(c, level, prec, false, defer_final);
+  int pend = level_relax(c, level, prec, false, true);
+  op_restrict(c, level, prec, pend);
+  op_set_ghost(c, level - 1, 0, prec);
+  op_coarse_ghost_load(c, level, prec);
+  op_vcycle(c, level - 1, prec);
+  op_correct(c, level, prec);
+  return level_relax(c, level, prec, true, defer_final);
 }
 static void reset_state(bsam_ctx *c) {
   HIPCK(hipMemsetAsync(c->d_state, 0, sizeof(int) * 4, c->stream));
@@ -734,8 +791,8 @@ static void multigrid_iterations(bsam_ctx *c, int *ncycles, double *hist, int ma
   // FillDown(q), source restore, FillDown(f) and the residual norm
   int done = 0;
   int n = iterate(c, &c->graph, &c->graph_exec, [&]() {
-    op_vcycle(c, c->finest);
-    op_fill_down(c, BSAM_FIELD_Q);
+    int pend = op_vcycle(c, c->finest, PREC_SOLUTION_FP64, true);
+    op_fill_down(c, BSAM_FIELD_Q, pend);
     op_source(c, 0);
     op_fill_down(c, BSAM_FIELD_F);
     queue_error(c, c->finest, c->cfg.qerrortol);
@@ -761,7 +818,13 @@ static void mixed_iterations(bsam_ctx *c, int *ncycles, double *hist, int maxhis
   int n = iterate(c, &c->graph_mixed, &c->graph_mixed_exec, [&]() {
     HIPCK(launch_defect_setup(c->nd(), c->dev(), c->d_all_ext, (int)c->all_ext.size(),
                               c->stream));
-    for (int k = 0; k < c->inner_cycles; k++) op_vcycle(c, c->finest, PREC_DEFECT_FP32);
+    // the last inner cycle's final ghost fill of the correction is never read
+    // (the correction is applied on interior cells, then restarts at zero)
+    for (int k = 0; k < c->inner_cycles; k++) {
+      int pend = op_vcycle(c, c->finest, PREC_DEFECT_FP32, true);
+      if (pend >= 0 && k + 1 < c->inner_cycles)
+        op_set_ghost(c, c->finest, pend, PREC_DEFECT_FP32);
+    }
     HIPCK(launch_apply_correction(c->nd(), c->dev(), c->d_all_int, (int)c->all_int.size(),
                                   c->stream));
     op_fill_down(c, BSAM_FIELD_Q);
@@ -1234,6 +1297,13 @@ int bsam_fused_ghost_size(const bsam_ctx *cc, int level, int ipass, int color) {
   int f = fused_index(ipass, color);
   if (f < 0 || !c->L(level).fused[f].ok) return -1;
   return c->L(level).fused[f].kept;
+}
+
+int bsam_ghost_rides_down(const bsam_ctx *cc, int level, int ipass) {
+  bsam_ctx *c = const_cast<bsam_ctx *>(cc);
+  if (!c || !c->committed || ipass < 0 || ipass > 2) return -1;
+  if (level < c->cfg.minlevel || level > c->finest) return -1;
+  return c->L(level).down_ok[ipass] ? 1 : 0;
 }
 
 int bsam_fused_ghost_apply_host(const bsam_ctx *cc, int level, int ipass, int color,

@@ -788,6 +788,22 @@ FusedGhost plan_fused_ghost(const std::vector<HPatch> &hp, const std::vector<Til
   return out;
 }
 
+bool program_reads_avoid(const std::vector<GhostOp> &prog,
+                         const std::vector<unsigned char> &written) {
+  auto hit = [&](long long off) {
+    return off >= 0 && off < (long long)written.size() && written[off];
+  };
+  for (const GhostOp &g : prog) {
+    if (hit(g.f0)) return false;
+    if (g.kind == 0) continue;
+    if (hit(g.f1)) return false;
+    int nc = g.kind == 1 ? 1 : g.kind == 2 ? 3 : 4;
+    for (int k = 0; k < nc; k++)
+      if (hit(g.c[k])) return false;
+  }
+  return true;
+}
+
 void locate_q_cell(const std::vector<HPatch> &hp, long long off, int &p, int &i, int &j,
                    int &k) {
   p = -1;

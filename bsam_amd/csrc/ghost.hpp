@@ -57,6 +57,12 @@ FusedGhost plan_fused_ghost(const std::vector<HPatch> &hp, const std::vector<Til
                             const std::vector<GhostOp> &prog, const std::vector<GhostOp> &next,
                             int color, std::string &why);
 
+// True if no update of `prog` reads a q-pool cell (variable-1 offset) in
+// `written`: the program may then run in the same launch as a kernel that
+// writes exactly those cells.
+bool program_reads_avoid(const std::vector<GhostOp> &prog,
+                         const std::vector<unsigned char> &written);
+
 // Patch and (i,j,k) (lower bound 0; k = 0 in 2D) of a q-pool offset.
 void locate_q_cell(const std::vector<HPatch> &hp, long long off, int &p, int &i, int &j,
                    int &k);

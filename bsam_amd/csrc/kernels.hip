@@ -15,6 +15,8 @@
 //    keeps its relative accuracy in e however small e gets.
 #include "kernels.hpp"
 
+#include <type_traits>
+
 namespace bsam {
 
 // Level kernels run 256-thread workgroups over tiles of 2^tile_lx x 2^tile_ly
@@ -98,28 +100,39 @@ __device__ __forceinline__ long long fat(const DPatch &P, int i, int j, int k) {
 // spacing hh: r1 = phi - dt/h^2 ULapMob(phi,mu), r2 = mu - Dfc(phi) +
 // eps^2/h^2 ULap(phi), with the undivided stencils of gridutilities.f90.
 // Defect form: Dfc(phi) becomes Dfc(base + phi) - Dfc(base).
+// The same from gathered values: a1/a2 hold phi/mu at the cell, +x, -x, +y,
+// -y (+z, -z).
 template <int ND, class T, bool DEF>
-__device__ __forceinline__ void op_at(const Problem &prm, const View<T> &a, T base, double hh,
-                                      long long c, T &r1, T &r2) {
+__device__ __forceinline__ void op_vals(const Problem &prm, T base, double hh, const T *a1,
+                                        const T *a2, T &r1, T &r2) {
   double h2 = hh * hh;
   T tmp1 = (T)(prm.dt / h2), tmp2 = (T)(prm.eps2 / h2);
-  const T *q1 = a.p, *q2 = a.p + a.vs;
-  T phi = q1[c], mu = q2[c];
-  long long sx = 1, sy = a.px, sz = a.pl;
+  T phi = a1[0], mu = a2[0];
   T lapmob, lap;
   if (ND == 2) {
-    lapmob = (((q2[c + sx] - mu) - (mu - q2[c - sx])) + (q2[c + sy] - mu)) -
-             (mu - q2[c - sy]);
-    lap = (((q1[c + sx] + q1[c - sx]) + q1[c + sy]) + q1[c - sy]) - (T)4 * phi;
+    lapmob = (((a2[1] - mu) - (mu - a2[2])) + (a2[3] - mu)) - (mu - a2[4]);
+    lap = (((a1[1] + a1[2]) + a1[3]) + a1[4]) - (T)4 * phi;
   } else {
-    lapmob = (((((q2[c + sx] - mu) - (mu - q2[c - sx])) + (q2[c + sy] - mu)) -
-               (mu - q2[c - sy])) + (q2[c + sz] - mu)) - (mu - q2[c - sz]);
-    lap = (((((q1[c + sx] + q1[c - sx]) + q1[c + sy]) + q1[c - sy]) + q1[c + sz]) +
-           q1[c - sz]) - (T)6 * phi;
+    lapmob = (((((a2[1] - mu) - (mu - a2[2])) + (a2[3] - mu)) - (mu - a2[4])) +
+              (a2[5] - mu)) - (mu - a2[6]);
+    lap = (((((a1[1] + a1[2]) + a1[3]) + a1[4]) + a1[5]) + a1[6]) - (T)6 * phi;
   }
   r1 = phi - tmp1 * lapmob;
   if (DEF) r2 = (mu - dfc_delta<T>(prm, base, phi)) + tmp2 * lap;
   else r2 = (mu - (T)dfc(prm, (double)phi)) + tmp2 * lap;
+}
+template <int ND, class T, bool DEF>
+__device__ __forceinline__ void op_at(const Problem &prm, const View<T> &a, T base, double hh,
+                                      long long c, T &r1, T &r2) {
+  const T *q1 = a.p, *q2 = a.p + a.vs;
+  long long off[7] = {0, 1, -1, a.px, -a.px, a.pl, -a.pl};
+  T a1[2 * ND + 1], a2[2 * ND + 1];
+#pragma unroll
+  for (int s = 0; s < 2 * ND + 1; s++) {
+    a1[s] = q1[c + off[s]];
+    a2[s] = q2[c + off[s]];
+  }
+  op_vals<ND, T, DEF>(prm, base, hh, a1, a2, r1, r2);
 }
 
 // Pointwise nonlinear Gauss-Seidel update of one cell: one Newton step on the
@@ -338,6 +351,77 @@ __device__ __forceinline__ void coarse_load_cell(const Problem &prm, const Lvl<T
   PA.f[po + PA.P.fvs] = R2 + c2;
 }
 
+// GetCoarseGhostPoints and CoarseLoadingFunction in one pass over the
+// ghost-extended coarse box (i,j,k in 0..cmx+1): ring cells copy the parent
+// into qc; interior cells evaluate the coarse operator reading the ring
+// values straight from the parent (the same values the copy stores).
+template <int ND, class T, bool DEF>
+__device__ __forceinline__ void coarse_ghost_load_cell(const Problem &prm, int nrvars,
+                                                       const Lvl<T> &C, const Lvl<T> &PA, int i,
+                                                       int j, int k) {
+  const DPatch &P = C.P;
+  auto on_ring = [&](int a, int b, int cc) {
+    return a == 0 || a == P.cmx + 1 || b == 0 || b == P.cmy + 1 ||
+           (ND == 3 && (cc == 0 || cc == P.cmz + 1));
+  };
+  if (on_ring(i, j, k)) {
+    coarse_ghost_cell<ND, T>(nrvars, C, PA, i, j, k);
+    return;
+  }
+  View<T> a = qview(C), cq = qcview(C), pq = qview(PA);
+  const T *f1 = C.f, *f2 = C.f + P.fvs;
+  constexpr int NC = ND == 2 ? 4 : 8;
+  T r1[NC], r2[NC];
+#pragma unroll
+  for (int s = 0; s < NC; s++) {
+    int fi = 2 * i - (s & 1), fj = 2 * j - ((s >> 1) & 1);
+    int fk = ND == 3 ? 2 * k - ((s >> 2) & 1) : 0;
+    long long cidx = a.at(fi, fj, fk);
+    T base = DEF ? C.qb[cidx] : (T)0;
+    T o1, o2;
+    op_at<ND, T, DEF>(prm, a, base, P.dx, cidx, o1, o2);
+    long long fo = fat(P, fi, fj, ND == 3 ? fk : 1);
+    r1[s] = f1[fo] - o1;
+    r2[s] = f2[fo] - o2;
+  }
+  T R1, R2;
+  if constexpr (ND == 2) {
+    R1 = (T)0.25 * (((r1[0] + r1[1]) + r1[2]) + r1[3]);
+    R2 = (T)0.25 * (((r2[0] + r2[1]) + r2[2]) + r2[3]);
+  } else {
+    R1 = (T)0.125 * (((((((r1[0] + r1[1]) + r1[2]) + r1[3]) + r1[4]) + r1[5]) + r1[6]) + r1[7]);
+    R2 = (T)0.125 * (((((((r2[0] + r2[1]) + r2[2]) + r2[3]) + r2[4]) + r2[5]) + r2[6]) + r2[7]);
+  }
+  int pi = P.mb[0][0] + i - 1, pj = P.mb[1][0] + j - 1,
+      pk = ND == 3 ? P.mb[2][0] + k - 1 : 1;
+  T base = (T)0;
+  if (DEF)
+    base = PA.qb[pi + (long long)PA.P.px * (pj + (long long)PA.P.py * (ND == 3 ? pk : 0))];
+  // the coarse stencil: qc inside the box, the parent on the ring
+  const int di[7] = {0, 1, -1, 0, 0, 0, 0}, dj[7] = {0, 0, 0, 1, -1, 0, 0},
+            dk[7] = {0, 0, 0, 0, 0, 1, -1};
+  T a1[2 * ND + 1], a2[2 * ND + 1];
+#pragma unroll
+  for (int s = 0; s < 2 * ND + 1; s++) {
+    int ci = i + di[s], cj = j + dj[s], ck = ND == 3 ? k + dk[s] : 0;
+    if (on_ring(ci, cj, ck)) {
+      long long pc = pq.at(P.mb[0][0] - 1 + ci, P.mb[1][0] - 1 + cj,
+                           ND == 3 ? P.mb[2][0] - 1 + ck : 0);
+      a1[s] = pq.p[pc];
+      a2[s] = pq.p[pc + pq.vs];
+    } else {
+      long long cc = cq.at(ci, cj, ck);
+      a1[s] = cq.p[cc];
+      a2[s] = cq.p[cc + cq.vs];
+    }
+  }
+  T c1, c2;
+  op_vals<ND, T, DEF>(prm, base, PA.P.dx, a1, a2, c1, c2);
+  long long po = fat(PA.P, pi, pj, pk);
+  PA.f[po] = R1 + c1;
+  PA.f[po + PA.P.fvs] = R2 + c2;
+}
+
 // CorrectFine (afasroutines.f90:842-963, mbc = 1): q += P1(parent q - qc) at
 // fine cell (i,j,k) of 0..mx+1, the separable (3,1)/4 prolongation of
 // BiLinProlongationP1 / TriLinProlongationP1 (gridutilities.f90:137-175,
@@ -481,8 +565,20 @@ __global__ __launch_bounds__(256) void k_ghost(Dev d, const GhostOp *ops, int n)
   ghost_op<T, DEF>(d.nrvars, fields<T>(d).q, ops[e]);
 }
 
+// A SetGhost program whose reads are disjoint from what the host kernel
+// writes, applied by the host kernel's threads (grid-stride) before their own
+// work; no thread of the host kernel reads what it writes.
+template <class T>
+__device__ __forceinline__ void ghost_alongside(const Dev &d, const GhostOp *ops, int n) {
+  T *q = fields<T>(d).q;
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x)
+    ghost_op<T, std::is_same<T, float>::value>(d.nrvars, q, ops[e]);
+}
+
 template <int ND, class T>
-__global__ __launch_bounds__(256) void k_restrict(Dev d, const Tile *tiles) {
+__global__ __launch_bounds__(256) void k_restrict(Dev d, const Tile *tiles, const GhostOp *gops,
+                                                  int ng) {
+  ghost_alongside<T>(d, gops, ng);
   Tile t = tiles[blockIdx.x];
   Lvl<T> C = level_ref<T>(d, t.patch);
   int i, j, k;
@@ -492,7 +588,9 @@ __global__ __launch_bounds__(256) void k_restrict(Dev d, const Tile *tiles) {
 }
 
 template <int ND, class T>
-__global__ __launch_bounds__(256) void k_filldown(Dev d, const Tile *tiles, int field) {
+__global__ __launch_bounds__(256) void k_filldown(Dev d, const Tile *tiles, int field,
+                                                  const GhostOp *gops, int ng) {
+  ghost_alongside<T>(d, gops, ng);
   Tile t = tiles[blockIdx.x];
   Lvl<T> C = level_ref<T>(d, t.patch);
   int i, j, k;
@@ -520,6 +618,17 @@ __global__ __launch_bounds__(256) void k_coarse_load(Dev d, const Tile *tiles) {
   tile_cell<ND>(d, t, i, j, k);
   if (!in_box<ND>(i, j, k, C.P.cmx, C.P.cmy, C.P.cmz)) return;
   coarse_load_cell<ND, T, DEF>(d.prm, C, level_ref<T>(d, C.P.parent), i, j, k);
+}
+
+// Tiles cover the ghost-extended coarse box 0..cmx+1.
+template <int ND, class T, bool DEF>
+__global__ __launch_bounds__(256) void k_coarse_ghost_load(Dev d, const Tile *tiles) {
+  Tile t = tiles[blockIdx.x];
+  Lvl<T> C = level_ref<T>(d, t.patch);
+  int i, j, k;
+  tile_cell<ND>(d, t, i, j, k);
+  if (!in_box<ND>(i, j, k, C.P.cmx + 1, C.P.cmy + 1, C.P.cmz + 1)) return;
+  coarse_ghost_load_cell<ND, T, DEF>(d.prm, d.nrvars, C, level_ref<T>(d, C.P.parent), i, j, k);
 }
 
 // Tiles cover the ghost-extended fine box 0..mx+1.
@@ -922,17 +1031,17 @@ hipError_t launch_ghost(int prec, const Dev &d, const GhostOp *ops, int n, hipSt
 }
 
 hipError_t launch_restrict(int nd, int prec, const Dev &d, const Tile *tiles, int nt,
-                           hipStream_t s) {
+                           const GhostOp *gops, int ng, hipStream_t s) {
   if (nt <= 0) return hipSuccess;
-  BSAM_LAUNCH_ND_T(k_restrict, nd, prec, grid1(nt), 256, d, tiles);
+  BSAM_LAUNCH_ND_T(k_restrict, nd, prec, grid1(nt), 256, d, tiles, gops, ng);
   return hipGetLastError();
 }
 
 hipError_t launch_filldown(int nd, const Dev &d, const Tile *tiles, int nt, int field,
-                           hipStream_t s) {
+                           const GhostOp *gops, int ng, hipStream_t s) {
   if (nt <= 0) return hipSuccess;
-  if (nd == 2) k_filldown<2, double><<<grid1(nt), 256, 0, s>>>(d, tiles, field);
-  else k_filldown<3, double><<<grid1(nt), 256, 0, s>>>(d, tiles, field);
+  if (nd == 2) k_filldown<2, double><<<grid1(nt), 256, 0, s>>>(d, tiles, field, gops, ng);
+  else k_filldown<3, double><<<grid1(nt), 256, 0, s>>>(d, tiles, field, gops, ng);
   return hipGetLastError();
 }
 
@@ -947,6 +1056,13 @@ hipError_t launch_coarse_load(int nd, int prec, const Dev &d, const Tile *tiles,
                               hipStream_t s) {
   if (nt <= 0) return hipSuccess;
   BSAM_LAUNCH_ND_PREC(k_coarse_load, nd, prec, grid1(nt), 256, d, tiles);
+  return hipGetLastError();
+}
+
+hipError_t launch_coarse_ghost_load(int nd, int prec, const Dev &d, const Tile *tiles, int nt,
+                                   hipStream_t s) {
+  if (nt <= 0) return hipSuccess;
+  BSAM_LAUNCH_ND_PREC(k_coarse_ghost_load, nd, prec, grid1(nt), 256, d, tiles);
   return hipGetLastError();
 }
 

@@ -13,14 +13,21 @@ hipError_t launch_relax(int nd, int prec, const Dev &d, const Tile *tiles, int n
 hipError_t launch_relax_fused(int nd, int prec, const Dev &d, const Tile *tiles, int nt,
                               int color, const int *start, const GhostOp *ops, hipStream_t s);
 hipError_t launch_ghost(int prec, const Dev &d, const GhostOp *ops, int n, hipStream_t s);
+// Restriction / fill-down of one level; gops[0..ng) is a SetGhost program of
+// that level applied in the same launch (its reads must be disjoint from the
+// cells the launch writes; ng = 0 for none).
 hipError_t launch_restrict(int nd, int prec, const Dev &d, const Tile *tiles, int nt,
-                           hipStream_t s);
+                           const GhostOp *gops, int ng, hipStream_t s);
 hipError_t launch_filldown(int nd, const Dev &d, const Tile *tiles, int nt, int field,
-                           hipStream_t s);
+                           const GhostOp *gops, int ng, hipStream_t s);
 hipError_t launch_coarse_ghost(int nd, int prec, const Dev &d, const Tile *tiles, int nt,
                                hipStream_t s);
 hipError_t launch_coarse_load(int nd, int prec, const Dev &d, const Tile *tiles, int nt,
                               hipStream_t s);
+// GetCoarseGhostPoints + CoarseLoadingFunction in one launch over the
+// ghost-extended coarse tiles.
+hipError_t launch_coarse_ghost_load(int nd, int prec, const Dev &d, const Tile *tiles, int nt,
+                                   hipStream_t s);
 hipError_t launch_correct(int nd, int prec, const Dev &d, const Tile *tiles, int nt,
                           hipStream_t s);
 hipError_t launch_source(int nd, const Dev &d, const Tile *tiles, int nt, int full,

@@ -96,6 +96,7 @@ def lib():
         L.bsam_ghost_program_apply_host.argtypes = [vp, C.c_int, C.c_int,
                                                     C.POINTER(C.POINTER(C.c_double))]
         L.bsam_fused_ghost_size.argtypes = [vp, C.c_int, C.c_int, C.c_int]
+        L.bsam_ghost_rides_down.argtypes = [vp, C.c_int, C.c_int]
         L.bsam_fused_ghost_apply_host.argtypes = [vp, C.c_int, C.c_int, C.c_int,
                                                   C.POINTER(C.POINTER(C.c_double))]
         _lib = L
@@ -284,6 +285,11 @@ class Engine:
         """Updates of SetGhost(level, ipass) applied inside the relaxation of
         `color`; -1 when that transition is not fused."""
         return int(lib().bsam_fused_ghost_size(self.ctx, level, ipass, color))
+
+    def ghost_rides_down(self, level: int, ipass: int) -> int:
+        """1 if SetGhost(level, ipass) runs inside the level's restriction /
+        fill-down launch."""
+        return int(lib().bsam_ghost_rides_down(self.ctx, level, ipass))
 
     def fused_ghost_apply_host(self, level: int, ipass: int, color: int,
                                qs: List[np.ndarray]):

@@ -199,6 +199,10 @@ int bsam_ghost_program_dst(const bsam_ctx *ctx, int level, int ipass, int *out,
  * application to host arrays. Updates the relaxation does not read and the
  * next SetGhost(level, color) rewrites are left out. */
 int bsam_fused_ghost_size(const bsam_ctx *ctx, int level, int ipass, int color);
+/* 1 if SetGhost(level, ipass) reads none of the parent cells that the
+ * level's restriction and fill-down write, so the engine runs it in the same
+ * launch as those; 0 if not; -1 for bad arguments. */
+int bsam_ghost_rides_down(const bsam_ctx *ctx, int level, int ipass);
 int bsam_fused_ghost_apply_host(const bsam_ctx *ctx, int level, int ipass, int color,
                                 double *const *q_patches);
 /* Apply a ghost-fill program on the host to per-patch q arrays (reference

@@ -74,3 +74,39 @@ def test_fused_plans_cover_the_2d_levels():
                 assert 0 <= n <= full, (name, level, ipass, color, n, full)
                 dropped += full - n
             assert dropped > 0, (name, level)
+
+
+@pytest.mark.parametrize("name", sorted(LAYOUTS))
+def test_ghost_riding_restriction_commutes(name):
+    """Where the engine runs SetGhost(level, p) in the same launch as the
+    level's restriction, the two must commute: SetGhost then RestrictSolution
+    and RestrictSolution then SetGhost leave every array bit-identical (the
+    program reads none of the parent cells the restriction writes, and the
+    restriction reads no ghost)."""
+    lay = LAYOUTS[name]()
+    eng = Engine(lay, device=-1, **KW)
+    checked = 0
+    for level in range(lay.minlevel + 1, lay.finestlevel + 1):
+        for p in (0, 1, 2):
+            if eng.ghost_rides_down(level, p) != 1:
+                continue
+            outs = []
+            for order in ("ghost_first", "restrict_first"):
+                ora = orc.Oracle(lay, **KW)
+                rng = np.random.default_rng((level - lay.minlevel) * 3 + p)
+                for n, pt in enumerate(ora.patches):
+                    for f in ("q", "qc"):
+                        ora.field(n, f)[...] = rng.uniform(-1, 1, lay.field_shape(pt, f))
+                if order == "ghost_first":
+                    ora.set_ghost(level, p)
+                    ora.restrict(level)
+                else:
+                    ora.restrict(level)
+                    ora.set_ghost(level, p)
+                outs.append([(ora.field(n, "q").copy(), ora.field(n, "qc").copy())
+                             for n in range(len(ora.patches))])
+            for (qa, ca), (qb, cb) in zip(*outs):
+                assert np.array_equal(qa, qb) and np.array_equal(ca, cb), (level, p)
+            checked += 1
+    if lay.finestlevel >= 1 and lay.ndims == 2:
+        assert checked > 0
