@@ -22,6 +22,9 @@ struct DPatch {
   long long qc;            // offset of qc(0,0,k0,1) in the qc pool
   long long qvs, fvs, cvs; // variable strides of q, f and qc
   double dx;
+  // operator constants of this grid, computed on the host exactly as the
+  // reference does per call (h2 = dx*dx): dt/h2, eps2/h2, (2*ndims*eps2)/h2
+  double dt_h2, eps2_h2, lap_h2;
 };
 
 // One ghost-cell update of a SetGhost program. Every source is a value that
@@ -54,6 +57,14 @@ struct TailLevel {
   int ngp[3];
   const GhostOp *gp[3];
   int lds[2][4];
+};
+
+// One array the tail kernel stages between the global pools and LDS:
+// `n` elements at `glob` and at LDS byte offset `lds`; `start` is the
+// array's first index in the flattened copy (prefix sum of n over the table).
+struct TailCopy {
+  void *glob;
+  int lds, n, start, pad;
 };
 
 struct Problem {

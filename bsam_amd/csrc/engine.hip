@@ -139,6 +139,7 @@ struct bsam_ctx {
   // hipGraph replay of the multigrid iteration
   // tile lists spanning levels: interior and ghost-extended tiles of all
   // levels, interior tiles of levels 0..finest (source)
+  double *f_saved = nullptr;   // f after the solve's source set-up (constant per solve)
   std::vector<Tile> all_int, all_ext, src_int;
   Tile *d_all_int = nullptr, *d_all_ext = nullptr, *d_src_int = nullptr;
   long long src_cells = 0;
@@ -146,6 +147,10 @@ struct bsam_ctx {
   int tail_threads = 1024;
   long long tail_lds_cap = -1;      // LDS bytes the tail may use (-1: device limit)
   size_t tail_lds[2] = {0, 0};      // dynamic LDS of the tail kernels per field set
+  // staging tables of the tail (per field set; entries ordered by level) and,
+  // per number of levels run, how many entries those levels own
+  TailCopy *d_tcin[2] = {nullptr, nullptr}, *d_tcout[2] = {nullptr, nullptr};
+  std::vector<int> tcin_n[2], tcout_n[2];
   int use_graph = 1;
   int fuse_ghost = 1;               // apply SetGhost inside the next relaxation
   int tail_top = 0;                 // highest level run by the tail kernels
@@ -204,11 +209,16 @@ void bsam_ctx::free_device() {
   F(e); F(qce); F(fe); F(qb);
   e = qce = fe = qb = nullptr;
   F(d_state);
+  F(f_saved);
   F(d_all_int);
   F(d_all_ext);
   F(d_src_int);
   F(d_tail);
   d_tail = nullptr;
+  for (int st = 0; st < 2; st++) {
+    F(d_tcin[st]);
+    F(d_tcout[st]);
+  }
   drop_graphs();
   q = qold = qc = f = rf = ftmp = nullptr;
   d_P = nullptr; d_partial = nullptr; d_hist = nullptr; d_state = nullptr;
@@ -304,6 +314,9 @@ static bool plan_tail(bsam_ctx *c, long long budget, hipStream_t s) {
   std::vector<TailLevel> tl;
   long long used[2] = {0, 0};
   bool open[2][4] = {{true, true, true, true}, {true, true, true, true}};
+  std::vector<TailCopy> cin[2], cout[2];
+  std::vector<int> nin[2], nout[2];
+  const int maxseg = tail_max_segments();
   for (int l = c->cfg.minlevel; l <= c->tail_top; l++) {
     LevelData &L = c->lev[l - minl];
     const DPatch &D = c->dp[L.patches[0]];
@@ -322,13 +335,32 @@ static bool plan_tail(bsam_ctx *c, long long budget, hipStream_t s) {
       for (int a : {0, 1, 3, 2}) {
         long long b = (bytes[a] + 15) & ~15LL;
         if (!usable || b == 0 || !open[set][a]) continue;
-        if (used[set] + b > budget) {
+        if (used[set] + b > budget || (int)cin[set].size() >= maxseg) {
           open[set][a] = false;   // larger levels would not fit either
           continue;
         }
         t.lds[set][a] = (int)used[set];
         used[set] += b;
+        // staging entry: pool address of the array
+        char *pool = nullptr;
+        long long off = 0;
+        if (set == 0) {
+          pool = a == 0 ? (char *)c->q : a == 1 ? (char *)c->qc : (char *)c->f;
+        } else {
+          pool = a == 0 ? (char *)c->e : a == 1 ? (char *)c->qce : a == 2 ? (char *)c->fe
+                                                                          : (char *)c->qb;
+        }
+        off = a == 0 || a == 3 ? D.q : a == 1 ? D.qc : D.f;
+        TailCopy tc{pool + off * sz, t.lds[set][a], (int)(bytes[a] / sz), 0, 0};
+        auto add = [&](std::vector<TailCopy> &v) {
+          tc.start = v.empty() ? 0 : v.back().start + v.back().n;
+          v.push_back(tc);
+        };
+        add(cin[set]);
+        if (a != 3) add(cout[set]);
       }
+      nin[set].push_back((int)cin[set].size());
+      nout[set].push_back((int)cout[set].size());
     }
     tl.push_back(t);
   }
@@ -341,6 +373,14 @@ static bool plan_tail(bsam_ctx *c, long long budget, hipStream_t s) {
   c->tail_lds[1] = (size_t)used[1];
   if (c->d_tail) (void)hipFree(c->d_tail);
   c->d_tail = dev_upload(tl, s);
+  for (int set = 0; set < 2; set++) {
+    if (c->d_tcin[set]) (void)hipFree(c->d_tcin[set]);
+    if (c->d_tcout[set]) (void)hipFree(c->d_tcout[set]);
+    c->d_tcin[set] = dev_upload(cin[set], s);
+    c->d_tcout[set] = dev_upload(cout[set], s);
+    c->tcin_n[set] = nin[set];
+    c->tcout_n[set] = nout[set];
+  }
   return true;
 }
 
@@ -430,6 +470,12 @@ static void commit(bsam_ctx *c) {
     D.qc = c->nc; D.cvs = cx * cy * cz;
     D.f = c->nf; D.fvs = fn;
     D.dx = h.dx;
+    {
+      double h2 = h.dx * h.dx;   // as RelaxGrid / Operator compute it per call
+      D.dt_h2 = c->cfg.dt / h2;
+      D.eps2_h2 = c->cfg.eps2 / h2;
+      D.lap_h2 = (nd == 2 ? 4.0 : 6.0) * c->cfg.eps2 / h2;
+    }
     h.qoff = D.q; h.qvs = D.qvs; h.px = ex; h.py = ey;
     c->nq += D.qvs * nv;
     c->nc += D.cvs * nv;
@@ -484,6 +530,7 @@ static void commit(bsam_ctx *c) {
   HIPCK(hipMalloc(&c->f, sizeof(double) * c->nf));
   HIPCK(hipMalloc(&c->rf, sizeof(double) * c->nf));
   HIPCK(hipMalloc(&c->ftmp, sizeof(double) * c->nf));
+  HIPCK(hipMalloc(&c->f_saved, sizeof(double) * c->nf));
   HIPCK(hipMemsetAsync(c->q, 0, sizeof(double) * c->nq, s));
   HIPCK(hipMemsetAsync(c->qold, 0, sizeof(double) * c->nq, s));
   HIPCK(hipMemsetAsync(c->qc, 0, sizeof(double) * c->nc, s));
@@ -536,7 +583,8 @@ static void commit(bsam_ctx *c) {
     int maxlds = 0;
     HIPCK(hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock,
                                 c->cfg.device));
-    long long budget = c->tail_lds_cap < 0 ? maxlds : std::min<long long>(c->tail_lds_cap, maxlds);
+    long long avail = maxlds - (long long)tail_static_lds();
+    long long budget = c->tail_lds_cap < 0 ? avail : std::min<long long>(c->tail_lds_cap, avail);
     // if the runtime refuses the planned dynamic LDS, plan again within the
     // default 64 KiB, then without LDS
     for (long long b : {budget, std::min<long long>(budget, 65536), 0LL}) {
@@ -700,9 +748,11 @@ static int op_vcycle(bsam_ctx *c, int level, int prec = PREC_SOLUTION_FP64,
                      bool defer_final = false) {
   if (level <= c->tail_top) {
     ProfScope ps(c, BSAM_PROF_TAIL, 0.0);
-    HIPCK(launch_tail_vcycle(c->nd(), prec, c->dev(), c->d_tail, level - c->cfg.minlevel + 1,
-                             c->cfg.nsmoothingpasses, c->tail_threads,
-                             c->tail_lds[prec == PREC_SOLUTION_FP64 ? 0 : 1], c->stream));
+    int nlev = level - c->cfg.minlevel + 1, set = prec == PREC_SOLUTION_FP64 ? 0 : 1;
+    HIPCK(launch_tail_vcycle(c->nd(), prec, c->dev(), c->d_tail, nlev, c->cfg.nsmoothingpasses,
+                             c->tail_threads, c->tail_lds[set], c->d_tcin[set],
+                             c->tcin_n[set][nlev - 1], c->d_tcout[set],
+                             c->tcout_n[set][nlev - 1], c->stream));
     return -1;
   }
   if (level == c->cfg.minlevel) return level_relax(c, level, prec, false, defer_final);
@@ -786,15 +836,20 @@ static void multigrid_iterations(bsam_ctx *c, int *ncycles, double *hist, int ma
   op_fill_down(c, BSAM_FIELD_Q);
   op_source(c, 1);
   op_fill_down(c, BSAM_FIELD_F);
+  // The loop's source restore and FillDown(f) (afasroutines.f90:84-90) rebuild
+  // f on every level from ftmp alone: UpdateSourceFunction adds 0 for this
+  // problem (f = ftmp + 0 on levels >= 0) and FillDown(f) then overwrites
+  // every covered cell by restriction. Their result is the same each
+  // iteration, so it is computed once here and restored by a copy.
+  HIPCK(launch_copy(c->f_saved, c->f, c->nf, c->stream));
   reset_state(c);
   // one iteration of the vcycleloop (afasroutines.f90:69-104): the V-cycle,
-  // FillDown(q), source restore, FillDown(f) and the residual norm
+  // FillDown(q), source restore + FillDown(f) (the copy), the residual norm
   int done = 0;
   int n = iterate(c, &c->graph, &c->graph_exec, [&]() {
     int pend = op_vcycle(c, c->finest, PREC_SOLUTION_FP64, true);
     op_fill_down(c, BSAM_FIELD_Q, pend);
-    op_source(c, 0);
-    op_fill_down(c, BSAM_FIELD_F);
+    HIPCK(launch_copy(c->f, c->f_saved, c->nf, c->stream));
     queue_error(c, c->finest, c->cfg.qerrortol);
   }, &done);
   report(c, n, done, ncycles, hist, maxhist, converged);

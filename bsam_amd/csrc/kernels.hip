@@ -96,17 +96,16 @@ __device__ __forceinline__ long long fat(const DPatch &P, int i, int j, int k) {
   return (i - 1) + (long long)P.mx * ((j - 1) + (long long)P.my * (k - 1));
 }
 
-// Operator2D/3D (problem.f90:231-257, :534-561) at cell c of view a with
-// spacing hh: r1 = phi - dt/h^2 ULapMob(phi,mu), r2 = mu - Dfc(phi) +
+// Operator2D/3D (problem.f90:231-257, :534-561) at cell c of view a on grid
+// G: r1 = phi - dt/h^2 ULapMob(phi,mu), r2 = mu - Dfc(phi) +
 // eps^2/h^2 ULap(phi), with the undivided stencils of gridutilities.f90.
 // Defect form: Dfc(phi) becomes Dfc(base + phi) - Dfc(base).
 // The same from gathered values: a1/a2 hold phi/mu at the cell, +x, -x, +y,
 // -y (+z, -z).
 template <int ND, class T, bool DEF>
-__device__ __forceinline__ void op_vals(const Problem &prm, T base, double hh, const T *a1,
+__device__ __forceinline__ void op_vals(const Problem &prm, T base, const DPatch &G, const T *a1,
                                         const T *a2, T &r1, T &r2) {
-  double h2 = hh * hh;
-  T tmp1 = (T)(prm.dt / h2), tmp2 = (T)(prm.eps2 / h2);
+  T tmp1 = (T)G.dt_h2, tmp2 = (T)G.eps2_h2;
   T phi = a1[0], mu = a2[0];
   T lapmob, lap;
   if (ND == 2) {
@@ -122,8 +121,8 @@ __device__ __forceinline__ void op_vals(const Problem &prm, T base, double hh, c
   else r2 = (mu - (T)dfc(prm, (double)phi)) + tmp2 * lap;
 }
 template <int ND, class T, bool DEF>
-__device__ __forceinline__ void op_at(const Problem &prm, const View<T> &a, T base, double hh,
-                                      long long c, T &r1, T &r2) {
+__device__ __forceinline__ void op_at(const Problem &prm, const View<T> &a, T base,
+                                      const DPatch &G, long long c, T &r1, T &r2) {
   const T *q1 = a.p, *q2 = a.p + a.vs;
   long long off[7] = {0, 1, -1, a.px, -a.px, a.pl, -a.pl};
   T a1[2 * ND + 1], a2[2 * ND + 1];
@@ -132,7 +131,7 @@ __device__ __forceinline__ void op_at(const Problem &prm, const View<T> &a, T ba
     a1[s] = q1[c + off[s]];
     a2[s] = q2[c + off[s]];
   }
-  op_vals<ND, T, DEF>(prm, base, hh, a1, a2, r1, r2);
+  op_vals<ND, T, DEF>(prm, base, G, a1, a2, r1, r2);
 }
 
 // Pointwise nonlinear Gauss-Seidel update of one cell: one Newton step on the
@@ -146,9 +145,7 @@ __device__ __forceinline__ void relax_cell_f(const Problem &prm, const Lvl<T> &L
   View<T> a = qview(L);
   long long c = a.at(i, j, k);
   T *q1 = a.p, *q2 = a.p + a.vs;
-  double h2 = P.dx * P.dx;
-  T tmp1 = (T)(prm.dt / h2), tmp2 = (T)(prm.eps2 / h2);
-  T tmp3 = (T)((ND == 2 ? 4.0 : 6.0) * prm.eps2 / h2);
+  T tmp1 = (T)P.dt_h2, tmp2 = (T)P.eps2_h2, tmp3 = (T)P.lap_h2;
   T phi = q1[c];
   T dc, d2;
   if (DEF) {
@@ -326,7 +323,7 @@ __device__ __forceinline__ void coarse_load_cell(const Problem &prm, const Lvl<T
     long long cidx = a.at(fi, fj, fk);
     T base = DEF ? C.qb[cidx] : (T)0;
     T o1, o2;
-    op_at<ND, T, DEF>(prm, a, base, P.dx, cidx, o1, o2);
+    op_at<ND, T, DEF>(prm, a, base, P, cidx, o1, o2);
     long long fo = fat(P, fi, fj, ND == 3 ? fk : 1);
     r1[s] = f1[fo] - o1;
     r2[s] = f2[fo] - o2;
@@ -345,7 +342,7 @@ __device__ __forceinline__ void coarse_load_cell(const Problem &prm, const Lvl<T
   if (DEF)
     base = PA.qb[pi + (long long)PA.P.px * (pj + (long long)PA.P.py * (ND == 3 ? pk : 0))];
   T c1, c2;
-  op_at<ND, T, DEF>(prm, cq, base, PA.P.dx, cq.at(i, j, k), c1, c2);
+  op_at<ND, T, DEF>(prm, cq, base, PA.P, cq.at(i, j, k), c1, c2);
   long long po = fat(PA.P, pi, pj, pk);
   PA.f[po] = R1 + c1;
   PA.f[po + PA.P.fvs] = R2 + c2;
@@ -379,7 +376,7 @@ __device__ __forceinline__ void coarse_ghost_load_cell(const Problem &prm, int n
     long long cidx = a.at(fi, fj, fk);
     T base = DEF ? C.qb[cidx] : (T)0;
     T o1, o2;
-    op_at<ND, T, DEF>(prm, a, base, P.dx, cidx, o1, o2);
+    op_at<ND, T, DEF>(prm, a, base, P, cidx, o1, o2);
     long long fo = fat(P, fi, fj, ND == 3 ? fk : 1);
     r1[s] = f1[fo] - o1;
     r2[s] = f2[fo] - o2;
@@ -416,7 +413,7 @@ __device__ __forceinline__ void coarse_ghost_load_cell(const Problem &prm, int n
     }
   }
   T c1, c2;
-  op_vals<ND, T, DEF>(prm, base, PA.P.dx, a1, a2, c1, c2);
+  op_vals<ND, T, DEF>(prm, base, PA.P, a1, a2, c1, c2);
   long long po = fat(PA.P, pi, pj, pk);
   PA.f[po] = R1 + c1;
   PA.f[po + PA.P.fvs] = R2 + c2;
@@ -492,7 +489,7 @@ __device__ __forceinline__ void defect_setup_cell(const Dev &d, const DPatch &P,
                (ND == 2 || (k >= 1 && k <= P.mz));
   if (!inner) return;
   double o1, o2;
-  op_at<ND, double, false>(d.prm, a, 0.0, P.dx, c, o1, o2);
+  op_at<ND, double, false>(d.prm, a, 0.0, P, c, o1, o2);
   long long fo = P.f + fat(P, i, j, ND == 3 ? k : 1);
   d.fe[fo] = (float)(d.f[fo] - o1);
   d.fe[fo + P.fvs] = (float)(d.f[fo + P.fvs] - o2);
@@ -712,29 +709,47 @@ __device__ __forceinline__ Lvl<T> tail_ref(const Dev &d, const TailLevel &t, int
   return L;
 }
 
-template <class T>
-__device__ __forceinline__ void copy_n(T *dst, const T *src, long long n) {
-  for (long long e = threadIdx.x; e < n; e += blockDim.x) dst[e] = src[e];
-}
+// Moves arrays between the global pools and LDS as listed by a host-built
+// table (the tail levels' LDS-resident arrays: q, qc, f, qb in; q, qc, f
+// out). The table goes to LDS first; the arrays are then copied as one
+// flattened index space with eight independent loads per thread in flight,
+// a few memory round trips in all. Ends with a barrier.
+constexpr int TAIL_MAXSEG = 96;
+__shared__ TailCopy tail_seg[TAIL_MAXSEG];
 
-// Moves the LDS-resident arrays of levels 0..nlev-1 between LDS and the
-// global pools (in: q, qc, f, qb; out: q, qc, f). Ends with a barrier.
 template <class T>
-__device__ void tail_stage(const Dev &d, const TailLevel *lv, int nlev, int set, bool in,
-                           int nrvars) {
-  for (int l = 0; l < nlev; l++) {
-    Lvl<T> G = level_ref<T>(d, lv[l].patch);
-    const int *o = lv[l].lds[set];
-    const DPatch &P = G.P;
-    T *ptr[3] = {G.q, G.qc, G.f};
-    long long n[3] = {P.qvs * nrvars, P.cvs * nrvars, P.fvs * nrvars};
-    for (int a = 0; a < 3; a++) {
-      if (o[a] < 0) continue;
-      T *sm = (T *)(tail_lds + o[a]);
-      if (in) copy_n(sm, ptr[a], n[a]);
-      else copy_n(ptr[a], sm, n[a]);
+__device__ void tail_stage(const TailCopy *tab, int ns, bool in) {
+  for (int e = threadIdx.x; e < ns; e += blockDim.x) tail_seg[e] = tab[e];
+  __syncthreads();
+  if (ns > 0) {
+    const int total = tail_seg[ns - 1].start + tail_seg[ns - 1].n;
+    constexpr int U = 8;
+    const int step = blockDim.x;
+    int seg = 0;   // indices ascend within and across batches
+    for (int e0 = threadIdx.x; e0 < total; e0 += U * step) {
+      T v[U];
+      int sg[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        int e = e0 + u * step;
+        sg[u] = -1;
+        if (e < total) {
+          while (seg + 1 < ns && tail_seg[seg + 1].start <= e) seg++;
+          sg[u] = seg;
+          const TailCopy &t = tail_seg[seg];
+          v[u] = in ? ((const T *)t.glob)[e - t.start]
+                    : ((const T *)(tail_lds + t.lds))[e - t.start];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++)
+        if (sg[u] >= 0) {
+          const TailCopy &t = tail_seg[sg[u]];
+          int k = e0 + u * step - t.start;
+          if (in) ((T *)(tail_lds + t.lds))[k] = v[u];
+          else ((T *)t.glob)[k] = v[u];
+        }
     }
-    if (in && o[3] >= 0) copy_n((T *)(tail_lds + o[3]), G.qb, P.qvs);
   }
   __syncthreads();
 }
@@ -849,10 +864,11 @@ __device__ void tail_relax(const Problem &prm, int nrvars, const Lvl<T> &L, cons
 // single-patch chain.
 template <int ND, class T, bool DEF>
 __global__ __launch_bounds__(1024) void k_tail_vcycle(Dev d, const TailLevel *lv, int nlev,
-                                                       int nsmooth, int lds) {
+                                                       int nsmooth, int lds, const TailCopy *cin,
+                                                       int nin, const TailCopy *cout, int nout) {
   const int set = DEF ? 1 : 0, nv = d.nrvars;
   const Problem prm = d.prm;
-  if (lds) tail_stage<T>(d, lv, nlev, set, true, nv);
+  if (lds) tail_stage<T>(cin, nin, true);
   for (int l = nlev - 1; l >= 1; l--) {
     Lvl<T> C = tail_ref<T>(d, lv[l], set, lds), PA = tail_ref<T>(d, lv[l - 1], set, lds);
     TailGhost gc;
@@ -890,7 +906,7 @@ __global__ __launch_bounds__(1024) void k_tail_vcycle(Dev d, const TailLevel *lv
     tail_ghost<T, DEF>(nv, C, gc.p0);
     tail_relax<ND, T, DEF>(prm, nv, C, gc, nsmooth);
   }
-  if (lds) tail_stage<T>(d, lv, nlev, set, false, nv);
+  if (lds) tail_stage<T>(cout, nout, false);
 }
 
 // FillDown (afasroutines.f90:223-239) from level nlev-1 down to 0 of the
@@ -946,7 +962,7 @@ __global__ __launch_bounds__(256) void k_resnorm(Dev d, const Tile *tiles, doubl
       Lvl<double> L{d.q + P.q, nullptr, nullptr, nullptr, P};
       View<double> a = qview(L);
       double o1, o2;
-      op_at<ND, double, false>(d.prm, a, 0.0, P.dx, a.at(i, j, k), o1, o2);
+      op_at<ND, double, false>(d.prm, a, 0.0, P, a.at(i, j, k), o1, o2);
       r1 = r1 - o1;
       r2 = r2 - o2;
     }
@@ -1116,18 +1132,31 @@ hipError_t launch_copy(double *dst, const double *src, long long n, hipStream_t 
 }
 
 hipError_t launch_tail_vcycle(int nd, int prec, const Dev &d, const TailLevel *lv, int nlev,
-                              int nsmooth, int threads, size_t lds, hipStream_t s) {
+                              int nsmooth, int threads, size_t lds, const TailCopy *cin,
+                              int nin, const TailCopy *cout, int nout, hipStream_t s) {
   if (nlev <= 0) return hipSuccess;
+  if (nin > TAIL_MAXSEG || nout > TAIL_MAXSEG) return hipErrorInvalidValue;
   int use = lds > 0;
   if (prec == PREC_SOLUTION_FP64) {
-    if (nd == 2) k_tail_vcycle<2, double, false><<<1, threads, lds, s>>>(d, lv, nlev, nsmooth, use);
-    else k_tail_vcycle<3, double, false><<<1, threads, lds, s>>>(d, lv, nlev, nsmooth, use);
+    if (nd == 2)
+      k_tail_vcycle<2, double, false><<<1, threads, lds, s>>>(d, lv, nlev, nsmooth, use, cin, nin,
+                                                             cout, nout);
+    else
+      k_tail_vcycle<3, double, false><<<1, threads, lds, s>>>(d, lv, nlev, nsmooth, use, cin, nin,
+                                                             cout, nout);
   } else {
-    if (nd == 2) k_tail_vcycle<2, float, true><<<1, threads, lds, s>>>(d, lv, nlev, nsmooth, use);
-    else k_tail_vcycle<3, float, true><<<1, threads, lds, s>>>(d, lv, nlev, nsmooth, use);
+    if (nd == 2)
+      k_tail_vcycle<2, float, true><<<1, threads, lds, s>>>(d, lv, nlev, nsmooth, use, cin, nin,
+                                                           cout, nout);
+    else
+      k_tail_vcycle<3, float, true><<<1, threads, lds, s>>>(d, lv, nlev, nsmooth, use, cin, nin,
+                                                           cout, nout);
   }
   return hipGetLastError();
 }
+
+int tail_max_segments() { return TAIL_MAXSEG; }
+size_t tail_static_lds() { return sizeof(TailCopy) * TAIL_MAXSEG; }
 
 hipError_t launch_tail_filldown(int nd, const Dev &d, const TailLevel *lv, int nlev, int field,
                                 int threads, hipStream_t s) {

@@ -44,7 +44,11 @@ hipError_t launch_copy(double *dst, const double *src, long long n, hipStream_t 
 // single-patch chain; the V-cycle with `lds` bytes of dynamic LDS
 // (TailLevel::lds; 0 keeps every array in the global pools).
 hipError_t launch_tail_vcycle(int nd, int prec, const Dev &d, const TailLevel *lv, int nlev,
-                              int nsmooth, int threads, size_t lds, hipStream_t s);
+                              int nsmooth, int threads, size_t lds, const TailCopy *cin,
+                              int nin, const TailCopy *cout, int nout, hipStream_t s);
+// Capacity of the tail's staging table and the static LDS it takes.
+int tail_max_segments();
+size_t tail_static_lds();
 hipError_t launch_tail_filldown(int nd, const Dev &d, const TailLevel *lv, int nlev, int field,
                                 int threads, hipStream_t s);
 // Raises the dynamic-LDS limit of the tail kernels to `lds` bytes.
