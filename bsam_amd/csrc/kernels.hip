@@ -687,12 +687,20 @@ extern __shared__ __align__(16) unsigned char tail_lds[];
 constexpr int TAIL_GR = 1;   // ghost updates per thread and pass held in registers
 constexpr int TAIL_FR = 2;   // cells per thread and colour whose f is held in registers
 
+// ceil(log2(x)) for x >= 1 (uniform values: scalar work).
+__device__ __forceinline__ int clog2(int x) { return x <= 1 ? 0 : 32 - __builtin_clz(x - 1); }
+
+// Calls f(i, j, k) for the cells of an ex x ey (x ez) box (lower corner
+// `base`), spread over the workgroup. The box is walked as if padded to
+// powers of two in x and y, so a flat index maps to (i, j, k) with shifts
+// and masks instead of integer divisions; padding lanes do nothing.
 template <int ND, class F>
 __device__ __forceinline__ void for_cells(int ex, int ey, int ez, int base, F f) {
-  int n = ex * ey * ez;
+  const int lx = clog2(ex), ly = clog2(ey);
+  const int n = (ez << (lx + ly));
   for (int e = threadIdx.x; e < n; e += blockDim.x) {
-    int i = e % ex, j = (e / ex) % ey, k = e / (ex * ey);
-    f(i + base, j + base, ND == 3 ? k + base : 0);
+    int i = e & ((1 << lx) - 1), j = (e >> lx) & ((1 << ly) - 1), k = e >> (lx + ly);
+    if (i < ex && j < ey) f(i + base, j + base, ND == 3 ? k + base : 0);
   }
 }
 
@@ -803,18 +811,31 @@ __device__ __forceinline__ void tail_ghost(int nrvars, const Lvl<T> &L, const Ta
   __syncthreads();
 }
 
-// Cell e of colour `color` on a level, colour-major: hx = ceil(mx/2) cells
-// per row, i = i0 + 2t with i0 chosen so that i + j (+ k) + color is odd (the
-// cells RelaxGrid updates in that pass). Returns false past the row's end.
+// Cells of one colour on a level, colour-major: each row holds hx =
+// ceil(mx/2) cells of the colour, i = i0 + 2t with i0 chosen so that
+// i + j (+ k) + color is odd (the cells RelaxGrid updates in that pass). Rows
+// and t are padded to powers of two (lh = log2 of the padded hx, ly of the
+// padded my) so that a flat index e < ncells maps with shifts and masks;
+// color_cell returns false for padding and past the row's end.
+struct ColorMap {
+  int lh, ly, n;
+};
 template <int ND>
-__device__ __forceinline__ bool color_cell(const DPatch &P, int color, int e, int &i, int &j,
-                                           int &k) {
-  int hx = (P.mx + 1) >> 1;
-  int t = e % hx;
-  j = (e / hx) % P.my + 1;
-  k = ND == 3 ? e / (hx * P.my) + 1 : 0;
+__device__ __forceinline__ ColorMap color_map(const DPatch &P) {
+  ColorMap m;
+  m.lh = clog2((P.mx + 1) >> 1);
+  m.ly = clog2(P.my);
+  m.n = (ND == 3 ? P.mz : 1) << (m.lh + m.ly);
+  return m;
+}
+template <int ND>
+__device__ __forceinline__ bool color_cell(const DPatch &P, const ColorMap &m, int color, int e,
+                                           int &i, int &j, int &k) {
+  int t = e & ((1 << m.lh) - 1);
+  j = ((e >> m.lh) & ((1 << m.ly) - 1)) + 1;
+  k = ND == 3 ? (e >> (m.lh + m.ly)) + 1 : 0;
   i = 1 + ((j + k + color) & 1) + 2 * t;
-  return i <= P.mx;
+  return i <= P.mx && j <= P.my;
 }
 
 // nsmooth red-black passes on a tail level. Threads walk the cells of each
@@ -824,7 +845,8 @@ template <int ND, class T, bool DEF>
 __device__ void tail_relax(const Problem &prm, int nrvars, const Lvl<T> &L, const TailGhost &g,
                            int nsmooth) {
   const DPatch &P = L.P;
-  const int n = ((P.mx + 1) >> 1) * P.my * (ND == 3 ? P.mz : 1);
+  const ColorMap m = color_map<ND>(P);
+  const int n = m.n;
   T f1[2][TAIL_FR], f2[2][TAIL_FR];
 #pragma unroll
   for (int c = 0; c < 2; c++)
@@ -832,7 +854,7 @@ __device__ void tail_relax(const Problem &prm, int nrvars, const Lvl<T> &L, cons
     for (int r = 0; r < TAIL_FR; r++) {
       int e = threadIdx.x + r * blockDim.x, i, j, k;
       f1[c][r] = f2[c][r] = (T)0;
-      if (e < n && color_cell<ND>(P, c + 1, e, i, j, k)) {
+      if (e < n && color_cell<ND>(P, m, c + 1, e, i, j, k)) {
         long long fo = fat(P, i, j, ND == 3 ? k : 1);
         f1[c][r] = L.f[fo];
         f2[c][r] = L.f[fo + P.fvs];
@@ -842,13 +864,13 @@ __device__ void tail_relax(const Problem &prm, int nrvars, const Lvl<T> &L, cons
 #pragma unroll
     for (int r = 0; r < TAIL_FR; r++) {
       int e = threadIdx.x + r * blockDim.x, i, j, k;
-      if (e < n && color_cell<ND>(P, c + 1, e, i, j, k))
+      if (e < n && color_cell<ND>(P, m, c + 1, e, i, j, k))
         relax_cell_f<ND, T, DEF>(prm, L, i, j, k, f1[c][r], f2[c][r]);
       __builtin_amdgcn_sched_barrier(0);
     }
     for (int e = threadIdx.x + TAIL_FR * blockDim.x; e < n; e += blockDim.x) {
       int i, j, k;
-      if (color_cell<ND>(P, c + 1, e, i, j, k)) relax_cell<ND, T, DEF>(prm, L, i, j, k);
+      if (color_cell<ND>(P, m, c + 1, e, i, j, k)) relax_cell<ND, T, DEF>(prm, L, i, j, k);
     }
     __syncthreads();
   };
