@@ -313,7 +313,7 @@ static bool plan_tail(bsam_ctx *c, long long budget, hipStream_t s) {
   const int minl = c->cfg.minlevel, nv = c->cfg.nrvars;
   std::vector<TailLevel> tl;
   long long used[2] = {0, 0};
-  bool open[2][4] = {{true, true, true, true}, {true, true, true, true}};
+  bool open[2] = {true, true};   // no level has failed to fit yet
   std::vector<TailCopy> cin[2], cout[2];
   std::vector<int> nin[2], nout[2];
   const int maxseg = tail_max_segments();
@@ -332,13 +332,19 @@ static bool plan_tail(bsam_ctx *c, long long budget, hipStream_t s) {
                             set == 1 ? D.qvs * sz : 0};
       for (int a = 0; a < 4; a++) t.lds[set][a] = -1;
       bool usable = set == 0 || c->cfg.precision == BSAM_PREC_MIXED;
+      // a level is in LDS when q, qc (and the FP32 base point) all fit, its f
+      // as well if room remains; levels in LDS form a prefix of the chain
+      long long core = ((bytes[0] + 15) & ~15LL) + ((bytes[1] + 15) & ~15LL) +
+                       ((bytes[3] + 15) & ~15LL);
+      int core_n = set == 1 ? 3 : 2;
+      bool fits = usable && open[set] && used[set] + core <= budget &&
+                  (int)cin[set].size() + core_n + 1 <= maxseg;
+      if (!fits) open[set] = false;
       for (int a : {0, 1, 3, 2}) {
+        if (!fits) break;
         long long b = (bytes[a] + 15) & ~15LL;
-        if (!usable || b == 0 || !open[set][a]) continue;
-        if (used[set] + b > budget || (int)cin[set].size() >= maxseg) {
-          open[set][a] = false;   // larger levels would not fit either
-          continue;
-        }
+        if (b == 0) continue;
+        if (a == 2 && used[set] + b > budget) continue;   // f stays in the pool
         t.lds[set][a] = (int)used[set];
         used[set] += b;
         // staging entry: pool address of the array

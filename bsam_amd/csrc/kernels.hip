@@ -704,16 +704,20 @@ __device__ __forceinline__ void for_cells(int ex, int ey, int ez, int base, F f)
   }
 }
 
-template <class T>
-__device__ __forceinline__ Lvl<T> tail_ref(const Dev &d, const TailLevel &t, int set,
-                                           bool lds) {
+// A tail level's arrays. With IN_LDS the level's q and qc (and the FP32 base
+// point) sit in LDS and are addressed from the LDS base itself, so the
+// compiler emits LDS instructions for them; its f may sit in LDS or in the
+// global pool. Without, every array is used in place in the global pools.
+template <class T, bool IN_LDS>
+__device__ __forceinline__ Lvl<T> tail_ref(const Dev &d, const TailLevel &t, int set) {
   Lvl<T> L = level_ref<T>(d, t.patch);
-  if (!lds) return L;
-  const int *o = t.lds[set];
-  if (o[0] >= 0) L.q = (T *)(tail_lds + o[0]);
-  if (o[1] >= 0) L.qc = (T *)(tail_lds + o[1]);
-  if (o[2] >= 0) L.f = (T *)(tail_lds + o[2]);
-  if (o[3] >= 0) L.qb = (const T *)(tail_lds + o[3]);
+  if constexpr (IN_LDS) {
+    const int *o = t.lds[set];
+    L.q = (T *)(tail_lds + o[0]);
+    L.qc = (T *)(tail_lds + o[1]);
+    if (o[2] >= 0) L.f = (T *)(tail_lds + o[2]);
+    if (std::is_same<T, float>::value) L.qb = (const T *)(tail_lds + o[3]);
+  }
   return L;
 }
 
@@ -882,51 +886,78 @@ __device__ void tail_relax(const Problem &prm, int nrvars, const Lvl<T> &L, cons
   }
 }
 
+// The tail's visit of level C = lv[l] on the way down: smoothing, then
+// RestrictSolution, SetGhost(parent, 0), GetCoarseGhostPoints and
+// CoarseLoadingFunction into the parent PA = lv[l - 1].
+template <int ND, class T, bool DEF, bool LC, bool LP>
+__device__ void tail_down(const Dev &d, const TailLevel *lv, int l, int set, int nsmooth) {
+  const int nv = d.nrvars;
+  const Problem prm = d.prm;
+  Lvl<T> C = tail_ref<T, LC>(d, lv[l], set), PA = tail_ref<T, LP>(d, lv[l - 1], set);
+  TailGhost gc;
+  TailPass gp;
+  tail_ghost_load(lv[l], gc);
+  tail_pass_load(lv[l - 1].ngp[0], lv[l - 1].gp[0], gp);
+  const DPatch &P = C.P;
+  int cz = ND == 3 ? P.cmz : 1;
+  tail_relax<ND, T, DEF>(prm, nv, C, gc, nsmooth);
+  for_cells<ND>(P.cmx, P.cmy, cz, 1,
+                [&](int i, int j, int k) { restrict_cell<ND, T>(nv, C, PA, i, j, k); });
+  __syncthreads();
+  tail_ghost<T, DEF>(nv, PA, gp);
+  for_cells<ND>(P.cmx + 2, P.cmy + 2, ND == 3 ? P.cmz + 2 : 1, 0,
+                [&](int i, int j, int k) { coarse_ghost_cell<ND, T>(nv, C, PA, i, j, k); });
+  __syncthreads();
+  for_cells<ND>(P.cmx, P.cmy, cz, 1,
+                [&](int i, int j, int k) { coarse_load_cell<ND, T, DEF>(prm, C, PA, i, j, k); });
+  __syncthreads();
+}
+
+// On the way up: CorrectFine from the parent, SetGhost(C, 0), smoothing.
+template <int ND, class T, bool DEF, bool LC, bool LP>
+__device__ void tail_up(const Dev &d, const TailLevel *lv, int l, int set, int nsmooth) {
+  const int nv = d.nrvars;
+  Lvl<T> C = tail_ref<T, LC>(d, lv[l], set), PA = tail_ref<T, LP>(d, lv[l - 1], set);
+  TailGhost gc;
+  tail_ghost_load(lv[l], gc);
+  const DPatch &P = C.P;
+  for_cells<ND>(P.mx + 2, P.my + 2, ND == 3 ? P.mz + 2 : 1, 0,
+                [&](int i, int j, int k) { correct_cell<ND, T>(nv, C, PA, i, j, k); });
+  __syncthreads();
+  tail_ghost<T, DEF>(nv, C, gc.p0);
+  tail_relax<ND, T, DEF>(d.prm, nv, C, gc, nsmooth);
+}
+
+template <int ND, class T, bool DEF, bool LB>
+__device__ void tail_bottom(const Dev &d, const TailLevel *lv, int set, int nsmooth) {
+  Lvl<T> B = tail_ref<T, LB>(d, lv[0], set);
+  TailGhost gb;
+  tail_ghost_load(lv[0], gb);
+  tail_relax<ND, T, DEF>(d.prm, d.nrvars, B, gb, nsmooth);
+}
+
 // AFASVcycle(top) (afasroutines.f90:488-521) on levels 0..nlev-1 of the
-// single-patch chain.
+// single-patch chain. Levels holding their q in LDS form a prefix of the
+// chain (coarsest first), so a visit pairs LDS/LDS, global/LDS or
+// global/global child and parent.
 template <int ND, class T, bool DEF>
 __global__ __launch_bounds__(1024) void k_tail_vcycle(Dev d, const TailLevel *lv, int nlev,
                                                        int nsmooth, int lds, const TailCopy *cin,
                                                        int nin, const TailCopy *cout, int nout) {
-  const int set = DEF ? 1 : 0, nv = d.nrvars;
-  const Problem prm = d.prm;
+  const int set = DEF ? 1 : 0;
+  auto in_lds = [&](int l) { return lds && lv[l].lds[set][0] >= 0; };
   if (lds) tail_stage<T>(cin, nin, true);
   for (int l = nlev - 1; l >= 1; l--) {
-    Lvl<T> C = tail_ref<T>(d, lv[l], set, lds), PA = tail_ref<T>(d, lv[l - 1], set, lds);
-    TailGhost gc;
-    TailPass gp;
-    tail_ghost_load(lv[l], gc);
-    tail_pass_load(lv[l - 1].ngp[0], lv[l - 1].gp[0], gp);
-    const DPatch &P = C.P;
-    int cz = ND == 3 ? P.cmz : 1;
-    tail_relax<ND, T, DEF>(prm, nv, C, gc, nsmooth);
-    for_cells<ND>(P.cmx, P.cmy, cz, 1,
-                  [&](int i, int j, int k) { restrict_cell<ND, T>(nv, C, PA, i, j, k); });
-    __syncthreads();
-    tail_ghost<T, DEF>(nv, PA, gp);
-    for_cells<ND>(P.cmx + 2, P.cmy + 2, ND == 3 ? P.cmz + 2 : 1, 0,
-                  [&](int i, int j, int k) { coarse_ghost_cell<ND, T>(nv, C, PA, i, j, k); });
-    __syncthreads();
-    for_cells<ND>(P.cmx, P.cmy, cz, 1,
-                  [&](int i, int j, int k) { coarse_load_cell<ND, T, DEF>(prm, C, PA, i, j, k); });
-    __syncthreads();
+    if (in_lds(l)) tail_down<ND, T, DEF, true, true>(d, lv, l, set, nsmooth);
+    else if (in_lds(l - 1)) tail_down<ND, T, DEF, false, true>(d, lv, l, set, nsmooth);
+    else tail_down<ND, T, DEF, false, false>(d, lv, l, set, nsmooth);
   }
-  {
-    Lvl<T> B = tail_ref<T>(d, lv[0], set, lds);
-    TailGhost gb;
-    tail_ghost_load(lv[0], gb);
-    tail_relax<ND, T, DEF>(prm, nv, B, gb, nsmooth);
-  }
+  if (in_lds(0)) tail_bottom<ND, T, DEF, true>(d, lv, set, nsmooth);
+  else tail_bottom<ND, T, DEF, false>(d, lv, set, nsmooth);
   for (int l = 1; l < nlev; l++) {
-    Lvl<T> C = tail_ref<T>(d, lv[l], set, lds), PA = tail_ref<T>(d, lv[l - 1], set, lds);
-    TailGhost gc;
-    tail_ghost_load(lv[l], gc);
-    const DPatch &P = C.P;
-    for_cells<ND>(P.mx + 2, P.my + 2, ND == 3 ? P.mz + 2 : 1, 0,
-                  [&](int i, int j, int k) { correct_cell<ND, T>(nv, C, PA, i, j, k); });
-    __syncthreads();
-    tail_ghost<T, DEF>(nv, C, gc.p0);
-    tail_relax<ND, T, DEF>(prm, nv, C, gc, nsmooth);
+    if (in_lds(l)) tail_up<ND, T, DEF, true, true>(d, lv, l, set, nsmooth);
+    else if (in_lds(l - 1)) tail_up<ND, T, DEF, false, true>(d, lv, l, set, nsmooth);
+    else tail_up<ND, T, DEF, false, false>(d, lv, l, set, nsmooth);
   }
   if (lds) tail_stage<T>(cout, nout, false);
 }
