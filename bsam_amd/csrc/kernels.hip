@@ -458,6 +458,61 @@ __device__ __forceinline__ void correct_cell(int nrvars, const Lvl<T> &C, const 
   }
 }
 
+// The same for the 2^d fine cells (2m..2m+1, 2n..2n+1(, 2o..2o+1)) of a
+// ghost-extended fine box, which all read the coarse corrections of cells
+// m..m+1, n..n+1(, o..o+1): each correction is formed once and every fine
+// cell is then updated with correct_cell's arithmetic.
+template <int ND, class T>
+__device__ __forceinline__ void correct_block(int nrvars, const Lvl<T> &C, const Lvl<T> &PA,
+                                              int m, int n, int o) {
+  const DPatch &P = C.P;
+  View<T> a = qview(C), cq = qcview(C), pq = qview(PA);
+  int o0 = P.mb[0][0] - 1, o1 = P.mb[1][0] - 1, o2 = ND == 3 ? P.mb[2][0] - 1 : 0;
+  constexpr int NZ = ND == 3 ? 2 : 1;
+  for (int v = 0; v < nrvars; v++) {
+    const T *cp = cq.p + v * cq.vs, *pp = pq.p + v * pq.vs;
+    T cr[2][2][NZ];   // cr[dx][dy][dz] = parent - qc at coarse (m+dx, n+dy, o+dz)
+#pragma unroll
+    for (int dz = 0; dz < NZ; dz++)
+#pragma unroll
+      for (int dy = 0; dy < 2; dy++)
+#pragma unroll
+        for (int dx = 0; dx < 2; dx++) {
+          int ci = m + dx, cj = n + dy, ck = ND == 3 ? o + dz : 0;
+          cr[dx][dy][dz] = pp[pq.at(o0 + ci, o1 + cj, ND == 3 ? o2 + ck : 0)] - cp[cq.at(ci, cj, ck)];
+        }
+    T *qv = a.p + v * a.vs;
+#pragma unroll
+    for (int fz = 0; fz < NZ; fz++)
+#pragma unroll
+      for (int fy = 0; fy < 2; fy++)
+#pragma unroll
+        for (int fx = 0; fx < 2; fx++) {
+          int i = 2 * m + fx, j = 2 * n + fy, k = ND == 3 ? 2 * o + fz : 0;
+          if (i > P.mx + 1 || j > P.my + 1 || (ND == 3 && k > P.mz + 1)) continue;
+          bool ex = fx == 0, ey = fy == 0, ez = fz == 0;
+          auto xpass = [&](int dy, int dz) {
+            T a0 = cr[0][dy][dz], a1 = cr[1][dy][dz];
+            return ex ? (T)3 * a0 + a1 : a0 + (T)3 * a1;
+          };
+          T p;
+          if constexpr (ND == 2) {
+            T b0 = xpass(0, 0), b1 = xpass(1, 0);
+            p = ey ? ((T)3 * b0 + b1) / (T)16 : (b0 + (T)3 * b1) / (T)16;
+          } else {
+            auto ypass = [&](int dz) {
+              T b0 = xpass(0, dz), b1 = xpass(1, dz);
+              return ey ? (T)3 * b0 + b1 : b0 + (T)3 * b1;
+            };
+            T c0 = ypass(0), c1 = ypass(1);
+            p = ez ? ((T)3 * c0 + c1) / (T)64 : (c0 + (T)3 * c1) / (T)64;
+          }
+          long long q0 = a.at(i, j, k);
+          qv[q0] = qv[q0] + p;
+        }
+  }
+}
+
 // GetSourceFunction + UpdateSourceFunction for the CH module (Source2D/3D,
 // SourceUpdate = 0; problem.f90:258-303): ftmp = (qold_1, -Dfe(qold_1)) and
 // f = ftmp + 0 when full != 0; only f = ftmp + 0 otherwise.
@@ -890,14 +945,11 @@ __device__ void tail_relax(const Problem &prm, int nrvars, const Lvl<T> &L, cons
 // RestrictSolution, SetGhost(parent, 0), GetCoarseGhostPoints and
 // CoarseLoadingFunction into the parent PA = lv[l - 1].
 template <int ND, class T, bool DEF, bool LC, bool LP>
-__device__ void tail_down(const Dev &d, const TailLevel *lv, int l, int set, int nsmooth) {
+__device__ void tail_down(const Dev &d, const TailLevel *lv, int l, int set, int nsmooth,
+                          const TailGhost &gc, const TailPass &gp) {
   const int nv = d.nrvars;
   const Problem prm = d.prm;
   Lvl<T> C = tail_ref<T, LC>(d, lv[l], set), PA = tail_ref<T, LP>(d, lv[l - 1], set);
-  TailGhost gc;
-  TailPass gp;
-  tail_ghost_load(lv[l], gc);
-  tail_pass_load(lv[l - 1].ngp[0], lv[l - 1].gp[0], gp);
   const DPatch &P = C.P;
   int cz = ND == 3 ? P.cmz : 1;
   tail_relax<ND, T, DEF>(prm, nv, C, gc, nsmooth);
@@ -915,24 +967,22 @@ __device__ void tail_down(const Dev &d, const TailLevel *lv, int l, int set, int
 
 // On the way up: CorrectFine from the parent, SetGhost(C, 0), smoothing.
 template <int ND, class T, bool DEF, bool LC, bool LP>
-__device__ void tail_up(const Dev &d, const TailLevel *lv, int l, int set, int nsmooth) {
+__device__ void tail_up(const Dev &d, const TailLevel *lv, int l, int set, int nsmooth,
+                        const TailGhost &gc) {
   const int nv = d.nrvars;
   Lvl<T> C = tail_ref<T, LC>(d, lv[l], set), PA = tail_ref<T, LP>(d, lv[l - 1], set);
-  TailGhost gc;
-  tail_ghost_load(lv[l], gc);
   const DPatch &P = C.P;
-  for_cells<ND>(P.mx + 2, P.my + 2, ND == 3 ? P.mz + 2 : 1, 0,
-                [&](int i, int j, int k) { correct_cell<ND, T>(nv, C, PA, i, j, k); });
+  for_cells<ND>(P.cmx + 1, P.cmy + 1, ND == 3 ? P.cmz + 1 : 1, 0,
+                [&](int m, int n, int o) { correct_block<ND, T>(nv, C, PA, m, n, o); });
   __syncthreads();
   tail_ghost<T, DEF>(nv, C, gc.p0);
   tail_relax<ND, T, DEF>(d.prm, nv, C, gc, nsmooth);
 }
 
 template <int ND, class T, bool DEF, bool LB>
-__device__ void tail_bottom(const Dev &d, const TailLevel *lv, int set, int nsmooth) {
+__device__ void tail_bottom(const Dev &d, const TailLevel *lv, int set, int nsmooth,
+                            const TailGhost &gb) {
   Lvl<T> B = tail_ref<T, LB>(d, lv[0], set);
-  TailGhost gb;
-  tail_ghost_load(lv[0], gb);
   tail_relax<ND, T, DEF>(d.prm, d.nrvars, B, gb, nsmooth);
 }
 
@@ -946,18 +996,28 @@ __global__ __launch_bounds__(1024) void k_tail_vcycle(Dev d, const TailLevel *lv
                                                        int nin, const TailCopy *cout, int nout) {
   const int set = DEF ? 1 : 0;
   auto in_lds = [&](int l) { return lds && lv[l].lds[set][0] >= 0; };
+  // each level's ghost updates are loaded one visit ahead, so their latency
+  // overlaps the previous level's work
+  TailGhost gcur, gnext;
+  tail_ghost_load(lv[nlev - 1], gcur);
   if (lds) tail_stage<T>(cin, nin, true);
   for (int l = nlev - 1; l >= 1; l--) {
-    if (in_lds(l)) tail_down<ND, T, DEF, true, true>(d, lv, l, set, nsmooth);
-    else if (in_lds(l - 1)) tail_down<ND, T, DEF, false, true>(d, lv, l, set, nsmooth);
-    else tail_down<ND, T, DEF, false, false>(d, lv, l, set, nsmooth);
+    tail_ghost_load(lv[l - 1], gnext);
+    if (in_lds(l)) tail_down<ND, T, DEF, true, true>(d, lv, l, set, nsmooth, gcur, gnext.p0);
+    else if (in_lds(l - 1))
+      tail_down<ND, T, DEF, false, true>(d, lv, l, set, nsmooth, gcur, gnext.p0);
+    else tail_down<ND, T, DEF, false, false>(d, lv, l, set, nsmooth, gcur, gnext.p0);
+    gcur = gnext;
   }
-  if (in_lds(0)) tail_bottom<ND, T, DEF, true>(d, lv, set, nsmooth);
-  else tail_bottom<ND, T, DEF, false>(d, lv, set, nsmooth);
+  if (nlev > 1) tail_ghost_load(lv[1], gnext);
+  if (in_lds(0)) tail_bottom<ND, T, DEF, true>(d, lv, set, nsmooth, gcur);
+  else tail_bottom<ND, T, DEF, false>(d, lv, set, nsmooth, gcur);
   for (int l = 1; l < nlev; l++) {
-    if (in_lds(l)) tail_up<ND, T, DEF, true, true>(d, lv, l, set, nsmooth);
-    else if (in_lds(l - 1)) tail_up<ND, T, DEF, false, true>(d, lv, l, set, nsmooth);
-    else tail_up<ND, T, DEF, false, false>(d, lv, l, set, nsmooth);
+    gcur = gnext;
+    if (l + 1 < nlev) tail_ghost_load(lv[l + 1], gnext);
+    if (in_lds(l)) tail_up<ND, T, DEF, true, true>(d, lv, l, set, nsmooth, gcur);
+    else if (in_lds(l - 1)) tail_up<ND, T, DEF, false, true>(d, lv, l, set, nsmooth, gcur);
+    else tail_up<ND, T, DEF, false, false>(d, lv, l, set, nsmooth, gcur);
   }
   if (lds) tail_stage<T>(cout, nout, false);
 }
