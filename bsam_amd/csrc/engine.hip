@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstring>
 #include <stdexcept>
+#include <climits>
 #include <mutex>
 #include <string>
 #include <unordered_map>
@@ -680,12 +681,16 @@ static void op_correct(bsam_ctx *c, int level, int prec = PREC_SOLUTION_FP64) {
 // FillDown, afasroutines.f90:223-239. For q, each SetGhost(level - 1, 0) runs
 // inside the next level's fill-down launch when its reads allow; `ghost` >= 0
 // is a SetGhost(finest, ghost) still pending from the caller, handled the
-// same way.
-static void op_fill_down(bsam_ctx *c, int field, int ghost = -1) {
+// same way. `bottom` limits the levels filled to those >= bottom (the whole
+// chain by default); the part below can be completed later by a call with
+// ghost = -1 and top = bottom.
+static void op_fill_down(bsam_ctx *c, int field, int ghost = -1, int top = INT_MAX,
+                         int bottom = INT_MIN) {
   if (field != BSAM_FIELD_F && field != BSAM_FIELD_Q) throw Error("fill_down: field must be F or Q");
   int low = std::max(c->cfg.minlevel + 1, c->tail_top + 1);
+  low = std::max(low, bottom + 1);
   int carry = ghost;   // SetGhost program of `level` not yet applied
-  int level = c->finest;
+  int level = std::min(top, c->finest);
   for (; level >= low; level--) {
     LevelData &L = c->L(level);
     const GhostOp *gops = nullptr;
@@ -706,7 +711,7 @@ static void op_fill_down(bsam_ctx *c, int field, int ghost = -1) {
     carry = field == BSAM_FIELD_Q ? 0 : -1;
   }
   if (carry >= 0) op_set_ghost(c, level, carry);
-  if (c->ntail() > 1) {
+  if (c->ntail() > 1 && bottom < c->tail_top && top >= c->tail_top) {
     ProfScope ps(c, BSAM_PROF_TAIL, 0.0);
     HIPCK(launch_tail_filldown(c->nd(), c->dev(), c->d_tail, c->ntail(),
                                field == BSAM_FIELD_F ? 1 : 2, c->tail_threads, c->stream));
@@ -852,12 +857,19 @@ static void multigrid_iterations(bsam_ctx *c, int *ncycles, double *hist, int ma
   // one iteration of the vcycleloop (afasroutines.f90:69-104): the V-cycle,
   // FillDown(q), source restore + FillDown(f) (the copy), the residual norm
   int done = 0;
+  //
+  // Inside the loop FillDown(q) stops at level 0: every level below the root
+  // is one patch that the next V-cycle overwrites whole (RestrictSolution
+  // from the level above, then SetGhost) before anything reads it, and the
+  // residual is taken on the finest level. The part below the root runs once
+  // after the loop, so the fields end as the reference's do.
   int n = iterate(c, &c->graph, &c->graph_exec, [&]() {
     int pend = op_vcycle(c, c->finest, PREC_SOLUTION_FP64, true);
-    op_fill_down(c, BSAM_FIELD_Q, pend);
+    op_fill_down(c, BSAM_FIELD_Q, pend, INT_MAX, 0);
     HIPCK(launch_copy(c->f, c->f_saved, c->nf, c->stream));
     queue_error(c, c->finest, c->cfg.qerrortol);
   }, &done);
+  op_fill_down(c, BSAM_FIELD_Q, -1, 0, INT_MIN);
   report(c, n, done, ncycles, hist, maxhist, converged);
 }
 
