@@ -47,24 +47,26 @@ struct Tile {
   int patch, i0, j0, k0;
 };
 
-// One level of the single-patch coarse chain, for the tail kernels: its patch,
-// its three SetGhost programs (ipass 0, 1, 2; offsets relative to the patch's
-// q origin) and where its arrays live: lds[set][a] is a byte offset into the
-// tail kernel's dynamic LDS or -1 for the global pool, for a = q, qc, f, qb and
-// set 0 (FP64 solution) or 1 (FP32 correction).
+// One level of the single-patch coarse chain, for the tail kernels: its patch
+// and a copy of the patch's record, its three SetGhost programs (ipass 0, 1,
+// 2; offsets relative to the patch's q origin) and where its arrays live:
+// lds[set][a] is a byte offset into the tail kernel's dynamic LDS or -1 for
+// the global pool, for a = q, qc, f, qb and set 0 (FP64 solution) or 1 (FP32
+// correction).
 struct TailLevel {
   int patch;
   int ngp[3];
   const GhostOp *gp[3];
   int lds[2][4];
+  DPatch P;
 };
 
-// One array the tail kernel stages between the global pools and LDS:
-// `n` elements at `glob` and at LDS byte offset `lds`; `start` is the
-// array's first index in the flattened copy (prefix sum of n over the table).
+// One chunk of an array the tail kernel stages between the global pools and
+// LDS: n <= 64 elements (one per lane of a wave) at `glob` and at LDS byte
+// offset `lds`.
 struct TailCopy {
   void *glob;
-  int lds, n, start, pad;
+  int lds, n;
 };
 
 struct Problem {

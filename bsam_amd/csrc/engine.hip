@@ -76,8 +76,9 @@ int fused_index(int ipass, int color) {
 const int kFusedPass[3] = {0, 1, 2}, kFusedColor[3] = {1, 2, 1};
 
 // The SetGhost programs of a single-patch level with offsets relative to the
-// patch's q origin, for the tail kernels. Returns false unless every update is
-// a copy within the patch (physical and periodic boundaries of a level that
+// patch's q origin, for the tail kernels; f1 carries the source packed with
+// the flip count (f0 | nflip << 28). Returns false unless every update is a
+// copy within the patch (physical and periodic boundaries of a level that
 // covers the domain).
 bool local_programs(const HPatch &h, const std::vector<GhostOp> (&gp)[3], int nv,
                     std::vector<GhostOp> (&out)[3]) {
@@ -87,12 +88,12 @@ bool local_programs(const HPatch &h, const std::vector<GhostOp> (&gp)[3], int nv
     for (GhostOp g : gp[ip]) {
       long long dst = g.dst - h.qoff, src = g.f0 - h.qoff;
       if (g.kind != 0 || dst < 0 || dst >= n || src < 0 || src >= n || g.dvs != n ||
-          g.fvs != n)
+          g.fvs != n || src >= (1 << 28) || g.nflip < 0 || g.nflip > 15)
         return false;
       (void)nv;
       g.dst = (int)dst;
       g.f0 = (int)src;
-      g.f1 = 0;
+      g.f1 = (int)src | (g.nflip << 28);
       out[ip].push_back(g);
     }
   }
@@ -317,12 +318,12 @@ static bool plan_tail(bsam_ctx *c, long long budget, hipStream_t s) {
   bool open[2] = {true, true};   // no level has failed to fit yet
   std::vector<TailCopy> cin[2], cout[2];
   std::vector<int> nin[2], nout[2];
-  const int maxseg = tail_max_segments();
   for (int l = c->cfg.minlevel; l <= c->tail_top; l++) {
     LevelData &L = c->lev[l - minl];
     const DPatch &D = c->dp[L.patches[0]];
     TailLevel t;
     t.patch = L.patches[0];
+    t.P = D;
     for (int ip = 0; ip < 3; ip++) {
       t.ngp[ip] = (int)L.gp[ip].size();
       t.gp[ip] = L.d_tgp[ip];
@@ -337,9 +338,7 @@ static bool plan_tail(bsam_ctx *c, long long budget, hipStream_t s) {
       // as well if room remains; levels in LDS form a prefix of the chain
       long long core = ((bytes[0] + 15) & ~15LL) + ((bytes[1] + 15) & ~15LL) +
                        ((bytes[3] + 15) & ~15LL);
-      int core_n = set == 1 ? 3 : 2;
-      bool fits = usable && open[set] && used[set] + core <= budget &&
-                  (int)cin[set].size() + core_n + 1 <= maxseg;
+      bool fits = usable && open[set] && used[set] + core <= budget;
       if (!fits) open[set] = false;
       for (int a : {0, 1, 3, 2}) {
         if (!fits) break;
@@ -358,13 +357,13 @@ static bool plan_tail(bsam_ctx *c, long long budget, hipStream_t s) {
                                                                           : (char *)c->qb;
         }
         off = a == 0 || a == 3 ? D.q : a == 1 ? D.qc : D.f;
-        TailCopy tc{pool + off * sz, t.lds[set][a], (int)(bytes[a] / sz), 0, 0};
-        auto add = [&](std::vector<TailCopy> &v) {
-          tc.start = v.empty() ? 0 : v.back().start + v.back().n;
-          v.push_back(tc);
-        };
-        add(cin[set]);
-        if (a != 3) add(cout[set]);
+        // staging chunks of one wave's width
+        const int n = (int)(bytes[a] / sz);
+        for (int e = 0; e < n; e += 64) {
+          TailCopy tc{pool + (off + e) * sz, t.lds[set][a] + (int)(e * sz), std::min(64, n - e)};
+          cin[set].push_back(tc);
+          if (a != 3) cout[set].push_back(tc);
+        }
       }
       nin[set].push_back((int)cin[set].size());
       nout[set].push_back((int)cout[set].size());
@@ -577,7 +576,7 @@ static void commit(bsam_ctx *c) {
   for (int l = c->cfg.minlevel; l <= std::min(0, c->finest); l++) {
     LevelData &L = c->lev[l - minl];
     std::vector<GhostOp> loc[3];
-    if (L.cells > c->tail_cells || L.patches.size() != 1 ||
+    if (L.cells > c->tail_cells || L.patches.size() != 1 || l - minl >= tail_max_levels() ||
         !local_programs(c->hp[L.patches[0]], L.gp, nv, loc))
       break;
     c->tail_top = l;

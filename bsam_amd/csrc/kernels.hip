@@ -765,7 +765,9 @@ __device__ __forceinline__ void for_cells(int ex, int ey, int ez, int base, F f)
 // global pool. Without, every array is used in place in the global pools.
 template <class T, bool IN_LDS>
 __device__ __forceinline__ Lvl<T> tail_ref(const Dev &d, const TailLevel &t, int set) {
-  Lvl<T> L = level_ref<T>(d, t.patch);
+  Fld<T> F = fields<T>(d);
+  const DPatch &P = t.P;
+  Lvl<T> L{F.q + P.q, F.qc + P.qc, F.f + P.f, F.qb ? F.qb + P.q : nullptr, P};
   if constexpr (IN_LDS) {
     const int *o = t.lds[set];
     L.q = (T *)(tail_lds + o[0]);
@@ -776,53 +778,65 @@ __device__ __forceinline__ Lvl<T> tail_ref(const Dev &d, const TailLevel &t, int
   return L;
 }
 
-// Moves arrays between the global pools and LDS as listed by a host-built
-// table (the tail levels' LDS-resident arrays: q, qc, f, qb in; q, qc, f
-// out). The table goes to LDS first; the arrays are then copied as one
-// flattened index space with eight independent loads per thread in flight,
-// a few memory round trips in all. Ends with a barrier.
-constexpr int TAIL_MAXSEG = 96;
-__shared__ TailCopy tail_seg[TAIL_MAXSEG];
+// The level table stays in global memory; the tail kernels take it as a
+// restrict pointer, so the compiler reads it with scalar loads, which do not
+// wait behind the vector loads in flight.
+constexpr int TAIL_MAXLEV = 24;
 
-template <class T>
-__device__ void tail_stage(const TailCopy *tab, int ns, bool in) {
-  for (int e = threadIdx.x; e < ns; e += blockDim.x) tail_seg[e] = tab[e];
-  __syncthreads();
-  if (ns > 0) {
-    const int total = tail_seg[ns - 1].start + tail_seg[ns - 1].n;
-    constexpr int U = 8;
-    const int step = blockDim.x;
-    int seg = 0;   // indices ascend within and across batches
-    for (int e0 = threadIdx.x; e0 < total; e0 += U * step) {
-      T v[U];
-      int sg[U];
+// Moves arrays between the global pools and LDS as listed by a host-built
+// table of 64-element chunks (the tail levels' LDS-resident arrays: q, qc, f,
+// qb in; q, qc, f out). Each wave takes every nw-th chunk, one element per
+// lane, U chunks per round: lane u < U reads the round's u-th table entry
+// with one vector load, and readlane hands each entry to the whole wave as a
+// uniform base. The data loads are branch-free (lane index clamped into the
+// chunk) so that they issue back to back; only the stores are predicated.
+// Staging in ends with a barrier.
+template <class T, bool in>
+__device__ void tail_stage(const TailCopy *tab, int nchunks) {
+  constexpr int U = 16;
+  static_assert(sizeof(TailCopy) == 16, "one int4 per table entry");
+  typedef __attribute__((address_space(1))) T GT;
+  typedef __attribute__((address_space(3))) T LT;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int nw = blockDim.x >> 6;
+  LT *lds = (LT *)tail_lds;
+  for (int c0 = wave; c0 < nchunks; c0 += U * nw) {
+    const __attribute__((address_space(1))) int *ent =
+        (const __attribute__((address_space(1))) int *)tab +
+        4 * min(c0 + (lane & (U - 1)) * nw, nchunks - 1);
+    const int4 e = make_int4(ent[0], ent[1], ent[2], ent[3]);
+    T v[U];
 #pragma unroll
-      for (int u = 0; u < U; u++) {
-        int e = e0 + u * step;
-        sg[u] = -1;
-        if (e < total) {
-          while (seg + 1 < ns && tail_seg[seg + 1].start <= e) seg++;
-          sg[u] = seg;
-          const TailCopy &t = tail_seg[seg];
-          v[u] = in ? ((const T *)t.glob)[e - t.start]
-                    : ((const T *)(tail_lds + t.lds))[e - t.start];
-        }
+    for (int u = 0; u < U; u++) {
+      const unsigned long long g = ((unsigned long long)(unsigned)__builtin_amdgcn_readlane(e.y, u)
+                                    << 32) |
+                                   (unsigned)__builtin_amdgcn_readlane(e.x, u);
+      const int o = __builtin_amdgcn_readlane(e.z, u) / (int)sizeof(T);
+      const int k = max(0, min(lane, __builtin_amdgcn_readlane(e.w, u) - 1));
+      if constexpr (in) v[u] = ((const GT *)g)[k];
+      else v[u] = lds[o + k];
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const unsigned long long g = ((unsigned long long)(unsigned)__builtin_amdgcn_readlane(e.y, u)
+                                    << 32) |
+                                   (unsigned)__builtin_amdgcn_readlane(e.x, u);
+      const int o = __builtin_amdgcn_readlane(e.z, u) / (int)sizeof(T);
+      const int n = c0 + u * nw < nchunks ? __builtin_amdgcn_readlane(e.w, u) : 0;
+      if (lane < n) {
+        if constexpr (in) lds[o + lane] = v[u];
+        else ((GT *)g)[lane] = v[u];
       }
-#pragma unroll
-      for (int u = 0; u < U; u++)
-        if (sg[u] >= 0) {
-          const TailCopy &t = tail_seg[sg[u]];
-          int k = e0 + u * step - t.start;
-          if (in) ((T *)(tail_lds + t.lds))[k] = v[u];
-          else ((T *)t.glob)[k] = v[u];
-        }
     }
   }
-  __syncthreads();
+  if constexpr (in) __syncthreads();
 }
 
 // One SetGhost program of a tail level, this thread's share of its updates
-// held in registers: destination, and source | nflip << 28.
+// held in registers as loaded (destination, and source | nflip << 28, which
+// the host stores in f1), so that the loads stay in flight until the program
+// is applied.
 struct TailPass {
   int n;
   const GhostOp *ops;
@@ -831,16 +845,20 @@ struct TailPass {
 struct TailGhost {
   TailPass p0, p1, p2;   // ipass 0, 1, 2
 };
+// The ghost programs live in global memory; loading through a global-space
+// pointer keeps the loads off the LDS wait counter.
+typedef const __attribute__((address_space(1))) GhostOp *GlobalOps;
 __device__ __forceinline__ void tail_pass_load(int n, const GhostOp *ops, TailPass &g) {
   g.n = n;
   g.ops = ops;
+  GlobalOps gops = (GlobalOps)ops;
 #pragma unroll
   for (int r = 0; r < TAIL_GR; r++) {
     int e = threadIdx.x + r * blockDim.x;
     g.dst[r] = g.srcf[r] = 0;
     if (e < n) {
-      g.dst[r] = ops[e].dst;
-      g.srcf[r] = ops[e].f0 | (ops[e].nflip << 28);
+      g.dst[r] = gops[e].dst;
+      g.srcf[r] = gops[e].f1;
     }
   }
 }
@@ -897,12 +915,20 @@ __device__ __forceinline__ bool color_cell(const DPatch &P, const ColorMap &m, i
   return i <= P.mx && j <= P.my;
 }
 
+// Element i of an array in LDS (lds) or in global memory, read with an LDS or
+// a global load: a flat load would also wait for the global loads in flight.
+template <class T>
+__device__ __forceinline__ T tail_load(const T *p, long long i, bool lds) {
+  if (lds) return ((const __attribute__((address_space(3))) T *)p)[i];
+  return ((const __attribute__((address_space(1))) T *)p)[i];
+}
+
 // nsmooth red-black passes on a tail level. Threads walk the cells of each
-// colour; the first TAIL_FR cells per thread and colour keep their f in
-// registers for the whole visit.
+// colour; the first TAIL_FR cells per thread and colour keep their f (in LDS
+// when f_lds) in registers for the whole visit.
 template <int ND, class T, bool DEF>
-__device__ void tail_relax(const Problem &prm, int nrvars, const Lvl<T> &L, const TailGhost &g,
-                           int nsmooth) {
+__device__ void tail_relax(const Problem &prm, int nrvars, const Lvl<T> &L, bool f_lds,
+                           const TailGhost &g, int nsmooth) {
   const DPatch &P = L.P;
   const ColorMap m = color_map<ND>(P);
   const int n = m.n;
@@ -915,8 +941,8 @@ __device__ void tail_relax(const Problem &prm, int nrvars, const Lvl<T> &L, cons
       f1[c][r] = f2[c][r] = (T)0;
       if (e < n && color_cell<ND>(P, m, c + 1, e, i, j, k)) {
         long long fo = fat(P, i, j, ND == 3 ? k : 1);
-        f1[c][r] = L.f[fo];
-        f2[c][r] = L.f[fo + P.fvs];
+        f1[c][r] = tail_load(L.f, fo, f_lds);
+        f2[c][r] = tail_load(L.f, fo + P.fvs, f_lds);
       }
     }
   auto half = [&](int c) {
@@ -945,14 +971,14 @@ __device__ void tail_relax(const Problem &prm, int nrvars, const Lvl<T> &L, cons
 // RestrictSolution, SetGhost(parent, 0), GetCoarseGhostPoints and
 // CoarseLoadingFunction into the parent PA = lv[l - 1].
 template <int ND, class T, bool DEF, bool LC, bool LP>
-__device__ void tail_down(const Dev &d, const TailLevel *lv, int l, int set, int nsmooth,
-                          const TailGhost &gc, const TailPass &gp) {
+__device__ void tail_down(const Dev &d, const TailLevel *__restrict__ lv, int l, int set,
+                          int nsmooth, const TailGhost &gc, const TailPass &gp) {
   const int nv = d.nrvars;
   const Problem prm = d.prm;
   Lvl<T> C = tail_ref<T, LC>(d, lv[l], set), PA = tail_ref<T, LP>(d, lv[l - 1], set);
   const DPatch &P = C.P;
   int cz = ND == 3 ? P.cmz : 1;
-  tail_relax<ND, T, DEF>(prm, nv, C, gc, nsmooth);
+  tail_relax<ND, T, DEF>(prm, nv, C, LC && lv[l].lds[set][2] >= 0, gc, nsmooth);
   for_cells<ND>(P.cmx, P.cmy, cz, 1,
                 [&](int i, int j, int k) { restrict_cell<ND, T>(nv, C, PA, i, j, k); });
   __syncthreads();
@@ -967,8 +993,8 @@ __device__ void tail_down(const Dev &d, const TailLevel *lv, int l, int set, int
 
 // On the way up: CorrectFine from the parent, SetGhost(C, 0), smoothing.
 template <int ND, class T, bool DEF, bool LC, bool LP>
-__device__ void tail_up(const Dev &d, const TailLevel *lv, int l, int set, int nsmooth,
-                        const TailGhost &gc) {
+__device__ void tail_up(const Dev &d, const TailLevel *__restrict__ lv, int l, int set,
+                        int nsmooth, const TailGhost &gc) {
   const int nv = d.nrvars;
   Lvl<T> C = tail_ref<T, LC>(d, lv[l], set), PA = tail_ref<T, LP>(d, lv[l - 1], set);
   const DPatch &P = C.P;
@@ -976,14 +1002,14 @@ __device__ void tail_up(const Dev &d, const TailLevel *lv, int l, int set, int n
                 [&](int m, int n, int o) { correct_block<ND, T>(nv, C, PA, m, n, o); });
   __syncthreads();
   tail_ghost<T, DEF>(nv, C, gc.p0);
-  tail_relax<ND, T, DEF>(d.prm, nv, C, gc, nsmooth);
+  tail_relax<ND, T, DEF>(d.prm, nv, C, LC && lv[l].lds[set][2] >= 0, gc, nsmooth);
 }
 
 template <int ND, class T, bool DEF, bool LB>
-__device__ void tail_bottom(const Dev &d, const TailLevel *lv, int set, int nsmooth,
+__device__ void tail_bottom(const Dev &d, const TailLevel *__restrict__ lv, int set, int nsmooth,
                             const TailGhost &gb) {
   Lvl<T> B = tail_ref<T, LB>(d, lv[0], set);
-  tail_relax<ND, T, DEF>(d.prm, d.nrvars, B, gb, nsmooth);
+  tail_relax<ND, T, DEF>(d.prm, d.nrvars, B, LB && lv[0].lds[set][2] >= 0, gb, nsmooth);
 }
 
 // AFASVcycle(top) (afasroutines.f90:488-521) on levels 0..nlev-1 of the
@@ -991,16 +1017,17 @@ __device__ void tail_bottom(const Dev &d, const TailLevel *lv, int set, int nsmo
 // chain (coarsest first), so a visit pairs LDS/LDS, global/LDS or
 // global/global child and parent.
 template <int ND, class T, bool DEF>
-__global__ __launch_bounds__(1024) void k_tail_vcycle(Dev d, const TailLevel *lv, int nlev,
-                                                       int nsmooth, int lds, const TailCopy *cin,
-                                                       int nin, const TailCopy *cout, int nout) {
+__global__ __launch_bounds__(1024) void k_tail_vcycle(Dev d, const TailLevel *__restrict__ lv,
+                                                       int nlev, int nsmooth, int lds,
+                                                       const TailCopy *cin, int nin,
+                                                       const TailCopy *cout, int nout) {
   const int set = DEF ? 1 : 0;
   auto in_lds = [&](int l) { return lds && lv[l].lds[set][0] >= 0; };
   // each level's ghost updates are loaded one visit ahead, so their latency
   // overlaps the previous level's work
   TailGhost gcur, gnext;
   tail_ghost_load(lv[nlev - 1], gcur);
-  if (lds) tail_stage<T>(cin, nin, true);
+  if (lds) tail_stage<T, true>(cin, nin);
   for (int l = nlev - 1; l >= 1; l--) {
     tail_ghost_load(lv[l - 1], gnext);
     if (in_lds(l)) tail_down<ND, T, DEF, true, true>(d, lv, l, set, nsmooth, gcur, gnext.p0);
@@ -1019,7 +1046,7 @@ __global__ __launch_bounds__(1024) void k_tail_vcycle(Dev d, const TailLevel *lv
     else if (in_lds(l - 1)) tail_up<ND, T, DEF, false, true>(d, lv, l, set, nsmooth, gcur);
     else tail_up<ND, T, DEF, false, false>(d, lv, l, set, nsmooth, gcur);
   }
-  if (lds) tail_stage<T>(cout, nout, false);
+  if (lds) tail_stage<T, false>(cout, nout);
 }
 
 // FillDown (afasroutines.f90:223-239) from level nlev-1 down to 0 of the
@@ -1248,7 +1275,7 @@ hipError_t launch_tail_vcycle(int nd, int prec, const Dev &d, const TailLevel *l
                               int nsmooth, int threads, size_t lds, const TailCopy *cin,
                               int nin, const TailCopy *cout, int nout, hipStream_t s) {
   if (nlev <= 0) return hipSuccess;
-  if (nin > TAIL_MAXSEG || nout > TAIL_MAXSEG) return hipErrorInvalidValue;
+  if (nlev > TAIL_MAXLEV) return hipErrorInvalidValue;
   int use = lds > 0;
   if (prec == PREC_SOLUTION_FP64) {
     if (nd == 2)
@@ -1268,8 +1295,8 @@ hipError_t launch_tail_vcycle(int nd, int prec, const Dev &d, const TailLevel *l
   return hipGetLastError();
 }
 
-int tail_max_segments() { return TAIL_MAXSEG; }
-size_t tail_static_lds() { return sizeof(TailCopy) * TAIL_MAXSEG; }
+int tail_max_levels() { return TAIL_MAXLEV; }
+size_t tail_static_lds() { return 0; }
 
 hipError_t launch_tail_filldown(int nd, const Dev &d, const TailLevel *lv, int nlev, int field,
                                 int threads, hipStream_t s) {

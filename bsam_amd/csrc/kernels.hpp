@@ -46,8 +46,8 @@ hipError_t launch_copy(double *dst, const double *src, long long n, hipStream_t 
 hipError_t launch_tail_vcycle(int nd, int prec, const Dev &d, const TailLevel *lv, int nlev,
                               int nsmooth, int threads, size_t lds, const TailCopy *cin,
                               int nin, const TailCopy *cout, int nout, hipStream_t s);
-// Capacity of the tail's staging table and the static LDS it takes.
-int tail_max_segments();
+// Capacity of the tail's level table, and the static LDS the tail kernel takes.
+int tail_max_levels();
 size_t tail_static_lds();
 hipError_t launch_tail_filldown(int nd, const Dev &d, const TailLevel *lv, int nlev, int field,
                                 int threads, hipStream_t s);
