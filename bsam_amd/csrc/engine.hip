@@ -408,6 +408,8 @@ static void commit(bsam_ctx *c) {
     else if (ly < 0) ly = std::min(3, 8 - lx);
     if (lx + ly > 8) throw Error("3D tiles: tile_x * tile_y must be at most 256");
   }
+  // the relaxation kernels give each thread one cell of a colour in its row
+  if (lx < 1) throw Error("tiles must be at least 2 cells wide (tile_x >= 2)");
   int minl = c->cfg.minlevel;
   c->finest = minl;
   for (auto &h : c->hp) c->finest = std::max(c->finest, h.level);
@@ -1200,6 +1202,7 @@ int bsam_tune(bsam_ctx *c, const char *key, double value) {
     int v = (int)value, lg = 0;
     while ((1 << lg) < v) lg++;
     if (v < 1 || (1 << lg) != v || v > 256) throw Error(k + " must be a power of two <= 256");
+    if (k == "tile_x" && v < 2) throw Error("tile_x must be at least 2");
     (k == "tile_x" ? c->tile_lx : c->tile_ly) = lg;
   } else if (k == "tail_lds") {
     if (c->committed || c->building) throw Error("tail_lds must be set before the hierarchy");

@@ -567,6 +567,21 @@ __device__ __forceinline__ void tile_cell(const Dev &d, const Tile &t, int &i, i
   j = t.j0 + ((tid >> lx) & ((1 << ly) - 1));
   k = ND == 3 ? t.k0 + (tid >> (lx + ly)) : 0;
 }
+// The relaxation kernels run half-size workgroups (RELAX_THREADS = half a
+// tile): thread tid takes the (tid mod TX/2)-th cell of colour `color` (i + j
+// (+ k) + color odd, the cells RelaxGrid updates in that pass) in its row of
+// the tile, so no lane idles on the other colour. Tiles are at least 2 wide.
+constexpr int RELAX_THREADS = 128;
+template <int ND>
+__device__ __forceinline__ void tile_color_cell(const Dev &d, const Tile &t, int color, int &i,
+                                                int &j, int &k) {
+  int tid = threadIdx.x, hx = d.tile_lx - 1, ly = d.tile_ly;
+  int ti = tid & ((1 << hx) - 1);
+  j = t.j0 + ((tid >> hx) & ((1 << ly) - 1));
+  k = ND == 3 ? t.k0 + (tid >> (hx + ly)) : 0;
+  int p = (j + k + color + 1) & 1;   // parity of the colour's i in this row
+  i = t.i0 + ((p - t.i0) & 1) + 2 * ti;
+}
 template <int ND>
 __device__ __forceinline__ bool in_box(int i, int j, int k, int ex, int ey, int ez) {
   return i <= ex && j <= ey && (ND == 2 || k <= ez);
@@ -574,15 +589,14 @@ __device__ __forceinline__ bool in_box(int i, int j, int k, int ex, int ey, int 
 
 // ---- level kernels ---------------------------------------------------------
 template <int ND, class T, bool DEF>
-__global__ __launch_bounds__(256) void k_relax(Dev d, const Tile *tiles, int ipass) {
+__global__ __launch_bounds__(RELAX_THREADS) void k_relax(Dev d, const Tile *tiles, int ipass) {
   Tile t = tiles[blockIdx.x];
   Lvl<T> L = level_ref<T>(d, t.patch);
   int i, j, k;
-  tile_cell<ND>(d, t, i, j, k);
-  if (!in_box<ND>(i, j, k, L.P.mx, L.P.my, L.P.mz)) return;
   // RelaxGrid's loop start 1+mod(j+ipass,2) (and +k in 3D) selects the
   // cells with i+j(+k)+ipass odd.
-  if (((i + j + (ND == 3 ? k : 0) + ipass) & 1) == 0) return;
+  tile_color_cell<ND>(d, t, ipass, i, j, k);
+  if (!in_box<ND>(i, j, k, L.P.mx, L.P.my, L.P.mz)) return;
   relax_cell<ND, T, DEF>(d.prm, L, i, j, k);
 }
 
@@ -591,8 +605,9 @@ __global__ __launch_bounds__(256) void k_relax(Dev d, const Tile *tiles, int ipa
 // (ops[start[b] .. start[b+1]), chosen on the host so that no update reads a
 // cell another tile relaxes), then relaxes its tile.
 template <int ND, class T, bool DEF>
-__global__ __launch_bounds__(256) void k_relax_fused(Dev d, const Tile *tiles, int color,
-                                                      const int *start, const GhostOp *ops) {
+__global__ __launch_bounds__(RELAX_THREADS) void k_relax_fused(Dev d, const Tile *tiles,
+                                                                int color, const int *start,
+                                                                const GhostOp *ops) {
   int s0 = start[blockIdx.x], s1 = start[blockIdx.x + 1];
   if (s1 > s0) {
     T *q = fields<T>(d).q;
@@ -603,9 +618,8 @@ __global__ __launch_bounds__(256) void k_relax_fused(Dev d, const Tile *tiles, i
   Tile t = tiles[blockIdx.x];
   Lvl<T> L = level_ref<T>(d, t.patch);
   int i, j, k;
-  tile_cell<ND>(d, t, i, j, k);
+  tile_color_cell<ND>(d, t, color, i, j, k);
   if (!in_box<ND>(i, j, k, L.P.mx, L.P.my, L.P.mz)) return;
-  if (((i + j + (ND == 3 ? k : 0) + color) & 1) == 0) return;
   relax_cell<ND, T, DEF>(d.prm, L, i, j, k);
 }
 
@@ -1167,14 +1181,17 @@ static inline dim3 grid1(int n) { return dim3((unsigned)n); }
 hipError_t launch_relax(int nd, int prec, const Dev &d, const Tile *tiles, int nt, int ipass,
                         hipStream_t s) {
   if (nt <= 0) return hipSuccess;
-  BSAM_LAUNCH_ND_PREC(k_relax, nd, prec, grid1(nt), 256, d, tiles, ipass);
+  if (d.tile_lx < 1) return hipErrorInvalidValue;
+  BSAM_LAUNCH_ND_PREC(k_relax, nd, prec, grid1(nt), RELAX_THREADS, d, tiles, ipass);
   return hipGetLastError();
 }
 
 hipError_t launch_relax_fused(int nd, int prec, const Dev &d, const Tile *tiles, int nt,
                               int color, const int *start, const GhostOp *ops, hipStream_t s) {
   if (nt <= 0) return hipSuccess;
-  BSAM_LAUNCH_ND_PREC(k_relax_fused, nd, prec, grid1(nt), 256, d, tiles, color, start, ops);
+  if (d.tile_lx < 1) return hipErrorInvalidValue;
+  BSAM_LAUNCH_ND_PREC(k_relax_fused, nd, prec, grid1(nt), RELAX_THREADS, d, tiles, color, start,
+                      ops);
   return hipGetLastError();
 }
 
