@@ -567,6 +567,18 @@ __device__ __forceinline__ void tile_cell(const Dev &d, const Tile &t, int &i, i
   j = t.j0 + ((tid >> lx) & ((1 << ly) - 1));
   k = ND == 3 ? t.k0 + (tid >> (lx + ly)) : 0;
 }
+// Workgroups are dispatched round-robin over the chip's 8 XCDs, each with its
+// own L2. A kernel launched over xcd_grid(nt) blocks runs tile
+// (b mod 8) * ceil(nt / 8) + b / 8 in block b, so each XCD walks a contiguous
+// run of the tile list (neighbouring tiles of a patch, whose halo rows and
+// cache lines overlap); blocks past the list return.
+constexpr int NXCD = 8;
+inline unsigned xcd_grid(int nt) { return (unsigned)(NXCD * ((nt + NXCD - 1) / NXCD)); }
+__device__ __forceinline__ int xcd_tile(int nt) {
+  const int per = (nt + NXCD - 1) / NXCD;
+  return (int)(blockIdx.x % NXCD) * per + (int)(blockIdx.x / NXCD);
+}
+
 // The relaxation kernels run half-size workgroups (RELAX_THREADS = half a
 // tile): thread tid takes the (tid mod TX/2)-th cell of colour `color` (i + j
 // (+ k) + color odd, the cells RelaxGrid updates in that pass) in its row of
@@ -589,8 +601,11 @@ __device__ __forceinline__ bool in_box(int i, int j, int k, int ex, int ey, int 
 
 // ---- level kernels ---------------------------------------------------------
 template <int ND, class T, bool DEF>
-__global__ __launch_bounds__(RELAX_THREADS) void k_relax(Dev d, const Tile *tiles, int ipass) {
-  Tile t = tiles[blockIdx.x];
+__global__ __launch_bounds__(RELAX_THREADS) void k_relax(Dev d, const Tile *tiles, int nt,
+                                                          int ipass) {
+  const int b = xcd_tile(nt);
+  if (b >= nt) return;
+  Tile t = tiles[b];
   Lvl<T> L = level_ref<T>(d, t.patch);
   int i, j, k;
   // RelaxGrid's loop start 1+mod(j+ipass,2) (and +k in 3D) selects the
@@ -605,17 +620,19 @@ __global__ __launch_bounds__(RELAX_THREADS) void k_relax(Dev d, const Tile *tile
 // (ops[start[b] .. start[b+1]), chosen on the host so that no update reads a
 // cell another tile relaxes), then relaxes its tile.
 template <int ND, class T, bool DEF>
-__global__ __launch_bounds__(RELAX_THREADS) void k_relax_fused(Dev d, const Tile *tiles,
+__global__ __launch_bounds__(RELAX_THREADS) void k_relax_fused(Dev d, const Tile *tiles, int nt,
                                                                 int color, const int *start,
                                                                 const GhostOp *ops) {
-  int s0 = start[blockIdx.x], s1 = start[blockIdx.x + 1];
+  const int b = xcd_tile(nt);
+  if (b >= nt) return;
+  int s0 = start[b], s1 = start[b + 1];
   if (s1 > s0) {
     T *q = fields<T>(d).q;
     for (int e = s0 + (int)threadIdx.x; e < s1; e += blockDim.x)
       ghost_op<T, DEF>(d.nrvars, q, ops[e]);
     __syncthreads();
   }
-  Tile t = tiles[blockIdx.x];
+  Tile t = tiles[b];
   Lvl<T> L = level_ref<T>(d, t.patch);
   int i, j, k;
   tile_color_cell<ND>(d, t, color, i, j, k);
@@ -642,10 +659,12 @@ __device__ __forceinline__ void ghost_alongside(const Dev &d, const GhostOp *ops
 }
 
 template <int ND, class T>
-__global__ __launch_bounds__(256) void k_restrict(Dev d, const Tile *tiles, const GhostOp *gops,
+__global__ __launch_bounds__(256) void k_restrict(Dev d, const Tile *tiles, int nt, const GhostOp *gops,
                                                   int ng) {
   ghost_alongside<T>(d, gops, ng);
-  Tile t = tiles[blockIdx.x];
+  const int b = xcd_tile(nt);
+  if (b >= nt) return;
+  Tile t = tiles[b];
   Lvl<T> C = level_ref<T>(d, t.patch);
   int i, j, k;
   tile_cell<ND>(d, t, i, j, k);
@@ -654,10 +673,12 @@ __global__ __launch_bounds__(256) void k_restrict(Dev d, const Tile *tiles, cons
 }
 
 template <int ND, class T>
-__global__ __launch_bounds__(256) void k_filldown(Dev d, const Tile *tiles, int field,
+__global__ __launch_bounds__(256) void k_filldown(Dev d, const Tile *tiles, int nt, int field,
                                                   const GhostOp *gops, int ng) {
   ghost_alongside<T>(d, gops, ng);
-  Tile t = tiles[blockIdx.x];
+  const int b = xcd_tile(nt);
+  if (b >= nt) return;
+  Tile t = tiles[b];
   Lvl<T> C = level_ref<T>(d, t.patch);
   int i, j, k;
   tile_cell<ND>(d, t, i, j, k);
@@ -667,8 +688,10 @@ __global__ __launch_bounds__(256) void k_filldown(Dev d, const Tile *tiles, int 
 
 // Tiles cover the ghost-extended coarse box 0..cmx+1.
 template <int ND, class T>
-__global__ __launch_bounds__(256) void k_coarse_ghost(Dev d, const Tile *tiles) {
-  Tile t = tiles[blockIdx.x];
+__global__ __launch_bounds__(256) void k_coarse_ghost(Dev d, const Tile *tiles, int nt) {
+  const int b = xcd_tile(nt);
+  if (b >= nt) return;
+  Tile t = tiles[b];
   Lvl<T> C = level_ref<T>(d, t.patch);
   int i, j, k;
   tile_cell<ND>(d, t, i, j, k);
@@ -677,8 +700,10 @@ __global__ __launch_bounds__(256) void k_coarse_ghost(Dev d, const Tile *tiles) 
 }
 
 template <int ND, class T, bool DEF>
-__global__ __launch_bounds__(256) void k_coarse_load(Dev d, const Tile *tiles) {
-  Tile t = tiles[blockIdx.x];
+__global__ __launch_bounds__(256) void k_coarse_load(Dev d, const Tile *tiles, int nt) {
+  const int b = xcd_tile(nt);
+  if (b >= nt) return;
+  Tile t = tiles[b];
   Lvl<T> C = level_ref<T>(d, t.patch);
   int i, j, k;
   tile_cell<ND>(d, t, i, j, k);
@@ -688,8 +713,10 @@ __global__ __launch_bounds__(256) void k_coarse_load(Dev d, const Tile *tiles) {
 
 // Tiles cover the ghost-extended coarse box 0..cmx+1.
 template <int ND, class T, bool DEF>
-__global__ __launch_bounds__(256) void k_coarse_ghost_load(Dev d, const Tile *tiles) {
-  Tile t = tiles[blockIdx.x];
+__global__ __launch_bounds__(256) void k_coarse_ghost_load(Dev d, const Tile *tiles, int nt) {
+  const int b = xcd_tile(nt);
+  if (b >= nt) return;
+  Tile t = tiles[b];
   Lvl<T> C = level_ref<T>(d, t.patch);
   int i, j, k;
   tile_cell<ND>(d, t, i, j, k);
@@ -699,8 +726,10 @@ __global__ __launch_bounds__(256) void k_coarse_ghost_load(Dev d, const Tile *ti
 
 // Tiles cover the ghost-extended fine box 0..mx+1.
 template <int ND, class T>
-__global__ __launch_bounds__(256) void k_correct(Dev d, const Tile *tiles) {
-  Tile t = tiles[blockIdx.x];
+__global__ __launch_bounds__(256) void k_correct(Dev d, const Tile *tiles, int nt) {
+  const int b = xcd_tile(nt);
+  if (b >= nt) return;
+  Tile t = tiles[b];
   Lvl<T> C = level_ref<T>(d, t.patch);
   int i, j, k;
   tile_cell<ND>(d, t, i, j, k);
@@ -709,8 +738,10 @@ __global__ __launch_bounds__(256) void k_correct(Dev d, const Tile *tiles) {
 }
 
 template <int ND>
-__global__ __launch_bounds__(256) void k_source(Dev d, const Tile *tiles, int full) {
-  Tile t = tiles[blockIdx.x];
+__global__ __launch_bounds__(256) void k_source(Dev d, const Tile *tiles, int nt, int full) {
+  const int b = xcd_tile(nt);
+  if (b >= nt) return;
+  Tile t = tiles[b];
   const DPatch &P = d.P[t.patch];
   int i, j, k;
   tile_cell<ND>(d, t, i, j, k);
@@ -720,8 +751,10 @@ __global__ __launch_bounds__(256) void k_source(Dev d, const Tile *tiles, int fu
 
 // Tiles cover the ghost-extended fine box 0..mx+1.
 template <int ND>
-__global__ __launch_bounds__(256) void k_defect_setup(Dev d, const Tile *tiles) {
-  Tile t = tiles[blockIdx.x];
+__global__ __launch_bounds__(256) void k_defect_setup(Dev d, const Tile *tiles, int nt) {
+  const int b = xcd_tile(nt);
+  if (b >= nt) return;
+  Tile t = tiles[b];
   const DPatch &P = d.P[t.patch];
   int i, j, k;
   tile_cell<ND>(d, t, i, j, k);
@@ -730,8 +763,10 @@ __global__ __launch_bounds__(256) void k_defect_setup(Dev d, const Tile *tiles) 
 }
 
 template <int ND>
-__global__ __launch_bounds__(256) void k_apply_correction(Dev d, const Tile *tiles) {
-  Tile t = tiles[blockIdx.x];
+__global__ __launch_bounds__(256) void k_apply_correction(Dev d, const Tile *tiles, int nt) {
+  const int b = xcd_tile(nt);
+  if (b >= nt) return;
+  Tile t = tiles[b];
   const DPatch &P = d.P[t.patch];
   int i, j, k;
   tile_cell<ND>(d, t, i, j, k);
@@ -1102,9 +1137,11 @@ __device__ __forceinline__ double block_sum(double s) {
 // Per-tile sum of squares of the residual f - A(q) (Residual,
 // afasroutines.f90:799-840; what = 0) or of f itself (what = 1).
 template <int ND>
-__global__ __launch_bounds__(256) void k_resnorm(Dev d, const Tile *tiles, double *partial,
+__global__ __launch_bounds__(256) void k_resnorm(Dev d, const Tile *tiles, int nt, double *partial,
                                                   int what) {
-  Tile t = tiles[blockIdx.x];
+  const int b = xcd_tile(nt);
+  if (b >= nt) return;
+  Tile t = tiles[b];
   const DPatch &P = d.P[t.patch];
   int i, j, k;
   tile_cell<ND>(d, t, i, j, k);
@@ -1123,7 +1160,7 @@ __global__ __launch_bounds__(256) void k_resnorm(Dev d, const Tile *tiles, doubl
     s = r1 * r1 + r2 * r2;
   }
   double tot = block_sum(s);
-  if (threadIdx.x == 0) partial[blockIdx.x] = tot;
+  if (threadIdx.x == 0) partial[b] = tot;
 }
 
 // Sum the per-tile partials in index order (strided, then a fixed tree) and
@@ -1150,7 +1187,6 @@ __global__ __launch_bounds__(256) void k_copy(double *dst, const double *src, lo
 }
 
 // ---- launchers -------------------------------------------------------------
-static inline dim3 grid1(int n) { return dim3((unsigned)n); }
 
 #define BSAM_LAUNCH_ND_PREC(KERNEL, nd, prec, grid, block, ...)                        \
   do {                                                                                 \
@@ -1182,7 +1218,7 @@ hipError_t launch_relax(int nd, int prec, const Dev &d, const Tile *tiles, int n
                         hipStream_t s) {
   if (nt <= 0) return hipSuccess;
   if (d.tile_lx < 1) return hipErrorInvalidValue;
-  BSAM_LAUNCH_ND_PREC(k_relax, nd, prec, grid1(nt), RELAX_THREADS, d, tiles, ipass);
+  BSAM_LAUNCH_ND_PREC(k_relax, nd, prec, xcd_grid(nt), RELAX_THREADS, d, tiles, nt, ipass);
   return hipGetLastError();
 }
 
@@ -1190,8 +1226,8 @@ hipError_t launch_relax_fused(int nd, int prec, const Dev &d, const Tile *tiles,
                               int color, const int *start, const GhostOp *ops, hipStream_t s) {
   if (nt <= 0) return hipSuccess;
   if (d.tile_lx < 1) return hipErrorInvalidValue;
-  BSAM_LAUNCH_ND_PREC(k_relax_fused, nd, prec, grid1(nt), RELAX_THREADS, d, tiles, color, start,
-                      ops);
+  BSAM_LAUNCH_ND_PREC(k_relax_fused, nd, prec, xcd_grid(nt), RELAX_THREADS, d, tiles, nt, color,
+                      start, ops);
   return hipGetLastError();
 }
 
@@ -1206,71 +1242,71 @@ hipError_t launch_ghost(int prec, const Dev &d, const GhostOp *ops, int n, hipSt
 hipError_t launch_restrict(int nd, int prec, const Dev &d, const Tile *tiles, int nt,
                            const GhostOp *gops, int ng, hipStream_t s) {
   if (nt <= 0) return hipSuccess;
-  BSAM_LAUNCH_ND_T(k_restrict, nd, prec, grid1(nt), 256, d, tiles, gops, ng);
+  BSAM_LAUNCH_ND_T(k_restrict, nd, prec, xcd_grid(nt), 256, d, tiles, nt, gops, ng);
   return hipGetLastError();
 }
 
 hipError_t launch_filldown(int nd, const Dev &d, const Tile *tiles, int nt, int field,
                            const GhostOp *gops, int ng, hipStream_t s) {
   if (nt <= 0) return hipSuccess;
-  if (nd == 2) k_filldown<2, double><<<grid1(nt), 256, 0, s>>>(d, tiles, field, gops, ng);
-  else k_filldown<3, double><<<grid1(nt), 256, 0, s>>>(d, tiles, field, gops, ng);
+  if (nd == 2) k_filldown<2, double><<<xcd_grid(nt), 256, 0, s>>>(d, tiles, nt, field, gops, ng);
+  else k_filldown<3, double><<<xcd_grid(nt), 256, 0, s>>>(d, tiles, nt, field, gops, ng);
   return hipGetLastError();
 }
 
 hipError_t launch_coarse_ghost(int nd, int prec, const Dev &d, const Tile *tiles, int nt,
                                hipStream_t s) {
   if (nt <= 0) return hipSuccess;
-  BSAM_LAUNCH_ND_T(k_coarse_ghost, nd, prec, grid1(nt), 256, d, tiles);
+  BSAM_LAUNCH_ND_T(k_coarse_ghost, nd, prec, xcd_grid(nt), 256, d, tiles, nt);
   return hipGetLastError();
 }
 
 hipError_t launch_coarse_load(int nd, int prec, const Dev &d, const Tile *tiles, int nt,
                               hipStream_t s) {
   if (nt <= 0) return hipSuccess;
-  BSAM_LAUNCH_ND_PREC(k_coarse_load, nd, prec, grid1(nt), 256, d, tiles);
+  BSAM_LAUNCH_ND_PREC(k_coarse_load, nd, prec, xcd_grid(nt), 256, d, tiles, nt);
   return hipGetLastError();
 }
 
 hipError_t launch_coarse_ghost_load(int nd, int prec, const Dev &d, const Tile *tiles, int nt,
                                    hipStream_t s) {
   if (nt <= 0) return hipSuccess;
-  BSAM_LAUNCH_ND_PREC(k_coarse_ghost_load, nd, prec, grid1(nt), 256, d, tiles);
+  BSAM_LAUNCH_ND_PREC(k_coarse_ghost_load, nd, prec, xcd_grid(nt), 256, d, tiles, nt);
   return hipGetLastError();
 }
 
 hipError_t launch_correct(int nd, int prec, const Dev &d, const Tile *tiles, int nt,
                           hipStream_t s) {
   if (nt <= 0) return hipSuccess;
-  BSAM_LAUNCH_ND_T(k_correct, nd, prec, grid1(nt), 256, d, tiles);
+  BSAM_LAUNCH_ND_T(k_correct, nd, prec, xcd_grid(nt), 256, d, tiles, nt);
   return hipGetLastError();
 }
 
 hipError_t launch_source(int nd, const Dev &d, const Tile *tiles, int nt, int full,
                          hipStream_t s) {
   if (nt <= 0) return hipSuccess;
-  BSAM_LAUNCH_ND(k_source, nd, grid1(nt), 256, d, tiles, full);
+  BSAM_LAUNCH_ND(k_source, nd, xcd_grid(nt), 256, d, tiles, nt, full);
   return hipGetLastError();
 }
 
 hipError_t launch_defect_setup(int nd, const Dev &d, const Tile *tiles, int nt,
                                hipStream_t s) {
   if (nt <= 0) return hipSuccess;
-  BSAM_LAUNCH_ND(k_defect_setup, nd, grid1(nt), 256, d, tiles);
+  BSAM_LAUNCH_ND(k_defect_setup, nd, xcd_grid(nt), 256, d, tiles, nt);
   return hipGetLastError();
 }
 
 hipError_t launch_apply_correction(int nd, const Dev &d, const Tile *tiles, int nt,
                                    hipStream_t s) {
   if (nt <= 0) return hipSuccess;
-  BSAM_LAUNCH_ND(k_apply_correction, nd, grid1(nt), 256, d, tiles);
+  BSAM_LAUNCH_ND(k_apply_correction, nd, xcd_grid(nt), 256, d, tiles, nt);
   return hipGetLastError();
 }
 
 hipError_t launch_resnorm(int nd, const Dev &d, const Tile *tiles, int nt, double *partial,
                           int what, hipStream_t s) {
   if (nt <= 0) return hipSuccess;
-  BSAM_LAUNCH_ND(k_resnorm, nd, grid1(nt), 256, d, tiles, partial, what);
+  BSAM_LAUNCH_ND(k_resnorm, nd, xcd_grid(nt), 256, d, tiles, nt, partial, what);
   return hipGetLastError();
 }
 
