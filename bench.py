@@ -15,9 +15,11 @@ resident in HBM. The roofline object is for the relaxation kernel: after the
 timed steps, the V-cycle's mix of relaxation launches (both colours on every
 level above the tail) is replayed back to back between two HIP events on the
 engine's stream, giving its mean launch duration; achieved = algorithmic bytes
-per launch / that duration. traffic is the measured HBM bytes per relaxation
-launch recorded in profiles/k_relax_c2_pmc.md (rocprofv3 FETCH_SIZE and
-WRITE_SIZE, calibrated on a copy kernel), null when that file is absent.
+per launch / that duration, at 48 B per cell-update (q read 16, f read 16, q
+written 16), i.e. 24 B per cell of the level for one colour's half-sweep.
+traffic is the measured HBM bytes per relaxation launch recorded in
+profiles/k_relax_c2_pmc.md (rocprofv3 FETCH_SIZE and WRITE_SIZE, calibrated on
+a copy kernel), null when that file is absent.
 cpu_baseline is the FP64 oracle (single-threaded C restatement of the
 reference) on the same workload.
 """

@@ -633,7 +633,7 @@ static void op_set_ghost(bsam_ctx *c, int level, int ipass, int prec = PREC_SOLU
 }
 static void op_relax(bsam_ctx *c, int level, int color, int prec = PREC_SOLUTION_FP64) {
   LevelData &L = c->L(level);
-  ProfScope ps(c, BSAM_PROF_RELAX, (prec == PREC_SOLUTION_FP64 ? 32.0 : 16.0) * L.cells);
+  ProfScope ps(c, BSAM_PROF_RELAX, (prec == PREC_SOLUTION_FP64 ? 24.0 : 12.0) * L.cells);
   HIPCK(launch_relax(c->nd(), prec, c->dev(), L.d_int, (int)L.t_int.size(), color, c->stream));
 }
 static LevelData &child_level(bsam_ctx *c, int level) {
@@ -740,7 +740,7 @@ static int level_relax(bsam_ctx *c, int level, int prec = PREC_SOLUTION_FP64,
       int f = pending >= 0 ? fused_index(pending, color) : -1;
       if (f >= 0 && c->fuse_ghost && L.fused[f].ok) {
         ProfScope ps(c, BSAM_PROF_RELAX,
-                     (prec == PREC_SOLUTION_FP64 ? 32.0 : 16.0) * L.cells);
+                     (prec == PREC_SOLUTION_FP64 ? 24.0 : 12.0) * L.cells);
         HIPCK(launch_relax_fused(c->nd(), prec, c->dev(), L.d_int, (int)L.t_int.size(), color,
                                  L.d_fstart[f], L.d_fops[f], c->stream));
       } else {
@@ -1303,7 +1303,7 @@ int bsam_profile_kernel(bsam_ctx *c, int kind, int reps, double *us_per_launch,
       for (int color = 1; color <= 2 && err == hipSuccess; color++) {
         err = launch_relax(c->nd(), PREC_SOLUTION_FP64, c->dev(), L.d_int, (int)L.t_int.size(),
                            color, c->stream);
-        bytes += 32.0 * L.cells;
+        bytes += 24.0 * L.cells;
         n++;
       }
     }

@@ -165,11 +165,12 @@ int bsam_tune_get(const bsam_ctx *ctx, const char *key, double *value);
  * launch of a kind whose bit is set in kind_mask is bracketed by HIP events
  * on the context's stream; bsam_profile_get then returns, per kind, the
  * number of launches, their summed duration (ms) and their summed
- * algorithmic HBM bytes (FP64, ghosts excluded: relax half-sweep 32 B -- q
- * read 16, f of the updated colour 8, q of the updated colour written 8 --,
- * residual norm 32 B, restriction 24 B, coarse loading 40 B, correction
- * 40 B, fill-down 20 B, source 32 B per cell of the level operated on; FP32
- * half of these). */
+ * algorithmic HBM bytes (FP64, ghosts excluded: relax 48 B per cell-update
+ * -- q read 16, f read 16, q written 16 per full red-black sweep of a cell --
+ * so 24 B per cell of the level for one colour's half-sweep, which updates
+ * half its cells; residual norm 32 B, restriction 24 B, coarse loading 40 B,
+ * correction 40 B, fill-down 20 B, source 32 B per cell of the level
+ * operated on; FP32 relaxation half of its figure). */
 enum {
   BSAM_PROF_RELAX = 0, BSAM_PROF_GHOST, BSAM_PROF_RESTRICT, BSAM_PROF_COARSE_GHOST,
   BSAM_PROF_COARSE_LOAD, BSAM_PROF_CORRECT, BSAM_PROF_FILLDOWN, BSAM_PROF_SOURCE,
