@@ -63,6 +63,15 @@ struct LevelData {
   bool down_ok[3] = {false, false, false};
   int *d_fstart[3] = {nullptr, nullptr, nullptr};
   GhostOp *d_fops[3] = {nullptr, nullptr, nullptr};
+  // the parent's SetGhost(0) inside this level's coarse ghost + loading
+  // launch, and its SetGhost(2) (the last of post-smoothing) inside this
+  // level's correction launch
+  RideGhost ride_cgl, ride_corr;
+  struct DRide {
+    GhostOp *rest = nullptr, *ops = nullptr;
+    int *start = nullptr;
+    int nrest = 0;
+  } d_ride_cgl, d_ride_corr;
   long long cells = 0;
 };
 
@@ -231,6 +240,12 @@ void bsam_ctx::free_device() {
       F(l.d_tgp[p]);
       F(l.d_fstart[p]);
       F(l.d_fops[p]);
+    }
+    for (LevelData::DRide *r : {&l.d_ride_cgl, &l.d_ride_corr}) {
+      F(r->rest);
+      F(r->start);
+      F(r->ops);
+      r->nrest = 0;
     }
   }
   if (h_state) (void)hipHostFree(h_state);
@@ -525,6 +540,11 @@ static void commit(bsam_ctx *c) {
               written[(size_t)(pa.qoff + i + pa.px * (j + (long long)pa.py * k))] = 1;
       }
       for (int ip = 0; ip < 3; ip++) L.down_ok[ip] = program_reads_avoid(L.gp[ip], written);
+      const LevelData &PL = c->lev[l - 1 - minl];
+      L.ride_cgl = plan_ride(PL.gp[0], coarse_ghost_load_reads(c->hp, L.t_cext, nd, c->tx(),
+                                                               c->ty(), c->tz()));
+      L.ride_corr =
+          plan_ride(PL.gp[2], correct_reads(c->hp, L.t_ext, nd, c->tx(), c->ty(), c->tz()));
     }
   }
   if (c->host_only) return;
@@ -568,6 +588,15 @@ static void commit(bsam_ctx *c) {
         L.d_fstart[f] = dev_upload(L.fused[f].start, s);
         L.d_fops[f] = dev_upload(L.fused[f].ops, s);
       }
+    auto up_ride = [&](const RideGhost &r, LevelData::DRide &d) {
+      if (!r.ok) return;
+      d.rest = dev_upload(r.rest, s);
+      d.nrest = (int)r.rest.size();
+      d.start = dev_upload(r.start, s);
+      d.ops = dev_upload(r.ops, s);
+    };
+    up_ride(L.ride_cgl, L.d_ride_cgl);
+    up_ride(L.ride_corr, L.d_ride_corr);
     maxt = std::max(maxt, L.t_int.size());
   }
   // tail: the longest run of single-patch levels from minlevel whose cell
@@ -668,16 +697,36 @@ static void op_coarse_load(bsam_ctx *c, int level, int prec = PREC_SOLUTION_FP64
   ProfScope ps(c, BSAM_PROF_COARSE_LOAD, 40.0 * L.cells);
   HIPCK(launch_coarse_load(c->nd(), prec, c->dev(), L.d_coarse, (int)L.t_coarse.size(), c->stream));
 }
-static void op_coarse_ghost_load(bsam_ctx *c, int level, int prec = PREC_SOLUTION_FP64) {
+// The parent's SetGhost(parent_ghost), still pending, rides inside the launch
+// when a plan for it exists (its SetGhost(0) in the coarse ghost + loading
+// launch, its SetGhost(2) in the correction launch); otherwise it is launched
+// first.
+static const LevelData::DRide *parent_ride(bsam_ctx *c, LevelData &L, int parent_ghost,
+                                           int level, int prec, bool correct) {
+  if (parent_ghost < 0) return nullptr;
+  const RideGhost &r = correct ? L.ride_corr : L.ride_cgl;
+  if (c->fuse_ghost && r.ok && parent_ghost == (correct ? 2 : 0))
+    return correct ? &L.d_ride_corr : &L.d_ride_cgl;
+  op_set_ghost(c, level - 1, parent_ghost, prec);
+  return nullptr;
+}
+static void op_coarse_ghost_load(bsam_ctx *c, int level, int prec = PREC_SOLUTION_FP64,
+                                 int parent_ghost = -1) {
   LevelData &L = child_level(c, level);
+  const LevelData::DRide *r = parent_ride(c, L, parent_ghost, level, prec, false);
   ProfScope ps(c, BSAM_PROF_COARSE_LOAD, 40.0 * L.cells);
   HIPCK(launch_coarse_ghost_load(c->nd(), prec, c->dev(), L.d_cext, (int)L.t_cext.size(),
-                                 c->stream));
+                                 r ? r->rest : nullptr, r ? r->nrest : 0,
+                                 r ? r->start : nullptr, r ? r->ops : nullptr, c->stream));
 }
-static void op_correct(bsam_ctx *c, int level, int prec = PREC_SOLUTION_FP64) {
+static void op_correct(bsam_ctx *c, int level, int prec = PREC_SOLUTION_FP64,
+                       int parent_ghost = -1) {
   LevelData &L = child_level(c, level);
+  const LevelData::DRide *r = parent_ride(c, L, parent_ghost, level, prec, true);
   ProfScope ps(c, BSAM_PROF_CORRECT, 40.0 * L.cells);
-  HIPCK(launch_correct(c->nd(), prec, c->dev(), L.d_ext, (int)L.t_ext.size(), c->stream));
+  HIPCK(launch_correct(c->nd(), prec, c->dev(), L.d_ext, (int)L.t_ext.size(),
+                       r ? r->rest : nullptr, r ? r->nrest : 0, r ? r->start : nullptr,
+                       r ? r->ops : nullptr, c->stream));
 }
 // FillDown, afasroutines.f90:223-239. For q, each SetGhost(level - 1, 0) runs
 // inside the next level's fill-down launch when its reads allow; `ghost` >= 0
@@ -770,10 +819,12 @@ static int op_vcycle(bsam_ctx *c, int level, int prec = PREC_SOLUTION_FP64,
   if (level == c->cfg.minlevel) return level_relax(c, level, prec, false, defer_final);
   int pend = level_relax(c, level, prec, false, true);
   op_restrict(c, level, prec, pend);
-  op_set_ghost(c, level - 1, 0, prec);
-  op_coarse_ghost_load(c, level, prec);
-  op_vcycle(c, level - 1, prec);
-  op_correct(c, level, prec);
+  // SetGhost(level - 1, 0) rides inside the coarse ghost + loading launch and
+  // the parent's last SetGhost of post-smoothing inside the correction launch
+  // (or each is launched just before, without a plan)
+  op_coarse_ghost_load(c, level, prec, 0);
+  int pp = op_vcycle(c, level - 1, prec, true);
+  op_correct(c, level, prec, pp);
   return level_relax(c, level, prec, true, defer_final);
 }
 static void reset_state(bsam_ctx *c) {

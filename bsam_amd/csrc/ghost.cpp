@@ -788,6 +788,96 @@ FusedGhost plan_fused_ghost(const std::vector<HPatch> &hp, const std::vector<Til
   return out;
 }
 
+// Parent cell (ci, cj, ck) of the child's coarse index space (0..cmx+1; the
+// footprint is 1..cmx) as a q-pool offset of the parent patch.
+static long long parent_offset(const std::vector<HPatch> &hp, const HPatch &h, int nd, int ci,
+                               int cj, int ck) {
+  const HPatch &pa = hp[h.parent];
+  long long pi = h.mb[0][0] - 1 + ci, pj = h.mb[1][0] - 1 + cj;
+  long long pk = nd == 3 ? h.mb[2][0] - 1 + ck : 0;
+  return pa.qoff + pi + pa.px * (pj + pa.py * pk);
+}
+
+std::vector<std::vector<long long>> coarse_ghost_load_reads(const std::vector<HPatch> &hp,
+                                                            const std::vector<Tile> &tiles,
+                                                            int nd, int tx, int ty, int tz) {
+  std::vector<std::vector<long long>> out(tiles.size());
+  for (size_t t = 0; t < tiles.size(); t++) {
+    const Tile &tl = tiles[t];
+    const HPatch &h = hp[tl.patch];
+    const int e[3] = {h.cmx[0] + 1, h.cmx[1] + 1, nd == 3 ? h.cmx[2] + 1 : 0};
+    auto ring = [&](int a, int b, int c) {
+      return a == 0 || a == e[0] || b == 0 || b == e[1] || (nd == 3 && (c == 0 || c == e[2]));
+    };
+    std::vector<long long> &r = out[t];
+    for (int k = tl.k0; k < tl.k0 + (nd == 3 ? tz : 1) && k <= e[2]; k++)
+      for (int j = tl.j0; j < tl.j0 + ty && j <= e[1]; j++)
+        for (int i = tl.i0; i < tl.i0 + tx && i <= e[0]; i++)
+          // the cell itself (a ring cell copies the parent) and its stencil
+          // neighbours on the ring (read from the parent by interior cells)
+          for (int s = 0; s < 2 * nd + 1; s++) {
+            const int d[7][3] = {{0, 0, 0}, {1, 0, 0}, {-1, 0, 0}, {0, 1, 0},
+                                 {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+            int a = i + d[s][0], b = j + d[s][1], c = k + d[s][2];
+            if (a < 0 || a > e[0] || b < 0 || b > e[1] || c < 0 || c > e[2]) continue;
+            if (!ring(a, b, c)) continue;
+            r.push_back(parent_offset(hp, h, nd, a, b, c));
+          }
+    std::sort(r.begin(), r.end());
+    r.erase(std::unique(r.begin(), r.end()), r.end());
+  }
+  return out;
+}
+
+std::vector<std::vector<long long>> correct_reads(const std::vector<HPatch> &hp,
+                                                  const std::vector<Tile> &tiles, int nd,
+                                                  int tx, int ty, int tz) {
+  std::vector<std::vector<long long>> out(tiles.size());
+  for (size_t t = 0; t < tiles.size(); t++) {
+    const Tile &tl = tiles[t];
+    const HPatch &h = hp[tl.patch];
+    const int e[3] = {h.mx[0] + 1, h.mx[1] + 1, nd == 3 ? h.mx[2] + 1 : 0};
+    std::vector<long long> &r = out[t];
+    // fine cell (i, j, k) reads parent cells i/2 + {0, 1} per dimension
+    for (int k = tl.k0; k < tl.k0 + (nd == 3 ? tz : 1) && k <= e[2]; k++)
+      for (int j = tl.j0; j < tl.j0 + ty && j <= e[1]; j++)
+        for (int i = tl.i0; i < tl.i0 + tx && i <= e[0]; i++)
+          for (int c = 0; c < (nd == 3 ? 2 : 1); c++)
+            for (int b = 0; b < 2; b++)
+              for (int a = 0; a < 2; a++)
+                r.push_back(parent_offset(hp, h, nd, (i >> 1) + a, (j >> 1) + b,
+                                          nd == 3 ? (k >> 1) + c : 0));
+    std::sort(r.begin(), r.end());
+    r.erase(std::unique(r.begin(), r.end()), r.end());
+  }
+  return out;
+}
+
+RideGhost plan_ride(const std::vector<GhostOp> &prog,
+                    const std::vector<std::vector<long long>> &reads) {
+  RideGhost out;
+  std::unordered_map<long long, int> at;   // destination -> update
+  for (size_t n = 0; n < prog.size(); n++) {
+    if (at.count(prog[n].dst)) return out;   // one update per destination
+    at[prog[n].dst] = (int)n;
+  }
+  std::vector<char> read_somewhere(prog.size(), 0);
+  out.start.assign(reads.size() + 1, 0);
+  for (size_t t = 0; t < reads.size(); t++) {
+    for (long long off : reads[t]) {
+      auto it = at.find(off);
+      if (it == at.end()) continue;
+      out.ops.push_back(prog[it->second]);
+      read_somewhere[it->second] = 1;
+    }
+    out.start[t + 1] = (int)out.ops.size();
+  }
+  for (size_t n = 0; n < prog.size(); n++)
+    if (!read_somewhere[n]) out.rest.push_back(prog[n]);
+  out.ok = true;
+  return out;
+}
+
 bool program_reads_avoid(const std::vector<GhostOp> &prog,
                          const std::vector<unsigned char> &written) {
   auto hit = [&](long long off) {

@@ -36,6 +36,32 @@ int build_ghost_program(const std::vector<HPatch> &hp, const std::vector<int> &p
                         int ndims, int level, int ipass, const Periodic &per,
                         std::vector<GhostOp> &out, std::string &msg);
 
+// A SetGhost program of the parent level applied in full inside a level
+// kernel of the child level that reads some of its destinations and writes
+// none of its sources (the child's coarse ghost + loading launch takes the
+// parent's SetGhost(0), its correction launch the parent's last SetGhost of
+// post-smoothing). Every update whose destination a tile reads is listed for
+// that tile (start/ops; an update may be listed for several tiles, each
+// writing the same value) and applied by it before its own work; the others
+// (rest) are applied grid-stride by all the launch's workgroups.
+struct RideGhost {
+  bool ok = false;
+  std::vector<int> start;
+  std::vector<GhostOp> ops;
+  std::vector<GhostOp> rest;
+};
+// Parent q-pool offsets (variable 1) that each tile of the child level's
+// coarse ghost + loading launch (ghost-extended coarse tiles, `tiles`) or
+// correction launch (ghost-extended fine tiles) may read.
+std::vector<std::vector<long long>> coarse_ghost_load_reads(const std::vector<HPatch> &hp,
+                                                            const std::vector<Tile> &tiles,
+                                                            int ndims, int tx, int ty, int tz);
+std::vector<std::vector<long long>> correct_reads(const std::vector<HPatch> &hp,
+                                                  const std::vector<Tile> &tiles, int ndims,
+                                                  int tx, int ty, int tz);
+RideGhost plan_ride(const std::vector<GhostOp> &prog,
+                    const std::vector<std::vector<long long>> &reads);
+
 // The ghost-cell updates of SetGhost(level, ipass) that a relaxation kernel
 // for colour `color` applies before relaxing, grouped by the tile that
 // applies them (ops[start[t] .. start[t+1]) for tile t of the level's

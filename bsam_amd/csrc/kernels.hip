@@ -615,6 +615,20 @@ __global__ __launch_bounds__(RELAX_THREADS) void k_relax(Dev d, const Tile *tile
   relax_cell<ND, T, DEF>(d.prm, L, i, j, k);
 }
 
+// Ghost updates a workgroup applies for its own tile before its work: the
+// updates ops[s0 .. s1) whose destinations the tile reads (the same update
+// may be applied by several tiles, each writing the same value). The caller
+// loads the bounds together with its tile record.
+template <class T>
+__device__ __forceinline__ void ghost_tile(const Dev &d, int s0, int s1, const GhostOp *ops) {
+  if (s1 > s0) {
+    T *q = fields<T>(d).q;
+    for (int e = s0 + (int)threadIdx.x; e < s1; e += blockDim.x)
+      ghost_op<T, std::is_same<T, float>::value>(d.nrvars, q, ops[e]);
+    __syncthreads();
+  }
+}
+
 // SetGhost(level, ipass) followed by the relaxation of colour `color`, in one
 // launch: each workgroup first applies the ghost updates its tile's cells read
 // (ops[start[b] .. start[b+1]), chosen on the host so that no update reads a
@@ -625,14 +639,9 @@ __global__ __launch_bounds__(RELAX_THREADS) void k_relax_fused(Dev d, const Tile
                                                                 const GhostOp *ops) {
   const int b = xcd_tile(nt);
   if (b >= nt) return;
-  int s0 = start[b], s1 = start[b + 1];
-  if (s1 > s0) {
-    T *q = fields<T>(d).q;
-    for (int e = s0 + (int)threadIdx.x; e < s1; e += blockDim.x)
-      ghost_op<T, DEF>(d.nrvars, q, ops[e]);
-    __syncthreads();
-  }
-  Tile t = tiles[b];
+  const Tile t = tiles[b];   // loaded together with the update bounds
+  const int s0 = start[b], s1 = start[b + 1];
+  ghost_tile<T>(d, s0, s1, ops);
   Lvl<T> L = level_ref<T>(d, t.patch);
   int i, j, k;
   tile_color_cell<ND>(d, t, color, i, j, k);
@@ -659,31 +668,35 @@ __device__ __forceinline__ void ghost_alongside(const Dev &d, const GhostOp *ops
 }
 
 template <int ND, class T>
-__global__ __launch_bounds__(256) void k_restrict(Dev d, const Tile *tiles, int nt, const GhostOp *gops,
-                                                  int ng) {
-  ghost_alongside<T>(d, gops, ng);
+__global__ __launch_bounds__(256) void k_restrict(Dev d, const Tile *tiles, int nt,
+                                                  const GhostOp *gops, int ng) {
   const int b = xcd_tile(nt);
-  if (b >= nt) return;
-  Tile t = tiles[b];
-  Lvl<T> C = level_ref<T>(d, t.patch);
-  int i, j, k;
-  tile_cell<ND>(d, t, i, j, k);
-  if (!in_box<ND>(i, j, k, C.P.cmx, C.P.cmy, C.P.cmz)) return;
-  restrict_cell<ND, T>(d.nrvars, C, level_ref<T>(d, C.P.parent), i, j, k);
+  if (b < nt) {
+    const Tile t = tiles[b];
+    Lvl<T> C = level_ref<T>(d, t.patch);
+    int i, j, k;
+    tile_cell<ND>(d, t, i, j, k);
+    if (in_box<ND>(i, j, k, C.P.cmx, C.P.cmy, C.P.cmz))
+      restrict_cell<ND, T>(d.nrvars, C, level_ref<T>(d, C.P.parent), i, j, k);
+  }
+  // the riding program reads no parent cell this launch writes and writes
+  // fine ghost cells it does not read, so it runs after the work
+  ghost_alongside<T>(d, gops, ng);
 }
 
 template <int ND, class T>
 __global__ __launch_bounds__(256) void k_filldown(Dev d, const Tile *tiles, int nt, int field,
                                                   const GhostOp *gops, int ng) {
-  ghost_alongside<T>(d, gops, ng);
   const int b = xcd_tile(nt);
-  if (b >= nt) return;
-  Tile t = tiles[b];
-  Lvl<T> C = level_ref<T>(d, t.patch);
-  int i, j, k;
-  tile_cell<ND>(d, t, i, j, k);
-  if (!in_box<ND>(i, j, k, C.P.cmx, C.P.cmy, C.P.cmz)) return;
-  filldown_cell<ND, T>(d.nrvars, C, level_ref<T>(d, C.P.parent), i, j, k, field);
+  if (b < nt) {
+    const Tile t = tiles[b];
+    Lvl<T> C = level_ref<T>(d, t.patch);
+    int i, j, k;
+    tile_cell<ND>(d, t, i, j, k);
+    if (in_box<ND>(i, j, k, C.P.cmx, C.P.cmy, C.P.cmz))
+      filldown_cell<ND, T>(d.nrvars, C, level_ref<T>(d, C.P.parent), i, j, k, field);
+  }
+  ghost_alongside<T>(d, gops, ng);   // as in k_restrict
 }
 
 // Tiles cover the ghost-extended coarse box 0..cmx+1.
@@ -711,30 +724,49 @@ __global__ __launch_bounds__(256) void k_coarse_load(Dev d, const Tile *tiles, i
   coarse_load_cell<ND, T, DEF>(d.prm, C, level_ref<T>(d, C.P.parent), i, j, k);
 }
 
-// Tiles cover the ghost-extended coarse box 0..cmx+1.
+// Tiles cover the ghost-extended coarse box 0..cmx+1. A SetGhost program of
+// the parent may ride along: gops[0..ng) grid-stride, and per tile the updates
+// it reads (tstart/tops; tstart null for none).
 template <int ND, class T, bool DEF>
-__global__ __launch_bounds__(256) void k_coarse_ghost_load(Dev d, const Tile *tiles, int nt) {
+__global__ __launch_bounds__(256) void k_coarse_ghost_load(Dev d, const Tile *tiles, int nt,
+                                                           const GhostOp *gops, int ng,
+                                                           const int *tstart,
+                                                           const GhostOp *tops) {
   const int b = xcd_tile(nt);
-  if (b >= nt) return;
-  Tile t = tiles[b];
-  Lvl<T> C = level_ref<T>(d, t.patch);
-  int i, j, k;
-  tile_cell<ND>(d, t, i, j, k);
-  if (!in_box<ND>(i, j, k, C.P.cmx + 1, C.P.cmy + 1, C.P.cmz + 1)) return;
-  coarse_ghost_load_cell<ND, T, DEF>(d.prm, d.nrvars, C, level_ref<T>(d, C.P.parent), i, j, k);
+  if (b < nt) {
+    const Tile t = tiles[b];
+    const int s0 = tstart ? tstart[b] : 0, s1 = tstart ? tstart[b + 1] : 0;
+    ghost_tile<T>(d, s0, s1, tops);
+    Lvl<T> C = level_ref<T>(d, t.patch);
+    int i, j, k;
+    tile_cell<ND>(d, t, i, j, k);
+    if (in_box<ND>(i, j, k, C.P.cmx + 1, C.P.cmy + 1, C.P.cmz + 1))
+      coarse_ghost_load_cell<ND, T, DEF>(d.prm, d.nrvars, C, level_ref<T>(d, C.P.parent), i, j,
+                                         k);
+  }
+  // the updates no tile reads, after the work (they touch nothing it uses)
+  ghost_alongside<T>(d, gops, ng);
 }
 
-// Tiles cover the ghost-extended fine box 0..mx+1.
+// Tiles cover the ghost-extended fine box 0..mx+1. A SetGhost program of the
+// parent may ride along as in k_coarse_ghost_load.
 template <int ND, class T>
-__global__ __launch_bounds__(256) void k_correct(Dev d, const Tile *tiles, int nt) {
+__global__ __launch_bounds__(256) void k_correct(Dev d, const Tile *tiles, int nt,
+                                                 const GhostOp *gops, int ng, const int *tstart,
+                                                 const GhostOp *tops) {
   const int b = xcd_tile(nt);
-  if (b >= nt) return;
-  Tile t = tiles[b];
-  Lvl<T> C = level_ref<T>(d, t.patch);
-  int i, j, k;
-  tile_cell<ND>(d, t, i, j, k);
-  if (!in_box<ND>(i, j, k, C.P.mx + 1, C.P.my + 1, C.P.mz + 1)) return;
-  correct_cell<ND, T>(d.nrvars, C, level_ref<T>(d, C.P.parent), i, j, k);
+  if (b < nt) {
+    const Tile t = tiles[b];
+    const int s0 = tstart ? tstart[b] : 0, s1 = tstart ? tstart[b + 1] : 0;
+    ghost_tile<T>(d, s0, s1, tops);
+    Lvl<T> C = level_ref<T>(d, t.patch);
+    int i, j, k;
+    tile_cell<ND>(d, t, i, j, k);
+    if (in_box<ND>(i, j, k, C.P.mx + 1, C.P.my + 1, C.P.mz + 1))
+      correct_cell<ND, T>(d.nrvars, C, level_ref<T>(d, C.P.parent), i, j, k);
+  }
+  // the updates no tile reads, after the work (they touch nothing it uses)
+  ghost_alongside<T>(d, gops, ng);
 }
 
 template <int ND>
@@ -1269,16 +1301,19 @@ hipError_t launch_coarse_load(int nd, int prec, const Dev &d, const Tile *tiles,
 }
 
 hipError_t launch_coarse_ghost_load(int nd, int prec, const Dev &d, const Tile *tiles, int nt,
-                                   hipStream_t s) {
+                                   const GhostOp *gops, int ng, const int *tstart,
+                                   const GhostOp *tops, hipStream_t s) {
   if (nt <= 0) return hipSuccess;
-  BSAM_LAUNCH_ND_PREC(k_coarse_ghost_load, nd, prec, xcd_grid(nt), 256, d, tiles, nt);
+  BSAM_LAUNCH_ND_PREC(k_coarse_ghost_load, nd, prec, xcd_grid(nt), 256, d, tiles, nt, gops, ng,
+                      tstart, tops);
   return hipGetLastError();
 }
 
 hipError_t launch_correct(int nd, int prec, const Dev &d, const Tile *tiles, int nt,
+                          const GhostOp *gops, int ng, const int *tstart, const GhostOp *tops,
                           hipStream_t s) {
   if (nt <= 0) return hipSuccess;
-  BSAM_LAUNCH_ND_T(k_correct, nd, prec, xcd_grid(nt), 256, d, tiles, nt);
+  BSAM_LAUNCH_ND_T(k_correct, nd, prec, xcd_grid(nt), 256, d, tiles, nt, gops, ng, tstart, tops);
   return hipGetLastError();
 }
 

@@ -25,10 +25,16 @@ hipError_t launch_coarse_ghost(int nd, int prec, const Dev &d, const Tile *tiles
 hipError_t launch_coarse_load(int nd, int prec, const Dev &d, const Tile *tiles, int nt,
                               hipStream_t s);
 // GetCoarseGhostPoints + CoarseLoadingFunction in one launch over the
-// ghost-extended coarse tiles.
+// ghost-extended coarse tiles, and CorrectFine over the ghost-extended fine
+// tiles. Either may apply a SetGhost program of the parent level in the same
+// launch: gops[0..ng) grid-stride (destinations the launch does not read) and,
+// per tile b, tops[tstart[b] .. tstart[b+1]) before its work (tstart null
+// for none); the launch writes none of the program's sources.
 hipError_t launch_coarse_ghost_load(int nd, int prec, const Dev &d, const Tile *tiles, int nt,
-                                   hipStream_t s);
+                                   const GhostOp *gops, int ng, const int *tstart,
+                                   const GhostOp *tops, hipStream_t s);
 hipError_t launch_correct(int nd, int prec, const Dev &d, const Tile *tiles, int nt,
+                          const GhostOp *gops, int ng, const int *tstart, const GhostOp *tops,
                           hipStream_t s);
 hipError_t launch_source(int nd, const Dev &d, const Tile *tiles, int nt, int full,
                          hipStream_t s);
