@@ -146,11 +146,14 @@ int bsam_sync(bsam_ctx *ctx);
  *                  captured hipGraph; 0: launch kernels one by one.
  *   "inner_cycles" FP32 V-cycles per defect-correction step (mixed
  *                  precision; 1..8, default 1).
- *   "tile_x", "tile_y"  level-kernel tile extents (powers of two; 2D
- *                  tile_x * tile_y = 256, default 32 x 8; 3D the third extent
- *                  fills 256 threads, default 16 x 8 x 2). Before the hierarchy.
- *   "fuse_ghost"   1 (default): apply each SetGhost inside the following
- *                  relaxation launch where the level allows; 0: separately.
+ *   "tile_x", "tile_y"  level-kernel tile extents (powers of two, tile_x
+ *                  at least 2; 2D tile_x * tile_y = 256, default 32 x 8; 3D
+ *                  the third extent fills 256 threads, default 16 x 8 x 2).
+ *                  Before the hierarchy.
+ *   "fuse_ghost"   1 (default): apply each SetGhost inside another level
+ *                  launch where the level allows (the following relaxation,
+ *                  the level's restriction or fill-down, the child level's
+ *                  coarse loading or correction); 0: separately.
  *   "tail_lds"     bytes of LDS the tail kernel may hold its levels' arrays
  *                  in (-1, default: the device limit; 0: none); before the
  *                  hierarchy. */
