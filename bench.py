@@ -20,12 +20,15 @@ written 16), i.e. 24 B per cell of the level for one colour's half-sweep.
 traffic is the measured HBM bytes per relaxation launch recorded in
 profiles/k_relax_c2_pmc.md (rocprofv3 FETCH_SIZE and WRITE_SIZE, calibrated on
 a copy kernel), null when that file is absent.
-cpu_baseline is the FP64 oracle (single-threaded C restatement of the
-reference) on the same workload.
+cpu_baseline is the reference itself (its Fortran modules built from the
+reference sources, oracle/_ref/harness) taking time steps of the same workload
+on one host core; where that binary is absent it is the FP64 oracle (the
+single-threaded C restatement of the reference).
 """
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -59,6 +62,48 @@ def measured_traffic():
     except OSError:
         pass
     return None
+
+
+def cpu_baseline_reference(layout, q0, params, budget_s):
+    """The reference's own Fortran modules (oracle/_ref/harness, built from the
+    reference sources with flang -O2) taking time steps of the same workload
+    on one host core, or None where the harness is not there. The +-1 well is
+    the reference's well under c = (phi+1)/2, alpha = 8, eps_ref = sqrt(2) eps,
+    dt_ref = dt/2 (mu unchanged). A 1-step run is subtracted from an N-step run
+    so that reading the input and building the tree do not count."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import refrun
+    if not refrun.available() or params["well"] != "plusminus":
+        return None
+    arrays = {}
+    for p in layout.ordered():
+        q = np.array(q0[p.id], dtype=np.float64)
+        q[..., 0] = (q[..., 0] + 1.0) / 2.0
+        arrays[p.id] = {"q": q, "f": np.zeros(layout.field_shape(p, "f")),
+                        "qc": np.zeros(layout.field_shape(p, "qc"))}
+    kw = dict(dt=params["dt"] / 2.0, alpha=8.0, eps=math.sqrt(2.0) * params["eps"],
+              nsmooth=params["nsmoothingpasses"], maxv=50, tol=params["qerrortol"])
+
+    def timed(steps):
+        t0 = time.perf_counter()
+        _, res, _ = refrun.run(layout, arrays, [("step", steps, 0)], **kw)
+        return time.perf_counter() - t0, len(res)
+
+    try:
+        t1, c1 = timed(1)
+        t3, _ = timed(3)
+        steps = max(3, int(budget_s / max((t3 - t1) / 2.0, 1e-3)) + 1)
+        tn, cn = timed(steps)
+    except (RuntimeError, OSError):
+        return None
+    if tn <= t1 or cn <= c1:
+        return None
+    per_v = layout.relax_cell_updates_per_vcycle(params["nsmoothingpasses"])
+    return {"value": (cn - c1) * per_v / (tn - t1), "unit": "cell-updates/s", "cores": 1,
+            "kind": "reference",
+            "sample": f"time steps 2..{steps} ({cn - c1} V-cycles) of the same workload from "
+                      f"t=0: the reference's Fortran modules (flang -O2, single-threaded) "
+                      f"on the mapped well, {tn - t1:.1f} s"}
 
 
 def cpu_baseline(layout, q0, params, budget_s):
@@ -231,7 +276,8 @@ def main():
         out["check"] = check
     eng.close()
     if args.cpu_budget > 0:
-        out["cpu_baseline"] = cpu_baseline(layout, q0, params, args.cpu_budget)
+        out["cpu_baseline"] = (cpu_baseline_reference(layout, q0, params, args.cpu_budget) or
+                               cpu_baseline(layout, q0, params, args.cpu_budget))
     print(json.dumps(out))
 
 

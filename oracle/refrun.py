@@ -1,9 +1,12 @@
 """Run the reference BSAM modules (oracle/_ref/harness) on a Layout.
 
-Used only in the build container, where the reference sources exist, to
-write golden fixtures and to cross-check the C oracle against the reference
-itself. Arrays travel as raw FP64 in Fortran order, patch by patch in
-ApplyOnLevel order (levels ascending, youngest patch first).
+Test and measurement infrastructure: the fixture generator and the tests
+use it to cross-check the C oracle against the reference itself, and
+bench.py's CPU-baseline leg times the reference's own time steps with it.
+The harness is built from the reference sources in the build container
+(oracle/Makefile) and travels as a binary. Arrays travel as raw FP64 in
+Fortran order, patch by patch in ApplyOnLevel order (levels ascending,
+youngest patch first).
 """
 
 from __future__ import annotations
@@ -14,7 +17,7 @@ import tempfile
 
 import numpy as np
 
-REPO = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HARNESS = os.path.join(REPO, "oracle", "_ref", "harness")
 
 

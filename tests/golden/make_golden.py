@@ -19,6 +19,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 sys.path.insert(0, HERE)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
 import refrun  # noqa: E402
 from golden_cases import LAYOUTS, OPS_CASES, STEP_CASES  # noqa: E402
