@@ -21,6 +21,8 @@ struct DPatch {
   long long f;             // offset of f(1,1,1,1) in the f/rf/ftmp pools
   long long qc;            // offset of qc(0,0,k0,1) in the qc pool
   long long qvs, fvs, cvs; // variable strides of q, f and qc
+  long long qr, rvs;       // offset and variable stride of qrte(1,1,1,1) (qrte pool)
+  long long fl;            // offset of errorflags(1,1,1) (error-flag pool)
   double dx;
   // operator constants of this grid, computed on the host exactly as the
   // reference does per call (h2 = dx*dx): dt/h2, eps2/h2, (2*ndims*eps2)/h2
@@ -82,6 +84,8 @@ struct Problem {
 struct Dev {
   double *q, *qold, *qc, *f, *rf, *ftmp;
   float *e, *qce, *fe, *qb;
+  double *qrte;   // relative truncation error per coarse cell (refined levels)
+  int *eflags;    // error flags per fine cell
   const DPatch *P;
   Problem prm;
   int nrvars;

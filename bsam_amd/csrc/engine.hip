@@ -121,9 +121,11 @@ struct bsam_ctx {
   std::vector<DPatch> dp;
   std::vector<LevelData> lev;
   int finest = 0;
-  long long nq = 0, nc = 0, nf = 0;
+  long long nq = 0, nc = 0, nf = 0, nr = 0, nfl = 0;
   double *q = nullptr, *qold = nullptr, *qc = nullptr, *f = nullptr, *rf = nullptr,
          *ftmp = nullptr;
+  double *qrte = nullptr;   // relative truncation error (EstimateLevelErrors)
+  int *eflags = nullptr;    // error flags (SetErrFlags)
   float *e = nullptr, *qce = nullptr, *fe = nullptr, *qb = nullptr;  // mixed precision
   int inner_cycles = 1;      // FP32 V-cycles per defect-correction step
   DPatch *d_P = nullptr;
@@ -183,6 +185,7 @@ struct bsam_ctx {
     Dev d;
     d.q = q; d.qold = qold; d.qc = qc; d.f = f; d.rf = rf; d.ftmp = ftmp;
     d.e = e; d.qce = qce; d.fe = fe; d.qb = qb;
+    d.qrte = qrte; d.eflags = eflags;
     d.P = d_P;
     d.prm = Problem{cfg.dt, cfg.eps2, cfg.fc_alpha, cfg.fc_shift, cfg.fe_num, cfg.fe_den};
     d.nrvars = cfg.nrvars;
@@ -221,6 +224,8 @@ void bsam_ctx::free_device() {
   e = qce = fe = qb = nullptr;
   F(d_state);
   F(f_saved);
+  F(qrte);
+  F(eflags);
   F(d_all_int);
   F(d_all_ext);
   F(d_src_int);
@@ -472,7 +477,7 @@ static void commit(bsam_ctx *c) {
       }
   }
   // pool offsets
-  c->nq = c->nc = c->nf = 0;
+  c->nq = c->nc = c->nf = c->nr = c->nfl = 0;
   c->dp.resize(n);
   for (int p = 0; p < n; p++) {
     HPatch &h = c->hp[p];
@@ -492,6 +497,15 @@ static void commit(bsam_ctx *c) {
     D.q = c->nq; D.qvs = ex * ey * ez;
     D.qc = c->nc; D.cvs = cx * cy * cz;
     D.f = c->nf; D.fvs = fn;
+    {
+      // qrte(1:cmx) with cmx = mx/2, or 1 for odd mx (AllocFields)
+      long long r = 1;
+      for (int d = 0; d < nd; d++) r *= h.mx[d] % 2 ? 1 : h.mx[d] / 2;
+      D.qr = c->nr; D.rvs = r;
+      D.fl = c->nfl;
+      c->nr += r * nv;
+      c->nfl += fn;
+    }
     D.dx = h.dx;
     {
       double h2 = h.dx * h.dx;   // as RelaxGrid / Operator compute it per call
@@ -559,6 +573,10 @@ static void commit(bsam_ctx *c) {
   HIPCK(hipMalloc(&c->rf, sizeof(double) * c->nf));
   HIPCK(hipMalloc(&c->ftmp, sizeof(double) * c->nf));
   HIPCK(hipMalloc(&c->f_saved, sizeof(double) * c->nf));
+  HIPCK(hipMalloc(&c->qrte, sizeof(double) * c->nr));
+  HIPCK(hipMalloc(&c->eflags, sizeof(int) * c->nfl));
+  HIPCK(hipMemsetAsync(c->qrte, 0, sizeof(double) * c->nr, s));
+  HIPCK(hipMemsetAsync(c->eflags, 0, sizeof(int) * c->nfl, s));
   HIPCK(hipMemsetAsync(c->q, 0, sizeof(double) * c->nq, s));
   HIPCK(hipMemsetAsync(c->qold, 0, sizeof(double) * c->nq, s));
   HIPCK(hipMemsetAsync(c->qc, 0, sizeof(double) * c->nc, s));
@@ -1086,10 +1104,11 @@ int bsam_hierarchy_commit(bsam_ctx *c) {
 int bsam_num_patches(const bsam_ctx *c) { return c ? (int)c->hp.size() : -1; }
 
 // Device bytes of a committed hierarchy's field pools (q, qold, qc, f, rf,
-// ftmp, plus the FP32 pools in mixed precision), excluding small tables.
+// ftmp, the saved f, qrte and the error flags, plus the FP32 pools in mixed
+// precision), excluding small tables.
 long long bsam_footprint_bytes(const bsam_ctx *c) {
   if (!c || !c->committed) return -1;
-  long long b = 8 * (2 * c->nq + c->nc + 3 * c->nf);
+  long long b = 8 * (2 * c->nq + c->nc + 4 * c->nf + c->nr) + 4 * c->nfl;
   if (c->cfg.precision == BSAM_PREC_MIXED) b += 4 * (2 * c->nq + c->nc + c->nf);
   return b;
 }
@@ -1106,6 +1125,7 @@ static void field_ptr(bsam_ctx *c, int patch, int field, double **base, long lon
     case BSAM_FIELD_F: *base = c->f; *off = D.f; *n = D.fvs * nv; break;
     case BSAM_FIELD_FTMP: *base = c->ftmp; *off = D.f; *n = D.fvs * nv; break;
     case BSAM_FIELD_RF: *base = c->rf; *off = D.f; *n = D.fvs * nv; break;
+    case BSAM_FIELD_QRTE: *base = c->qrte; *off = D.qr; *n = D.rvs * nv; break;
     default: throw Error("unknown field");
   }
 }
@@ -1143,6 +1163,54 @@ int bsam_download(bsam_ctx *c, int patch, int field, double *host, long n) {
   field_ptr(c, patch, field, &b, &off, &len);
   if (n != len) throw Error("download: size mismatch");
   HIPCK(hipMemcpyAsync(host, b + off, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  HIPCK(hipStreamSynchronize(c->stream));
+  API_END(c)
+}
+
+int bsam_truncation_error(bsam_ctx *c, int level) {
+  API_BEGIN(c)
+  c->need_device();
+  if (level <= c->cfg.minlevel || level > c->finest) throw Error("truncation_error: bad level");
+  // EstimateLevelErrors' truncation error (bsamroutines.f90:971-981)
+  op_restrict(c, level);
+  op_set_ghost(c, level - 1, 0);
+  LevelData &L = c->L(level);
+  HIPCK(launch_coarse_ghost(c->nd(), PREC_SOLUTION_FP64, c->dev(), L.d_cext,
+                            (int)L.t_cext.size(), c->stream));
+  // CopyQToQold on level and level - 1: the patches of consecutive levels
+  // occupy one contiguous range of the pool (patches are added by level)
+  long long beg = LLONG_MAX, end = 0;
+  for (int l = level - 1; l <= level; l++)
+    for (int p : c->L(l).patches) {
+      beg = std::min(beg, c->dp[p].q);
+      end = std::max(end, c->dp[p].q + c->dp[p].qvs * c->cfg.nrvars);
+    }
+  HIPCK(launch_copy(c->qold + beg, c->q + beg, end - beg, c->stream));
+  HIPCK(launch_trunc_error(c->nd(), c->dev(), L.d_coarse, (int)L.t_coarse.size(), c->stream));
+  API_END(c)
+}
+
+int bsam_error_flags(bsam_ctx *c, int level, double qtolerance) {
+  API_BEGIN(c)
+  c->need_device();
+  if (level < c->cfg.minlevel || level > c->finest) throw Error("error_flags: bad level");
+  LevelData &L = c->L(level);
+  // tol = qtolerance(level)/h/h (/h in 3D), as SetErrFlags2D/3D evaluate it
+  const double h = c->hp[L.patches.front()].dx;
+  double tol = qtolerance / h / h;
+  if (c->nd() == 3) tol = tol / h;
+  HIPCK(launch_error_flags(c->nd(), c->dev(), L.d_int, (int)L.t_int.size(), tol, c->stream));
+  API_END(c)
+}
+
+int bsam_download_flags(bsam_ctx *c, int patch, int *host, long n) {
+  API_BEGIN(c)
+  c->need_device();
+  if (patch < 0 || patch >= (int)c->hp.size()) throw Error("patch index out of range");
+  const DPatch &D = c->dp[patch];
+  if (n != D.fvs) throw Error("download_flags: size mismatch");
+  HIPCK(hipMemcpyAsync(host, c->eflags + D.fl, sizeof(int) * n, hipMemcpyDeviceToHost,
+                       c->stream));
   HIPCK(hipStreamSynchronize(c->stream));
   API_END(c)
 }

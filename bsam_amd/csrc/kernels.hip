@@ -348,6 +348,39 @@ __device__ __forceinline__ void coarse_load_cell(const Problem &prm, const Lvl<T
   PA.f[po + PA.P.fvs] = R2 + c2;
 }
 
+// RelativeTruncationError (afasroutines.f90:716-797) at coarse cell (i,j,k)
+// of patch C: qrte = R(A_h(q)) - A_2h(qc), the fine operator at the 2^d
+// children restricted in Restriction2D/3D's order, the coarse one on qc with
+// the parent's spacing.
+template <int ND>
+__device__ __forceinline__ void trunc_error_cell(const Dev &d, const Lvl<double> &C,
+                                                 const DPatch &PA, int i, int j, int k) {
+  const DPatch &P = C.P;
+  View<double> a = qview(C), cq = qcview(C);
+  constexpr int NC = ND == 2 ? 4 : 8;
+  double r1[NC], r2[NC];
+#pragma unroll
+  for (int s = 0; s < NC; s++) {
+    int fi = 2 * i - (s & 1), fj = 2 * j - ((s >> 1) & 1);
+    int fk = ND == 3 ? 2 * k - ((s >> 2) & 1) : 0;
+    op_at<ND, double, false>(d.prm, a, 0.0, P, a.at(fi, fj, fk), r1[s], r2[s]);
+  }
+  double R1, R2;
+  if constexpr (ND == 2) {
+    R1 = 0.25 * (((r1[0] + r1[1]) + r1[2]) + r1[3]);
+    R2 = 0.25 * (((r2[0] + r2[1]) + r2[2]) + r2[3]);
+  } else {
+    R1 = 0.125 * (((((((r1[0] + r1[1]) + r1[2]) + r1[3]) + r1[4]) + r1[5]) + r1[6]) + r1[7]);
+    R2 = 0.125 * (((((((r2[0] + r2[1]) + r2[2]) + r2[3]) + r2[4]) + r2[5]) + r2[6]) + r2[7]);
+  }
+  double c1, c2;
+  op_at<ND, double, false>(d.prm, cq, 0.0, PA, cq.at(i, j, k), c1, c2);
+  const int cx = P.mx & 1 ? 1 : P.mx >> 1, cy = P.my & 1 ? 1 : P.my >> 1;
+  long long o = P.qr + (i - 1) + (long long)cx * ((j - 1) + (long long)cy * (ND == 3 ? k - 1 : 0));
+  d.qrte[o] = R1 - c1;
+  d.qrte[o + P.rvs] = R2 - c2;
+}
+
 // GetCoarseGhostPoints and CoarseLoadingFunction in one pass over the
 // ghost-extended coarse box (i,j,k in 0..cmx+1): ring cells copy the parent
 // into qc; interior cells evaluate the coarse operator reading the ring
@@ -767,6 +800,51 @@ __global__ __launch_bounds__(256) void k_correct(Dev d, const Tile *tiles, int n
   }
   // the updates no tile reads, after the work (they touch nothing it uses)
   ghost_alongside<T>(d, gops, ng);
+}
+
+// RelativeTruncationError over the coarse interior tiles of a level.
+template <int ND>
+__global__ __launch_bounds__(256) void k_trunc_error(Dev d, const Tile *tiles, int nt) {
+  const int b = xcd_tile(nt);
+  if (b >= nt) return;
+  Tile t = tiles[b];
+  Lvl<double> C = level_ref<double>(d, t.patch);
+  int i, j, k;
+  tile_cell<ND>(d, t, i, j, k);
+  const DPatch &P = C.P;
+  if (!in_box<ND>(i, j, k, P.mx >> 1, P.my >> 1, P.mz >> 1)) return;
+  trunc_error_cell<ND>(d, C, d.P[P.parent], i, j, k);
+}
+
+// SetErrFlags2D/3D (bsamroutines.f90:1093-1157) over the interior tiles of a
+// level: a fine cell is flagged when its coarse cell's largest |qrte| over
+// the variables exceeds tol (= qtolerance / h^d); cells outside the coarse
+// cells' children are cleared.
+template <int ND>
+__global__ __launch_bounds__(256) void k_error_flags(Dev d, const Tile *tiles, int nt,
+                                                     double tol) {
+  const int b = xcd_tile(nt);
+  if (b >= nt) return;
+  Tile t = tiles[b];
+  const DPatch &P = d.P[t.patch];
+  int i, j, k;
+  tile_cell<ND>(d, t, i, j, k);
+  if (!in_box<ND>(i, j, k, P.mx, P.my, P.mz)) return;
+  const int ci = (i + 1) >> 1, cj = (j + 1) >> 1, ck = ND == 3 ? (k + 1) >> 1 : 1;
+  int flag = 0;
+  if (ci <= P.mx >> 1 && cj <= P.my >> 1 && (ND == 2 || ck <= P.mz >> 1)) {
+    const int cx = P.mx & 1 ? 1 : P.mx >> 1, cy = P.my & 1 ? 1 : P.my >> 1;
+    long long o =
+        P.qr + (ci - 1) + (long long)cx * ((cj - 1) + (long long)cy * (ND == 3 ? ck - 1 : 0));
+    double m = fabs(d.qrte[o]);
+    for (int v = 1; v < d.nrvars; v++) {
+      double a = fabs(d.qrte[o + v * P.rvs]);
+      if (a > m) m = a;
+    }
+    flag = m > tol ? 1 : 0;
+  }
+  d.eflags[P.fl + (i - 1) + (long long)P.mx * ((j - 1) + (long long)P.my * (ND == 3 ? k - 1 : 0))] =
+      flag;
 }
 
 template <int ND>
@@ -1321,6 +1399,19 @@ hipError_t launch_source(int nd, const Dev &d, const Tile *tiles, int nt, int fu
                          hipStream_t s) {
   if (nt <= 0) return hipSuccess;
   BSAM_LAUNCH_ND(k_source, nd, xcd_grid(nt), 256, d, tiles, nt, full);
+  return hipGetLastError();
+}
+
+hipError_t launch_trunc_error(int nd, const Dev &d, const Tile *tiles, int nt, hipStream_t s) {
+  if (nt <= 0) return hipSuccess;
+  BSAM_LAUNCH_ND(k_trunc_error, nd, xcd_grid(nt), 256, d, tiles, nt);
+  return hipGetLastError();
+}
+
+hipError_t launch_error_flags(int nd, const Dev &d, const Tile *tiles, int nt, double tol,
+                              hipStream_t s) {
+  if (nt <= 0) return hipSuccess;
+  BSAM_LAUNCH_ND(k_error_flags, nd, xcd_grid(nt), 256, d, tiles, nt, tol);
   return hipGetLastError();
 }
 

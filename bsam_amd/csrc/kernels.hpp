@@ -38,6 +38,11 @@ hipError_t launch_correct(int nd, int prec, const Dev &d, const Tile *tiles, int
                           hipStream_t s);
 hipError_t launch_source(int nd, const Dev &d, const Tile *tiles, int nt, int full,
                          hipStream_t s);
+// RelativeTruncationError over a level's coarse interior tiles (qrte) and
+// SetErrFlags from it over its interior tiles (tol = qtolerance / h^d).
+hipError_t launch_trunc_error(int nd, const Dev &d, const Tile *tiles, int nt, hipStream_t s);
+hipError_t launch_error_flags(int nd, const Dev &d, const Tile *tiles, int nt, double tol,
+                              hipStream_t s);
 hipError_t launch_defect_setup(int nd, const Dev &d, const Tile *tiles, int nt, hipStream_t s);
 hipError_t launch_apply_correction(int nd, const Dev &d, const Tile *tiles, int nt,
                                    hipStream_t s);

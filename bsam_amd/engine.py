@@ -21,9 +21,9 @@ from .layout import Layout
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libbsamgpu.so")
 
-FIELD_F, FIELD_Q, FIELD_QOLD, FIELD_QC, FIELD_FTMP, FIELD_RF = 1, 2, 3, 4, 5, 6
+FIELD_F, FIELD_Q, FIELD_QOLD, FIELD_QC, FIELD_FTMP, FIELD_RF, FIELD_QRTE = 1, 2, 3, 4, 5, 6, 7
 FIELDS = {"f": FIELD_F, "q": FIELD_Q, "qold": FIELD_QOLD, "qc": FIELD_QC,
-          "ftmp": FIELD_FTMP, "rf": FIELD_RF}
+          "ftmp": FIELD_FTMP, "rf": FIELD_RF, "qrte": FIELD_QRTE}
 PREC_FP64, PREC_MIXED = 0, 1
 
 # Double-well constants: the reference module (problem.f90:781-813) with
@@ -79,8 +79,11 @@ def lib():
         for n in ("bsam_set_ghost", "bsam_relax"):
             getattr(L, n).argtypes = [vp, C.c_int, C.c_int]
         for n in ("bsam_restrict_solution", "bsam_coarse_ghost_points", "bsam_coarse_load",
-                  "bsam_correct_fine", "bsam_fill_down", "bsam_vcycle"):
+                  "bsam_correct_fine", "bsam_fill_down", "bsam_vcycle",
+                  "bsam_truncation_error"):
             getattr(L, n).argtypes = [vp, C.c_int]
+        L.bsam_error_flags.argtypes = [vp, C.c_int, C.c_double]
+        L.bsam_download_flags.argtypes = [vp, C.c_int, ip, C.c_long]
         for n in ("bsam_source", "bsam_copy_qold", "bsam_sync"):
             getattr(L, n).argtypes = [vp]
         L.bsam_error_afas.argtypes = [vp, C.c_int, dp]
@@ -199,6 +202,22 @@ class Engine:
     def fill_down(self, field: int):
         """field 1 = f (sourcefield), 2 = q (solutionfield)."""
         self._ck(lib().bsam_fill_down(self.ctx, field))
+
+    def truncation_error(self, level: int):
+        """EstimateLevelErrors' truncation error on ``level`` into qrte."""
+        self._ck(lib().bsam_truncation_error(self.ctx, level))
+
+    def error_flags(self, level: int, qtolerance: float):
+        """SetErrFlags (default tagging) on ``level`` from qrte."""
+        self._ck(lib().bsam_error_flags(self.ctx, level, qtolerance))
+
+    def flags(self, n: int) -> np.ndarray:
+        """Patch ``n``'s error flags errorflags(1:mx) (int32, Fortran order)."""
+        shape = self.layout.field_shape(self.patches[n], "errorflags")
+        flat = np.empty(int(np.prod(shape)), dtype=np.int32)
+        self._ck(lib().bsam_download_flags(self.ctx, n, flat.ctypes.data_as(C.POINTER(C.c_int)),
+                                           flat.size))
+        return flat.reshape(shape, order="F")
 
     def source(self):
         self._ck(lib().bsam_source(self.ctx))

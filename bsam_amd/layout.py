@@ -189,6 +189,11 @@ class Layout:
                          for d in range(3)) + (nv,)
         if field in ("f", "rf", "ftmp"):
             return tuple(p.mx[d] if d < nd else 1 for d in range(3)) + (nv,)
+        if field == "qrte":
+            return tuple((1 if p.mx[d] % 2 else p.mx[d] // 2) if d < nd else 1
+                         for d in range(3)) + (nv,)
+        if field == "errorflags":
+            return tuple(p.mx[d] if d < nd else 1 for d in range(3))
         raise KeyError(field)
 
     def relax_cell_updates_per_vcycle(self, nsmoothingpasses: int = 2) -> int:

@@ -37,7 +37,8 @@ enum {
   BSAM_FIELD_QOLD = 3,  /* previous time level (ghost-extended) */
   BSAM_FIELD_QC = 4,    /* coarse-under-fine restriction (ghost-extended) */
   BSAM_FIELD_FTMP = 5,  /* time-step source part (interior) */
-  BSAM_FIELD_RF = 6     /* residual scratch (interior) */
+  BSAM_FIELD_RF = 6,    /* residual scratch (interior) */
+  BSAM_FIELD_QRTE = 7   /* relative truncation error qrte(1:cmx) (interior, coarse) */
 };
 
 /* Smoothing precision. */
@@ -113,6 +114,19 @@ int bsam_coarse_load(bsam_ctx *ctx, int level);
 int bsam_correct_fine(bsam_ctx *ctx, int level);
 /* FillDown(field), lib/afasroutines.f90:223-239; field BSAM_FIELD_F or _Q */
 int bsam_fill_down(bsam_ctx *ctx, int field);
+/* The truncation-error part of EstimateLevelErrors (lib/bsamroutines.f90:
+ * 971-981) on a refined level: RestrictSolution, SetGhost(level - 1, 0),
+ * GetCoarseGhostPoints, CopyQToQold on level and level - 1, then
+ * RelativeTruncationError (lib/afasroutines.f90:716-797) into
+ * BSAM_FIELD_QRTE of the level's patches. Replaces the reference's
+ * ApplyOnLevel(level, RelativeTruncationError, ...) and the calls before it. */
+int bsam_truncation_error(bsam_ctx *ctx, int level);
+/* SetErrFlags2D/3D (lib/bsamroutines.f90:1093-1157), the default error
+ * tagging, on every patch of `level` from its qrte with qtolerance(level);
+ * bsam_download_flags copies a patch's errorflags(1:mx) (n = interior cells)
+ * to the host. */
+int bsam_error_flags(bsam_ctx *ctx, int level, double qtolerance);
+int bsam_download_flags(bsam_ctx *ctx, int patch, int *host, long n);
 /* GetSourceFunction then UpdateSourceFunction on levels finest..0,
  * lib/afasroutines.f90:57-64 */
 int bsam_source(bsam_ctx *ctx);

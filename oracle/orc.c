@@ -66,6 +66,8 @@ void orc_destroy(orc_hier *h) {
     orc_patch *p = &h->p[n];
     free(p->q); free(p->qold); free(p->qc); free(p->f); free(p->rf);
     free(p->ftmp);
+    free(p->qrte);
+    free(p->errorflags);
   }
   free(h->p);
   free(h->lev_list);
@@ -126,7 +128,12 @@ int orc_add_patch(orc_hier *h, int id, int level, int parent, const int *mx,
   p->f = calloc((size_t)p->nf, sizeof(double));
   p->rf = calloc((size_t)p->nf, sizeof(double));
   p->ftmp = calloc((size_t)p->nf, sizeof(double));
-  if (!p->q || !p->qold || !p->qc || !p->f || !p->rf || !p->ftmp)
+  p->nr = (long)p->cmx[0] * p->cmx[1] * p->cmx[2] * h->nrvars;
+  p->nfl = p->fs[3];
+  p->qrte = calloc((size_t)p->nr, sizeof(double));
+  p->errorflags = calloc((size_t)p->nfl, sizeof(int));
+  if (!p->q || !p->qold || !p->qc || !p->f || !p->rf || !p->ftmp || !p->qrte ||
+      !p->errorflags)
     return fail(h, "out of memory");
   if (level > h->finestlevel) h->finestlevel = level;
   return h->npatches++;
@@ -196,6 +203,7 @@ double *orc_field(orc_hier *h, int n, int field) {
     case ORC_F: return p->f;
     case ORC_RF: return p->rf;
     case ORC_FTMP: return p->ftmp;
+    case ORC_QRTE: return p->qrte;
   }
   return NULL;
 }
@@ -207,8 +215,16 @@ long orc_field_len(orc_hier *h, int n, int field) {
     case ORC_Q: case ORC_QOLD: return p->nq;
     case ORC_QC: return p->nc;
     case ORC_F: case ORC_RF: case ORC_FTMP: return p->nf;
+    case ORC_QRTE: return p->nr;
   }
   return -1;
+}
+
+int *orc_error_flag_field(orc_hier *h, int n) {
+  return n < 0 || n >= h->npatches ? NULL : h->p[n].errorflags;
+}
+long orc_error_flag_len(orc_hier *h, int n) {
+  return n < 0 || n >= h->npatches ? -1 : h->p[n].nfl;
 }
 
 const char *orc_error(orc_hier *h) { return h->err; }
@@ -445,6 +461,67 @@ static void coarse_load_patch(orc_hier *h, orc_patch *p) {
         int pk = h->ndims == 3 ? p->mb[2][0] + k - 1 : 1;
         F(pa, pi, pj, pk, 1) = restrict_at(h, p->rf, p->fs, ONE3, i, j, k, 1) + r1;
         F(pa, pi, pj, pk, 2) = restrict_at(h, p->rf, p->fs, ONE3, i, j, k, 2) + r2;
+      }
+}
+
+/* RelativeTruncationError, lib/afasroutines.f90:716-797:
+ * qrte(1:cmx) = Restriction(A_h(q)) - A_2h(qc), the coarse operator with
+ * ch = parent dx; A_h(q) goes to a temporary as in the reference (rf is not
+ * touched). */
+#define QR(p, i, j, k, v)                                                  \
+  ((p)->qrte[(long)((i) - 1) + (long)((j) - 1) * (p)->cmx[0] +             \
+             (long)((k) - 1) * (p)->cmx[0] * (p)->cmx[1] +                 \
+             (long)((v) - 1) * (p)->cmx[0] * (p)->cmx[1] * (p)->cmx[2]])
+static void truncation_error_patch(orc_hier *h, orc_patch *p) {
+  orc_patch *pa = &h->p[p->parent];
+  double *ah = malloc((size_t)p->nf * sizeof(double));
+  if (!ah) return;
+  int kmax = h->ndims == 3 ? p->mx[2] : 1;
+  for (int k = 1; k <= kmax; k++)
+    for (int j = 1; j <= p->mx[1]; j++)
+      for (int i = 1; i <= p->mx[0]; i++) {
+        double r1, r2;
+        operator_at(h, p, acc_q, p->dx, i, j, k, &r1, &r2);
+        ah[FX(p, i, j, k, 1)] = r1;
+        ah[FX(p, i, j, k, 2)] = r2;
+      }
+  int ck = h->ndims == 3 ? p->mx[2] / 2 : 1;
+  for (int k = 1; k <= ck; k++)
+    for (int j = 1; j <= p->mx[1] / 2; j++)
+      for (int i = 1; i <= p->mx[0] / 2; i++) {
+        double c1, c2;
+        operator_at(h, p, acc_qc, pa->dx, i, j, k, &c1, &c2);
+        QR(p, i, j, k, 1) = restrict_at(h, ah, p->fs, ONE3, i, j, k, 1) - c1;
+        QR(p, i, j, k, 2) = restrict_at(h, ah, p->fs, ONE3, i, j, k, 2) - c2;
+      }
+  free(ah);
+}
+
+/* SetErrFlags2D/3D, lib/bsamroutines.f90:1093-1157: flag the 2^d children
+ * of every coarse cell whose largest |qrte| over the variables exceeds
+ * qtolerance / h^d. */
+static void error_flags_patch(orc_hier *h, orc_patch *p, double qtol) {
+  const int nd = h->ndims;
+  const double hh = p->dx;
+  double tol = qtol / hh / hh;
+  if (nd == 3) tol = tol / hh;
+  for (long e = 0; e < p->nfl; e++) p->errorflags[e] = 0;
+  int ck = nd == 3 ? p->cmx[2] : 1;
+  for (int i = 1; i <= p->cmx[0]; i++)
+    for (int j = 1; j <= p->cmx[1]; j++)
+      for (int k = 1; k <= ck; k++) {
+        double m = 0.0;
+        for (int v = 1; v <= h->nrvars; v++) {
+          double a = fabs(QR(p, i, j, k, v));
+          if (v == 1 || a > m) m = a;
+        }
+        if (m > tol)
+          for (int c = 0; c < (nd == 3 ? 2 : 1); c++)
+            for (int b = 0; b < 2; b++)
+              for (int a = 0; a < 2; a++) {
+                int fi = 2 * i - a, fj = 2 * j - b, fk = nd == 3 ? 2 * k - c : 1;
+                p->errorflags[FX(p, fi, fj, fk, 1)] = 1;
+              }
       }
 }
 
@@ -831,6 +908,28 @@ int orc_coarse_load_level(orc_hier *h, int level) {
 }
 int orc_correct_level(orc_hier *h, int level) {
   FOR_LEVEL_CHILDREN(correct_patch)
+}
+
+int orc_truncation_error(orc_hier *h, int level) {
+  if (!level_ok(h, level) || level <= h->minlevel)
+    return fail(h, "truncation_error: bad level %d", level);
+  if (orc_restrict_level(h, level) || orc_set_ghost(h, level - 1, 0) ||
+      orc_coarse_ghost_level(h, level))
+    return -1;
+  for (int l = level; l >= level - 1; l--)
+    for (int a = 0; a < nlev(h, l); a++) {
+      orc_patch *p = lev_patch(h, l, a);
+      for (long e = 0; e < p->nq; e++) p->qold[e] = p->q[e];
+    }
+  for (int a = 0; a < nlev(h, level); a++) truncation_error_patch(h, lev_patch(h, level, a));
+  return 0;
+}
+
+int orc_error_flags(orc_hier *h, int level, double qtolerance) {
+  if (!level_ok(h, level)) return fail(h, "error_flags: bad level %d", level);
+  for (int a = 0; a < nlev(h, level); a++)
+    error_flags_patch(h, lev_patch(h, level, a), qtolerance);
+  return 0;
 }
 
 /* FillDown, lib/afasroutines.f90:223-239. */

@@ -21,7 +21,7 @@ extern "C" {
 
 #define ORC_MAXLEV 24
 
-enum { ORC_Q = 0, ORC_QOLD = 1, ORC_QC = 2, ORC_F = 3, ORC_RF = 4, ORC_FTMP = 5 };
+enum { ORC_Q = 0, ORC_QOLD = 1, ORC_QC = 2, ORC_F = 3, ORC_RF = 4, ORC_FTMP = 5, ORC_QRTE = 6 };
 
 typedef struct orc_patch {
   int id;              /* caller's id (ngrid in the harness) */
@@ -39,6 +39,11 @@ typedef struct orc_patch {
   int ql[3], cl[3];
   long nq, nc, nf;     /* element counts */
   double *q, *qold, *qc, *f, *rf, *ftmp;
+  /* relative truncation error qrte(1:cmx, 1:nrvars) and the error flags
+     errorflags(1:mx) (AllocFields, lib/bsamstorage.f90:75-97) */
+  long nr, nfl;
+  double *qrte;
+  int *errorflags;
 } orc_patch;
 
 typedef struct orc_hier {
@@ -90,6 +95,15 @@ int orc_update_source(orc_hier *h);
 int orc_vcycle(orc_hier *h, int level);
 double orc_error_afas(orc_hier *h, int level);
 int orc_copy_qold(orc_hier *h);
+/* EstimateLevelErrors' truncation error (lib/bsamroutines.f90:971-981) on
+ * `level`: RestrictSolution, SetGhost(level-1, 0), GetCoarseGhostPoints,
+ * qold = q on level and level-1, RelativeTruncationError into qrte. */
+int orc_truncation_error(orc_hier *h, int level);
+/* SetErrFlags2D/3D (lib/bsamroutines.f90:1093-1157) on every patch of
+ * `level` from its qrte, with qtolerance(level). */
+int orc_error_flags(orc_hier *h, int level, double qtolerance);
+int *orc_error_flag_field(orc_hier *h, int patch);
+long orc_error_flag_len(orc_hier *h, int patch);
 /* MultigridIterations; writes up to maxhist residuals, returns #cycles. */
 int orc_multigrid_iterations(orc_hier *h, double *hist, int maxhist);
 /* One time step on a fixed hierarchy: ghosts, FillDown, qold copy, solve. */

@@ -14,7 +14,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liborc.so")
 
-FIELDS = {"q": 0, "qold": 1, "qc": 2, "f": 3, "rf": 4, "ftmp": 5}
+FIELDS = {"q": 0, "qold": 1, "qc": 2, "f": 3, "rf": 4, "ftmp": 5, "qrte": 6}
 
 # Problem constants. REFERENCE_CH is the reference module's double well
 # (problems/simple_cahn-hilliard/problem.f90:781-813); PLUSMINUS_CH is the
@@ -51,8 +51,13 @@ def lib():
             getattr(L, name).argtypes = [vp, C.c_int, C.c_int]
         for name in ("orc_restrict_level", "orc_coarse_ghost_level",
                      "orc_coarse_load_level", "orc_correct_level",
-                     "orc_fill_down", "orc_vcycle"):
+                     "orc_fill_down", "orc_vcycle", "orc_truncation_error"):
             getattr(L, name).argtypes = [vp, C.c_int]
+        L.orc_error_flags.argtypes = [vp, C.c_int, C.c_double]
+        L.orc_error_flag_field.restype = ip
+        L.orc_error_flag_field.argtypes = [vp, C.c_int]
+        L.orc_error_flag_len.restype = C.c_long
+        L.orc_error_flag_len.argtypes = [vp, C.c_int]
         for name in ("orc_source", "orc_update_source", "orc_copy_qold"):
             getattr(L, name).argtypes = [vp]
         L.orc_error_afas.restype = C.c_double
@@ -146,6 +151,22 @@ class Oracle:
 
     def update_source(self):
         lib().orc_update_source(self.h)
+
+    def truncation_error(self, level):
+        """EstimateLevelErrors' truncation error on ``level`` into qrte."""
+        self._ck(lib().orc_truncation_error(self.h, level))
+
+    def error_flags(self, level, qtolerance):
+        """SetErrFlags on ``level`` from qrte."""
+        self._ck(lib().orc_error_flags(self.h, level, qtolerance))
+
+    def flags(self, n):
+        """Writable Fortran-order view of patch ``n``'s error flags."""
+        L = lib()
+        ptr = L.orc_error_flag_field(self.h, n)
+        cnt = L.orc_error_flag_len(self.h, n)
+        flat = np.ctypeslib.as_array(ptr, shape=(cnt,))
+        return flat.reshape(self.layout.field_shape(self.patches[n], "errorflags"), order="F")
 
     def vcycle(self, level):
         self._ck(lib().orc_vcycle(self.h, level))

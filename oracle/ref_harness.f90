@@ -23,11 +23,11 @@
 !
 ! Ops: setghost L ipass | relax L color | restrict L | coarseghost L |
 !      coarseload L | correct L | filldown field | source 0 0 | vcycle L 0 |
-!      residual L 0 | mgiter 0 0 | step nsteps 0 | copyqold 0 0
+!      residual L 0 | mgiter 0 0 | step nsteps 0 | copyqold 0 0 | truncerr L 0
 !
 ! Input stream: for every level minlevel..finestlevel in level-list order
 ! (ApplyOnLevel order), the arrays q, f, qc of each patch.
-! Output stream: the same order, arrays q, f, qc, rf.
+! Output stream: the same order, arrays q, f, qc, rf, qold, qrte.
 ! Residual history (mgiter/step ops): MultigridIterations prints one line per
 ! V-cycle to stdout, after a 'harness step N' line per time step.
 program RefHarness
@@ -104,6 +104,16 @@ program RefHarness
     case('residual')
       r = ErrorAFAS(a1)
       write(14,'(i6,1x,i6,1x,es25.17)') 0, 0, r
+    case('truncerr')
+      ! The truncation-error part of EstimateLevelErrors
+      ! (lib/bsamroutines.f90:971-981), whose module keeps it private: the
+      ! same public calls in the same order, CopyQToQold as qold = q.
+      call ApplyOnLevel(a1, RestrictSolution, dummy)
+      call SetGhost(a1 - 1, 0)
+      call ApplyOnLevel(a1, GetCoarseGhostPoints, dummy)
+      call ApplyOnLevel(a1, CopyToOld, dummy)
+      call ApplyOnLevel(a1 - 1, CopyToOld, dummy)
+      call ApplyOnLevel(a1, RelativeTruncationError, dummy)
     case('copyqold')
       do lev = minlevel, finestlevel
         call ApplyOnLevel(lev, CopyToOld, dummy)

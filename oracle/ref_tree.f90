@@ -135,6 +135,8 @@ contains
     write(h_unit) info%f
     write(h_unit) info%qc
     write(h_unit) info%rf
+    write(h_unit) info%qold
+    write(h_unit) info%qrte
     DumpPatch = err_ok
   end function DumpPatch
 

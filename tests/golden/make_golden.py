@@ -52,7 +52,10 @@ def main():
         out, res, _ = refrun.run(lay, fields, ops)
         arrs = {}
         pack_fields("in", lay, fields, ("q", "f", "qc"), arrs)
-        pack_fields("out", lay, out, ("q", "f", "qc"), arrs)
+        names = ("q", "f", "qc")
+        if any(op[0] == "truncerr" for op in ops):
+            names += ("qold", "qrte")
+        pack_fields("out", lay, out, names, arrs)
         np.savez_compressed(os.path.join(HERE, f"ops_{case}.npz"), **arrs)
 
     for case, spec in STEP_CASES.items():

@@ -167,3 +167,25 @@ def test_tail_schedule_places_levels_in_lds():
     eng = Engine(lay, device=0, tune={"tail_lds": 0}, **KW)
     assert eng.tune_get("tail_lds_fp64") == 0
     eng.close()
+
+
+@pytest.mark.parametrize("name", sorted(LAYOUTS))
+def test_truncation_error_and_flags(name):
+    """EstimateLevelErrors' truncation error (bit-identical qrte, q, qold, qc)
+    and the default error flags at three tolerances around the data's scale,
+    on every refined level."""
+    lay = LAYOUTS[name]()
+    for level in range(max(lay.minlevel + 1, 1), lay.finestlevel + 1):
+        eng, ora = pair(lay, random_fields(lay, seed=23 + level))
+        eng.truncation_error(level)
+        ora.truncation_error(level)
+        assert_same(eng, ora, names=("q", "qold", "qc", "qrte"))
+        pats = [n for n, p in enumerate(ora.patches) if p.level == level]
+        h = ora.patches[pats[0]].dx
+        scale = max(float(np.abs(ora.field(n, "qrte")).max()) for n in pats) * h ** lay.ndims
+        for frac in (0.01, 0.3, 2.0):
+            eng.error_flags(level, frac * scale)
+            ora.error_flags(level, frac * scale)
+            for n in pats:
+                assert np.array_equal(eng.flags(n), ora.flags(n)), (level, frac, n)
+        eng.close()

@@ -41,6 +41,8 @@ def run_ops(o, ops):
             o.fill_down(*a)
         elif name == "vcycle":
             o.vcycle(*a)
+        elif name == "truncerr":
+            o.truncation_error(*a)
         else:
             raise KeyError(name)
 
@@ -56,8 +58,10 @@ def test_ops_match_reference(case):
             o.field(n, f)[...] = z[f"in/{n}/{f}"]
     run_ops(o, ops)
     for n, p in enumerate(o.patches):
-        for f in ("q", "f", "qc"):
-            assert np.array_equal(o.field(n, f), z[f"out/{n}/{f}"]), (case, p.id, f)
+        for f in ("q", "f", "qc", "qold", "qrte"):
+            key = f"out/{n}/{f}"
+            if f in ("q", "f", "qc") or key in z:
+                assert np.array_equal(o.field(n, f), z[key]), (case, p.id, f)
 
 
 @pytest.mark.parametrize("case", sorted(k for k in STEP_CASES if k != "plusminus_map"))

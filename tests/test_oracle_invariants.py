@@ -9,7 +9,7 @@ import pytest
 
 import orc
 from bsam_amd.layout import Layout
-from helpers import layout_amr2d, layout_amr2d_tiles
+from helpers import layout_amr2d, layout_amr2d_tiles, layout_amr3d
 
 
 def centers(p, ndims, shape):
@@ -152,3 +152,35 @@ def test_operator_on_constants():
     rf = o.field(n, "rf")
     np.testing.assert_allclose(-rf[..., 0], 0.8, rtol=1e-15)
     np.testing.assert_allclose(-rf[..., 1], -0.3 - 2.0 * (0.8 - 0.5) ** 3, rtol=1e-14)
+
+
+@pytest.mark.parametrize("mk", [layout_amr2d, layout_amr3d])
+def test_error_flags_threshold(mk):
+    """SetErrFlags2D/3D (lib/bsamroutines.f90:1093-1157), which the reference's
+    module keeps private: the 2^d children of a coarse cell are flagged exactly
+    when the largest |qrte| over the variables is strictly above
+    qtolerance / h^d, and every other flag is cleared. One coarse cell holds
+    (0.9 x, -x) with x and h powers of two, so every threshold is exact."""
+    lay = mk()
+    o = orc.Oracle(lay, dt=1e-3, eps=0.03)
+    level = lay.finestlevel
+    nd = lay.ndims
+    pats = [(n, p) for n, p in enumerate(o.patches) if p.level == level]
+    n0, p0 = pats[0]
+    x = 2.0 ** 5
+    h = p0.dx
+    cell = (1, 2, 1 if nd == 3 else 0)
+    for tol_factor, flagged in ((0.5, True), (0.95, True), (1.0, False), (2.0, False)):
+        for n, p in pats:
+            o.field(n, "qrte")[...] = 0.0
+            o.flags(n)[...] = 7
+        o.field(n0, "qrte")[cell + (0,)] = 0.9 * x
+        o.field(n0, "qrte")[cell + (1,)] = -x
+        o.error_flags(level, tol_factor * x * h ** nd)
+        for n, p in pats:
+            want = np.zeros(lay.field_shape(p, "errorflags"), dtype=np.int32)
+            if n == n0 and flagged:
+                ci, cj, ck = cell
+                want[2 * ci:2 * ci + 2, 2 * cj:2 * cj + 2,
+                     (2 * ck if nd == 3 else 0):(2 * ck + 2 if nd == 3 else 1)] = 1
+            assert np.array_equal(o.flags(n), want), (tol_factor, p.id)
