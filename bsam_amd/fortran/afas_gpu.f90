@@ -8,6 +8,12 @@
 !   call GpuDescribeHierarchy()          ! after every regrid (AMR)
 !   call GpuMultigridIterations()        ! in place of MultigridIterations
 !
+! and, in EstimateLevelErrors (lib/bsamroutines.f90:963-1002), the relative
+! truncation error and the default error tagging of a refined level:
+!
+!   call GpuRelativeTruncationError(level) ! RestrictSolution .. RelativeTruncationError
+!   call GpuSetErrFlags(level)             ! SetErrFlags2D/3D on every patch
+!
 ! The tree stays on the host. GpuDescribeHierarchy walks it with
 ! ApplyOnLevel (levels minlevel..finestlevel, youngest patch first) and hands
 ! every nodeinfo's geometry to the engine; GpuMultigridIterations uploads the
@@ -20,7 +26,8 @@ module AFASGPU
   implicit none
   private
   public :: GpuOpen, GpuClose, GpuDescribeHierarchy, GpuMultigridIterations, &
-            GpuSetGhost, GpuUploadAll, GpuDownloadAll, GpuApplyGhostHost, gpu_ctx, &
+            GpuSetGhost, GpuUploadAll, GpuDownloadAll, GpuApplyGhostHost, &
+            GpuRelativeTruncationError, GpuSetErrFlags, gpu_ctx, &
             gpu_npatch, gpu_cycles, gpu_converged, gpu_residuals
 
   type(c_ptr), save :: gpu_ctx = c_null_ptr
@@ -29,7 +36,7 @@ module AFASGPU
   ! q-array address of each patch in engine order, to find parent indices
   integer(c_intptr_t), allocatable, save :: qaddr(:)
   type(c_ptr), allocatable, save :: qptr(:)
-  integer, save :: walk_field = 0, walk_dir = 0
+  integer, save :: walk_field = 0, walk_dir = 0, walk_level = -1
 
 contains
 
@@ -146,41 +153,64 @@ contains
     call check(bsam_hierarchy_commit(gpu_ctx), 'bsam_hierarchy_commit')
   end subroutine GpuDescribeHierarchy
 
+  ! Copies one patch array between the host nodeinfo and the engine:
+  ! walk_dir 1 uploads, 0 downloads; walk_field names the array (a field id,
+  ! or 0 for the error flags); walk_level >= 0 restricts the copy to one
+  ! level (the walk still numbers every patch).
   integer function MovePatch(info, dummy)
     use NodeInfoDef
     use TreeOps, only: err_ok
     type(nodeinfo) :: info
     type(funcparam) :: dummy
-    integer(c_int) :: f
+    integer(c_int) :: f, p
+    type(c_ptr) :: a
+    integer(c_long) :: n
     gpu_npatch = gpu_npatch + 1
-    f = walk_field
-    if (walk_dir == 1) then
-      if (f == BSAM_FIELD_Q) then
-        call check(bsam_upload(gpu_ctx, gpu_npatch - 1, f, c_loc(info%q), &
-                               int(size(info%q), c_long)), 'bsam_upload')
-      else
-        call check(bsam_upload(gpu_ctx, gpu_npatch - 1, f, c_loc(info%qold), &
-                               int(size(info%qold), c_long)), 'bsam_upload')
-      end if
-    else
-      call check(bsam_download(gpu_ctx, gpu_npatch - 1, f, c_loc(info%q), &
-                               int(size(info%q), c_long)), 'bsam_download')
-    end if
     MovePatch = err_ok
+    if (walk_level >= 0 .and. info%level /= walk_level) return
+    f = walk_field
+    p = gpu_npatch - 1
+    select case (f)
+    case (BSAM_FIELD_Q)
+      a = c_loc(info%q)
+      n = size(info%q)
+    case (BSAM_FIELD_QOLD)
+      a = c_loc(info%qold)
+      n = size(info%qold)
+    case (BSAM_FIELD_QC)
+      a = c_loc(info%qc)
+      n = size(info%qc)
+    case (BSAM_FIELD_QRTE)
+      a = c_loc(info%qrte)
+      n = size(info%qrte)
+    case default
+      call check(bsam_download_flags(gpu_ctx, p, c_loc(info%errorflags), &
+                                     int(size(info%errorflags), c_long)), 'bsam_download_flags')
+      return
+    end select
+    if (walk_dir == 1) then
+      call check(bsam_upload(gpu_ctx, p, f, a, n), 'bsam_upload')
+    else
+      call check(bsam_download(gpu_ctx, p, f, a, n), 'bsam_download')
+    end if
   end function MovePatch
 
-  subroutine walk(field, dir)
+  subroutine walk(field, dir, onlevel)
     use NodeInfoDef
     use TreeOps, only: ApplyOnLevel
     integer, intent(in) :: field, dir
+    integer, intent(in), optional :: onlevel
     type(funcparam) :: dummy
     integer :: level
     walk_field = field
     walk_dir = dir
+    walk_level = -1
+    if (present(onlevel)) walk_level = onlevel
     gpu_npatch = 0
     do level = minlevel, finestlevel
       call ApplyOnLevel(level, MovePatch, dummy)
     end do
+    walk_level = -1
   end subroutine walk
 
   subroutine GpuUploadAll()
@@ -205,6 +235,37 @@ contains
     lconverged = conv /= 0
     call GpuDownloadAll()
   end subroutine GpuMultigridIterations
+
+  ! The truncation-error part of EstimateLevelErrors (lib/bsamroutines.f90:
+  ! 971-981) on the GPU: uploads q of every patch, restricts level onto
+  ! level - 1, fills the coarse ghost layer, copies q to qold on both levels
+  ! and evaluates RelativeTruncationError (lib/afasroutines.f90:716-797) on
+  ! the level's patches. Downloads q and qold of levels level - 1 and level,
+  ! qc and qrte of level: the host arrays are then what the reference calls
+  ! leave.
+  subroutine GpuRelativeTruncationError(level)
+    integer, intent(in) :: level
+    call walk(BSAM_FIELD_Q, 1)
+    call check(bsam_truncation_error(gpu_ctx, level), 'bsam_truncation_error')
+    call walk(BSAM_FIELD_Q, 0, level - 1)
+    call walk(BSAM_FIELD_QOLD, 0, level - 1)
+    call walk(BSAM_FIELD_Q, 0, level)
+    call walk(BSAM_FIELD_QOLD, 0, level)
+    call walk(BSAM_FIELD_QC, 0, level)
+    call walk(BSAM_FIELD_QRTE, 0, level)
+  end subroutine GpuRelativeTruncationError
+
+  ! SetErrFlags2D/3D (lib/bsamroutines.f90:1093-1157), the default tagging
+  ! of ErrFlag, on every patch of a level from the device qrte that
+  ! GpuRelativeTruncationError left, with qtolerance(level); the flags land
+  ! in each nodeinfo's errorflags.
+  subroutine GpuSetErrFlags(level)
+    use NodeInfoDef, only: qtolerance
+    integer, intent(in) :: level
+    call check(bsam_error_flags(gpu_ctx, level, real(qtolerance(level), c_double)), &
+               'bsam_error_flags')
+    call walk(0, 0, level)
+  end subroutine GpuSetErrFlags
 
   ! SetGhost(level, ipass) (lib/boundary.f90:58-89) on the device copy.
   subroutine GpuSetGhost(level, ipass)

@@ -6,7 +6,8 @@ module bsam_gpu
   implicit none
 
   integer(c_int), parameter :: BSAM_FIELD_F = 1, BSAM_FIELD_Q = 2, BSAM_FIELD_QOLD = 3, &
-                               BSAM_FIELD_QC = 4, BSAM_FIELD_FTMP = 5, BSAM_FIELD_RF = 6
+                               BSAM_FIELD_QC = 4, BSAM_FIELD_FTMP = 5, BSAM_FIELD_RF = 6, &
+                               BSAM_FIELD_QRTE = 7
   integer(c_int), parameter :: BSAM_PREC_FP64 = 0, BSAM_PREC_MIXED = 1
 
   type, bind(C) :: bsam_cfg
@@ -101,6 +102,30 @@ module bsam_gpu
       integer(c_int), intent(out) :: ncycles, converged
       real(c_double), intent(out) :: hist(*)
       integer(c_int), value :: maxhist
+      integer(c_int) :: rc
+    end function
+    function bsam_truncation_error(ctx, level) bind(C, name='bsam_truncation_error') &
+        result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: level
+      integer(c_int) :: rc
+    end function
+    function bsam_error_flags(ctx, level, qtolerance) bind(C, name='bsam_error_flags') &
+        result(rc)
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: level
+      real(c_double), value :: qtolerance
+      integer(c_int) :: rc
+    end function
+    function bsam_download_flags(ctx, patch, host, n) bind(C, name='bsam_download_flags') &
+        result(rc)
+      import :: c_ptr, c_int, c_long
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: patch
+      type(c_ptr), value :: host
+      integer(c_long), value :: n
       integer(c_int) :: rc
     end function
     function bsam_sync(ctx) bind(C, name='bsam_sync') result(rc)

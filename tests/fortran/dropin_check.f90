@@ -7,6 +7,11 @@
 !                                  the reference SetGhost, every level/pass
 !   dropin_check gpu  SPEC INPUT   GpuMultigridIterations vs the reference
 !                                  MultigridIterations, NSTEP time steps
+!   dropin_check trunc SPEC INPUT  GpuRelativeTruncationError vs the
+!                                  reference calls of EstimateLevelErrors
+!                                  (q, qold, qc, qrte), and GpuSetErrFlags'
+!                                  flags vs the SetErrFlags rule on that
+!                                  qrte, every refined level
 !
 ! Prints "max_abs_diff <value>" (0 expected) and "cycles ...".
 module DropinBuffers
@@ -15,6 +20,10 @@ module DropinBuffers
   integer, save :: pos = 0
   real, save :: worst = 0.0
   integer, save :: mode = 0      ! 0 save q, 1 compare q, 2 save qold, 3 restore q, 4 restore qold
+                                 ! 5 save, 6 compare, 7 poison: q, qold (+ qc, qrte on fulllevel)
+  integer, save :: fulllevel = -1
+  real, save :: tol = 0.0, scale = 0.0, hlev = 0.0
+  integer, save :: nflagged = 0, nbadflags = 0
 end module DropinBuffers
 
 module DropinCallbacks
@@ -27,6 +36,7 @@ contains
     type(nodeinfo)  :: info
     type(funcparam) :: dummy
     integer :: n
+    BufPatch = err_ok
     n = size(info%q)
     select case (mode)
     case (0)
@@ -39,10 +49,69 @@ contains
       info%q = reshape(buf(pos+1:pos+n), shape(info%q))
     case (4)
       info%qold = reshape(buf(pos+1:pos+n), shape(info%qold))
+    case (5, 6, 7)
+      call arr(info%q)
+      call arr(info%qold)
+      if (info%level == fulllevel) then
+        call arr(info%qc)
+        call arr(info%qrte)
+      end if
+      return
     end select
     pos = pos + n
-    BufPatch = err_ok
+  contains
+    subroutine arr(a)
+      real, intent(inout) :: a(:,:,:,:)
+      integer :: m
+      m = size(a)
+      select case (mode)
+      case (5)
+        buf(pos+1:pos+m) = reshape(a, [m])
+      case (6)
+        worst = max(worst, maxval(abs(buf(pos+1:pos+m) - reshape(a, [m]))))
+      case (7)
+        a = 1.0e300
+      end select
+      pos = pos + m
+    end subroutine arr
   end function BufPatch
+
+  ! Largest |qrte| of the patches of fulllevel (mode 0), or the check of
+  ! their error flags against SetErrFlags2D/3D (lib/bsamroutines.f90:
+  ! 1093-1157) on their qrte at tolerance tol (mode 1).
+  integer function FlagPatch(info, dummy)
+    use NodeInfoDef
+    use TreeOps, only: err_ok
+    use DropinBuffers
+    implicit none
+    type(nodeinfo)  :: info
+    type(funcparam) :: dummy
+    integer :: i, j, k, c(3), want, nd
+    FlagPatch = err_ok
+    if (info%level /= fulllevel) return
+    nd = ndims
+    c = 1
+    c(1:nd) = info%mx(1:nd) / 2
+    if (mode == 0) then
+      scale = max(scale, maxval(abs(info%qrte(1:c(1), 1:c(2), 1:c(3), :))))
+      hlev = info%dx(1)
+      return
+    end if
+    do k = 1, info%mx(3)
+      do j = 1, info%mx(2)
+        do i = 1, info%mx(1)
+          want = 0
+          if ((i + 1) / 2 <= c(1) .and. (j + 1) / 2 <= c(2) .and. &
+              (nd == 2 .or. (k + 1) / 2 <= c(3))) then
+            if (maxval(abs(info%qrte((i + 1) / 2, (j + 1) / 2, &
+                                     merge(1, (k + 1) / 2, nd == 2), :))) > tol) want = 1
+          end if
+          if (info%errorflags(i, j, k) /= want) nbadflags = nbadflags + 1
+          nflagged = nflagged + info%errorflags(i, j, k)
+        end do
+      end do
+    end do
+  end function FlagPatch
 
   integer function SizePatch(info, dummy)
     use NodeInfoDef
@@ -51,7 +120,7 @@ contains
     implicit none
     type(nodeinfo)  :: info
     type(funcparam) :: dummy
-    pos = pos + size(info%q)
+    pos = pos + size(info%q) + size(info%qold) + size(info%qc) + size(info%qrte)
     SizePatch = err_ok
   end function SizePatch
 end module DropinCallbacks
@@ -60,8 +129,9 @@ program DropinCheck
   use iso_c_binding
   use NodeInfoDef
   use TreeOps
-  use Boundary, only: SetGhost
-  use AFASRoutines, only: FillDown, MultigridIterations
+  use Boundary, only: SetGhost, GetCoarseGhostPoints
+  use AFASRoutines, only: FillDown, MultigridIterations, RestrictSolution, &
+                          RelativeTruncationError
   use HarnessState
   use HarnessCallbacks
   use HarnessTree
@@ -110,6 +180,39 @@ program DropinCheck
     end do
     print '(a,es12.4)', 'max_abs_diff ', worst
     print '(a,i0)', 'patches ', gpu_npatch
+  else if (trim(what) == 'trunc') then
+    call GpuOpen(0, 0)
+    call GpuDescribeHierarchy()
+    do lev = max(minlevel + 1, 1), finestlevel
+      fulllevel = lev
+      ! the truncation-error calls of EstimateLevelErrors
+      ! (lib/bsamroutines.f90:971-981), on the host
+      call load_inputs()
+      call ApplyOnLevel(lev, RestrictSolution, dummy)
+      call SetGhost(lev - 1, 0)
+      call ApplyOnLevel(lev, GetCoarseGhostPoints, dummy)
+      call ApplyOnLevel(lev, CopyToOld, dummy)
+      call ApplyOnLevel(lev - 1, CopyToOld, dummy)
+      call ApplyOnLevel(lev, RelativeTruncationError, dummy)
+      call walk_levels(5, lev - 1, lev)   ! buf <- reference arrays
+      call load_inputs()
+      call walk_levels(7, lev - 1, lev)   ! poison what the engine must write
+      call GpuRelativeTruncationError(lev)
+      call walk_levels(6, lev - 1, lev)
+      ! default error tagging at a tolerance inside the data's range
+      mode = 0
+      scale = 0.0
+      call ApplyOnLevel(lev, FlagPatch, dummy)
+      qtolerance(lev) = 0.3 * scale * hlev**ndims
+      tol = qtolerance(lev) / hlev / hlev
+      if (ndims == 3) tol = tol / hlev
+      call GpuSetErrFlags(lev)
+      mode = 1
+      call ApplyOnLevel(lev, FlagPatch, dummy)
+    end do
+    if (nbadflags > 0) worst = max(worst, 1.0)
+    print '(a,i0,a,i0)', 'flagged ', nflagged, ' bad_flags ', nbadflags
+    print '(a,es12.4)', 'max_abs_diff ', worst
   else
     call GpuOpen(0, 0)
     call GpuDescribeHierarchy()
@@ -155,6 +258,16 @@ contains
       call ApplyOnLevel(l, BufPatch, dummy)
     end do
   end subroutine walk_buf
+
+  subroutine walk_levels(m, l0, l1)
+    integer, intent(in) :: m, l0, l1
+    integer :: l
+    mode = m
+    pos = 0
+    do l = l0, l1
+      call ApplyOnLevel(l, BufPatch, dummy)
+    end do
+  end subroutine walk_levels
 
   subroutine copy_qold()
     integer :: l

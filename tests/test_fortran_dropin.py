@@ -75,3 +75,18 @@ def test_gpu_multigrid_iterations_replace_reference(tmp_path, lname):
     diff, out = run("gpu", spec, inp, "2")
     assert "converged 1" in out, out
     assert diff == 0.0, out
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("lname", ["amr2d", "amr2d_userbc", "amr3d", "periodic2d", "periodic3d"])
+def test_gpu_truncation_error_replaces_reference(tmp_path, lname):
+    """GpuRelativeTruncationError leaves the reference arrays q, qold, qc and
+    qrte bit for bit as the reference calls of EstimateLevelErrors do, and
+    GpuSetErrFlags' flags follow SetErrFlags on that qrte."""
+    if not os.path.exists(BIN):
+        pytest.skip("dropin_check not built (needs the reference build)")
+    spec, inp = write_case(str(tmp_path), lname, seed=3)
+    diff, out = run("trunc", spec, inp)
+    assert diff == 0.0, out
+    flagged = [l for l in out.splitlines() if l.startswith("flagged")]
+    assert flagged and int(flagged[-1].split()[1]) > 0 and flagged[-1].split()[3] == "0", out
