@@ -183,7 +183,7 @@ program DropinCheck
   else if (trim(what) == 'trunc') then
     call GpuOpen(0, 0)
     call GpuDescribeHierarchy()
-    do lev = max(minlevel + 1, 1), finestlevel
+    do lev = max(minlevel + 1, 0), finestlevel
       fulllevel = lev
       ! the truncation-error calls of EstimateLevelErrors
       ! (lib/bsamroutines.f90:971-981), on the host

@@ -103,11 +103,11 @@ OPS_CASES = {
     "periodic3d_seq": ("periodic3d", 11, OPS),
     "periodic3d_xz_vcycle": ("periodic3d_xz", 12, [("vcycle", 2)]),
     # EstimateLevelErrors' truncation error on the two refined levels
-    "amr2d_truncerr": ("amr2d", 13, [("truncerr", 2), ("truncerr", 1)]),
-    "userbc_truncerr": ("amr2d_userbc", 14, [("truncerr", 2), ("truncerr", 1)]),
-    "amr3d_truncerr": ("amr3d", 15, [("truncerr", 2), ("truncerr", 1)]),
-    "periodic2d_truncerr": ("periodic2d", 16, [("truncerr", 2), ("truncerr", 1)]),
-    "periodic3d_truncerr": ("periodic3d", 17, [("truncerr", 1)]),
+    "amr2d_truncerr": ("amr2d", 13, [("truncerr", 2), ("truncerr", 1), ("truncerr", 0)]),
+    "userbc_truncerr": ("amr2d_userbc", 14, [("truncerr", 2), ("truncerr", 1), ("truncerr", 0)]),
+    "amr3d_truncerr": ("amr3d", 15, [("truncerr", 2), ("truncerr", 1), ("truncerr", 0)]),
+    "periodic2d_truncerr": ("periodic2d", 16, [("truncerr", 2), ("truncerr", 1), ("truncerr", 0)]),
+    "periodic3d_truncerr": ("periodic3d", 17, [("truncerr", 1), ("truncerr", 0)]),
 }
 
 

@@ -173,9 +173,10 @@ def test_tail_schedule_places_levels_in_lds():
 def test_truncation_error_and_flags(name):
     """EstimateLevelErrors' truncation error (bit-identical qrte, q, qold, qc)
     and the default error flags at three tolerances around the data's scale,
-    on every refined level."""
+    on the root level and every refined level (AMR estimates from the root
+    up)."""
     lay = LAYOUTS[name]()
-    for level in range(max(lay.minlevel + 1, 1), lay.finestlevel + 1):
+    for level in range(max(lay.minlevel + 1, 0), lay.finestlevel + 1):
         eng, ora = pair(lay, random_fields(lay, seed=23 + level))
         eng.truncation_error(level)
         ora.truncation_error(level)
