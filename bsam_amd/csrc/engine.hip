@@ -117,6 +117,9 @@ struct bsam_ctx {
   bool host_only = true;
   hipStream_t stream = nullptr;
   bool building = false, committed = false;
+  bool had_commit = false;    // hp/dp/lev/device state describe a committed hierarchy
+  long long commits_reused = 0;
+  std::vector<HPatch> staged;  // patches added since bsam_hierarchy_begin
   std::vector<HPatch> hp;
   std::vector<DPatch> dp;
   std::vector<LevelData> lev;
@@ -410,6 +413,44 @@ static bool plan_tail(bsam_ctx *c, long long budget, hipStream_t s) {
   return true;
 }
 
+// Every field pool starts at zero after a commit.
+static void zero_pools(bsam_ctx *c) {
+  hipStream_t s = c->stream;
+  HIPCK(hipMemsetAsync(c->qrte, 0, sizeof(double) * c->nr, s));
+  HIPCK(hipMemsetAsync(c->eflags, 0, sizeof(int) * c->nfl, s));
+  HIPCK(hipMemsetAsync(c->q, 0, sizeof(double) * c->nq, s));
+  HIPCK(hipMemsetAsync(c->qold, 0, sizeof(double) * c->nq, s));
+  HIPCK(hipMemsetAsync(c->qc, 0, sizeof(double) * c->nc, s));
+  HIPCK(hipMemsetAsync(c->f, 0, sizeof(double) * c->nf, s));
+  HIPCK(hipMemsetAsync(c->rf, 0, sizeof(double) * c->nf, s));
+  HIPCK(hipMemsetAsync(c->ftmp, 0, sizeof(double) * c->nf, s));
+  HIPCK(hipMemsetAsync(c->f_saved, 0, sizeof(double) * c->nf, s));
+  if (c->cfg.precision == BSAM_PREC_MIXED) {
+    HIPCK(hipMemsetAsync(c->e, 0, sizeof(float) * c->nq, s));
+    HIPCK(hipMemsetAsync(c->qb, 0, sizeof(float) * c->nq, s));
+    HIPCK(hipMemsetAsync(c->qce, 0, sizeof(float) * c->nc, s));
+    HIPCK(hipMemsetAsync(c->fe, 0, sizeof(float) * c->nf, s));
+  }
+}
+
+// True if two patch lists describe the same hierarchy (ids aside): same
+// order, levels, parents and geometry.
+static bool same_geometry(const std::vector<HPatch> &a, const std::vector<HPatch> &b) {
+  if (a.size() != b.size()) return false;
+  for (size_t i = 0; i < a.size(); i++) {
+    const HPatch &x = a[i], &y = b[i];
+    if (x.level != y.level || x.parent != y.parent || x.dx != y.dx) return false;
+    for (int d = 0; d < 3; d++) {
+      if (x.mx[d] != y.mx[d] || x.xlower[d] != y.xlower[d]) return false;
+      for (int e = 0; e < 2; e++)
+        if (x.mb[d][e] != y.mb[d][e] || x.mg[d][e] != y.mg[d][e]) return false;
+    }
+    for (int f = 0; f < 6; f++)
+      if (x.mthbc[f] != y.mthbc[f]) return false;
+  }
+  return true;
+}
+
 static void commit(bsam_ctx *c) {
   const int nd = c->nd(), nv = c->cfg.nrvars;
   int n = (int)c->hp.size();
@@ -555,10 +596,14 @@ static void commit(bsam_ctx *c) {
       }
       for (int ip = 0; ip < 3; ip++) L.down_ok[ip] = program_reads_avoid(L.gp[ip], written);
       const LevelData &PL = c->lev[l - 1 - minl];
-      L.ride_cgl = plan_ride(PL.gp[0], coarse_ghost_load_reads(c->hp, L.t_cext, nd, c->tx(),
-                                                               c->ty(), c->tz()));
-      L.ride_corr =
-          plan_ride(PL.gp[2], correct_reads(c->hp, L.t_ext, nd, c->tx(), c->ty(), c->tz()));
+      // only the reads of cells the parent's program writes matter to a ride
+      std::vector<int> at;
+      if (destination_index(PL.gp[0], at))
+        L.ride_cgl = plan_ride(PL.gp[0], coarse_ghost_load_reads(c->hp, L.t_cext, nd, c->tx(),
+                                                                 c->ty(), c->tz(), &at));
+      if (destination_index(PL.gp[2], at))
+        L.ride_corr = plan_ride(PL.gp[2], correct_reads(c->hp, L.t_ext, nd, c->tx(), c->ty(),
+                                                        c->tz(), &at));
     }
   }
   if (c->host_only) return;
@@ -575,24 +620,13 @@ static void commit(bsam_ctx *c) {
   HIPCK(hipMalloc(&c->f_saved, sizeof(double) * c->nf));
   HIPCK(hipMalloc(&c->qrte, sizeof(double) * c->nr));
   HIPCK(hipMalloc(&c->eflags, sizeof(int) * c->nfl));
-  HIPCK(hipMemsetAsync(c->qrte, 0, sizeof(double) * c->nr, s));
-  HIPCK(hipMemsetAsync(c->eflags, 0, sizeof(int) * c->nfl, s));
-  HIPCK(hipMemsetAsync(c->q, 0, sizeof(double) * c->nq, s));
-  HIPCK(hipMemsetAsync(c->qold, 0, sizeof(double) * c->nq, s));
-  HIPCK(hipMemsetAsync(c->qc, 0, sizeof(double) * c->nc, s));
-  HIPCK(hipMemsetAsync(c->f, 0, sizeof(double) * c->nf, s));
-  HIPCK(hipMemsetAsync(c->rf, 0, sizeof(double) * c->nf, s));
-  HIPCK(hipMemsetAsync(c->ftmp, 0, sizeof(double) * c->nf, s));
   if (c->cfg.precision == BSAM_PREC_MIXED) {
     HIPCK(hipMalloc(&c->e, sizeof(float) * c->nq));
     HIPCK(hipMalloc(&c->qb, sizeof(float) * c->nq));
     HIPCK(hipMalloc(&c->qce, sizeof(float) * c->nc));
     HIPCK(hipMalloc(&c->fe, sizeof(float) * c->nf));
-    HIPCK(hipMemsetAsync(c->e, 0, sizeof(float) * c->nq, s));
-    HIPCK(hipMemsetAsync(c->qb, 0, sizeof(float) * c->nq, s));
-    HIPCK(hipMemsetAsync(c->qce, 0, sizeof(float) * c->nc, s));
-    HIPCK(hipMemsetAsync(c->fe, 0, sizeof(float) * c->nf, s));
   }
+  zero_pools(c);
   c->d_P = dev_upload(c->dp, s);
   size_t maxt = 1;
   for (auto &L : c->lev) {
@@ -1056,10 +1090,7 @@ const char *bsam_last_error(const bsam_ctx *c) { return c ? c->err.c_str() : g_e
 int bsam_hierarchy_begin(bsam_ctx *c) {
   API_BEGIN(c)
   if (!c->host_only) HIPCK(hipStreamSynchronize(c->stream));
-  c->free_device();
-  c->hp.clear();
-  c->dp.clear();
-  c->lev.clear();
+  c->staged.clear();
   c->committed = false;
   c->building = true;
   API_END(c)
@@ -1087,7 +1118,7 @@ int bsam_add_patch(bsam_ctx *c, int id, int level, int parent, const int *mx,
   }
   for (int fc = 0; fc < 6; fc++) h.mthbc[fc] = fc < 2 * nd ? mthbc[fc] : 1;
   h.dx = dx;
-  c->hp.push_back(h);
+  c->staged.push_back(h);
   API_END(c)
 }
 
@@ -1095,7 +1126,23 @@ int bsam_hierarchy_commit(bsam_ctx *c) {
   API_BEGIN(c)
   if (!c->building) throw Error("call bsam_hierarchy_begin first");
   if (!c->host_only) HIPCK(hipSetDevice(c->cfg.device));
-  commit(c);
+  if (c->had_commit && same_geometry(c->hp, c->staged)) {
+    // The regrid produced the hierarchy already committed (the reference
+    // regrids every step, mostly to the same patches): keep its plans,
+    // tables, pools and graphs, and zero the pools as a fresh commit does.
+    for (size_t i = 0; i < c->hp.size(); i++) c->hp[i].id = c->staged[i].id;
+    if (!c->host_only) zero_pools(c);
+    c->commits_reused++;
+  } else {
+    c->had_commit = false;
+    c->free_device();
+    c->hp = std::move(c->staged);
+    c->dp.clear();
+    c->lev.clear();
+    commit(c);
+    c->had_commit = true;
+  }
+  c->staged.clear();
   c->building = false;
   c->committed = true;
   API_END(c)
@@ -1357,6 +1404,7 @@ int bsam_tune_get(const bsam_ctx *cc, const char *key, double *value) {
   else if (k == "tail_lds") *value = (double)c->tail_lds_cap;
   else if (k == "inner_cycles") *value = c->inner_cycles;
   else if (k == "graph") *value = c->use_graph;
+  else if (k == "commits_reused") *value = (double)c->commits_reused;
   else if (k == "fuse_ghost") *value = c->fuse_ghost;
   else if (k == "tile_x") *value = c->tx();
   else if (k == "tile_y") *value = c->ty();

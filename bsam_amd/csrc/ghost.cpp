@@ -47,9 +47,37 @@ inline R S(int x) { return R{x, x, 1}; }
 inline R G(int s, int e, int st) { return R{s, e, st}; }
 using Sec = std::array<R, 3>;
 
+// Ghost-cell expressions of one patch, keyed by ext-linear index: a dense
+// slot table over the ghost-extended box and the entries in insertion order
+// (sorted by key once the program is complete).
+struct GhostMap {
+  std::vector<int> slot;
+  std::vector<std::pair<long long, Expr>> ent;
+  void init(long long extvol) { slot.assign((size_t)extvol, -1); }
+  Expr &operator[](long long k) {
+    int &s = slot[(size_t)k];
+    if (s < 0) {
+      s = (int)ent.size();
+      ent.emplace_back(k, Expr());
+    }
+    return ent[(size_t)s].second;
+  }
+  const Expr *find(long long k) const {
+    int s = slot[(size_t)k];
+    return s < 0 ? nullptr : &ent[(size_t)s].second;
+  }
+  // entry numbers by ascending key
+  std::vector<int> order() const {
+    std::vector<int> o(ent.size());
+    for (size_t n = 0; n < o.size(); n++) o[n] = (int)n;
+    std::sort(o.begin(), o.end(), [&](int a, int b) { return ent[(size_t)a].first < ent[(size_t)b].first; });
+    return o;
+  }
+};
+
 struct PatchMap {
   int p = -1;
-  std::map<long long, Expr> m;  // ext-linear index -> expression
+  GhostMap m;  // ext-linear index -> expression
 };
 
 struct Builder {
@@ -481,8 +509,7 @@ struct Builder {
   // interior cell or an untouched ghost is a plain copy of its old value.
   Expr source(PatchMap &pm, const HPatch &h, int i, int j, int k) {
     if (!interior(h, i, j, k)) {
-      auto it = pm.m.find(key(h, i, j, k));
-      if (it != pm.m.end()) return it->second;
+      if (const Expr *e = pm.m.find(key(h, i, j, k))) return *e;
     }
     Expr e;
     e.kind = 0;
@@ -585,9 +612,9 @@ struct Builder {
             e.f0 = Cell{pm.p, si[0], si[1], si[2]};
             e.nflip = flip;
           } else {
-            auto it = pm.m.find(key(h, si[0], si[1], si[2]));
-            if (it != pm.m.end()) {
-              e = it->second;
+            const Expr *it = pm.m.find(key(h, si[0], si[1], si[2]));
+            if (it) {
+              e = *it;
               e.nflip += flip;
             } else {
               e.kind = 0;
@@ -632,7 +659,11 @@ int build_ghost_program(const std::vector<HPatch> &hp, const std::vector<int> &p
   out.clear();
   Builder B(hp, ndims, msg);
   std::vector<PatchMap> maps(patches.size());
-  for (size_t a = 0; a < patches.size(); a++) maps[a].p = patches[a];
+  for (size_t a = 0; a < patches.size(); a++) {
+    maps[a].p = patches[a];
+    const HPatch &h = hp[patches[a]];
+    maps[a].m.init((long long)(h.mx[0] + 2) * (h.mx[1] + 2) * (ndims == 3 ? h.mx[2] + 2 : 1));
+  }
   if (level > 0) {
     for (auto &pm : maps) {
       if (ndims == 2) B.interp2d(pm, ipass);
@@ -658,7 +689,8 @@ int build_ghost_program(const std::vector<HPatch> &hp, const std::vector<int> &p
   for (auto &pm : maps) B.physical_bc(pm);
   for (auto &pm : maps) {
     const HPatch &h = hp[pm.p];
-    for (auto &kv : pm.m) {
+    for (int n : pm.m.order()) {
+      const auto &kv = pm.m.ent[(size_t)n];
       const Expr &e = kv.second;
       Cell dc = B.cell_of(h, pm.p, kv.first);
       if (B.interior(h, dc.i, dc.j, dc.k)) {
@@ -684,14 +716,20 @@ int build_ghost_program(const std::vector<HPatch> &hp, const std::vector<int> &p
     }
   }
   // every source must be a value the program leaves untouched
-  std::unordered_set<int> written;
-  for (auto &g : out) written.insert(g.dst);
+  std::vector<char> written;
+  {
+    size_t n = 0;
+    for (auto &g : out) n = std::max(n, (size_t)g.dst + 1);
+    written.assign(n, 0);
+    for (auto &g : out) written[(size_t)g.dst] = 1;
+  }
+  auto wr = [&](int off) { return off >= 0 && (size_t)off < written.size() && written[(size_t)off]; };
   for (auto &g : out) {
-    bool bad = written.count(g.f0) > 0;
+    bool bad = wr(g.f0);
     if (g.kind > 0) {
-      bad = bad || written.count(g.f1) > 0;
+      bad = bad || wr(g.f1);
       int nc = g.kind == 1 ? 1 : (g.kind == 2 ? 3 : 4);
-      for (int q = 0; q < nc; q++) bad = bad || written.count(g.c[q]) > 0;
+      for (int q = 0; q < nc; q++) bad = bad || wr(g.c[q]);
     }
     if (bad) {
       msg = "ghost program reads a cell it writes";
@@ -723,9 +761,9 @@ FusedGhost plan_fused_ghost(const std::vector<HPatch> &hp, const std::vector<Til
     k = (int)(r / (h.px * h.py));
   };
   // first interior tile of every patch; tiles of a patch are k, j, i nested
-  std::unordered_map<int, int> first;
+  std::vector<int> first(hp.size(), -1);
   for (size_t t = 0; t < tiles.size(); t++)
-    if (!first.count(tiles[t].patch)) first[tiles[t].patch] = (int)t;
+    if (first[tiles[t].patch] < 0) first[tiles[t].patch] = (int)t;
   auto inside = [&](int p, int i, int j, int k) {
     const HPatch &h = hp[p];
     return i >= 1 && i <= h.mx[0] && j >= 1 && j <= h.mx[1] &&
@@ -735,16 +773,21 @@ FusedGhost plan_fused_ghost(const std::vector<HPatch> &hp, const std::vector<Til
     const HPatch &h = hp[p];
     int nx = (h.mx[0] + tx - 1) / tx, ny = (h.mx[1] + ty - 1) / ty;
     int a = (i - 1) / tx, b = (j - 1) / ty, cz = nd == 3 ? (k - 1) / tz : 0;
-    return first.at(p) + (cz * ny + b) * nx + a;
+    return first[p] + (cz * ny + b) * nx + a;
   };
   auto relaxed = [&](int i, int j, int k) { return ((i + j + (nd == 3 ? k : 0) + color) & 1) == 1; };
-  std::unordered_set<int> next_dst;
-  for (const GhostOp &g : next) next_dst.insert(g.dst);
+  std::vector<char> next_dst;   // destinations of the next program
+  {
+    size_t n = 0;
+    for (const GhostOp &g : next) n = std::max(n, (size_t)g.dst + 1);
+    next_dst.assign(n, 0);
+    for (const GhostOp &g : next) next_dst[(size_t)g.dst] = 1;
+  }
   std::vector<std::vector<GhostOp>> per(tiles.size());
   for (const GhostOp &g : prog) {
     int p, i, j, k;
     locate(g.dst, p, i, j, k);
-    if (p < 0 || !first.count(p)) {
+    if (p < 0 || first[p] < 0) {
       why = "ghost destination outside the level's patches";
       return out;
     }
@@ -758,7 +801,7 @@ FusedGhost plan_fused_ghost(const std::vector<HPatch> &hp, const std::vector<Til
     }
     // the interior cell a face ghost borders is the only reader of the ghost
     bool read = outside == 1 && relaxed(c[0], c[1], nd == 3 ? c[2] : 0);
-    if (!read && next_dst.count(g.dst)) continue;
+    if (!read && (size_t)g.dst < next_dst.size() && next_dst[(size_t)g.dst]) continue;
     int owner = tile_of(p, c[0], c[1], c[2]);
     long long srcs[2] = {g.f0, g.f1};
     int nsrc = g.kind == 0 ? 1 : 2;
@@ -769,7 +812,7 @@ FusedGhost plan_fused_ghost(const std::vector<HPatch> &hp, const std::vector<Til
         why = "ghost source outside the q pool";
         return out;
       }
-      if (!first.count(sp) || !inside(sp, si, sj, sk)) continue;   // not relaxed here
+      if (first[sp] < 0 || !inside(sp, si, sj, sk)) continue;   // not relaxed here
       if (!relaxed(si, sj, nd == 3 ? sk : 0)) continue;
       if (tile_of(sp, si, sj, sk) != owner) {
         why = "an update reads a cell relaxed by another tile";
@@ -798,9 +841,37 @@ static long long parent_offset(const std::vector<HPatch> &hp, const HPatch &h, i
   return pa.qoff + pi + pa.px * (pj + pa.py * pk);
 }
 
+bool destination_index(const std::vector<GhostOp> &prog, std::vector<int> &at) {
+  long long n = 0;
+  for (const GhostOp &g : prog) n = std::max(n, (long long)g.dst + 1);
+  at.assign((size_t)n, -1);
+  for (size_t u = 0; u < prog.size(); u++) {
+    int &a = at[(size_t)prog[u].dst];
+    if (a >= 0) return false;
+    a = (int)u;
+  }
+  return true;
+}
+
+static inline bool listed(const std::vector<int> *only, long long off) {
+  return !only || (off >= 0 && off < (long long)only->size() && (*only)[(size_t)off] >= 0);
+}
+
+// True if the parent cells (child coarse indices lo..hi per dimension, as
+// parent_offset takes them) are all interior cells of the parent: a SetGhost
+// program of the parent level writes none of them.
+static bool parent_interior(const std::vector<HPatch> &hp, const HPatch &h, int nd,
+                            const int lo[3], const int hi[3]) {
+  const HPatch &pa = hp[h.parent];
+  for (int d = 0; d < nd; d++)
+    if (h.mb[d][0] - 1 + lo[d] < 1 || h.mb[d][0] - 1 + hi[d] > pa.mx[d]) return false;
+  return true;
+}
+
 std::vector<std::vector<long long>> coarse_ghost_load_reads(const std::vector<HPatch> &hp,
                                                             const std::vector<Tile> &tiles,
-                                                            int nd, int tx, int ty, int tz) {
+                                                            int nd, int tx, int ty, int tz,
+                                                            const std::vector<int> *only) {
   std::vector<std::vector<long long>> out(tiles.size());
   for (size_t t = 0; t < tiles.size(); t++) {
     const Tile &tl = tiles[t];
@@ -810,6 +881,13 @@ std::vector<std::vector<long long>> coarse_ghost_load_reads(const std::vector<HP
       return a == 0 || a == e[0] || b == 0 || b == e[1] || (nd == 3 && (c == 0 || c == e[2]));
     };
     std::vector<long long> &r = out[t];
+    if (only) {
+      // every cell read lies within one cell of the tile
+      int lo[3] = {tl.i0 - 1, tl.j0 - 1, nd == 3 ? tl.k0 - 1 : 0};
+      int hi[3] = {std::min(tl.i0 + tx, e[0]), std::min(tl.j0 + ty, e[1]),
+                   nd == 3 ? std::min(tl.k0 + tz, e[2]) : 0};
+      if (parent_interior(hp, h, nd, lo, hi)) continue;
+    }
     for (int k = tl.k0; k < tl.k0 + (nd == 3 ? tz : 1) && k <= e[2]; k++)
       for (int j = tl.j0; j < tl.j0 + ty && j <= e[1]; j++)
         for (int i = tl.i0; i < tl.i0 + tx && i <= e[0]; i++)
@@ -821,7 +899,8 @@ std::vector<std::vector<long long>> coarse_ghost_load_reads(const std::vector<HP
             int a = i + d[s][0], b = j + d[s][1], c = k + d[s][2];
             if (a < 0 || a > e[0] || b < 0 || b > e[1] || c < 0 || c > e[2]) continue;
             if (!ring(a, b, c)) continue;
-            r.push_back(parent_offset(hp, h, nd, a, b, c));
+            long long off = parent_offset(hp, h, nd, a, b, c);
+            if (listed(only, off)) r.push_back(off);
           }
     std::sort(r.begin(), r.end());
     r.erase(std::unique(r.begin(), r.end()), r.end());
@@ -831,22 +910,32 @@ std::vector<std::vector<long long>> coarse_ghost_load_reads(const std::vector<HP
 
 std::vector<std::vector<long long>> correct_reads(const std::vector<HPatch> &hp,
                                                   const std::vector<Tile> &tiles, int nd,
-                                                  int tx, int ty, int tz) {
+                                                  int tx, int ty, int tz,
+                                                  const std::vector<int> *only) {
   std::vector<std::vector<long long>> out(tiles.size());
   for (size_t t = 0; t < tiles.size(); t++) {
     const Tile &tl = tiles[t];
     const HPatch &h = hp[tl.patch];
     const int e[3] = {h.mx[0] + 1, h.mx[1] + 1, nd == 3 ? h.mx[2] + 1 : 0};
     std::vector<long long> &r = out[t];
+    if (only) {
+      int lo[3] = {tl.i0 >> 1, tl.j0 >> 1, nd == 3 ? tl.k0 >> 1 : 0};
+      int hi[3] = {(std::min(tl.i0 + tx - 1, e[0]) >> 1) + 1,
+                   (std::min(tl.j0 + ty - 1, e[1]) >> 1) + 1,
+                   nd == 3 ? (std::min(tl.k0 + tz - 1, e[2]) >> 1) + 1 : 0};
+      if (parent_interior(hp, h, nd, lo, hi)) continue;
+    }
     // fine cell (i, j, k) reads parent cells i/2 + {0, 1} per dimension
     for (int k = tl.k0; k < tl.k0 + (nd == 3 ? tz : 1) && k <= e[2]; k++)
       for (int j = tl.j0; j < tl.j0 + ty && j <= e[1]; j++)
         for (int i = tl.i0; i < tl.i0 + tx && i <= e[0]; i++)
           for (int c = 0; c < (nd == 3 ? 2 : 1); c++)
             for (int b = 0; b < 2; b++)
-              for (int a = 0; a < 2; a++)
-                r.push_back(parent_offset(hp, h, nd, (i >> 1) + a, (j >> 1) + b,
-                                          nd == 3 ? (k >> 1) + c : 0));
+              for (int a = 0; a < 2; a++) {
+                long long off = parent_offset(hp, h, nd, (i >> 1) + a, (j >> 1) + b,
+                                              nd == 3 ? (k >> 1) + c : 0);
+                if (listed(only, off)) r.push_back(off);
+              }
     std::sort(r.begin(), r.end());
     r.erase(std::unique(r.begin(), r.end()), r.end());
   }
@@ -856,19 +945,16 @@ std::vector<std::vector<long long>> correct_reads(const std::vector<HPatch> &hp,
 RideGhost plan_ride(const std::vector<GhostOp> &prog,
                     const std::vector<std::vector<long long>> &reads) {
   RideGhost out;
-  std::unordered_map<long long, int> at;   // destination -> update
-  for (size_t n = 0; n < prog.size(); n++) {
-    if (at.count(prog[n].dst)) return out;   // one update per destination
-    at[prog[n].dst] = (int)n;
-  }
+  std::vector<int> at;   // destination -> update
+  if (!destination_index(prog, at)) return out;   // one update per destination
   std::vector<char> read_somewhere(prog.size(), 0);
   out.start.assign(reads.size() + 1, 0);
   for (size_t t = 0; t < reads.size(); t++) {
     for (long long off : reads[t]) {
-      auto it = at.find(off);
-      if (it == at.end()) continue;
-      out.ops.push_back(prog[it->second]);
-      read_somewhere[it->second] = 1;
+      if (off < 0 || off >= (long long)at.size() || at[(size_t)off] < 0) continue;
+      int u = at[(size_t)off];
+      out.ops.push_back(prog[u]);
+      read_somewhere[u] = 1;
     }
     out.start[t + 1] = (int)out.ops.size();
   }

@@ -52,13 +52,19 @@ struct RideGhost {
 };
 // Parent q-pool offsets (variable 1) that each tile of the child level's
 // coarse ghost + loading launch (ghost-extended coarse tiles, `tiles`) or
-// correction launch (ghost-extended fine tiles) may read.
+// correction launch (ghost-extended fine tiles) may read, ascending; with
+// `only`, just the offsets off with only[off] >= 0.
 std::vector<std::vector<long long>> coarse_ghost_load_reads(const std::vector<HPatch> &hp,
                                                             const std::vector<Tile> &tiles,
-                                                            int ndims, int tx, int ty, int tz);
+                                                            int ndims, int tx, int ty, int tz,
+                                                            const std::vector<int> *only = nullptr);
 std::vector<std::vector<long long>> correct_reads(const std::vector<HPatch> &hp,
                                                   const std::vector<Tile> &tiles, int ndims,
-                                                  int tx, int ty, int tz);
+                                                  int tx, int ty, int tz,
+                                                  const std::vector<int> *only = nullptr);
+// at[off] = the update of `prog` that writes q-pool offset off, -1 where
+// none; false if two updates share a destination.
+bool destination_index(const std::vector<GhostOp> &prog, std::vector<int> &at);
 RideGhost plan_ride(const std::vector<GhostOp> &prog,
                     const std::vector<std::vector<long long>> &reads);
 

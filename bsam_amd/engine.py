@@ -137,6 +137,13 @@ class Engine:
             raise BsamError(L.bsam_last_error(None).decode())
         for key, val in (tune or {}).items():
             self._ck(L.bsam_tune(self.ctx, key.encode(), float(val)))
+        self.describe(layout)
+
+    def describe(self, layout: Layout):
+        """Hands the engine a hierarchy (after a regrid). Patch fields start
+        at zero; a hierarchy equal to the current one keeps its plans."""
+        L = lib()
+        self.layout = layout
         self.patches = layout.ordered()
         idx = layout.index_map()
         self._ck(L.bsam_hierarchy_begin(self.ctx))
@@ -262,7 +269,8 @@ class Engine:
 
     def tune_get(self, key: str) -> float:
         """A tuning value, or a schedule property: tail_top, tail_lds_fp64,
-        tail_lds_fp32."""
+        tail_lds_fp32, commits_reused (hand-overs that kept the committed
+        hierarchy)."""
         v = C.c_double()
         self._ck(lib().bsam_tune_get(self.ctx, key.encode(), C.byref(v)))
         return v.value

@@ -128,6 +128,13 @@ module bsam_gpu
       integer(c_long), value :: n
       integer(c_int) :: rc
     end function
+    function bsam_tune_get(ctx, key, value) bind(C, name='bsam_tune_get') result(rc)
+      import :: c_ptr, c_int, c_double, c_char
+      type(c_ptr), value :: ctx
+      character(kind=c_char), intent(in) :: key(*)
+      real(c_double), intent(out) :: value
+      integer(c_int) :: rc
+    end function
     function bsam_sync(ctx) bind(C, name='bsam_sync') result(rc)
       import :: c_ptr, c_int
       type(c_ptr), value :: ctx

@@ -80,7 +80,10 @@ const char *bsam_last_error(const bsam_ctx *ctx);
  * mthbc[6], dx, xlower[3]; parent is the parent's patch index (-1 for the
  * coarsest patch). Commit builds the device patch table, the ghost-fill
  * programs and the tile lists, and allocates field storage (q, qold zeroed as
- * in AllocFields, bsamstorage.f90:116-117). */
+ * in AllocFields, bsamstorage.f90:116-117). A commit whose patch list equals
+ * the committed one in order and geometry (ids may differ: MakeNewGrid draws
+ * them at random, bsamroutines.f90:1665-1666) keeps the plans, tables,
+ * storage and graphs and only zeroes the fields. */
 int bsam_hierarchy_begin(bsam_ctx *ctx);
 int bsam_add_patch(bsam_ctx *ctx, int id, int level, int parent, const int *mx,
                    const int *mbounds, const int *mglobal, const int *mthbc,
@@ -175,7 +178,8 @@ int bsam_tune(bsam_ctx *ctx, const char *key, double value);
 /* Reads a tuning value back, or a property of the committed schedule:
  * "tail_top" (highest level run by the tail kernel; below minlevel when
  * none), "tail_lds_fp64" and "tail_lds_fp32" (dynamic LDS bytes of the tail
- * kernels). No counterpart in the reference. */
+ * kernels), "commits_reused" (commits that found the hierarchy already
+ * committed). No counterpart in the reference. */
 int bsam_tune_get(const bsam_ctx *ctx, const char *key, double *value);
 
 /* Kernel timing. Between bsam_profile_begin and bsam_profile_end, every

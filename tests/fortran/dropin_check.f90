@@ -20,7 +20,8 @@ module DropinBuffers
   integer, save :: pos = 0
   real, save :: worst = 0.0
   integer, save :: mode = 0      ! 0 save q, 1 compare q, 2 save qold, 3 restore q, 4 restore qold
-                                 ! 5 save, 6 compare, 7 poison: q, qold (+ qc, qrte on fulllevel)
+                                 ! 5 save, 6 compare: q, qold (+ qc, qrte on fulllevel);
+                                 ! 7 poisons the same arrays but q (the input)
   integer, save :: fulllevel = -1
   real, save :: tol = 0.0, scale = 0.0, hlev = 0.0
   integer, save :: nflagged = 0, nbadflags = 0
@@ -50,7 +51,7 @@ contains
     case (4)
       info%qold = reshape(buf(pos+1:pos+n), shape(info%qold))
     case (5, 6, 7)
-      call arr(info%q)
+      if (mode /= 7) call arr(info%q)
       call arr(info%qold)
       if (info%level == fulllevel) then
         call arr(info%qc)

@@ -11,6 +11,7 @@ import pytest
 
 import orc
 from bsam_amd.engine import Engine
+from bsam_amd.layout import Layout
 from helpers import (layout_amr2d, layout_amr2d_tiles, layout_amr3d, layout_periodic2d,
                      layout_periodic3d, layout_single, noise_q, random_fields)
 
@@ -190,3 +191,63 @@ def test_truncation_error_and_flags(name):
             for n in pats:
                 assert np.array_equal(eng.flags(n), ora.flags(n)), (level, frac, n)
         eng.close()
+
+
+@pytest.mark.parametrize("name", ["amr2d", "amr3d", "periodic2d_xy"])
+def test_recommit_same_hierarchy(name):
+    """A regrid that hands over the hierarchy already committed keeps its
+    plans and graphs (commits_reused counts it) and restarts from zeroed
+    fields: two time steps after it equal a fresh context's bit for bit. A
+    changed hierarchy is committed afresh."""
+    lay = LAYOUTS[name]()
+    q0 = noise_q(lay, seed=31)
+    q1 = noise_q(lay, seed=32)
+    eng = Engine(lay, device=0, **KW)
+    for n, p in enumerate(eng.patches):
+        eng.upload(n, "q", q0[p.id])
+    eng.time_step()
+    eng.describe(LAYOUTS[name]())
+    assert eng.tune_get("commits_reused") == 1
+    fresh = Engine(lay, device=0, **KW)
+    for e in (eng, fresh):
+        for n, p in enumerate(e.patches):
+            e.upload(n, "q", q1[p.id])
+    for _ in range(2):
+        h1, _ = eng.time_step()
+        h2, _ = fresh.time_step()
+        assert h1 == h2
+    for n in range(len(eng.patches)):
+        for field in ("q", "qold", "f"):
+            assert np.array_equal(eng.download(n, field), fresh.download(n, field)), (n, field)
+    fresh.close()
+    root = lay.patches[0]
+    other = Layout(lay.ndims, root.mx, root.dx, xlower=root.xlower, mthbc=root.mthbc,
+                   minlevel=lay.minlevel)
+    eng.describe(other)
+    assert eng.tune_get("commits_reused") == 1
+    eng.close()
+
+
+@pytest.mark.parametrize("name", ["amr2d", "amr3d", "periodic2d_x_user_y"])
+def test_frame_export(tmp_path, name):
+    """A frame written from the engine's q after two time steps is the frame
+    the oracle's q gives, byte for byte, and reads back to those arrays."""
+    from bsam_amd import frames
+    lay = LAYOUTS[name]()
+    q0 = noise_q(lay, seed=41)
+    eng, ora = pair(lay, {k: {"q": v} for k, v in q0.items()}, names=("q",))
+    for _ in range(2):
+        eng.time_step()
+        ora.time_step()
+    root = lay.patches[0]
+    xu = [root.xlower[d] + root.mx[d] * root.dx for d in range(lay.ndims)]
+    frames.write_frame(str(tmp_path / "gpu.dat"), 0.002, lay.finestlevel,
+                       frames.frame_from_engine(eng, xu))
+    qo = {p.id: ora.field(n, "q") for n, p in enumerate(ora.patches) if p.level >= 0}
+    frames.write_frame(str(tmp_path / "ora.dat"), 0.002, lay.finestlevel,
+                       frames.frame_from_arrays(lay, qo, xu))
+    assert (tmp_path / "gpu.dat").read_bytes() == (tmp_path / "ora.dat").read_bytes()
+    _, _, ps = frames.read_frame(str(tmp_path / "gpu.dat"))
+    for fp in ps:
+        assert np.isfinite(fp.q).all()
+    eng.close()
