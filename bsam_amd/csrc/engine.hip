@@ -133,6 +133,9 @@ struct bsam_ctx {
   int inner_cycles = 1;      // FP32 V-cycles per defect-correction step
   DPatch *d_P = nullptr;
   double *d_partial = nullptr;
+  double *d_stats = nullptr;   // per-patch sums of the after-step statistics
+  int *d_stats_list = nullptr;  // their patches: levels finest..0, level-list order
+  int n_stats = 0;
   int partial_cap = 0;
   double *d_hist = nullptr;
   int *d_state = nullptr;
@@ -227,6 +230,9 @@ void bsam_ctx::free_device() {
   e = qce = fe = qb = nullptr;
   F(d_state);
   F(f_saved);
+  F(d_stats);
+  F(d_stats_list);
+  n_stats = 0;
   F(qrte);
   F(eflags);
   F(d_all_int);
@@ -1247,6 +1253,73 @@ int bsam_error_flags(bsam_ctx *c, int level, double qtolerance) {
   double tol = qtolerance / h / h;
   if (c->nd() == 3) tol = tol / h;
   HIPCK(launch_error_flags(c->nd(), c->dev(), L.d_int, (int)L.t_int.size(), tol, c->stream));
+  API_END(c)
+}
+
+int bsam_error_flags_user(bsam_ctx *c, int level, double qtolerance) {
+  API_BEGIN(c)
+  c->need_device();
+  if (level < c->cfg.minlevel || level > c->finest) throw Error("error_flags_user: bad level");
+  LevelData &L = c->L(level);
+  HIPCK(launch_error_flags_user(c->nd(), c->dev(), L.d_int, (int)L.t_int.size(), qtolerance,
+                                c->stream));
+  API_END(c)
+}
+
+int bsam_after_step_stats(bsam_ctx *c, double *mass, double *energy) {
+  API_BEGIN(c)
+  c->need_device();
+  if (!mass || !energy) throw Error("null output pointer");
+  const int nd = c->nd();
+  std::vector<int> list;
+  for (int l = c->finest; l >= 0; l--)
+    for (int p : c->L(l).patches) list.push_back(p);
+  if (c->n_stats != (int)list.size()) {
+    HIPCK(hipStreamSynchronize(c->stream));
+    if (c->d_stats) (void)hipFree(c->d_stats);
+    if (c->d_stats_list) (void)hipFree(c->d_stats_list);
+    c->d_stats = nullptr;
+    c->d_stats_list = nullptr;
+    c->n_stats = 0;
+    HIPCK(hipMalloc(&c->d_stats, sizeof(double) * 6 * std::max<size_t>(1, list.size())));
+    HIPCK(hipMalloc(&c->d_stats_list, sizeof(int) * std::max<size_t>(1, list.size())));
+    c->n_stats = (int)list.size();
+  }
+  HIPCK(hipMemcpyAsync(c->d_stats_list, list.data(), sizeof(int) * list.size(),
+                       hipMemcpyHostToDevice, c->stream));
+  // minima of the double well whose convex and concave parts are Dfc and Dfe
+  const double w = std::sqrt(c->cfg.fe_num / c->cfg.fe_den / c->cfg.fc_alpha);
+  const double lo = c->cfg.fc_shift - w, hi = c->cfg.fc_shift + w;
+  HIPCK(launch_after_step(nd, c->dev(), c->d_stats_list, (int)list.size(), lo, hi, c->d_stats,
+                          c->stream));
+  std::vector<double> t(6 * list.size());
+  HIPCK(hipMemcpyAsync(t.data(), c->d_stats, sizeof(double) * t.size(), hipMemcpyDeviceToHost,
+                       c->stream));
+  HIPCK(hipStreamSynchronize(c->stream));
+  // AfterStep2D/3D's accumulation, patch by patch in the reference's order
+  double m = 0.0, e = 0.0;
+  for (size_t n = 0; n < list.size(); n++) {
+    const HPatch &h = c->hp[list[n]];
+    const double *s = &t[6 * n];
+    const double hh = h.dx, ch = hh * 2.0;
+    if (nd == 2) {
+      const double h2 = hh * hh, ch2 = ch * ch;
+      const double ef = h2 * (s[2] + c->cfg.eps2 / h2 / 4.0 * s[3]);
+      if (h.level == 0) {
+        m = m + h2 * s[0];
+        e = e + ef;
+      } else {
+        const double ec = ch2 * (s[4] + c->cfg.eps2 / ch2 / 4.0 * s[5]);
+        m = m + h2 * s[0] - ch2 * s[1];
+        e = e + ef - ec;
+      }
+    } else {
+      const double h3 = hh * hh * hh, ch3 = ch * ch * ch;
+      m = h.level == 0 ? m + h3 * s[0] : m + h3 * s[0] - ch3 * s[1];
+    }
+  }
+  *mass = m;
+  *energy = e;
   API_END(c)
 }
 

@@ -847,6 +847,109 @@ __global__ __launch_bounds__(256) void k_error_flags(Dev d, const Tile *tiles, i
       flag;
 }
 
+// SetErrFlagsUser2D/3D (problems/simple_cahn-hilliard/problem.f90:135-163,
+// :425-452) over the interior tiles of a level: a cell is flagged where the
+// undivided central gradient of variable 1, sqrt(gx^2 + gy^2 [+ gz^2]), is
+// strictly above qtolerance(level), and cleared elsewhere.
+template <int ND>
+__global__ __launch_bounds__(256) void k_error_flags_user(Dev d, const Tile *tiles, int nt,
+                                                          double qtol) {
+  const int b = xcd_tile(nt);
+  if (b >= nt) return;
+  Tile t = tiles[b];
+  const DPatch &P = d.P[t.patch];
+  int i, j, k;
+  tile_cell<ND>(d, t, i, j, k);
+  if (!in_box<ND>(i, j, k, P.mx, P.my, P.mz)) return;
+  const long long sy = P.px, sz = (long long)P.px * P.py;
+  const double *q = d.q + P.q + i + sy * j + (ND == 3 ? sz * k : 0);
+  double gx = q[1] - q[-1];
+  double gy = q[sy] - q[-sy];
+  double s2 = gx * gx + gy * gy;
+  if constexpr (ND == 3) {
+    double gz = q[sz] - q[-sz];
+    s2 = s2 + gz * gz;
+  }
+  d.eflags[P.fl + (i - 1) + (long long)P.mx * ((j - 1) + (long long)P.my * (ND == 3 ? k - 1 : 0))] =
+      fabs(sqrt(s2)) > qtol ? 1 : 0;
+}
+
+// The sums of AfterStep2D/3D (problem.f90:102-134, :398-424) for one patch
+// per workgroup (patch list `pl`): out[6n..6n+5] = sum of q1 over the
+// interior; sum of the parent's q1 under the patch (refined patches); and in
+// 2D the two sums of Energy2D (problem.f90:330-361) -- of Ff(c) and of the
+// four squared differences per cell -- for the patch and for the parent's
+// cells under it (ghost ring included). Each sum is reduced in a fixed order
+// (strided per thread, then the wave and workgroup trees), so repeated calls
+// agree bit for bit.
+__device__ __forceinline__ double well_ff(double c, double lo, double hi, double alpha) {
+  double c0 = c - lo, cm1 = c - hi;
+  return c0 * c0 * cm1 * cm1 * alpha / 4.0;
+}
+
+template <int ND>
+__device__ __forceinline__ void energy_sums(const double *q, long long px, int m1, int m2,
+                                            double lo, double hi, double alpha, double &sff,
+                                            double &sg) {
+  // q points at c(0, 0); cells 1..m1 x 1..m2
+  const int n = m1 * m2;
+  for (int e = threadIdx.x; e < n; e += blockDim.x) {
+    const int i = 1 + e % m1, j = 1 + e / m1;
+    const double *c = q + i + px * j;
+    sff += well_ff(c[0], lo, hi, alpha);
+    double a = c[1] - c[0], b = c[0] - c[-1], u = c[px] - c[0], w = c[0] - c[-px];
+    sg += a * a + b * b + u * u + w * w;
+  }
+}
+
+__device__ __forceinline__ double block_sum(double v, double *lds) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  const int wv = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) lds[wv] = v;
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < nw; w++) t += lds[w];
+  return t;
+}
+
+template <int ND>
+__global__ __launch_bounds__(256) void k_after_step(Dev d, const int *pl, double lo, double hi,
+                                                    double *out) {
+  __shared__ double lds[4];
+  const DPatch &P = d.P[pl[blockIdx.x]];
+  const long long sy = P.px, sz = (long long)P.px * P.py;
+  const double alpha = d.prm.fc_alpha;
+  double s[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  const int nz = ND == 3 ? P.mz : 1;
+  const long long n = (long long)P.mx * P.my * nz;
+  for (long long e = threadIdx.x; e < n; e += blockDim.x) {
+    const int i = 1 + (int)(e % P.mx), j = 1 + (int)((e / P.mx) % P.my);
+    const int k = ND == 3 ? 1 + (int)(e / ((long long)P.mx * P.my)) : 0;
+    s[0] += d.q[P.q + i + sy * j + sz * k];
+  }
+  if constexpr (ND == 2) energy_sums<ND>(d.q + P.q, sy, P.mx, P.my, lo, hi, alpha, s[2], s[3]);
+  if (P.level > 0) {
+    const DPatch &A = d.P[P.parent];
+    const long long ay = A.px, az = (long long)A.px * A.py;
+    const int c1 = P.mx / 2, c2 = P.my / 2, c3 = ND == 3 ? P.mz / 2 : 1;
+    const int i0 = P.mb[0][0] - 1, j0 = P.mb[1][0] - 1, k0 = ND == 3 ? P.mb[2][0] - 1 : 0;
+    const long long m = (long long)c1 * c2 * c3;
+    for (long long e = threadIdx.x; e < m; e += blockDim.x) {
+      const int i = 1 + (int)(e % c1), j = 1 + (int)((e / c1) % c2);
+      const int k = ND == 3 ? 1 + (int)(e / ((long long)c1 * c2)) : 0;
+      s[1] += d.q[A.q + (i0 + i) + ay * (j0 + j) + az * (k0 + k)];
+    }
+    if constexpr (ND == 2)
+      energy_sums<ND>(d.q + A.q + i0 + ay * j0, ay, c1, c2, lo, hi, alpha, s[4], s[5]);
+  }
+  for (int v = 0; v < 6; v++) {
+    double t = block_sum(s[v], lds);
+    if (threadIdx.x == 0) out[6 * blockIdx.x + v] = t;
+  }
+}
+
 template <int ND>
 __global__ __launch_bounds__(256) void k_source(Dev d, const Tile *tiles, int nt, int full) {
   const int b = xcd_tile(nt);
@@ -1412,6 +1515,20 @@ hipError_t launch_error_flags(int nd, const Dev &d, const Tile *tiles, int nt, d
                               hipStream_t s) {
   if (nt <= 0) return hipSuccess;
   BSAM_LAUNCH_ND(k_error_flags, nd, xcd_grid(nt), 256, d, tiles, nt, tol);
+  return hipGetLastError();
+}
+
+hipError_t launch_error_flags_user(int nd, const Dev &d, const Tile *tiles, int nt,
+                                   double qtol, hipStream_t s) {
+  if (nt <= 0) return hipSuccess;
+  BSAM_LAUNCH_ND(k_error_flags_user, nd, xcd_grid(nt), 256, d, tiles, nt, qtol);
+  return hipGetLastError();
+}
+
+hipError_t launch_after_step(int nd, const Dev &d, const int *patches, int n, double lo,
+                             double hi, double *out, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  BSAM_LAUNCH_ND(k_after_step, nd, dim3((unsigned)n), 256, d, patches, lo, hi, out);
   return hipGetLastError();
 }
 

@@ -43,6 +43,12 @@ hipError_t launch_source(int nd, const Dev &d, const Tile *tiles, int nt, int fu
 hipError_t launch_trunc_error(int nd, const Dev &d, const Tile *tiles, int nt, hipStream_t s);
 hipError_t launch_error_flags(int nd, const Dev &d, const Tile *tiles, int nt, double tol,
                               hipStream_t s);
+// SetErrFlagsUser2D/3D on the interior tiles of a level (q ghosts set).
+hipError_t launch_error_flags_user(int nd, const Dev &d, const Tile *tiles, int nt,
+                                   double qtol, hipStream_t s);
+// The six per-patch sums of AfterStep2D/3D for patches[0..n) into out[6n].
+hipError_t launch_after_step(int nd, const Dev &d, const int *patches, int n, double lo,
+                             double hi, double *out, hipStream_t s);
 hipError_t launch_defect_setup(int nd, const Dev &d, const Tile *tiles, int nt, hipStream_t s);
 hipError_t launch_apply_correction(int nd, const Dev &d, const Tile *tiles, int nt,
                                    hipStream_t s);

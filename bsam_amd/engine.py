@@ -84,6 +84,8 @@ def lib():
             getattr(L, n).argtypes = [vp, C.c_int]
         L.bsam_error_flags.argtypes = [vp, C.c_int, C.c_double]
         L.bsam_download_flags.argtypes = [vp, C.c_int, ip, C.c_long]
+        L.bsam_error_flags_user.argtypes = [vp, C.c_int, C.c_double]
+        L.bsam_after_step_stats.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         for n in ("bsam_source", "bsam_copy_qold", "bsam_sync"):
             getattr(L, n).argtypes = [vp]
         L.bsam_error_afas.argtypes = [vp, C.c_int, dp]
@@ -217,6 +219,15 @@ class Engine:
     def error_flags(self, level: int, qtolerance: float):
         """SetErrFlags (default tagging) on ``level`` from qrte."""
         self._ck(lib().bsam_error_flags(self.ctx, level, qtolerance))
+
+    def error_flags_user(self, level: int, qtolerance: float):
+        self._ck(lib().bsam_error_flags_user(self.ctx, level, qtolerance))
+
+    def after_step_stats(self):
+        """(mass, energy) of the composite grid after a step; energy is 0 in 3D."""
+        m, e = C.c_double(), C.c_double()
+        self._ck(lib().bsam_after_step_stats(self.ctx, C.byref(m), C.byref(e)))
+        return m.value, e.value
 
     def flags(self, n: int) -> np.ndarray:
         """Patch ``n``'s error flags errorflags(1:mx) (int32, Fortran order)."""

@@ -13,6 +13,11 @@
 !
 !   call GpuRelativeTruncationError(level) ! RestrictSolution .. RelativeTruncationError
 !   call GpuSetErrFlags(level)             ! SetErrFlags2D/3D on every patch
+!   call GpuSetErrFlagsUser(level)         ! or the problem's SetErrFlagsUser2D/3D
+!
+! and TakeTimeSteps' after-step statistics (mass, 2D energy):
+!
+!   call GpuAfterStepStatistics()          ! integralresult(1:2)
 !
 ! The tree stays on the host. GpuDescribeHierarchy walks it with
 ! ApplyOnLevel (levels minlevel..finestlevel, youngest patch first) and hands
@@ -27,7 +32,8 @@ module AFASGPU
   private
   public :: GpuOpen, GpuClose, GpuDescribeHierarchy, GpuMultigridIterations, &
             GpuSetGhost, GpuUploadAll, GpuDownloadAll, GpuApplyGhostHost, &
-            GpuRelativeTruncationError, GpuSetErrFlags, gpu_ctx, &
+            GpuRelativeTruncationError, GpuSetErrFlags, GpuSetErrFlagsUser, &
+            GpuAfterStepStatistics, gpu_ctx, &
             gpu_npatch, gpu_cycles, gpu_converged, gpu_residuals
 
   type(c_ptr), save :: gpu_ctx = c_null_ptr
@@ -266,6 +272,31 @@ contains
                'bsam_error_flags')
     call walk(0, 0, level)
   end subroutine GpuSetErrFlags
+
+  ! ErrFlag's user tagging (lib/bsamroutines.f90:1061-1075: the problem
+  ! module's SetErrFlagsUser2D/3D, problem.f90:135-163, :425-452) on every
+  ! patch of a level: uploads the level's q (ghosts as AMR's SetGhost(level,
+  ! 0) left them), tags on the GPU with qtolerance(level) and downloads each
+  ! nodeinfo's errorflags.
+  subroutine GpuSetErrFlagsUser(level)
+    use NodeInfoDef, only: qtolerance
+    integer, intent(in) :: level
+    call walk(BSAM_FIELD_Q, 1, level)
+    call check(bsam_error_flags_user(gpu_ctx, level, real(qtolerance(level), c_double)), &
+               'bsam_error_flags_user')
+    call walk(0, 0, level)
+  end subroutine GpuSetErrFlagsUser
+
+  ! TakeTimeSteps' after-step statistics (lib/bsamroutines.f90:690-697) from
+  ! the device q the last solve left: integralresult(1) the mass of
+  ! variable 1 on the composite grid, integralresult(2) the 2D energy.
+  subroutine GpuAfterStepStatistics()
+    use NodeInfoDef, only: integralresult
+    real(c_double) :: m, e
+    call check(bsam_after_step_stats(gpu_ctx, m, e), 'bsam_after_step_stats')
+    integralresult(1) = m
+    integralresult(2) = e
+  end subroutine GpuAfterStepStatistics
 
   ! SetGhost(level, ipass) (lib/boundary.f90:58-89) on the device copy.
   subroutine GpuSetGhost(level, ipass)

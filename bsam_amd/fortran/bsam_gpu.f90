@@ -119,6 +119,21 @@ module bsam_gpu
       real(c_double), value :: qtolerance
       integer(c_int) :: rc
     end function
+    function bsam_error_flags_user(ctx, level, qtolerance) &
+        bind(C, name='bsam_error_flags_user') result(rc)
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: level
+      real(c_double), value :: qtolerance
+      integer(c_int) :: rc
+    end function
+    function bsam_after_step_stats(ctx, mass, energy) bind(C, name='bsam_after_step_stats') &
+        result(rc)
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: mass, energy
+      integer(c_int) :: rc
+    end function
     function bsam_download_flags(ctx, patch, host, n) bind(C, name='bsam_download_flags') &
         result(rc)
       import :: c_ptr, c_int, c_long

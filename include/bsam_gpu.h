@@ -130,6 +130,22 @@ int bsam_truncation_error(bsam_ctx *ctx, int level);
  * to the host. */
 int bsam_error_flags(bsam_ctx *ctx, int level, double qtolerance);
 int bsam_download_flags(bsam_ctx *ctx, int patch, int *host, long n);
+/* ErrFlag's user tagging (errflagopt = errflaguser, lib/bsamroutines.f90:
+ * 1061-1075) with the Cahn-Hilliard module's SetErrFlagsUser2D/3D
+ * (problems/simple_cahn-hilliard/problem.f90:135-163, :425-452): every cell
+ * of `level` whose undivided central gradient of variable 1 exceeds
+ * qtolerance(level) is flagged, from the device q with the ghosts that
+ * SetGhost(level, 0) leaves (AMR calls it first, bsamroutines.f90:946). */
+int bsam_error_flags_user(bsam_ctx *ctx, int level, double qtolerance);
+/* TakeTimeSteps' after-step statistics (lib/bsamroutines.f90:690-697:
+ * AfterStepStatistics :896-939 over levels finest..0 with AfterStep2D/3D,
+ * problem.f90:102-134, :398-424): the mass of variable 1 on the composite
+ * grid and, in 2D, the free energy Energy2D (problem.f90:330-361) with the
+ * double well whose convex and concave parts the configuration's Dfc and
+ * Dfe are; *energy is 0 in 3D, as the reference leaves it. Per-patch sums
+ * are reduced on the device in a fixed order and combined on the host in the
+ * reference's order. Synchronises the stream. */
+int bsam_after_step_stats(bsam_ctx *ctx, double *mass, double *energy);
 /* GetSourceFunction then UpdateSourceFunction on levels finest..0,
  * lib/afasroutines.f90:57-64 */
 int bsam_source(bsam_ctx *ctx);

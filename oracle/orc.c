@@ -525,6 +525,109 @@ static void error_flags_patch(orc_hier *h, orc_patch *p, double qtol) {
       }
 }
 
+/* SetErrFlagsUser2D/3D (problems/simple_cahn-hilliard/problem.f90:135-163,
+ * :425-452), ErrFlag's errflaguser case (lib/bsamroutines.f90:1061-1075):
+ * every interior cell whose undivided central gradient of variable 1,
+ * sqrt((q(i+1)-q(i-1))^2 + (q(j+1)-q(j-1))^2 [+ (q(k+1)-q(k-1))^2]), is
+ * strictly above qtolerance(level) is flagged; all others are cleared. */
+static void error_flags_user_patch(orc_hier *h, orc_patch *p, double qtol) {
+  const int nd = h->ndims;
+  int kz = nd == 3 ? p->mx[2] : 1;
+  for (int k = 1; k <= kz; k++)
+    for (int j = 1; j <= p->mx[1]; j++)
+      for (int i = 1; i <= p->mx[0]; i++) {
+        double gx = Q(p, i + 1, j, k, 1) - Q(p, i - 1, j, k, 1);
+        double gy = Q(p, i, j + 1, k, 1) - Q(p, i, j - 1, k, 1);
+        double s2 = gx * gx + gy * gy;
+        if (nd == 3) {
+          double gz = Q(p, i, j, k + 1, 1) - Q(p, i, j, k - 1, 1);
+          s2 = s2 + gz * gz;
+        }
+        p->errorflags[FX(p, i, j, k, 1)] = fabs(sqrt(s2)) > qtol ? 1 : 0;
+      }
+}
+
+/* The double well of the problem module, Ff(c) = alpha c^2 (c-1)^2 / 4
+ * (problem.f90:769-780), written for minima lo, hi: the hierarchy's
+ * Dfc = fc_alpha (c-s)^3 and Dfe = (fe_num/fe_den)(c-s) are its convex and
+ * concave parts when lo, hi = s -+ sqrt((fe_num/fe_den)/fc_alpha) (0 and 1
+ * for the reference's constants, so (c-lo) is c exactly). */
+static void well_minima(const orc_hier *h, double *lo, double *hi) {
+  double w = sqrt(h->fe_num / h->fe_den / h->fc_alpha);
+  *lo = h->fc_shift - w;
+  *hi = h->fc_shift + w;
+}
+
+static inline double ff(const orc_hier *h, double lo, double hi, double c) {
+  double c0 = c - lo, cm1 = c - hi;
+  return c0 * c0 * cm1 * cm1 * h->fc_alpha / 4.0;
+}
+
+/* Energy2D (problem.f90:330-361) of c(0:m1+1, 0:m2+1) = variable 1 of `src`
+ * from cell (i0, j0) on, grid spacing hh. Sums run in array element order. */
+static double energy2d(const orc_hier *h, const orc_patch *src, int i0, int j0, int m1,
+                       int m2, double hh) {
+  double lo, hi;
+  well_minima(h, &lo, &hi);
+#define C2(a, b) Q(src, i0 + (a), j0 + (b), 1, 1)
+  const double h2 = hh * hh;
+  double sff = 0.0;
+  for (int j = 1; j <= m2; j++)
+    for (int i = 1; i <= m1; i++) sff += ff(h, lo, hi, C2(i, j));
+  double sg = 0.0;
+  for (int j = 1; j <= m2; j++)
+    for (int i = 1; i <= m1; i++) {
+      double a = C2(i + 1, j) - C2(i, j);       /* f1(i, j) */
+      double b = C2(i, j) - C2(i - 1, j);       /* f1(i-1, j) */
+      double c = C2(i, j + 1) - C2(i, j);       /* f2(i, j) */
+      double d = C2(i, j) - C2(i, j - 1);       /* f2(i, j-1) */
+      sg += a * a + b * b + c * c + d * d;
+    }
+#undef C2
+  const double tmp = h->eps2 / h2 / 4.0;
+  return h2 * (sff + tmp * sg);
+}
+
+/* AfterStep2D/3D (problem.f90:102-134, :398-424) through AfterStepStatistics
+ * (lib/bsamroutines.f90:896-939): mass (variable 1) and, in 2D, energy of the
+ * composite grid, each refined patch adding its own sum and removing its
+ * parent's cells under it. */
+static void after_step_patch(orc_hier *h, orc_patch *p, double out[2]) {
+  const int nd = h->ndims;
+  const orc_patch *pa = &h->p[p->parent];
+  const double hh = p->dx, ch = hh * 2.0;
+  int kz = nd == 3 ? p->mx[2] : 1;
+  double sq = 0.0;
+  for (int k = 1; k <= kz; k++)
+    for (int j = 1; j <= p->mx[1]; j++)
+      for (int i = 1; i <= p->mx[0]; i++) sq += Q(p, i, j, k, 1);
+  const int cm[3] = {p->mx[0] / 2, p->mx[1] / 2, nd == 3 ? p->mx[2] / 2 : 1};
+  double sc = 0.0;
+  if (p->level != 0) {
+    int ck = nd == 3 ? cm[2] : 1;
+    for (int k = 1; k <= ck; k++)
+      for (int j = 1; j <= cm[1]; j++)
+        for (int i = 1; i <= cm[0]; i++)
+          sc += Q(pa, p->mb[0][0] - 1 + i, p->mb[1][0] - 1 + j,
+                  nd == 3 ? p->mb[2][0] - 1 + k : 1, 1);
+  }
+  if (nd == 2) {
+    const double h2 = hh * hh, ch2 = ch * ch;
+    if (p->level == 0) {
+      out[0] = out[0] + h2 * sq;
+      out[1] = out[1] + energy2d(h, p, 0, 0, p->mx[0], p->mx[1], hh);
+    } else {
+      out[0] = out[0] + h2 * sq - ch2 * sc;
+      out[1] = out[1] + energy2d(h, p, 0, 0, p->mx[0], p->mx[1], hh) -
+               energy2d(h, pa, p->mb[0][0] - 1, p->mb[1][0] - 1, cm[0], cm[1], ch);
+    }
+  } else {
+    const double h3 = hh * hh * hh, ch3 = ch * ch * ch;
+    if (p->level == 0) out[0] = out[0] + h3 * sq;
+    else out[0] = out[0] + h3 * sq - ch3 * sc;
+  }
+}
+
 /* CorrectFine, lib/afasroutines.f90:842-963, mbc = 1: the coarse correction
  * (parent q minus the restricted qc, ghost ring included) is added to q over
  * 0:mx+1 with BiLinProlongationP1 / TriLinProlongationP1
@@ -929,6 +1032,22 @@ int orc_error_flags(orc_hier *h, int level, double qtolerance) {
   if (!level_ok(h, level)) return fail(h, "error_flags: bad level %d", level);
   for (int a = 0; a < nlev(h, level); a++)
     error_flags_patch(h, lev_patch(h, level, a), qtolerance);
+  return 0;
+}
+
+int orc_error_flags_user(orc_hier *h, int level, double qtolerance) {
+  if (!level_ok(h, level)) return fail(h, "bad level %d", level);
+  for (int a = 0; a < nlev(h, level); a++)
+    error_flags_user_patch(h, lev_patch(h, level, a), qtolerance);
+  return 0;
+}
+
+/* TakeTimeSteps' statistics loop (lib/bsamroutines.f90:690-697): levels
+ * finest down to the root, patches in level-list order. */
+int orc_after_step_stats(orc_hier *h, double *out) {
+  out[0] = out[1] = 0.0;
+  for (int level = h->finestlevel; level >= 0; level--)
+    for (int a = 0; a < nlev(h, level); a++) after_step_patch(h, lev_patch(h, level, a), out);
   return 0;
 }
 

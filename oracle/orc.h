@@ -102,6 +102,11 @@ int orc_truncation_error(orc_hier *h, int level);
 /* SetErrFlags2D/3D (lib/bsamroutines.f90:1093-1157) on every patch of
  * `level` from its qrte, with qtolerance(level). */
 int orc_error_flags(orc_hier *h, int level, double qtolerance);
+/* SetErrFlagsUser2D/3D (problem.f90:135-163, :425-452) on every patch of
+ * `level` from q (ghosts as SetGhost(level, 0) leaves them). */
+int orc_error_flags_user(orc_hier *h, int level, double qtolerance);
+/* Mass and (2D) energy after a step (AfterStepStatistics, AfterStep2D/3D). */
+int orc_after_step_stats(orc_hier *h, double *out);
 int *orc_error_flag_field(orc_hier *h, int patch);
 long orc_error_flag_len(orc_hier *h, int patch);
 /* MultigridIterations; writes up to maxhist residuals, returns #cycles. */

@@ -54,6 +54,8 @@ def lib():
                      "orc_fill_down", "orc_vcycle", "orc_truncation_error"):
             getattr(L, name).argtypes = [vp, C.c_int]
         L.orc_error_flags.argtypes = [vp, C.c_int, C.c_double]
+        L.orc_error_flags_user.argtypes = [vp, C.c_int, C.c_double]
+        L.orc_after_step_stats.argtypes = [vp, dp]
         L.orc_error_flag_field.restype = ip
         L.orc_error_flag_field.argtypes = [vp, C.c_int]
         L.orc_error_flag_len.restype = C.c_long
@@ -159,6 +161,16 @@ class Oracle:
     def error_flags(self, level, qtolerance):
         """SetErrFlags on ``level`` from qrte."""
         self._ck(lib().orc_error_flags(self.h, level, qtolerance))
+
+    def error_flags_user(self, level, qtolerance):
+        """SetErrFlagsUser2D/3D on ``level`` from q."""
+        self._ck(lib().orc_error_flags_user(self.h, level, qtolerance))
+
+    def after_step_stats(self):
+        """(mass, energy) of the composite grid; energy is 0 in 3D."""
+        out = (C.c_double * 2)()
+        self._ck(lib().orc_after_step_stats(self.h, out))
+        return out[0], out[1]
 
     def flags(self, n):
         """Writable Fortran-order view of patch ``n``'s error flags."""

@@ -23,11 +23,13 @@
 !
 ! Ops: setghost L ipass | relax L color | restrict L | coarseghost L |
 !      coarseload L | correct L | filldown field | source 0 0 | vcycle L 0 |
-!      residual L 0 | mgiter 0 0 | step nsteps 0 | copyqold 0 0 | truncerr L 0
+!      residual L 0 | mgiter 0 0 | step nsteps 0 | copyqold 0 0 | truncerr L 0 |
+!      userflags L N (qtolerance(L) = N * 1e-3) | stats 0 0
 !
 ! Input stream: for every level minlevel..finestlevel in level-list order
 ! (ApplyOnLevel order), the arrays q, f, qc of each patch.
-! Output stream: the same order, arrays q, f, qc, rf, qold, qrte.
+! Output stream: the same order, arrays q, f, qc, rf, qold, qrte and the
+! error flags (as reals).
 ! Residual history (mgiter/step ops): MultigridIterations prints one line per
 ! V-cycle to stdout, after a 'harness step N' line per time step.
 program RefHarness
@@ -114,6 +116,19 @@ program RefHarness
       call ApplyOnLevel(a1, CopyToOld, dummy)
       call ApplyOnLevel(a1 - 1, CopyToOld, dummy)
       call ApplyOnLevel(a1, RelativeTruncationError, dummy)
+    case('userflags')
+      ! ErrFlag's errflaguser case on every patch of the level
+      qtolerance(a1) = real(a2) * 1.0e-3
+      call ApplyOnLevel(a1, UserFlagPatch, dummy)
+    case('stats')
+      ! TakeTimeSteps' after-step statistics (lib/bsamroutines.f90:690-697):
+      ! levels finest to root, integralresult(1:2) = mass, energy
+      integralresult = 0.0
+      do lev = finestlevel, rootlevel, -1
+        call ApplyOnLevel(lev, StatsPatch, dummy)
+      end do
+      write(14,'(i6,1x,i6,1x,es25.17)') -1, 1, integralresult(1)
+      write(14,'(i6,1x,i6,1x,es25.17)') -1, 2, integralresult(2)
     case('copyqold')
       do lev = minlevel, finestlevel
         call ApplyOnLevel(lev, CopyToOld, dummy)

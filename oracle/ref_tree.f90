@@ -137,8 +137,84 @@ contains
     write(h_unit) info%rf
     write(h_unit) info%qold
     write(h_unit) info%qrte
+    write(h_unit) real(info%errorflags)
     DumpPatch = err_ok
   end function DumpPatch
+
+  ! ErrFlag's user tagging (lib/bsamroutines.f90:1061-1075) on one patch,
+  ! with the same array sections: Problem's SetErrFlagsUser2D/3D.
+  integer function UserFlagPatch(info, dummy)
+    use NodeInfoDef
+    use TreeOps, only: err_ok
+    use Problem, only: SetErrFlagsUser2D, SetErrFlagsUser3D
+    implicit none
+    type(nodeinfo)  :: info
+    type(funcparam) :: dummy
+    integer :: nrvars
+    integer, dimension(1:maxdims) :: mx, cmx
+    real :: h
+    real, dimension(1:maxdims) :: xlower
+    nrvars = info%nrvars
+    h = info%dx(1)
+    xlower = 0.0
+    xlower(1:ndims) = info%xlower(1:ndims)
+    mx = 1
+    mx(1:ndims) = info%mx(1:ndims)
+    cmx = 1
+    cmx(1:ndims) = mx(1:ndims)/2
+    select case(ndims)
+    case(2)
+      call SetErrFlagsUser2D(info%qrte(1:cmx(1),1:cmx(2),1,1:nrvars), &
+                             info%q(0:mx(1)+1,0:mx(2)+1,1,1:nrvars), &
+                             info%errorflags(1:mx(1),1:mx(2),1), &
+                             mx(1:2),cmx(1:2),nrvars,h,xlower(1:2),info%level)
+    case(3)
+      call SetErrFlagsUser3D(info%qrte(1:cmx(1),1:cmx(2),1:cmx(3),1:nrvars), &
+                             info%q(0:mx(1)+1,0:mx(2)+1,0:mx(3)+1,1:nrvars), &
+                             info%errorflags(1:mx(1),1:mx(2),1:mx(3)), &
+                             mx(1:3),cmx(1:3),nrvars,h,xlower(1:3),info%level)
+    end select
+    UserFlagPatch = err_ok
+  end function UserFlagPatch
+
+  ! AfterStepStatistics (lib/bsamroutines.f90:896-939) on one patch, with
+  ! the same array sections: Problem's AfterStep2D/3D accumulate mass and
+  ! (2D) energy into integralresult.
+  integer function StatsPatch(info, dummy)
+    use NodeInfoDef
+    use TreeOps, only: err_ok, GetParentInfo
+    use Problem, only: AfterStep2D, AfterStep3D
+    implicit none
+    type(nodeinfo)  :: info
+    type(funcparam) :: dummy
+    type(nodeinfo), pointer :: parent
+    integer :: ierror, nrvars
+    integer, dimension(1:maxdims) :: mx
+    integer, dimension(1:maxdims,1:2) :: mb
+    real :: h
+    real, dimension(1:maxdims) :: xlower
+    mx = 1
+    mx(1:ndims) = info%mx(1:ndims)
+    nrvars = info%nrvars
+    h = info%dx(1)
+    xlower = 0.0
+    xlower(1:ndims) = info%xlower(1:ndims)
+    mb = 1
+    mb(1:ndims,1:2) = info%mbounds(1:ndims,1:2)
+    ierror = GetParentInfo(parent)
+    select case(ndims)
+    case(2)
+      call AfterStep2D(info%q(0:mx(1)+1,0:mx(2)+1,1,1:nrvars), &
+                       parent%q(mb(1,1)-1:mb(1,2)+1,mb(2,1)-1:mb(2,2)+1,1,1:nrvars), &
+                       mx(1:2),nrvars,h,xlower(1:2),info%level)
+    case(3)
+      call AfterStep3D(info%q(0:mx(1)+1,0:mx(2)+1,0:mx(3)+1,1:nrvars), &
+                       parent%q(mb(1,1)-1:mb(1,2)+1,mb(2,1)-1:mb(2,2)+1, &
+                                mb(3,1)-1:mb(3,2)+1,1:nrvars), &
+                       mx(1:3),nrvars,h,xlower(1:3),info%level)
+    end select
+    StatsPatch = err_ok
+  end function StatsPatch
 
   integer function ListPatch(info, dummy)
     use NodeInfoDef

@@ -51,7 +51,8 @@ def run(layout, arrays, ops, dt=1e-3, alpha=1.0, eps=0.03, nsmooth=2, maxv=20,
         tol=1e-8, keep=None):
     """Run ``ops`` on the reference. ``arrays`` maps patch id -> dict of
     'q', 'f', 'qc' Fortran-order arrays. Returns (outputs, residuals, listing)
-    with outputs: id -> {'q','f','qc','rf','qold','qrte'}."""
+    with outputs: id -> {'q','f','qc','rf','qold','qrte','errorflags'}
+    (errorflags as int32)."""
     if not available():
         raise RuntimeError("reference harness not built (make -C oracle ref)")
     tmp = keep or tempfile.mkdtemp(prefix="bsamref_")
@@ -76,11 +77,12 @@ def run(layout, arrays, ops, dt=1e-3, alpha=1.0, eps=0.03, nsmooth=2, maxv=20,
     out, pos = {}, 0
     for p in order:
         d = {}
-        for name in ("q", "f", "qc", "rf", "qold", "qrte"):
+        for name in ("q", "f", "qc", "rf", "qold", "qrte", "errorflags"):
             shape = layout.field_shape(p, name)
             n = int(np.prod(shape))
             d[name] = raw[pos:pos + n].reshape(shape, order="F").copy()
             pos += n
+        d["errorflags"] = d["errorflags"].astype(np.int32)
         out[p.id] = d
     assert pos == raw.size
     res = []

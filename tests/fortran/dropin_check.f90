@@ -9,9 +9,11 @@
 !                                  MultigridIterations, NSTEP time steps
 !   dropin_check trunc SPEC INPUT  GpuRelativeTruncationError vs the
 !                                  reference calls of EstimateLevelErrors
-!                                  (q, qold, qc, qrte), and GpuSetErrFlags'
+!                                  (q, qold, qc, qrte), GpuSetErrFlags'
 !                                  flags vs the SetErrFlags rule on that
-!                                  qrte, every refined level
+!                                  qrte, and GpuSetErrFlagsUser vs the
+!                                  reference SetErrFlagsUser2D/3D, every
+!                                  level from the root
 !
 ! Prints "max_abs_diff <value>" (0 expected) and "cycles ...".
 module DropinBuffers
@@ -25,6 +27,7 @@ module DropinBuffers
   integer, save :: fulllevel = -1
   real, save :: tol = 0.0, scale = 0.0, hlev = 0.0
   integer, save :: nflagged = 0, nbadflags = 0
+  integer, allocatable, save :: ibuf(:)
 end module DropinBuffers
 
 module DropinCallbacks
@@ -77,9 +80,10 @@ contains
     end subroutine arr
   end function BufPatch
 
-  ! Largest |qrte| of the patches of fulllevel (mode 0), or the check of
-  ! their error flags against SetErrFlags2D/3D (lib/bsamroutines.f90:
-  ! 1093-1157) on their qrte at tolerance tol (mode 1).
+  ! Largest |qrte| of the patches of fulllevel (mode 0), the check of their
+  ! error flags against SetErrFlags2D/3D (lib/bsamroutines.f90:1093-1157) on
+  ! their qrte at tolerance tol (mode 1), or their flags saved to (mode 2)
+  ! and compared with (mode 3) ibuf.
   integer function FlagPatch(info, dummy)
     use NodeInfoDef
     use TreeOps, only: err_ok
@@ -98,6 +102,18 @@ contains
       hlev = info%dx(1)
       return
     end if
+    if (mode == 2 .or. mode == 3) then
+      ! save (2) or compare (3) the patch's flags
+      i = size(info%errorflags)
+      if (mode == 2) then
+        ibuf(pos+1:pos+i) = reshape(info%errorflags, [i])
+      else
+        nbadflags = nbadflags + count(ibuf(pos+1:pos+i) /= reshape(info%errorflags, [i]))
+        nflagged = nflagged + sum(info%errorflags)
+      end if
+      pos = pos + i
+      return
+    end if
     do k = 1, info%mx(3)
       do j = 1, info%mx(2)
         do i = 1, info%mx(1)
@@ -113,6 +129,16 @@ contains
       end do
     end do
   end function FlagPatch
+
+  integer function ZeroFlags(info, dummy)
+    use NodeInfoDef
+    use TreeOps, only: err_ok
+    implicit none
+    type(nodeinfo)  :: info
+    type(funcparam) :: dummy
+    info%errorflags = 7
+    ZeroFlags = err_ok
+  end function ZeroFlags
 
   integer function SizePatch(info, dummy)
     use NodeInfoDef
@@ -164,7 +190,7 @@ program DropinCheck
     call ApplyOnLevel(lev, SizePatch, dummy)
   end do
   total = pos
-  allocate(buf(total))
+  allocate(buf(total), ibuf(total))
 
   if (trim(what) == 'host') then
     call GpuOpen(-1, 0)
@@ -209,6 +235,20 @@ program DropinCheck
       if (ndims == 3) tol = tol / hlev
       call GpuSetErrFlags(lev)
       mode = 1
+      call ApplyOnLevel(lev, FlagPatch, dummy)
+      ! the problem's user tagging after SetGhost(lev, 0), as AMR runs it:
+      ! reference SetErrFlagsUser2D/3D on the host vs GpuSetErrFlagsUser
+      call load_inputs()
+      call SetGhost(lev, 0)
+      qtolerance(lev) = 0.05
+      call ApplyOnLevel(lev, UserFlagPatch, dummy)
+      mode = 2
+      pos = 0
+      call ApplyOnLevel(lev, FlagPatch, dummy)
+      call ApplyOnLevel(lev, ZeroFlags, dummy)
+      call GpuSetErrFlagsUser(lev)
+      mode = 3
+      pos = 0
       call ApplyOnLevel(lev, FlagPatch, dummy)
     end do
     if (nbadflags > 0) worst = max(worst, 1.0)

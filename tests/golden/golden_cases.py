@@ -91,6 +91,10 @@ OPS = [("setghost", 1, 0), ("setghost", 2, 1), ("setghost", 2, 2), ("relax", 2, 
 
 # case -> (layout, seed, op sequence): one fixture per case with the
 # inputs and the fields after the whole sequence
+USERFLAG_OPS = [("setghost", 0, 0), ("userflags", 0, 900), ("setghost", 1, 0),
+                ("userflags", 1, 1200), ("setghost", 2, 0), ("userflags", 2, 1500),
+                ("stats", 0, 0)]
+
 OPS_CASES = {
     "amr2d_seq": ("amr2d", 1, OPS),
     "amr2d_vcycle": ("amr2d", 2, [("vcycle", 2)]),
@@ -108,6 +112,13 @@ OPS_CASES = {
     "amr3d_truncerr": ("amr3d", 15, [("truncerr", 2), ("truncerr", 1), ("truncerr", 0)]),
     "periodic2d_truncerr": ("periodic2d", 16, [("truncerr", 2), ("truncerr", 1), ("truncerr", 0)]),
     "periodic3d_truncerr": ("periodic3d", 17, [("truncerr", 1), ("truncerr", 0)]),
+    # ErrFlag's user tagging (qtolerance = N * 1e-3) after SetGhost(level, 0),
+    # as AMR runs it, then the after-step statistics (mass, 2D energy)
+    "amr2d_userflags": ("amr2d", 18, USERFLAG_OPS),
+    "userbc_userflags": ("amr2d_userbc", 19, USERFLAG_OPS),
+    "amr3d_userflags": ("amr3d", 20, USERFLAG_OPS),
+    "periodic2d_userflags": ("periodic2d", 21, USERFLAG_OPS),
+    "periodic3d_userflags": ("periodic3d", 22, USERFLAG_OPS),
 }
 
 

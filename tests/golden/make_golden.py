@@ -55,6 +55,11 @@ def main():
         names = ("q", "f", "qc")
         if any(op[0] == "truncerr" for op in ops):
             names += ("qold", "qrte")
+        if any(op[0] == "userflags" for op in ops):
+            names += ("errorflags",)
+        stats = [r for s, _, r in res if s == -1]
+        if stats:
+            arrs["stats"] = np.array(stats)
         pack_fields("out", lay, out, names, arrs)
         np.savez_compressed(os.path.join(HERE, f"ops_{case}.npz"), **arrs)
 

@@ -251,3 +251,57 @@ def test_frame_export(tmp_path, name):
     for fp in ps:
         assert np.isfinite(fp.q).all()
     eng.close()
+
+
+@pytest.mark.parametrize("name", sorted(LAYOUTS))
+def test_user_error_flags(name):
+    """ErrFlag's user tagging (SetErrFlagsUser2D/3D of the Cahn-Hilliard
+    module) after SetGhost(level, 0), as AMR runs it, on the root and every
+    refined level at three tolerances: the flags equal the oracle's."""
+    lay = LAYOUTS[name]()
+    for level in range(max(lay.minlevel, 0), lay.finestlevel + 1):
+        eng, ora = pair(lay, random_fields(lay, seed=51 + level))
+        eng.set_ghost(level, 0)
+        ora.set_ghost(level, 0)
+        pats = [n for n, p in enumerate(ora.patches) if p.level == level]
+        for tol in (0.4, 1.0, 1.8):
+            eng.error_flags_user(level, tol)
+            ora.error_flags_user(level, tol)
+            on = size = 0
+            for n in pats:
+                got = eng.flags(n)
+                assert np.array_equal(got, ora.flags(n)), (level, tol, n)
+                on, size = on + int(got.sum()), size + got.size
+            assert 0 < on < size or tol != 1.0, (level, on, size)
+        eng.close()
+
+
+@pytest.mark.parametrize("name", sorted(LAYOUTS))
+def test_after_step_stats(name):
+    """Mass and (2D) energy of the composite grid after two time steps, as
+    TakeTimeSteps computes them (AfterStepStatistics with AfterStep2D/3D).
+    The device reduces each patch's sums in another order than the
+    reference's sequential SUM, so they agree to 1e-12 of the sum of the
+    magnitudes of the terms (the bound of a reordered sum)."""
+    lay = LAYOUTS[name]()
+    q0 = noise_q(lay, seed=61)
+    eng, ora = pair(lay, {k: {"q": v} for k, v in q0.items()}, names=("q",))
+    for _ in range(2):
+        eng.time_step()
+        ora.time_step()
+        m, e = eng.after_step_stats()
+        mo, eo = ora.after_step_stats()
+        # magnitude of the terms: h^d sum |q1| per patch plus its parent's cells
+        scale = 0.0
+        for n, p in enumerate(ora.patches):
+            if p.level < 0:
+                continue
+            q = ora.field(n, "q")
+            inner = q[(slice(1, -1),) * lay.ndims + (0,) * (3 - lay.ndims) + (0,)]
+            scale += p.dx ** lay.ndims * np.abs(inner).sum() * (2 if p.level > 0 else 1)
+        assert abs(m - mo) <= 1e-12 * scale, (m, mo, scale)
+        if lay.ndims == 2:
+            assert e > 0 and abs(e - eo) <= 1e-12 * max(abs(eo), 1.0) * 8, (e, eo)
+        else:
+            assert e == 0.0 and eo == 0.0
+    eng.close()

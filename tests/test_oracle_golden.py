@@ -23,6 +23,8 @@ def load(name):
 
 
 def run_ops(o, ops):
+    """Runs ops on the oracle; returns the statistics the stats ops gave."""
+    stats = []
     for op in ops:
         name, a = op[0], op[1:]
         if name == "setghost":
@@ -43,8 +45,13 @@ def run_ops(o, ops):
             o.vcycle(*a)
         elif name == "truncerr":
             o.truncation_error(*a)
+        elif name == "userflags":
+            o.error_flags_user(a[0], a[1] * 1e-3)
+        elif name == "stats":
+            stats.extend(o.after_step_stats())
         else:
             raise KeyError(name)
+    return stats
 
 
 @pytest.mark.parametrize("case", sorted(OPS_CASES))
@@ -56,12 +63,17 @@ def test_ops_match_reference(case):
     for n, _ in enumerate(o.patches):
         for f in ("q", "f", "qc"):
             o.field(n, f)[...] = z[f"in/{n}/{f}"]
-    run_ops(o, ops)
+    stats = run_ops(o, ops)
     for n, p in enumerate(o.patches):
         for f in ("q", "f", "qc", "qold", "qrte"):
             key = f"out/{n}/{f}"
             if f in ("q", "f", "qc") or key in z:
                 assert np.array_equal(o.field(n, f), z[key]), (case, p.id, f)
+        if f"out/{n}/errorflags" in z:
+            assert np.array_equal(o.flags(n), z[f"out/{n}/errorflags"]), (case, p.id)
+    if "stats" in z:
+        # the reference sums in array element order, as the oracle does
+        assert stats == list(z["stats"]), (stats, z["stats"])
 
 
 @pytest.mark.parametrize("case", sorted(k for k in STEP_CASES if k != "plusminus_map"))
