@@ -92,6 +92,13 @@ struct Dev {
   int tile_lx, tile_ly;   // level-kernel tiles: 2^tile_lx x 2^tile_ly (x the rest of 256)
 };
 
+// One workgroup's share of the after-step statistics: rows (2D) or planes
+// (3D) r0..r1-1 of a patch's interior (part 0) or of its parent's cells
+// under it (part 1).
+struct StatTask {
+  int patch, part, r0, r1;
+};
+
 // Which field set a level operation acts on.
 enum { PREC_SOLUTION_FP64 = 0, PREC_DEFECT_FP32 = 1 };
 

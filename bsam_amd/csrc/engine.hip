@@ -133,8 +133,8 @@ struct bsam_ctx {
   int inner_cycles = 1;      // FP32 V-cycles per defect-correction step
   DPatch *d_P = nullptr;
   double *d_partial = nullptr;
-  double *d_stats = nullptr;   // per-patch sums of the after-step statistics
-  int *d_stats_list = nullptr;  // their patches: levels finest..0, level-list order
+  double *d_stats = nullptr;      // task sums of the after-step statistics
+  StatTask *d_stats_list = nullptr;  // their tasks
   int n_stats = 0;
   int partial_cap = 0;
   double *d_hist = nullptr;
@@ -1271,51 +1271,77 @@ int bsam_after_step_stats(bsam_ctx *c, double *mass, double *energy) {
   c->need_device();
   if (!mass || !energy) throw Error("null output pointer");
   const int nd = c->nd();
+  // tasks of about 4096 cells: runs of rows (2D) or planes (3D) of every
+  // patch of levels finest..0 (level-list order) and of its parent section
   std::vector<int> list;
   for (int l = c->finest; l >= 0; l--)
     for (int p : c->L(l).patches) list.push_back(p);
-  if (c->n_stats != (int)list.size()) {
+  std::vector<StatTask> tasks;
+  std::vector<int> first(list.size() * 2 + 1, 0);
+  for (size_t n = 0; n < list.size(); n++) {
+    const HPatch &h = c->hp[list[n]];
+    for (int part = 0; part < 2; part++) {
+      first[2 * n + part] = (int)tasks.size();
+      if (part == 1 && h.level == 0) continue;
+      int m1 = part ? h.mx[0] / 2 : h.mx[0], m2 = part ? h.mx[1] / 2 : h.mx[1];
+      int m3 = nd == 3 ? (part ? h.mx[2] / 2 : h.mx[2]) : 1;
+      int rows = nd == 3 ? m3 : m2, per = nd == 3 ? m1 * m2 : m1;
+      int step = std::max(1, 4096 / std::max(1, per));
+      for (int r = 1; r <= rows; r += step)
+        tasks.push_back(StatTask{list[n], part, r, std::min(rows + 1, r + step)});
+    }
+  }
+  first[2 * list.size()] = (int)tasks.size();
+  if (c->n_stats < (int)tasks.size()) {
     HIPCK(hipStreamSynchronize(c->stream));
     if (c->d_stats) (void)hipFree(c->d_stats);
     if (c->d_stats_list) (void)hipFree(c->d_stats_list);
     c->d_stats = nullptr;
     c->d_stats_list = nullptr;
     c->n_stats = 0;
-    HIPCK(hipMalloc(&c->d_stats, sizeof(double) * 6 * std::max<size_t>(1, list.size())));
-    HIPCK(hipMalloc(&c->d_stats_list, sizeof(int) * std::max<size_t>(1, list.size())));
-    c->n_stats = (int)list.size();
+    HIPCK(hipMalloc(&c->d_stats, sizeof(double) * 3 * tasks.size()));
+    HIPCK(hipMalloc(&c->d_stats_list, sizeof(StatTask) * tasks.size()));
+    c->n_stats = (int)tasks.size();
   }
-  HIPCK(hipMemcpyAsync(c->d_stats_list, list.data(), sizeof(int) * list.size(),
+  HIPCK(hipMemcpyAsync(c->d_stats_list, tasks.data(), sizeof(StatTask) * tasks.size(),
                        hipMemcpyHostToDevice, c->stream));
   // minima of the double well whose convex and concave parts are Dfc and Dfe
   const double w = std::sqrt(c->cfg.fe_num / c->cfg.fe_den / c->cfg.fc_alpha);
   const double lo = c->cfg.fc_shift - w, hi = c->cfg.fc_shift + w;
-  HIPCK(launch_after_step(nd, c->dev(), c->d_stats_list, (int)list.size(), lo, hi, c->d_stats,
-                          c->stream));
-  std::vector<double> t(6 * list.size());
+  HIPCK(launch_after_step(nd, c->dev(), c->d_stats_list, (int)tasks.size(), lo, hi,
+                          c->d_stats, c->stream));
+  std::vector<double> t(3 * tasks.size());
   HIPCK(hipMemcpyAsync(t.data(), c->d_stats, sizeof(double) * t.size(), hipMemcpyDeviceToHost,
                        c->stream));
   HIPCK(hipStreamSynchronize(c->stream));
+  // per patch and part: the task sums in row order
+  auto part_sums = [&](size_t n, int part, double s[3]) {
+    s[0] = s[1] = s[2] = 0.0;
+    for (int k = first[2 * n + part]; k < first[2 * n + part + 1]; k++)
+      for (int v = 0; v < 3; v++) s[v] += t[3 * (size_t)k + v];
+  };
   // AfterStep2D/3D's accumulation, patch by patch in the reference's order
   double m = 0.0, e = 0.0;
   for (size_t n = 0; n < list.size(); n++) {
     const HPatch &h = c->hp[list[n]];
-    const double *s = &t[6 * n];
+    double a[3], b[3];
+    part_sums(n, 0, a);
+    part_sums(n, 1, b);
     const double hh = h.dx, ch = hh * 2.0;
     if (nd == 2) {
       const double h2 = hh * hh, ch2 = ch * ch;
-      const double ef = h2 * (s[2] + c->cfg.eps2 / h2 / 4.0 * s[3]);
+      const double ef = h2 * (a[1] + c->cfg.eps2 / h2 / 4.0 * a[2]);
       if (h.level == 0) {
-        m = m + h2 * s[0];
+        m = m + h2 * a[0];
         e = e + ef;
       } else {
-        const double ec = ch2 * (s[4] + c->cfg.eps2 / ch2 / 4.0 * s[5]);
-        m = m + h2 * s[0] - ch2 * s[1];
+        const double ec = ch2 * (b[1] + c->cfg.eps2 / ch2 / 4.0 * b[2]);
+        m = m + h2 * a[0] - ch2 * b[0];
         e = e + ef - ec;
       }
     } else {
       const double h3 = hh * hh * hh, ch3 = ch * ch * ch;
-      m = h.level == 0 ? m + h3 * s[0] : m + h3 * s[0] - ch3 * s[1];
+      m = h.level == 0 ? m + h3 * a[0] : m + h3 * a[0] - ch3 * b[0];
     }
   }
   *mass = m;

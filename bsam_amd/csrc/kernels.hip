@@ -874,32 +874,16 @@ __global__ __launch_bounds__(256) void k_error_flags_user(Dev d, const Tile *til
       fabs(sqrt(s2)) > qtol ? 1 : 0;
 }
 
-// The sums of AfterStep2D/3D (problem.f90:102-134, :398-424) for one patch
-// per workgroup (patch list `pl`): out[6n..6n+5] = sum of q1 over the
-// interior; sum of the parent's q1 under the patch (refined patches); and in
-// 2D the two sums of Energy2D (problem.f90:330-361) -- of Ff(c) and of the
-// four squared differences per cell -- for the patch and for the parent's
-// cells under it (ghost ring included). Each sum is reduced in a fixed order
-// (strided per thread, then the wave and workgroup trees), so repeated calls
-// agree bit for bit.
+// The sums of AfterStep2D/3D (problem.f90:102-134, :398-424), one task per
+// workgroup: a run of rows (2D) or planes (3D) of a patch's interior (part
+// 0) or of its parent's cells under it (part 1). Each task writes three
+// sums: of q1; and in 2D the two sums of Energy2D (problem.f90:330-361) --
+// of Ff(c) and of the four squared differences per cell (ghost ring read).
+// Each sum is reduced in a fixed order (strided per thread, then the wave
+// and workgroup trees), so repeated calls agree bit for bit.
 __device__ __forceinline__ double well_ff(double c, double lo, double hi, double alpha) {
   double c0 = c - lo, cm1 = c - hi;
   return c0 * c0 * cm1 * cm1 * alpha / 4.0;
-}
-
-template <int ND>
-__device__ __forceinline__ void energy_sums(const double *q, long long px, int m1, int m2,
-                                            double lo, double hi, double alpha, double &sff,
-                                            double &sg) {
-  // q points at c(0, 0); cells 1..m1 x 1..m2
-  const int n = m1 * m2;
-  for (int e = threadIdx.x; e < n; e += blockDim.x) {
-    const int i = 1 + e % m1, j = 1 + e / m1;
-    const double *c = q + i + px * j;
-    sff += well_ff(c[0], lo, hi, alpha);
-    double a = c[1] - c[0], b = c[0] - c[-1], u = c[px] - c[0], w = c[0] - c[-px];
-    sg += a * a + b * b + u * u + w * w;
-  }
 }
 
 __device__ __forceinline__ double block_sum(double v, double *lds) {
@@ -915,38 +899,53 @@ __device__ __forceinline__ double block_sum(double v, double *lds) {
 }
 
 template <int ND>
-__global__ __launch_bounds__(256) void k_after_step(Dev d, const int *pl, double lo, double hi,
-                                                    double *out) {
+__global__ __launch_bounds__(256) void k_after_step(Dev d, const StatTask *tasks, double lo,
+                                                    double hi, double *out) {
   __shared__ double lds[4];
-  const DPatch &P = d.P[pl[blockIdx.x]];
-  const long long sy = P.px, sz = (long long)P.px * P.py;
+  const StatTask T = tasks[blockIdx.x];
+  const DPatch &P = d.P[T.patch];
   const double alpha = d.prm.fc_alpha;
-  double s[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-  const int nz = ND == 3 ? P.mz : 1;
-  const long long n = (long long)P.mx * P.my * nz;
-  for (long long e = threadIdx.x; e < n; e += blockDim.x) {
-    const int i = 1 + (int)(e % P.mx), j = 1 + (int)((e / P.mx) % P.my);
-    const int k = ND == 3 ? 1 + (int)(e / ((long long)P.mx * P.my)) : 0;
-    s[0] += d.q[P.q + i + sy * j + sz * k];
-  }
-  if constexpr (ND == 2) energy_sums<ND>(d.q + P.q, sy, P.mx, P.my, lo, hi, alpha, s[2], s[3]);
-  if (P.level > 0) {
+  // the box c(1..m1, 1..m2[, 1..m3]) of the task's array, c(0,0,0) at base
+  const double *base;
+  long long sy, sz;
+  int m1, m2;
+  if (T.part == 0) {
+    base = d.q + P.q;
+    sy = P.px;
+    sz = (long long)P.px * P.py;
+    m1 = P.mx;
+    m2 = P.my;
+  } else {
     const DPatch &A = d.P[P.parent];
-    const long long ay = A.px, az = (long long)A.px * A.py;
-    const int c1 = P.mx / 2, c2 = P.my / 2, c3 = ND == 3 ? P.mz / 2 : 1;
-    const int i0 = P.mb[0][0] - 1, j0 = P.mb[1][0] - 1, k0 = ND == 3 ? P.mb[2][0] - 1 : 0;
-    const long long m = (long long)c1 * c2 * c3;
-    for (long long e = threadIdx.x; e < m; e += blockDim.x) {
-      const int i = 1 + (int)(e % c1), j = 1 + (int)((e / c1) % c2);
-      const int k = ND == 3 ? 1 + (int)(e / ((long long)c1 * c2)) : 0;
-      s[1] += d.q[A.q + (i0 + i) + ay * (j0 + j) + az * (k0 + k)];
-    }
-    if constexpr (ND == 2)
-      energy_sums<ND>(d.q + A.q + i0 + ay * j0, ay, c1, c2, lo, hi, alpha, s[4], s[5]);
+    sy = A.px;
+    sz = (long long)A.px * A.py;
+    base = d.q + A.q + (P.mb[0][0] - 1) + sy * (P.mb[1][0] - 1) +
+           (ND == 3 ? sz * (P.mb[2][0] - 1) : 0);
+    m1 = P.mx / 2;
+    m2 = P.my / 2;
   }
-  for (int v = 0; v < 6; v++) {
-    double t = block_sum(s[v], lds);
-    if (threadIdx.x == 0) out[6 * blockIdx.x + v] = t;
+  // rows r0..r1-1 (2D: j; 3D: k) of the box
+  const int per = m1 * (ND == 3 ? m2 : 1);
+  const int n = per * (T.r1 - T.r0);
+  double sq = 0.0, sff = 0.0, sg = 0.0;
+  for (int e = threadIdx.x; e < n; e += blockDim.x) {
+    const int r = T.r0 + e / per, w = e % per;
+    const int i = 1 + w % m1;
+    const int j = ND == 3 ? 1 + w / m1 : r;
+    const int k = ND == 3 ? r : 0;
+    const double *c = base + i + sy * j + sz * k;
+    sq += c[0];
+    if constexpr (ND == 2) {
+      sff += well_ff(c[0], lo, hi, alpha);
+      double a = c[1] - c[0], b = c[0] - c[-1], u = c[sy] - c[0], v = c[0] - c[-sy];
+      sg += a * a + b * b + u * u + v * v;
+    }
+  }
+  double t0 = block_sum(sq, lds), t1 = block_sum(sff, lds), t2 = block_sum(sg, lds);
+  if (threadIdx.x == 0) {
+    out[3 * blockIdx.x] = t0;
+    out[3 * blockIdx.x + 1] = t1;
+    out[3 * blockIdx.x + 2] = t2;
   }
 }
 
@@ -1525,10 +1524,10 @@ hipError_t launch_error_flags_user(int nd, const Dev &d, const Tile *tiles, int 
   return hipGetLastError();
 }
 
-hipError_t launch_after_step(int nd, const Dev &d, const int *patches, int n, double lo,
+hipError_t launch_after_step(int nd, const Dev &d, const StatTask *tasks, int n, double lo,
                              double hi, double *out, hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  BSAM_LAUNCH_ND(k_after_step, nd, dim3((unsigned)n), 256, d, patches, lo, hi, out);
+  BSAM_LAUNCH_ND(k_after_step, nd, dim3((unsigned)n), 256, d, tasks, lo, hi, out);
   return hipGetLastError();
 }
 

@@ -46,8 +46,8 @@ hipError_t launch_error_flags(int nd, const Dev &d, const Tile *tiles, int nt, d
 // SetErrFlagsUser2D/3D on the interior tiles of a level (q ghosts set).
 hipError_t launch_error_flags_user(int nd, const Dev &d, const Tile *tiles, int nt,
                                    double qtol, hipStream_t s);
-// The six per-patch sums of AfterStep2D/3D for patches[0..n) into out[6n].
-hipError_t launch_after_step(int nd, const Dev &d, const int *patches, int n, double lo,
+// The sums of AfterStep2D/3D for tasks[0..n) into out[3n].
+hipError_t launch_after_step(int nd, const Dev &d, const StatTask *tasks, int n, double lo,
                              double hi, double *out, hipStream_t s);
 hipError_t launch_defect_setup(int nd, const Dev &d, const Tile *tiles, int nt, hipStream_t s);
 hipError_t launch_apply_correction(int nd, const Dev &d, const Tile *tiles, int nt,
