@@ -13,6 +13,7 @@
 
 #include "bsam_gpu.h"
 #include "ghost.hpp"
+#include "regrid.hpp"
 #include "kernels.hpp"
 
 using namespace bsam;
@@ -1347,6 +1348,53 @@ int bsam_after_step_stats(bsam_ctx *c, double *mass, double *energy) {
   *mass = m;
   *energy = e;
   API_END(c)
+}
+
+int bsam_regrid_level(const bsam_regrid_params *rp, int npatch, const int *mx,
+                      const int *mglobal, int *const *flags, int max_grids, int *ngrids,
+                      int *parent, int *mbounds) {
+  try {
+    if (!rp || !mx || !mglobal || !flags || !ngrids || npatch < 0 || max_grids < 0)
+      throw Error("regrid: null argument");
+    RegridParams p;
+    p.ndims = rp->ndims;
+    p.level = rp->level;
+    p.ibuffer = rp->ibuffer;
+    p.mgp = rp->minimumgridpoints;
+    p.desfill = rp->desiredfillratio;
+    HPatch root;
+    for (int d = 0; d < 3; d++) root.mx[d] = d < rp->ndims ? rp->root_mx[d] : 1;
+    for (int f = 0; f < 6; f++) root.mthbc[f] = f < 2 * rp->ndims ? rp->root_mthbc[f] : 1;
+    p.per = periodic_setup(root, rp->ndims);
+    std::vector<RegridPatch> ps((size_t)npatch);
+    for (int k = 0; k < npatch; k++) {
+      for (int d = 0; d < 3; d++) {
+        ps[k].mx[d] = d < rp->ndims ? mx[3 * k + d] : 1;
+        ps[k].mg[d][0] = d < rp->ndims ? mglobal[6 * k + 2 * d] : 1;
+        ps[k].mg[d][1] = d < rp->ndims ? mglobal[6 * k + 2 * d + 1] : 1;
+        if (ps[k].mx[d] < 1) throw Error("regrid: mx must be positive");
+      }
+      if (!flags[k]) throw Error("regrid: null flags");
+      ps[k].flags = flags[k];
+    }
+    std::vector<NewGrid> out;
+    std::string msg;
+    if (regrid_level(p, ps, out, msg)) throw Error(msg);
+    if ((int)out.size() > max_grids) throw Error("regrid: more new subgrids than max_grids");
+    *ngrids = (int)out.size();
+    for (size_t g = 0; g < out.size(); g++) {
+      if (parent) parent[g] = out[g].parent;
+      if (mbounds)
+        for (int d = 0; d < 3; d++) {
+          mbounds[6 * g + 2 * d] = out[g].mb[d][0];
+          mbounds[6 * g + 2 * d + 1] = out[g].mb[d][1];
+        }
+    }
+    return 0;
+  } catch (const std::exception &e) {
+    g_err = e.what();
+    return -1;
+  }
 }
 
 int bsam_download_flags(bsam_ctx *c, int patch, int *host, long n) {

@@ -50,6 +50,56 @@ class Cfg(C.Structure):
 _lib = None
 
 
+class RegridParams(C.Structure):
+    _fields_ = [("ndims", C.c_int), ("level", C.c_int), ("ibuffer", C.c_int),
+                ("minimumgridpoints", C.c_int), ("desiredfillratio", C.c_double),
+                ("root_mx", C.c_int * 3), ("root_mthbc", C.c_int * 6)]
+
+
+def regrid_level(ndims: int, level: int, ibuffer: int, minimumgridpoints: int,
+                 desiredfillratio: float, root_mx, root_mthbc, patches, flags):
+    """New subgrids of one level (BufferAndList, InflateEdgeTags, NewSubGrids)
+    from its patches [(mx, mglobal)] in level-list order and their error flags
+    (int32 arrays of shape mx, Fortran order; buffered in place). Returns
+    [(parent index, mbounds)] in creation order."""
+    L = lib()
+    rp = RegridParams(ndims, level, ibuffer, minimumgridpoints, desiredfillratio,
+                      (C.c_int * 3)(*[int(v) for v in list(root_mx)[:3]] + [1] * (3 - len(root_mx))),
+                      (C.c_int * 6)(*[int(v) for v in list(root_mthbc)[:6]] +
+                                    [1] * (6 - len(root_mthbc))))
+    n = len(patches)
+    mx = np.zeros(3 * max(n, 1), dtype=np.int32)
+    mg = np.ones(6 * max(n, 1), dtype=np.int32)
+    ptrs = (C.POINTER(C.c_int) * max(n, 1))()
+    keep = []
+    for k, ((pmx, pmg), fl) in enumerate(zip(patches, flags)):
+        for d in range(3):
+            mx[3 * k + d] = pmx[d] if d < ndims else 1
+            if d < ndims:
+                mg[6 * k + 2 * d], mg[6 * k + 2 * d + 1] = pmg[d][0], pmg[d][1]
+        if fl.dtype != np.int32 or not fl.flags.f_contiguous:
+            raise ValueError("flags must be Fortran-ordered int32 arrays")
+        keep.append(fl)
+        ptrs[k] = fl.ctypes.data_as(C.POINTER(C.c_int))
+    cap = 1
+    for (pmx, _) in patches:
+        c = 1
+        for d in range(ndims):
+            c *= pmx[d]
+        cap += c
+    parent = np.zeros(cap, dtype=np.int32)
+    mb = np.zeros(6 * cap, dtype=np.int32)
+    ng = C.c_int()
+    ip = C.POINTER(C.c_int)
+    rc = L.bsam_regrid_level(C.byref(rp), n, mx.ctypes.data_as(ip), mg.ctypes.data_as(ip),
+                             ptrs, cap, C.byref(ng), parent.ctypes.data_as(ip),
+                             mb.ctypes.data_as(ip))
+    if rc != 0:
+        raise BsamError(L.bsam_last_error(None).decode())
+    return [(int(parent[g]), tuple((int(mb[6 * g + 2 * d]), int(mb[6 * g + 2 * d + 1]))
+                                   for d in range(3))) for g in range(ng.value)]
+
+
 def lib():
     """Load libbsamgpu.so (raises if it has not been built)."""
     global _lib
@@ -85,6 +135,8 @@ def lib():
         L.bsam_error_flags.argtypes = [vp, C.c_int, C.c_double]
         L.bsam_download_flags.argtypes = [vp, C.c_int, ip, C.c_long]
         L.bsam_error_flags_user.argtypes = [vp, C.c_int, C.c_double]
+        L.bsam_regrid_level.argtypes = [C.POINTER(RegridParams), C.c_int, ip, ip,
+                                        C.POINTER(ip), C.c_int, ip, ip, ip]
         L.bsam_after_step_stats.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         for n in ("bsam_source", "bsam_copy_qold", "bsam_sync"):
             getattr(L, n).argtypes = [vp]

@@ -170,6 +170,32 @@ int bsam_time_step(bsam_ctx *ctx, int *ncycles, double *hist, int maxhist,
 /* Wait for all work queued on the context's stream. */
 int bsam_sync(bsam_ctx *ctx);
 
+/* ---- regrid: host-side clustering of a level's flags -------------------
+ * The integer core of the regrid of one level, on the host: ErrFlag's
+ * BufferAndList for every patch (lib/bsamroutines.f90:1159-1205), the
+ * inflation of edge tags into neighbouring and periodic-image patches
+ * (InflateEdgeTags :1207-1280, GetPeriodicTagOffset boundary.f90:1046-1201)
+ * and, per patch in level-list order, Berger-Rigoutsos clustering into new
+ * subgrids (RefineGrid / NewSubGrids / GetSignatures / GridFlagRatio
+ * bsamroutines.f90:1334-1643). Context-free. The level's npatch patches are
+ * given in level-list order by mx[3*p..], mglobal[6*p..] (lo, hi per
+ * dimension) and their errorflags(1:mx) (Fortran order, flags[p]; buffered
+ * in place as the reference buffers them). On return *ngrids new subgrids in
+ * creation order (MakeNewGrid order): parent[g] (index into the patch list)
+ * and mbounds[6*g..] in the parent's coarse indices. Returns 0, -1 with the
+ * message in bsam_last_error(NULL) (also when more than max_grids result). */
+typedef struct bsam_regrid_params {
+  int ndims, level;
+  int ibuffer;               /* ibuffer(level) */
+  int minimumgridpoints;     /* minimumgridpoints(level) */
+  double desiredfillratio;   /* desiredfillratios(level) */
+  int root_mx[3];            /* the root patch (PeriodicSetup's offsets) */
+  int root_mthbc[6];
+} bsam_regrid_params;
+int bsam_regrid_level(const bsam_regrid_params *rp, int npatch, const int *mx,
+                      const int *mglobal, int *const *flags, int max_grids, int *ngrids,
+                      int *parent, int *mbounds);
+
 /* Tuning knobs (no counterpart in the reference):
  *   "tail_cells"   levels from minlevel up whose single patch has at most
  *                  this many cells run inside one workgroup (default 4096;

@@ -33,10 +33,16 @@ def variant(name: str, steps: int):
     if name == "neumann_l3":
         g["mthbc"] = {i: 1 for i in range(1, 7)}
         g["maxlevel"] = 3
+    if name == "3d_l1":
+        # the 3D problem (QInit3D, SetErrFlagsUser3D, f25.12 frames) on a
+        # 32 x 16 x 32 root, periodic in x, user BC elsewhere, one level
+        g.update(ndims=3, maxlevel=1, minlevel=-3, mx={1: 32, 2: 16, 3: 32},
+                 mglobal={(1, 1): 1, (1, 2): 32, (2, 1): 1, (2, 2): 16, (3, 1): 1, (3, 2): 32})
+        deck["rundata"]["dt"] = 0.01
     return deck
 
 
-@pytest.mark.parametrize("name,steps", [("shipped", 6), ("neumann_l3", 3)])
+@pytest.mark.parametrize("name,steps", [("shipped", 6), ("neumann_l3", 3), ("3d_l1", 3)])
 def test_program_frames_identical(tmp_path, name, steps):
     for prog in (bp.REF_PROGRAM, bp.GPU_PROGRAM):
         if not os.path.exists(prog):
