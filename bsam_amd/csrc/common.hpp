@@ -99,6 +99,15 @@ struct StatTask {
   int patch, part, r0, r1;
 };
 
+// A box of cells copied between two q pools (TransferValues): nx*ny*nz cells
+// of each of nv variables, from src + i*1 + j*ssy + k*ssz + v*ssv to the
+// same place relative to dst with the destination's strides.
+struct BoxCopy {
+  long long s, d;
+  long long ssy, ssz, ssv, dsy, dsz, dsv;
+  int nx, ny, nz, nv;
+};
+
 // Which field set a level operation acts on.
 enum { PREC_SOLUTION_FP64 = 0, PREC_DEFECT_FP32 = 1 };
 

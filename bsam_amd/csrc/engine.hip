@@ -51,6 +51,7 @@ template <class T> T *dev_upload(const std::vector<T> &v, hipStream_t s) {
 
 struct LevelData {
   std::vector<int> patches;  // patch indices in level-list order
+  int *d_patch_list = nullptr;  // the same on the device (regrid kernels)
   std::vector<Tile> t_int, t_ext, t_coarse, t_cext;
   Tile *d_int = nullptr, *d_ext = nullptr, *d_coarse = nullptr, *d_cext = nullptr;
   std::vector<GhostOp> gp[3];
@@ -249,7 +250,7 @@ void bsam_ctx::free_device() {
   q = qold = qc = f = rf = ftmp = nullptr;
   d_P = nullptr; d_partial = nullptr; d_hist = nullptr; d_state = nullptr;
   for (auto &l : lev) {
-    F(l.d_int); F(l.d_ext); F(l.d_coarse); F(l.d_cext);
+    F(l.d_int); F(l.d_ext); F(l.d_coarse); F(l.d_cext); F(l.d_patch_list);
     for (int p = 0; p < 3; p++) {
       F(l.d_gp[p]);
       F(l.d_tgp[p]);
@@ -1348,6 +1349,108 @@ int bsam_after_step_stats(bsam_ctx *c, double *mass, double *energy) {
   *mass = m;
   *energy = e;
   API_END(c)
+}
+
+// Same geometry (level, parent, mx, mbounds, mglobal, mthbc, dx, xlower) for
+// the patches of levels minlevel..top of two contexts, in the same order.
+static bool same_levels(const bsam_ctx *a, const bsam_ctx *b, int top) {
+  if (a->cfg.minlevel != b->cfg.minlevel || a->cfg.ndims != b->cfg.ndims ||
+      a->cfg.nrvars != b->cfg.nrvars)
+    return false;
+  std::vector<HPatch> x, y;
+  for (const HPatch &h : a->hp)
+    if (h.level <= top) x.push_back(h);
+  for (const HPatch &h : b->hp)
+    if (h.level <= top) y.push_back(h);
+  return same_geometry(x, y);
+}
+
+int bsam_copy_levels(bsam_ctx *dst, const bsam_ctx *src, int top) {
+  API_BEGIN(dst)
+  dst->need_device();
+  if (!src || src->host_only || !src->committed) throw Error("copy_levels: bad source context");
+  if (src->cfg.device != dst->cfg.device) throw Error("copy_levels: contexts on different devices");
+  if (top < dst->cfg.minlevel || top > dst->finest || top > src->finest)
+    throw Error("copy_levels: bad level");
+  if (!same_levels(dst, src, top)) throw Error("copy_levels: hierarchies differ below the level");
+  // levels minlevel..top occupy the same leading range of both q pools
+  long long end = 0;
+  for (size_t p = 0; p < dst->hp.size(); p++)
+    if (dst->hp[p].level <= top)
+      end = std::max(end, dst->dp[p].q + dst->dp[p].qvs * dst->cfg.nrvars);
+  HIPCK(hipStreamSynchronize(src->stream));
+  HIPCK(hipMemcpyAsync(dst->q, src->q, sizeof(double) * end, hipMemcpyDeviceToDevice,
+                       dst->stream));
+  API_END(dst)
+}
+
+int bsam_init_fields(bsam_ctx *c, int level) {
+  API_BEGIN(c)
+  c->need_device();
+  if (level <= c->cfg.minlevel || level > c->finest || level < 1)
+    throw Error("init_fields: bad level");
+  LevelData &L = c->L(level);
+  for (int p : L.patches)
+    for (int d = 0; d < c->nd(); d++)
+      if (c->hp[p].mx[d] % 2) throw Error("init_fields: refined patches have even mx");
+  if (!L.d_patch_list) L.d_patch_list = dev_upload(L.patches, c->stream);
+  HIPCK(launch_init_fields(c->nd(), c->dev(), L.d_patch_list, (int)L.patches.size(), c->stream));
+  API_END(c)
+}
+
+int bsam_transfer_values(bsam_ctx *dst, const bsam_ctx *src, int level) {
+  API_BEGIN(dst)
+  dst->need_device();
+  if (!src || src->host_only || !src->committed) throw Error("transfer: bad source context");
+  if (src->cfg.device != dst->cfg.device) throw Error("transfer: contexts on different devices");
+  if (src->cfg.ndims != dst->cfg.ndims || src->cfg.nrvars != dst->cfg.nrvars)
+    throw Error("transfer: contexts differ in ndims or nrvars");
+  if (level < dst->cfg.minlevel || level > dst->finest) throw Error("transfer: bad level");
+  if (level > src->finest) return 0;   // no old grids on that level
+  const int nd = dst->nd(), nv = dst->cfg.nrvars;
+  std::vector<BoxCopy> boxes;
+  // every (new, old) pair of the level: the overlap of their interiors in
+  // the level's global index space, interior cells of all variables
+  for (int a : dst->L(level).patches)
+    for (int b : src->lev[level - src->cfg.minlevel].patches) {
+      const HPatch &t = dst->hp[a], &o = src->hp[b];
+      int lo[3] = {1, 1, 1}, hi[3] = {1, 1, 1};
+      bool empty = false;
+      for (int d = 0; d < nd; d++) {
+        lo[d] = std::max(t.mg[d][0], o.mg[d][0]);
+        hi[d] = std::min(t.mg[d][1], o.mg[d][1]);
+        empty = empty || hi[d] < lo[d];
+      }
+      if (empty) continue;
+      const DPatch &T = dst->dp[a], &O = src->dp[b];
+      BoxCopy B;
+      auto off = [&](const DPatch &P, const HPatch &h) {
+        long long i = lo[0] - h.mg[0][0] + 1, j = lo[1] - h.mg[1][0] + 1;
+        long long k = nd == 3 ? lo[2] - h.mg[2][0] + 1 : 0;
+        return P.q + i + (long long)P.px * (j + (long long)P.py * k);
+      };
+      B.s = off(O, o);
+      B.d = off(T, t);
+      B.ssy = O.px;
+      B.ssz = (long long)O.px * O.py;
+      B.ssv = O.qvs;
+      B.dsy = T.px;
+      B.dsz = (long long)T.px * T.py;
+      B.dsv = T.qvs;
+      B.nx = hi[0] - lo[0] + 1;
+      B.ny = hi[1] - lo[1] + 1;
+      B.nz = nd == 3 ? hi[2] - lo[2] + 1 : 1;
+      B.nv = nv;
+      boxes.push_back(B);
+    }
+  if (boxes.empty()) return 0;
+  HIPCK(hipStreamSynchronize(src->stream));
+  BoxCopy *d_boxes = dev_upload(boxes, dst->stream);
+  hipError_t e = launch_box_copy(dst->q, src->q, d_boxes, (int)boxes.size(), dst->stream);
+  (void)hipStreamSynchronize(dst->stream);
+  (void)hipFree(d_boxes);
+  HIPCK(e);
+  API_END(dst)
 }
 
 int bsam_regrid_level(const bsam_regrid_params *rp, int npatch, const int *mx,

@@ -949,6 +949,111 @@ __global__ __launch_bounds__(256) void k_after_step(Dev d, const StatTask *tasks
   }
 }
 
+// InitFields for a grid created by the regrid (lib/bsamroutines.f90:1685-1772,
+// mbc = 1), one workgroup per patch of the list: qc(0:cmx+1) = the parent's
+// q under the patch with one ring, and q(0:mx+1) = BiLinProlongationP1MC /
+// TriLinProlongationP1MC of it (lib/gridutilities.f90:219-279, :588-686):
+// linear 3:1 interpolation dimension by dimension (/16 in 2D, /64 in 3D),
+// then every interior cell adds its coarse cell's mass correction
+// a - (sum of its 2^d children's predictions)/2^d. qold = q.
+__device__ __forceinline__ double lin31(double a0, double a1, bool even) {
+  return even ? 3.0 * a0 + a1 : a0 + 3.0 * a1;
+}
+
+template <int ND>
+__device__ __forceinline__ double p1_pred(const double *a, long long ay, long long az, int i,
+                                          int j, int k) {
+  const int m = i >> 1, n = j >> 1;
+  const bool ei = (i & 1) == 0, ej = (j & 1) == 0;
+  if constexpr (ND == 2) {
+    const double b0 = lin31(a[m + ay * n], a[m + 1 + ay * n], ei);
+    const double b1 = lin31(a[m + ay * (n + 1)], a[m + 1 + ay * (n + 1)], ei);
+    return lin31(b0, b1, ej) / 16.0;
+  } else {
+    const int o = k >> 1;
+    const bool ek = (k & 1) == 0;
+    double cz[2];
+    for (int t = 0; t < 2; t++) {
+      const double *ap = a + az * (o + t);
+      const double b0 = lin31(ap[m + ay * n], ap[m + 1 + ay * n], ei);
+      const double b1 = lin31(ap[m + ay * (n + 1)], ap[m + 1 + ay * (n + 1)], ei);
+      cz[t] = lin31(b0, b1, ej);
+    }
+    return lin31(cz[0], cz[1], ek) / 64.0;
+  }
+}
+
+template <int ND>
+__global__ __launch_bounds__(256) void k_init_fields(Dev d, const int *patches) {
+  const DPatch &P = d.P[patches[blockIdx.x]];
+  const DPatch &A = d.P[P.parent];
+  const long long ay = A.px, az = (long long)A.px * A.py;
+  const long long fy = P.px, fz = (long long)P.px * P.py;
+  const int cx = P.mx / 2, cy = P.my / 2, cz = ND == 3 ? P.mz / 2 : 0;
+  const long long cpy = cx + 2, cpz = (long long)(cx + 2) * (cy + 2);
+  for (int v = 0; v < d.nrvars; v++) {
+    // a(0,0,0): the parent cell one below the patch's footprint
+    const double *a = d.q + A.q + v * A.qvs + (P.mb[0][0] - 1) + ay * (P.mb[1][0] - 1) +
+                      (ND == 3 ? az * (P.mb[2][0] - 1) : 0);
+    double *qc = d.qc + P.qc + v * P.cvs;
+    const int nc = (cx + 2) * (cy + 2) * (ND == 3 ? cz + 2 : 1);
+    for (int e = threadIdx.x; e < nc; e += blockDim.x) {
+      const int i = e % (cx + 2), j = (e / (cx + 2)) % (cy + 2);
+      const int k = ND == 3 ? e / ((cx + 2) * (cy + 2)) : 0;
+      qc[i + cpy * j + cpz * k] = a[i + ay * j + az * k];
+    }
+    double *q = d.q + P.q + v * P.qvs;
+    double *qo = d.qold + P.q + v * P.qvs;
+    const int ex = P.mx + 2, ey = P.my + 2, ez = ND == 3 ? P.mz + 2 : 1;
+    const int nf = ex * ey * ez;
+    for (int e = threadIdx.x; e < nf; e += blockDim.x) {
+      const int i = e % ex, j = (e / ex) % ey, k = ND == 3 ? e / (ex * ey) : 0;
+      double p = p1_pred<ND>(a, ay, az, i, j, k);
+      const bool inner = i >= 1 && i <= P.mx && j >= 1 && j <= P.my &&
+                         (ND == 2 || (k >= 1 && k <= P.mz));
+      if (inner) {
+        const int ic = (i + 1) >> 1, jc = (j + 1) >> 1, kc = ND == 3 ? (k + 1) >> 1 : 0;
+        double sum;
+        if constexpr (ND == 2) {
+          sum = ((p1_pred<2>(a, ay, az, 2 * ic, 2 * jc, 0) +
+                  p1_pred<2>(a, ay, az, 2 * ic - 1, 2 * jc, 0)) +
+                 p1_pred<2>(a, ay, az, 2 * ic, 2 * jc - 1, 0)) +
+                p1_pred<2>(a, ay, az, 2 * ic - 1, 2 * jc - 1, 0);
+          p = p + (a[ic + ay * jc] - 0.25 * sum);
+        } else {
+          sum = 0.0;
+          const int ci[8][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {1, 1, 0},
+                                {0, 0, 1}, {1, 0, 1}, {0, 1, 1}, {1, 1, 1}};
+          for (int t = 0; t < 8; t++) {
+            double c = p1_pred<3>(a, ay, az, 2 * ic - ci[t][0], 2 * jc - ci[t][1],
+                                  2 * kc - ci[t][2]);
+            sum = t == 0 ? c : sum + c;
+          }
+          p = p + (a[ic + ay * jc + az * kc] - 0.125 * sum);
+        }
+      }
+      q[i + fy * j + fz * k] = p;
+      qo[i + fy * j + fz * k] = p;
+    }
+  }
+}
+
+// Box copies between two q pools (TransferValues' overlaps), one workgroup
+// per box.
+__global__ __launch_bounds__(256) void k_box_copy(double *dst, const double *src,
+                                                  const BoxCopy *boxes) {
+  const BoxCopy B = boxes[blockIdx.x];
+  const long long n = (long long)B.nx * B.ny * B.nz * B.nv;
+  for (long long e = threadIdx.x; e < n; e += blockDim.x) {
+    const int i = (int)(e % B.nx);
+    const int j = (int)((e / B.nx) % B.ny);
+    const int k = (int)((e / ((long long)B.nx * B.ny)) % B.nz);
+    const int v = (int)(e / ((long long)B.nx * B.ny * B.nz));
+    dst[B.d + i + B.dsy * j + B.dsz * k + B.dsv * v] =
+        src[B.s + i + B.ssy * j + B.ssz * k + B.ssv * v];
+  }
+}
+
 template <int ND>
 __global__ __launch_bounds__(256) void k_source(Dev d, const Tile *tiles, int nt, int full) {
   const int b = xcd_tile(nt);
@@ -1528,6 +1633,19 @@ hipError_t launch_after_step(int nd, const Dev &d, const StatTask *tasks, int n,
                              double hi, double *out, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   BSAM_LAUNCH_ND(k_after_step, nd, dim3((unsigned)n), 256, d, tasks, lo, hi, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_init_fields(int nd, const Dev &d, const int *patches, int n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  BSAM_LAUNCH_ND(k_init_fields, nd, dim3((unsigned)n), 256, d, patches);
+  return hipGetLastError();
+}
+
+hipError_t launch_box_copy(double *dst, const double *src, const BoxCopy *boxes, int n,
+                           hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  k_box_copy<<<dim3((unsigned)n), 256, 0, s>>>(dst, src, boxes);
   return hipGetLastError();
 }
 

@@ -46,6 +46,11 @@ hipError_t launch_error_flags(int nd, const Dev &d, const Tile *tiles, int nt, d
 // SetErrFlagsUser2D/3D on the interior tiles of a level (q ghosts set).
 hipError_t launch_error_flags_user(int nd, const Dev &d, const Tile *tiles, int nt,
                                    double qtol, hipStream_t s);
+// InitFields (MC prolongation from the parent) for patches[0..n).
+hipError_t launch_init_fields(int nd, const Dev &d, const int *patches, int n, hipStream_t s);
+// Box copies src -> dst.
+hipError_t launch_box_copy(double *dst, const double *src, const BoxCopy *boxes, int n,
+                           hipStream_t s);
 // The sums of AfterStep2D/3D for tasks[0..n) into out[3n].
 hipError_t launch_after_step(int nd, const Dev &d, const StatTask *tasks, int n, double lo,
                              double hi, double *out, hipStream_t s);

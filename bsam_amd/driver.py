@@ -1,0 +1,291 @@
+"""The BSAM time-step driver with its regrid every step, on the MI355X engine.
+
+Reads the reference's input deck (rundata.dat, griddata.dat,
+problemdata.dat: the namelists of BSAMSolver, lib/bsamroutines.f90:68-100,
+RootInit :216-254, SetProb problem.f90:12-37) and runs TakeTimeSteps
+(lib/bsamroutines.f90:582-789) for the Cahn-Hilliard problem:
+
+* every step AMR(rootlevel) (:941-961): per level, SetGhost(level, 0), the
+  error estimate (EstimateLevelErrors :963-1002: the truncation error and
+  the default or the problem's own tagging, on the GPU), the clustering of
+  the flags into new subgrids (bsam_regrid_level, on the host), and
+  GridAdapt (:1288-1332): the new level's hierarchy is committed, the
+  levels below copied over, the new patches initialised (QInit on initial
+  grids, else InitFields' mass-corrected prolongation) and given the values
+  of the old grids they overlap (TransferValues), all on the GPU;
+* FillDown on the first step, qold = q, MultigridIterations (eqfn times);
+* the after-step statistics (out/stats.dat) if asked for, and the frames
+  out/mNNNNN.dat (frame 0 after the first regrid).
+
+The hierarchy under construction lives in a spare engine context; three
+contexts serve any depth: the hierarchy before the step (the old grids) and
+two that alternate as the new one grows level by level.
+"""
+
+from __future__ import annotations
+
+import os
+import re
+import time
+from typing import Dict, List, Optional
+
+import numpy as np
+
+from . import frames
+from .engine import FIELD_Q, Engine, regrid_level
+from .layout import Layout
+from .problem import qinit2d, qinit3d
+
+ERRFLAG_DEFAULT = 1
+ERRFLAG_USER = 10
+
+
+def read_namelist(path: str) -> Dict:
+    """The first namelist group of a file as {name: value} with indexed
+    entries as {name: {index: value}} (indices as ints or int tuples)."""
+    text = open(path).read()
+    m = re.search(r"&\w+(.*?)/", text, re.S)
+    body = m.group(1) if m else ""
+    out: Dict = {}
+    for key, idx, val in re.findall(r"(\w+)\s*(?:\(([^)]*)\))?\s*=\s*([^\s,/]+)", body):
+        key = key.lower()
+        v = val.strip()
+        lv = v.lower()
+        if lv in (".true.", "t", ".t."):
+            x = True
+        elif lv in (".false.", "f", ".f."):
+            x = False
+        elif re.fullmatch(r"[-+]?\d+", v):
+            x = int(v)
+        else:
+            x = float(lv.replace("d", "e"))
+        if idx:
+            ii = tuple(int(t) for t in idx.split(","))
+            out.setdefault(key, {})[ii[0] if len(ii) == 1 else ii] = x
+        else:
+            out[key] = x
+    return out
+
+
+def _arr(v, n, default, base=1):
+    """A namelist array (dict by index or scalar) as a list over base..base+n-1."""
+    if isinstance(v, dict):
+        return [v.get(i, default) for i in range(base, base + n)]
+    if v is None:
+        return [default] * n
+    return [v] * n
+
+
+class Driver:
+    """TakeTimeSteps on the GPU engine, regridding every step."""
+
+    def __init__(self, workdir: str, device: int = 0):
+        self.workdir = workdir
+        rd = read_namelist(os.path.join(workdir, "rundata.dat"))
+        gd = read_namelist(os.path.join(workdir, "griddata.dat"))
+        pd = read_namelist(os.path.join(workdir, "problemdata.dat"))
+        # BSAMSolver's defaults (lib/bsamroutines.f90:74-89) under the deck
+        self.dt = float(rd.get("dt", 0.0))
+        self.maxvcycles = int(rd.get("maxvcycles", 20))
+        self.nsmooth = int(rd.get("nsmoothingpasses", 2))
+        self.qerrortol = float(rd.get("qerrortol", 1.0e-6))
+        self.outframes = int(rd.get("outframes", 1))
+        self.timeiterations = int(rd.get("timeiterations", 1))
+        self.stats = bool(rd.get("getafterstepstats", False))
+        self.eqfn = int(rd.get("eqfn", 6))
+        if rd.get("restart", False) or rd.get("syncelliptic", False):
+            raise NotImplementedError("restart and syncelliptic runs are not supported")
+        if int(rd.get("errortype", 1)) != 1 or int(rd.get("updateauxfreq", 1)) != 1:
+            raise NotImplementedError("errortype 1 and updateauxfreq 1 only")
+        # RootInit's defaults (:237-254)
+        self.ndims = nd = int(gd.get("ndims", 2))
+        self.maxlevel = int(gd.get("maxlevel", 0))
+        self.minlevel = int(gd.get("minlevel", 0))
+        depth = self.maxlevel + 1
+        self.qtol = _arr(gd.get("qtolerance"), depth, 1.0e-6, 0)
+        self.ibuffer = _arr(gd.get("ibuffer"), depth, 0, 0)
+        self.mgp = _arr(gd.get("minimumgridpoints"), depth, 2, 0)
+        self.fill = _arr(gd.get("desiredfillratios"), depth, 8.0e-01, 0)
+        self.errflag = _arr(gd.get("errflagopt"), depth, ERRFLAG_DEFAULT, 0)
+        self.mx = [int(v) for v in _arr(gd.get("mx"), nd, 1)]
+        self.xlower = [float(v) for v in _arr(gd.get("xlower"), nd, 0.0)]
+        self.xupper = [float(v) for v in _arr(gd.get("xupper"), nd, 1.0)]
+        self.mthbc = [int(v) for v in _arr(gd.get("mthbc"), 2 * nd, 1)]
+        if int(gd.get("mbc", 1)) != 1 or int(gd.get("maux", 0)) != 0 or int(gd.get("nrvars", 1)) != 2:
+            raise NotImplementedError("mbc = 1, maux = 0 and nrvars = 2 only")
+        self.alpha = float(pd.get("alpha", 1.0))
+        self.eps = float(pd.get("eps", 0.0))
+        # the root grid: dx = (xupper - xlower) / mx (RootInit :363-364)
+        dx = (self.xupper[0] - self.xlower[0]) / float(self.mx[0])
+        xl = tuple(self.xlower) + (0.0,) * (3 - nd)
+        self.root_layout = Layout(nd, self.mx, dx, xlower=xl, mthbc=self.mthbc,
+                                  minlevel=self.minlevel, nrvars=2)
+        self.minlevel = self.root_layout.minlevel
+        self.device = device
+        self._engines: List[Engine] = []
+        self.timing = {"regrid_s": 0.0, "solve_s": 0.0, "frames_s": 0.0, "steps": 0}
+
+    # -- engines --------------------------------------------------------------
+    def _engine(self, k: int, layout: Layout) -> Engine:
+        while len(self._engines) <= k:
+            self._engines.append(None)
+        if self._engines[k] is None:
+            self._engines[k] = Engine(layout, dt=self.dt, eps=self.eps, alpha=self.alpha,
+                                      well="reference", nsmoothingpasses=self.nsmooth,
+                                      maxvcycles=self.maxvcycles, qerrortol=self.qerrortol,
+                                      device=self.device)
+        else:
+            self._engines[k].describe(layout)
+        return self._engines[k]
+
+    def close(self):
+        for e in self._engines:
+            if e is not None:
+                e.close()
+        self._engines = []
+
+    # -- initial condition ----------------------------------------------------
+    def _qinit(self, p) -> np.ndarray:
+        if self.ndims == 2:
+            return qinit2d(p.mx, p.dx, p.xlower)
+        return qinit3d(p.mx, p.dx, p.xlower, self.eps)
+
+    def _upload_qinit(self, eng: Engine, level: int):
+        for n, p in enumerate(eng.patches):
+            if p.level == level:
+                q = self._qinit(p)
+                eng.upload(n, "q", q)
+                eng.upload(n, "qold", q)
+
+    # -- AMR --------------------------------------------------------------------
+    def _flags(self, eng: Engine, level: int):
+        if self.errflag[level] == ERRFLAG_DEFAULT:
+            eng.error_flags(level, self.qtol[level])
+        elif self.errflag[level] == ERRFLAG_USER:
+            eng.error_flags_user(level, self.qtol[level])
+        else:
+            raise ValueError(f"errflagopt({level}) = {self.errflag[level]} is not supported")
+        out = []
+        for n, p in enumerate(eng.patches):
+            if p.level == level:
+                out.append((p, np.asfortranarray(eng.flags(n).astype(np.int32))))
+        return out
+
+    def _amr(self, old: int, initial: bool) -> int:
+        """AMR(rootlevel) from the hierarchy in engine `old`; returns the
+        engine that holds the new hierarchy."""
+        spares = [k for k in range(3) if k != old]
+        cur = old
+        lay = self._engines[old].layout
+        nb = 0   # which spare builds next
+        level = 0
+        while True:
+            eng = self._engines[cur]
+            eng.set_ghost(level, 0)
+            if level >= self.maxlevel:
+                break
+            # EstimateLevelErrors(level): its truncation-error part always
+            # runs (the restriction onto level-1 is part of the state)
+            eng.truncation_error(level)
+            tagged = self._flags(eng, level)
+            root = lay.patches[0]
+            new = regrid_level(self.ndims, level, int(self.ibuffer[level]), int(self.mgp[level]),
+                               float(self.fill[level]), root.mx, root.mthbc,
+                               [(p.mx, p.mglobal) for p, _ in tagged], [f for _, f in tagged])
+            if not new and lay.finestlevel == level:
+                nlay = None   # the hierarchy already ends at this level
+            else:
+                # GridAdapt(level): the levels up to `level` and the new grids
+                nlay, idmap = self._truncated(lay, level)
+                for k, mb in new:
+                    nlay.add_child(idmap[tagged[k][0].id], mb[:self.ndims])
+                nxt = spares[nb % 2]
+                nb += 1
+                neng = self._engine(nxt, nlay)
+                neng.copy_levels_from(eng, level)
+                if new:
+                    if initial:
+                        self._upload_qinit(neng, level + 1)   # InitFields' QInit
+                    else:
+                        neng.init_fields(level + 1)
+                        neng.transfer_values_from(self._engines[old], level + 1)
+                cur, lay = nxt, nlay
+            if not new:
+                # AMR(level+1) on the empty level: its EstimateLevelErrors
+                # still refills this level's ghosts (SetGhost(level, 0))
+                if level + 1 < self.maxlevel:
+                    self._engines[cur].set_ghost(level, 0)
+                break
+            level += 1
+        return cur
+
+    def _truncated(self, lay: Layout, top: int):
+        """The levels <= top of `lay` as a new Layout in the same creation
+        order, and the map from `lay`'s patch ids to the new ones."""
+        root = lay.patches[0]
+        out = Layout(lay.ndims, root.mx[:lay.ndims], root.dx, xlower=root.xlower,
+                     mthbc=root.mthbc, minlevel=lay.minlevel, nrvars=lay.nrvars)
+        idmap = {pid: pid for pid in lay.patches if lay.patches[pid].level <= 0}
+        for level in range(1, top + 1):
+            for pid in lay.creation.get(level, []):
+                p = lay.patches[pid]
+                idmap[pid] = out.add_child(idmap[p.parent_id], p.mbounds[:lay.ndims])
+        return out, idmap
+
+    # -- frames and statistics ------------------------------------------------
+    def _write_frame(self, eng: Engine, n: int, t: float):
+        t0 = time.perf_counter()
+        path = os.path.join(self.workdir, "out", f"m{n:05d}.dat")
+        frames.write_frame(path, t, eng.layout.finestlevel,
+                           frames.frame_from_engine(eng, self.xupper))
+        self.timing["frames_s"] += time.perf_counter() - t0
+
+    def _mesh_size(self, lay: Layout) -> int:
+        """GetMeshSize (:836-858) summed over levels >= 0."""
+        total = 0
+        for level in range(0, lay.finestlevel + 1):
+            for pid in lay.level_ids(level):
+                p = lay.patches[pid]
+                m = c = 1
+                for d in range(self.ndims):
+                    m *= p.mx[d]
+                    c *= p.mx[d] // 2
+                total += m if level == 0 else m - c
+        return total
+
+    # -- TakeTimeSteps ----------------------------------------------------------
+    def run(self):
+        os.makedirs(os.path.join(self.workdir, "out"), exist_ok=True)
+        cur = 0
+        eng = self._engine(0, self.root_layout)
+        self._upload_qinit(eng, 0)           # Initialq on the root
+        eng.set_ghost(0, 0)
+        itperprint = self.timeiterations // self.outframes
+        currenttime = 0.0
+        first = True
+        for n in range(1, self.outframes + 1):
+            for it in range(1, itperprint + 1):
+                t0 = time.perf_counter()
+                cur = self._amr(cur, initial=first)
+                eng = self._engines[cur]
+                if first:
+                    eng.fill_down(FIELD_Q)
+                eng.copy_qold()
+                self.timing["regrid_s"] += time.perf_counter() - t0
+                if first:
+                    self._write_frame(eng, 0, currenttime)
+                first = False
+                t0 = time.perf_counter()
+                for _ in range(self.eqfn):
+                    eng.multigrid_iterations()
+                eng.sync()
+                self.timing["solve_s"] += time.perf_counter() - t0
+                self.timing["steps"] += 1
+                currenttime = real_time = float((n - 1) * itperprint + it) * self.dt
+                if self.stats:
+                    mass, energy = eng.after_step_stats()
+                    with open(os.path.join(self.workdir, "out", "stats.dat"), "a") as fh:
+                        fh.write(f"{real_time:25.12f}{mass:25.12f}{energy:25.12f} "
+                                 f"{self._mesh_size(eng.layout):8d}\n")
+            self._write_frame(eng, n, currenttime)
+        return self.timing

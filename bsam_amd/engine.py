@@ -135,6 +135,9 @@ def lib():
         L.bsam_error_flags.argtypes = [vp, C.c_int, C.c_double]
         L.bsam_download_flags.argtypes = [vp, C.c_int, ip, C.c_long]
         L.bsam_error_flags_user.argtypes = [vp, C.c_int, C.c_double]
+        L.bsam_copy_levels.argtypes = [vp, vp, C.c_int]
+        L.bsam_init_fields.argtypes = [vp, C.c_int]
+        L.bsam_transfer_values.argtypes = [vp, vp, C.c_int]
         L.bsam_regrid_level.argtypes = [C.POINTER(RegridParams), C.c_int, ip, ip,
                                         C.POINTER(ip), C.c_int, ip, ip, ip]
         L.bsam_after_step_stats.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
@@ -271,6 +274,18 @@ class Engine:
     def error_flags(self, level: int, qtolerance: float):
         """SetErrFlags (default tagging) on ``level`` from qrte."""
         self._ck(lib().bsam_error_flags(self.ctx, level, qtolerance))
+
+    def copy_levels_from(self, src: "Engine", top: int):
+        """q of levels minlevel..top from another engine with the same patches there."""
+        self._ck(lib().bsam_copy_levels(self.ctx, src.ctx, top))
+
+    def init_fields(self, level: int):
+        """InitFields (MC prolongation from the parent) for the patches of `level`."""
+        self._ck(lib().bsam_init_fields(self.ctx, level))
+
+    def transfer_values_from(self, src: "Engine", level: int):
+        """TransferValues: q of the cells `level`'s patches share with src's."""
+        self._ck(lib().bsam_transfer_values(self.ctx, src.ctx, level))
 
     def error_flags_user(self, level: int, qtolerance: float):
         self._ck(lib().bsam_error_flags_user(self.ctx, level, qtolerance))

@@ -184,6 +184,23 @@ int bsam_sync(bsam_ctx *ctx);
  * creation order (MakeNewGrid order): parent[g] (index into the patch list)
  * and mbounds[6*g..] in the parent's coarse indices. Returns 0, -1 with the
  * message in bsam_last_error(NULL) (also when more than max_grids result). */
+/* GridAdapt's field work (lib/bsamroutines.f90:1288-1332) between the
+ * hierarchy before a regrid (src) and the one being built (dst, committed
+ * with the new patches of level+1; same device):
+ *   bsam_copy_levels     q of levels minlevel..top, whose patches must be
+ *                        those of src in the same order (they persist);
+ *   bsam_init_fields     InitFields (:1685-1772) for every patch of `level`
+ *                        of dst: qc = the parent's q under it with one ring,
+ *                        q = qold = BiLinProlongationP1MC / TriLinProlongation
+ *                        P1MC of it (gridutilities.f90:219-279, :588-686);
+ *   bsam_transfer_values TransferValues / TransferOverlap (:1774-1867): every
+ *                        new patch of `level` takes the q of the interior cells
+ *                        it shares with src's patches of that level (the old
+ *                        grids). */
+int bsam_copy_levels(bsam_ctx *dst, const bsam_ctx *src, int top);
+int bsam_init_fields(bsam_ctx *ctx, int level);
+int bsam_transfer_values(bsam_ctx *dst, const bsam_ctx *src, int level);
+
 typedef struct bsam_regrid_params {
   int ndims, level;
   int ibuffer;               /* ibuffer(level) */

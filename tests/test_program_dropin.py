@@ -11,6 +11,7 @@ with its ghost layer, as printed.
 """
 
 import os
+import time
 
 import pytest
 
@@ -56,3 +57,33 @@ def test_program_frames_identical(tmp_path, name, steps):
     assert sorted(gpu) == sorted(ref), (sorted(gpu), out[-2000:])
     for n in sorted(ref):
         assert gpu[n] == ref[n], f"frame {n}: {first_difference(gpu[n], ref[n])}"
+
+
+@pytest.mark.parametrize("name,steps", [("shipped", 6), ("neumann_l3", 3), ("3d_l1", 3)])
+def test_native_driver_frames_identical(tmp_path, name, steps):
+    """The engine's own time-step driver (bsam_amd/driver.py: the regrid's
+    clustering on the host, its tagging, InitFields and TransferValues on
+    the GPU) on the same deck writes the reference program's frames byte
+    for byte."""
+    if not os.path.exists(bp.REF_PROGRAM):
+        pytest.skip("reference program not built (needs the reference build)")
+    from bsam_amd.driver import Driver
+    deck = variant(name, steps)
+    ref, _, t_ref = bp.run(bp.REF_PROGRAM, deck, str(tmp_path / "ref"))
+    work = str(tmp_path / "native")
+    bp.write_deck(work, deck)
+    drv = Driver(work, device=0)
+    t0 = time.perf_counter()
+    timing = drv.run()
+    t_native = time.perf_counter() - t0
+    drv.close()
+    print(f"{name}: {steps} steps, reference program {t_ref:.2f} s, native driver "
+          f"{t_native:.2f} s ({timing})")
+    got = {}
+    for f in sorted(os.listdir(os.path.join(work, "out"))):
+        if f.startswith("m") and f.endswith(".dat"):
+            with open(os.path.join(work, "out", f), "rb") as fh:
+                got[int(f[1:6])] = fh.read()
+    assert sorted(got) == sorted(ref)
+    for n in sorted(ref):
+        assert got[n] == ref[n], f"frame {n}: {first_difference(got[n], ref[n])}"
