@@ -123,7 +123,11 @@ class Driver:
         self.minlevel = self.root_layout.minlevel
         self.device = device
         self._engines: List[Engine] = []
-        self.timing = {"regrid_s": 0.0, "solve_s": 0.0, "frames_s": 0.0, "steps": 0}
+        # wall time by phase: the regrid (of which the hierarchy hand-overs,
+        # the tagging with its download, the clustering and the field work),
+        # the solves and the frames
+        self.timing = {"regrid_s": 0.0, "commit_s": 0.0, "tag_s": 0.0, "cluster_s": 0.0,
+                       "fields_s": 0.0, "solve_s": 0.0, "frames_s": 0.0, "steps": 0}
 
     # -- engines --------------------------------------------------------------
     def _engine(self, k: int, layout: Layout) -> Engine:
@@ -186,12 +190,17 @@ class Driver:
                 break
             # EstimateLevelErrors(level): its truncation-error part always
             # runs (the restriction onto level-1 is part of the state)
+            t0 = time.perf_counter()
             eng.truncation_error(level)
             tagged = self._flags(eng, level)
+            t1 = time.perf_counter()
             root = lay.patches[0]
             new = regrid_level(self.ndims, level, int(self.ibuffer[level]), int(self.mgp[level]),
                                float(self.fill[level]), root.mx, root.mthbc,
                                [(p.mx, p.mglobal) for p, _ in tagged], [f for _, f in tagged])
+            t2 = time.perf_counter()
+            self.timing["tag_s"] += t1 - t0
+            self.timing["cluster_s"] += t2 - t1
             if not new and lay.finestlevel == level:
                 nlay = None   # the hierarchy already ends at this level
             else:
@@ -201,7 +210,9 @@ class Driver:
                     nlay.add_child(idmap[tagged[k][0].id], mb[:self.ndims])
                 nxt = spares[nb % 2]
                 nb += 1
+                t0 = time.perf_counter()
                 neng = self._engine(nxt, nlay)
+                t1 = time.perf_counter()
                 neng.copy_levels_from(eng, level)
                 if new:
                     if initial:
@@ -209,6 +220,9 @@ class Driver:
                     else:
                         neng.init_fields(level + 1)
                         neng.transfer_values_from(self._engines[old], level + 1)
+                neng.sync()
+                self.timing["commit_s"] += t1 - t0
+                self.timing["fields_s"] += time.perf_counter() - t1
                 cur, lay = nxt, nlay
             if not new:
                 # AMR(level+1) on the empty level: its EstimateLevelErrors
@@ -289,3 +303,23 @@ class Driver:
                                  f"{self._mesh_size(eng.layout):8d}\n")
             self._write_frame(eng, n, currenttime)
         return self.timing
+
+
+def main(argv=None):
+    """python -m bsam_amd.driver WORKDIR [--device N]: runs the deck in
+    WORKDIR (rundata.dat, griddata.dat, problemdata.dat) and writes WORKDIR/out."""
+    import argparse
+    import json
+    ap = argparse.ArgumentParser(description="BSAM time steps with regrid on the GPU engine")
+    ap.add_argument("workdir")
+    ap.add_argument("--device", type=int, default=0)
+    a = ap.parse_args(argv)
+    drv = Driver(a.workdir, device=a.device)
+    try:
+        print(json.dumps(drv.run()))
+    finally:
+        drv.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -87,3 +87,32 @@ def test_native_driver_frames_identical(tmp_path, name, steps):
     assert sorted(got) == sorted(ref)
     for n in sorted(ref):
         assert got[n] == ref[n], f"frame {n}: {first_difference(got[n], ref[n])}"
+
+
+def test_native_driver_statistics(tmp_path):
+    """With getafterstepstats, out/stats.dat holds time, mass, energy and
+    composite mesh size per step (TakeTimeSteps :690-697, format
+    3(f25.12),1x,i8). The time and mesh size must be the reference
+    program's; mass and energy are reduced on the device in another order
+    than the reference's sequential sums, so they agree to 1e-12 of their
+    magnitude (the printed 12 decimals can differ in the last place)."""
+    if not os.path.exists(bp.REF_PROGRAM):
+        pytest.skip("reference program not built (needs the reference build)")
+    from bsam_amd.driver import Driver
+    deck = variant("shipped", 4)
+    deck["rundata"]["getafterstepstats"] = True
+    bp.run(bp.REF_PROGRAM, deck, str(tmp_path / "ref"))
+    work = str(tmp_path / "native")
+    bp.write_deck(work, deck)
+    drv = Driver(work, device=0)
+    drv.run()
+    drv.close()
+    rows = []
+    for d in ("ref", "native"):
+        with open(str(tmp_path / d / "out" / "stats.dat")) as fh:
+            rows.append([l.split() for l in fh if l.strip()])
+    assert len(rows[0]) == len(rows[1]) == 4
+    for a, b in zip(*rows):
+        assert a[0] == b[0] and a[3] == b[3], (a, b)
+        for x, y in zip(a[1:3], b[1:3]):
+            assert abs(float(x) - float(y)) <= 1e-12 * max(1.0, abs(float(x))) + 1.1e-12, (a, b)
