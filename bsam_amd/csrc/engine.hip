@@ -120,6 +120,7 @@ struct bsam_ctx {
   hipStream_t stream = nullptr;
   bool building = false, committed = false;
   bool had_commit = false;    // hp/dp/lev/device state describe a committed hierarchy
+  bool planned_fuse = false;  // that commit built the fusion and riding plans
   long long commits_reused = 0;
   std::vector<HPatch> staged;  // patches added since bsam_hierarchy_begin
   std::vector<HPatch> hp;
@@ -584,7 +585,9 @@ static void commit(bsam_ctx *c) {
       if (build_ghost_program(c->hp, L.patches, nd, l, ip, per, L.gp[ip], msg))
         throw Error("ghost program level " + std::to_string(l) + ": " + msg);
     }
-    for (int f = 0; f < 3; f++) {
+    // the fusion and riding plans serve the V-cycle only; a context that
+    // does not fuse (fuse_ghost 0) goes without them
+    for (int f = 0; f < 3 && c->fuse_ghost; f++) {
       std::string why;
       L.fused[f] = plan_fused_ghost(c->hp, L.t_int, nd, c->tx(), c->ty(), c->tz(),
                                     L.gp[kFusedPass[f]], L.gp[kFusedColor[f]], kFusedColor[f],
@@ -606,10 +609,10 @@ static void commit(bsam_ctx *c) {
       const LevelData &PL = c->lev[l - 1 - minl];
       // only the reads of cells the parent's program writes matter to a ride
       std::vector<int> at;
-      if (destination_index(PL.gp[0], at))
+      if (c->fuse_ghost && destination_index(PL.gp[0], at))
         L.ride_cgl = plan_ride(PL.gp[0], coarse_ghost_load_reads(c->hp, L.t_cext, nd, c->tx(),
                                                                  c->ty(), c->tz(), &at));
-      if (destination_index(PL.gp[2], at))
+      if (c->fuse_ghost && destination_index(PL.gp[2], at))
         L.ride_corr = plan_ride(PL.gp[2], correct_reads(c->hp, L.t_ext, nd, c->tx(), c->ty(),
                                                         c->tz(), &at));
     }
@@ -1134,7 +1137,8 @@ int bsam_hierarchy_commit(bsam_ctx *c) {
   API_BEGIN(c)
   if (!c->building) throw Error("call bsam_hierarchy_begin first");
   if (!c->host_only) HIPCK(hipSetDevice(c->cfg.device));
-  if (c->had_commit && same_geometry(c->hp, c->staged)) {
+  if (c->had_commit && c->planned_fuse == (c->fuse_ghost != 0) &&
+      same_geometry(c->hp, c->staged)) {
     // The regrid produced the hierarchy already committed (the reference
     // regrids every step, mostly to the same patches): keep its plans,
     // tables, pools and graphs, and zero the pools as a fresh commit does.
@@ -1149,6 +1153,7 @@ int bsam_hierarchy_commit(bsam_ctx *c) {
     c->lev.clear();
     commit(c);
     c->had_commit = true;
+    c->planned_fuse = c->fuse_ghost != 0;
   }
   c->staged.clear();
   c->building = false;

@@ -17,9 +17,11 @@ RootInit :216-254, SetProb problem.f90:12-37) and runs TakeTimeSteps
 * the after-step statistics (out/stats.dat) if asked for, and the frames
   out/mNNNNN.dat (frame 0 after the first regrid).
 
-The hierarchy under construction lives in a spare engine context; three
-contexts serve any depth: the hierarchy before the step (the old grids) and
-two that alternate as the new one grows level by level.
+The hierarchy under construction lives in spare engine contexts: two solver
+contexts alternate as the hierarchy of the step (the one before the step
+keeps the old grids), and two lighter ones, which never run a V-cycle and so
+skip the V-cycle's fusion plans at commit, alternate for the intermediate
+hierarchies of a deeper regrid.
 """
 
 from __future__ import annotations
@@ -130,14 +132,18 @@ class Driver:
                        "fields_s": 0.0, "solve_s": 0.0, "frames_s": 0.0, "steps": 0}
 
     # -- engines --------------------------------------------------------------
+    SOLVERS = (0, 1)     # contexts that hold a step's hierarchy and solve
+    SCRATCH = (2, 3)     # contexts for intermediate hierarchies only
+
     def _engine(self, k: int, layout: Layout) -> Engine:
         while len(self._engines) <= k:
             self._engines.append(None)
         if self._engines[k] is None:
+            tune = None if k in self.SOLVERS else {"fuse_ghost": 0, "tail_cells": 0, "graph": 0}
             self._engines[k] = Engine(layout, dt=self.dt, eps=self.eps, alpha=self.alpha,
                                       well="reference", nsmoothingpasses=self.nsmooth,
                                       maxvcycles=self.maxvcycles, qerrortol=self.qerrortol,
-                                      device=self.device)
+                                      device=self.device, tune=tune)
         else:
             self._engines[k].describe(layout)
         return self._engines[k]
@@ -178,7 +184,7 @@ class Driver:
     def _amr(self, old: int, initial: bool) -> int:
         """AMR(rootlevel) from the hierarchy in engine `old`; returns the
         engine that holds the new hierarchy."""
-        spares = [k for k in range(3) if k != old]
+        final = self.SOLVERS[1] if old == self.SOLVERS[0] else self.SOLVERS[0]
         cur = old
         lay = self._engines[old].layout
         nb = 0   # which spare builds next
@@ -208,7 +214,9 @@ class Driver:
                 nlay, idmap = self._truncated(lay, level)
                 for k, mb in new:
                     nlay.add_child(idmap[tagged[k][0].id], mb[:self.ndims])
-                nxt = spares[nb % 2]
+                # the last level's hierarchy goes straight to the solver context
+                last = bool(new) and level + 1 >= self.maxlevel
+                nxt = final if last else self.SCRATCH[nb % 2]
                 nb += 1
                 t0 = time.perf_counter()
                 neng = self._engine(nxt, nlay)
@@ -231,6 +239,14 @@ class Driver:
                     self._engines[cur].set_ghost(level, 0)
                 break
             level += 1
+        if cur in self.SCRATCH:
+            # the hierarchy ended early: hand it to the solver context
+            lay = self._engines[cur].layout
+            t0 = time.perf_counter()
+            neng = self._engine(final, lay)
+            self.timing["commit_s"] += time.perf_counter() - t0
+            neng.copy_levels_from(self._engines[cur], lay.finestlevel)
+            cur = final
         return cur
 
     def _truncated(self, lay: Layout, top: int):

@@ -136,6 +136,9 @@ def lib():
         L.bsam_download_flags.argtypes = [vp, C.c_int, ip, C.c_long]
         L.bsam_error_flags_user.argtypes = [vp, C.c_int, C.c_double]
         L.bsam_copy_levels.argtypes = [vp, vp, C.c_int]
+        L.bsam_format_rows.restype = C.c_long
+        L.bsam_format_rows.argtypes = [C.POINTER(C.c_double), C.c_long, C.c_int, C.c_long,
+                                       C.c_long, C.c_int, C.c_char_p, C.c_long]
         L.bsam_init_fields.argtypes = [vp, C.c_int]
         L.bsam_transfer_values.argtypes = [vp, vp, C.c_int]
         L.bsam_regrid_level.argtypes = [C.POINTER(RegridParams), C.c_int, ip, ip,

@@ -22,6 +22,7 @@ the values by whitespace, so it reads both the 2D (``es20.12``) and the 3D
 
 from __future__ import annotations
 
+import ctypes as C
 import math
 from dataclasses import dataclass
 from typing import Dict, List, Sequence, Tuple
@@ -63,6 +64,26 @@ def _es20(x: float) -> str:
     return f"{body:>20s}"
 
 
+def format_rows(vals: np.ndarray, kind: int) -> str:
+    """Rows of values (a 2D array, one row a line) in the frame files'
+    number format: es20.12 (kind 0) or f25.12 (kind 1), two blanks apart;
+    formatted by the library (bsam_format_rows)."""
+    from .engine import lib
+    a = np.asarray(vals, dtype=np.float64)
+    if a.ndim != 2:
+        raise ValueError("rows of values expected")
+    nrows, ncols = a.shape
+    L = lib()
+    w = 20 if kind == 0 else 25
+    cap = nrows * (ncols * (w + 2) - 2 + 1) + 1
+    buf = C.create_string_buffer(cap)
+    n = L.bsam_format_rows(a.ctypes.data_as(C.POINTER(C.c_double)), nrows, ncols,
+                           a.strides[0] // 8, a.strides[1] // 8, kind, buf, cap)
+    if n < 0:
+        raise RuntimeError("bsam_format_rows failed")
+    return buf.raw[:n].decode()
+
+
 def write_frame(path: str, time: float, finestlevel: int,
                 patches: Sequence[FramePatch]) -> None:
     """Writes a frame as WriteQ does; ``patches`` in frame order."""
@@ -77,10 +98,7 @@ def write_frame(path: str, time: float, finestlevel: int,
         out.append(" ".join(f"{v:8d}" for d in range(nd) for v in p.mglobal[d]))
         q = np.asarray(p.q, dtype=np.float64)
         flat = q.reshape(-1, p.nrvars, order="F")
-        if nd == 2:
-            out.extend("  ".join(_es20(v) for v in row) for row in flat.tolist())
-        else:
-            out.extend("  ".join(_f25(v) for v in row) for row in flat.tolist())
+        out.append(format_rows(flat, 0 if nd == 2 else 1).rstrip("\n"))
     with open(path, "w") as fh:
         fh.write("\n".join(out) + "\n")
 

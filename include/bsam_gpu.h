@@ -201,6 +201,16 @@ int bsam_copy_levels(bsam_ctx *dst, const bsam_ctx *src, int top);
 int bsam_init_fields(bsam_ctx *ctx, int level);
 int bsam_transfer_values(bsam_ctx *dst, const bsam_ctx *src, int level);
 
+/* The frame files' number formats (WriteQ2D / WriteQ3D,
+ * lib/bsaminputoutput.f90:138-168): nrows rows of ncols values
+ * (vals[r*rowstride + c*colstride]) as text, es20.12 (kind 0, 2D frames; a
+ * three-digit exponent drops the letter E as Fortran does) or f25.12 (kind
+ * 1, 3D frames), two blanks between values, a newline after each row. Returns
+ * the number of characters written to out (no terminating zero), or -1 if
+ * cap is too small. No device work. */
+long bsam_format_rows(const double *vals, long nrows, int ncols, long rowstride,
+                      long colstride, int kind, char *out, long cap);
+
 typedef struct bsam_regrid_params {
   int ndims, level;
   int ibuffer;               /* ibuffer(level) */
@@ -229,7 +239,9 @@ int bsam_regrid_level(const bsam_regrid_params *rp, int npatch, const int *mx,
  *   "fuse_ghost"   1 (default): apply each SetGhost inside another level
  *                  launch where the level allows (the following relaxation,
  *                  the level's restriction or fill-down, the child level's
- *                  coarse loading or correction); 0: separately.
+ *                  coarse loading or correction); 0: separately, and a
+ *                  hierarchy committed while it is 0 skips building those
+ *                  plans (contexts that never run a V-cycle).
  *   "tail_lds"     bytes of LDS the tail kernel may hold its levels' arrays
  *                  in (-1, default: the device limit; 0: none); before the
  *                  hierarchy. */

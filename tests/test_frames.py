@@ -61,6 +61,11 @@ def test_number_formats_match_flang(tmp_path):
     got = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout
     want = [frames._es20(v) for v in VALUES] + [frames._f25(v) for v in VALUES if abs(v) < 1e12]
     assert got.splitlines() == want
+    # the library's formatter, which writes the frames' values
+    lib_rows = (frames.format_rows(np.array(VALUES)[:, None], 0).splitlines() +
+                frames.format_rows(np.array([v for v in VALUES if abs(v) < 1e12])[:, None],
+                                   1).splitlines())
+    assert lib_rows == want
 
 
 def test_write_read_3d(tmp_path):
