@@ -1386,6 +1386,8 @@ int bsam_copy_levels(bsam_ctx *dst, const bsam_ctx *src, int top) {
   HIPCK(hipStreamSynchronize(src->stream));
   HIPCK(hipMemcpyAsync(dst->q, src->q, sizeof(double) * end, hipMemcpyDeviceToDevice,
                        dst->stream));
+  // complete before returning: the source may be re-committed or freed next
+  HIPCK(hipStreamSynchronize(dst->stream));
   API_END(dst)
 }
 

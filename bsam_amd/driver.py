@@ -113,6 +113,8 @@ class Driver:
         self.xlower = [float(v) for v in _arr(gd.get("xlower"), nd, 0.0)]
         self.xupper = [float(v) for v in _arr(gd.get("xupper"), nd, 1.0)]
         self.mthbc = [int(v) for v in _arr(gd.get("mthbc"), 2 * nd, 1)]
+        if any(float(v) != 0.0 for v in _arr(gd.get("qpo"), 2, 0.0)):
+            raise NotImplementedError("quasi-periodic offsets (qpo) other than 0")
         if int(gd.get("mbc", 1)) != 1 or int(gd.get("maux", 0)) != 0 or int(gd.get("nrvars", 1)) != 2:
             raise NotImplementedError("mbc = 1, maux = 0 and nrvars = 2 only")
         self.alpha = float(pd.get("alpha", 1.0))

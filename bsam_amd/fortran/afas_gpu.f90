@@ -63,6 +63,12 @@ contains
     use ProblemDef
     integer, intent(in) :: device, precision
     type(bsam_cfg) :: cfg
+    if (any(qpo(1:2) /= 0.0)) then
+      ! PeriodicBC's quasi-periodic offset (boundary.f90:432-461) is not
+      ! part of the engine's ghost programs
+      print *, 'AFASGPU: quasi-periodic offsets (qpo) other than 0 are not supported'
+      stop 1
+    end if
     cfg%ndims = ndims
     cfg%nrvars = 2
     cfg%mbc = 1

@@ -196,7 +196,9 @@ int bsam_sync(bsam_ctx *ctx);
  *   bsam_transfer_values TransferValues / TransferOverlap (:1774-1867): every
  *                        new patch of `level` takes the q of the interior cells
  *                        it shares with src's patches of that level (the old
- *                        grids). */
+ *                        grids).
+ * bsam_copy_levels and bsam_transfer_values wait for src's stream first and
+ * return when their copies are done, so src may change afterwards. */
 int bsam_copy_levels(bsam_ctx *dst, const bsam_ctx *src, int top);
 int bsam_init_fields(bsam_ctx *ctx, int level);
 int bsam_transfer_values(bsam_ctx *dst, const bsam_ctx *src, int level);
