@@ -63,6 +63,8 @@ def regrid_level(ndims: int, level: int, ibuffer: int, minimumgridpoints: int,
     (int32 arrays of shape mx, Fortran order; buffered in place). Returns
     [(parent index, mbounds)] in creation order."""
     L = lib()
+    if ndims not in (2, 3):
+        raise BsamError("regrid: ndims must be 2 or 3")
     rp = RegridParams(ndims, level, ibuffer, minimumgridpoints, desiredfillratio,
                       (C.c_int * 3)(*[int(v) for v in list(root_mx)[:3]] + [1] * (3 - len(root_mx))),
                       (C.c_int * 6)(*[int(v) for v in list(root_mthbc)[:6]] +

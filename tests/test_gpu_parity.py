@@ -305,3 +305,22 @@ def test_after_step_stats(name):
         else:
             assert e == 0.0 and eo == 0.0
     eng.close()
+
+
+def test_regrid_entry_points_report_errors():
+    """The regrid's field entry points refuse what does not fit: levels
+    whose patches differ, a level without a parent, contexts of another
+    shape."""
+    from bsam_amd.engine import BsamError
+    a = Engine(layout_amr2d(), device=0, **KW)
+    b = Engine(layout_single(32), device=0, **KW)
+    with pytest.raises(BsamError):
+        a.copy_levels_from(b, 1)            # level 1 differs
+    with pytest.raises(BsamError):
+        a.init_fields(0)                    # the root has no parent grid
+    c = Engine(layout_single(8, ndims=3), device=0, **KW)
+    with pytest.raises(BsamError):
+        a.transfer_values_from(c, 0)        # 2D from 3D
+    a.copy_levels_from(Engine(layout_amr2d(), device=0, **KW), 0)
+    for e in (a, b, c):
+        e.close()

@@ -142,3 +142,15 @@ def test_initial_hierarchy_matches_reference(tmp_path):
     want = [(p.level, p.mx, p.mglobal) for p in ps]
     assert lay.finestlevel == finest
     assert got == want
+
+
+def test_regrid_rejects_bad_parameters():
+    from bsam_amd.engine import BsamError
+    f = np.zeros((8, 8, 1), dtype=np.int32, order="F")
+    with pytest.raises(BsamError):
+        regrid_level(2, 0, 1, 0, 0.9, (8, 8, 1), (1, 1, 1, 1), [((8, 8, 1), ((1, 8), (1, 8), (1, 1)))], [f])
+    with pytest.raises(BsamError):
+        regrid_level(4, 0, 1, 2, 0.9, (8, 8, 1), (1, 1, 1, 1), [((8, 8, 1), ((1, 8), (1, 8), (1, 1)))], [f])
+    f[:] = 1
+    assert regrid_level(2, 0, 1, 2, 0.9, (8, 8, 1), (1, 1, 1, 1),
+                        [((8, 8, 1), ((1, 8), (1, 8), (1, 1)))], [f]) == [(0, ((1, 8), (1, 8), (1, 1)))]
