@@ -4,8 +4,9 @@ Initial conditions use a counter-based SplitMix64 generator keyed by
 (seed, level, global cell index), so every patch of every level draws the
 same values however the hierarchy is cut into patches. Hierarchies are built
 on the host the way a regrid would: tag cells, cover the tags with aligned
-blocks, create child patches (the reference's own regrid, Berger-Rigoutsos,
-is host work outside the solver and is not reproduced here).
+blocks, create child patches. These benchmark hierarchies stay fixed over
+their steps; the reference's own regrid (Berger-Rigoutsos every step) is
+bsam_regrid_level with bsam_amd/driver.py.
 """
 
 from __future__ import annotations
