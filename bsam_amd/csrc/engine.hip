@@ -143,6 +143,7 @@ struct bsam_ctx {
   double *d_hist = nullptr;
   int *d_state = nullptr;
   int *h_state = nullptr;
+  int last_status = -1;      // of the last solve: BSAM_SOLVE_* (bsam_solve_status)
   // kernel timing with HIP events on the context's stream (bsam_profile_*)
   unsigned prof_mask = 0;
   std::vector<hipEvent_t> ev_pool;
@@ -929,6 +930,7 @@ static int iterate(bsam_ctx *c, hipGraph_t *g, hipGraphExec_t *exec, F body, int
   bool graph = c->use_graph && c->prof_mask == 0;
   if (graph) capture(c, g, exec, body);
   int it = 0, done = 0;
+  c->h_state[1] = c->h_state[2] = 0;
   for (it = 1; it <= c->cfg.maxvcycles; it++) {
     if (graph) HIPCK(hipGraphLaunch(*exec, c->stream));
     else body();
@@ -939,10 +941,16 @@ static int iterate(bsam_ctx *c, hipGraph_t *g, hipGraphExec_t *exec, F body, int
       done = 1;
       break;
     }
+    // a residual that is not finite ends the solve: every later one is too
+    if (c->h_state[2]) break;
   }
+  c->last_status = done ? BSAM_SOLVE_CONVERGED
+                        : (c->h_state[2] ? BSAM_SOLVE_NONFINITE : BSAM_SOLVE_MAXCYCLES);
   *done_out = done;
-  return done ? it : c->cfg.maxvcycles;
+  return done || c->h_state[2] ? std::min(it, c->cfg.maxvcycles) : c->cfg.maxvcycles;
 }
+
+int bsam_solve_status(const bsam_ctx *c) { return c ? c->last_status : -1; }
 
 static void report(bsam_ctx *c, int n, int done, int *ncycles, double *hist, int maxhist,
                    int *converged) {

@@ -1482,7 +1482,8 @@ __global__ __launch_bounds__(256) void k_resnorm(Dev d, const Tile *tiles, int n
 
 // Sum the per-tile partials in index order (strided, then a fixed tree) and
 // finish ErrorAFAS: sqrt(sum / count) into hist[state[0]]; state[0] counts
-// the calls and state[1] is set once a residual falls below tol.
+// the calls, state[1] is set once a residual falls below tol and state[2]
+// once a residual is not finite.
 __global__ __launch_bounds__(256) void k_finalize(const double *partial, int n, double count,
                                                    double tol, double *hist, int *state) {
   double s = 0.0;
@@ -1494,6 +1495,7 @@ __global__ __launch_bounds__(256) void k_finalize(const double *partial, int n, 
     hist[it] = r;
     state[0] = it + 1;
     if (r < tol) state[1] = 1;
+    if (!isfinite(r)) state[2] = 1;
   }
 }
 

@@ -138,6 +138,7 @@ def lib():
         L.bsam_download_flags.argtypes = [vp, C.c_int, ip, C.c_long]
         L.bsam_error_flags_user.argtypes = [vp, C.c_int, C.c_double]
         L.bsam_copy_levels.argtypes = [vp, vp, C.c_int]
+        L.bsam_solve_status.argtypes = [vp]
         L.bsam_format_rows.restype = C.c_long
         L.bsam_format_rows.argtypes = [C.POINTER(C.c_double), C.c_long, C.c_int, C.c_long,
                                        C.c_long, C.c_int, C.c_char_p, C.c_long]
@@ -291,6 +292,12 @@ class Engine:
     def transfer_values_from(self, src: "Engine", level: int):
         """TransferValues: q of the cells `level`'s patches share with src's."""
         self._ck(lib().bsam_transfer_values(self.ctx, src.ctx, level))
+
+    SOLVE_STATUS = {0: "converged", 1: "maxcycles", 2: "nonfinite", -1: None}
+
+    def solve_status(self) -> Optional[str]:
+        """How the last solve ended: converged, maxcycles or nonfinite."""
+        return self.SOLVE_STATUS[int(lib().bsam_solve_status(self.ctx))]
 
     def error_flags_user(self, level: int, qtolerance: float):
         self._ck(lib().bsam_error_flags_user(self.ctx, level, qtolerance))

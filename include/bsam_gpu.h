@@ -169,6 +169,14 @@ int bsam_time_step(bsam_ctx *ctx, int *ncycles, double *hist, int maxhist,
                    int *converged);
 /* Wait for all work queued on the context's stream. */
 int bsam_sync(bsam_ctx *ctx);
+/* How the last solve (bsam_multigrid_iterations / bsam_time_step) ended:
+ * converged below qerrortol, stopped at maxvcycles, or stopped at the first
+ * residual that is not finite (NaN or Inf; the reference iterates on to
+ * maxvcycles there, lib/afasroutines.f90:69-104, with every later residual
+ * also non-finite); -1 before any solve. The reference reports nothing: its
+ * lconverged is set and never read. No counterpart in the reference. */
+enum { BSAM_SOLVE_CONVERGED = 0, BSAM_SOLVE_MAXCYCLES = 1, BSAM_SOLVE_NONFINITE = 2 };
+int bsam_solve_status(const bsam_ctx *ctx);
 
 /* ---- regrid: host-side clustering of a level's flags -------------------
  * The integer core of the regrid of one level, on the host: ErrFlag's

@@ -324,3 +324,33 @@ def test_regrid_entry_points_report_errors():
     a.copy_levels_from(Engine(layout_amr2d(), device=0, **KW), 0)
     for e in (a, b, c):
         e.close()
+
+
+def test_solve_status():
+    """A solve reports how it ended: converged, stopped at maxvcycles (a
+    tolerance it cannot reach), or stopped at the first non-finite residual
+    (a NaN in the initial state) instead of iterating on."""
+    lay = layout_amr2d()
+    q0 = noise_q(lay, seed=71)
+    kw = dict(KW)
+    eng = Engine(lay, device=0, **kw)
+    for n, p in enumerate(eng.patches):
+        eng.upload(n, "q", q0[p.id])
+    hist, conv = eng.time_step()
+    assert conv and eng.solve_status() == "converged"
+    kw.update(maxvcycles=2, qerrortol=1e-30)
+    eng2 = Engine(lay, device=0, **kw)
+    for n, p in enumerate(eng2.patches):
+        eng2.upload(n, "q", q0[p.id])
+    hist, conv = eng2.time_step()
+    assert not conv and len(hist) == 2 and eng2.solve_status() == "maxcycles"
+    bad = q0[lay.patches[0].id].copy()
+    bad[3, 3, 0, 0] = np.nan
+    eng3 = Engine(lay, device=0, **KW)
+    for n, p in enumerate(eng3.patches):
+        eng3.upload(n, "q", bad if p.id == 0 else q0[p.id])
+    hist, conv = eng3.time_step()
+    assert not conv and len(hist) == 1 and not np.isfinite(hist[0])
+    assert eng3.solve_status() == "nonfinite"
+    for e in (eng, eng2, eng3):
+        e.close()
