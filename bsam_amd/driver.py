@@ -95,8 +95,10 @@ class Driver:
         self.timeiterations = int(rd.get("timeiterations", 1))
         self.stats = bool(rd.get("getafterstepstats", False))
         self.eqfn = int(rd.get("eqfn", 6))
-        if rd.get("restart", False) or rd.get("syncelliptic", False):
-            raise NotImplementedError("restart and syncelliptic runs are not supported")
+        self.restart = bool(rd.get("restart", False))
+        self.restartframe = int(rd.get("restartframe", 0))
+        if rd.get("syncelliptic", False):
+            raise NotImplementedError("syncelliptic runs are not supported")
         if int(rd.get("errortype", 1)) != 1 or int(rd.get("updateauxfreq", 1)) != 1:
             raise NotImplementedError("errortype 1 and updateauxfreq 1 only")
         # RootInit's defaults (:237-254)
@@ -286,34 +288,71 @@ class Driver:
         return total
 
     # -- TakeTimeSteps ----------------------------------------------------------
+    def _read_restart(self) -> float:
+        """ReadQ (lib/bsaminputoutput.f90:174-398) into solver context 0: the
+        frame's root (checked against the deck) and its refined patches,
+        which serve the first regrid as the old grids. Returns the time."""
+        path = os.path.join(self.workdir, "out", f"m{self.restartframe:05d}.dat")
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"restart frame {path} does not exist")
+        t, finest, ps = frames.read_frame(path)
+        root = ps[0]
+        small = 1.0e-08
+        if (root.level != 0 or root.ndims != self.ndims or root.nrvars != 2 or
+                tuple(root.mx) != tuple(self.mx[:self.ndims]) or
+                any(root.dx[d] - self.root_layout.patches[0].dx > small for d in range(self.ndims)) or
+                any(abs(root.xlower[d] - self.xlower[d]) > small for d in range(self.ndims)) or
+                any(abs(root.xupper[d] - self.xupper[d]) > small for d in range(self.ndims))):
+            raise ValueError("restart frame does not match the deck's root grid")
+        if finest > self.maxlevel:
+            raise ValueError("restart frame is finer than maxlevel")
+        lay, ids = frames.layout_from_frame(ps, self.mthbc, minlevel=self.minlevel)
+        eng = self._engine(0, lay)
+        pos = {p.id: n for n, p in enumerate(eng.patches)}
+        for i, fp in enumerate(ps):
+            q = np.asfortranarray(fp.q.reshape(lay.field_shape(lay.patches[ids[i]], "q"), order="F"))
+            eng.upload(pos[ids[i]], "q", q)
+            if fp.level == 0:
+                eng.upload(pos[ids[i]], "qold", q)   # CopyQToQold on the root
+        return t
+
     def run(self):
         os.makedirs(os.path.join(self.workdir, "out"), exist_ok=True)
         cur = 0
-        eng = self._engine(0, self.root_layout)
-        self._upload_qinit(eng, 0)           # Initialq on the root
+        if self.restart:
+            if self.restartframe >= self.outframes:
+                raise ValueError("outframes must be greater than restartframe")
+            starttime = self._read_restart()
+            firstframe = self.restartframe + 1
+            eng = self._engines[0]
+        else:
+            eng = self._engine(0, self.root_layout)
+            self._upload_qinit(eng, 0)           # Initialq on the root
+            starttime, firstframe = 0.0, 1
         eng.set_ghost(0, 0)
         itperprint = self.timeiterations // self.outframes
-        currenttime = 0.0
-        first = True
-        for n in range(1, self.outframes + 1):
+        currenttime = starttime
+        first = True                             # TakeTimeSteps' firstamr
+        initial = not self.restart               # initial grids take QInit
+        for n in range(firstframe, self.outframes + 1):
             for it in range(1, itperprint + 1):
                 t0 = time.perf_counter()
-                cur = self._amr(cur, initial=first)
+                cur = self._amr(cur, initial=initial)
                 eng = self._engines[cur]
                 if first:
                     eng.fill_down(FIELD_Q)
                 eng.copy_qold()
                 self.timing["regrid_s"] += time.perf_counter() - t0
-                if first:
+                if first and not self.restart:
                     self._write_frame(eng, 0, currenttime)
-                first = False
+                first = initial = False
                 t0 = time.perf_counter()
                 for _ in range(self.eqfn):
                     eng.multigrid_iterations()
                 eng.sync()
                 self.timing["solve_s"] += time.perf_counter() - t0
                 self.timing["steps"] += 1
-                currenttime = real_time = float((n - 1) * itperprint + it) * self.dt
+                currenttime = real_time = starttime + float((n - firstframe) * itperprint + it) * self.dt
                 if self.stats:
                     mass, energy = eng.after_step_stats()
                     with open(os.path.join(self.workdir, "out", "stats.dat"), "a") as fh:

@@ -13,6 +13,7 @@ with its ghost layer, as printed.
 import os
 import time
 
+import numpy as np
 import pytest
 
 import bsam_program as bp
@@ -116,3 +117,67 @@ def test_native_driver_statistics(tmp_path):
         assert a[0] == b[0] and a[3] == b[3], (a, b)
         for x, y in zip(a[1:3], b[1:3]):
             assert abs(float(x) - float(y)) <= 1e-12 * max(1.0, abs(float(x))) + 1.1e-12, (a, b)
+
+
+def test_native_driver_restart_3d(tmp_path):
+    """Restart from a frame (restart = .true., restartframe = 1): ReadQ's
+    hierarchy serves the first regrid as the old grids. In 3D the reference
+    program's restart works (its 2D reader does not match its 2D writer), so
+    the frames written after the restart must equal the reference program's
+    restart byte for byte."""
+    if not os.path.exists(bp.REF_PROGRAM):
+        pytest.skip("reference program not built (needs the reference build)")
+    from bsam_amd.driver import Driver
+    deck = variant("3d_l1", 3)
+    ref, _, _ = bp.run(bp.REF_PROGRAM, deck, str(tmp_path / "first"))
+    deck["rundata"].update(restart=True, restartframe=1)
+    outs = {}
+    for who in ("ref", "native"):
+        work = tmp_path / who
+        bp.write_deck(str(work), deck)
+        (work / "out" / "m00001.dat").write_bytes(ref[1])
+        if who == "ref":
+            frames_, _, _ = bp.run(bp.REF_PROGRAM, deck, str(work))
+        else:
+            drv = Driver(str(work), device=0)
+            drv.run()
+            drv.close()
+            frames_ = {int(f[1:6]): (work / "out" / f).read_bytes()
+                       for f in os.listdir(work / "out") if f.endswith(".dat")}
+        outs[who] = frames_
+    assert sorted(outs["ref"]) == sorted(outs["native"]) == [1, 2, 3]
+    for n in (2, 3):
+        assert outs["native"][n] == outs["ref"][n], \
+            f"frame {n}: {first_difference(outs['native'][n], outs['ref'][n])}"
+
+
+def test_native_driver_restart_2d(tmp_path):
+    """The 2D restart: the reference program cannot read its own 2D frames
+    back (ReadQ2D reads f25.12 where WriteQ2D wrote es20.12); the driver's
+    reader can. Restarted from frame 1 of the shipped deck, the run
+    continues on the same hierarchies as the run without a restart, with q
+    equal up to the 13 digits the frame keeps."""
+    if not os.path.exists(bp.REF_PROGRAM):
+        pytest.skip("reference program not built (needs the reference build)")
+    from bsam_amd import frames
+    from bsam_amd.driver import Driver
+    deck = variant("shipped", 3)
+    whole = tmp_path / "whole"
+    bp.write_deck(str(whole), deck)
+    drv = Driver(str(whole), device=0)
+    drv.run()
+    drv.close()
+    deck["rundata"].update(restart=True, restartframe=1)
+    rs = tmp_path / "restart"
+    bp.write_deck(str(rs), deck)
+    (rs / "out" / "m00001.dat").write_bytes((whole / "out" / "m00001.dat").read_bytes())
+    drv = Driver(str(rs), device=0)
+    drv.run()
+    drv.close()
+    for n in (2, 3):
+        ta, fa, pa = frames.read_frame(str(whole / "out" / f"m{n:05d}.dat"))
+        tb, fb, pb = frames.read_frame(str(rs / "out" / f"m{n:05d}.dat"))
+        assert ta == tb and fa == fb
+        assert [(p.level, p.mx, p.mglobal) for p in pa] == [(p.level, p.mx, p.mglobal) for p in pb]
+        worst = max(float(np.abs(a.q - b.q).max()) for a, b in zip(pa, pb))
+        assert worst < 1e-8, (n, worst)
