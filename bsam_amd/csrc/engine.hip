@@ -1413,35 +1413,42 @@ int bsam_init_fields(bsam_ctx *c, int level) {
   API_END(c)
 }
 
-int bsam_transfer_values(bsam_ctx *dst, const bsam_ctx *src, int level) {
-  API_BEGIN(dst)
+// q of every (dst, src) patch pair of `level` where their boxes overlap in
+// the level's global index space: the interiors (ghosts false), or the
+// ghost-extended boxes (ghosts true). Boxes are listed dst-major, src in
+// level-list order; launches are split wherever a box would write a cell an
+// earlier box of the same launch writes, so a later src patch wins where
+// ghost-extended boxes overlap, as copies made one patch after another do.
+static void copy_level_boxes(bsam_ctx *dst, const bsam_ctx *src, int level, bool ghosts,
+                             const char *what) {
   dst->need_device();
-  if (!src || src->host_only || !src->committed) throw Error("transfer: bad source context");
-  if (src->cfg.device != dst->cfg.device) throw Error("transfer: contexts on different devices");
+  if (!src || src->host_only || !src->committed) throw Error(std::string(what) + ": bad source context");
+  if (src->cfg.device != dst->cfg.device)
+    throw Error(std::string(what) + ": contexts on different devices");
   if (src->cfg.ndims != dst->cfg.ndims || src->cfg.nrvars != dst->cfg.nrvars)
-    throw Error("transfer: contexts differ in ndims or nrvars");
-  if (level < dst->cfg.minlevel || level > dst->finest) throw Error("transfer: bad level");
-  if (level > src->finest) return 0;   // no old grids on that level
-  const int nd = dst->nd(), nv = dst->cfg.nrvars;
+    throw Error(std::string(what) + ": contexts differ in ndims or nrvars");
+  if (level < dst->cfg.minlevel || level > dst->finest) throw Error(std::string(what) + ": bad level");
+  if (level < src->cfg.minlevel || level > src->finest) return;   // src has no patches there
+  const int nd = dst->nd(), nv = dst->cfg.nrvars, g = ghosts ? 1 : 0;   // ghost layers
   std::vector<BoxCopy> boxes;
-  // every (new, old) pair of the level: the overlap of their interiors in
-  // the level's global index space, interior cells of all variables
+  struct Placed { int patch, lo[3], hi[3]; };
+  std::vector<Placed> placed;
   for (int a : dst->L(level).patches)
     for (int b : src->lev[level - src->cfg.minlevel].patches) {
       const HPatch &t = dst->hp[a], &o = src->hp[b];
-      int lo[3] = {1, 1, 1}, hi[3] = {1, 1, 1};
+      Placed w{a, {1, 1, 1}, {1, 1, 1}};
       bool empty = false;
       for (int d = 0; d < nd; d++) {
-        lo[d] = std::max(t.mg[d][0], o.mg[d][0]);
-        hi[d] = std::min(t.mg[d][1], o.mg[d][1]);
-        empty = empty || hi[d] < lo[d];
+        w.lo[d] = std::max(t.mg[d][0], o.mg[d][0]) - g;
+        w.hi[d] = std::min(t.mg[d][1], o.mg[d][1]) + g;
+        empty = empty || w.hi[d] < w.lo[d];
       }
       if (empty) continue;
       const DPatch &T = dst->dp[a], &O = src->dp[b];
       BoxCopy B;
       auto off = [&](const DPatch &P, const HPatch &h) {
-        long long i = lo[0] - h.mg[0][0] + 1, j = lo[1] - h.mg[1][0] + 1;
-        long long k = nd == 3 ? lo[2] - h.mg[2][0] + 1 : 0;
+        long long i = w.lo[0] - h.mg[0][0] + 1, j = w.lo[1] - h.mg[1][0] + 1;
+        long long k = nd == 3 ? w.lo[2] - h.mg[2][0] + 1 : 0;
         return P.q + i + (long long)P.px * (j + (long long)P.py * k);
       };
       B.s = off(O, o);
@@ -1452,19 +1459,45 @@ int bsam_transfer_values(bsam_ctx *dst, const bsam_ctx *src, int level) {
       B.dsy = T.px;
       B.dsz = (long long)T.px * T.py;
       B.dsv = T.qvs;
-      B.nx = hi[0] - lo[0] + 1;
-      B.ny = hi[1] - lo[1] + 1;
-      B.nz = nd == 3 ? hi[2] - lo[2] + 1 : 1;
+      B.nx = w.hi[0] - w.lo[0] + 1;
+      B.ny = w.hi[1] - w.lo[1] + 1;
+      B.nz = nd == 3 ? w.hi[2] - w.lo[2] + 1 : 1;
       B.nv = nv;
       boxes.push_back(B);
+      placed.push_back(w);
     }
-  if (boxes.empty()) return 0;
+  if (boxes.empty()) return;
+  std::vector<int> cuts{0};
+  for (size_t i = 1; i < placed.size(); i++)
+    for (size_t j = (size_t)cuts.back(); j < i; j++) {
+      bool meet = placed[i].patch == placed[j].patch;
+      for (int d = 0; d < nd && meet; d++)
+        meet = placed[i].lo[d] <= placed[j].hi[d] && placed[j].lo[d] <= placed[i].hi[d];
+      if (meet) {
+        cuts.push_back((int)i);
+        break;
+      }
+    }
+  cuts.push_back((int)boxes.size());
   HIPCK(hipStreamSynchronize(src->stream));
   BoxCopy *d_boxes = dev_upload(boxes, dst->stream);
-  hipError_t e = launch_box_copy(dst->q, src->q, d_boxes, (int)boxes.size(), dst->stream);
+  hipError_t e = hipSuccess;
+  for (size_t c = 0; c + 1 < cuts.size() && e == hipSuccess; c++)
+    e = launch_box_copy(dst->q, src->q, d_boxes + cuts[c], cuts[c + 1] - cuts[c], dst->stream);
   (void)hipStreamSynchronize(dst->stream);
   (void)hipFree(d_boxes);
   HIPCK(e);
+}
+
+int bsam_transfer_values(bsam_ctx *dst, const bsam_ctx *src, int level) {
+  API_BEGIN(dst)
+  copy_level_boxes(dst, src, level, false, "transfer");
+  API_END(dst)
+}
+
+int bsam_overlay_level(bsam_ctx *dst, const bsam_ctx *src, int level) {
+  API_BEGIN(dst)
+  copy_level_boxes(dst, src, level, true, "overlay");
   API_END(dst)
 }
 

@@ -15,7 +15,10 @@ RootInit :216-254, SetProb problem.f90:12-37) and runs TakeTimeSteps
   of the old grids they overlap (TransferValues), all on the GPU;
 * FillDown on the first step, qold = q, MultigridIterations (eqfn times);
 * the after-step statistics (out/stats.dat) if asked for, and the frames
-  out/mNNNNN.dat (frame 0 after the first regrid).
+  out/mNNNNN.dat (frame 0 after the first regrid), each followed by the
+  uniform-mesh output out/uNNNNN.dat if asked for (on the GPU: a context of
+  one domain-wide patch per level, filled by prolongation and the level's
+  patches).
 
 The hierarchy under construction lives in spare engine contexts: two solver
 contexts alternate as the hierarchy of the step (the one before the step
@@ -78,6 +81,40 @@ def _arr(v, n, default, base=1):
     return [v] * n
 
 
+def uniform_layout(layout: Layout):
+    """WriteUniformMeshQ's uniform grids as a hierarchy: one patch over the
+    whole domain on every level 0..finest of `layout` (no levels below the
+    root). Returns it and the id of its finest patch."""
+    nd = layout.ndims
+    root = layout.patches[0]
+    lay = Layout(nd, root.mx[:nd], root.dx, xlower=root.xlower, mthbc=root.mthbc,
+                 minlevel=0, nrvars=layout.nrvars)
+    top = 0
+    for _ in range(layout.finestlevel):
+        top = lay.add_child(top, [(1, lay.patches[top].mx[d]) for d in range(nd)])
+    return lay, top
+
+
+def uniform_mesh(u: Engine, eng: Engine, top: int) -> np.ndarray:
+    """The finest uniform grid of WriteUniformMeshQ (lib/bsaminputoutput.f90:
+    468-498) for `eng`'s hierarchy, built on the GPU in `u` (an engine on
+    uniform_layout(eng.layout)): the root's q with its ghosts; then per level
+    the MC prolongation of the level below over the domain (InitFields'
+    BiLin/TriLinProlongationP1MC, ghosts included) with the level's patches
+    copied over it, ghosts included, in level-list order
+    (CopyPatchToUniformGrid :543-577). Returns q(0:mx+1, ..., 1:nrvars),
+    Fortran order, downloaded."""
+    u.overlay_level_from(eng, 0)
+    for level in range(1, eng.layout.finestlevel + 1):
+        u.init_fields(level)
+        u.overlay_level_from(eng, level)
+    n = next(n for n, p in enumerate(u.patches) if p.id == top)
+    p = u.layout.patches[top]
+    nd = u.layout.ndims
+    return u.download(n, "q").reshape(tuple(p.mx[d] + 2 for d in range(nd)) + (u.layout.nrvars,),
+                                      order="F")
+
+
 class Driver:
     """TakeTimeSteps on the GPU engine, regridding every step."""
 
@@ -97,6 +134,7 @@ class Driver:
         self.eqfn = int(rd.get("eqfn", 6))
         self.restart = bool(rd.get("restart", False))
         self.restartframe = int(rd.get("restartframe", 0))
+        self.uniform = bool(rd.get("outputuniformmesh", False))
         if rd.get("syncelliptic", False):
             raise NotImplementedError("syncelliptic runs are not supported")
         if int(rd.get("errortype", 1)) != 1 or int(rd.get("updateauxfreq", 1)) != 1:
@@ -129,6 +167,7 @@ class Driver:
         self.minlevel = self.root_layout.minlevel
         self.device = device
         self._engines: List[Engine] = []
+        self.engine: Optional[Engine] = None   # the last step's hierarchy, after run()
         # wall time by phase: the regrid (of which the hierarchy hand-overs,
         # the tagging with its download, the clustering and the field work),
         # the solves and the frames
@@ -138,6 +177,7 @@ class Driver:
     # -- engines --------------------------------------------------------------
     SOLVERS = (0, 1)     # contexts that hold a step's hierarchy and solve
     SCRATCH = (2, 3)     # contexts for intermediate hierarchies only
+    UNIFORM = 4          # context of the uniform grids of the uniform-mesh output
 
     def _engine(self, k: int, layout: Layout) -> Engine:
         while len(self._engines) <= k:
@@ -274,6 +314,28 @@ class Driver:
                            frames.frame_from_engine(eng, self.xupper))
         self.timing["frames_s"] += time.perf_counter() - t0
 
+    def _write_uniform(self, eng: Engine, t: float):
+        """WriteUniformMeshQ (lib/bsaminputoutput.f90:438-541) of the current
+        hierarchy, written as a frame of one level-0 patch at the finest
+        level's resolution to out/uNNNNN.dat, numbered by restartframe and
+        not by the frame, as the reference names it: each frame overwrites
+        the previous one."""
+        t0 = time.perf_counter()
+        nd, finest = self.ndims, eng.layout.finestlevel
+        lay, top = uniform_layout(eng.layout)
+        q = uniform_mesh(self._engine(self.UNIFORM, lay), eng, top)
+        mxuf = lay.patches[top].mx[:nd]
+        # dx(1:ndims) = (xupper - xlower) / mx on the root, halved per level
+        dx = [(self.xupper[d] - self.xlower[d]) / float(self.mx[d]) for d in range(nd)]
+        for _ in range(finest):
+            dx = [v / 2 for v in dx]
+        fp = frames.FramePatch(0, nd, 2, tuple(dx), tuple(self.xlower[:nd]),
+                               tuple(self.xupper[:nd]), tuple(mxuf),
+                               tuple((1, m) for m in mxuf), q)
+        frames.write_frame(os.path.join(self.workdir, "out", f"u{self.restartframe:05d}.dat"),
+                           t, 0, [fp])
+        self.timing["frames_s"] += time.perf_counter() - t0
+
     def _mesh_size(self, lay: Layout) -> int:
         """GetMeshSize (:836-858) summed over levels >= 0."""
         total = 0
@@ -345,6 +407,8 @@ class Driver:
                 self.timing["regrid_s"] += time.perf_counter() - t0
                 if first and not self.restart:
                     self._write_frame(eng, 0, currenttime)
+                    if self.uniform:
+                        self._write_uniform(eng, currenttime)
                 first = initial = False
                 t0 = time.perf_counter()
                 for _ in range(self.eqfn):
@@ -359,6 +423,9 @@ class Driver:
                         fh.write(f"{real_time:25.12f}{mass:25.12f}{energy:25.12f} "
                                  f"{self._mesh_size(eng.layout):8d}\n")
             self._write_frame(eng, n, currenttime)
+            if self.uniform:
+                self._write_uniform(eng, currenttime)
+        self.engine = eng
         return self.timing
 
 

@@ -144,6 +144,7 @@ def lib():
                                        C.c_long, C.c_int, C.c_char_p, C.c_long]
         L.bsam_init_fields.argtypes = [vp, C.c_int]
         L.bsam_transfer_values.argtypes = [vp, vp, C.c_int]
+        L.bsam_overlay_level.argtypes = [vp, vp, C.c_int]
         L.bsam_regrid_level.argtypes = [C.POINTER(RegridParams), C.c_int, ip, ip,
                                         C.POINTER(ip), C.c_int, ip, ip, ip]
         L.bsam_after_step_stats.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
@@ -292,6 +293,12 @@ class Engine:
     def transfer_values_from(self, src: "Engine", level: int):
         """TransferValues: q of the cells `level`'s patches share with src's."""
         self._ck(lib().bsam_transfer_values(self.ctx, src.ctx, level))
+
+    def overlay_level_from(self, src: "Engine", level: int):
+        """CopyPatchToUniformGrid over `level`: src's patches, ghosts
+        included, copied into this engine's patches of that level in
+        level-list order."""
+        self._ck(lib().bsam_overlay_level(self.ctx, src.ctx, level))
 
     SOLVE_STATUS = {0: "converged", 1: "maxcycles", 2: "nonfinite", -1: None}
 

@@ -205,11 +205,21 @@ int bsam_solve_status(const bsam_ctx *ctx);
  *                        new patch of `level` takes the q of the interior cells
  *                        it shares with src's patches of that level (the old
  *                        grids).
- * bsam_copy_levels and bsam_transfer_values wait for src's stream first and
- * return when their copies are done, so src may change afterwards. */
+ *   bsam_overlay_level   CopyPatchToUniformGrid (lib/bsaminputoutput.f90:
+ *                        543-577) over a level: every src patch of `level`,
+ *                        in level-list order, copies q over its ghost-extended
+ *                        box (0:mx+1) into the dst patches of that level where
+ *                        they overlap; a later patch wins where two boxes
+ *                        meet. WriteUniformMeshQ (:438-541) is this on a dst
+ *                        whose levels are each one patch over the domain,
+ *                        with bsam_init_fields between levels.
+ * bsam_copy_levels, bsam_transfer_values and bsam_overlay_level wait for src's
+ * stream first and return when their copies are done, so src may change
+ * afterwards. */
 int bsam_copy_levels(bsam_ctx *dst, const bsam_ctx *src, int top);
 int bsam_init_fields(bsam_ctx *ctx, int level);
 int bsam_transfer_values(bsam_ctx *dst, const bsam_ctx *src, int level);
+int bsam_overlay_level(bsam_ctx *dst, const bsam_ctx *src, int level);
 
 /* The frame files' number formats (WriteQ2D / WriteQ3D,
  * lib/bsaminputoutput.f90:138-168): nrows rows of ncols values

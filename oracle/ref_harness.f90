@@ -24,7 +24,9 @@
 ! Ops: setghost L ipass | relax L color | restrict L | coarseghost L |
 !      coarseload L | correct L | filldown field | source 0 0 | vcycle L 0 |
 !      residual L 0 | mgiter 0 0 | step nsteps 0 | copyqold 0 0 | truncerr L 0 |
-!      userflags L N (qtolerance(L) = N * 1e-3) | stats 0 0
+!      userflags L N (qtolerance(L) = N * 1e-3) | stats 0 0 |
+!      uniform R 0 (WriteUniformMeshQ at time 0 with restartframe R: it
+!      writes ./out/uRRRRR.dat under the working directory)
 !
 ! Input stream: for every level minlevel..finestlevel in level-list order
 ! (ApplyOnLevel order), the arrays q, f, qc of each patch.
@@ -39,6 +41,7 @@ program RefHarness
   use BSAMStorage, only: AllocFields
   use Boundary, only: SetGhost, PeriodicSetup, GetCoarseGhostPoints
   use AFASRoutines
+  use BSAMInputOutput, only: WriteUniformMeshQ
   use HarnessState
   use HarnessCallbacks
   use HarnessTree
@@ -129,6 +132,9 @@ program RefHarness
       end do
       write(14,'(i6,1x,i6,1x,es25.17)') -1, 1, integralresult(1)
       write(14,'(i6,1x,i6,1x,es25.17)') -1, 2, integralresult(2)
+    case('uniform')
+      restartframe = a1
+      call WriteUniformMeshQ(0, 0.0)
     case('copyqold')
       do lev = minlevel, finestlevel
         call ApplyOnLevel(lev, CopyToOld, dummy)

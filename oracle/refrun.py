@@ -56,11 +56,13 @@ def run(layout, arrays, ops, dt=1e-3, alpha=1.0, eps=0.03, nsmooth=2, maxv=20,
     if not available():
         raise RuntimeError("reference harness not built (make -C oracle ref)")
     tmp = keep or tempfile.mkdtemp(prefix="bsamref_")
+    os.makedirs(tmp, exist_ok=True)
     spec = os.path.join(tmp, "spec.txt")
     fin = os.path.join(tmp, "in.bin")
     fout = os.path.join(tmp, "out.bin")
     fres = os.path.join(tmp, "res.txt")
     write_spec(spec, layout, dt, alpha, eps, nsmooth, maxv, tol, ops)
+    os.makedirs(os.path.join(tmp, "out"), exist_ok=True)   # the 'uniform' op's file
     order = layout.ordered()
     with open(fin, "wb") as fh:
         for p in order:

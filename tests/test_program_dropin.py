@@ -17,6 +17,8 @@ import numpy as np
 import pytest
 
 import bsam_program as bp
+import refrun
+from bsam_amd import frames
 
 pytestmark = pytest.mark.gpu
 
@@ -181,3 +183,68 @@ def test_native_driver_restart_2d(tmp_path):
         assert [(p.level, p.mx, p.mglobal) for p in pa] == [(p.level, p.mx, p.mglobal) for p in pb]
         worst = max(float(np.abs(a.q - b.q).max()) for a, b in zip(pa, pb))
         assert worst < 1e-8, (n, worst)
+
+
+def test_native_driver_uniform_mesh_3d(tmp_path):
+    """With outputuniformmesh every frame is followed by WriteUniformMeshQ
+    (lib/bsaminputoutput.f90:438-541), written to out/u<restartframe>.dat.
+    The reference program built here aborts in the second such call of a
+    run (its unit 54 stays open after the first, and the next call's
+    write-backspace-read fails in the Fortran runtime), so the run compared
+    is a restart from frame 1 that writes one frame: its uniform mesh,
+    out/u00001.dat, must be the reference program's byte for byte."""
+    if not os.path.exists(bp.REF_PROGRAM):
+        pytest.skip("reference program not built (needs the reference build)")
+    from bsam_amd.driver import Driver
+    deck = variant("3d_l1", 2)
+    ref, _, _ = bp.run(bp.REF_PROGRAM, deck, str(tmp_path / "first"))
+    deck["rundata"].update(restart=True, restartframe=1, outputuniformmesh=True,
+                           outframes=2, timeiterations=2)
+    outs = {}
+    for who in ("ref", "native"):
+        work = tmp_path / who
+        bp.write_deck(str(work), deck)
+        (work / "out" / "m00001.dat").write_bytes(ref[1])
+        if who == "ref":
+            bp.run(bp.REF_PROGRAM, deck, str(work))
+        else:
+            drv = Driver(str(work), device=0)
+            drv.run()
+            drv.close()
+        outs[who] = (work / "out" / "u00001.dat").read_bytes()
+    assert outs["native"] == outs["ref"], first_difference(outs["native"], outs["ref"])
+
+
+@pytest.mark.parametrize("name,steps", [("shipped", 3), ("neumann_l3", 2)])
+def test_native_driver_uniform_mesh_2d(tmp_path, name, steps):
+    """The 2D uniform mesh of a whole run (one file per frame, the last one
+    kept): the reference program cannot finish such a run (see the 3D test)
+    nor restart from a 2D frame, so the reference side is WriteUniformMeshQ
+    itself, called by the reference harness on the driver's final hierarchy
+    and arrays: the driver's out/u00000.dat equals the harness's file but
+    for the time in its first line (the harness writes 0)."""
+    if not os.path.exists(bp.REF_PROGRAM) or not refrun.available():
+        pytest.skip("reference program or harness not built (needs the reference build)")
+    from bsam_amd.driver import Driver
+    deck = variant(name, steps)
+    deck["rundata"]["outputuniformmesh"] = True
+    work = tmp_path / "native"
+    bp.write_deck(str(work), deck)
+    drv = Driver(str(work), device=0)
+    drv.run()
+    eng = drv.engine
+    arrays = {}
+    for n, p in enumerate(eng.patches):
+        q = eng.download(n, "q")
+        arrays[p.id] = {"q": q, "f": np.zeros(eng.layout.field_shape(p, "f"), order="F"),
+                        "qc": np.zeros(eng.layout.field_shape(p, "qc"), order="F")}
+    lay = eng.layout
+    drv.close()
+    refrun.run(lay, arrays, [("uniform", 0, 0)], keep=str(tmp_path / "ref"))
+    want = (tmp_path / "ref" / "out" / "u00000.dat").read_bytes().split(b"\n", 1)
+    got = (work / "out" / "u00000.dat").read_bytes().split(b"\n", 1)
+    assert got[1] == want[1], first_difference(got[1], want[1])
+    _, top, _ = frames.read_frame(str(work / "out" / f"m{steps:05d}.dat"))
+    _, _, ps = frames.read_frame(str(work / "out" / "u00000.dat"))
+    assert top == lay.finestlevel
+    assert ps[0].mx == tuple(m * 2 ** top for m in lay.patches[0].mx[:2])
