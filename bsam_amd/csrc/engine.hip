@@ -186,6 +186,7 @@ struct bsam_ctx {
   int ty() const { return 1 << tile_ly; }
   int tz() const { return nd() == 3 ? 256 >> (tile_lx + tile_ly) : 1; }
   int nd() const { return cfg.ndims; }
+  int comps() const { return cfg.errortype == 2 ? 2 : 1; }   // residual norms kept apart
   int ntail() const { return tail_top - cfg.minlevel + 1; }
   LevelData &L(int level) {
     if (level < cfg.minlevel || level > finest) throw Error("level out of range");
@@ -709,7 +710,7 @@ static void commit(bsam_ctx *c) {
   c->d_all_ext = dev_upload(c->all_ext, s);
   c->d_src_int = dev_upload(c->src_int, s);
   c->partial_cap = (int)maxt;
-  HIPCK(hipMalloc(&c->d_partial, sizeof(double) * maxt));
+  HIPCK(hipMalloc(&c->d_partial, sizeof(double) * 2 * maxt));   // two components (errortype 2)
   int hmax = std::max(c->cfg.maxvcycles, 1) + 1;
   HIPCK(hipMalloc(&c->d_hist, sizeof(double) * hmax));
   HIPCK(hipMalloc(&c->d_state, sizeof(int) * 4));
@@ -900,10 +901,13 @@ static void queue_error(bsam_ctx *c, int level, double tol) {
   int nt = (int)L.t_int.size();
   {
     ProfScope ps(c, BSAM_PROF_RESNORM, 32.0 * L.cells);
-    HIPCK(launch_resnorm(c->nd(), c->dev(), L.d_int, nt, c->d_partial, 0, c->stream));
+    HIPCK(launch_resnorm(c->nd(), c->dev(), L.d_int, nt, c->d_partial, 0, c->comps(),
+                         c->stream));
   }
-  double count = (double)(c->cfg.nrvars * L.cells);
-  HIPCK(launch_finalize(c->d_partial, nt, count, tol, c->d_hist, c->d_state, c->stream));
+  // L2Error counts nrvars values per cell, L2ComponentErrors one
+  double count = (double)(c->comps() == 2 ? L.cells : c->cfg.nrvars * L.cells);
+  HIPCK(launch_finalize(c->d_partial, nt, count, tol, c->d_hist, c->d_state, c->comps(),
+                        c->stream));
 }
 
 // Captures `body` on the context's stream into *exec (once per hierarchy).
@@ -1059,6 +1063,10 @@ bsam_ctx *bsam_ctx_create(const bsam_cfg *cfg) {
   }
   if (cfg->minlevel > 0) {
     g_err = "minlevel must be <= 0";
+    return nullptr;
+  }
+  if (cfg->errortype < 0 || cfg->errortype > 2) {
+    g_err = "ErrorAFAS: only errortype = 1, 2 are supported";
     return nullptr;
   }
   if (cfg->precision != BSAM_PREC_FP64 && cfg->precision != BSAM_PREC_MIXED) {

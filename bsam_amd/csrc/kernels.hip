@@ -1455,14 +1455,14 @@ __device__ __forceinline__ double block_sum(double s) {
 // afasroutines.f90:799-840; what = 0) or of f itself (what = 1).
 template <int ND>
 __global__ __launch_bounds__(256) void k_resnorm(Dev d, const Tile *tiles, int nt, double *partial,
-                                                  int what) {
+                                                  int what, int comps) {
   const int b = xcd_tile(nt);
   if (b >= nt) return;
   Tile t = tiles[b];
   const DPatch &P = d.P[t.patch];
   int i, j, k;
   tile_cell<ND>(d, t, i, j, k);
-  double s = 0.0;
+  double s = 0.0, s2 = 0.0;
   if (in_box<ND>(i, j, k, P.mx, P.my, P.mz)) {
     long long fo = P.f + fat(P, i, j, ND == 3 ? k : 1);
     double r1 = d.f[fo], r2 = d.f[fo + P.fvs];
@@ -1474,23 +1474,44 @@ __global__ __launch_bounds__(256) void k_resnorm(Dev d, const Tile *tiles, int n
       r1 = r1 - o1;
       r2 = r2 - o2;
     }
-    s = r1 * r1 + r2 * r2;
+    s = comps == 2 ? r1 * r1 : r1 * r1 + r2 * r2;
+    s2 = r2 * r2;
   }
   double tot = block_sum(s);
   if (threadIdx.x == 0) partial[b] = tot;
+  if (comps == 2) {   // errortype 2: the second component's sum after the first's
+    __syncthreads();
+    tot = block_sum(s2);
+    if (threadIdx.x == 0) partial[nt + b] = tot;
+  }
 }
 
 // Sum the per-tile partials in index order (strided, then a fixed tree) and
-// finish ErrorAFAS: sqrt(sum / count) into hist[state[0]]; state[0] counts
-// the calls, state[1] is set once a residual falls below tol and state[2]
-// once a residual is not finite.
+// finish ErrorAFAS: sqrt(sum / count) into hist[state[0]] (errortype 1), or
+// with comps 2 the larger of the two components' sqrt(sum / count), the
+// second component's partials following the first's (errortype 2,
+// lib/afasroutines.f90:128-142; a NaN in either wins); state[0] counts the
+// calls, state[1] is set once a residual falls below tol and state[2] once a
+// residual is not finite.
 __global__ __launch_bounds__(256) void k_finalize(const double *partial, int n, double count,
-                                                   double tol, double *hist, int *state) {
+                                                   double tol, double *hist, int *state,
+                                                   int comps) {
   double s = 0.0;
   for (int e = threadIdx.x; e < n; e += blockDim.x) s += partial[e];
   double tot = block_sum(s);
+  double tot2 = 0.0;
+  if (comps == 2) {
+    __syncthreads();
+    s = 0.0;
+    for (int e = threadIdx.x; e < n; e += blockDim.x) s += partial[n + e];
+    tot2 = block_sum(s);
+  }
   if (threadIdx.x == 0) {
     double r = sqrt(tot / count);
+    if (comps == 2) {
+      const double r2 = sqrt(tot2 / count);
+      r = (isnan(r) || r >= r2) ? r : r2;
+    }
     int it = state[0];
     hist[it] = r;
     state[0] = it + 1;
@@ -1666,15 +1687,15 @@ hipError_t launch_apply_correction(int nd, const Dev &d, const Tile *tiles, int 
 }
 
 hipError_t launch_resnorm(int nd, const Dev &d, const Tile *tiles, int nt, double *partial,
-                          int what, hipStream_t s) {
+                          int what, int comps, hipStream_t s) {
   if (nt <= 0) return hipSuccess;
-  BSAM_LAUNCH_ND(k_resnorm, nd, xcd_grid(nt), 256, d, tiles, nt, partial, what);
+  BSAM_LAUNCH_ND(k_resnorm, nd, xcd_grid(nt), 256, d, tiles, nt, partial, what, comps);
   return hipGetLastError();
 }
 
 hipError_t launch_finalize(const double *partial, int n, double count, double tol,
-                           double *hist, int *state, hipStream_t s) {
-  k_finalize<<<1, 256, 0, s>>>(partial, n, count, tol, hist, state);
+                           double *hist, int *state, int comps, hipStream_t s) {
+  k_finalize<<<1, 256, 0, s>>>(partial, n, count, tol, hist, state, comps);
   return hipGetLastError();
 }
 

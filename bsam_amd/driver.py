@@ -137,8 +137,11 @@ class Driver:
         self.uniform = bool(rd.get("outputuniformmesh", False))
         if rd.get("syncelliptic", False):
             raise NotImplementedError("syncelliptic runs are not supported")
-        if int(rd.get("errortype", 1)) != 1 or int(rd.get("updateauxfreq", 1)) != 1:
-            raise NotImplementedError("errortype 1 and updateauxfreq 1 only")
+        self.errortype = int(rd.get("errortype", 1))
+        if self.errortype not in (1, 2):
+            raise ValueError("ErrorAFAS: only errortype = 1, 2 are supported")
+        if int(rd.get("updateauxfreq", 1)) != 1:
+            raise NotImplementedError("updateauxfreq 1 only")
         # RootInit's defaults (:237-254)
         self.ndims = nd = int(gd.get("ndims", 2))
         self.maxlevel = int(gd.get("maxlevel", 0))
@@ -187,7 +190,8 @@ class Driver:
             self._engines[k] = Engine(layout, dt=self.dt, eps=self.eps, alpha=self.alpha,
                                       well="reference", nsmoothingpasses=self.nsmooth,
                                       maxvcycles=self.maxvcycles, qerrortol=self.qerrortol,
-                                      device=self.device, tune=tune)
+                                      device=self.device, tune=tune,
+                                      errortype=self.errortype)
         else:
             self._engines[k].describe(layout)
         return self._engines[k]

@@ -44,7 +44,8 @@ class Cfg(C.Structure):
                 ("fc_alpha", C.c_double), ("fc_shift", C.c_double),
                 ("fe_num", C.c_double), ("fe_den", C.c_double),
                 ("nsmoothingpasses", C.c_int), ("maxvcycles", C.c_int),
-                ("qerrortol", C.c_double), ("precision", C.c_int), ("device", C.c_int)]
+                ("qerrortol", C.c_double), ("precision", C.c_int), ("device", C.c_int),
+                ("errortype", C.c_int)]
 
 
 _lib = None
@@ -189,11 +190,12 @@ class Engine:
                  well: str = "reference", nsmoothingpasses: int = 2,
                  maxvcycles: int = 20, qerrortol: float = 1e-8,
                  precision: int = PREC_FP64, device: int = 0,
-                 tune: Optional[Dict[str, float]] = None):
+                 tune: Optional[Dict[str, float]] = None, errortype: int = 1):
         L = lib()
         fa, fs, fn, fd = WELLS[well](alpha)
         cfg = Cfg(layout.ndims, layout.nrvars, layout.mbc, layout.minlevel, dt, eps * eps,
-                  fa, fs, fn, fd, nsmoothingpasses, maxvcycles, qerrortol, precision, device)
+                  fa, fs, fn, fd, nsmoothingpasses, maxvcycles, qerrortol, precision, device,
+                  errortype)
         self.layout = layout
         self.maxvcycles = maxvcycles
         self.ctx = L.bsam_ctx_create(C.byref(cfg))

@@ -84,6 +84,7 @@ contains
     cfg%qerrortol = qerrortol
     cfg%precision = precision
     cfg%device = device
+    cfg%errortype = errortype
     gpu_ctx = bsam_ctx_create(cfg)
     if (.not. c_associated(gpu_ctx)) then
       print *, 'AFASGPU: cannot create the engine context: ', &

@@ -17,6 +17,7 @@ module bsam_gpu
     integer(c_int) :: nsmoothingpasses, maxvcycles
     real(c_double) :: qerrortol
     integer(c_int) :: precision, device
+    integer(c_int) :: errortype
   end type bsam_cfg
 
   interface

@@ -65,6 +65,10 @@ typedef struct bsam_cfg {
   int precision;          /* BSAM_PREC_* */
   int device;             /* HIP device ordinal; -1 = host-only context
                              (geometry and ghost programs, no device memory) */
+  int errortype;          /* ErrorAFAS (lib/afasroutines.f90:108-152): 1 (or 0)
+                             the blended RMS of f - A(q) over both components
+                             (L2Error), 2 the larger of the two components'
+                             RMS (L2ComponentErrors) */
 } bsam_cfg;
 
 /* Context lifetime. Replaces the module globals of NodeInfoDef
@@ -151,7 +155,7 @@ int bsam_after_step_stats(bsam_ctx *ctx, double *mass, double *energy);
 int bsam_source(bsam_ctx *ctx);
 /* AFASVcycle(level), lib/afasroutines.f90:488-521 */
 int bsam_vcycle(bsam_ctx *ctx, int level);
-/* ErrorAFAS(level) with errortype 1, lib/afasroutines.f90:108-179 */
+/* ErrorAFAS(level) with the context's errortype, lib/afasroutines.f90:108-208 */
 int bsam_error_afas(bsam_ctx *ctx, int level, double *residual);
 /* qold = q on every level (CopyQToQold, lib/bsamroutines.f90:568-580) */
 int bsam_copy_qold(bsam_ctx *ctx);

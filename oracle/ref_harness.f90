@@ -26,7 +26,8 @@
 !      residual L 0 | mgiter 0 0 | step nsteps 0 | copyqold 0 0 | truncerr L 0 |
 !      userflags L N (qtolerance(L) = N * 1e-3) | stats 0 0 |
 !      uniform R 0 (WriteUniformMeshQ at time 0 with restartframe R: it
-!      writes ./out/uRRRRR.dat under the working directory)
+!      writes ./out/uRRRRR.dat under the working directory) |
+!      errortype E 0 (ErrorAFAS's errortype for the ops that follow)
 !
 ! Input stream: for every level minlevel..finestlevel in level-list order
 ! (ApplyOnLevel order), the arrays q, f, qc of each patch.
@@ -132,6 +133,8 @@ program RefHarness
       end do
       write(14,'(i6,1x,i6,1x,es25.17)') -1, 1, integralresult(1)
       write(14,'(i6,1x,i6,1x,es25.17)') -1, 2, integralresult(2)
+    case('errortype')
+      errortype = a1
     case('uniform')
       restartframe = a1
       call WriteUniformMeshQ(0, 0.0)
