@@ -135,6 +135,7 @@ struct bsam_ctx {
   float *e = nullptr, *qce = nullptr, *fe = nullptr, *qb = nullptr;  // mixed precision
   int inner_cycles = 1;      // FP32 V-cycles per defect-correction step
   DPatch *d_P = nullptr;
+  std::vector<TailLevel> h_tail;   // host copy of the tail level table (d_tail)
   double *d_partial = nullptr;
   double *d_stats = nullptr;      // task sums of the after-step statistics
   StatTask *d_stats_list = nullptr;  // their tasks
@@ -245,6 +246,7 @@ void bsam_ctx::free_device() {
   F(d_src_int);
   F(d_tail);
   d_tail = nullptr;
+  h_tail.clear();
   for (int st = 0; st < 2; st++) {
     F(d_tcin[st]);
     F(d_tcout[st]);
@@ -413,6 +415,7 @@ static bool plan_tail(bsam_ctx *c, long long budget, hipStream_t s) {
   c->tail_lds[1] = (size_t)used[1];
   if (c->d_tail) (void)hipFree(c->d_tail);
   c->d_tail = dev_upload(tl, s);
+  c->h_tail = tl;
   for (int set = 0; set < 2; set++) {
     if (c->d_tcin[set]) (void)hipFree(c->d_tcin[set]);
     if (c->d_tcout[set]) (void)hipFree(c->d_tcout[set]);
@@ -1495,6 +1498,28 @@ static void copy_level_boxes(bsam_ctx *dst, const bsam_ctx *src, int level, bool
   (void)hipStreamSynchronize(dst->stream);
   (void)hipFree(d_boxes);
   HIPCK(e);
+}
+
+int bsam_set_dt(bsam_ctx *c, double dt) {
+  API_BEGIN(c)
+  if (!std::isfinite(dt) || dt < 0.0) throw Error("set_dt: dt must be finite and >= 0");
+  if (dt == c->cfg.dt) return 0;
+  c->cfg.dt = dt;
+  for (size_t p = 0; p < c->dp.size(); p++) {
+    const double h2 = c->dp[p].dx * c->dp[p].dx;   // as at commit
+    c->dp[p].dt_h2 = dt / h2;
+  }
+  for (TailLevel &t : c->h_tail)
+    if (t.patch >= 0 && (size_t)t.patch < c->dp.size()) t.P.dt_h2 = c->dp[t.patch].dt_h2;
+  if (!c->host_only && c->committed) {
+    HIPCK(hipStreamSynchronize(c->stream));
+    HIPCK(hipMemcpy(c->d_P, c->dp.data(), sizeof(DPatch) * c->dp.size(), hipMemcpyHostToDevice));
+    if (c->d_tail && !c->h_tail.empty())
+      HIPCK(hipMemcpy(c->d_tail, c->h_tail.data(), sizeof(TailLevel) * c->h_tail.size(),
+                      hipMemcpyHostToDevice));
+  }
+  c->drop_graphs();   // graphs hold the old problem constants as kernel arguments
+  API_END(c)
 }
 
 int bsam_transfer_values(bsam_ctx *dst, const bsam_ctx *src, int level) {

@@ -135,8 +135,8 @@ class Driver:
         self.restart = bool(rd.get("restart", False))
         self.restartframe = int(rd.get("restartframe", 0))
         self.uniform = bool(rd.get("outputuniformmesh", False))
-        if rd.get("syncelliptic", False):
-            raise NotImplementedError("syncelliptic runs are not supported")
+        # a restart never synchronises (BSAMSolver :133-138)
+        self.sync = bool(rd.get("syncelliptic", False)) and not self.restart
         self.errortype = int(rd.get("errortype", 1))
         if self.errortype not in (1, 2):
             raise ValueError("ErrorAFAS: only errortype = 1, 2 are supported")
@@ -169,6 +169,7 @@ class Driver:
                                   minlevel=self.minlevel, nrvars=2)
         self.minlevel = self.root_layout.minlevel
         self.device = device
+        self._dt_now = self.dt                  # dt of the pass under way
         self._engines: List[Engine] = []
         self.engine: Optional[Engine] = None   # the last step's hierarchy, after run()
         # wall time by phase: the regrid (of which the hierarchy hand-overs,
@@ -187,13 +188,15 @@ class Driver:
             self._engines.append(None)
         if self._engines[k] is None:
             tune = None if k in self.SOLVERS else {"fuse_ghost": 0, "tail_cells": 0, "graph": 0}
-            self._engines[k] = Engine(layout, dt=self.dt, eps=self.eps, alpha=self.alpha,
+            self._engines[k] = Engine(layout, dt=self._dt_now, eps=self.eps, alpha=self.alpha,
                                       well="reference", nsmoothingpasses=self.nsmooth,
                                       maxvcycles=self.maxvcycles, qerrortol=self.qerrortol,
                                       device=self.device, tune=tune,
                                       errortype=self.errortype)
         else:
             self._engines[k].describe(layout)
+            if self._engines[k].dt != self._dt_now:
+                self._engines[k].set_dt(self._dt_now)
         return self._engines[k]
 
     def close(self):
@@ -400,6 +403,31 @@ class Driver:
         currenttime = starttime
         first = True                             # TakeTimeSteps' firstamr
         initial = not self.restart               # initial grids take QInit
+        frame0 = not self.restart                # outputinitialdata
+        if self.sync:
+            # the synchronisation pass (:620-622, :694-699, :748-756): the
+            # first regrid and solve with dt = 0, which leaves c and makes mu
+            # consistent with it; no frame, no statistics, no step counted
+            self._dt_now = 0.0
+            for e in self._engines:
+                if e is not None:
+                    e.set_dt(0.0)
+            t0 = time.perf_counter()
+            cur = self._amr(cur, initial=True)
+            eng = self._engines[cur]
+            eng.fill_down(FIELD_Q)
+            eng.copy_qold()
+            self.timing["regrid_s"] += time.perf_counter() - t0
+            t0 = time.perf_counter()
+            for _ in range(self.eqfn):
+                eng.multigrid_iterations()
+            eng.sync()
+            self.timing["solve_s"] += time.perf_counter() - t0
+            self._dt_now = self.dt
+            for e in self._engines:
+                if e is not None and e.dt != self._dt_now:
+                    e.set_dt(self._dt_now)
+            first = initial = False
         for n in range(firstframe, self.outframes + 1):
             for it in range(1, itperprint + 1):
                 t0 = time.perf_counter()
@@ -409,10 +437,11 @@ class Driver:
                     eng.fill_down(FIELD_Q)
                 eng.copy_qold()
                 self.timing["regrid_s"] += time.perf_counter() - t0
-                if first and not self.restart:
+                if frame0:
                     self._write_frame(eng, 0, currenttime)
                     if self.uniform:
                         self._write_uniform(eng, currenttime)
+                    frame0 = False
                 first = initial = False
                 t0 = time.perf_counter()
                 for _ in range(self.eqfn):

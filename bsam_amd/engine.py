@@ -146,6 +146,7 @@ def lib():
         L.bsam_init_fields.argtypes = [vp, C.c_int]
         L.bsam_transfer_values.argtypes = [vp, vp, C.c_int]
         L.bsam_overlay_level.argtypes = [vp, vp, C.c_int]
+        L.bsam_set_dt.argtypes = [vp, C.c_double]
         L.bsam_regrid_level.argtypes = [C.POINTER(RegridParams), C.c_int, ip, ip,
                                         C.POINTER(ip), C.c_int, ip, ip, ip]
         L.bsam_after_step_stats.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
@@ -198,6 +199,7 @@ class Engine:
                   errortype)
         self.layout = layout
         self.maxvcycles = maxvcycles
+        self.dt = float(dt)
         self.ctx = L.bsam_ctx_create(C.byref(cfg))
         if not self.ctx:
             raise BsamError(L.bsam_last_error(None).decode())
@@ -295,6 +297,11 @@ class Engine:
     def transfer_values_from(self, src: "Engine", level: int):
         """TransferValues: q of the cells `level`'s patches share with src's."""
         self._ck(lib().bsam_transfer_values(self.ctx, src.ctx, level))
+
+    def set_dt(self, dt: float):
+        """The time step of the problem constants from now on."""
+        self._ck(lib().bsam_set_dt(self.ctx, float(dt)))
+        self.dt = float(dt)
 
     def overlay_level_from(self, src: "Engine", level: int):
         """CopyPatchToUniformGrid over `level`: src's patches, ghosts

@@ -93,6 +93,14 @@ contains
     end if
   end subroutine GpuOpen
 
+  ! The module's dt as it is now: TakeTimeSteps sets it to 0 for the first
+  ! pass of a syncelliptic run and back after (lib/bsamroutines.f90:620-622,
+  ! :748-750). The engine returns at once when it is unchanged.
+  subroutine GpuSetDt()
+    use NodeInfoDef, only: dt
+    call check(bsam_set_dt(gpu_ctx, real(dt, c_double)), 'bsam_set_dt')
+  end subroutine GpuSetDt
+
   subroutine GpuClose()
     integer(c_int) :: rc
     if (c_associated(gpu_ctx)) rc = bsam_ctx_destroy(gpu_ctx)
@@ -239,6 +247,7 @@ contains
   subroutine GpuMultigridIterations()
     use NodeInfoDef
     integer(c_int) :: n, conv
+    call GpuSetDt()
     call GpuUploadAll()
     call check(bsam_multigrid_iterations(gpu_ctx, n, gpu_residuals, &
                                          int(size(gpu_residuals), c_int), conv), &
@@ -258,6 +267,7 @@ contains
   ! leave.
   subroutine GpuRelativeTruncationError(level)
     integer, intent(in) :: level
+    call GpuSetDt()
     call walk(BSAM_FIELD_Q, 1)
     call check(bsam_truncation_error(gpu_ctx, level), 'bsam_truncation_error')
     call walk(BSAM_FIELD_Q, 0, level - 1)

@@ -151,6 +151,12 @@ module bsam_gpu
       real(c_double), intent(out) :: value
       integer(c_int) :: rc
     end function
+    function bsam_set_dt(ctx, dt) bind(C, name='bsam_set_dt') result(rc)
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: dt
+      integer(c_int) :: rc
+    end function
     function bsam_sync(ctx) bind(C, name='bsam_sync') result(rc)
       import :: c_ptr, c_int
       type(c_ptr), value :: ctx

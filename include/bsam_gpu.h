@@ -220,6 +220,13 @@ int bsam_solve_status(const bsam_ctx *ctx);
  * bsam_copy_levels, bsam_transfer_values and bsam_overlay_level wait for src's
  * stream first and return when their copies are done, so src may change
  * afterwards. */
+/* The time step of the problem constants (bsam_cfg.dt) from now on, for
+ * every solve, truncation-error estimate and source set-up. TakeTimeSteps
+ * runs the first pass of a syncelliptic run with dt = 0 and restores it
+ * after (lib/bsamroutines.f90:620-622, :748-750); the module variable dt
+ * changes under MultigridIterations, so the Fortran binding passes it on
+ * each call. dt must be finite and >= 0. Waits for the context's stream. */
+int bsam_set_dt(bsam_ctx *ctx, double dt);
 int bsam_copy_levels(bsam_ctx *dst, const bsam_ctx *src, int top);
 int bsam_init_fields(bsam_ctx *ctx, int level);
 int bsam_transfer_values(bsam_ctx *dst, const bsam_ctx *src, int level);

@@ -34,6 +34,10 @@ def first_difference(a: bytes, b: bytes) -> str:
 def variant(name: str, steps: int):
     deck = bp.shipped_deck(steps)
     g = deck["griddata"]
+    if name.endswith("_sync"):
+        # syncelliptic: a first regrid and solve with dt = 0 before the steps
+        deck["rundata"]["syncelliptic"] = True
+        name = name[:-len("_sync")]
     if name == "neumann_l3":
         g["mthbc"] = {i: 1 for i in range(1, 7)}
         g["maxlevel"] = 3
@@ -46,7 +50,8 @@ def variant(name: str, steps: int):
     return deck
 
 
-@pytest.mark.parametrize("name,steps", [("shipped", 6), ("neumann_l3", 3), ("3d_l1", 3)])
+@pytest.mark.parametrize("name,steps", [("shipped", 6), ("neumann_l3", 3), ("3d_l1", 3),
+                                        ("shipped_sync", 3), ("3d_l1_sync", 2)])
 def test_program_frames_identical(tmp_path, name, steps):
     for prog in (bp.REF_PROGRAM, bp.GPU_PROGRAM):
         if not os.path.exists(prog):
@@ -62,7 +67,8 @@ def test_program_frames_identical(tmp_path, name, steps):
         assert gpu[n] == ref[n], f"frame {n}: {first_difference(gpu[n], ref[n])}"
 
 
-@pytest.mark.parametrize("name,steps", [("shipped", 6), ("neumann_l3", 3), ("3d_l1", 3)])
+@pytest.mark.parametrize("name,steps", [("shipped", 6), ("neumann_l3", 3), ("3d_l1", 3),
+                                        ("shipped_sync", 3), ("3d_l1_sync", 2)])
 def test_native_driver_frames_identical(tmp_path, name, steps):
     """The engine's own time-step driver (bsam_amd/driver.py: the regrid's
     clustering on the host, its tagging, InitFields and TransferValues on
