@@ -254,3 +254,25 @@ def test_native_driver_uniform_mesh_2d(tmp_path, name, steps):
     _, _, ps = frames.read_frame(str(work / "out" / "u00000.dat"))
     assert top == lay.finestlevel
     assert ps[0].mx == tuple(m * 2 ** top for m in lay.patches[0].mx[:2])
+
+
+def test_sync_pass_changes_the_run(tmp_path):
+    """The syncelliptic pass is not a no-op: after it mu is the one of the
+    initial c, so the run's frames differ from the run without it, while
+    frame 0 still holds the initial c (the pass leaves c as it was, to the
+    solve's tolerance)."""
+    from bsam_amd.driver import Driver
+    out = {}
+    for sync in (False, True):
+        deck = variant("shipped", 2)
+        deck["rundata"]["syncelliptic"] = sync
+        work = tmp_path / ("sync" if sync else "plain")
+        bp.write_deck(str(work), deck)
+        drv = Driver(str(work), device=0)
+        drv.run()
+        drv.close()
+        out[sync] = {n: frames.read_frame(str(work / "out" / f"m{n:05d}.dat")) for n in (0, 1, 2)}
+    assert out[True][2][2][0].q.tobytes() != out[False][2][2][0].q.tobytes()
+    c_sync = out[True][0][2][0].q[..., 0]
+    c_plain = out[False][0][2][0].q[..., 0]
+    assert np.abs(c_sync[1:-1, 1:-1] - c_plain[1:-1, 1:-1]).max() < 1e-6
