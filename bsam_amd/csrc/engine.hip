@@ -167,6 +167,7 @@ struct bsam_ctx {
   std::vector<Tile> all_int, all_ext, src_int;
   Tile *d_all_int = nullptr, *d_all_ext = nullptr, *d_src_int = nullptr;
   long long src_cells = 0;
+  long long all_int_cells = 0, all_ext_cells = 0;   // interior / ghost-extended cells, all levels
   long long tail_cells = 4096;
   int tail_threads = 1024;
   long long tail_lds_cap = -1;      // LDS bytes the tail may use (-1: device limit)
@@ -700,8 +701,12 @@ static void commit(bsam_ctx *c) {
   c->all_ext.clear();
   c->src_int.clear();
   c->src_cells = 0;
+  c->all_int_cells = c->all_ext_cells = 0;
+  for (const DPatch &P : c->dp)
+    c->all_ext_cells += (long long)(P.mx + 2) * (P.my + 2) * (c->nd() == 3 ? P.mz + 2 : 1);
   for (int l = c->cfg.minlevel; l <= c->finest; l++) {
     LevelData &L = c->lev[l - minl];
+    c->all_int_cells += L.cells;
     c->all_int.insert(c->all_int.end(), L.t_int.begin(), L.t_int.end());
     c->all_ext.insert(c->all_ext.end(), L.t_ext.begin(), L.t_ext.end());
     if (l >= 0) {
@@ -856,7 +861,7 @@ static int level_relax(bsam_ctx *c, int level, int prec = PREC_SOLUTION_FP64,
     for (int color = 1; color <= 2; color++) {
       int f = pending >= 0 ? fused_index(pending, color) : -1;
       if (f >= 0 && c->fuse_ghost && L.fused[f].ok) {
-        ProfScope ps(c, BSAM_PROF_RELAX,
+        ProfScope ps(c, BSAM_PROF_RELAX_FUSED,
                      (prec == PREC_SOLUTION_FP64 ? 24.0 : 12.0) * L.cells);
         HIPCK(launch_relax_fused(c->nd(), prec, c->dev(), L.d_int, (int)L.t_int.size(), color,
                                  L.d_fstart[f], L.d_fops[f], c->stream));
@@ -982,7 +987,10 @@ static void multigrid_iterations(bsam_ctx *c, int *ncycles, double *hist, int ma
   // problem (f = ftmp + 0 on levels >= 0) and FillDown(f) then overwrites
   // every covered cell by restriction. Their result is the same each
   // iteration, so it is computed once here and restored by a copy.
-  HIPCK(launch_copy(c->f_saved, c->f, c->nf, c->stream));
+  {
+    ProfScope ps(c, BSAM_PROF_COPY, 16.0 * c->nf);
+    HIPCK(launch_copy(c->f_saved, c->f, c->nf, c->stream));
+  }
   reset_state(c);
   // one iteration of the vcycleloop (afasroutines.f90:69-104): the V-cycle,
   // FillDown(q), source restore + FillDown(f) (the copy), the residual norm
@@ -996,7 +1004,10 @@ static void multigrid_iterations(bsam_ctx *c, int *ncycles, double *hist, int ma
   int n = iterate(c, &c->graph, &c->graph_exec, [&]() {
     int pend = op_vcycle(c, c->finest, PREC_SOLUTION_FP64, true);
     op_fill_down(c, BSAM_FIELD_Q, pend, INT_MAX, 0);
-    HIPCK(launch_copy(c->f, c->f_saved, c->nf, c->stream));
+    {
+      ProfScope ps(c, BSAM_PROF_COPY, 16.0 * c->nf);
+      HIPCK(launch_copy(c->f, c->f_saved, c->nf, c->stream));
+    }
     queue_error(c, c->finest, c->cfg.qerrortol);
   }, &done);
   op_fill_down(c, BSAM_FIELD_Q, -1, 0, INT_MIN);
@@ -1019,8 +1030,11 @@ static void mixed_iterations(bsam_ctx *c, int *ncycles, double *hist, int maxhis
   reset_state(c);
   int done = 0;
   int n = iterate(c, &c->graph_mixed, &c->graph_mixed_exec, [&]() {
-    HIPCK(launch_defect_setup(c->nd(), c->dev(), c->d_all_ext, (int)c->all_ext.size(),
-                              c->stream));
+    {
+      ProfScope ps(c, BSAM_PROF_DEFECT, 52.0 * c->all_ext_cells);
+      HIPCK(launch_defect_setup(c->nd(), c->dev(), c->d_all_ext, (int)c->all_ext.size(),
+                                c->stream));
+    }
     // the last inner cycle's final ghost fill of the correction is never read
     // (the correction is applied on interior cells, then restarts at zero)
     for (int k = 0; k < c->inner_cycles; k++) {
@@ -1028,8 +1042,11 @@ static void mixed_iterations(bsam_ctx *c, int *ncycles, double *hist, int maxhis
       if (pend >= 0 && k + 1 < c->inner_cycles)
         op_set_ghost(c, c->finest, pend, PREC_DEFECT_FP32);
     }
-    HIPCK(launch_apply_correction(c->nd(), c->dev(), c->d_all_int, (int)c->all_int.size(),
-                                  c->stream));
+    {
+      ProfScope ps(c, BSAM_PROF_DEFECT, 40.0 * c->all_int_cells);
+      HIPCK(launch_apply_correction(c->nd(), c->dev(), c->d_all_int, (int)c->all_int.size(),
+                                    c->stream));
+    }
     op_fill_down(c, BSAM_FIELD_Q);
     for (int l = 0; l <= c->finest; l++) op_set_ghost(c, l, 0);
     queue_error(c, c->finest, c->cfg.qerrortol);
@@ -1068,7 +1085,8 @@ bsam_ctx *bsam_ctx_create(const bsam_cfg *cfg) {
     g_err = "minlevel must be <= 0";
     return nullptr;
   }
-  if (cfg->errortype < 0 || cfg->errortype > 2) {
+  if (cfg->errortype != 1 && cfg->errortype != 2) {
+    // the reference stops on any other value (lib/afasroutines.f90:143-146)
     g_err = "ErrorAFAS: only errortype = 1, 2 are supported";
     return nullptr;
   }
@@ -1320,6 +1338,7 @@ int bsam_after_step_stats(bsam_ctx *c, double *mass, double *energy) {
   first[2 * list.size()] = (int)tasks.size();
   if (c->n_stats < (int)tasks.size()) {
     HIPCK(hipStreamSynchronize(c->stream));
+    DeviceLock lock(g_device_lock);
     if (c->d_stats) (void)hipFree(c->d_stats);
     if (c->d_stats_list) (void)hipFree(c->d_stats_list);
     c->d_stats = nullptr;
@@ -1491,12 +1510,19 @@ static void copy_level_boxes(bsam_ctx *dst, const bsam_ctx *src, int level, bool
     }
   cuts.push_back((int)boxes.size());
   HIPCK(hipStreamSynchronize(src->stream));
-  BoxCopy *d_boxes = dev_upload(boxes, dst->stream);
+  BoxCopy *d_boxes = nullptr;
+  {
+    DeviceLock lock(g_device_lock);
+    d_boxes = dev_upload(boxes, dst->stream);
+  }
   hipError_t e = hipSuccess;
   for (size_t c = 0; c + 1 < cuts.size() && e == hipSuccess; c++)
     e = launch_box_copy(dst->q, src->q, d_boxes + cuts[c], cuts[c + 1] - cuts[c], dst->stream);
   (void)hipStreamSynchronize(dst->stream);
-  (void)hipFree(d_boxes);
+  {
+    DeviceLock lock(g_device_lock);
+    (void)hipFree(d_boxes);
+  }
   HIPCK(e);
 }
 
@@ -1512,11 +1538,15 @@ int bsam_set_dt(bsam_ctx *c, double dt) {
   for (TailLevel &t : c->h_tail)
     if (t.patch >= 0 && (size_t)t.patch < c->dp.size()) t.P.dt_h2 = c->dp[t.patch].dt_h2;
   if (!c->host_only && c->committed) {
-    HIPCK(hipStreamSynchronize(c->stream));
-    HIPCK(hipMemcpy(c->d_P, c->dp.data(), sizeof(DPatch) * c->dp.size(), hipMemcpyHostToDevice));
+    // on the context's own stream (no legacy-stream work that another
+    // thread's capture would forbid); the host tables stay alive until the
+    // copies are done
+    HIPCK(hipMemcpyAsync(c->d_P, c->dp.data(), sizeof(DPatch) * c->dp.size(),
+                         hipMemcpyHostToDevice, c->stream));
     if (c->d_tail && !c->h_tail.empty())
-      HIPCK(hipMemcpy(c->d_tail, c->h_tail.data(), sizeof(TailLevel) * c->h_tail.size(),
-                      hipMemcpyHostToDevice));
+      HIPCK(hipMemcpyAsync(c->d_tail, c->h_tail.data(), sizeof(TailLevel) * c->h_tail.size(),
+                           hipMemcpyHostToDevice, c->stream));
+    HIPCK(hipStreamSynchronize(c->stream));
   }
   c->drop_graphs();   // graphs hold the old problem constants as kernel arguments
   API_END(c)
@@ -1660,6 +1690,21 @@ int bsam_error_afas(bsam_ctx *c, int level, double *residual) {
   if (residual) *residual = r;
   API_END(c)
 }
+int bsam_source_norm(bsam_ctx *c, int level, double *rms) {
+  API_BEGIN(c)
+  c->need_device();
+  LevelData &L = c->L(level);
+  int nt = (int)L.t_int.size();
+  HIPCK(launch_resnorm(c->nd(), c->dev(), L.d_int, nt, c->d_partial, 1, 1, c->stream));
+  HIPCK(hipMemsetAsync(c->d_state, 0, sizeof(int) * 4, c->stream));
+  HIPCK(launch_finalize(c->d_partial, nt, (double)(c->cfg.nrvars * L.cells), -1.0, c->d_hist,
+                        c->d_state, 1, c->stream));
+  double r = 0.0;
+  HIPCK(hipMemcpyAsync(&r, c->d_hist, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCK(hipStreamSynchronize(c->stream));
+  if (rms) *rms = r;
+  API_END(c)
+}
 int bsam_copy_qold(bsam_ctx *c) {
   API_BEGIN(c)
   c->need_device();
@@ -1678,7 +1723,10 @@ int bsam_time_step(bsam_ctx *c, int *ncycles, double *hist, int maxhist, int *co
   c->need_device();
   for (int level = 0; level <= c->finest; level++) op_set_ghost(c, level, 0);
   op_fill_down(c, BSAM_FIELD_Q);
-  HIPCK(launch_copy(c->qold, c->q, c->nq, c->stream));
+  {
+    ProfScope ps(c, BSAM_PROF_COPY, 16.0 * c->nq);
+    HIPCK(launch_copy(c->qold, c->q, c->nq, c->stream));
+  }
   solve(c, ncycles, hist, maxhist, converged);
   API_END(c)
 }
@@ -1818,6 +1866,51 @@ int bsam_profile_kernel(bsam_ctx *c, int kind, int reps, double *us_per_launch,
   if (us_per_launch) *us_per_launch = 1e3 * ms / (double)n;
   if (bytes_per_launch) *bytes_per_launch = bytes / (double)n;
   API_END(c)
+}
+
+int bsam_copy_rate(int device, long long bytes, int reps, double *gbs) {
+  if (bytes < 8 || reps < 1 || !gbs) {
+    g_err = "copy_rate: bytes >= 8, reps >= 1 and an output pointer are needed";
+    return -1;
+  }
+  double *a = nullptr, *b = nullptr;
+  hipEvent_t e[2] = {nullptr, nullptr};
+  hipStream_t s = nullptr;
+  long long n = bytes / 8;
+  hipError_t err;
+  {
+    DeviceLock lock(g_device_lock);
+    err = hipSetDevice(device);
+    if (err == hipSuccess) err = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    if (err == hipSuccess) err = hipMalloc(&a, 8 * n);
+    if (err == hipSuccess) err = hipMalloc(&b, 8 * n);
+    if (err == hipSuccess) err = hipEventCreate(&e[0]);
+    if (err == hipSuccess) err = hipEventCreate(&e[1]);
+  }
+  if (err == hipSuccess) err = hipMemsetAsync(a, 0, 8 * n, s);
+  if (err == hipSuccess) err = launch_copy(b, a, n, s);   // first touch, untimed
+  if (err == hipSuccess) err = hipEventRecord(e[0], s);
+  for (int r = 0; r < reps && err == hipSuccess; r++)
+    err = (r & 1) ? launch_copy(a, b, n, s) : launch_copy(b, a, n, s);
+  if (err == hipSuccess) err = hipEventRecord(e[1], s);
+  if (err == hipSuccess) err = hipEventSynchronize(e[1]);
+  float ms = 0.f;
+  if (err == hipSuccess) err = hipEventElapsedTime(&ms, e[0], e[1]);
+  {
+    DeviceLock lock(g_device_lock);
+    if (s) (void)hipStreamSynchronize(s);
+    for (hipEvent_t x : e)
+      if (x) (void)hipEventDestroy(x);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    if (s) (void)hipStreamDestroy(s);
+  }
+  if (err != hipSuccess) {
+    g_err = std::string("copy_rate: ") + hipGetErrorString(err);
+    return -2;
+  }
+  *gbs = 2.0 * 8.0 * (double)n * reps / (1e-3 * ms) / 1e9;
+  return 0;
 }
 
 int bsam_profile_get(const bsam_ctx *cc, int kind, long long *launches, double *ms,

@@ -103,6 +103,15 @@ def regrid_level(ndims: int, level: int, ibuffer: int, minimumgridpoints: int,
                                    for d in range(3))) for g in range(ng.value)]
 
 
+def copy_rate(device: int = 0, nbytes: int = 1 << 30, reps: int = 20) -> float:
+    """Measured device-to-device copy rate (GB/s, read plus write bytes) with
+    the engine's copy kernel on buffers of nbytes."""
+    g = C.c_double()
+    if lib().bsam_copy_rate(device, nbytes, reps, C.byref(g)) != 0:
+        raise BsamError(lib().bsam_last_error(None).decode())
+    return g.value
+
+
 def lib():
     """Load libbsamgpu.so (raises if it has not been built)."""
     global _lib
@@ -153,6 +162,8 @@ def lib():
         for n in ("bsam_source", "bsam_copy_qold", "bsam_sync"):
             getattr(L, n).argtypes = [vp]
         L.bsam_error_afas.argtypes = [vp, C.c_int, dp]
+        L.bsam_source_norm.argtypes = [vp, C.c_int, dp]
+        L.bsam_copy_rate.argtypes = [C.c_int, C.c_longlong, C.c_int, dp]
         for n in ("bsam_multigrid_iterations", "bsam_time_step"):
             getattr(L, n).argtypes = [vp, ip, dp, C.c_int, ip]
         L.bsam_tune.argtypes = [vp, C.c_char_p, C.c_double]
@@ -343,6 +354,13 @@ class Engine:
         self._ck(lib().bsam_error_afas(self.ctx, level, C.byref(r)))
         return r.value
 
+    def source_norm(self, level: int) -> float:
+        """RMS of f over the level's cells (both variables): the denominator
+        of the relative residual RMS(f - A(q)) / RMS(f)."""
+        r = C.c_double()
+        self._ck(lib().bsam_source_norm(self.ctx, level, C.byref(r)))
+        return r.value
+
     def copy_qold(self):
         self._ck(lib().bsam_copy_qold(self.ctx))
 
@@ -383,7 +401,7 @@ class Engine:
 
     # -- kernel timing ----------------------------------------------------------
     PROF_KINDS = ("relax", "ghost", "restrict", "coarse_ghost", "coarse_load", "correct",
-                  "filldown", "source", "resnorm", "tail")
+                  "filldown", "source", "resnorm", "tail", "relax_fused", "copy", "defect")
 
     def profile_begin(self, kinds=("relax",)):
         mask = 0

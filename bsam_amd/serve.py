@@ -57,7 +57,8 @@ class Result:
     cycles: List[int] = field(default_factory=list)
     converged: bool = True
     footprint: int = 0
-    phi: Optional[np.ndarray] = None   # final root phi (kept when asked)
+    phi: Optional[np.ndarray] = None   # final root q (kept when asked)
+    mass: List[float] = field(default_factory=list)   # per step (kept when asked)
 
     @property
     def latency(self) -> float:
@@ -90,7 +91,10 @@ def request_layout(r: Request) -> Layout:
 
 def solve_request(r: Request, precision: int, device: int = 0, keep_phi: bool = False,
                   tune: Optional[Dict[str, float]] = None, on_commit=None) -> Result:
-    """Runs one request start to finish on its own engine context."""
+    """Runs one request start to finish on its own engine context. With
+    keep_phi the result also keeps the root patch's final q and the composite
+    mass after every step (TakeTimeSteps' statistic, one stream
+    synchronisation per step)."""
     lay = request_layout(r)
     eng = E.Engine(lay, dt=r.dt, eps=r.eps, well="plusminus", maxvcycles=60,
                    qerrortol=1e-8, precision=precision, device=device, tune=tune)
@@ -104,6 +108,8 @@ def solve_request(r: Request, precision: int, device: int = 0, keep_phi: bool = 
         hist, conv = eng.time_step()
         res.cycles.append(len(hist))
         res.converged = res.converged and conv
+        if keep_phi:
+            res.mass.append(eng.after_step_stats()[0])
     if keep_phi:
         res.phi = eng.download(lay.index_map()[0], "q")
     eng.close()

@@ -197,57 +197,62 @@ def config_c1(root: int = 128, seed: int = 1):
 
 def nested_random_layout(root: int, nlevels: int, counts, seed: int,
                          smin: int = 8, smax: int = 128, margin: int = 1,
-                         max_tries: int = 200000) -> Layout:
-    """A properly nested synthetic hierarchy: on each level, `counts[l]`
-    patches with side lengths (fine cells) drawn log-uniform in [smin, smax]
-    (even; the upper end capped by the parent's size), placed at random inside
-    a parent drawn in proportion to its area,
-    at least `margin` coarse cells from the parent's edges and not
-    overlapping their siblings."""
+                         tries_per_patch: int = 4000) -> Layout:
+    """A properly nested synthetic 2D hierarchy: on each level, `counts[l]`
+    patches whose side lengths (fine cells, even) are drawn independently
+    per dimension log-uniform in [smin, smax], whatever the parents' sizes.
+
+    The level's patches are placed largest first. Each goes to a random free
+    position, at least `margin` coarse cells from the parent's edges and not
+    overlapping a sibling, in a parent drawn in proportion to its remaining
+    free area among the parents large enough to hold it. A patch that finds
+    no room in `tries_per_patch` attempts is left out (none is on the
+    benchmark configuration; `layout.dropped` counts them per level)."""
     rng = np.random.default_rng(seed)
     lay = Layout(2, (root, root), 1.0 / root)
     occ = {0: np.zeros((root, root), bool)}
+    lay.dropped = []
     for lev in range(1, nlevels + 1):
         parents = [lay.patches[i] for i in lay.level_ids(lev - 1)]
-        area = np.array([p.mx[0] * p.mx[1] for p in parents], float)
-        prob = area / area.sum()
-        made = tries = 0
-        while made < counts[lev - 1] and tries < max_tries:
-            tries += 1
-            par = parents[rng.choice(len(parents), p=prob)]
-            # log-uniform side in [smin, smax], capped by what the parent holds
-            hi = [min(smax, 2 * (par.mx[d] - 2 * margin)) for d in range(2)]
-            if min(hi) < smin:
-                continue
-            side = [np.exp(rng.uniform(np.log(smin), np.log(hi[d]))) for d in range(2)]
-            cs = [max(smin // 2, int(round(v / 2.0))) for v in side]   # coarse cells
-            lo = []
-            ok = True
-            for d in range(2):
-                room = par.mx[d] - 2 * margin - cs[d]
-                if room < 0:
-                    ok = False
+        sides = np.exp(rng.uniform(np.log(smin), np.log(smax), size=(counts[lev - 1], 2)))
+        cs = np.maximum(smin // 2, np.round(sides / 2.0).astype(int))   # coarse cells
+        order = np.argsort(-(cs[:, 0] * cs[:, 1]), kind="stable")
+        free = np.array([float(p.mx[0] * p.mx[1]) for p in parents])
+        dropped = 0
+        for n in order:
+            cx, cy = int(cs[n, 0]), int(cs[n, 1])
+            cand = [i for i, p in enumerate(parents)
+                    if p.mx[0] - 2 * margin >= cx and p.mx[1] - 2 * margin >= cy]
+            placed = False
+            if cand:
+                w = free[cand] / free[cand].sum()
+                for _ in range(tries_per_patch):
+                    pi = cand[rng.choice(len(cand), p=w)]
+                    par = parents[pi]
+                    lo0 = margin + 1 + int(rng.integers(0, par.mx[0] - 2 * margin - cx + 1))
+                    lo1 = margin + 1 + int(rng.integers(0, par.mx[1] - 2 * margin - cy + 1))
+                    sl = (slice(lo0 - 1, lo0 - 1 + cx), slice(lo1 - 1, lo1 - 1 + cy))
+                    o = occ[par.id]
+                    if o[sl].any():
+                        continue
+                    o[sl] = True
+                    free[pi] -= cx * cy
+                    cid = lay.add_child(par.id, ((lo0, lo0 + cx - 1), (lo1, lo1 + cy - 1)))
+                    occ[cid] = np.zeros((2 * cx, 2 * cy), bool)
+                    placed = True
                     break
-                lo.append(margin + 1 + int(rng.integers(0, room + 1)))
-            if not ok:
-                continue
-            sl = (slice(lo[0] - 1, lo[0] - 1 + cs[0]), slice(lo[1] - 1, lo[1] - 1 + cs[1]))
-            o = occ[par.id]
-            if o[sl].any():
-                continue
-            o[sl] = True
-            cid = lay.add_child(par.id, ((lo[0], lo[0] + cs[0] - 1), (lo[1], lo[1] + cs[1] - 1)))
-            occ[cid] = np.zeros((2 * cs[0], 2 * cs[1]), bool)
-            made += 1
+            dropped += not placed
+        lay.dropped.append(dropped)
     return lay
 
 
 C3_PARAMS = dict(eps=0.03, dt=1e-3, well="plusminus", nsmoothingpasses=2, qerrortol=1e-8)
 
 
-def config_c3(root: int = 512, seed: int = 3, counts=(220, 640, 1140)):
-    """2D root 512^2 plus 3 refined levels of ~2000 uneven patches (sides
-    8..128, log-uniform), phi0 = 0.1 + U(-0.05, 0.05)."""
+def config_c3(root: int = 512, seed: int = 3, counts=(300, 640, 1060)):
+    """2D root 512^2 plus 3 refined levels of 2000 uneven patches (sides
+    8..128 fine cells, log-uniform per dimension on every level: about
+    3.9 M refined cells), phi0 = 0.1 + U(-0.05, 0.05)."""
     lay = nested_random_layout(root, 3, counts, seed)
     return lay, noise_q(lay, seed, 0.1, 0.05)
 

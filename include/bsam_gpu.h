@@ -65,10 +65,11 @@ typedef struct bsam_cfg {
   int precision;          /* BSAM_PREC_* */
   int device;             /* HIP device ordinal; -1 = host-only context
                              (geometry and ghost programs, no device memory) */
-  int errortype;          /* ErrorAFAS (lib/afasroutines.f90:108-152): 1 (or 0)
-                             the blended RMS of f - A(q) over both components
+  int errortype;          /* ErrorAFAS (lib/afasroutines.f90:108-152): 1 the
+                             blended RMS of f - A(q) over both components
                              (L2Error), 2 the larger of the two components'
-                             RMS (L2ComponentErrors) */
+                             RMS (L2ComponentErrors); any other value is
+                             refused, as the reference stops on it (:143-146) */
 } bsam_cfg;
 
 /* Context lifetime. Replaces the module globals of NodeInfoDef
@@ -157,6 +158,12 @@ int bsam_source(bsam_ctx *ctx);
 int bsam_vcycle(bsam_ctx *ctx, int level);
 /* ErrorAFAS(level) with the context's errortype, lib/afasroutines.f90:108-208 */
 int bsam_error_afas(bsam_ctx *ctx, int level, double *residual);
+/* The RMS of the source f over the cells of `level` (both variables, the
+ * count ErrorAFAS's errortype 1 divides by): the denominator of the relative
+ * residual RMS(f - A(q)) / RMS(f) that the mixed-precision path reports beside
+ * the reference's absolute test (lib/afasroutines.f90:94-101). No counterpart
+ * in the reference. */
+int bsam_source_norm(bsam_ctx *ctx, int level, double *rms);
 /* qold = q on every level (CopyQToQold, lib/bsamroutines.f90:568-580) */
 int bsam_copy_qold(bsam_ctx *ctx);
 /* MultigridIterations, lib/afasroutines.f90:38-106. Writes the number of
@@ -297,7 +304,15 @@ int bsam_tune_get(const bsam_ctx *ctx, const char *key, double *value);
 enum {
   BSAM_PROF_RELAX = 0, BSAM_PROF_GHOST, BSAM_PROF_RESTRICT, BSAM_PROF_COARSE_GHOST,
   BSAM_PROF_COARSE_LOAD, BSAM_PROF_CORRECT, BSAM_PROF_FILLDOWN, BSAM_PROF_SOURCE,
-  BSAM_PROF_RESNORM, BSAM_PROF_TAIL, BSAM_PROF_NKINDS
+  BSAM_PROF_RESNORM, BSAM_PROF_TAIL,
+  BSAM_PROF_RELAX_FUSED,  /* relaxation launches that also apply the preceding
+                             SetGhost program (same bytes as RELAX) */
+  BSAM_PROF_COPY,         /* whole-pool copies (source restore, qold = q):
+                             16 B per element */
+  BSAM_PROF_DEFECT,       /* mixed precision: defect set-up (q, f read, qb, e, fe
+                             written: 52 B per ghost-extended cell) and q += e
+                             (40 B per interior cell) */
+  BSAM_PROF_NKINDS
 };
 int bsam_profile_begin(bsam_ctx *ctx, unsigned kind_mask);
 int bsam_profile_end(bsam_ctx *ctx);
@@ -310,6 +325,14 @@ int bsam_profile_get(const bsam_ctx *ctx, int kind, long long *launches, double 
  * per launch. Changes the fields as relaxation does. */
 int bsam_profile_kernel(bsam_ctx *ctx, int kind, int reps, double *us_per_launch,
                         double *bytes_per_launch);
+
+/* Device-to-device copy rate of `device`: `reps` copies of `bytes` bytes
+ * between two buffers of that size with the engine's copy kernel, timed with
+ * HIP events; *gbs = 2 * bytes * reps / time (read plus write) in GB/s. The
+ * measured ceiling a bandwidth-bound kernel is compared with beside the
+ * chip's specification. Allocates and frees the two buffers. No counterpart
+ * in the reference. */
+int bsam_copy_rate(int device, long long bytes, int reps, double *gbs);
 
 /* Introspection of the integer maps (valid on host-only contexts too).
  * Ghost-fill program of (level, ipass): number of ghost-cell updates, and

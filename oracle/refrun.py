@@ -12,6 +12,7 @@ youngest patch first).
 from __future__ import annotations
 
 import os
+import resource
 import subprocess
 import tempfile
 
@@ -19,6 +20,14 @@ import numpy as np
 
 REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HARNESS = os.path.join(REPO, "oracle", "_ref", "harness")
+
+
+def _big_stack():
+    """The reference keeps array temporaries on the stack (a 128^3 patch's
+    exceed the usual 8 MiB): run the harness with the stack limit raised to
+    the hard limit."""
+    _, hard = resource.getrlimit(resource.RLIMIT_STACK)
+    resource.setrlimit(resource.RLIMIT_STACK, (hard, hard))
 
 
 def available() -> bool:
@@ -72,7 +81,7 @@ def run(layout, arrays, ops, dt=1e-3, alpha=1.0, eps=0.03, nsmooth=2, maxv=20,
                 assert arr.shape == layout.field_shape(p, name), (p.id, name, arr.shape)
                 fh.write(arr.tobytes(order="F"))
     proc = subprocess.run([HARNESS, spec, fin, fout, fres], cwd=tmp,
-                          capture_output=True, text=True)
+                          capture_output=True, text=True, preexec_fn=_big_stack)
     if proc.returncode != 0:
         raise RuntimeError(f"harness failed: {proc.stdout}\n{proc.stderr}")
     raw = np.fromfile(fout, dtype=np.float64)

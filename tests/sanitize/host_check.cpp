@@ -95,7 +95,7 @@ int check_hierarchy(int nd, int root, const int bc[6]) {
   cfg.ndims = nd; cfg.nrvars = 2; cfg.mbc = 1; cfg.minlevel = minlevel;
   cfg.dt = 1e-3; cfg.eps2 = 9e-4; cfg.fc_alpha = 1; cfg.fc_shift = 0.5;
   cfg.fe_num = 1; cfg.fe_den = 4; cfg.nsmoothingpasses = 2; cfg.maxvcycles = 10;
-  cfg.qerrortol = 1e-8; cfg.precision = 0; cfg.device = -1;
+  cfg.qerrortol = 1e-8; cfg.precision = 0; cfg.device = -1; cfg.errortype = 1;
   bsam_ctx *c = bsam_ctx_create(&cfg);
   if (!c) return 1;
   for (int rep = 0; rep < 2; rep++) {   // the second commit reuses the first

@@ -34,11 +34,13 @@ def test_errors_are_reported_not_fatal():
     cfg = engine.Cfg(2, 2, 2, 0, 1e-3, 1e-3, 1, 0.5, 1, 4, 2, 10, 1e-8, 0, -1)
     assert not engine.lib().bsam_ctx_create(ctypes.byref(cfg))
     assert b"mbc" in engine.lib().bsam_last_error(None)
-    # ErrorAFAS knows errortype 1 and 2 (0 reads as 1)
-    cfg = engine.Cfg(2, 2, 1, 0, 1e-3, 1e-3, 1, 0.5, 1, 4, 2, 10, 1e-8, 0, -1, 3)
-    assert not engine.lib().bsam_ctx_create(ctypes.byref(cfg))
-    assert b"errortype" in engine.lib().bsam_last_error(None)
-    for et in (0, 1, 2):
+    # ErrorAFAS knows errortype 1 and 2 and, like the reference
+    # (lib/afasroutines.f90:143-146), refuses every other value
+    for bad in (0, 3, -1):
+        cfg = engine.Cfg(2, 2, 1, 0, 1e-3, 1e-3, 1, 0.5, 1, 4, 2, 10, 1e-8, 0, -1, bad)
+        assert not engine.lib().bsam_ctx_create(ctypes.byref(cfg))
+        assert b"errortype" in engine.lib().bsam_last_error(None)
+    for et in (1, 2):
         e = engine.Engine(lay, dt=1e-3, eps=0.03, device=-1, errortype=et)
         e.close()
 
