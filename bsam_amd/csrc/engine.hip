@@ -1310,18 +1310,18 @@ int bsam_error_flags_user(bsam_ctx *c, int level, double qtolerance) {
   API_END(c)
 }
 
-int bsam_after_step_stats(bsam_ctx *c, double *mass, double *energy) {
-  API_BEGIN(c)
-  c->need_device();
-  if (!mass || !energy) throw Error("null output pointer");
+// ---- after-step statistics ------------------------------------------------------
+// Tasks of about 4096 cells: runs of rows (2D) or planes (3D) of every patch
+// of levels finest..0 (level-list order) and of its parent section. `first`
+// gets, per patch of the list and part, the index of its first task.
+static void stats_tasks(bsam_ctx *c, std::vector<int> &list, std::vector<StatTask> &tasks,
+                        std::vector<int> &first) {
   const int nd = c->nd();
-  // tasks of about 4096 cells: runs of rows (2D) or planes (3D) of every
-  // patch of levels finest..0 (level-list order) and of its parent section
-  std::vector<int> list;
+  list.clear();
+  tasks.clear();
   for (int l = c->finest; l >= 0; l--)
     for (int p : c->L(l).patches) list.push_back(p);
-  std::vector<StatTask> tasks;
-  std::vector<int> first(list.size() * 2 + 1, 0);
+  first.assign(list.size() * 2 + 1, 0);
   for (size_t n = 0; n < list.size(); n++) {
     const HPatch &h = c->hp[list[n]];
     for (int part = 0; part < 2; part++) {
@@ -1336,27 +1336,24 @@ int bsam_after_step_stats(bsam_ctx *c, double *mass, double *energy) {
     }
   }
   first[2 * list.size()] = (int)tasks.size();
-  if (c->n_stats < (int)tasks.size()) {
-    HIPCK(hipStreamSynchronize(c->stream));
-    DeviceLock lock(g_device_lock);
-    if (c->d_stats) (void)hipFree(c->d_stats);
-    if (c->d_stats_list) (void)hipFree(c->d_stats_list);
-    c->d_stats = nullptr;
-    c->d_stats_list = nullptr;
-    c->n_stats = 0;
-    HIPCK(hipMalloc(&c->d_stats, sizeof(double) * 3 * tasks.size()));
-    HIPCK(hipMalloc(&c->d_stats_list, sizeof(StatTask) * tasks.size()));
-    c->n_stats = (int)tasks.size();
-  }
-  HIPCK(hipMemcpyAsync(c->d_stats_list, tasks.data(), sizeof(StatTask) * tasks.size(),
+}
+
+// Runs the task sums on the device with the task table at d_tasks and the
+// sums (3 per task) at d_out, both device memory of the context's device,
+// then combines them on the host: AfterStep2D/3D's accumulation, patch by
+// patch in the reference's order.
+static void stats_run(bsam_ctx *c, const std::vector<int> &list,
+                      const std::vector<StatTask> &tasks, const std::vector<int> &first,
+                      StatTask *d_tasks, double *d_out, double *mass, double *energy) {
+  const int nd = c->nd();
+  HIPCK(hipMemcpyAsync(d_tasks, tasks.data(), sizeof(StatTask) * tasks.size(),
                        hipMemcpyHostToDevice, c->stream));
   // minima of the double well whose convex and concave parts are Dfc and Dfe
   const double w = std::sqrt(c->cfg.fe_num / c->cfg.fe_den / c->cfg.fc_alpha);
   const double lo = c->cfg.fc_shift - w, hi = c->cfg.fc_shift + w;
-  HIPCK(launch_after_step(nd, c->dev(), c->d_stats_list, (int)tasks.size(), lo, hi,
-                          c->d_stats, c->stream));
+  HIPCK(launch_after_step(nd, c->dev(), d_tasks, (int)tasks.size(), lo, hi, d_out, c->stream));
   std::vector<double> t(3 * tasks.size());
-  HIPCK(hipMemcpyAsync(t.data(), c->d_stats, sizeof(double) * t.size(), hipMemcpyDeviceToHost,
+  HIPCK(hipMemcpyAsync(t.data(), d_out, sizeof(double) * t.size(), hipMemcpyDeviceToHost,
                        c->stream));
   HIPCK(hipStreamSynchronize(c->stream));
   // per patch and part: the task sums in row order
@@ -1365,7 +1362,6 @@ int bsam_after_step_stats(bsam_ctx *c, double *mass, double *energy) {
     for (int k = first[2 * n + part]; k < first[2 * n + part + 1]; k++)
       for (int v = 0; v < 3; v++) s[v] += t[3 * (size_t)k + v];
   };
-  // AfterStep2D/3D's accumulation, patch by patch in the reference's order
   double m = 0.0, e = 0.0;
   for (size_t n = 0; n < list.size(); n++) {
     const HPatch &h = c->hp[list[n]];
@@ -1391,6 +1387,73 @@ int bsam_after_step_stats(bsam_ctx *c, double *mass, double *energy) {
   }
   *mass = m;
   *energy = e;
+}
+
+int bsam_after_step_stats(bsam_ctx *c, double *mass, double *energy) {
+  API_BEGIN(c)
+  c->need_device();
+  if (!mass || !energy) throw Error("null output pointer");
+  std::vector<int> list, first;
+  std::vector<StatTask> tasks;
+  stats_tasks(c, list, tasks, first);
+  if (c->n_stats < (int)tasks.size()) {
+    HIPCK(hipStreamSynchronize(c->stream));
+    DeviceLock lock(g_device_lock);
+    if (c->d_stats) (void)hipFree(c->d_stats);
+    if (c->d_stats_list) (void)hipFree(c->d_stats_list);
+    c->d_stats = nullptr;
+    c->d_stats_list = nullptr;
+    c->n_stats = 0;
+    HIPCK(hipMalloc(&c->d_stats, sizeof(double) * 3 * tasks.size()));
+    HIPCK(hipMalloc(&c->d_stats_list, sizeof(StatTask) * tasks.size()));
+    c->n_stats = (int)tasks.size();
+  }
+  stats_run(c, list, tasks, first, c->d_stats_list, c->d_stats, mass, energy);
+  API_END(c)
+}
+
+// Allocation-free variant: the task table and the task sums live in
+// `scratch`, a file-scope device array that every GPU the library runs on
+// holds (its own copy, at its own address) until the process ends. Host code
+// never uses the array's name as a device address: per device, the first
+// call resolves the array's address with hipGetSymbolAddress (that device
+// current) and keeps it in g_scratch_ptr. Nothing in the scratch needs
+// clearing: every call writes the whole task table it uses and the kernel
+// writes every sum the call reads back. Calls on one GPU share the array;
+// g_scratch_mutex serialises them within this process (each process has its
+// own copy).
+constexpr int kStatsMaxTasks = BSAM_STATS_NOALLOC_MAX_TASKS;
+constexpr size_t kScratchTaskBytes = sizeof(StatTask) * (size_t)kStatsMaxTasks;
+constexpr size_t kScratchBytes = kScratchTaskBytes + sizeof(double) * 3 * (size_t)kStatsMaxTasks;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+constexpr int kMaxDevices = 64;
+static void *g_scratch_ptr[kMaxDevices] = {nullptr};
+static std::mutex g_scratch_mutex;
+
+int bsam_after_step_stats_noalloc(bsam_ctx *c, double *mass, double *energy) {
+  API_BEGIN(c)
+  c->need_device();
+  if (!mass || !energy) throw Error("null output pointer");
+  const int dev = c->cfg.device;
+  if (dev < 0 || dev >= kMaxDevices) throw Error("after_step_stats_noalloc: device ordinal out of range");
+  std::vector<int> list, first;
+  std::vector<StatTask> tasks;
+  stats_tasks(c, list, tasks, first);
+  if ((long long)tasks.size() > kStatsMaxTasks)
+    throw Error("after_step_stats_noalloc: the hierarchy needs " + std::to_string(tasks.size()) +
+                " statistics tasks, more than the " + std::to_string(kStatsMaxTasks) +
+                " its fixed scratch holds (use bsam_after_step_stats)");
+  std::lock_guard<std::mutex> guard(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    HIPCK(hipSetDevice(dev));
+    void *ptr = nullptr;
+    HIPCK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) throw Error("after_step_stats_noalloc: no device address for the scratch array");
+    g_scratch_ptr[dev] = ptr;
+  }
+  unsigned char *base = static_cast<unsigned char *>(g_scratch_ptr[dev]);
+  stats_run(c, list, tasks, first, reinterpret_cast<StatTask *>(base),
+            reinterpret_cast<double *>(base + kScratchTaskBytes), mass, energy);
   API_END(c)
 }
 

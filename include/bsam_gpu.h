@@ -151,6 +151,22 @@ int bsam_error_flags_user(bsam_ctx *ctx, int level, double qtolerance);
  * are reduced on the device in a fixed order and combined on the host in the
  * reference's order. Synchronises the stream. */
 int bsam_after_step_stats(bsam_ctx *ctx, double *mass, double *energy);
+/* The same statistics without allocating device memory. No counterpart in
+ * the reference: it is for callers that cannot allocate (tight serving
+ * loops that report the statistics per step, harnesses that forbid
+ * hipMalloc), where bsam_after_step_stats allocates its task table and task
+ * sums on first use and whenever a hierarchy needs more tasks. This variant
+ * keeps both in a fixed scratch array that every GPU the library runs on
+ * holds until the process ends (1.25 MiB), and calls no hipMalloc/hipFree.
+ * Limit: BSAM_STATS_NOALLOC_MAX_TASKS tasks, a task being a run of rows (2D)
+ * or planes (3D) of about 4096 cells of one patch of levels finest..0 or of
+ * its parent's section under it (14.2 M cells of 1963 3D patches take about
+ * 4000); a larger hierarchy is refused with an error. Calls on one GPU share
+ * the scratch; within one process they are serialised, so calls from
+ * several threads or contexts on one GPU do not interleave there. Results
+ * are bit-identical to bsam_after_step_stats. Synchronises the stream. */
+#define BSAM_STATS_NOALLOC_MAX_TASKS 32768
+int bsam_after_step_stats_noalloc(bsam_ctx *ctx, double *mass, double *energy);
 /* GetSourceFunction then UpdateSourceFunction on levels finest..0,
  * lib/afasroutines.f90:57-64 */
 int bsam_source(bsam_ctx *ctx);
