@@ -54,6 +54,10 @@ struct LevelData {
   int *d_patch_list = nullptr;  // the same on the device (regrid kernels)
   std::vector<Tile> t_int, t_ext, t_coarse, t_cext;
   Tile *d_int = nullptr, *d_ext = nullptr, *d_coarse = nullptr, *d_cext = nullptr;
+  // linear tiles (LIN_TILE cells each): the ghost-extended coarse box of the
+  // coarse ghost + loading launch, the 2x2(x2) fine blocks of the correction
+  std::vector<Tile> t_cgl, t_blk;
+  Tile *d_cgl = nullptr, *d_blk = nullptr;
   std::vector<GhostOp> gp[3];
   GhostOp *d_gp[3] = {nullptr, nullptr, nullptr};
   GhostOp *d_tgp[3] = {nullptr, nullptr, nullptr};  // patch-local copies (tail levels)
@@ -257,6 +261,7 @@ void bsam_ctx::free_device() {
   d_P = nullptr; d_partial = nullptr; d_hist = nullptr; d_state = nullptr;
   for (auto &l : lev) {
     F(l.d_int); F(l.d_ext); F(l.d_coarse); F(l.d_cext); F(l.d_patch_list);
+    F(l.d_cgl); F(l.d_blk);
     for (int p = 0; p < 3; p++) {
       F(l.d_gp[p]);
       F(l.d_tgp[p]);
@@ -332,6 +337,9 @@ static void build_tiles(bsam_ctx *c, LevelData &L, int level) {
       for (int j = lo; j < lo + ey; j += TY)
         for (int i = lo; i < lo + ex; i += TX) out.push_back(Tile{p, i, j, k});
   };
+  auto lin = [&](std::vector<Tile> &out, int p, long long n) {
+    for (long long e = 0; e < n; e += LIN_TILE) out.push_back(Tile{p, (int)e, 0, 0});
+  };
   for (int p : L.patches) {
     const HPatch &h = c->hp[p];
     cover(L.t_int, p, 1, h.mx[0], h.mx[1], h.mx[2]);
@@ -339,6 +347,8 @@ static void build_tiles(bsam_ctx *c, LevelData &L, int level) {
     if (level > c->cfg.minlevel) {
       cover(L.t_coarse, p, 1, h.cmx[0], h.cmx[1], h.cmx[2]);
       cover(L.t_cext, p, 0, h.cmx[0] + 2, h.cmx[1] + 2, h.cmx[2] + 2);
+      lin(L.t_cgl, p, (long long)(h.cmx[0] + 2) * (h.cmx[1] + 2) * (nd == 3 ? h.cmx[2] + 2 : 1));
+      lin(L.t_blk, p, (long long)(h.cmx[0] + 1) * (h.cmx[1] + 1) * (nd == 3 ? h.cmx[2] + 1 : 1));
     }
     L.cells += (long long)h.mx[0] * h.mx[1] * (nd == 3 ? h.mx[2] : 1);
   }
@@ -616,11 +626,9 @@ static void commit(bsam_ctx *c) {
       // only the reads of cells the parent's program writes matter to a ride
       std::vector<int> at;
       if (c->fuse_ghost && destination_index(PL.gp[0], at))
-        L.ride_cgl = plan_ride(PL.gp[0], coarse_ghost_load_reads(c->hp, L.t_cext, nd, c->tx(),
-                                                                 c->ty(), c->tz(), &at));
+        L.ride_cgl = plan_ride(PL.gp[0], coarse_ghost_load_reads(c->hp, L.t_cgl, nd, &at));
       if (c->fuse_ghost && destination_index(PL.gp[2], at))
-        L.ride_corr = plan_ride(PL.gp[2], correct_reads(c->hp, L.t_ext, nd, c->tx(), c->ty(),
-                                                        c->tz(), &at));
+        L.ride_corr = plan_ride(PL.gp[2], correct_reads(c->hp, L.t_blk, nd, &at));
     }
   }
   if (c->host_only) return;
@@ -651,6 +659,8 @@ static void commit(bsam_ctx *c) {
     L.d_ext = dev_upload(L.t_ext, s);
     L.d_coarse = dev_upload(L.t_coarse, s);
     L.d_cext = dev_upload(L.t_cext, s);
+    L.d_cgl = dev_upload(L.t_cgl, s);
+    L.d_blk = dev_upload(L.t_blk, s);
     for (int ip = 0; ip < 3; ip++) L.d_gp[ip] = dev_upload(L.gp[ip], s);
     for (int f = 0; f < 3; f++)
       if (L.fused[f].ok) {
@@ -788,7 +798,7 @@ static void op_coarse_ghost_load(bsam_ctx *c, int level, int prec = PREC_SOLUTIO
   LevelData &L = child_level(c, level);
   const LevelData::DRide *r = parent_ride(c, L, parent_ghost, level, prec, false);
   ProfScope ps(c, BSAM_PROF_COARSE_LOAD, 40.0 * L.cells);
-  HIPCK(launch_coarse_ghost_load(c->nd(), prec, c->dev(), L.d_cext, (int)L.t_cext.size(),
+  HIPCK(launch_coarse_ghost_load(c->nd(), prec, c->dev(), L.d_cgl, (int)L.t_cgl.size(),
                                  r ? r->rest : nullptr, r ? r->nrest : 0,
                                  r ? r->start : nullptr, r ? r->ops : nullptr, c->stream));
 }
@@ -797,7 +807,7 @@ static void op_correct(bsam_ctx *c, int level, int prec = PREC_SOLUTION_FP64,
   LevelData &L = child_level(c, level);
   const LevelData::DRide *r = parent_ride(c, L, parent_ghost, level, prec, true);
   ProfScope ps(c, BSAM_PROF_CORRECT, 40.0 * L.cells);
-  HIPCK(launch_correct(c->nd(), prec, c->dev(), L.d_ext, (int)L.t_ext.size(),
+  HIPCK(launch_correct(c->nd(), prec, c->dev(), L.d_blk, (int)L.t_blk.size(),
                        r ? r->rest : nullptr, r ? r->nrest : 0, r ? r->start : nullptr,
                        r ? r->ops : nullptr, c->stream));
 }

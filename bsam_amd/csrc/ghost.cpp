@@ -15,6 +15,8 @@
 // property and refuses programs that violate it.
 #include "ghost.hpp"
 
+#include <climits>
+
 #include <algorithm>
 #include <array>
 #include <cstring>
@@ -868,40 +870,53 @@ static bool parent_interior(const std::vector<HPatch> &hp, const HPatch &h, int 
   return true;
 }
 
+// The cells (i, j, k) of linear tile `tl` over a box of extents e[0..2]
+// (e[2] = 1 in 2D), in order.
+template <class F>
+static void for_lin_cells(const Tile &tl, const int e[3], F f) {
+  const long long n = (long long)e[0] * e[1] * e[2];
+  for (long long x = tl.i0; x < std::min<long long>(tl.i0 + LIN_TILE, n); x++)
+    f((int)(x % e[0]), (int)((x / e[0]) % e[1]), (int)(x / ((long long)e[0] * e[1])));
+}
+
 std::vector<std::vector<long long>> coarse_ghost_load_reads(const std::vector<HPatch> &hp,
                                                             const std::vector<Tile> &tiles,
-                                                            int nd, int tx, int ty, int tz,
-                                                            const std::vector<int> *only) {
+                                                            int nd, const std::vector<int> *only) {
   std::vector<std::vector<long long>> out(tiles.size());
   for (size_t t = 0; t < tiles.size(); t++) {
     const Tile &tl = tiles[t];
     const HPatch &h = hp[tl.patch];
+    const int box[3] = {h.cmx[0] + 2, h.cmx[1] + 2, nd == 3 ? h.cmx[2] + 2 : 1};
     const int e[3] = {h.cmx[0] + 1, h.cmx[1] + 1, nd == 3 ? h.cmx[2] + 1 : 0};
     auto ring = [&](int a, int b, int c) {
       return a == 0 || a == e[0] || b == 0 || b == e[1] || (nd == 3 && (c == 0 || c == e[2]));
     };
     std::vector<long long> &r = out[t];
     if (only) {
-      // every cell read lies within one cell of the tile
-      int lo[3] = {tl.i0 - 1, tl.j0 - 1, nd == 3 ? tl.k0 - 1 : 0};
-      int hi[3] = {std::min(tl.i0 + tx, e[0]), std::min(tl.j0 + ty, e[1]),
-                   nd == 3 ? std::min(tl.k0 + tz, e[2]) : 0};
+      // every cell read lies within one cell of the tile's cells
+      int lo[3] = {INT_MAX, INT_MAX, nd == 3 ? INT_MAX : 0}, hi[3] = {INT_MIN, INT_MIN, 0};
+      for_lin_cells(tl, box, [&](int i, int j, int k) {
+        const int v[3] = {i, j, k};
+        for (int d = 0; d < nd; d++) {
+          lo[d] = std::min(lo[d], v[d] - 1);
+          hi[d] = std::max(hi[d], std::min(v[d] + 1, e[d]));
+        }
+      });
       if (parent_interior(hp, h, nd, lo, hi)) continue;
     }
-    for (int k = tl.k0; k < tl.k0 + (nd == 3 ? tz : 1) && k <= e[2]; k++)
-      for (int j = tl.j0; j < tl.j0 + ty && j <= e[1]; j++)
-        for (int i = tl.i0; i < tl.i0 + tx && i <= e[0]; i++)
-          // the cell itself (a ring cell copies the parent) and its stencil
-          // neighbours on the ring (read from the parent by interior cells)
-          for (int s = 0; s < 2 * nd + 1; s++) {
-            const int d[7][3] = {{0, 0, 0}, {1, 0, 0}, {-1, 0, 0}, {0, 1, 0},
-                                 {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
-            int a = i + d[s][0], b = j + d[s][1], c = k + d[s][2];
-            if (a < 0 || a > e[0] || b < 0 || b > e[1] || c < 0 || c > e[2]) continue;
-            if (!ring(a, b, c)) continue;
-            long long off = parent_offset(hp, h, nd, a, b, c);
-            if (listed(only, off)) r.push_back(off);
-          }
+    for_lin_cells(tl, box, [&](int i, int j, int k) {
+      // the cell itself (a ring cell copies the parent) and its stencil
+      // neighbours on the ring (read from the parent by interior cells)
+      for (int s = 0; s < 2 * nd + 1; s++) {
+        const int d[7][3] = {{0, 0, 0}, {1, 0, 0}, {-1, 0, 0}, {0, 1, 0},
+                             {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+        int a = i + d[s][0], b = j + d[s][1], c = k + d[s][2];
+        if (a < 0 || a > e[0] || b < 0 || b > e[1] || c < 0 || c > e[2]) continue;
+        if (!ring(a, b, c)) continue;
+        long long off = parent_offset(hp, h, nd, a, b, c);
+        if (listed(only, off)) r.push_back(off);
+      }
+    });
     std::sort(r.begin(), r.end());
     r.erase(std::unique(r.begin(), r.end()), r.end());
   }
@@ -910,32 +925,33 @@ std::vector<std::vector<long long>> coarse_ghost_load_reads(const std::vector<HP
 
 std::vector<std::vector<long long>> correct_reads(const std::vector<HPatch> &hp,
                                                   const std::vector<Tile> &tiles, int nd,
-                                                  int tx, int ty, int tz,
                                                   const std::vector<int> *only) {
   std::vector<std::vector<long long>> out(tiles.size());
   for (size_t t = 0; t < tiles.size(); t++) {
     const Tile &tl = tiles[t];
     const HPatch &h = hp[tl.patch];
-    const int e[3] = {h.mx[0] + 1, h.mx[1] + 1, nd == 3 ? h.mx[2] + 1 : 0};
+    const int box[3] = {h.cmx[0] + 1, h.cmx[1] + 1, nd == 3 ? h.cmx[2] + 1 : 1};
     std::vector<long long> &r = out[t];
     if (only) {
-      int lo[3] = {tl.i0 >> 1, tl.j0 >> 1, nd == 3 ? tl.k0 >> 1 : 0};
-      int hi[3] = {(std::min(tl.i0 + tx - 1, e[0]) >> 1) + 1,
-                   (std::min(tl.j0 + ty - 1, e[1]) >> 1) + 1,
-                   nd == 3 ? (std::min(tl.k0 + tz - 1, e[2]) >> 1) + 1 : 0};
+      int lo[3] = {INT_MAX, INT_MAX, nd == 3 ? INT_MAX : 0}, hi[3] = {INT_MIN, INT_MIN, 0};
+      for_lin_cells(tl, box, [&](int m, int n, int o) {
+        const int v[3] = {m, n, o};
+        for (int d = 0; d < nd; d++) {
+          lo[d] = std::min(lo[d], v[d]);
+          hi[d] = std::max(hi[d], v[d] + 1);
+        }
+      });
       if (parent_interior(hp, h, nd, lo, hi)) continue;
     }
-    // fine cell (i, j, k) reads parent cells i/2 + {0, 1} per dimension
-    for (int k = tl.k0; k < tl.k0 + (nd == 3 ? tz : 1) && k <= e[2]; k++)
-      for (int j = tl.j0; j < tl.j0 + ty && j <= e[1]; j++)
-        for (int i = tl.i0; i < tl.i0 + tx && i <= e[0]; i++)
-          for (int c = 0; c < (nd == 3 ? 2 : 1); c++)
-            for (int b = 0; b < 2; b++)
-              for (int a = 0; a < 2; a++) {
-                long long off = parent_offset(hp, h, nd, (i >> 1) + a, (j >> 1) + b,
-                                              nd == 3 ? (k >> 1) + c : 0);
-                if (listed(only, off)) r.push_back(off);
-              }
+    // block (m, n, o) reads parent cells m + {0, 1} per dimension
+    for_lin_cells(tl, box, [&](int m, int n, int o) {
+      for (int c = 0; c < (nd == 3 ? 2 : 1); c++)
+        for (int b = 0; b < 2; b++)
+          for (int a = 0; a < 2; a++) {
+            long long off = parent_offset(hp, h, nd, m + a, n + b, nd == 3 ? o + c : 0);
+            if (listed(only, off)) r.push_back(off);
+          }
+    });
     std::sort(r.begin(), r.end());
     r.erase(std::unique(r.begin(), r.end()), r.end());
   }

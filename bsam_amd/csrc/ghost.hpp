@@ -50,17 +50,20 @@ struct RideGhost {
   std::vector<GhostOp> ops;
   std::vector<GhostOp> rest;
 };
+// Linear tiles: LIN_TILE consecutive cells (x fastest) of a box of one
+// patch, the first at cell Tile::i0 of the box.
+constexpr int LIN_TILE = 256;
 // Parent q-pool offsets (variable 1) that each tile of the child level's
-// coarse ghost + loading launch (ghost-extended coarse tiles, `tiles`) or
-// correction launch (ghost-extended fine tiles) may read, ascending; with
-// `only`, just the offsets off with only[off] >= 0.
+// coarse ghost + loading launch (linear tiles over the ghost-extended coarse
+// box) or correction launch (linear tiles over the 2x2(x2) fine blocks
+// 0..cmx) may read, ascending; with `only`, just the offsets off with
+// only[off] >= 0.
 std::vector<std::vector<long long>> coarse_ghost_load_reads(const std::vector<HPatch> &hp,
                                                             const std::vector<Tile> &tiles,
-                                                            int ndims, int tx, int ty, int tz,
+                                                            int ndims,
                                                             const std::vector<int> *only = nullptr);
 std::vector<std::vector<long long>> correct_reads(const std::vector<HPatch> &hp,
                                                   const std::vector<Tile> &tiles, int ndims,
-                                                  int tx, int ty, int tz,
                                                   const std::vector<int> *only = nullptr);
 // at[off] = the update of `prog` that writes q-pool offset off, -1 where
 // none; false if two updates share a destination.

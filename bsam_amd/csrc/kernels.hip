@@ -627,6 +627,21 @@ __device__ __forceinline__ void tile_color_cell(const Dev &d, const Tile &t, int
   int p = (j + k + color + 1) & 1;   // parity of the colour's i in this row
   i = t.i0 + ((p - t.i0) & 1) + 2 * ti;
 }
+// Linear tiles: tile t covers LIN_TILE consecutive cells (x fastest, then y,
+// z) of an ex x ey (x ez) box of one patch, starting at cell t.i0 of the
+// box, so a launch over a level's boxes keeps every lane busy but those of
+// each box's last tile. Returns false for lanes past the box.
+template <int ND>
+__device__ __forceinline__ bool lin_cell(const Tile &t, int ex, int ey, int ez, int &i, int &j,
+                                         int &k) {
+  const int e = t.i0 + (int)threadIdx.x;
+  if (e >= ex * ey * (ND == 3 ? ez : 1)) return false;
+  i = e % ex;
+  const int r = e / ex;
+  j = ND == 3 ? r % ey : r;
+  k = ND == 3 ? r / ey : 0;
+  return true;
+}
 template <int ND>
 __device__ __forceinline__ bool in_box(int i, int j, int k, int ex, int ey, int ez) {
   return i <= ex && j <= ey && (ND == 2 || k <= ez);
@@ -757,9 +772,10 @@ __global__ __launch_bounds__(256) void k_coarse_load(Dev d, const Tile *tiles, i
   coarse_load_cell<ND, T, DEF>(d.prm, C, level_ref<T>(d, C.P.parent), i, j, k);
 }
 
-// Tiles cover the ghost-extended coarse box 0..cmx+1. A SetGhost program of
-// the parent may ride along: gops[0..ng) grid-stride, and per tile the updates
-// it reads (tstart/tops; tstart null for none).
+// Linear tiles over the ghost-extended coarse box 0..cmx+1 (LIN_TILE cells
+// each). A SetGhost program of the parent may ride along: gops[0..ng)
+// grid-stride, and per tile the updates it reads (tstart/tops; tstart null
+// for none).
 template <int ND, class T, bool DEF>
 __global__ __launch_bounds__(256) void k_coarse_ghost_load(Dev d, const Tile *tiles, int nt,
                                                            const GhostOp *gops, int ng,
@@ -772,8 +788,7 @@ __global__ __launch_bounds__(256) void k_coarse_ghost_load(Dev d, const Tile *ti
     ghost_tile<T>(d, s0, s1, tops);
     Lvl<T> C = level_ref<T>(d, t.patch);
     int i, j, k;
-    tile_cell<ND>(d, t, i, j, k);
-    if (in_box<ND>(i, j, k, C.P.cmx + 1, C.P.cmy + 1, C.P.cmz + 1))
+    if (lin_cell<ND>(t, C.P.cmx + 2, C.P.cmy + 2, C.P.cmz + 2, i, j, k))
       coarse_ghost_load_cell<ND, T, DEF>(d.prm, d.nrvars, C, level_ref<T>(d, C.P.parent), i, j,
                                          k);
   }
@@ -781,8 +796,10 @@ __global__ __launch_bounds__(256) void k_coarse_ghost_load(Dev d, const Tile *ti
   ghost_alongside<T>(d, gops, ng);
 }
 
-// Tiles cover the ghost-extended fine box 0..mx+1. A SetGhost program of the
-// parent may ride along as in k_coarse_ghost_load.
+// Linear tiles over the 2x2(x2) fine blocks (m, n(, o)) = 0..cmx of the
+// ghost-extended fine box 0..mx+1: each thread corrects the 2^d fine cells of
+// one block, which share their coarse corrections (correct_block). A SetGhost
+// program of the parent may ride along as in k_coarse_ghost_load.
 template <int ND, class T>
 __global__ __launch_bounds__(256) void k_correct(Dev d, const Tile *tiles, int nt,
                                                  const GhostOp *gops, int ng, const int *tstart,
@@ -793,10 +810,9 @@ __global__ __launch_bounds__(256) void k_correct(Dev d, const Tile *tiles, int n
     const int s0 = tstart ? tstart[b] : 0, s1 = tstart ? tstart[b + 1] : 0;
     ghost_tile<T>(d, s0, s1, tops);
     Lvl<T> C = level_ref<T>(d, t.patch);
-    int i, j, k;
-    tile_cell<ND>(d, t, i, j, k);
-    if (in_box<ND>(i, j, k, C.P.mx + 1, C.P.my + 1, C.P.mz + 1))
-      correct_cell<ND, T>(d.nrvars, C, level_ref<T>(d, C.P.parent), i, j, k);
+    int m, n, o;
+    if (lin_cell<ND>(t, C.P.cmx + 1, C.P.cmy + 1, C.P.cmz + 1, m, n, o))
+      correct_block<ND, T>(d.nrvars, C, level_ref<T>(d, C.P.parent), m, n, o);
   }
   // the updates no tile reads, after the work (they touch nothing it uses)
   ghost_alongside<T>(d, gops, ng);
