@@ -33,14 +33,14 @@ struct DPatch {
 // the same program never writes, so all updates of a program run
 // concurrently. kind: 0 copy, 1 and 2 two fine + coarse quadratic
 // interpolation with a single coarse value or a 3-point coarse interpolant,
-// 3 and 4 the 3D face forms (two fine + bilinear coarse, two fine + four
-// coarse with 3-point rows). nflip counts trailing "2 - x" boundary maps.
+// 3 the 3D face form (two fine + bilinear coarse, four values). nflip counts
+// trailing "2 - x" boundary maps. 48 bytes.
 struct GhostOp {
   int dst, dvs;            // q-pool offset (variable 1) and variable stride
   int kind, nflip;
   int f0, f1, fvs;         // fine (or copy) sources and their variable stride
   int cvs;                 // parent variable stride
-  int c[8];                // parent sources
+  int c[4];                // parent sources
 };
 
 // A tile of cells of one patch: origin (i0, j0, k0) in the patch's index

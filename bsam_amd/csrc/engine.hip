@@ -69,6 +69,7 @@ struct LevelData {
   bool down_ok[3] = {false, false, false};
   int *d_fstart[3] = {nullptr, nullptr, nullptr};
   GhostOp *d_fops[3] = {nullptr, nullptr, nullptr};
+  GhostOp *d_fpre[3] = {nullptr, nullptr, nullptr};   // launched before the fused relaxation
   // the parent's SetGhost(0) inside this level's coarse ghost + loading
   // launch, and its SetGhost(2) (the last of post-smoothing) inside this
   // level's correction launch
@@ -267,6 +268,7 @@ void bsam_ctx::free_device() {
       F(l.d_tgp[p]);
       F(l.d_fstart[p]);
       F(l.d_fops[p]);
+      F(l.d_fpre[p]);
     }
     for (LevelData::DRide *r : {&l.d_ride_cgl, &l.d_ride_corr}) {
       F(r->rest);
@@ -666,6 +668,7 @@ static void commit(bsam_ctx *c) {
       if (L.fused[f].ok) {
         L.d_fstart[f] = dev_upload(L.fused[f].start, s);
         L.d_fops[f] = dev_upload(L.fused[f].ops, s);
+        L.d_fpre[f] = dev_upload(L.fused[f].pre, s);
       }
     auto up_ride = [&](const RideGhost &r, LevelData::DRide &d) {
       if (!r.ok) return;
@@ -860,7 +863,8 @@ static void op_source(bsam_ctx *c, int full) {
 // The smoothing passes of AFASVcycle (afasroutines.f90:493-497, 512-516): per
 // pass, relax colour 1, SetGhost(1), relax colour 2, SetGhost(2). With
 // ghost0_first, a SetGhost(level, 0) precedes them. Each SetGhost is applied
-// inside the following relaxation launch where the level's fused plan allows.
+// inside the following relaxation launch where the level's fused plan allows
+// (its updates that read cells other tiles relax in a launch just before).
 // With defer_last the final SetGhost is not launched but returned (its ipass)
 // for the caller to apply.
 static int level_relax(bsam_ctx *c, int level, int prec = PREC_SOLUTION_FP64,
@@ -871,6 +875,11 @@ static int level_relax(bsam_ctx *c, int level, int prec = PREC_SOLUTION_FP64,
     for (int color = 1; color <= 2; color++) {
       int f = pending >= 0 ? fused_index(pending, color) : -1;
       if (f >= 0 && c->fuse_ghost && L.fused[f].ok) {
+        if (!L.fused[f].pre.empty()) {
+          ProfScope pg(c, BSAM_PROF_GHOST, 32.0 * L.fused[f].pre.size());
+          HIPCK(launch_ghost(prec, c->dev(), L.d_fpre[f], (int)L.fused[f].pre.size(),
+                             c->stream));
+        }
         ProfScope ps(c, BSAM_PROF_RELAX_FUSED,
                      (prec == PREC_SOLUTION_FP64 ? 24.0 : 12.0) * L.cells);
         HIPCK(launch_relax_fused(c->nd(), prec, c->dev(), L.d_int, (int)L.t_int.size(), color,
@@ -2038,6 +2047,15 @@ int bsam_fused_ghost_size(const bsam_ctx *cc, int level, int ipass, int color) {
   return c->L(level).fused[f].kept;
 }
 
+int bsam_fused_ghost_pre_size(const bsam_ctx *cc, int level, int ipass, int color) {
+  bsam_ctx *c = const_cast<bsam_ctx *>(cc);
+  if (!c || !c->committed) return -1;
+  if (level < c->cfg.minlevel || level > c->finest) return -1;
+  int f = fused_index(ipass, color);
+  if (f < 0 || !c->L(level).fused[f].ok) return -1;
+  return (int)c->L(level).fused[f].pre.size();
+}
+
 int bsam_ghost_rides_down(const bsam_ctx *cc, int level, int ipass) {
   bsam_ctx *c = const_cast<bsam_ctx *>(cc);
   if (!c || !c->committed || ipass < 0 || ipass > 2) return -1;
@@ -2054,7 +2072,11 @@ int bsam_fused_ghost_apply_host(const bsam_ctx *cc, int level, int ipass, int co
   if (f < 0) throw Error("no fused form of this ipass and colour");
   const FusedGhost &g = c->L(level).fused[f];
   if (!g.ok) throw Error("this transition is not fused on this level");
-  apply_ghost_program_host(c->hp, g.ops, c->cfg.nrvars, q_patches);
+  // the updates launched before the relaxation and those applied inside it
+  // (no update reads what another writes, so one combined application)
+  std::vector<GhostOp> all(g.pre);
+  all.insert(all.end(), g.ops.begin(), g.ops.end());
+  apply_ghost_program_host(c->hp, all, c->cfg.nrvars, q_patches);
   API_END(c)
 }
 

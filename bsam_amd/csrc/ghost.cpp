@@ -805,6 +805,7 @@ FusedGhost plan_fused_ghost(const std::vector<HPatch> &hp, const std::vector<Til
     bool read = outside == 1 && relaxed(c[0], c[1], nd == 3 ? c[2] : 0);
     if (!read && (size_t)g.dst < next_dst.size() && next_dst[(size_t)g.dst]) continue;
     int owner = tile_of(p, c[0], c[1], c[2]);
+    bool crosses = false;
     long long srcs[2] = {g.f0, g.f1};
     int nsrc = g.kind == 0 ? 1 : 2;
     for (int s2 = 0; s2 < nsrc; s2++) {
@@ -816,19 +817,19 @@ FusedGhost plan_fused_ghost(const std::vector<HPatch> &hp, const std::vector<Til
       }
       if (first[sp] < 0 || !inside(sp, si, sj, sk)) continue;   // not relaxed here
       if (!relaxed(si, sj, nd == 3 ? sk : 0)) continue;
-      if (tile_of(sp, si, sj, sk) != owner) {
-        why = "an update reads a cell relaxed by another tile";
-        return out;
-      }
+      if (tile_of(sp, si, sj, sk) != owner) crosses = true;
     }
-    per[owner].push_back(g);
+    // an update that reads a cell another tile relaxes in the same launch
+    // runs in a launch of its own just before
+    if (crosses) out.pre.push_back(g);
+    else per[owner].push_back(g);
   }
   out.start.assign(tiles.size() + 1, 0);
   for (size_t t = 0; t < tiles.size(); t++) {
     out.start[t + 1] = out.start[t] + (int)per[t].size();
     out.ops.insert(out.ops.end(), per[t].begin(), per[t].end());
   }
-  out.kept = (int)out.ops.size();
+  out.kept = (int)(out.ops.size() + out.pre.size());
   out.ok = true;
   return out;
 }

@@ -76,14 +76,18 @@ RideGhost plan_ride(const std::vector<GhostOp> &prog,
 // applies them (ops[start[t] .. start[t+1]) for tile t of the level's
 // interior tile list). Updates whose destination the relaxation does not read
 // and that the next program, SetGhost(level, color), rewrites are dropped
-// (their value is never observed). Built only if every kept update reads no
-// cell of colour `color` outside its own tile, so the tiles of one launch
-// never race; `why` says otherwise.
+// (their value is never observed). A kept update that reads a cell of colour
+// `color` outside its own tile goes to `pre`, a program of its own the
+// engine launches just before the fused relaxation (it then reads the cells
+// before they are relaxed, as SetGhost before RelaxPatch does); the updates
+// in `ops` read nothing another tile of the launch writes, so the tiles never
+// race.
 struct FusedGhost {
   bool ok = false;
-  int kept = 0;
+  int kept = 0;                // ops + pre
   std::vector<int> start;
   std::vector<GhostOp> ops;
+  std::vector<GhostOp> pre;
 };
 // tiles: interior tiles of the level (origin i0/j0/k0 = 1 + multiples of the
 // tile extents tx, ty, tz; k0 = 0 in 2D), in patch order, k, j, i nested.

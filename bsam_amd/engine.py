@@ -176,6 +176,7 @@ def lib():
         L.bsam_ghost_program_apply_host.argtypes = [vp, C.c_int, C.c_int,
                                                     C.POINTER(C.POINTER(C.c_double))]
         L.bsam_fused_ghost_size.argtypes = [vp, C.c_int, C.c_int, C.c_int]
+        L.bsam_fused_ghost_pre_size.argtypes = [vp, C.c_int, C.c_int, C.c_int]
         L.bsam_ghost_rides_down.argtypes = [vp, C.c_int, C.c_int]
         L.bsam_fused_ghost_apply_host.argtypes = [vp, C.c_int, C.c_int, C.c_int,
                                                   C.POINTER(C.POINTER(C.c_double))]

@@ -362,6 +362,10 @@ int bsam_ghost_program_dst(const bsam_ctx *ctx, int level, int ipass, int *out,
  * application to host arrays. Updates the relaxation does not read and the
  * next SetGhost(level, color) rewrites are left out. */
 int bsam_fused_ghost_size(const bsam_ctx *ctx, int level, int ipass, int color);
+/* Of those, the updates that read a cell another tile of the relaxation
+ * relaxes: they run in a launch of their own just before it (-1 when the
+ * transition is not fused). */
+int bsam_fused_ghost_pre_size(const bsam_ctx *ctx, int level, int ipass, int color);
 /* 1 if SetGhost(level, ipass) reads none of the parent cells that the
  * level's restriction and fill-down write, so the engine runs it in the same
  * launch as those; 0 if not; -1 for bad arguments. */
