@@ -6,6 +6,61 @@
 
 namespace bsam {
 
+// Layout of every field array on the device. Each row (x the fastest
+// index) of a patch array is split by the parity of the x index: the row's
+// cells of even index first, then those of odd index. With rows of n cells
+// (h = ceil(n/2) per parity, a row taking 2*h cells, one padding slot when n
+// is odd), cell (i, j, k) of a plane of ny rows sits at
+// (i & 1)*h + (i >> 1) + 2*h*j + 2*h*ny*k. Ghost-extended arrays (q, qold,
+// qc, e, qb) index rows from the ghost (i = 0..n-1); interior arrays (f,
+// ftmp, rf, fe) use i - 1 for i = 1..mx. All cells of one red-black colour in
+// a row share the parity of i, so a half-sweep of RelaxGrid reads and writes
+// whole half-rows instead of every other element. Arrays cross the ABI in the
+// reference layout and are permuted on the host (to_device / from_device).
+struct Split {
+  long long xo, rs, pl;   // offset of the odd half of a row, row stride, plane stride
+};
+__host__ __device__ __forceinline__ Split split_of(long long n, long long ny) {
+  const long long h = (n + 1) >> 1;
+  return Split{h, 2 * h, 2 * h * ny};
+}
+__host__ __device__ __forceinline__ long long scell(const Split &s, int i, int j, int k) {
+  return (i & 1) * s.xo + (i >> 1) + s.rs * j + s.pl * k;
+}
+// Offset of ghost-extended cell (i, j, k) (lower bound 0) of an array with
+// rows of px cells and py rows per plane.
+__host__ __device__ __forceinline__ long long qcell(long long px, long long py, int i, int j,
+                                                   int k) {
+  return scell(split_of(px, py), i, j, k);
+}
+// The cell (i, j, k) at offset r of such an array (inverse of qcell).
+inline void qcell_index(long long r, long long px, long long py, int &i, int &j, int &k) {
+  const Split s = split_of(px, py);
+  k = (int)(r / s.pl);
+  const long long m = r % s.pl;
+  j = (int)(m / s.rs);
+  const long long x = m % s.rs;
+  i = (int)(x < s.xo ? 2 * x : 2 * (x - s.xo) + 1);
+}
+// Between the reference layout (planes of n*ny cells, `planes` of them, the
+// variables' planes following each other) and the device layout.
+inline void to_device(const double *in, double *out, long long n, long long ny,
+                      long long planes) {
+  const Split s = split_of(n, ny);
+  for (long long p = 0; p < planes; p++)
+    for (long long j = 0; j < ny; j++)
+      for (long long i = 0; i < n; i++)
+        out[p * s.pl + scell(s, (int)i, (int)j, 0)] = in[p * n * ny + i + n * j];
+}
+inline void from_device(const double *in, double *out, long long n, long long ny,
+                        long long planes) {
+  const Split s = split_of(n, ny);
+  for (long long p = 0; p < planes; p++)
+    for (long long j = 0; j < ny; j++)
+      for (long long i = 0; i < n; i++)
+        out[p * n * ny + i + n * j] = in[p * s.pl + scell(s, (int)i, (int)j, 0)];
+}
+
 // Device record of one patch. Field storage keeps the reference layout
 // (column-major, variable slowest, ghost-extended q with lower bound 0 for
 // mbc = 1), so uploads and downloads are plain copies; offsets index the
@@ -100,12 +155,14 @@ struct StatTask {
 };
 
 // A box of cells copied between two q pools (TransferValues): nx*ny*nz cells
-// of each of nv variables, from src + i*1 + j*ssy + k*ssz + v*ssv to the
-// same place relative to dst with the destination's strides.
+// of each of nv variables; cell (i, j, k) of the box, variable v, is the
+// source array's cell (si + i, sj + j, sk + k) at s + v*ssv (array of rows of
+// spx cells, spy rows per plane), and likewise in the destination.
 struct BoxCopy {
-  long long s, d;
-  long long ssy, ssz, ssv, dsy, dsz, dsv;
+  long long s, d, ssv, dsv;
   int nx, ny, nz, nv;
+  int si, sj, sk, spx, spy;
+  int di, dj, dk, dpx, dpy;
 };
 
 // Which field set a level operation acts on.

@@ -562,9 +562,10 @@ static void commit(bsam_ctx *c) {
       D.mb[d][0] = h.mb[d][0];
       D.mb[d][1] = h.mb[d][1];
     }
-    D.q = c->nq; D.qvs = ex * ey * ez;
-    D.qc = c->nc; D.cvs = cx * cy * cz;
-    D.f = c->nf; D.fvs = fn;
+    // device planes of rows of n cells take 2 * ceil(n / 2) cells per row
+    D.q = c->nq; D.qvs = split_of(ex, ey).pl * ez;
+    D.qc = c->nc; D.cvs = split_of(cx, cy).pl * cz;
+    D.f = c->nf; D.fvs = split_of(h.mx[0], h.mx[1]).pl * (nd == 3 ? h.mx[2] : 1);
     {
       // qrte(1:cmx) with cmx = mx/2, or 1 for odd mx (AllocFields)
       long long r = 1;
@@ -621,7 +622,7 @@ static void commit(bsam_ctx *c) {
         for (int k = (nd == 3 ? h.mb[2][0] : 0); k <= (nd == 3 ? h.mb[2][1] : 0); k++)
           for (int j = h.mb[1][0]; j <= h.mb[1][1]; j++)
             for (int i = h.mb[0][0]; i <= h.mb[0][1]; i++)
-              written[(size_t)(pa.qoff + i + pa.px * (j + (long long)pa.py * k))] = 1;
+              written[(size_t)(pa.qoff + qcell(pa.px, pa.py, i, j, k))] = 1;
       }
       for (int ip = 0; ip < 3; ip++) L.down_ok[ip] = program_reads_avoid(L.gp[ip], written);
       const LevelData &PL = c->lev[l - 1 - minl];
@@ -1229,21 +1230,43 @@ long long bsam_footprint_bytes(const bsam_ctx *c) {
   return b;
 }
 
+// A patch's field: its pool and offset, its length in the reference layout,
+// and its plane shape (rows of rx cells, ry rows per plane, `planes` planes
+// over all variables; rx = 0 for arrays the device keeps in the reference
+// layout). Device planes take split_of(rx, ry).pl cells.
+struct FieldShape {
+  long long rx = 0, ry = 0, planes = 0;
+};
 static void field_ptr(bsam_ctx *c, int patch, int field, double **base, long long *off,
-                      long long *n) {
+                      long long *n, FieldShape *shape = nullptr) {
   if (patch < 0 || patch >= (int)c->hp.size()) throw Error("patch index out of range");
   const DPatch &D = c->dp[patch];
-  int nv = c->cfg.nrvars;
+  const int nv = c->cfg.nrvars, nd = c->nd();
+  FieldShape fs;
   switch (field) {
-    case BSAM_FIELD_Q: *base = c->q; *off = D.q; *n = D.qvs * nv; break;
-    case BSAM_FIELD_QOLD: *base = c->qold; *off = D.q; *n = D.qvs * nv; break;
-    case BSAM_FIELD_QC: *base = c->qc; *off = D.qc; *n = D.cvs * nv; break;
-    case BSAM_FIELD_F: *base = c->f; *off = D.f; *n = D.fvs * nv; break;
-    case BSAM_FIELD_FTMP: *base = c->ftmp; *off = D.f; *n = D.fvs * nv; break;
-    case BSAM_FIELD_RF: *base = c->rf; *off = D.f; *n = D.fvs * nv; break;
+    case BSAM_FIELD_Q:
+    case BSAM_FIELD_QOLD:
+      *base = field == BSAM_FIELD_Q ? c->q : c->qold;
+      *off = D.q;
+      fs = FieldShape{D.px, D.py, (nd == 3 ? D.mz + 2 : 1) * (long long)nv};
+      break;
+    case BSAM_FIELD_QC:
+      *base = c->qc;
+      *off = D.qc;
+      fs = FieldShape{D.cmx + 2, D.cmy + 2, (nd == 3 ? D.cmz + 2 : 1) * (long long)nv};
+      break;
+    case BSAM_FIELD_F:
+    case BSAM_FIELD_FTMP:
+    case BSAM_FIELD_RF:
+      *base = field == BSAM_FIELD_F ? c->f : field == BSAM_FIELD_FTMP ? c->ftmp : c->rf;
+      *off = D.f;
+      fs = FieldShape{D.mx, D.my, (long long)D.mz * nv};
+      break;
     case BSAM_FIELD_QRTE: *base = c->qrte; *off = D.qr; *n = D.rvs * nv; break;
     default: throw Error("unknown field");
   }
+  if (fs.rx > 0) *n = fs.rx * fs.ry * fs.planes;
+  if (shape) *shape = fs;
 }
 
 long bsam_field_len(const bsam_ctx *cc, int patch, int field) {
@@ -1264,10 +1287,21 @@ int bsam_upload(bsam_ctx *c, int patch, int field, const double *host, long n) {
   c->need_device();
   double *b;
   long long off, len;
-  field_ptr(c, patch, field, &b, &off, &len);
+  FieldShape fs;
+  field_ptr(c, patch, field, &b, &off, &len, &fs);
   if (n != len) throw Error("upload: size mismatch");
-  HIPCK(hipMemcpyAsync(b + off, host, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
-  HIPCK(hipStreamSynchronize(c->stream));
+  if (!host && n > 0) throw Error("upload: null host array");
+  if (fs.rx > 0) {   // reference planes into the device's split planes
+    const long long dn = split_of(fs.rx, fs.ry).pl * fs.planes;
+    std::vector<double> tmp((size_t)dn, 0.0);
+    to_device(host, tmp.data(), fs.rx, fs.ry, fs.planes);
+    HIPCK(hipMemcpyAsync(b + off, tmp.data(), sizeof(double) * dn, hipMemcpyHostToDevice,
+                         c->stream));
+    HIPCK(hipStreamSynchronize(c->stream));
+  } else {
+    HIPCK(hipMemcpyAsync(b + off, host, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    HIPCK(hipStreamSynchronize(c->stream));
+  }
   API_END(c)
 }
 
@@ -1276,10 +1310,21 @@ int bsam_download(bsam_ctx *c, int patch, int field, double *host, long n) {
   c->need_device();
   double *b;
   long long off, len;
-  field_ptr(c, patch, field, &b, &off, &len);
+  FieldShape fs;
+  field_ptr(c, patch, field, &b, &off, &len, &fs);
   if (n != len) throw Error("download: size mismatch");
-  HIPCK(hipMemcpyAsync(host, b + off, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
-  HIPCK(hipStreamSynchronize(c->stream));
+  if (!host && n > 0) throw Error("download: null host array");
+  if (fs.rx > 0) {   // the device's split planes back into reference planes
+    const long long dn = split_of(fs.rx, fs.ry).pl * fs.planes;
+    std::vector<double> tmp((size_t)dn);
+    HIPCK(hipMemcpyAsync(tmp.data(), b + off, sizeof(double) * dn, hipMemcpyDeviceToHost,
+                         c->stream));
+    HIPCK(hipStreamSynchronize(c->stream));
+    from_device(tmp.data(), host, fs.rx, fs.ry, fs.planes);
+  } else {
+    HIPCK(hipMemcpyAsync(host, b + off, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    HIPCK(hipStreamSynchronize(c->stream));
+  }
   API_END(c)
 }
 
@@ -1558,18 +1603,20 @@ static void copy_level_boxes(bsam_ctx *dst, const bsam_ctx *src, int level, bool
       if (empty) continue;
       const DPatch &T = dst->dp[a], &O = src->dp[b];
       BoxCopy B;
-      auto off = [&](const DPatch &P, const HPatch &h) {
-        long long i = w.lo[0] - h.mg[0][0] + 1, j = w.lo[1] - h.mg[1][0] + 1;
-        long long k = nd == 3 ? w.lo[2] - h.mg[2][0] + 1 : 0;
-        return P.q + i + (long long)P.px * (j + (long long)P.py * k);
-      };
-      B.s = off(O, o);
-      B.d = off(T, t);
-      B.ssy = O.px;
-      B.ssz = (long long)O.px * O.py;
+      // the box's first cell in each patch's index space
+      B.s = O.q;
+      B.d = T.q;
+      B.si = w.lo[0] - o.mg[0][0] + 1;
+      B.sj = w.lo[1] - o.mg[1][0] + 1;
+      B.sk = nd == 3 ? w.lo[2] - o.mg[2][0] + 1 : 0;
+      B.di = w.lo[0] - t.mg[0][0] + 1;
+      B.dj = w.lo[1] - t.mg[1][0] + 1;
+      B.dk = nd == 3 ? w.lo[2] - t.mg[2][0] + 1 : 0;
+      B.spx = O.px;
+      B.spy = O.py;
+      B.dpx = T.px;
+      B.dpy = T.py;
       B.ssv = O.qvs;
-      B.dsy = T.px;
-      B.dsz = (long long)T.px * T.py;
       B.dsv = T.qvs;
       B.nx = w.hi[0] - w.lo[0] + 1;
       B.ny = w.hi[1] - w.lo[1] + 1;

@@ -633,7 +633,7 @@ struct Builder {
 
 long long qoff(const std::vector<HPatch> &hp, const Cell &c) {
   const HPatch &h = hp[c.p];
-  return h.qoff + c.i + h.px * (c.j + h.py * (long long)c.k);
+  return h.qoff + qcell(h.px, h.py, c.i, c.j, c.k);
 }
 
 }  // namespace
@@ -757,10 +757,7 @@ FusedGhost plan_fused_ghost(const std::vector<HPatch> &hp, const std::vector<Til
       return;
     }
     const HPatch &h = hp[p];
-    long long r = off - h.qoff;
-    i = (int)(r % h.px);
-    j = (int)((r / h.px) % h.py);
-    k = (int)(r / (h.px * h.py));
+    qcell_index(off - h.qoff, h.px, h.py, i, j, k);
   };
   // first interior tile of every patch; tiles of a patch are k, j, i nested
   std::vector<int> first(hp.size(), -1);
@@ -841,7 +838,7 @@ static long long parent_offset(const std::vector<HPatch> &hp, const HPatch &h, i
   const HPatch &pa = hp[h.parent];
   long long pi = h.mb[0][0] - 1 + ci, pj = h.mb[1][0] - 1 + cj;
   long long pk = nd == 3 ? h.mb[2][0] - 1 + ck : 0;
-  return pa.qoff + pi + pa.px * (pj + pa.py * pk);
+  return pa.qoff + qcell(pa.px, pa.py, (int)pi, (int)pj, (int)pk);
 }
 
 bool destination_index(const std::vector<GhostOp> &prog, std::vector<int> &at) {
@@ -1010,19 +1007,27 @@ void locate_q_cell(const std::vector<HPatch> &hp, long long off, int &p, int &i,
     return;
   }
   const HPatch &h = hp[p];
-  long long r = off - h.qoff;
-  i = (int)(r % h.px);
-  j = (int)((r / h.px) % h.py);
-  k = (int)(r / (h.px * h.py));
+  qcell_index(off - h.qoff, h.px, h.py, i, j, k);
 }
 
 void apply_ghost_program_host(const std::vector<HPatch> &hp, const std::vector<GhostOp> &ops,
                               int nrvars, double *const *q_patches) {
-  auto val = [&](long long off, int v, long long vs) {
-    int p, i, j, k;
+  // a device offset's element in the patch's reference-layout host array,
+  // and that array's variable stride (the device's may hold padding)
+  auto host_index = [&](long long off, int &p) {
+    int i, j, k;
     locate_q_cell(hp, off, p, i, j, k);
     const HPatch &h = hp[p];
-    return q_patches[p][off - h.qoff + (long long)v * vs];
+    return i + h.px * (j + h.py * (long long)k);
+  };
+  auto host_vs = [&](int p) {
+    const HPatch &h = hp[p];
+    return h.px * h.py * (h.qvs / split_of(h.px, h.py).pl);
+  };
+  auto val = [&](long long off, int v, long long) {
+    int p;
+    long long at = host_index(off, p);
+    return q_patches[p][at + (long long)v * host_vs(p)];
   };
   std::vector<double> res(ops.size() * nrvars);
   for (size_t e = 0; e < ops.size(); e++) {
@@ -1052,10 +1057,10 @@ void apply_ghost_program_host(const std::vector<HPatch> &hp, const std::vector<G
   }
   for (size_t e = 0; e < ops.size(); e++) {
     const GhostOp &g = ops[e];
-    int p, i, j, k;
-    locate_q_cell(hp, g.dst, p, i, j, k);
+    int p;
+    long long at = host_index(g.dst, p);
     for (int v = 0; v < nrvars; v++)
-      q_patches[p][g.dst - hp[p].qoff + (long long)v * g.dvs] = res[e * nrvars + v];
+      q_patches[p][at + (long long)v * host_vs(p)] = res[e * nrvars + v];
   }
 }
 

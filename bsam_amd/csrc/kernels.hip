@@ -76,24 +76,36 @@ template <class T> __device__ __forceinline__ Lvl<T> level_ref(const Dev &d, int
   return Lvl<T>{F.q + P.q, F.qc + P.qc, F.f + P.f, F.qb ? F.qb + P.q : nullptr, P};
 }
 
-// A ghost-extended array with lower bound 0: element (i,j,k,v) at
-// p + i + px*(j + py*k) + v*vs (k = 0 in 2D).
+// A ghost-extended array with lower bound 0 in the device layout (rows
+// split by the parity of i): element (i,j,k,v) at
+// p + (i & 1)*xo + (i >> 1) + rs*j + pl*k + v*vs (k = 0 in 2D). The y
+// neighbours of a cell are rs apart, the z neighbours pl; the x neighbours
+// lie in the other half of the row.
 template <class T> struct View {
   T *p;
-  long long px, pl, vs;
+  long long xo, rs, pl, vs;
   __device__ __forceinline__ long long at(int i, int j, int k) const {
-    return i + px * j + pl * k;
+    return (i & 1) * xo + (i >> 1) + rs * j + pl * k;
   }
 };
+template <class T> __device__ __forceinline__ View<T> view_of(T *p, long long n, long long ny,
+                                                              long long vs) {
+  const Split s = split_of(n, ny);
+  return View<T>{p, s.xo, s.rs, s.pl, vs};
+}
 template <class T> __device__ __forceinline__ View<T> qview(const Lvl<T> &L) {
-  return View<T>{L.q, L.P.px, (long long)L.P.px * L.P.py, L.P.qvs};
+  return view_of<T>(L.q, L.P.px, L.P.py, L.P.qvs);
 }
 template <class T> __device__ __forceinline__ View<T> qcview(const Lvl<T> &L) {
-  return View<T>{L.qc, L.P.cmx + 2, (long long)(L.P.cmx + 2) * (L.P.cmy + 2), L.P.cvs};
+  return view_of<T>(L.qc, L.P.cmx + 2, L.P.cmy + 2, L.P.cvs);
 }
-// Interior f-type arrays (lower bound 1).
+// Interior f-type arrays (lower bound 1; planes split by the parity of i - 1).
 __device__ __forceinline__ long long fat(const DPatch &P, int i, int j, int k) {
-  return (i - 1) + (long long)P.mx * ((j - 1) + (long long)P.my * (k - 1));
+  return scell(split_of(P.mx, P.my), i - 1, j - 1, k - 1);
+}
+// Ghost-extended cell (i, j, k) of a patch's q-layout arrays, relative to q(0,0,0,1).
+__device__ __forceinline__ long long qat(const DPatch &P, int i, int j, int k) {
+  return qcell(P.px, P.py, i, j, k);
 }
 
 // Operator2D/3D (problem.f90:231-257, :534-561) at cell c of view a on grid
@@ -122,14 +134,16 @@ __device__ __forceinline__ void op_vals(const Problem &prm, T base, const DPatch
 }
 template <int ND, class T, bool DEF>
 __device__ __forceinline__ void op_at(const Problem &prm, const View<T> &a, T base,
-                                      const DPatch &G, long long c, T &r1, T &r2) {
+                                      const DPatch &G, int i, int j, int k, T &r1, T &r2) {
   const T *q1 = a.p, *q2 = a.p + a.vs;
-  long long off[7] = {0, 1, -1, a.px, -a.px, a.pl, -a.pl};
+  const long long c = a.at(i, j, k);
+  long long at[7] = {c, a.at(i + 1, j, k), a.at(i - 1, j, k), c + a.rs, c - a.rs, c + a.pl,
+                     c - a.pl};
   T a1[2 * ND + 1], a2[2 * ND + 1];
 #pragma unroll
   for (int s = 0; s < 2 * ND + 1; s++) {
-    a1[s] = q1[c + off[s]];
-    a2[s] = q2[c + off[s]];
+    a1[s] = q1[at[s]];
+    a2[s] = q2[at[s]];
   }
   op_vals<ND, T, DEF>(prm, base, G, a1, a2, r1, r2);
 }
@@ -143,7 +157,8 @@ __device__ __forceinline__ void relax_cell_f(const Problem &prm, const Lvl<T> &L
                                              int k, T fv1, T fv2) {
   const DPatch &P = L.P;
   View<T> a = qview(L);
-  long long c = a.at(i, j, k);
+  const long long c = a.at(i, j, k);
+  const long long cxp = a.at(i + 1, j, k), cxm = a.at(i - 1, j, k);   // the other half-row
   T *q1 = a.p, *q2 = a.p + a.vs;
   T tmp1 = (T)P.dt_h2, tmp2 = (T)P.eps2_h2, tmp3 = (T)P.lap_h2;
   T phi = q1[c];
@@ -157,20 +172,20 @@ __device__ __forceinline__ void relax_cell_f(const Problem &prm, const Lvl<T> &L
     d2 = (T)d2fc(prm, (double)phi);
   }
   T a12, a21, b1, b2;
-  long long sx = 1, sy = a.px, sz = a.pl;
+  long long sy = a.rs, sz = a.pl;
   if (ND == 2) {
     a12 = tmp1 * (T)4;
     a21 = -d2 - tmp3;
-    b1 = fv1 + tmp1 * (((q2[c - sx] + q2[c + sx]) + q2[c - sy]) + q2[c + sy]);
+    b1 = fv1 + tmp1 * (((q2[cxm] + q2[cxp]) + q2[c - sy]) + q2[c + sy]);
     b2 = ((fv2 + dc) - d2 * phi) -
-         tmp2 * (((q1[c + sx] + q1[c - sx]) + q1[c + sy]) + q1[c - sy]);
+         tmp2 * (((q1[cxp] + q1[cxm]) + q1[c + sy]) + q1[c - sy]);
   } else {
     a12 = tmp1 * (T)6;
     a21 = -d2 - tmp3;
-    b1 = fv1 + tmp1 * (((((q2[c - sx] + q2[c + sx]) + q2[c - sy]) + q2[c + sy]) +
+    b1 = fv1 + tmp1 * (((((q2[cxm] + q2[cxp]) + q2[c - sy]) + q2[c + sy]) +
                          q2[c - sz]) + q2[c + sz]);
     b2 = ((fv2 + dc) - d2 * phi) -
-         tmp2 * (((((q1[c + sx] + q1[c - sx]) + q1[c + sy]) + q1[c - sy]) + q1[c + sz]) +
+         tmp2 * (((((q1[cxp] + q1[cxm]) + q1[c + sy]) + q1[c - sy]) + q1[c + sz]) +
                  q1[c - sz]);
   }
   T det = (T)1 - a12 * a21;
@@ -323,7 +338,7 @@ __device__ __forceinline__ void coarse_load_cell(const Problem &prm, const Lvl<T
     long long cidx = a.at(fi, fj, fk);
     T base = DEF ? C.qb[cidx] : (T)0;
     T o1, o2;
-    op_at<ND, T, DEF>(prm, a, base, P, cidx, o1, o2);
+    op_at<ND, T, DEF>(prm, a, base, P, fi, fj, fk, o1, o2);
     long long fo = fat(P, fi, fj, ND == 3 ? fk : 1);
     r1[s] = f1[fo] - o1;
     r2[s] = f2[fo] - o2;
@@ -339,10 +354,9 @@ __device__ __forceinline__ void coarse_load_cell(const Problem &prm, const Lvl<T
   int pi = P.mb[0][0] + i - 1, pj = P.mb[1][0] + j - 1,
       pk = ND == 3 ? P.mb[2][0] + k - 1 : 1;
   T base = (T)0;
-  if (DEF)
-    base = PA.qb[pi + (long long)PA.P.px * (pj + (long long)PA.P.py * (ND == 3 ? pk : 0))];
+  if (DEF) base = PA.qb[qat(PA.P, pi, pj, ND == 3 ? pk : 0)];
   T c1, c2;
-  op_at<ND, T, DEF>(prm, cq, base, PA.P, cq.at(i, j, k), c1, c2);
+  op_at<ND, T, DEF>(prm, cq, base, PA.P, i, j, ND == 3 ? k : 0, c1, c2);
   long long po = fat(PA.P, pi, pj, pk);
   PA.f[po] = R1 + c1;
   PA.f[po + PA.P.fvs] = R2 + c2;
@@ -363,7 +377,7 @@ __device__ __forceinline__ void trunc_error_cell(const Dev &d, const Lvl<double>
   for (int s = 0; s < NC; s++) {
     int fi = 2 * i - (s & 1), fj = 2 * j - ((s >> 1) & 1);
     int fk = ND == 3 ? 2 * k - ((s >> 2) & 1) : 0;
-    op_at<ND, double, false>(d.prm, a, 0.0, P, a.at(fi, fj, fk), r1[s], r2[s]);
+    op_at<ND, double, false>(d.prm, a, 0.0, P, fi, fj, fk, r1[s], r2[s]);
   }
   double R1, R2;
   if constexpr (ND == 2) {
@@ -374,7 +388,7 @@ __device__ __forceinline__ void trunc_error_cell(const Dev &d, const Lvl<double>
     R2 = 0.125 * (((((((r2[0] + r2[1]) + r2[2]) + r2[3]) + r2[4]) + r2[5]) + r2[6]) + r2[7]);
   }
   double c1, c2;
-  op_at<ND, double, false>(d.prm, cq, 0.0, PA, cq.at(i, j, k), c1, c2);
+  op_at<ND, double, false>(d.prm, cq, 0.0, PA, i, j, ND == 3 ? k : 0, c1, c2);
   const int cx = P.mx & 1 ? 1 : P.mx >> 1, cy = P.my & 1 ? 1 : P.my >> 1;
   long long o = P.qr + (i - 1) + (long long)cx * ((j - 1) + (long long)cy * (ND == 3 ? k - 1 : 0));
   d.qrte[o] = R1 - c1;
@@ -409,7 +423,7 @@ __device__ __forceinline__ void coarse_ghost_load_cell(const Problem &prm, int n
     long long cidx = a.at(fi, fj, fk);
     T base = DEF ? C.qb[cidx] : (T)0;
     T o1, o2;
-    op_at<ND, T, DEF>(prm, a, base, P, cidx, o1, o2);
+    op_at<ND, T, DEF>(prm, a, base, P, fi, fj, fk, o1, o2);
     long long fo = fat(P, fi, fj, ND == 3 ? fk : 1);
     r1[s] = f1[fo] - o1;
     r2[s] = f2[fo] - o2;
@@ -425,8 +439,7 @@ __device__ __forceinline__ void coarse_ghost_load_cell(const Problem &prm, int n
   int pi = P.mb[0][0] + i - 1, pj = P.mb[1][0] + j - 1,
       pk = ND == 3 ? P.mb[2][0] + k - 1 : 1;
   T base = (T)0;
-  if (DEF)
-    base = PA.qb[pi + (long long)PA.P.px * (pj + (long long)PA.P.py * (ND == 3 ? pk : 0))];
+  if (DEF) base = PA.qb[qat(PA.P, pi, pj, ND == 3 ? pk : 0)];
   // the coarse stencil: qc inside the box, the parent on the ring
   const int di[7] = {0, 1, -1, 0, 0, 0, 0}, dj[7] = {0, 0, 0, 1, -1, 0, 0},
             dk[7] = {0, 0, 0, 0, 0, 1, -1};
@@ -554,7 +567,7 @@ __device__ __forceinline__ void source_cell(const Dev &d, const DPatch &P, int i
                                             int k, int full) {
   long long fo = P.f + fat(P, i, j, ND == 3 ? k : 1);
   if (full) {
-    double c = d.qold[P.q + i + (long long)P.px * (j + (long long)P.py * k)];
+    double c = d.qold[P.q + qat(P, i, j, k)];
     d.ftmp[fo] = c;
     d.ftmp[fo + P.fvs] = -dfe(d.prm, c);
   }
@@ -577,7 +590,7 @@ __device__ __forceinline__ void defect_setup_cell(const Dev &d, const DPatch &P,
                (ND == 2 || (k >= 1 && k <= P.mz));
   if (!inner) return;
   double o1, o2;
-  op_at<ND, double, false>(d.prm, a, 0.0, P, c, o1, o2);
+  op_at<ND, double, false>(d.prm, a, 0.0, P, i, j, k, o1, o2);
   long long fo = P.f + fat(P, i, j, ND == 3 ? k : 1);
   d.fe[fo] = (float)(d.f[fo] - o1);
   d.fe[fo + P.fvs] = (float)(d.f[fo + P.fvs] - o2);
@@ -587,7 +600,7 @@ __device__ __forceinline__ void defect_setup_cell(const Dev &d, const DPatch &P,
 template <int ND>
 __device__ __forceinline__ void apply_correction_cell(const Dev &d, const DPatch &P, int i,
                                                       int j, int k) {
-  long long c = P.q + i + (long long)P.px * (j + (long long)P.py * k);
+  long long c = P.q + qat(P, i, j, k);
   d.q[c] = d.q[c] + (double)d.e[c];
   d.q[c + P.qvs] = d.q[c + P.qvs] + (double)d.e[c + P.qvs];
 }
@@ -597,6 +610,20 @@ template <int ND>
 __device__ __forceinline__ void tile_cell(const Dev &d, const Tile &t, int &i, int &j, int &k) {
   int tid = threadIdx.x, lx = d.tile_lx, ly = d.tile_ly;
   i = t.i0 + (tid & ((1 << lx) - 1));
+  j = t.j0 + ((tid >> lx) & ((1 << ly) - 1));
+  k = ND == 3 ? t.k0 + (tid >> (lx + ly)) : 0;
+}
+// The same cells with the lanes of each tile row ordered by the parity of i:
+// the first half of the row's lanes take the cells of i0's parity, the
+// second half the others, so one load or store of the row's lanes touches a
+// contiguous run of each half-row in the split layout. For kernels whose
+// thread works on one fine cell.
+template <int ND>
+__device__ __forceinline__ void tile_cell_px(const Dev &d, const Tile &t, int &i, int &j,
+                                             int &k) {
+  int tid = threadIdx.x, lx = d.tile_lx, ly = d.tile_ly;
+  const int l = tid & ((1 << lx) - 1), hx = lx - 1;
+  i = t.i0 + 2 * (l & ((1 << hx) - 1)) + (l >> hx);
   j = t.j0 + ((tid >> lx) & ((1 << ly) - 1));
   k = ND == 3 ? t.k0 + (tid >> (lx + ly)) : 0;
 }
@@ -844,7 +871,7 @@ __global__ __launch_bounds__(256) void k_error_flags(Dev d, const Tile *tiles, i
   Tile t = tiles[b];
   const DPatch &P = d.P[t.patch];
   int i, j, k;
-  tile_cell<ND>(d, t, i, j, k);
+  tile_cell_px<ND>(d, t, i, j, k);
   if (!in_box<ND>(i, j, k, P.mx, P.my, P.mz)) return;
   const int ci = (i + 1) >> 1, cj = (j + 1) >> 1, ck = ND == 3 ? (k + 1) >> 1 : 1;
   int flag = 0;
@@ -875,15 +902,16 @@ __global__ __launch_bounds__(256) void k_error_flags_user(Dev d, const Tile *til
   Tile t = tiles[b];
   const DPatch &P = d.P[t.patch];
   int i, j, k;
-  tile_cell<ND>(d, t, i, j, k);
+  tile_cell_px<ND>(d, t, i, j, k);
   if (!in_box<ND>(i, j, k, P.mx, P.my, P.mz)) return;
-  const long long sy = P.px, sz = (long long)P.px * P.py;
-  const double *q = d.q + P.q + i + sy * j + (ND == 3 ? sz * k : 0);
-  double gx = q[1] - q[-1];
-  double gy = q[sy] - q[-sy];
+  const View<double> a = view_of<double>(d.q + P.q, P.px, P.py, P.qvs);
+  const long long c = a.at(i, j, ND == 3 ? k : 0);
+  const double *q = a.p + c;
+  double gx = a.p[a.at(i + 1, j, ND == 3 ? k : 0)] - a.p[a.at(i - 1, j, ND == 3 ? k : 0)];
+  double gy = q[a.rs] - q[-a.rs];
   double s2 = gx * gx + gy * gy;
   if constexpr (ND == 3) {
-    double gz = q[sz] - q[-sz];
+    double gz = q[a.pl] - q[-a.pl];
     s2 = s2 + gz * gz;
   }
   d.eflags[P.fl + (i - 1) + (long long)P.mx * ((j - 1) + (long long)P.my * (ND == 3 ? k - 1 : 0))] =
@@ -921,24 +949,28 @@ __global__ __launch_bounds__(256) void k_after_step(Dev d, const StatTask *tasks
   const StatTask T = tasks[blockIdx.x];
   const DPatch &P = d.P[T.patch];
   const double alpha = d.prm.fc_alpha;
-  // the box c(1..m1, 1..m2[, 1..m3]) of the task's array, c(0,0,0) at base
+  // the box c(1..m1, 1..m2[, 1..m3]) of the task's array: cell (i, j, k) of
+  // the box is the array's cell (o0 + i, o1 + j, o2 + k), rows of rx cells
   const double *base;
-  long long sy, sz;
-  int m1, m2;
+  long long rx, ry;
+  int m1, m2, o0, o1, o2;
   if (T.part == 0) {
     base = d.q + P.q;
-    sy = P.px;
-    sz = (long long)P.px * P.py;
+    rx = P.px;
+    ry = P.py;
     m1 = P.mx;
     m2 = P.my;
+    o0 = o1 = o2 = 0;
   } else {
     const DPatch &A = d.P[P.parent];
-    sy = A.px;
-    sz = (long long)A.px * A.py;
-    base = d.q + A.q + (P.mb[0][0] - 1) + sy * (P.mb[1][0] - 1) +
-           (ND == 3 ? sz * (P.mb[2][0] - 1) : 0);
+    rx = A.px;
+    ry = A.py;
+    base = d.q + A.q;
     m1 = P.mx / 2;
     m2 = P.my / 2;
+    o0 = P.mb[0][0] - 1;
+    o1 = P.mb[1][0] - 1;
+    o2 = ND == 3 ? P.mb[2][0] - 1 : 0;
   }
   // rows r0..r1-1 (2D: j; 3D: k) of the box
   const int per = m1 * (ND == 3 ? m2 : 1);
@@ -949,11 +981,14 @@ __global__ __launch_bounds__(256) void k_after_step(Dev d, const StatTask *tasks
     const int i = 1 + w % m1;
     const int j = ND == 3 ? 1 + w / m1 : r;
     const int k = ND == 3 ? r : 0;
-    const double *c = base + i + sy * j + sz * k;
+    const Split sp = split_of(rx, ry);
+    const double *c = base + scell(sp, o0 + i, o1 + j, o2 + k);
     sq += c[0];
     if constexpr (ND == 2) {
       sff += well_ff(c[0], lo, hi, alpha);
-      double a = c[1] - c[0], b = c[0] - c[-1], u = c[sy] - c[0], v = c[0] - c[-sy];
+      const double cxp = base[scell(sp, o0 + i + 1, o1 + j, 0)];
+      const double cxm = base[scell(sp, o0 + i - 1, o1 + j, 0)];
+      double a = cxp - c[0], b = c[0] - cxm, u = c[sp.rs] - c[0], v = c[0] - c[-sp.rs];
       sg += a * a + b * b + u * u + v * v;
     }
   }
@@ -976,23 +1011,32 @@ __device__ __forceinline__ double lin31(double a0, double a1, bool even) {
   return even ? 3.0 * a0 + a1 : a0 + 3.0 * a1;
 }
 
+// The parent's cells under a patch: a(i, j, k) is the parent's q at
+// (b0 + i, b1 + j, b2 + k) (b the cell one below the patch's footprint).
+struct ParentBox {
+  const double *q;
+  long long px, py;
+  int b0, b1, b2;
+  __device__ __forceinline__ double operator()(int i, int j, int k) const {
+    return q[qcell(px, py, b0 + i, b1 + j, b2 + k)];
+  }
+};
+
 template <int ND>
-__device__ __forceinline__ double p1_pred(const double *a, long long ay, long long az, int i,
-                                          int j, int k) {
+__device__ __forceinline__ double p1_pred(const ParentBox &a, int i, int j, int k) {
   const int m = i >> 1, n = j >> 1;
   const bool ei = (i & 1) == 0, ej = (j & 1) == 0;
   if constexpr (ND == 2) {
-    const double b0 = lin31(a[m + ay * n], a[m + 1 + ay * n], ei);
-    const double b1 = lin31(a[m + ay * (n + 1)], a[m + 1 + ay * (n + 1)], ei);
+    const double b0 = lin31(a(m, n, 0), a(m + 1, n, 0), ei);
+    const double b1 = lin31(a(m, n + 1, 0), a(m + 1, n + 1, 0), ei);
     return lin31(b0, b1, ej) / 16.0;
   } else {
     const int o = k >> 1;
     const bool ek = (k & 1) == 0;
     double cz[2];
     for (int t = 0; t < 2; t++) {
-      const double *ap = a + az * (o + t);
-      const double b0 = lin31(ap[m + ay * n], ap[m + 1 + ay * n], ei);
-      const double b1 = lin31(ap[m + ay * (n + 1)], ap[m + 1 + ay * (n + 1)], ei);
+      const double b0 = lin31(a(m, n, o + t), a(m + 1, n, o + t), ei);
+      const double b1 = lin31(a(m, n + 1, o + t), a(m + 1, n + 1, o + t), ei);
       cz[t] = lin31(b0, b1, ej);
     }
     return lin31(cz[0], cz[1], ek) / 64.0;
@@ -1003,20 +1047,17 @@ template <int ND>
 __global__ __launch_bounds__(256) void k_init_fields(Dev d, const int *patches) {
   const DPatch &P = d.P[patches[blockIdx.x]];
   const DPatch &A = d.P[P.parent];
-  const long long ay = A.px, az = (long long)A.px * A.py;
-  const long long fy = P.px, fz = (long long)P.px * P.py;
   const int cx = P.mx / 2, cy = P.my / 2, cz = ND == 3 ? P.mz / 2 : 0;
-  const long long cpy = cx + 2, cpz = (long long)(cx + 2) * (cy + 2);
   for (int v = 0; v < d.nrvars; v++) {
     // a(0,0,0): the parent cell one below the patch's footprint
-    const double *a = d.q + A.q + v * A.qvs + (P.mb[0][0] - 1) + ay * (P.mb[1][0] - 1) +
-                      (ND == 3 ? az * (P.mb[2][0] - 1) : 0);
+    const ParentBox a{d.q + A.q + v * A.qvs, A.px, A.py, P.mb[0][0] - 1, P.mb[1][0] - 1,
+                      ND == 3 ? P.mb[2][0] - 1 : 0};
     double *qc = d.qc + P.qc + v * P.cvs;
     const int nc = (cx + 2) * (cy + 2) * (ND == 3 ? cz + 2 : 1);
     for (int e = threadIdx.x; e < nc; e += blockDim.x) {
       const int i = e % (cx + 2), j = (e / (cx + 2)) % (cy + 2);
       const int k = ND == 3 ? e / ((cx + 2) * (cy + 2)) : 0;
-      qc[i + cpy * j + cpz * k] = a[i + ay * j + az * k];
+      qc[qcell(cx + 2, cy + 2, i, j, k)] = a(i, j, k);
     }
     double *q = d.q + P.q + v * P.qvs;
     double *qo = d.qold + P.q + v * P.qvs;
@@ -1024,32 +1065,30 @@ __global__ __launch_bounds__(256) void k_init_fields(Dev d, const int *patches) 
     const int nf = ex * ey * ez;
     for (int e = threadIdx.x; e < nf; e += blockDim.x) {
       const int i = e % ex, j = (e / ex) % ey, k = ND == 3 ? e / (ex * ey) : 0;
-      double p = p1_pred<ND>(a, ay, az, i, j, k);
+      double p = p1_pred<ND>(a, i, j, k);
       const bool inner = i >= 1 && i <= P.mx && j >= 1 && j <= P.my &&
                          (ND == 2 || (k >= 1 && k <= P.mz));
       if (inner) {
         const int ic = (i + 1) >> 1, jc = (j + 1) >> 1, kc = ND == 3 ? (k + 1) >> 1 : 0;
         double sum;
         if constexpr (ND == 2) {
-          sum = ((p1_pred<2>(a, ay, az, 2 * ic, 2 * jc, 0) +
-                  p1_pred<2>(a, ay, az, 2 * ic - 1, 2 * jc, 0)) +
-                 p1_pred<2>(a, ay, az, 2 * ic, 2 * jc - 1, 0)) +
-                p1_pred<2>(a, ay, az, 2 * ic - 1, 2 * jc - 1, 0);
-          p = p + (a[ic + ay * jc] - 0.25 * sum);
+          sum = ((p1_pred<2>(a, 2 * ic, 2 * jc, 0) + p1_pred<2>(a, 2 * ic - 1, 2 * jc, 0)) +
+                 p1_pred<2>(a, 2 * ic, 2 * jc - 1, 0)) +
+                p1_pred<2>(a, 2 * ic - 1, 2 * jc - 1, 0);
+          p = p + (a(ic, jc, 0) - 0.25 * sum);
         } else {
           sum = 0.0;
           const int ci[8][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {1, 1, 0},
                                 {0, 0, 1}, {1, 0, 1}, {0, 1, 1}, {1, 1, 1}};
           for (int t = 0; t < 8; t++) {
-            double c = p1_pred<3>(a, ay, az, 2 * ic - ci[t][0], 2 * jc - ci[t][1],
-                                  2 * kc - ci[t][2]);
+            double c = p1_pred<3>(a, 2 * ic - ci[t][0], 2 * jc - ci[t][1], 2 * kc - ci[t][2]);
             sum = t == 0 ? c : sum + c;
           }
-          p = p + (a[ic + ay * jc + az * kc] - 0.125 * sum);
+          p = p + (a(ic, jc, kc) - 0.125 * sum);
         }
       }
-      q[i + fy * j + fz * k] = p;
-      qo[i + fy * j + fz * k] = p;
+      q[qat(P, i, j, k)] = p;
+      qo[qat(P, i, j, k)] = p;
     }
   }
 }
@@ -1065,8 +1104,8 @@ __global__ __launch_bounds__(256) void k_box_copy(double *dst, const double *src
     const int j = (int)((e / B.nx) % B.ny);
     const int k = (int)((e / ((long long)B.nx * B.ny)) % B.nz);
     const int v = (int)(e / ((long long)B.nx * B.ny * B.nz));
-    dst[B.d + i + B.dsy * j + B.dsz * k + B.dsv * v] =
-        src[B.s + i + B.ssy * j + B.ssz * k + B.ssv * v];
+    dst[B.d + qcell(B.dpx, B.dpy, B.di + i, B.dj + j, B.dk + k) + B.dsv * v] =
+        src[B.s + qcell(B.spx, B.spy, B.si + i, B.sj + j, B.sk + k) + B.ssv * v];
   }
 }
 
@@ -1077,7 +1116,7 @@ __global__ __launch_bounds__(256) void k_source(Dev d, const Tile *tiles, int nt
   Tile t = tiles[b];
   const DPatch &P = d.P[t.patch];
   int i, j, k;
-  tile_cell<ND>(d, t, i, j, k);
+  tile_cell_px<ND>(d, t, i, j, k);
   if (!in_box<ND>(i, j, k, P.mx, P.my, P.mz)) return;
   source_cell<ND>(d, P, i, j, k, full);
 }
@@ -1090,7 +1129,7 @@ __global__ __launch_bounds__(256) void k_defect_setup(Dev d, const Tile *tiles, 
   Tile t = tiles[b];
   const DPatch &P = d.P[t.patch];
   int i, j, k;
-  tile_cell<ND>(d, t, i, j, k);
+  tile_cell_px<ND>(d, t, i, j, k);
   if (!in_box<ND>(i, j, k, P.mx + 1, P.my + 1, P.mz + 1)) return;
   defect_setup_cell<ND>(d, P, i, j, k);
 }
@@ -1102,7 +1141,7 @@ __global__ __launch_bounds__(256) void k_apply_correction(Dev d, const Tile *til
   Tile t = tiles[b];
   const DPatch &P = d.P[t.patch];
   int i, j, k;
-  tile_cell<ND>(d, t, i, j, k);
+  tile_cell_px<ND>(d, t, i, j, k);
   if (!in_box<ND>(i, j, k, P.mx, P.my, P.mz)) return;
   apply_correction_cell<ND>(d, P, i, j, k);
 }
@@ -1477,7 +1516,7 @@ __global__ __launch_bounds__(256) void k_resnorm(Dev d, const Tile *tiles, int n
   Tile t = tiles[b];
   const DPatch &P = d.P[t.patch];
   int i, j, k;
-  tile_cell<ND>(d, t, i, j, k);
+  tile_cell_px<ND>(d, t, i, j, k);
   double s = 0.0, s2 = 0.0;
   if (in_box<ND>(i, j, k, P.mx, P.my, P.mz)) {
     long long fo = P.f + fat(P, i, j, ND == 3 ? k : 1);
@@ -1486,7 +1525,7 @@ __global__ __launch_bounds__(256) void k_resnorm(Dev d, const Tile *tiles, int n
       Lvl<double> L{d.q + P.q, nullptr, nullptr, nullptr, P};
       View<double> a = qview(L);
       double o1, o2;
-      op_at<ND, double, false>(d.prm, a, 0.0, P, a.at(i, j, k), o1, o2);
+      op_at<ND, double, false>(d.prm, a, 0.0, P, i, j, k, o1, o2);
       r1 = r1 - o1;
       r2 = r2 - o2;
     }
