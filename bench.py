@@ -204,6 +204,8 @@ def main():
     ap.add_argument("--check", action="store_true",
                     help="mixed precision: also run the FP64 path and report the "
                          "difference (max |phi|, |mu| and mass) after the steps")
+    ap.add_argument("--inner-cycles", type=int, default=1,
+                    help="mixed precision: FP32 V-cycles per FP64 defect-correction step")
     ap.add_argument("--no-profile", action="store_true",
                     help="skip the event-timed pass (roofline null)")
     args = ap.parse_args()
@@ -219,9 +221,11 @@ def main():
     finest = layout.finestlevel
 
     def make_engine(precision):
+        tune = {"inner_cycles": args.inner_cycles} if precision == E.PREC_MIXED else None
         eng = E.Engine(layout, dt=params["dt"], eps=params["eps"], well=params["well"],
                        nsmoothingpasses=params["nsmoothingpasses"], maxvcycles=50,
-                       qerrortol=params["qerrortol"], precision=precision, device=0)
+                       qerrortol=params["qerrortol"], precision=precision, device=0,
+                       tune=tune)
         for n, p in enumerate(eng.patches):
             eng.upload(n, "q", q0[p.id])
         return eng

@@ -58,6 +58,8 @@ struct LevelData {
   // coarse ghost + loading launch, the 2x2(x2) fine blocks of the correction
   std::vector<Tile> t_cgl, t_blk;
   Tile *d_cgl = nullptr, *d_blk = nullptr;
+  // linear tiles in device order over each patch's ghost-extended q array
+  std::vector<Tile> t_qlin;
   std::vector<GhostOp> gp[3];
   GhostOp *d_gp[3] = {nullptr, nullptr, nullptr};
   GhostOp *d_tgp[3] = {nullptr, nullptr, nullptr};  // patch-local copies (tail levels)
@@ -169,7 +171,7 @@ struct bsam_ctx {
   // tile lists spanning levels: interior and ghost-extended tiles of all
   // levels, interior tiles of levels 0..finest (source)
   double *f_saved = nullptr;   // f after the solve's source set-up (constant per solve)
-  std::vector<Tile> all_int, all_ext, src_int;
+  std::vector<Tile> all_int, all_ext, src_int;   // all_ext: linear device-order tiles
   Tile *d_all_int = nullptr, *d_all_ext = nullptr, *d_src_int = nullptr;
   long long src_cells = 0;
   long long all_int_cells = 0, all_ext_cells = 0;   // interior / ghost-extended cells, all levels
@@ -346,6 +348,7 @@ static void build_tiles(bsam_ctx *c, LevelData &L, int level) {
     const HPatch &h = c->hp[p];
     cover(L.t_int, p, 1, h.mx[0], h.mx[1], h.mx[2]);
     cover(L.t_ext, p, 0, h.mx[0] + 2, h.mx[1] + 2, h.mx[2] + 2);
+    lin(L.t_qlin, p, split_of(h.mx[0] + 2, h.mx[1] + 2).pl * (nd == 3 ? h.mx[2] + 2 : 1));
     if (level > c->cfg.minlevel) {
       cover(L.t_coarse, p, 1, h.cmx[0], h.cmx[1], h.cmx[2]);
       cover(L.t_cext, p, 0, h.cmx[0] + 2, h.cmx[1] + 2, h.cmx[2] + 2);
@@ -722,7 +725,7 @@ static void commit(bsam_ctx *c) {
     LevelData &L = c->lev[l - minl];
     c->all_int_cells += L.cells;
     c->all_int.insert(c->all_int.end(), L.t_int.begin(), L.t_int.end());
-    c->all_ext.insert(c->all_ext.end(), L.t_ext.begin(), L.t_ext.end());
+    c->all_ext.insert(c->all_ext.end(), L.t_qlin.begin(), L.t_qlin.end());
     if (l >= 0) {
       c->src_int.insert(c->src_int.end(), L.t_int.begin(), L.t_int.end());
       c->src_cells += L.cells;

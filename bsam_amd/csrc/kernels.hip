@@ -669,6 +669,24 @@ __device__ __forceinline__ bool lin_cell(const Tile &t, int ex, int ey, int ez, 
   k = ND == 3 ? r / ey : 0;
   return true;
 }
+// Linear tiles in device order: tile t covers LIN_TILE consecutive device
+// positions of a patch's ghost-extended q array (rows of px cells split by
+// the parity of x), from position t.i0, so consecutive lanes touch
+// consecutive addresses. Returns the cell (i, j, k), false past the array or
+// on a row's padding slot.
+template <int ND>
+__device__ __forceinline__ bool lin_qcell(const Tile &t, const DPatch &P, int &i, int &j,
+                                          int &k) {
+  const Split s = split_of(P.px, P.py);
+  const long long e = t.i0 + (long long)threadIdx.x;
+  if (e >= s.pl * (ND == 3 ? P.mz + 2 : 1)) return false;
+  k = ND == 3 ? (int)(e / s.pl) : 0;
+  const long long m = e - (long long)k * s.pl;
+  j = (int)(m / s.rs);
+  const long long x = m - (long long)j * s.rs;
+  i = (int)(x < s.xo ? 2 * x : 2 * (x - s.xo) + 1);
+  return i < P.px;
+}
 template <int ND>
 __device__ __forceinline__ bool in_box(int i, int j, int k, int ex, int ey, int ez) {
   return i <= ex && j <= ey && (ND == 2 || k <= ez);
@@ -1121,7 +1139,7 @@ __global__ __launch_bounds__(256) void k_source(Dev d, const Tile *tiles, int nt
   source_cell<ND>(d, P, i, j, k, full);
 }
 
-// Tiles cover the ghost-extended fine box 0..mx+1.
+// Linear tiles in device order over the ghost-extended fine box 0..mx+1.
 template <int ND>
 __global__ __launch_bounds__(256) void k_defect_setup(Dev d, const Tile *tiles, int nt) {
   const int b = xcd_tile(nt);
@@ -1129,8 +1147,7 @@ __global__ __launch_bounds__(256) void k_defect_setup(Dev d, const Tile *tiles, 
   Tile t = tiles[b];
   const DPatch &P = d.P[t.patch];
   int i, j, k;
-  tile_cell_px<ND>(d, t, i, j, k);
-  if (!in_box<ND>(i, j, k, P.mx + 1, P.my + 1, P.mz + 1)) return;
+  if (!lin_qcell<ND>(t, P, i, j, k)) return;
   defect_setup_cell<ND>(d, P, i, j, k);
 }
 
