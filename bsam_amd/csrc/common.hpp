@@ -71,7 +71,7 @@ struct DPatch {
   int cmx, cmy, cmz;       // coarse extents (mx/2 ...), qc pitch cmx+2
   int level, parent;       // parent patch index, -1 if none
   int mb[3][2];            // mbounds in the parent's local coarse indices
-  int pad0;
+  int group;               // independent sub-problem (a served request) the patch belongs to
   long long q;             // offset of q(0,0,k0,1) in the q and qold pools
   long long f;             // offset of f(1,1,1,1) in the f/rf/ftmp pools
   long long qc;            // offset of qc(0,0,k0,1) in the qc pool
@@ -89,10 +89,12 @@ struct DPatch {
 // concurrently. kind: 0 copy, 1 and 2 two fine + coarse quadratic
 // interpolation with a single coarse value or a 3-point coarse interpolant,
 // 3 the 3D face form (two fine + bilinear coarse, four values). nflip counts
-// trailing "2 - x" boundary maps. 48 bytes.
+// trailing "2 - x" boundary maps; group is the destination patch's group.
+// 48 bytes.
 struct GhostOp {
   int dst, dvs;            // q-pool offset (variable 1) and variable stride
-  int kind, nflip;
+  int kind;
+  short nflip, group;
   int f0, f1, fvs;         // fine (or copy) sources and their variable stride
   int cvs;                 // parent variable stride
   int c[4];                // parent sources
@@ -142,6 +144,9 @@ struct Dev {
   double *qrte;   // relative truncation error per coarse cell (refined levels)
   int *eflags;    // error flags per fine cell
   const DPatch *P;
+  // per group: iteration count, converged, non-finite, stopped; the kernels of
+  // a solve skip the patches and ghost updates of a group that has stopped
+  const int *gstate;
   Problem prm;
   int nrvars;
   int tile_lx, tile_ly;   // level-kernel tiles: 2^tile_lx x 2^tile_ly (x the rest of 256)

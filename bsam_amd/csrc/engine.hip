@@ -82,6 +82,12 @@ struct LevelData {
     int nrest = 0;
   } d_ride_cgl, d_ride_corr;
   long long cells = 0;
+  // per group: its interior tiles of this level (first, count; a group's
+  // patches are contiguous in the level list) and its cells
+  std::vector<int> granges;
+  std::vector<double> gcells;
+  int *d_granges = nullptr;
+  double *d_gcells = nullptr;
 };
 
 // Transition index of SetGhost(ipass) followed by relaxation of `color`.
@@ -148,9 +154,19 @@ struct bsam_ctx {
   StatTask *d_stats_list = nullptr;  // their tasks
   int n_stats = 0;
   int partial_cap = 0;
-  double *d_hist = nullptr;
-  int *d_state = nullptr;
+  double *d_hist = nullptr;   // ngroups residual histories of hmax entries
+  int *d_state = nullptr;     // 4 ints per group (k_finalize)
   int *h_state = nullptr;
+  int hmax = 0;
+  // independent sub-problems solved side by side (bsam_set_groups): a
+  // group's patches never touch another's, each has its own eps and
+  // residual test, and a group that has converged is left untouched
+  int ngroups = 1;
+  std::vector<double> group_eps2;          // per group (committed)
+  std::vector<int> staged_group;           // per staged patch
+  std::vector<double> staged_eps2;
+  int staged_ngroups = 1;
+  std::vector<int> group_state;            // per group after the last solve: 4 ints
   int last_status = -1;      // of the last solve: BSAM_SOLVE_* (bsam_solve_status)
   // kernel timing with HIP events on the context's stream (bsam_profile_*)
   unsigned prof_mask = 0;
@@ -207,6 +223,7 @@ struct bsam_ctx {
     d.e = e; d.qce = qce; d.fe = fe; d.qb = qb;
     d.qrte = qrte; d.eflags = eflags;
     d.P = d_P;
+    d.gstate = d_state;
     d.prm = Problem{cfg.dt, cfg.eps2, cfg.fc_alpha, cfg.fc_shift, cfg.fe_num, cfg.fe_den};
     d.nrvars = cfg.nrvars;
     d.tile_lx = tile_lx;
@@ -265,6 +282,7 @@ void bsam_ctx::free_device() {
   for (auto &l : lev) {
     F(l.d_int); F(l.d_ext); F(l.d_coarse); F(l.d_cext); F(l.d_patch_list);
     F(l.d_cgl); F(l.d_blk);
+    F(l.d_granges); F(l.d_gcells);
     for (int p = 0; p < 3; p++) {
       F(l.d_gp[p]);
       F(l.d_tgp[p]);
@@ -469,7 +487,8 @@ static bool same_geometry(const std::vector<HPatch> &a, const std::vector<HPatch
   if (a.size() != b.size()) return false;
   for (size_t i = 0; i < a.size(); i++) {
     const HPatch &x = a[i], &y = b[i];
-    if (x.level != y.level || x.parent != y.parent || x.dx != y.dx) return false;
+    if (x.level != y.level || x.parent != y.parent || x.dx != y.dx || x.group != y.group)
+      return false;
     for (int d = 0; d < 3; d++) {
       if (x.mx[d] != y.mx[d] || x.xlower[d] != y.xlower[d]) return false;
       for (int e = 0; e < 2; e++)
@@ -536,7 +555,13 @@ static void commit(bsam_ctx *c) {
   for (int l = minl; l <= c->finest; l++) {
     auto &P = c->lev[l - minl].patches;
     if (P.empty()) throw Error("level " + std::to_string(l) + " has no patches");
-    if (l <= 0 && P.size() != 1) throw Error("levels <= 0 hold one patch");
+    // levels <= 0 hold one patch (one per group when there are several)
+    if (l <= 0) {
+      std::vector<int> per_group(c->ngroups, 0);
+      for (int p : P) per_group[c->hp[p].group]++;
+      for (int k : per_group)
+        if (k != 1) throw Error("levels <= 0 hold one patch (of each group)");
+    }
     // same-level interiors must not overlap (GhostOverlap copies assume it)
     for (size_t a = 0; a < P.size(); a++)
       for (size_t b = a + 1; b < P.size(); b++) {
@@ -560,7 +585,7 @@ static void commit(bsam_ctx *c) {
     D.mx = h.mx[0]; D.my = h.mx[1]; D.mz = nd == 3 ? h.mx[2] : 1;
     D.px = (int)ex; D.py = (int)ey;
     D.cmx = h.cmx[0]; D.cmy = h.cmx[1]; D.cmz = nd == 3 ? h.cmx[2] : 1;
-    D.level = h.level; D.parent = h.parent;
+    D.level = h.level; D.parent = h.parent; D.group = h.group;
     for (int d = 0; d < 3; d++) {
       D.mb[d][0] = h.mb[d][0];
       D.mb[d][1] = h.mb[d][1];
@@ -582,8 +607,9 @@ static void commit(bsam_ctx *c) {
     {
       double h2 = h.dx * h.dx;   // as RelaxGrid / Operator compute it per call
       D.dt_h2 = c->cfg.dt / h2;
-      D.eps2_h2 = c->cfg.eps2 / h2;
-      D.lap_h2 = (nd == 2 ? 4.0 : 6.0) * c->cfg.eps2 / h2;
+      const double eps2 = c->group_eps2[(size_t)h.group];   // the group's eps^2
+      D.eps2_h2 = eps2 / h2;
+      D.lap_h2 = (nd == 2 ? 4.0 : 6.0) * eps2 / h2;
     }
     h.qoff = D.q; h.qvs = D.qvs; h.px = ex; h.py = ey;
     c->nq += D.qvs * nv;
@@ -598,6 +624,11 @@ static void commit(bsam_ctx *c) {
     for (int d = 0; d < nd; d++)
       if ((root.mthbc[2 * d] == 2) != (root.mthbc[2 * d + 1] == 2))
         throw Error("periodic boundaries must pair opposite faces");
+    if (c->ngroups > 1)
+      for (int p : c->lev[0 - minl].patches)
+        for (int fc = 0; fc < 2 * nd; fc++)
+          if (c->hp[p].mthbc[fc] == 2)
+            throw Error("groups: periodic boundaries would join the groups' roots");
   }
   for (int l = minl; l <= c->finest; l++) {
     LevelData &L = c->lev[l - minl];
@@ -606,6 +637,33 @@ static void commit(bsam_ctx *c) {
       std::string msg;
       if (build_ghost_program(c->hp, L.patches, nd, l, ip, per, L.gp[ip], msg))
         throw Error("ghost program level " + std::to_string(l) + ": " + msg);
+      if (c->ngroups > 1) set_op_groups(c->hp, L.gp[ip]);
+    }
+    // each group's patches: contiguous in the level list, so its interior
+    // tiles are one range of the level's tile list
+    {
+      const int G = c->ngroups;
+      L.granges.assign(2 * G, 0);
+      L.gcells.assign(G, 0.0);
+      std::vector<int> seen(G, 0);
+      int prev = -1;
+      for (int p : L.patches) {
+        const int g = c->hp[p].group;
+        if (g != prev && seen[g])
+          throw Error("level " + std::to_string(l) + ": a group's patches must be contiguous");
+        seen[g] = 1;
+        prev = g;
+      }
+      for (size_t t = 0; t < L.t_int.size(); t++) {
+        const int g = c->hp[L.t_int[t].patch].group;
+        if (L.granges[2 * g + 1] == 0) L.granges[2 * g] = (int)t;
+        L.granges[2 * g + 1]++;
+      }
+      for (int p : L.patches) {
+        const HPatch &h = c->hp[p];
+        const double cells = (double)h.mx[0] * h.mx[1] * (nd == 3 ? h.mx[2] : 1);
+        L.gcells[h.group] += (c->cfg.errortype == 2 ? 1.0 : (double)nv) * cells;
+      }
     }
     // the fusion and riding plans serve the V-cycle only; a context that
     // does not fuse (fuse_ghost 0) goes without them
@@ -736,10 +794,18 @@ static void commit(bsam_ctx *c) {
   c->d_src_int = dev_upload(c->src_int, s);
   c->partial_cap = (int)maxt;
   HIPCK(hipMalloc(&c->d_partial, sizeof(double) * 2 * maxt));   // two components (errortype 2)
-  int hmax = std::max(c->cfg.maxvcycles, 1) + 1;
-  HIPCK(hipMalloc(&c->d_hist, sizeof(double) * hmax));
-  HIPCK(hipMalloc(&c->d_state, sizeof(int) * 4));
-  HIPCK(hipHostMalloc(&c->h_state, sizeof(int) * 4));
+  const int hmax = std::max(c->cfg.maxvcycles, 1) + 1;
+  c->hmax = hmax;
+  const int G = c->ngroups;
+  HIPCK(hipMalloc(&c->d_hist, sizeof(double) * hmax * G));
+  HIPCK(hipMalloc(&c->d_state, sizeof(int) * 4 * G));
+  HIPCK(hipMemsetAsync(c->d_state, 0, sizeof(int) * 4 * G, s));
+  HIPCK(hipHostMalloc(&c->h_state, sizeof(int) * 4 * G));
+  for (int l = minl; l <= c->finest; l++) {
+    LevelData &L = c->lev[l - minl];
+    L.d_granges = dev_upload(L.granges, s);
+    L.d_gcells = dev_upload(L.gcells, s);
+  }
   HIPCK(hipStreamSynchronize(s));
 }
 
@@ -924,7 +990,7 @@ static int op_vcycle(bsam_ctx *c, int level, int prec = PREC_SOLUTION_FP64,
   return level_relax(c, level, prec, true, defer_final);
 }
 static void reset_state(bsam_ctx *c) {
-  HIPCK(hipMemsetAsync(c->d_state, 0, sizeof(int) * 4, c->stream));
+  HIPCK(hipMemsetAsync(c->d_state, 0, sizeof(int) * 4 * c->ngroups, c->stream));
 }
 // queue ErrorAFAS(level): residual norm into d_hist[state[0]]
 static void queue_error(bsam_ctx *c, int level, double tol) {
@@ -937,8 +1003,12 @@ static void queue_error(bsam_ctx *c, int level, double tol) {
   }
   // L2Error counts nrvars values per cell, L2ComponentErrors one
   double count = (double)(c->comps() == 2 ? L.cells : c->cfg.nrvars * L.cells);
-  HIPCK(launch_finalize(c->d_partial, nt, count, tol, c->d_hist, c->d_state, c->comps(),
-                        c->stream));
+  if (c->ngroups > 1 && tol >= 0.0)   // a solve: each group its own test
+    HIPCK(launch_finalize(c->d_partial, nt, count, tol, c->d_hist, c->d_state, c->comps(),
+                          c->stream, c->ngroups, L.d_granges, L.d_gcells, c->hmax));
+  else
+    HIPCK(launch_finalize(c->d_partial, nt, count, tol, c->d_hist, c->d_state, c->comps(),
+                          c->stream));
 }
 
 // Captures `body` on the context's stream into *exec (once per hierarchy).
@@ -964,25 +1034,35 @@ template <class F>
 static int iterate(bsam_ctx *c, hipGraph_t *g, hipGraphExec_t *exec, F body, int *done_out) {
   bool graph = c->use_graph && c->prof_mask == 0;
   if (graph) capture(c, g, exec, body);
-  int it = 0, done = 0;
-  c->h_state[1] = c->h_state[2] = 0;
-  for (it = 1; it <= c->cfg.maxvcycles; it++) {
+  const int G = c->ngroups;
+  for (int g = 0; g < 4 * G; g++) c->h_state[g] = 0;
+  for (int it = 1; it <= c->cfg.maxvcycles; it++) {
     if (graph) HIPCK(hipGraphLaunch(*exec, c->stream));
     else body();
-    HIPCK(hipMemcpyAsync(c->h_state, c->d_state, sizeof(int) * 4, hipMemcpyDeviceToHost,
+    HIPCK(hipMemcpyAsync(c->h_state, c->d_state, sizeof(int) * 4 * G, hipMemcpyDeviceToHost,
                          c->stream));
     HIPCK(hipStreamSynchronize(c->stream));
-    if (c->h_state[1]) {
-      done = 1;
-      break;
-    }
-    // a residual that is not finite ends the solve: every later one is too
-    if (c->h_state[2]) break;
+    // a group stops once its residual is below the tolerance, or is not
+    // finite (every later one is not either); the solve ends when all have
+    bool all = true;
+    for (int g = 0; g < G; g++) all = all && c->h_state[4 * g + 3];
+    if (all) break;
   }
-  c->last_status = done ? BSAM_SOLVE_CONVERGED
-                        : (c->h_state[2] ? BSAM_SOLVE_NONFINITE : BSAM_SOLVE_MAXCYCLES);
-  *done_out = done;
-  return done || c->h_state[2] ? std::min(it, c->cfg.maxvcycles) : c->cfg.maxvcycles;
+  c->group_state.assign(c->h_state, c->h_state + 4 * G);
+  // the stopped flags only hold within a solve: entry points called after it
+  // act on every group
+  reset_state(c);
+  bool conv = true, nonfinite = false;
+  int n = 0;
+  for (int g = 0; g < G; g++) {
+    conv = conv && c->h_state[4 * g + 1];
+    nonfinite = nonfinite || c->h_state[4 * g + 2];
+    n = std::max(n, c->h_state[4 * g]);   // histories recorded: iterations run
+  }
+  c->last_status = conv ? BSAM_SOLVE_CONVERGED
+                        : (nonfinite ? BSAM_SOLVE_NONFINITE : BSAM_SOLVE_MAXCYCLES);
+  *done_out = conv;
+  return n;
 }
 
 int bsam_solve_status(const bsam_ctx *c) { return c ? c->last_status : -1; }
@@ -1162,8 +1242,29 @@ int bsam_hierarchy_begin(bsam_ctx *c) {
   API_BEGIN(c)
   if (!c->host_only) HIPCK(hipStreamSynchronize(c->stream));
   c->staged.clear();
+  c->staged_group.clear();
+  c->staged_eps2.clear();
+  c->staged_ngroups = 1;
   c->committed = false;
   c->building = true;
+  API_END(c)
+}
+
+int bsam_set_groups(bsam_ctx *c, int ngroups, const int *group, const double *eps2) {
+  API_BEGIN(c)
+  if (!c->building) throw Error("set_groups: call it between bsam_hierarchy_begin and commit");
+  if (ngroups < 1 || ngroups > 32767) throw Error("set_groups: ngroups must be 1..32767");
+  if (!group) throw Error("set_groups: null group array");
+  c->staged_group.assign(group, group + c->staged.size());
+  for (int g : c->staged_group)
+    if (g < 0 || g >= ngroups) throw Error("set_groups: group index out of range");
+  c->staged_eps2.clear();
+  if (eps2) {
+    c->staged_eps2.assign(eps2, eps2 + ngroups);
+    for (double e : c->staged_eps2)
+      if (!std::isfinite(e) || e < 0.0) throw Error("set_groups: eps^2 must be finite and >= 0");
+  }
+  c->staged_ngroups = ngroups;
   API_END(c)
 }
 
@@ -1197,8 +1298,16 @@ int bsam_hierarchy_commit(bsam_ctx *c) {
   API_BEGIN(c)
   if (!c->building) throw Error("call bsam_hierarchy_begin first");
   if (!c->host_only) HIPCK(hipSetDevice(c->cfg.device));
-  if (c->had_commit && c->planned_fuse == (c->fuse_ghost != 0) &&
-      same_geometry(c->hp, c->staged)) {
+  // groups (one unless bsam_set_groups staged several, for these patches)
+  const int G = c->staged_ngroups;
+  if (G > 1 && c->staged_group.size() != c->staged.size())
+    throw Error("set_groups was called before every patch was added");
+  for (size_t i = 0; i < c->staged.size(); i++)
+    c->staged[i].group = G > 1 ? c->staged_group[i] : 0;
+  std::vector<double> eps2(G, c->cfg.eps2);
+  if (!c->staged_eps2.empty()) eps2 = c->staged_eps2;
+  if (c->had_commit && c->planned_fuse == (c->fuse_ghost != 0) && G == c->ngroups &&
+      eps2 == c->group_eps2 && same_geometry(c->hp, c->staged)) {
     // The regrid produced the hierarchy already committed (the reference
     // regrids every step, mostly to the same patches): keep its plans,
     // tables, pools and graphs, and zero the pools as a fresh commit does.
@@ -1208,6 +1317,8 @@ int bsam_hierarchy_commit(bsam_ctx *c) {
   } else {
     c->had_commit = false;
     c->free_device();
+    c->ngroups = G;
+    c->group_eps2 = eps2;
     c->hp = std::move(c->staged);
     c->dp.clear();
     c->lev.clear();
@@ -1382,12 +1493,13 @@ int bsam_error_flags_user(bsam_ctx *c, int level, double qtolerance) {
 // of levels finest..0 (level-list order) and of its parent section. `first`
 // gets, per patch of the list and part, the index of its first task.
 static void stats_tasks(bsam_ctx *c, std::vector<int> &list, std::vector<StatTask> &tasks,
-                        std::vector<int> &first) {
+                        std::vector<int> &first, int group = -1) {
   const int nd = c->nd();
   list.clear();
   tasks.clear();
   for (int l = c->finest; l >= 0; l--)
-    for (int p : c->L(l).patches) list.push_back(p);
+    for (int p : c->L(l).patches)
+      if (group < 0 || c->hp[p].group == group) list.push_back(p);
   first.assign(list.size() * 2 + 1, 0);
   for (size_t n = 0; n < list.size(); n++) {
     const HPatch &h = c->hp[list[n]];
@@ -1411,7 +1523,8 @@ static void stats_tasks(bsam_ctx *c, std::vector<int> &list, std::vector<StatTas
 // patch in the reference's order.
 static void stats_run(bsam_ctx *c, const std::vector<int> &list,
                       const std::vector<StatTask> &tasks, const std::vector<int> &first,
-                      StatTask *d_tasks, double *d_out, double *mass, double *energy) {
+                      StatTask *d_tasks, double *d_out, double *mass, double *energy,
+                      double eps2) {
   const int nd = c->nd();
   HIPCK(hipMemcpyAsync(d_tasks, tasks.data(), sizeof(StatTask) * tasks.size(),
                        hipMemcpyHostToDevice, c->stream));
@@ -1438,12 +1551,12 @@ static void stats_run(bsam_ctx *c, const std::vector<int> &list,
     const double hh = h.dx, ch = hh * 2.0;
     if (nd == 2) {
       const double h2 = hh * hh, ch2 = ch * ch;
-      const double ef = h2 * (a[1] + c->cfg.eps2 / h2 / 4.0 * a[2]);
+      const double ef = h2 * (a[1] + eps2 / h2 / 4.0 * a[2]);
       if (h.level == 0) {
         m = m + h2 * a[0];
         e = e + ef;
       } else {
-        const double ec = ch2 * (b[1] + c->cfg.eps2 / ch2 / 4.0 * b[2]);
+        const double ec = ch2 * (b[1] + eps2 / ch2 / 4.0 * b[2]);
         m = m + h2 * a[0] - ch2 * b[0];
         e = e + ef - ec;
       }
@@ -1475,7 +1588,54 @@ int bsam_after_step_stats(bsam_ctx *c, double *mass, double *energy) {
     HIPCK(hipMalloc(&c->d_stats_list, sizeof(StatTask) * tasks.size()));
     c->n_stats = (int)tasks.size();
   }
-  stats_run(c, list, tasks, first, c->d_stats_list, c->d_stats, mass, energy);
+  stats_run(c, list, tasks, first, c->d_stats_list, c->d_stats, mass, energy, c->cfg.eps2);
+  API_END(c)
+}
+
+int bsam_after_step_stats_group(bsam_ctx *c, int group, double *mass, double *energy) {
+  API_BEGIN(c)
+  c->need_device();
+  if (!mass || !energy) throw Error("null output pointer");
+  if (group < 0 || group >= c->ngroups) throw Error("after_step_stats_group: bad group");
+  std::vector<int> list, first;
+  std::vector<StatTask> tasks;
+  stats_tasks(c, list, tasks, first, group);
+  if (tasks.empty()) throw Error("after_step_stats_group: the group has no patch at level >= 0");
+  if (c->n_stats < (int)tasks.size()) {
+    HIPCK(hipStreamSynchronize(c->stream));
+    DeviceLock lock(g_device_lock);
+    if (c->d_stats) (void)hipFree(c->d_stats);
+    if (c->d_stats_list) (void)hipFree(c->d_stats_list);
+    c->d_stats = nullptr;
+    c->d_stats_list = nullptr;
+    c->n_stats = 0;
+    HIPCK(hipMalloc(&c->d_stats, sizeof(double) * 3 * tasks.size()));
+    HIPCK(hipMalloc(&c->d_stats_list, sizeof(StatTask) * tasks.size()));
+    c->n_stats = (int)tasks.size();
+  }
+  stats_run(c, list, tasks, first, c->d_stats_list, c->d_stats, mass, energy,
+            c->group_eps2[(size_t)group]);
+  API_END(c)
+}
+
+int bsam_num_groups(const bsam_ctx *c) { return c ? c->ngroups : -1; }
+
+int bsam_group_result(bsam_ctx *c, int group, int *ncycles, double *hist, int maxhist,
+                      int *status) {
+  API_BEGIN(c)
+  c->need_device();
+  if (group < 0 || group >= c->ngroups) throw Error("group_result: bad group");
+  if (c->group_state.size() != (size_t)(4 * c->ngroups)) throw Error("group_result: no solve yet");
+  const int *st = &c->group_state[(size_t)(4 * group)];
+  if (ncycles) *ncycles = st[0];
+  if (status)
+    *status = st[1] ? BSAM_SOLVE_CONVERGED : (st[2] ? BSAM_SOLVE_NONFINITE : BSAM_SOLVE_MAXCYCLES);
+  if (hist && maxhist > 0) {
+    const int m = std::min(st[0], maxhist);
+    HIPCK(hipMemcpyAsync(hist, c->d_hist + (size_t)group * c->hmax, sizeof(double) * m,
+                         hipMemcpyDeviceToHost, c->stream));
+    HIPCK(hipStreamSynchronize(c->stream));
+  }
   API_END(c)
 }
 
@@ -1520,7 +1680,7 @@ int bsam_after_step_stats_noalloc(bsam_ctx *c, double *mass, double *energy) {
   }
   unsigned char *base = static_cast<unsigned char *>(g_scratch_ptr[dev]);
   stats_run(c, list, tasks, first, reinterpret_cast<StatTask *>(base),
-            reinterpret_cast<double *>(base + kScratchTaskBytes), mass, energy);
+            reinterpret_cast<double *>(base + kScratchTaskBytes), mass, energy, c->cfg.eps2);
   API_END(c)
 }
 
@@ -1818,6 +1978,7 @@ int bsam_error_afas(bsam_ctx *c, int level, double *residual) {
   queue_error(c, level, -1.0);
   double r = 0.0;
   HIPCK(hipMemcpyAsync(&r, c->d_hist, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  reset_state(c);   // a non-finite residual must not stop the next calls' kernels
   HIPCK(hipStreamSynchronize(c->stream));
   if (residual) *residual = r;
   API_END(c)
@@ -1828,11 +1989,12 @@ int bsam_source_norm(bsam_ctx *c, int level, double *rms) {
   LevelData &L = c->L(level);
   int nt = (int)L.t_int.size();
   HIPCK(launch_resnorm(c->nd(), c->dev(), L.d_int, nt, c->d_partial, 1, 1, c->stream));
-  HIPCK(hipMemsetAsync(c->d_state, 0, sizeof(int) * 4, c->stream));
+  reset_state(c);
   HIPCK(launch_finalize(c->d_partial, nt, (double)(c->cfg.nrvars * L.cells), -1.0, c->d_hist,
                         c->d_state, 1, c->stream));
   double r = 0.0;
   HIPCK(hipMemcpyAsync(&r, c->d_hist, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  reset_state(c);
   HIPCK(hipStreamSynchronize(c->stream));
   if (rms) *rms = r;
   API_END(c)

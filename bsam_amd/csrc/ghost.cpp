@@ -704,7 +704,7 @@ int build_ghost_program(const std::vector<HPatch> &hp, const std::vector<int> &p
       g.dst = (int)qoff(hp, dc);
       g.dvs = (int)h.qvs;
       g.kind = e.kind;
-      g.nflip = e.nflip;
+      g.nflip = (short)e.nflip;
       g.f0 = (int)qoff(hp, e.f0);
       g.fvs = (int)hp[e.f0.p].qvs;
       if (e.kind > 0) {
@@ -992,6 +992,16 @@ bool program_reads_avoid(const std::vector<GhostOp> &prog,
       if (hit(g.c[k])) return false;
   }
   return true;
+}
+
+void set_op_groups(const std::vector<HPatch> &hp, std::vector<GhostOp> &ops) {
+  std::vector<long long> qoffs(hp.size());
+  for (size_t n = 0; n < hp.size(); n++) qoffs[n] = hp[n].qoff;
+  for (GhostOp &g : ops) {
+    auto it = std::upper_bound(qoffs.begin(), qoffs.end(), (long long)g.dst);
+    const int p = (int)(it - qoffs.begin()) - 1;
+    g.group = (short)(p >= 0 ? hp[p].group : 0);
+  }
 }
 
 void locate_q_cell(const std::vector<HPatch> &hp, long long off, int &p, int &i, int &j,

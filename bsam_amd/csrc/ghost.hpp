@@ -18,6 +18,7 @@ struct HPatch {
   int mthbc[6] = {1, 1, 1, 1, 1, 1};
   double dx = 0.0, xlower[3] = {0.0, 0.0, 0.0};
   long long qoff = 0, qvs = 0, px = 0, py = 0;
+  int group = 0;   // independent sub-problem the patch belongs to
 };
 
 // Periodic set-up of the hierarchy (PeriodicSetup, boundary.f90:893-963):
@@ -101,6 +102,9 @@ FusedGhost plan_fused_ghost(const std::vector<HPatch> &hp, const std::vector<Til
 // writes exactly those cells.
 bool program_reads_avoid(const std::vector<GhostOp> &prog,
                          const std::vector<unsigned char> &written);
+
+// Sets every update's group to its destination patch's group.
+void set_op_groups(const std::vector<HPatch> &hp, std::vector<GhostOp> &ops);
 
 // Patch and (i,j,k) (lower bound 0; k = 0 in 2D) of a q-pool offset.
 void locate_q_cell(const std::vector<HPatch> &hp, long long off, int &p, int &i, int &j,

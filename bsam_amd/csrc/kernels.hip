@@ -60,6 +60,15 @@ template <> __device__ __forceinline__ Fld<float> fields<float>(const Dev &d) {
   return Fld<float>{d.e, d.qce, d.fe, d.qb};
 }
 
+// A group (independent sub-problem) that has stopped iterating: the kernels
+// of a solve leave its patches and ghost cells untouched.
+__device__ __forceinline__ bool stopped(const Dev &d, int group) {
+  return d.gstate && d.gstate[4 * group + 3];
+}
+__device__ __forceinline__ bool stopped_patch(const Dev &d, int patch) {
+  return d.gstate && d.gstate[4 * d.P[patch].group + 3];
+}
+
 // One patch's arrays, each addressed from its own origin: q(0,0,0,1),
 // qc(0,0,0,1), f(1,1,1,1) and the defect base qb(0,0,0). The arrays may live in
 // the global pools or, in the tail kernels, in LDS; the layout (pitches and
@@ -699,6 +708,7 @@ __global__ __launch_bounds__(RELAX_THREADS) void k_relax(Dev d, const Tile *tile
   const int b = xcd_tile(nt);
   if (b >= nt) return;
   Tile t = tiles[b];
+  if (stopped_patch(d, t.patch)) return;
   Lvl<T> L = level_ref<T>(d, t.patch);
   int i, j, k;
   // RelaxGrid's loop start 1+mod(j+ipass,2) (and +k in 3D) selects the
@@ -734,6 +744,7 @@ __global__ __launch_bounds__(RELAX_THREADS) void k_relax_fused(Dev d, const Tile
   if (b >= nt) return;
   const Tile t = tiles[b];   // loaded together with the update bounds
   const int s0 = start[b], s1 = start[b + 1];
+  if (stopped_patch(d, t.patch)) return;   // the tile's updates are its patch's group's
   ghost_tile<T>(d, s0, s1, ops);
   Lvl<T> L = level_ref<T>(d, t.patch);
   int i, j, k;
@@ -747,7 +758,9 @@ template <class T, bool DEF>
 __global__ __launch_bounds__(256) void k_ghost(Dev d, const GhostOp *ops, int n) {
   int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n) return;
-  ghost_op<T, DEF>(d.nrvars, fields<T>(d).q, ops[e]);
+  const GhostOp &g = ops[e];
+  if (stopped(d, g.group)) return;
+  ghost_op<T, DEF>(d.nrvars, fields<T>(d).q, g);
 }
 
 // A SetGhost program whose reads are disjoint from what the host kernel
@@ -757,14 +770,15 @@ template <class T>
 __device__ __forceinline__ void ghost_alongside(const Dev &d, const GhostOp *ops, int n) {
   T *q = fields<T>(d).q;
   for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x)
-    ghost_op<T, std::is_same<T, float>::value>(d.nrvars, q, ops[e]);
+    if (!stopped(d, ops[e].group))
+      ghost_op<T, std::is_same<T, float>::value>(d.nrvars, q, ops[e]);
 }
 
 template <int ND, class T>
 __global__ __launch_bounds__(256) void k_restrict(Dev d, const Tile *tiles, int nt,
                                                   const GhostOp *gops, int ng) {
   const int b = xcd_tile(nt);
-  if (b < nt) {
+  if (b < nt && !stopped_patch(d, tiles[b].patch)) {
     const Tile t = tiles[b];
     Lvl<T> C = level_ref<T>(d, t.patch);
     int i, j, k;
@@ -781,7 +795,7 @@ template <int ND, class T>
 __global__ __launch_bounds__(256) void k_filldown(Dev d, const Tile *tiles, int nt, int field,
                                                   const GhostOp *gops, int ng) {
   const int b = xcd_tile(nt);
-  if (b < nt) {
+  if (b < nt && !stopped_patch(d, tiles[b].patch)) {
     const Tile t = tiles[b];
     Lvl<T> C = level_ref<T>(d, t.patch);
     int i, j, k;
@@ -798,6 +812,7 @@ __global__ __launch_bounds__(256) void k_coarse_ghost(Dev d, const Tile *tiles, 
   const int b = xcd_tile(nt);
   if (b >= nt) return;
   Tile t = tiles[b];
+  if (stopped_patch(d, t.patch)) return;
   Lvl<T> C = level_ref<T>(d, t.patch);
   int i, j, k;
   tile_cell<ND>(d, t, i, j, k);
@@ -810,6 +825,7 @@ __global__ __launch_bounds__(256) void k_coarse_load(Dev d, const Tile *tiles, i
   const int b = xcd_tile(nt);
   if (b >= nt) return;
   Tile t = tiles[b];
+  if (stopped_patch(d, t.patch)) return;
   Lvl<T> C = level_ref<T>(d, t.patch);
   int i, j, k;
   tile_cell<ND>(d, t, i, j, k);
@@ -827,7 +843,8 @@ __global__ __launch_bounds__(256) void k_coarse_ghost_load(Dev d, const Tile *ti
                                                            const int *tstart,
                                                            const GhostOp *tops) {
   const int b = xcd_tile(nt);
-  if (b < nt) {
+  // the tile's updates are of its patch's parent, in the same group
+  if (b < nt && !stopped_patch(d, tiles[b].patch)) {
     const Tile t = tiles[b];
     const int s0 = tstart ? tstart[b] : 0, s1 = tstart ? tstart[b + 1] : 0;
     ghost_tile<T>(d, s0, s1, tops);
@@ -850,7 +867,7 @@ __global__ __launch_bounds__(256) void k_correct(Dev d, const Tile *tiles, int n
                                                  const GhostOp *gops, int ng, const int *tstart,
                                                  const GhostOp *tops) {
   const int b = xcd_tile(nt);
-  if (b < nt) {
+  if (b < nt && !stopped_patch(d, tiles[b].patch)) {
     const Tile t = tiles[b];
     const int s0 = tstart ? tstart[b] : 0, s1 = tstart ? tstart[b + 1] : 0;
     ghost_tile<T>(d, s0, s1, tops);
@@ -1145,6 +1162,7 @@ __global__ __launch_bounds__(256) void k_defect_setup(Dev d, const Tile *tiles, 
   const int b = xcd_tile(nt);
   if (b >= nt) return;
   Tile t = tiles[b];
+  if (stopped_patch(d, t.patch)) return;
   const DPatch &P = d.P[t.patch];
   int i, j, k;
   if (!lin_qcell<ND>(t, P, i, j, k)) return;
@@ -1156,6 +1174,7 @@ __global__ __launch_bounds__(256) void k_apply_correction(Dev d, const Tile *til
   const int b = xcd_tile(nt);
   if (b >= nt) return;
   Tile t = tiles[b];
+  if (stopped_patch(d, t.patch)) return;
   const DPatch &P = d.P[t.patch];
   int i, j, k;
   tile_cell_px<ND>(d, t, i, j, k);
@@ -1460,6 +1479,7 @@ __global__ __launch_bounds__(1024) void k_tail_vcycle(Dev d, const TailLevel *__
                                                        const TailCopy *cin, int nin,
                                                        const TailCopy *cout, int nout) {
   const int set = DEF ? 1 : 0;
+  if (stopped_patch(d, lv[0].patch)) return;   // one chain, one group
   auto in_lds = [&](int l) { return lds && lv[l].lds[set][0] >= 0; };
   // each level's ghost updates are loaded one visit ahead, so their latency
   // overlaps the previous level's work
@@ -1494,6 +1514,7 @@ template <int ND>
 __global__ __launch_bounds__(1024) void k_tail_filldown(Dev d, const TailLevel *lv, int nlev,
                                                          int field) {
   const int nv = d.nrvars;
+  if (stopped_patch(d, lv[0].patch)) return;
   for (int l = nlev - 1; l >= 1; l--) {
     Lvl<double> C = level_ref<double>(d, lv[l].patch), PA = level_ref<double>(d, lv[l - 1].patch);
     TailPass gp;
@@ -1564,31 +1585,44 @@ __global__ __launch_bounds__(256) void k_resnorm(Dev d, const Tile *tiles, int n
 // second component's partials following the first's (errortype 2,
 // lib/afasroutines.f90:128-142; a NaN in either wins); state[0] counts the
 // calls, state[1] is set once a residual falls below tol and state[2] once a
-// residual is not finite.
+// residual is not finite, state[3] (stopped) once either is. One workgroup
+// per group: group g (blockIdx.x) sums its tiles granges[2g] ..
+// granges[2g] + granges[2g+1] - 1 of the n partials with count gcount[g], in
+// the order a solve of that group alone sums its own, and writes
+// hist[g * hstride + state[4g]] and state[4g .. 4g+3]; without granges the
+// single group takes all n tiles and `count`. A group already stopped keeps
+// its history and state.
 __global__ __launch_bounds__(256) void k_finalize(const double *partial, int n, double count,
-                                                   double tol, double *hist, int *state,
-                                                   int comps) {
+                                                   const int *granges, const double *gcount,
+                                                   double tol, double *hist, int hstride,
+                                                   int *state, int comps) {
+  const int g = blockIdx.x;
+  int *st = state + 4 * g;
+  if (st[3]) return;
+  const int t0 = granges ? granges[2 * g] : 0, m = granges ? granges[2 * g + 1] : n;
+  const double cnt = granges ? gcount[g] : count;
   double s = 0.0;
-  for (int e = threadIdx.x; e < n; e += blockDim.x) s += partial[e];
+  for (int e = threadIdx.x; e < m; e += blockDim.x) s += partial[t0 + e];
   double tot = block_sum(s);
   double tot2 = 0.0;
   if (comps == 2) {
     __syncthreads();
     s = 0.0;
-    for (int e = threadIdx.x; e < n; e += blockDim.x) s += partial[n + e];
+    for (int e = threadIdx.x; e < m; e += blockDim.x) s += partial[n + t0 + e];
     tot2 = block_sum(s);
   }
   if (threadIdx.x == 0) {
-    double r = sqrt(tot / count);
+    double r = sqrt(tot / cnt);
     if (comps == 2) {
-      const double r2 = sqrt(tot2 / count);
+      const double r2 = sqrt(tot2 / cnt);
       r = (isnan(r) || r >= r2) ? r : r2;
     }
-    int it = state[0];
-    hist[it] = r;
-    state[0] = it + 1;
-    if (r < tol) state[1] = 1;
-    if (!isfinite(r)) state[2] = 1;
+    int it = st[0];
+    hist[(long long)g * hstride + it] = r;
+    st[0] = it + 1;
+    if (r < tol) st[1] = 1;
+    if (!isfinite(r)) st[2] = 1;
+    if (st[1] || st[2]) st[3] = 1;
   }
 }
 
@@ -1766,8 +1800,10 @@ hipError_t launch_resnorm(int nd, const Dev &d, const Tile *tiles, int nt, doubl
 }
 
 hipError_t launch_finalize(const double *partial, int n, double count, double tol,
-                           double *hist, int *state, int comps, hipStream_t s) {
-  k_finalize<<<1, 256, 0, s>>>(partial, n, count, tol, hist, state, comps);
+                           double *hist, int *state, int comps, hipStream_t s,
+                           int ngroups, const int *granges, const double *gcount, int hstride) {
+  k_finalize<<<ngroups, 256, 0, s>>>(partial, n, count, granges, gcount, tol, hist, hstride,
+                                     state, comps);
   return hipGetLastError();
 }
 

@@ -60,7 +60,9 @@ hipError_t launch_apply_correction(int nd, const Dev &d, const Tile *tiles, int 
 hipError_t launch_resnorm(int nd, const Dev &d, const Tile *tiles, int nt, double *partial,
                           int what, int comps, hipStream_t s);
 hipError_t launch_finalize(const double *partial, int n, double count, double tol,
-                           double *hist, int *state, int comps, hipStream_t s);
+                           double *hist, int *state, int comps, hipStream_t s,
+                           int ngroups = 1, const int *granges = nullptr,
+                           const double *gcount = nullptr, int hstride = 0);
 hipError_t launch_copy(double *dst, const double *src, long long n, hipStream_t s);
 // Tail kernels: one workgroup of `threads` over levels 0..nlev-1 of the
 // single-patch chain; the V-cycle with `lds` bytes of dynamic LDS

@@ -162,6 +162,10 @@ def lib():
         for n in ("bsam_source", "bsam_copy_qold", "bsam_sync"):
             getattr(L, n).argtypes = [vp]
         L.bsam_error_afas.argtypes = [vp, C.c_int, dp]
+        L.bsam_set_groups.argtypes = [vp, C.c_int, ip, dp]
+        L.bsam_num_groups.argtypes = [vp]
+        L.bsam_group_result.argtypes = [vp, C.c_int, ip, dp, C.c_int, ip]
+        L.bsam_after_step_stats_group.argtypes = [vp, C.c_int, dp, dp]
         L.bsam_source_norm.argtypes = [vp, C.c_int, dp]
         L.bsam_copy_rate.argtypes = [C.c_int, C.c_longlong, C.c_int, dp]
         for n in ("bsam_multigrid_iterations", "bsam_time_step"):
@@ -458,3 +462,115 @@ class Engine:
             assert a.flags.f_contiguous and a.dtype == np.float64
             ptrs[n] = _dptr(a)
         self._ck(lib().bsam_ghost_program_apply_host(self.ctx, level, ipass, ptrs))
+
+
+class BatchEngine(Engine):
+    """Several independent hierarchies of one shape class (same ndims and
+    minlevel) solved side by side in one engine context, one group each
+    (bsam_set_groups): every level launch spans all of them, each keeps its
+    own eps and residual test, and a group that has converged is left
+    untouched, so each hierarchy's fields end as they would alone.
+
+    Group g's patches keep their geometry but are moved apart in the global
+    index space (x offset 4 g times the level's extent), so no two groups'
+    ghost-extended boxes meet. Patches are added level by level, each level
+    group by group in the hierarchy's own level-list order.
+    """
+
+    def __init__(self, layouts: List[Layout], eps: List[float], dt: float, alpha: float = 1.0,
+                 well: str = "reference", nsmoothingpasses: int = 2,
+                 maxvcycles: int = 20, qerrortol: float = 1e-8,
+                 precision: int = PREC_FP64, device: int = 0,
+                 tune: Optional[Dict[str, float]] = None, errortype: int = 1):
+        if not layouts or len(layouts) != len(eps):
+            raise BsamError("BatchEngine needs one eps per layout")
+        nd, minl = layouts[0].ndims, layouts[0].minlevel
+        for lay in layouts:
+            if lay.ndims != nd or lay.minlevel != minl:
+                raise BsamError("batched hierarchies must share ndims and minlevel")
+        self.layouts = layouts
+        self.eps = [float(e) for e in eps]
+        super().__init__(layouts[0], dt, self.eps[0], alpha, well, nsmoothingpasses,
+                         maxvcycles, qerrortol, precision, device, tune, errortype)
+
+    def describe(self, layout: Layout):
+        L = lib()
+        lays = self.layouts
+        nd = lays[0].ndims
+        lo = min(l.minlevel for l in lays)
+        hi = max(l.finestlevel for l in lays)
+        self.index = {}          # (group, patch index in its layout) -> engine index
+        self.patches = []
+        groups = []
+        maps = [lay.index_map() for lay in lays]
+        self._ck(L.bsam_hierarchy_begin(self.ctx))
+        n = 0
+        for level in range(lo, hi + 1):
+            for g, lay in enumerate(lays):
+                root = lay.patches[0]
+                # the level's extent in x and the group's offset in it
+                w = root.mx[0] * 2 ** level if level >= 0 else root.mx[0] // 2 ** (-level)
+                if level < 0 and root.mx[0] % 2 ** (-level):
+                    raise BsamError("batched roots must halve down to their coarsest level")
+                shift = 4 * g * max(w, 1)
+                order = lay.ordered()
+                for k, p in enumerate(order):
+                    if p.level != level:
+                        continue
+                    mb = [v for d in range(3) for v in p.mbounds[d]]
+                    mg = [v for d in range(3) for v in p.mglobal[d]]
+                    mg[0] += shift
+                    mg[1] += shift
+                    par = (self.index[(g, maps[g][p.parent_id])]
+                           if p.parent_id is not None else -1)
+                    self._ck(L.bsam_add_patch(self.ctx, n, p.level, par, _ia(p.mx), _ia(mb),
+                                              _ia(mg), _ia(p.mthbc), p.dx,
+                                              (C.c_double * 3)(*p.xlower)))
+                    self.index[(g, k)] = n
+                    self.patches.append(p)
+                    groups.append(g)
+                    n += 1
+        eps2 = (C.c_double * len(lays))(*[e * e for e in self.eps])
+        self._ck(L.bsam_set_groups(self.ctx, len(lays), _ia(groups), eps2))
+        self._ck(L.bsam_hierarchy_commit(self.ctx))
+        self._group_layout = {self.index[key]: lays[key[0]] for key in self.index}
+
+    def upload(self, n: int, name: str, arr: np.ndarray, group: int = 0):
+        """Field `name` of patch n (index in group `group`'s own layout order)."""
+        e = self.index[(group, n)]
+        a = np.asfortranarray(arr, dtype=np.float64)
+        lay = self.layouts[group]
+        shape = lay.field_shape(lay.ordered()[n], name)
+        if a.shape != shape:
+            raise BsamError(f"shape {a.shape} != {shape}")
+        flat = a.reshape(-1, order="F")
+        self._ck(lib().bsam_upload(self.ctx, e, FIELDS[name], _dptr(flat), flat.size))
+
+    def download(self, n: int, name: str, group: int = 0) -> np.ndarray:
+        e = self.index[(group, n)]
+        lay = self.layouts[group]
+        shape = lay.field_shape(lay.ordered()[n], name)
+        flat = np.empty(int(np.prod(shape)), dtype=np.float64)
+        self._ck(lib().bsam_download(self.ctx, e, FIELDS[name], _dptr(flat), flat.size))
+        return flat.reshape(shape, order="F")
+
+    def group_results(self):
+        """Per group of the last solve: (residual history, converged)."""
+        out = []
+        for g in range(len(self.layouts)):
+            n, st = C.c_int(), C.c_int()
+            hist = (C.c_double * self.maxvcycles)()
+            self._ck(lib().bsam_group_result(self.ctx, g, C.byref(n), hist, self.maxvcycles,
+                                             C.byref(st)))
+            out.append((list(hist[:n.value]), st.value == 0))
+        return out
+
+    def time_steps(self):
+        """One time step of every group: per group (history, converged)."""
+        self.time_step()
+        return self.group_results()
+
+    def after_step_stats(self, group: int = 0):
+        m, e = C.c_double(), C.c_double()
+        self._ck(lib().bsam_after_step_stats_group(self.ctx, group, C.byref(m), C.byref(e)))
+        return m.value, e.value

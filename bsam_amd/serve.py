@@ -125,12 +125,56 @@ def footprint_estimate(r: Request, precision: int) -> int:
     return b
 
 
+def solve_batch(reqs: List[Request], precision: int, device: int = 0, keep_phi: bool = False,
+                tune: Optional[Dict[str, float]] = None) -> List[Result]:
+    """Runs requests of one shape class (same ndims, side, steps and dt)
+    start to finish in one engine context, one group each
+    (engine.BatchEngine): every launch of a time step spans all of them, each
+    request keeps its own eps and residual test and is left untouched once
+    converged, so its fields end as they would alone (tests/test_serve.py)."""
+    r0 = reqs[0]
+    for r in reqs:
+        if (r.ndims, r.side, r.steps, r.dt) != (r0.ndims, r0.side, r0.steps, r0.dt):
+            raise ValueError("a batch holds requests of one shape class")
+    lays = [request_layout(r) for r in reqs]
+    eng = E.BatchEngine(lays, [r.eps for r in reqs], dt=r0.dt, well="plusminus",
+                        maxvcycles=60, qerrortol=1e-8, precision=precision, device=device,
+                        tune=tune)
+    foot = eng.footprint_bytes()
+    out = [Result(r.rid, r.arrival, 0.0, 0.0, footprint=foot // len(reqs)) for r in reqs]
+    for g, (r, lay) in enumerate(zip(reqs, lays)):
+        q0 = noise_q(lay, r.seed, 0.0, 0.05)
+        for n, p in enumerate(lay.ordered()):
+            eng.upload(n, "q", q0[p.id], group=g)
+    for _ in range(r0.steps):
+        for g, (hist, conv) in enumerate(eng.time_steps()):
+            out[g].cycles.append(len(hist))
+            out[g].converged = out[g].converged and conv
+            if keep_phi:
+                out[g].mass.append(eng.after_step_stats(g)[0])
+    if keep_phi:
+        for g, lay in enumerate(lays):
+            out[g].phi = eng.download(lay.index_map()[0], "q", group=g)
+    eng.close()
+    return out
+
+
+def shape_class(r: Request):
+    return (r.ndims, r.side, r.steps, r.dt)
+
+
 class Server:
+    """Continuous-batching server. With batch = 1 every admitted request runs
+    on its own context; with batch > 1 the requests waiting when a slot frees
+    are admitted in batches of up to `batch` requests of the head request's
+    shape class (solve_batch), so small requests share their launches."""
+
     def __init__(self, budget_bytes: int = 64 << 30, max_active: int = 8,
                  precision: int = E.PREC_MIXED, device: int = 0,
                  tune: Optional[Dict[str, float]] = None, keep_phi: bool = False,
-                 solve_fn=None, footprint_fn=None):
+                 solve_fn=None, footprint_fn=None, batch: int = 1, batch_fn=None):
         self.solve_fn = solve_fn or solve_request
+        self.batch_fn = batch_fn or solve_batch
         self.footprint_fn = footprint_fn or footprint_estimate
         self.budget = budget_bytes
         self.max_active = max_active
@@ -138,6 +182,7 @@ class Server:
         self.device = device
         self.tune = tune
         self.keep_phi = keep_phi
+        self.batch = max(1, int(batch))
 
     def run(self, requests: List[Request], realtime: bool = True) -> Dict:
         """Serve `requests`; with realtime, each is released at its arrival
@@ -156,22 +201,39 @@ class Server:
         t0 = time.perf_counter()
         now = lambda: time.perf_counter() - t0
 
-        def worker(r: Request, admitted: float):
+        def worker(unit: List[Request], admitted: float):
             try:
-                res = self.solve_fn(r, self.precision, self.device, self.keep_phi, self.tune)
-                res.arrival = r.arrival if realtime else 0.0
-                res.admitted = admitted
-                res.finished = now()
+                if len(unit) == 1 and self.batch == 1:
+                    res = [self.solve_fn(unit[0], self.precision, self.device, self.keep_phi,
+                                         self.tune)]
+                else:
+                    res = self.batch_fn(unit, self.precision, self.device, self.keep_phi,
+                                        self.tune)
+                fin = now()
+                for r, x in zip(unit, res):
+                    x.arrival = r.arrival if realtime else 0.0
+                    x.admitted = admitted
+                    x.finished = fin
                 with cv:
-                    results.append(res)
+                    results.extend(res)
             except BaseException as e:  # reported after the run
                 with cv:
                     errors.append(e)
             finally:
                 with cv:
-                    used[0] -= foot[r.rid]
+                    used[0] -= sum(foot[r.rid] for r in unit)
                     active[0] -= 1
                     cv.notify_all()
+
+        def next_unit(queue: List[Request]) -> List[Request]:
+            """The queue head and the waiting requests of its shape class,
+            up to `batch`, as many as the free memory takes (FIFO)."""
+            head = queue[0]
+            unit = [head] + [r for r in queue[1:] if shape_class(r) == shape_class(head)]
+            unit = unit[:self.batch]
+            while unit and used[0] + sum(foot[r.rid] for r in unit) > self.budget:
+                unit.pop()
+            return unit
 
         threads = []
         queue: List[Request] = []
@@ -182,14 +244,17 @@ class Server:
                 while idx < len(pending) and (not realtime or pending[idx].arrival <= t):
                     queue.append(pending[idx])
                     idx += 1
-                while queue and active[0] < self.max_active and \
-                        used[0] + foot[queue[0].rid] <= self.budget:
-                    r = queue.pop(0)
-                    used[0] += foot[r.rid]
+                while queue and active[0] < self.max_active:
+                    unit = next_unit(queue)
+                    if not unit:
+                        break
+                    for r in unit:
+                        queue.remove(r)
+                    used[0] += sum(foot[r.rid] for r in unit)
                     active[0] += 1
                     peak[0] = max(peak[0], used[0])
                     peak[1] = max(peak[1], active[0])
-                    th = threading.Thread(target=worker, args=(r, now()), daemon=True)
+                    th = threading.Thread(target=worker, args=(unit, now()), daemon=True)
                     th.start()
                     threads.append(th)
                 if idx >= len(pending) and not queue and not active[0]:
@@ -249,10 +314,12 @@ def main():
     ap.add_argument("--baseline", action="store_true", help="also run the FP64 serial baseline")
     ap.add_argument("--burst", action="store_true",
                     help="all requests waiting at t = 0 (capacity rather than arrival-bound)")
+    ap.add_argument("--batch", type=int, default=1,
+                    help="requests of one shape class admitted together into one context")
     args = ap.parse_args()
     reqs = make_requests(args.requests, args.rate, steps=args.steps)
     prec = E.PREC_MIXED if args.precision == "mixed" else E.PREC_FP64
-    srv = Server(int(args.budget_gb * (1 << 30)), args.max_active, prec)
+    srv = Server(int(args.budget_gb * (1 << 30)), args.max_active, prec, batch=args.batch)
     out = srv.run(reqs, realtime=not args.burst)
     out["peak_active"] = srv.peak_active
     out["peak_bytes"] = srv.peak_used

@@ -94,6 +94,27 @@ int bsam_add_patch(bsam_ctx *ctx, int id, int level, int parent, const int *mx,
                    const int *mbounds, const int *mglobal, const int *mthbc,
                    double dx, const double *xlower);
 int bsam_hierarchy_commit(bsam_ctx *ctx);
+/* Independent sub-problems in one hierarchy (no counterpart in the
+ * reference; for serving many requests with shared launches). Called after
+ * the last bsam_add_patch and before the commit: group[p] (0..ngroups-1)
+ * for each patch added, eps2[g] the problem's eps^2 for group g (NULL: the
+ * configuration's for all). The groups' patches must not touch one
+ * another's (place each group's root apart in the global index space), and
+ * on every level each group's patches must be consecutive in the level
+ * list. A solve (bsam_multigrid_iterations, bsam_time_step) then runs every
+ * level launch over all groups, tests each group's residual on its own
+ * finest-level patches (summed in the order a solve of that group alone
+ * sums them) and leaves a group untouched from the iteration its test
+ * passes (or its residual is not finite), so each group's fields end as
+ * they would in a context of its own. The solve's return values are the
+ * largest iteration count and whether every group converged; per group:
+ * bsam_group_result. Without this call the hierarchy is one group. */
+int bsam_set_groups(bsam_ctx *ctx, int ngroups, const int *group, const double *eps2);
+int bsam_num_groups(const bsam_ctx *ctx);
+/* Group g's last solve: iterations, residual history (up to maxhist) and
+ * status (BSAM_SOLVE_*). */
+int bsam_group_result(bsam_ctx *ctx, int group, int *ncycles, double *hist, int maxhist,
+                      int *status);
 int bsam_num_patches(const bsam_ctx *ctx);
 /* Device bytes held by the committed hierarchy's field pools (the quantity a
  * server admits requests by; no counterpart in the reference). */
@@ -151,6 +172,9 @@ int bsam_error_flags_user(bsam_ctx *ctx, int level, double qtolerance);
  * are reduced on the device in a fixed order and combined on the host in the
  * reference's order. Synchronises the stream. */
 int bsam_after_step_stats(bsam_ctx *ctx, double *mass, double *energy);
+/* The same statistics over one group's patches (bsam_set_groups), with that
+ * group's eps. */
+int bsam_after_step_stats_group(bsam_ctx *ctx, int group, double *mass, double *energy);
 /* The same statistics without allocating device memory. No counterpart in
  * the reference: it is for callers that cannot allocate (tight serving
  * loops that report the statistics per step, harnesses that forbid

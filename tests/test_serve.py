@@ -76,3 +76,63 @@ def test_results_same_alone_and_in_the_mix(precision):
         assert res.converged and a.converged
         assert res.cycles == a.cycles
         assert np.array_equal(res.phi, a.phi), res.rid
+
+
+def test_batches_form_by_shape_class_within_budget():
+    """A fake batch solve (no GPU): admitted units hold one shape class, at
+    most `batch` requests, and never exceed the memory budget."""
+    reqs = [serve.Request(i, 2 if i % 3 else 3, 64 if i % 2 else 32, 0.03, i,
+                          arrival=0.0, steps=1) for i in range(14)]
+    units, lock = [], threading.Lock()
+
+    def fake_batch(unit, prec, dev, keep, tune):
+        with lock:
+            units.append([r.rid for r in unit])
+        time.sleep(0.02)
+        return [serve.Result(r.rid, r.arrival, 0.0, 0.0, cycles=[4]) for r in unit]
+
+    srv = serve.Server(budget_bytes=300, max_active=2, batch=3, batch_fn=fake_batch,
+                       footprint_fn=lambda r, p: 100)
+    out = srv.run(reqs, realtime=False)
+    assert out["requests"] == 14 and srv.peak_used <= 300
+    by = {r.rid: r for r in reqs}
+    seen = []
+    for u in units:
+        assert 1 <= len(u) <= 3
+        assert len({serve.shape_class(by[i]) for i in u}) == 1
+        seen += u
+    assert sorted(seen) == list(range(14))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [E.PREC_FP64, E.PREC_MIXED])
+@pytest.mark.parametrize("ndims,side", [(2, 64), (3, 16)])
+def test_batched_solves_match_solo(precision, ndims, side):
+    """Requests solved together in one context (one group each) end with the
+    fields, V-cycle counts and per-step mass of each request solved alone,
+    bit for bit, although their eps (hence their convergence) differ."""
+    reqs = [serve.Request(i, ndims, side, 0.01 + 0.012 * i, 40 + i, steps=3) for i in range(4)]
+    batch = serve.solve_batch(reqs, precision, keep_phi=True)
+    for r, b in zip(reqs, batch):
+        a = serve.solve_request(r, precision, keep_phi=True)
+        assert b.rid == r.rid and a.converged and b.converged
+        assert b.cycles == a.cycles, (r.rid, a.cycles, b.cycles)
+        assert b.mass == a.mass, r.rid
+        assert np.array_equal(b.phi, a.phi), r.rid
+    assert len({tuple(b.cycles) for b in batch}) > 1   # the groups stopped apart
+
+
+@pytest.mark.gpu
+def test_batched_server_matches_unbatched():
+    reqs = [serve.Request(i, 2, 64 if i % 2 else 128, 0.02 + 0.004 * i, 60 + i, steps=2)
+            for i in range(8)]
+    one = {r.rid: r for r in serve.Server(budget_bytes=1 << 30, max_active=4,
+                                          precision=E.PREC_FP64, keep_phi=True)
+           .run(reqs, realtime=False)["results"]}
+    srv = serve.Server(budget_bytes=1 << 30, max_active=4, precision=E.PREC_FP64,
+                       keep_phi=True, batch=4)
+    out = srv.run(reqs, realtime=False)
+    assert out["requests"] == 8
+    for res in out["results"]:
+        a = one[res.rid]
+        assert res.cycles == a.cycles and np.array_equal(res.phi, a.phi), res.rid
