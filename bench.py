@@ -206,6 +206,11 @@ def main():
                          "difference (max |phi|, |mu| and mass) after the steps")
     ap.add_argument("--inner-cycles", type=int, default=1,
                     help="mixed precision: FP32 V-cycles per FP64 defect-correction step")
+    ap.add_argument("--serving", type=int, default=256,
+                    help="requests of the serving configuration to serve after the solve "
+                         "measurement (0 to skip)")
+    ap.add_argument("--serving-rate", type=float, default=50.0,
+                    help="Poisson arrival rate of the served requests (per second)")
     ap.add_argument("--no-profile", action="store_true",
                     help="skip the event-timed pass (roofline null)")
     args = ap.parse_args()
@@ -357,10 +362,38 @@ def main():
     if check:
         out["check"] = check
     eng.close()
+    if args.serving > 0:
+        out["serving"] = serving(args.serving, args.serving_rate)
     if args.cpu_budget > 0:
         out["cpu_baseline"] = (cpu_baseline_reference(layout, q0, params, args.cpu_budget) or
                                cpu_baseline_oracle(layout, q0, params, args.cpu_budget))
     print(json.dumps(out))
+
+
+def serving(n, rate):
+    """The serving configuration: n Cahn-Hilliard solve requests with Poisson
+    arrivals at `rate` per second (2D side in {64..1024} or 3D side in
+    {32, 64, 128}, eps ~ U(0.01, 0.05), seeded noise, 20 time steps each to
+    the 1e-8 residual), served with continuous batching (batches of up to 8
+    requests of one shape class per context, 8 contexts at a time, admission
+    by HBM footprint) in mixed precision; beside it the same requests solved
+    one after another in plain FP64 (the serial baseline)."""
+    from bsam_amd import engine as E
+    from bsam_amd import serve
+    reqs = serve.make_requests(n, rate, seed=5, steps=20)
+    srv = serve.Server(budget_bytes=64 << 30, max_active=8, precision=E.PREC_MIXED, batch=8)
+    got = srv.run(reqs, realtime=True)
+    base = serve.serial(reqs, E.PREC_FP64)
+    return {
+        "requests": n, "arrival_rate_per_s": rate, "steps_per_request": 20,
+        "mode": "mixed precision, batches of <= 8 same-shape requests, <= 8 active contexts",
+        "solves_per_s": got["solves_per_s"], "requests_per_s": got["requests_per_s"],
+        "p50_latency_s": got["p50_latency_s"], "p99_latency_s": got["p99_latency_s"],
+        "makespan_s": got["makespan_s"], "all_converged": got["all_converged"],
+        "peak_active_contexts": srv.peak_active, "peak_bytes": srv.peak_used,
+        "serial_fp64_solves_per_s": base["solves_per_s"],
+        "speedup_vs_serial_fp64": got["solves_per_s"] / base["solves_per_s"],
+    }
 
 
 if __name__ == "__main__":
