@@ -352,8 +352,13 @@ def main():
             "relative_residual": last_res / rms_f if rms_f else None,
             # TakeTimeSteps' after-step statistics of the final state
             "mass": mass, "energy": energy if layout.ndims == 2 else None,
+            # level-kernel tile shape, chosen at commit by timing the relaxation
+            # under each candidate shape on this hierarchy (tile_auto)
             "tile": [eng.tune_get("tile_x"), eng.tune_get("tile_y")] +
                     ([eng.tune_get("tile_z")] if layout.ndims == 3 else []),
+            "tile_auto": {"on": bool(eng.tune_get("tile_auto")),
+                          "candidates_timed": int(eng.tune_get("tile_auto_candidates")),
+                          "relax_rounds_us": eng.tune_get("tile_auto_us")},
         },
         "roofline": roof,
         "kernels": kinds,

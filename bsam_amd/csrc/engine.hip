@@ -1,10 +1,12 @@
 // engine.hip -- host side of the FAS multigrid engine: hierarchy commit,
 // ghost-fill programs, level orchestration and the C ABI of bsam_gpu.h.
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <stdexcept>
 #include <climits>
+#include <map>
 #include <mutex>
 #include <string>
 #include <unordered_map>
@@ -207,6 +209,13 @@ struct bsam_ctx {
   hipGraphExec_t graph_exec = nullptr, graph_mixed_exec = nullptr;
 
   int tile_lx = -1, tile_ly = -1;   // tile extents (log2); -1: per-dimension default
+  // tile shape chosen at commit by timing the relaxation (unless tile_x /
+  // tile_y were set): the choice, its time and whether it came from the
+  // process-wide cache of hierarchy shape classes
+  bool tile_user = false;
+  int tile_auto = 1;
+  double tile_auto_us = 0.0;
+  int tile_auto_cached = 0, tile_auto_candidates = 0;
   int tx() const { return 1 << tile_lx; }
   int ty() const { return 1 << tile_ly; }
   int tz() const { return nd() == 3 ? 256 >> (tile_lx + tile_ly) : 1; }
@@ -500,6 +509,146 @@ static bool same_geometry(const std::vector<HPatch> &a, const std::vector<HPatch
   return true;
 }
 
+// The tile shape of the level kernels, chosen per hierarchy: every candidate
+// shape times both colours of the plain FP64 relaxation (the bulk of a
+// V-cycle's launches) on each level of at least 4096 cells at or above the
+// root, three rounds between HIP events on the context's stream, over
+// scratch pools of the hierarchy's size. The first candidate is the default
+// shape; another replaces it only if more than 3 % faster, since the timing
+// leaves out the ghost updates the fused relaxations apply, whose cost also
+// depends on the shape (a full-solve sweep of the 3D workload ranks the
+// default first, 1.3 % ahead of the relaxation-only winner). Tile shape changes
+// which workgroup relaxes a cell, never its arithmetic. Winners are kept per
+// shape class (dimension and the patch extents of every level) for the
+// process, so recommitting a hierarchy of the same shape does not time again.
+static std::mutex g_tile_cache_mutex;
+static std::map<std::string, std::pair<int, int>> g_tile_cache;
+
+static void choose_tiles(bsam_ctx *c) {
+  const int nd = c->nd(), minl = c->cfg.minlevel;
+  std::vector<std::vector<int>> levels;   // patches of each timed level
+  std::string key = std::to_string(nd) + ":";
+  for (int l = 0; l <= c->finest; l++) {
+    std::vector<int> ps;
+    long long cells = 0;
+    std::vector<std::array<int, 3>> ext;
+    for (int p = 0; p < (int)c->hp.size(); p++)
+      if (c->hp[p].level == l) {
+        ps.push_back(p);
+        const HPatch &h = c->hp[p];
+        cells += (long long)h.mx[0] * h.mx[1] * (nd == 3 ? h.mx[2] : 1);
+        ext.push_back({h.mx[0], h.mx[1], h.mx[2]});
+      }
+    std::sort(ext.begin(), ext.end());
+    key += "L" + std::to_string(l);
+    for (auto &e : ext) key += "," + std::to_string(e[0]) + "x" + std::to_string(e[1]) + "x" +
+                                 std::to_string(e[2]);
+    if (cells >= 4096) levels.push_back(ps);
+  }
+  (void)minl;
+  if (levels.empty()) return;
+  {
+    std::lock_guard<std::mutex> g(g_tile_cache_mutex);
+    auto it = g_tile_cache.find(key);
+    if (it != g_tile_cache.end()) {
+      c->tile_lx = it->second.first;
+      c->tile_ly = it->second.second;
+      c->tile_auto_cached = 1;
+      return;
+    }
+  }
+  // candidate shapes (log2 extents): 256 threads' cells per tile
+  std::vector<std::pair<int, int>> cand;
+  if (nd == 2) cand = {{5, 3}, {6, 2}, {4, 4}, {7, 1}, {3, 5}};
+  else cand = {{4, 3}, {3, 3}, {4, 2}, {5, 2}, {4, 4}, {5, 3}, {3, 2}, {5, 1}};
+  double *q = nullptr, *f = nullptr;
+  DPatch *dP = nullptr;
+  Tile *dt = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipStream_t s = c->stream;
+  double best = 1e300;
+  int bl = c->tile_lx, bly = c->tile_ly;
+  hipError_t err = hipSuccess;
+  {
+    DeviceLock lock(g_device_lock);
+    err = hipMalloc(&q, sizeof(double) * c->nq);
+    if (err == hipSuccess) err = hipMalloc(&f, sizeof(double) * c->nf);
+    if (err == hipSuccess) err = hipEventCreate(&e0);
+    if (err == hipSuccess) err = hipEventCreate(&e1);
+  }
+  if (err == hipSuccess) err = hipMemsetAsync(q, 0, sizeof(double) * c->nq, s);
+  if (err == hipSuccess) err = hipMemsetAsync(f, 0, sizeof(double) * c->nf, s);
+  if (err == hipSuccess) {
+    DeviceLock lock(g_device_lock);
+    dP = dev_upload(c->dp, s);
+  }
+  for (auto &cd : cand) {
+    if (err != hipSuccess) break;
+    const int lx = cd.first, ly = cd.second;
+    const int TX = 1 << lx, TY = 1 << ly, TZ = nd == 3 ? 256 >> (lx + ly) : 1;
+    std::vector<Tile> tiles;
+    std::vector<std::pair<int, int>> ranges;   // per timed level: first tile, count
+    for (auto &ps : levels) {
+      const int first = (int)tiles.size();
+      for (int p : ps) {
+        const HPatch &h = c->hp[p];
+        for (int k = (nd == 3 ? 1 : 0); k < (nd == 3 ? 1 + h.mx[2] : 1); k += TZ)
+          for (int j = 1; j < 1 + h.mx[1]; j += TY)
+            for (int i = 1; i < 1 + h.mx[0]; i += TX) tiles.push_back(Tile{p, i, j, k});
+      }
+      ranges.push_back({first, (int)tiles.size() - first});
+    }
+    {
+      DeviceLock lock(g_device_lock);
+      dt = dev_upload(tiles, s);
+    }
+    Dev d = c->dev();
+    d.q = q;
+    d.f = f;
+    d.P = dP;
+    d.gstate = nullptr;
+    d.tile_lx = lx;
+    d.tile_ly = ly;
+    for (int r = 0; r < 4 && err == hipSuccess; r++) {   // round 0 warms up
+      if (r == 1) err = hipEventRecord(e0, s);
+      for (auto &rg : ranges)
+        for (int color = 1; color <= 2 && err == hipSuccess; color++)
+          err = launch_relax(nd, PREC_SOLUTION_FP64, d, dt + rg.first, rg.second, color, s);
+    }
+    if (err == hipSuccess) err = hipEventRecord(e1, s);
+    if (err == hipSuccess) err = hipEventSynchronize(e1);
+    float ms = 0.f;
+    if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+    {
+      DeviceLock lock(g_device_lock);
+      (void)hipFree(dt);
+      dt = nullptr;
+    }
+    c->tile_auto_candidates++;
+    const double margin = c->tile_auto_candidates == 1 ? 1.0 : 0.97;
+    if (err == hipSuccess && ms / 3.0 < best * margin) {
+      best = ms / 3.0;
+      bl = lx;
+      bly = ly;
+    }
+  }
+  {
+    DeviceLock lock(g_device_lock);
+    (void)hipStreamSynchronize(s);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (q) (void)hipFree(q);
+    if (f) (void)hipFree(f);
+    if (dP) (void)hipFree(dP);
+  }
+  HIPCK(err);
+  c->tile_lx = bl;
+  c->tile_ly = bly;
+  c->tile_auto_us = best * 1e3;
+  std::lock_guard<std::mutex> g(g_tile_cache_mutex);
+  g_tile_cache[key] = {bl, bly};
+}
+
 static void commit(bsam_ctx *c) {
   const int nd = c->nd(), nv = c->cfg.nrvars;
   int n = (int)c->hp.size();
@@ -507,6 +656,7 @@ static void commit(bsam_ctx *c) {
   // tile shape: 32 x 8 in 2D and 16 x 8 x 2 in 3D unless tuned (the fastest
   // of the tile sweep on the 2D and 3D AMR workloads)
   int &lx = c->tile_lx, &ly = c->tile_ly;
+  if (!c->tile_user) lx = ly = -1;   // the defaults, or the chooser's pick below
   if (nd == 2) {
     if (lx < 0 && ly < 0) lx = 5, ly = 3;
     else if (lx < 0) lx = 8 - ly;
@@ -617,6 +767,7 @@ static void commit(bsam_ctx *c) {
     c->nf += D.fvs * nv;
   }
   if (c->nq >= (1LL << 31)) throw Error("q pool exceeds 2^31 elements");
+  if (!c->host_only && c->tile_auto && !c->tile_user) choose_tiles(c);
   Periodic per;
   if (c->finest >= 0) {
     const HPatch &root = c->hp[c->lev[0 - minl].patches[0]];
@@ -2043,6 +2194,10 @@ int bsam_tune(bsam_ctx *c, const char *key, double value) {
     if (v < 1 || (1 << lg) != v || v > 256) throw Error(k + " must be a power of two <= 256");
     if (k == "tile_x" && v < 2) throw Error("tile_x must be at least 2");
     (k == "tile_x" ? c->tile_lx : c->tile_ly) = lg;
+    c->tile_user = true;
+  } else if (k == "tile_auto") {
+    if (c->committed || c->building) throw Error("tile_auto must be set before the hierarchy");
+    c->tile_auto = value != 0.0;
   } else if (k == "tail_lds") {
     if (c->committed || c->building) throw Error("tail_lds must be set before the hierarchy");
     c->tail_lds_cap = (long long)value;
@@ -2083,6 +2238,10 @@ int bsam_tune_get(const bsam_ctx *cc, const char *key, double *value) {
   else if (k == "tile_y") *value = c->ty();
   else if (k == "tile_z") *value = c->tz();
   else if (k == "tail_top") *value = c->tail_top;
+  else if (k == "tile_auto") *value = c->tile_auto && !c->tile_user;
+  else if (k == "tile_auto_us") *value = c->tile_auto_us;
+  else if (k == "tile_auto_cached") *value = c->tile_auto_cached;
+  else if (k == "tile_auto_candidates") *value = c->tile_auto_candidates;
   else if (k == "tail_lds_fp64") *value = (double)c->tail_lds[0];
   else if (k == "tail_lds_fp32") *value = (double)c->tail_lds[1];
   else throw Error("unknown tuning key " + k);

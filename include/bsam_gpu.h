@@ -313,7 +313,14 @@ int bsam_regrid_level(const bsam_regrid_params *rp, int npatch, const int *mx,
  *   "tile_x", "tile_y"  level-kernel tile extents (powers of two, tile_x
  *                  at least 2; 2D tile_x * tile_y = 256, default 32 x 8; 3D
  *                  the third extent fills 256 threads, default 16 x 8 x 2).
- *                  Before the hierarchy.
+ *                  Before the hierarchy; setting either turns tile_auto off.
+ *   "tile_auto"    1 (default): each commit of a device context picks the
+ *                  tile shape by timing the plain relaxation under each
+ *                  candidate shape on the hierarchy's levels of >= 4096
+ *                  cells (the default shape is kept unless another is more
+ *                  than 3 % faster); winners are cached per hierarchy shape
+ *                  for the process. 0: the default shape. Before the
+ *                  hierarchy.
  *   "fuse_ghost"   1 (default): apply each SetGhost inside another level
  *                  launch where the level allows (the following relaxation,
  *                  the level's restriction or fill-down, the child level's
@@ -328,7 +335,10 @@ int bsam_tune(bsam_ctx *ctx, const char *key, double value);
  * "tail_top" (highest level run by the tail kernel; below minlevel when
  * none), "tail_lds_fp64" and "tail_lds_fp32" (dynamic LDS bytes of the tail
  * kernels), "commits_reused" (commits that found the hierarchy already
- * committed). No counterpart in the reference. */
+ * committed), "tile_z", and of the last commit's tile choice
+ * "tile_auto_candidates" (shapes timed; 0 when taken from the cache) and
+ * "tile_auto_us" (the chosen shape's relaxation rounds). No counterpart in
+ * the reference. */
 int bsam_tune_get(const bsam_ctx *ctx, const char *key, double *value);
 
 /* Kernel timing. Between bsam_profile_begin and bsam_profile_end, every
