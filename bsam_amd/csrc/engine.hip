@@ -170,6 +170,7 @@ struct bsam_ctx {
   int staged_ngroups = 1;
   std::vector<int> group_state;            // per group after the last solve: 4 ints
   int last_status = -1;      // of the last solve: BSAM_SOLVE_* (bsam_solve_status)
+  int last_iters = 0;        // iterations the last solve took
   // kernel timing with HIP events on the context's stream (bsam_profile_*)
   unsigned prof_mask = 0;
   std::vector<hipEvent_t> ev_pool;
@@ -1187,9 +1188,19 @@ static int iterate(bsam_ctx *c, hipGraph_t *g, hipGraphExec_t *exec, F body, int
   if (graph) capture(c, g, exec, body);
   const int G = c->ngroups;
   for (int g = 0; g < 4 * G; g++) c->h_state[g] = 0;
-  for (int it = 1; it <= c->cfg.maxvcycles; it++) {
-    if (graph) HIPCK(hipGraphLaunch(*exec, c->stream));
-    else body();
+  // Iterations are queued back to back, as many as the previous solve took
+  // less one, before the first read-back of the stop state, then one at a
+  // time: the kernels leave a solve that has met its test untouched, so
+  // queued iterations past convergence change nothing (their launches return
+  // at once) and the result equals stopping at the test.
+  int it = 0;
+  while (it < c->cfg.maxvcycles) {
+    const int k = it == 0 ? std::max(1, std::min(c->last_iters - 1, c->cfg.maxvcycles)) : 1;
+    for (int r = 0; r < k; r++) {
+      if (graph) HIPCK(hipGraphLaunch(*exec, c->stream));
+      else body();
+    }
+    it += k;
     HIPCK(hipMemcpyAsync(c->h_state, c->d_state, sizeof(int) * 4 * G, hipMemcpyDeviceToHost,
                          c->stream));
     HIPCK(hipStreamSynchronize(c->stream));
@@ -1210,6 +1221,7 @@ static int iterate(bsam_ctx *c, hipGraph_t *g, hipGraphExec_t *exec, F body, int
     nonfinite = nonfinite || c->h_state[4 * g + 2];
     n = std::max(n, c->h_state[4 * g]);   // histories recorded: iterations run
   }
+  c->last_iters = n;
   c->last_status = conv ? BSAM_SOLVE_CONVERGED
                         : (nonfinite ? BSAM_SOLVE_NONFINITE : BSAM_SOLVE_MAXCYCLES);
   *done_out = conv;

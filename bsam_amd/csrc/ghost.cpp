@@ -15,10 +15,9 @@
 // property and refuses programs that violate it.
 #include "ghost.hpp"
 
-#include <climits>
-
 #include <algorithm>
 #include <array>
+#include <climits>
 #include <cstring>
 #include <initializer_list>
 #include <map>
@@ -801,8 +800,10 @@ FusedGhost plan_fused_ghost(const std::vector<HPatch> &hp, const std::vector<Til
     // the interior cell a face ghost borders is the only reader of the ghost
     bool read = outside == 1 && relaxed(c[0], c[1], nd == 3 ? c[2] : 0);
     if (!read && (size_t)g.dst < next_dst.size() && next_dst[(size_t)g.dst]) continue;
-    int owner = tile_of(p, c[0], c[1], c[2]);
-    bool crosses = false;
+    // the tiles of this launch that relax a source of the update (coarse
+    // sources belong to another level; f0/f1 are the level's cells)
+    int src_tile = -1;
+    bool several = false;
     long long srcs[2] = {g.f0, g.f1};
     int nsrc = g.kind == 0 ? 1 : 2;
     for (int s2 = 0; s2 < nsrc; s2++) {
@@ -814,11 +815,24 @@ FusedGhost plan_fused_ghost(const std::vector<HPatch> &hp, const std::vector<Til
       }
       if (first[sp] < 0 || !inside(sp, si, sj, sk)) continue;   // not relaxed here
       if (!relaxed(si, sj, nd == 3 ? sk : 0)) continue;
-      if (tile_of(sp, si, sj, sk) != owner) crosses = true;
+      int t = tile_of(sp, si, sj, sk);
+      if (src_tile >= 0 && t != src_tile) several = true;
+      src_tile = t;
     }
-    // an update that reads a cell another tile relaxes in the same launch
-    // runs in a launch of its own just before
-    if (crosses) out.pre.push_back(g);
+    // The update must read its sources before they are relaxed and, when the
+    // relaxation reads its destination, be written before that read. The
+    // tile that relaxes the sources applies it before relaxing; an update
+    // whose destination is read goes to the reading tile, and to `pre` if a
+    // source is relaxed by another tile (or by several tiles).
+    int owner;
+    if (read) {
+      owner = tile_of(p, c[0], c[1], c[2]);
+      if (src_tile >= 0 && src_tile != owner) owner = -1;
+    } else {
+      owner = src_tile >= 0 ? src_tile : tile_of(p, c[0], c[1], c[2]);
+    }
+    if (several) owner = -1;
+    if (owner < 0) out.pre.push_back(g);
     else per[owner].push_back(g);
   }
   out.start.assign(tiles.size() + 1, 0);
@@ -827,8 +841,65 @@ FusedGhost plan_fused_ghost(const std::vector<HPatch> &hp, const std::vector<Til
     out.ops.insert(out.ops.end(), per[t].begin(), per[t].end());
   }
   out.kept = (int)(out.ops.size() + out.pre.size());
+  if (!fused_plan_race_free(hp, tiles, nd, tx, ty, tz, color, out, why)) return out;
   out.ok = true;
   return out;
+}
+
+// Checks a fused plan from the cells' side: marks every cell the launch
+// relaxes with its tile and every ghost cell a relaxed cell's stencil reads
+// with the reading tile, then requires each tile's updates to read relaxed
+// cells of that tile only and to write ghost cells that no other tile reads.
+bool fused_plan_race_free(const std::vector<HPatch> &hp, const std::vector<Tile> &tiles, int nd,
+                          int tx, int ty, int tz, int color, const FusedGhost &plan,
+                          std::string &why) {
+  if (tiles.empty()) return true;
+  long long lo = LLONG_MAX, hi = 0;
+  for (const Tile &t : tiles) {
+    const HPatch &h = hp[t.patch];
+    lo = std::min(lo, h.qoff);
+    hi = std::max(hi, h.qoff + h.qvs);
+  }
+  std::vector<int> relaxed_by((size_t)(hi - lo), -1), read_by((size_t)(hi - lo), -1);
+  const int nz = nd == 3 ? tz : 1;
+  for (size_t t = 0; t < tiles.size(); t++) {
+    const Tile &tl = tiles[t];
+    const HPatch &h = hp[tl.patch];
+    const int mz = nd == 3 ? h.mx[2] : 1;
+    for (int k = tl.k0; k < tl.k0 + nz; k++)
+      for (int j = tl.j0; j < tl.j0 + ty; j++)
+        for (int i = tl.i0; i < tl.i0 + tx; i++) {
+          const int kk = nd == 3 ? k : 0;
+          if (i > h.mx[0] || j > h.mx[1] || (nd == 3 && k > mz)) continue;
+          if (((i + j + kk + color) & 1) != 1) continue;
+          relaxed_by[(size_t)(h.qoff + qcell(h.px, h.py, i, j, kk) - lo)] = (int)t;
+          const int di[6] = {1, -1, 0, 0, 0, 0}, dj[6] = {0, 0, 1, -1, 0, 0},
+                    dk[6] = {0, 0, 0, 0, 1, -1};
+          for (int n = 0; n < 2 * nd; n++) {
+            const int a = i + di[n], b = j + dj[n], c = kk + dk[n];
+            const bool ghost = a < 1 || a > h.mx[0] || b < 1 || b > h.mx[1] ||
+                               (nd == 3 && (c < 1 || c > mz));
+            if (!ghost) continue;
+            int &r = read_by[(size_t)(h.qoff + qcell(h.px, h.py, a, b, c) - lo)];
+            r = (r == -1 || r == (int)t) ? (int)t : -2;
+          }
+        }
+  }
+  auto at = [&](const std::vector<int> &m, long long off) {
+    return off >= lo && off < hi ? m[(size_t)(off - lo)] : -1;
+  };
+  for (size_t t = 0; t < tiles.size(); t++)
+    for (int e = plan.start[t]; e < plan.start[t + 1]; e++) {
+      const GhostOp &g = plan.ops[(size_t)e];
+      const int r = at(relaxed_by, g.f0), r1 = g.kind > 0 ? at(relaxed_by, g.f1) : -1;
+      const int w = at(read_by, g.dst);
+      if ((r >= 0 && r != (int)t) || (r1 >= 0 && r1 != (int)t) ||
+          (w != -1 && w != (int)t) || at(relaxed_by, g.dst) >= 0) {
+        why = "fused ghost update shared between tiles";
+        return false;
+      }
+    }
+  return true;
 }
 
 // Parent cell (ci, cj, ck) of the child's coarse index space (0..cmx+1; the

@@ -77,12 +77,16 @@ RideGhost plan_ride(const std::vector<GhostOp> &prog,
 // applies them (ops[start[t] .. start[t+1]) for tile t of the level's
 // interior tile list). Updates whose destination the relaxation does not read
 // and that the next program, SetGhost(level, color), rewrites are dropped
-// (their value is never observed). A kept update that reads a cell of colour
-// `color` outside its own tile goes to `pre`, a program of its own the
-// engine launches just before the fused relaxation (it then reads the cells
-// before they are relaxed, as SetGhost before RelaxPatch does); the updates
-// in `ops` read nothing another tile of the launch writes, so the tiles never
-// race.
+// (their value is never observed). A kept update goes to the tile that
+// relaxes its sources of colour `color` (it reads them before they are
+// relaxed, as SetGhost before RelaxPatch does), or, if it has none, to the
+// tile of the interior cell its destination borders; an update whose
+// destination the relaxation reads goes to the reading tile. An update that
+// fits none of these (a source relaxed by another tile than the one reading
+// the destination, or sources in several tiles) goes to `pre`, a program of
+// its own the engine launches just before the fused relaxation. The updates
+// in `ops` read nothing another tile of the launch writes and write nothing
+// another tile reads, so the tiles never race.
 struct FusedGhost {
   bool ok = false;
   int kept = 0;                // ops + pre
@@ -96,6 +100,12 @@ FusedGhost plan_fused_ghost(const std::vector<HPatch> &hp, const std::vector<Til
                             int ndims, int tx, int ty, int tz,
                             const std::vector<GhostOp> &prog, const std::vector<GhostOp> &next,
                             int color, std::string &why);
+// Race check of a plan (run by plan_fused_ghost before it accepts one): each
+// tile's updates read only cells that tile relaxes (or cells the launch does
+// not relax) and write only ghost cells that no other tile's stencil reads.
+bool fused_plan_race_free(const std::vector<HPatch> &hp, const std::vector<Tile> &tiles, int nd,
+                          int tx, int ty, int tz, int color, const FusedGhost &plan,
+                          std::string &why);
 
 // True if no update of `prog` reads a q-pool cell (variable-1 offset) in
 // `written`: the program may then run in the same launch as a kernel that

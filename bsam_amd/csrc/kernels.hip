@@ -329,6 +329,36 @@ __device__ __forceinline__ void coarse_ghost_cell(int nrvars, const Lvl<T> &C,
   for (int v = 0; v < nrvars; v++) cq.p[cc + v * cq.vs] = pq.p[pc + v * pq.vs];
 }
 
+// R(f - A_h q) at coarse cell (i,j,k): the fine residual of its 2^d children,
+// child s at (2i - (s&1), 2j - (s>>1 & 1)(, 2k - (s>>2 & 1))), restricted as
+// Restriction2D/3D does (gridutilities.f90:89-110, :406-433), the sum taken in
+// child order and scaled by 1/2^d. The children are visited one at a time and
+// summed as they come, which is the same sum in the same order, so only one
+// fine stencil is live in registers.
+template <int ND, class T, bool DEF>
+__device__ __forceinline__ void restricted_residual(const Problem &prm, const Lvl<T> &C,
+                                                    const View<T> &a, int i, int j, int k,
+                                                    T &R1, T &R2) {
+  const DPatch &P = C.P;
+  const T *f1 = C.f, *f2 = C.f + P.fvs;
+  constexpr int NC = ND == 2 ? 4 : 8;
+  T s1 = (T)0, s2 = (T)0;
+#pragma unroll 1
+  for (int s = 0; s < NC; s++) {
+    int fi = 2 * i - (s & 1), fj = 2 * j - ((s >> 1) & 1);
+    int fk = ND == 3 ? 2 * k - ((s >> 2) & 1) : 0;
+    T base = DEF ? C.qb[a.at(fi, fj, fk)] : (T)0;
+    T o1, o2;
+    op_at<ND, T, DEF>(prm, a, base, P, fi, fj, fk, o1, o2);
+    long long fo = fat(P, fi, fj, ND == 3 ? fk : 1);
+    const T r1 = f1[fo] - o1, r2 = f2[fo] - o2;
+    s1 = s == 0 ? r1 : s1 + r1;
+    s2 = s == 0 ? r2 : s2 + r2;
+  }
+  R1 = (T)(ND == 2 ? 0.25 : 0.125) * s1;
+  R2 = (T)(ND == 2 ? 0.25 : 0.125) * s2;
+}
+
 // CoarseLoadingFunction (afasroutines.f90:646-714): parent f(mbounds) =
 // R(f - A_h q) + A_2h(qc), the FAS right-hand side, with the fine residual
 // recomputed in registers for the 2^d children of coarse cell (i,j,k).
@@ -337,29 +367,8 @@ __device__ __forceinline__ void coarse_load_cell(const Problem &prm, const Lvl<T
                                                  const Lvl<T> &PA, int i, int j, int k) {
   const DPatch &P = C.P;
   View<T> a = qview(C), cq = qcview(C);
-  const T *f1 = C.f, *f2 = C.f + P.fvs;
-  constexpr int NC = ND == 2 ? 4 : 8;
-  T r1[NC], r2[NC];
-#pragma unroll
-  for (int s = 0; s < NC; s++) {
-    int fi = 2 * i - (s & 1), fj = 2 * j - ((s >> 1) & 1);
-    int fk = ND == 3 ? 2 * k - ((s >> 2) & 1) : 0;
-    long long cidx = a.at(fi, fj, fk);
-    T base = DEF ? C.qb[cidx] : (T)0;
-    T o1, o2;
-    op_at<ND, T, DEF>(prm, a, base, P, fi, fj, fk, o1, o2);
-    long long fo = fat(P, fi, fj, ND == 3 ? fk : 1);
-    r1[s] = f1[fo] - o1;
-    r2[s] = f2[fo] - o2;
-  }
   T R1, R2;
-  if constexpr (ND == 2) {
-    R1 = (T)0.25 * (((r1[0] + r1[1]) + r1[2]) + r1[3]);
-    R2 = (T)0.25 * (((r2[0] + r2[1]) + r2[2]) + r2[3]);
-  } else {
-    R1 = (T)0.125 * (((((((r1[0] + r1[1]) + r1[2]) + r1[3]) + r1[4]) + r1[5]) + r1[6]) + r1[7]);
-    R2 = (T)0.125 * (((((((r2[0] + r2[1]) + r2[2]) + r2[3]) + r2[4]) + r2[5]) + r2[6]) + r2[7]);
-  }
+  restricted_residual<ND, T, DEF>(prm, C, a, i, j, k, R1, R2);
   int pi = P.mb[0][0] + i - 1, pj = P.mb[1][0] + j - 1,
       pk = ND == 3 ? P.mb[2][0] + k - 1 : 1;
   T base = (T)0;
@@ -422,29 +431,8 @@ __device__ __forceinline__ void coarse_ghost_load_cell(const Problem &prm, int n
     return;
   }
   View<T> a = qview(C), cq = qcview(C), pq = qview(PA);
-  const T *f1 = C.f, *f2 = C.f + P.fvs;
-  constexpr int NC = ND == 2 ? 4 : 8;
-  T r1[NC], r2[NC];
-#pragma unroll
-  for (int s = 0; s < NC; s++) {
-    int fi = 2 * i - (s & 1), fj = 2 * j - ((s >> 1) & 1);
-    int fk = ND == 3 ? 2 * k - ((s >> 2) & 1) : 0;
-    long long cidx = a.at(fi, fj, fk);
-    T base = DEF ? C.qb[cidx] : (T)0;
-    T o1, o2;
-    op_at<ND, T, DEF>(prm, a, base, P, fi, fj, fk, o1, o2);
-    long long fo = fat(P, fi, fj, ND == 3 ? fk : 1);
-    r1[s] = f1[fo] - o1;
-    r2[s] = f2[fo] - o2;
-  }
   T R1, R2;
-  if constexpr (ND == 2) {
-    R1 = (T)0.25 * (((r1[0] + r1[1]) + r1[2]) + r1[3]);
-    R2 = (T)0.25 * (((r2[0] + r2[1]) + r2[2]) + r2[3]);
-  } else {
-    R1 = (T)0.125 * (((((((r1[0] + r1[1]) + r1[2]) + r1[3]) + r1[4]) + r1[5]) + r1[6]) + r1[7]);
-    R2 = (T)0.125 * (((((((r2[0] + r2[1]) + r2[2]) + r2[3]) + r2[4]) + r2[5]) + r2[6]) + r2[7]);
-  }
+  restricted_residual<ND, T, DEF>(prm, C, a, i, j, k, R1, R2);
   int pi = P.mb[0][0] + i - 1, pj = P.mb[1][0] + j - 1,
       pk = ND == 3 ? P.mb[2][0] + k - 1 : 1;
   T base = (T)0;
@@ -1592,7 +1580,7 @@ __global__ __launch_bounds__(256) void k_resnorm(Dev d, const Tile *tiles, int n
 // hist[g * hstride + state[4g]] and state[4g .. 4g+3]; without granges the
 // single group takes all n tiles and `count`. A group already stopped keeps
 // its history and state.
-__global__ __launch_bounds__(256) void k_finalize(const double *partial, int n, double count,
+__global__ __launch_bounds__(1024) void k_finalize(const double *partial, int n, double count,
                                                    const int *granges, const double *gcount,
                                                    double tol, double *hist, int hstride,
                                                    int *state, int comps) {
@@ -1802,8 +1790,10 @@ hipError_t launch_resnorm(int nd, const Dev &d, const Tile *tiles, int nt, doubl
 hipError_t launch_finalize(const double *partial, int n, double count, double tol,
                            double *hist, int *state, int comps, hipStream_t s,
                            int ngroups, const int *granges, const double *gcount, int hstride) {
-  k_finalize<<<ngroups, 256, 0, s>>>(partial, n, count, granges, gcount, tol, hist, hstride,
-                                     state, comps);
+  // one workgroup reads all partials of its group: a full-size group keeps
+  // 16 waves of loads in flight
+  k_finalize<<<ngroups, 1024, 0, s>>>(partial, n, count, granges, gcount, tol, hist, hstride,
+                                      state, comps);
   return hipGetLastError();
 }
 

@@ -76,6 +76,26 @@ def test_fused_plans_cover_the_2d_levels():
             assert dropped > 0, (name, level)
 
 
+def test_fused_plans_pass_the_race_check_in_3d():
+    """Every transition of every 3D level has an accepted fused plan: the
+    engine checks each plan from the cells' side (every cell the launch
+    relaxes marked with its tile, every ghost a relaxed stencil reads marked
+    with the reading tile) and refuses one whose tile updates read another
+    tile's relaxed cells or write a ghost another tile reads. The updates
+    launched ahead of the relaxation (tangentially diagonal fine sources in
+    a neighbouring tile) stay a minority of the kept ones."""
+    from bsam_amd import engine as E
+    for name in ("amr3d", "amr3d_user_bc", "amr3d_mixed_bc", "periodic3d_xz"):
+        lay = LAYOUTS[name]()
+        eng = Engine(lay, device=-1, **KW)
+        for level in range(lay.minlevel, lay.finestlevel + 1):
+            for ipass, color in ((0, 1), (1, 2), (2, 1)):
+                n = eng.fused_ghost_size(level, ipass, color)
+                assert n >= 0, (name, level, ipass, color)
+                pre = E.lib().bsam_fused_ghost_pre_size(eng.ctx, level, ipass, color)
+                assert 0 <= pre <= n // 2 + 1, (name, level, ipass, color, pre, n)
+
+
 @pytest.mark.parametrize("name", sorted(LAYOUTS))
 def test_ghost_riding_restriction_commutes(name):
     """Where the engine runs SetGhost(level, p) in the same launch as the
