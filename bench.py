@@ -24,7 +24,11 @@ cell of the level for one colour's half-sweep; the other kinds' figures are
 in include/bsam_gpu.h), peak = the MI355X's 8 TB/s. traffic = the measured
 HBM bytes per launch of that kernel on this workload, from the committed
 rocprofv3 counter summary profiles/<kernel>_<workload>_pmc.md (FETCH_SIZE and
-WRITE_SIZE calibrated on a copy kernel), null when there is none. The line
+WRITE_SIZE calibrated on a copy kernel), null when there is none. The events
+are stream markers and also count each launch's dispatch latency (a few us);
+kernel_trace_avg_us / frac_kernel_trace give the same kernel's mean duration
+from the committed kernel-trace summary profiles/<kernel>_<workload>_kernel_trace.md
+and the fraction it implies. The line
 also carries the measured copy rate of the device, the whole V-cycle's
 algorithmic bytes per second (296 B per cell per V(2,2) level in 2D, 284 in
 3D, 96 at the coarsest level, plus 104 B per finest-level cell per iteration)
@@ -91,6 +95,20 @@ def measured_traffic(kernel, workload):
             for line in fh:
                 if line.startswith("traffic_bytes_per_launch"):
                     return float(line.split("=")[1])
+    except OSError:
+        pass
+    return None
+
+
+def kernel_trace_avg_us(kernel, workload):
+    """Mean kernel duration (us) of `kernel` on `workload` in the committed
+    rocprofv3 kernel-trace summary, or None."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_kernel_trace.md")
+    try:
+        with open(path) as fh:
+            for line in fh:
+                if line.startswith("avg_ns"):
+                    return float(line.split("=")[1]) / 1e3
     except OSError:
         pass
     return None
@@ -213,6 +231,10 @@ def main():
                     help="Poisson arrival rate of the served requests (per second)")
     ap.add_argument("--no-profile", action="store_true",
                     help="skip the event-timed pass (roofline null)")
+    ap.add_argument("--tile", default=None,
+                    help="level-kernel tile as X,Y (powers of two; the z extent fills 256 "
+                         "cells in 3D) instead of the shape timed at commit, e.g. for "
+                         "counter runs whose serialised kernels would distort that timing")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the solver path here is single-GPU "
@@ -226,7 +248,10 @@ def main():
     finest = layout.finestlevel
 
     def make_engine(precision):
-        tune = {"inner_cycles": args.inner_cycles} if precision == E.PREC_MIXED else None
+        tune = {"inner_cycles": args.inner_cycles} if precision == E.PREC_MIXED else {}
+        if args.tile:
+            tx, ty = (int(v) for v in args.tile.split(","))
+            tune.update(tile_x=tx, tile_y=ty)
         eng = E.Engine(layout, dt=params["dt"], eps=params["eps"], well=params["well"],
                        nsmoothingpasses=params["nsmoothingpasses"], maxvcycles=50,
                        qerrortol=params["qerrortol"], precision=precision, device=0,
@@ -302,8 +327,9 @@ def main():
         bpl = v["bytes"] / v["launches"]
         achieved = bpl / avg_s / 1e9
         kname = KERNELS[dom]
-        traffic = measured_traffic(kname, args.config + ("_mixed" if prec == E.PREC_MIXED
-                                                         else ""))
+        wname = args.config + ("_mixed" if prec == E.PREC_MIXED else "")
+        traffic = measured_traffic(kname, wname)
+        trace_us = kernel_trace_avg_us(kname, wname)
         copy_gbs = E.copy_rate(0, 1 << 30, 20)
         vb = vcycle_bytes(layout)
         roof = {
@@ -314,6 +340,11 @@ def main():
             "launches": v["launches"], "share_of_gpu_time": v["ms"] / total_ms,
             "copy_peak_GBs": copy_gbs, "frac_of_copy_peak": achieved / copy_gbs,
             "traffic_ratio": traffic / bpl if traffic else None,
+            # the committed kernel trace's mean duration of the same kernel:
+            # the events above also count each launch's dispatch latency
+            "kernel_trace_avg_us": trace_us,
+            "frac_kernel_trace": bpl / (trace_us * 1e-6) / 1e9 / HBM_PEAK_GBS if trace_us
+            else None,
             "vcycle_bytes": vb,
             "vcycle_GBs": vb * cycles / elapsed / 1e9,
             "vcycle_frac": vb * cycles / elapsed / 1e9 / HBM_PEAK_GBS,
