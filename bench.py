@@ -20,8 +20,9 @@ initial state with HIP events around every kernel launch on the engine's
 stream (bsam_profile_*; graphs off). The kernel kind with the largest summed
 duration is the dominant kernel; achieved = its algorithmic bytes per launch
 / its mean launch duration (relaxation: 48 B per cell-update, i.e. 24 B per
-cell of the level for one colour's half-sweep; the other kinds' figures are
-in include/bsam_gpu.h), peak = the MI355X's 8 TB/s. traffic = the measured
+cell of the level for one colour's half-sweep, plus 32 B per ghost update the
+fused relaxation applies; the other kinds' figures are in
+include/bsam_gpu.h), peak = the MI355X's 8 TB/s. traffic = the measured
 HBM bytes per launch of that kernel on this workload, from the committed
 rocprofv3 counter summary profiles/<kernel>_<workload>_pmc.md (FETCH_SIZE and
 WRITE_SIZE calibrated on a copy kernel), null when there is none. The events

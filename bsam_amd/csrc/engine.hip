@@ -74,6 +74,13 @@ struct LevelData {
   int *d_fstart[3] = {nullptr, nullptr, nullptr};
   GhostOp *d_fops[3] = {nullptr, nullptr, nullptr};
   GhostOp *d_fpre[3] = {nullptr, nullptr, nullptr};   // launched before the fused relaxation
+  // the next launch's pre updates applied at the end of a relaxation launch,
+  // per launch kind (post_key): the updates and, for the next launch, what
+  // still runs before it
+  PostGhost post[4];
+  int *d_pstart[4] = {nullptr, nullptr, nullptr, nullptr};
+  GhostOp *d_pops[4] = {nullptr, nullptr, nullptr, nullptr};
+  GhostOp *d_prest[4] = {nullptr, nullptr, nullptr, nullptr};
   // the parent's SetGhost(0) inside this level's coarse ghost + loading
   // launch, and its SetGhost(2) (the last of post-smoothing) inside this
   // level's correction launch
@@ -100,6 +107,19 @@ int fused_index(int ipass, int color) {
   return -1;
 }
 const int kFusedPass[3] = {0, 1, 2}, kFusedColor[3] = {1, 2, 1};
+// Relaxation launches that can end with the next launch's pre updates, by the
+// SetGhost pending before them and their colour: 0: nothing pending, colour 1
+// (the first of pre-smoothing); 1: SetGhost(0) -> 1; 2: SetGhost(2) -> 1;
+// 3: SetGhost(1) -> 2. The next launch is SetGhost(colour) -> the other
+// colour.
+int post_key(int pending, int color) {
+  if (pending < 0 && color == 1) return 0;
+  if (pending == 0 && color == 1) return 1;
+  if (pending == 2 && color == 1) return 2;
+  if (pending == 1 && color == 2) return 3;
+  return -1;
+}
+const int kPostPending[4] = {-1, 0, 2, 1}, kPostColor[4] = {1, 1, 1, 2};
 
 // The SetGhost programs of a single-patch level with offsets relative to the
 // patch's q origin, for the tail kernels; f1 carries the source packed with
@@ -204,6 +224,7 @@ struct bsam_ctx {
   std::vector<int> tcin_n[2], tcout_n[2];
   int use_graph = 1;
   int fuse_ghost = 1;               // apply SetGhost inside the next relaxation
+  int post_ghost = 1;               // and the next launch's pre updates at a launch's end
   int tail_top = 0;                 // highest level run by the tail kernels
   TailLevel *d_tail = nullptr;      // levels minlevel..tail_top
   hipGraph_t graph = nullptr, graph_mixed = nullptr;
@@ -299,6 +320,11 @@ void bsam_ctx::free_device() {
       F(l.d_fstart[p]);
       F(l.d_fops[p]);
       F(l.d_fpre[p]);
+    }
+    for (int k = 0; k < 4; k++) {
+      F(l.d_pstart[k]);
+      F(l.d_pops[k]);
+      F(l.d_prest[k]);
     }
     for (LevelData::DRide *r : {&l.d_ride_cgl, &l.d_ride_corr}) {
       F(r->rest);
@@ -825,6 +851,17 @@ static void commit(bsam_ctx *c) {
                                     L.gp[kFusedPass[f]], L.gp[kFusedColor[f]], kFusedColor[f],
                                     why);
     }
+    for (int k = 0; k < 4 && c->fuse_ghost; k++) {
+      L.post[k] = PostGhost();
+      const int color = kPostColor[k], pend = kPostPending[k];
+      const int fb = pend >= 0 ? fused_index(pend, color) : -1;
+      const int fn = fused_index(color, 3 - color);
+      if (!L.fused[fn].ok || L.fused[fn].pre.empty()) continue;
+      std::string why;
+      L.post[k] = plan_post_ghost(c->hp, L.t_int, nd, c->tx(), c->ty(), c->tz(), color,
+                                  fb >= 0 && L.fused[fb].ok ? &L.fused[fb] : nullptr,
+                                  L.fused[fn].pre, why);
+    }
     if (l > minl) {
       // parent cells under this level's patches: what its restriction and
       // fill-down write
@@ -883,6 +920,12 @@ static void commit(bsam_ctx *c) {
         L.d_fstart[f] = dev_upload(L.fused[f].start, s);
         L.d_fops[f] = dev_upload(L.fused[f].ops, s);
         L.d_fpre[f] = dev_upload(L.fused[f].pre, s);
+      }
+    for (int k = 0; k < 4; k++)
+      if (L.post[k].ok) {
+        L.d_pstart[k] = dev_upload(L.post[k].start, s);
+        L.d_pops[k] = dev_upload(L.post[k].ops, s);
+        L.d_prest[k] = dev_upload(L.post[k].rest, s);
       }
     auto up_ride = [&](const RideGhost &r, LevelData::DRide &d) {
       if (!r.ok) return;
@@ -965,7 +1008,8 @@ static void commit(bsam_ctx *c) {
 static void op_set_ghost(bsam_ctx *c, int level, int ipass, int prec = PREC_SOLUTION_FP64) {
   LevelData &L = c->L(level);
   if (ipass < 0 || ipass > 2) throw Error("ipass must be 0, 1 or 2");
-  ProfScope ps(c, BSAM_PROF_GHOST, 32.0 * L.gp[ipass].size());
+  ProfScope ps(c, BSAM_PROF_GHOST,
+               (prec == PREC_SOLUTION_FP64 ? 32.0 : 16.0) * L.gp[ipass].size());
   HIPCK(launch_ghost(prec, c->dev(), L.d_gp[ipass], (int)L.gp[ipass].size(), c->stream));
 }
 static void op_relax(bsam_ctx *c, int level, int color, int prec = PREC_SOLUTION_FP64) {
@@ -1093,23 +1137,47 @@ static int level_relax(bsam_ctx *c, int level, int prec = PREC_SOLUTION_FP64,
                        bool ghost0_first = false, bool defer_last = false) {
   LevelData &L = c->L(level);
   int pending = ghost0_first ? 0 : -1;   // SetGhost program not yet applied
+  int moved = -1;   // post plan the previous launch ran: its rest replaces this pre
+  const double relax_bytes = (prec == PREC_SOLUTION_FP64 ? 24.0 : 12.0) * L.cells;
   for (int pass = 1; pass <= c->cfg.nsmoothingpasses; pass++)
     for (int color = 1; color <= 2; color++) {
+      const bool last = pass == c->cfg.nsmoothingpasses && color == 2;
       int f = pending >= 0 ? fused_index(pending, color) : -1;
-      if (f >= 0 && c->fuse_ghost && L.fused[f].ok) {
-        if (!L.fused[f].pre.empty()) {
-          ProfScope pg(c, BSAM_PROF_GHOST, 32.0 * L.fused[f].pre.size());
-          HIPCK(launch_ghost(prec, c->dev(), L.d_fpre[f], (int)L.fused[f].pre.size(),
-                             c->stream));
+      const bool fused = f >= 0 && c->fuse_ghost && L.fused[f].ok;
+      // this launch ends with the next launch's pre updates where a plan
+      // exists (the next launch is SetGhost(color) -> the other colour)
+      int k = (!last && c->fuse_ghost && c->post_ghost) ? post_key(pending, color) : -1;
+      if (k >= 0 && !L.post[k].ok) k = -1;
+      const int *pst = k >= 0 ? L.d_pstart[k] : nullptr;
+      const GhostOp *pop = k >= 0 ? L.d_pops[k] : nullptr;
+      if (fused) {
+        const std::vector<GhostOp> &pre = moved >= 0 ? L.post[moved].rest : L.fused[f].pre;
+        if (!pre.empty()) {
+          ProfScope pg(c, BSAM_PROF_GHOST,
+                       (prec == PREC_SOLUTION_FP64 ? 32.0 : 16.0) * pre.size());
+          HIPCK(launch_ghost(prec, c->dev(), moved >= 0 ? L.d_prest[moved] : L.d_fpre[f],
+                             (int)pre.size(), c->stream));
         }
+        // the sweep's bytes plus 32 B per ghost update applied in the launch
+        // (16 B read, 16 B written: the stand-alone ghost launch's figure)
+        const double nops = (double)L.fused[f].ops.size() + (k >= 0 ? L.post[k].ops.size() : 0);
         ProfScope ps(c, BSAM_PROF_RELAX_FUSED,
-                     (prec == PREC_SOLUTION_FP64 ? 24.0 : 12.0) * L.cells);
+                     relax_bytes + (prec == PREC_SOLUTION_FP64 ? 32.0 : 16.0) * nops);
         HIPCK(launch_relax_fused(c->nd(), prec, c->dev(), L.d_int, (int)L.t_int.size(), color,
-                                 L.d_fstart[f], L.d_fops[f], c->stream));
+                                 L.d_fstart[f], L.d_fops[f], pst, pop, c->stream));
       } else {
         if (pending >= 0) op_set_ghost(c, level, pending, prec);
-        op_relax(c, level, color, prec);
+        if (k >= 0) {
+          ProfScope ps(c, BSAM_PROF_RELAX_FUSED,
+                       relax_bytes +
+                           (prec == PREC_SOLUTION_FP64 ? 32.0 : 16.0) * L.post[k].ops.size());
+          HIPCK(launch_relax_fused(c->nd(), prec, c->dev(), L.d_int, (int)L.t_int.size(), color,
+                                   nullptr, nullptr, pst, pop, c->stream));
+        } else {
+          op_relax(c, level, color, prec);
+        }
       }
+      moved = k;
       pending = color;
     }
   if (defer_last) return pending;
@@ -2228,6 +2296,9 @@ int bsam_tune(bsam_ctx *c, const char *key, double value) {
   } else if (k == "fuse_ghost") {
     c->fuse_ghost = value != 0.0;
     c->drop_graphs();
+  } else if (k == "post_ghost") {
+    c->post_ghost = value != 0.0;
+    c->drop_graphs();
   } else {
     throw Error("unknown tuning key " + k);
   }
@@ -2246,6 +2317,7 @@ int bsam_tune_get(const bsam_ctx *cc, const char *key, double *value) {
   else if (k == "graph") *value = c->use_graph;
   else if (k == "commits_reused") *value = (double)c->commits_reused;
   else if (k == "fuse_ghost") *value = c->fuse_ghost;
+  else if (k == "post_ghost") *value = c->post_ghost;
   else if (k == "tile_x") *value = c->tx();
   else if (k == "tile_y") *value = c->ty();
   else if (k == "tile_z") *value = c->tz();
@@ -2439,11 +2511,50 @@ int bsam_fused_ghost_pre_size(const bsam_ctx *cc, int level, int ipass, int colo
   return (int)c->L(level).fused[f].pre.size();
 }
 
+int bsam_post_ghost_size(const bsam_ctx *cc, int level, int pending, int color, int part) {
+  bsam_ctx *c = const_cast<bsam_ctx *>(cc);
+  if (!c || !c->committed || part < 0 || part > 1) return -1;
+  if (level < c->cfg.minlevel || level > c->finest) return -1;
+  const int k = post_key(pending, color);
+  if (k < 0 || !c->L(level).post[k].ok) return -1;
+  const PostGhost &g = c->L(level).post[k];
+  return (int)(part == 0 ? g.ops.size() : g.rest.size());
+}
+
 int bsam_ghost_rides_down(const bsam_ctx *cc, int level, int ipass) {
   bsam_ctx *c = const_cast<bsam_ctx *>(cc);
   if (!c || !c->committed || ipass < 0 || ipass > 2) return -1;
   if (level < c->cfg.minlevel || level > c->finest) return -1;
   return c->L(level).down_ok[ipass] ? 1 : 0;
+}
+
+int bsam_fused_ghost_part_apply_host(const bsam_ctx *cc, int level, int ipass, int color,
+                                     int part, double *const *q_patches) {
+  bsam_ctx *c = const_cast<bsam_ctx *>(cc);
+  API_BEGIN(c)
+  if (!c->committed) throw Error("hierarchy not committed");
+  if (level < c->cfg.minlevel || level > c->finest) throw Error("level out of range");
+  int f = fused_index(ipass, color);
+  if (f < 0) throw Error("no fused form of this ipass and colour");
+  const FusedGhost &g = c->L(level).fused[f];
+  if (!g.ok) throw Error("this transition is not fused on this level");
+  if (part != 0 && part != 1) throw Error("part must be 0 (in the launch) or 1 (before it)");
+  apply_ghost_program_host(c->hp, part == 0 ? g.ops : g.pre, c->cfg.nrvars, q_patches);
+  API_END(c)
+}
+
+int bsam_post_ghost_apply_host(const bsam_ctx *cc, int level, int pending, int color, int part,
+                               double *const *q_patches) {
+  bsam_ctx *c = const_cast<bsam_ctx *>(cc);
+  API_BEGIN(c)
+  if (!c->committed) throw Error("hierarchy not committed");
+  if (level < c->cfg.minlevel || level > c->finest) throw Error("level out of range");
+  const int k = post_key(pending, color);
+  if (k < 0 || !c->L(level).post[k].ok) throw Error("this launch has no post plan");
+  if (part != 0 && part != 1) throw Error("part must be 0 (moved) or 1 (still before)");
+  const PostGhost &g = c->L(level).post[k];
+  apply_ghost_program_host(c->hp, part == 0 ? g.ops : g.rest, c->cfg.nrvars, q_patches);
+  API_END(c)
 }
 
 int bsam_fused_ghost_apply_host(const bsam_ctx *cc, int level, int ipass, int color,

@@ -740,15 +740,25 @@ int build_ghost_program(const std::vector<HPatch> &hp, const std::vector<int> &p
   return 0;
 }
 
-FusedGhost plan_fused_ghost(const std::vector<HPatch> &hp, const std::vector<Tile> &tiles,
-                            int nd, int tx, int ty, int tz,
-                            const std::vector<GhostOp> &prog, const std::vector<GhostOp> &next,
-                            int color, std::string &why) {
-  FusedGhost out;
-  // patches by q offset (pools are laid out in patch order)
-  std::vector<long long> qoffs(hp.size());
-  for (size_t n = 0; n < hp.size(); n++) qoffs[n] = hp[n].qoff;
-  auto locate = [&](long long off, int &p, int &i, int &j, int &k) {
+namespace {
+// Cells of a level's relaxation launch by geometry: the patch and local index
+// of a q-pool offset (pools are laid out in patch order), whether a cell is
+// interior, the interior tile holding it (tiles of a patch are k, j, i
+// nested, from the patch's first tile) and whether colour `color` relaxes it.
+struct LaunchGeometry {
+  const std::vector<HPatch> &hp;
+  int nd, tx, ty, tz, color;
+  std::vector<long long> qoffs;
+  std::vector<int> first;
+  LaunchGeometry(const std::vector<HPatch> &h, const std::vector<Tile> &tiles, int nd_, int tx_,
+                 int ty_, int tz_, int color_)
+      : hp(h), nd(nd_), tx(tx_), ty(ty_), tz(tz_), color(color_), qoffs(h.size()),
+        first(h.size(), -1) {
+    for (size_t n = 0; n < hp.size(); n++) qoffs[n] = hp[n].qoff;
+    for (size_t t = 0; t < tiles.size(); t++)
+      if (first[tiles[t].patch] < 0) first[tiles[t].patch] = (int)t;
+  }
+  void locate(long long off, int &p, int &i, int &j, int &k) const {
     auto it = std::upper_bound(qoffs.begin(), qoffs.end(), off);
     p = (int)(it - qoffs.begin()) - 1;
     if (p < 0 || off >= hp[p].qoff + hp[p].qvs) {
@@ -757,23 +767,63 @@ FusedGhost plan_fused_ghost(const std::vector<HPatch> &hp, const std::vector<Til
     }
     const HPatch &h = hp[p];
     qcell_index(off - h.qoff, h.px, h.py, i, j, k);
-  };
-  // first interior tile of every patch; tiles of a patch are k, j, i nested
-  std::vector<int> first(hp.size(), -1);
-  for (size_t t = 0; t < tiles.size(); t++)
-    if (first[tiles[t].patch] < 0) first[tiles[t].patch] = (int)t;
-  auto inside = [&](int p, int i, int j, int k) {
+  }
+  bool inside(int p, int i, int j, int k) const {
     const HPatch &h = hp[p];
     return i >= 1 && i <= h.mx[0] && j >= 1 && j <= h.mx[1] &&
            (nd == 2 || (k >= 1 && k <= h.mx[2]));
-  };
-  auto tile_of = [&](int p, int i, int j, int k) {
+  }
+  int tile_of(int p, int i, int j, int k) const {
     const HPatch &h = hp[p];
     int nx = (h.mx[0] + tx - 1) / tx, ny = (h.mx[1] + ty - 1) / ty;
     int a = (i - 1) / tx, b = (j - 1) / ty, cz = nd == 3 ? (k - 1) / tz : 0;
     return first[p] + (cz * ny + b) * nx + a;
-  };
-  auto relaxed = [&](int i, int j, int k) { return ((i + j + (nd == 3 ? k : 0) + color) & 1) == 1; };
+  }
+  bool relaxed(int i, int j, int k) const {
+    return ((i + j + (nd == 3 ? k : 0) + color) & 1) == 1;
+  }
+  // the tile of this launch that relaxes the cell at `off`, -1 if none
+  int relaxing_tile(long long off) const {
+    int p, i, j, k;
+    locate(off, p, i, j, k);
+    if (p < 0 || first[p] < 0 || !inside(p, i, j, k) || !relaxed(i, j, k)) return -1;
+    return tile_of(p, i, j, k);
+  }
+  // for a ghost cell: the patch, the interior cell it borders (clamped
+  // index) and how many dimensions it lies outside in; p < 0 if the offset
+  // is not a cell of a tiled patch
+  void border(long long off, int &p, int c[3], int &outside) const {
+    int i, j, k;
+    locate(off, p, i, j, k);
+    outside = 0;
+    if (p < 0 || first[p] < 0) {
+      p = -1;
+      return;
+    }
+    const HPatch &h = hp[p];
+    int m[3] = {h.mx[0], h.mx[1], nd == 3 ? h.mx[2] : 1};
+    c[0] = i;
+    c[1] = j;
+    c[2] = nd == 3 ? k : 1;
+    for (int d = 0; d < 3; d++) {
+      if (c[d] < 1 || c[d] > m[d]) outside++;
+      c[d] = std::min(std::max(c[d], 1), m[d]);
+    }
+  }
+};
+}  // namespace
+
+FusedGhost plan_fused_ghost(const std::vector<HPatch> &hp, const std::vector<Tile> &tiles,
+                            int nd, int tx, int ty, int tz,
+                            const std::vector<GhostOp> &prog, const std::vector<GhostOp> &next,
+                            int color, std::string &why) {
+  FusedGhost out;
+  const LaunchGeometry G(hp, tiles, nd, tx, ty, tz, color);
+  auto locate = [&](long long off, int &p, int &i, int &j, int &k) { G.locate(off, p, i, j, k); };
+  const std::vector<int> &first = G.first;
+  auto inside = [&](int p, int i, int j, int k) { return G.inside(p, i, j, k); };
+  auto tile_of = [&](int p, int i, int j, int k) { return G.tile_of(p, i, j, k); };
+  auto relaxed = [&](int i, int j, int k) { return G.relaxed(i, j, k); };
   std::vector<char> next_dst;   // destinations of the next program
   {
     size_t n = 0;
@@ -846,21 +896,29 @@ FusedGhost plan_fused_ghost(const std::vector<HPatch> &hp, const std::vector<Til
   return out;
 }
 
-// Checks a fused plan from the cells' side: marks every cell the launch
-// relaxes with its tile and every ghost cell a relaxed cell's stencil reads
-// with the reading tile, then requires each tile's updates to read relaxed
-// cells of that tile only and to write ghost cells that no other tile reads.
-bool fused_plan_race_free(const std::vector<HPatch> &hp, const std::vector<Tile> &tiles, int nd,
-                          int tx, int ty, int tz, int color, const FusedGhost &plan,
-                          std::string &why) {
-  if (tiles.empty()) return true;
+namespace {
+// The cells of a relaxation launch from the cells' side: every cell colour
+// `color` relaxes marked with its tile, every ghost cell a relaxed cell's
+// 2*nd-point stencil reads marked with the reading tile (-2 if several), over
+// the q-pool range [lo, hi) of the launch's patches.
+struct LaunchCells {
   long long lo = LLONG_MAX, hi = 0;
+  std::vector<int> relaxed_by, read_by;
+  int at(const std::vector<int> &m, long long off) const {
+    return off >= lo && off < hi ? m[(size_t)(off - lo)] : -1;
+  }
+};
+LaunchCells launch_cells(const std::vector<HPatch> &hp, const std::vector<Tile> &tiles, int nd,
+                         int tx, int ty, int tz, int color) {
+  LaunchCells lc;
   for (const Tile &t : tiles) {
     const HPatch &h = hp[t.patch];
-    lo = std::min(lo, h.qoff);
-    hi = std::max(hi, h.qoff + h.qvs);
+    lc.lo = std::min(lc.lo, h.qoff);
+    lc.hi = std::max(lc.hi, h.qoff + h.qvs);
   }
-  std::vector<int> relaxed_by((size_t)(hi - lo), -1), read_by((size_t)(hi - lo), -1);
+  if (tiles.empty()) lc.lo = 0;
+  lc.relaxed_by.assign((size_t)(lc.hi - lc.lo), -1);
+  lc.read_by.assign((size_t)(lc.hi - lc.lo), -1);
   const int nz = nd == 3 ? tz : 1;
   for (size_t t = 0; t < tiles.size(); t++) {
     const Tile &tl = tiles[t];
@@ -872,7 +930,7 @@ bool fused_plan_race_free(const std::vector<HPatch> &hp, const std::vector<Tile>
           const int kk = nd == 3 ? k : 0;
           if (i > h.mx[0] || j > h.mx[1] || (nd == 3 && k > mz)) continue;
           if (((i + j + kk + color) & 1) != 1) continue;
-          relaxed_by[(size_t)(h.qoff + qcell(h.px, h.py, i, j, kk) - lo)] = (int)t;
+          lc.relaxed_by[(size_t)(h.qoff + qcell(h.px, h.py, i, j, kk) - lc.lo)] = (int)t;
           const int di[6] = {1, -1, 0, 0, 0, 0}, dj[6] = {0, 0, 1, -1, 0, 0},
                     dk[6] = {0, 0, 0, 0, 1, -1};
           for (int n = 0; n < 2 * nd; n++) {
@@ -880,22 +938,162 @@ bool fused_plan_race_free(const std::vector<HPatch> &hp, const std::vector<Tile>
             const bool ghost = a < 1 || a > h.mx[0] || b < 1 || b > h.mx[1] ||
                                (nd == 3 && (c < 1 || c > mz));
             if (!ghost) continue;
-            int &r = read_by[(size_t)(h.qoff + qcell(h.px, h.py, a, b, c) - lo)];
+            int &r = lc.read_by[(size_t)(h.qoff + qcell(h.px, h.py, a, b, c) - lc.lo)];
             r = (r == -1 || r == (int)t) ? (int)t : -2;
           }
         }
   }
-  auto at = [&](const std::vector<int> &m, long long off) {
-    return off >= lo && off < hi ? m[(size_t)(off - lo)] : -1;
+  return lc;
+}
+// Sources of an update: f0, f1 (fine) and the coarse c[] by kind.
+int op_sources(const GhostOp &g, long long out[6]) {
+  int n = 0;
+  out[n++] = g.f0;
+  if (g.kind > 0) {
+    out[n++] = g.f1;
+    const int nc = g.kind == 1 ? 1 : (g.kind == 2 ? 3 : 4);
+    for (int q = 0; q < nc; q++) out[n++] = g.c[q];
+  }
+  return n;
+}
+}  // namespace
+
+// Checks a fused plan from the cells' side: each tile's updates may read
+// relaxed cells of that tile only and write ghost cells that no other tile
+// reads.
+bool fused_plan_race_free(const std::vector<HPatch> &hp, const std::vector<Tile> &tiles, int nd,
+                          int tx, int ty, int tz, int color, const FusedGhost &plan,
+                          std::string &why) {
+  if (tiles.empty()) return true;
+  const LaunchCells lc = launch_cells(hp, tiles, nd, tx, ty, tz, color);
+  for (size_t t = 0; t < tiles.size(); t++)
+    for (int e = plan.start[t]; e < plan.start[t + 1]; e++) {
+      const GhostOp &g = plan.ops[(size_t)e];
+      long long src[6];
+      const int ns = op_sources(g, src);
+      bool bad = lc.at(lc.relaxed_by, g.dst) >= 0;
+      for (int q = 0; q < ns; q++) {
+        const int r = lc.at(lc.relaxed_by, src[q]);
+        bad = bad || (r >= 0 && r != (int)t);
+      }
+      const int w = lc.at(lc.read_by, g.dst);
+      if (bad || (w != -1 && w != (int)t)) {
+        why = "fused ghost update shared between tiles";
+        return false;
+      }
+    }
+  return true;
+}
+
+PostGhost plan_post_ghost(const std::vector<HPatch> &hp, const std::vector<Tile> &tiles, int nd,
+                          int tx, int ty, int tz, int color, const FusedGhost *before,
+                          const std::vector<GhostOp> &cand, std::string &why) {
+  PostGhost out;
+  const LaunchGeometry G(hp, tiles, nd, tx, ty, tz, color);
+  // the tiles whose in-launch updates (applied before their relaxation)
+  // write or read each cell: -2 for several
+  std::unordered_map<long long, int> bdst, bsrc;
+  auto mark = [](std::unordered_map<long long, int> &m, long long off, int t) {
+    auto it = m.find(off);
+    if (it == m.end()) m.emplace(off, t);
+    else if (it->second != t) it->second = -2;
+  };
+  if (before && before->ok)
+    for (size_t t = 0; t < tiles.size(); t++)
+      for (int e = before->start[t]; e < before->start[t + 1]; e++) {
+        const GhostOp &g = before->ops[(size_t)e];
+        mark(bdst, g.dst, (int)t);
+        long long src[6];
+        const int ns = op_sources(g, src);
+        for (int q = 0; q < ns; q++) mark(bsrc, src[q], (int)t);
+      }
+  auto other = [](const std::unordered_map<long long, int> &m, long long off, int t) {
+    auto it = m.find(off);
+    return it != m.end() && it->second != t;
+  };
+  std::vector<std::vector<GhostOp>> per(tiles.size());
+  for (const GhostOp &g : cand) {
+    int p, c[3], outside;
+    G.border(g.dst, p, c, outside);
+    if (p < 0) {
+      why = "ghost destination outside the level's patches";
+      return out;
+    }
+    // the tile relaxing the update's fine sources of this colour; it applies
+    // the update after relaxing them
+    int owner = -1;
+    bool several = false;
+    const int nf = g.kind == 0 ? 1 : 2;
+    const long long fs[2] = {g.f0, g.f1};
+    for (int q = 0; q < nf; q++) {
+      const int t = G.relaxing_tile(fs[q]);
+      if (t < 0) continue;
+      if (owner >= 0 && t != owner) several = true;
+      owner = t;
+    }
+    const int near = G.tile_of(p, c[0], c[1], c[2]);
+    if (owner < 0) owner = near;
+    // the relaxation reads a face ghost through the one interior cell it
+    // borders, in that cell's tile
+    const bool read = outside == 1 && G.relaxed(c[0], c[1], c[2]);
+    bool keep = several || (read && near != owner) || other(bdst, g.dst, owner) ||
+                other(bsrc, g.dst, owner);
+    long long src[6];
+    const int ns = op_sources(g, src);
+    for (int q = 0; q < ns && !keep; q++) keep = other(bdst, src[q], owner);
+    if (keep) out.rest.push_back(g);
+    else per[(size_t)owner].push_back(g);
+  }
+  out.start.assign(tiles.size() + 1, 0);
+  for (size_t t = 0; t < tiles.size(); t++) {
+    out.start[t + 1] = out.start[t] + (int)per[t].size();
+    out.ops.insert(out.ops.end(), per[t].begin(), per[t].end());
+  }
+  if (!post_plan_race_free(hp, tiles, nd, tx, ty, tz, color, before, out, why)) return out;
+  out.ok = true;
+  return out;
+}
+
+// Checks a post plan from the cells' side: a tile's updates run after its
+// relaxation, so they may read cells relaxed by that tile only, cells no
+// in-launch update of another tile writes, and write ghost cells that no
+// other tile's stencil or in-launch update reads or writes.
+bool post_plan_race_free(const std::vector<HPatch> &hp, const std::vector<Tile> &tiles, int nd,
+                         int tx, int ty, int tz, int color, const FusedGhost *before,
+                         const PostGhost &plan, std::string &why) {
+  if (tiles.empty()) return true;
+  const LaunchCells lc = launch_cells(hp, tiles, nd, tx, ty, tz, color);
+  std::vector<int> wrote((size_t)(lc.hi - lc.lo), -1), used((size_t)(lc.hi - lc.lo), -1);
+  auto put = [&](std::vector<int> &m, long long off, int t) {
+    if (off < lc.lo || off >= lc.hi) return;
+    int &r = m[(size_t)(off - lc.lo)];
+    r = (r == -1 || r == t) ? t : -2;
+  };
+  if (before && before->ok)
+    for (size_t t = 0; t < tiles.size(); t++)
+      for (int e = before->start[t]; e < before->start[t + 1]; e++) {
+        const GhostOp &g = before->ops[(size_t)e];
+        put(wrote, g.dst, (int)t);
+        long long src[6];
+        const int ns = op_sources(g, src);
+        for (int q = 0; q < ns; q++) put(used, src[q], (int)t);
+      }
+  auto foreign = [&](const std::vector<int> &m, long long off, int t) {
+    const int r = lc.at(m, off);
+    return r != -1 && r != t;
   };
   for (size_t t = 0; t < tiles.size(); t++)
     for (int e = plan.start[t]; e < plan.start[t + 1]; e++) {
       const GhostOp &g = plan.ops[(size_t)e];
-      const int r = at(relaxed_by, g.f0), r1 = g.kind > 0 ? at(relaxed_by, g.f1) : -1;
-      const int w = at(read_by, g.dst);
-      if ((r >= 0 && r != (int)t) || (r1 >= 0 && r1 != (int)t) ||
-          (w != -1 && w != (int)t) || at(relaxed_by, g.dst) >= 0) {
-        why = "fused ghost update shared between tiles";
+      const int tt = (int)t;
+      bool bad = lc.at(lc.relaxed_by, g.dst) >= 0 || foreign(lc.read_by, g.dst, tt) ||
+                 foreign(wrote, g.dst, tt) || foreign(used, g.dst, tt);
+      long long src[6];
+      const int ns = op_sources(g, src);
+      for (int q = 0; q < ns; q++)
+        bad = bad || foreign(lc.relaxed_by, src[q], tt) || foreign(wrote, src[q], tt);
+      if (bad) {
+        why = "post ghost update shared between tiles";
         return false;
       }
     }

@@ -100,6 +100,37 @@ FusedGhost plan_fused_ghost(const std::vector<HPatch> &hp, const std::vector<Til
                             int ndims, int tx, int ty, int tz,
                             const std::vector<GhostOp> &prog, const std::vector<GhostOp> &next,
                             int color, std::string &why);
+// Updates of SetGhost(level, color) that the relaxation launch of `color`
+// just before the one they are read in can apply at its end, each in a tile
+// after that tile's relaxation and a workgroup barrier (`cand`: the `pre`
+// updates of the next launch's fused plan; `before`: the fused plan applied
+// inside this launch, or null). An update goes to the tile that relaxes its
+// fine sources of colour `color` (it must read them relaxed), or, with none,
+// to the tile of the interior cell its destination borders. It stays in
+// `rest` (still launched before the next relaxation) if its sources of that
+// colour lie in several tiles, if this launch's relaxation reads its
+// destination in another tile, or if an in-launch update of another tile
+// writes one of its sources or writes or reads its destination. Between this
+// launch and the next nothing else touches the level, and the next launch
+// relaxes the other colour, so the updates read what SetGhost(level, color)
+// would.
+struct PostGhost {
+  bool ok = false;
+  std::vector<int> start;
+  std::vector<GhostOp> ops;
+  std::vector<GhostOp> rest;
+};
+PostGhost plan_post_ghost(const std::vector<HPatch> &hp, const std::vector<Tile> &tiles, int nd,
+                          int tx, int ty, int tz, int color, const FusedGhost *before,
+                          const std::vector<GhostOp> &cand, std::string &why);
+// Race check of a post plan (run by plan_post_ghost before it accepts one),
+// from the cells' side: each tile's updates read cells relaxed by that tile
+// or by none, and cells no other tile's in-launch update writes; they write
+// ghost cells no other tile's stencil or in-launch update reads or writes.
+bool post_plan_race_free(const std::vector<HPatch> &hp, const std::vector<Tile> &tiles, int nd,
+                         int tx, int ty, int tz, int color, const FusedGhost *before,
+                         const PostGhost &plan, std::string &why);
+
 // Race check of a plan (run by plan_fused_ghost before it accepts one): each
 // tile's updates read only cells that tile relaxes (or cells the launch does
 // not relax) and write only ghost cells that no other tile's stencil reads.

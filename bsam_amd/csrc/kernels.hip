@@ -723,22 +723,35 @@ __device__ __forceinline__ void ghost_tile(const Dev &d, int s0, int s1, const G
 // SetGhost(level, ipass) followed by the relaxation of colour `color`, in one
 // launch: each workgroup first applies the ghost updates its tile's cells read
 // (ops[start[b] .. start[b+1]), chosen on the host so that no update reads a
-// cell another tile relaxes), then relaxes its tile.
+// cell another tile relaxes), then relaxes its tile. With post lists, the
+// workgroup then synchronises and applies pops[pstart[b] .. pstart[b+1]):
+// updates of the next SetGhost(level, color) that read cells this tile has
+// just relaxed (chosen on the host so that no other tile touches their
+// cells in this launch). start or pstart null: no list.
 template <int ND, class T, bool DEF>
 __global__ __launch_bounds__(RELAX_THREADS) void k_relax_fused(Dev d, const Tile *tiles, int nt,
                                                                 int color, const int *start,
-                                                                const GhostOp *ops) {
+                                                                const GhostOp *ops,
+                                                                const int *pstart,
+                                                                const GhostOp *pops) {
   const int b = xcd_tile(nt);
   if (b >= nt) return;
   const Tile t = tiles[b];   // loaded together with the update bounds
-  const int s0 = start[b], s1 = start[b + 1];
+  const int s0 = start ? start[b] : 0, s1 = start ? start[b + 1] : 0;
+  const int p0 = pstart ? pstart[b] : 0, p1 = pstart ? pstart[b + 1] : 0;
   if (stopped_patch(d, t.patch)) return;   // the tile's updates are its patch's group's
   ghost_tile<T>(d, s0, s1, ops);
   Lvl<T> L = level_ref<T>(d, t.patch);
   int i, j, k;
   tile_color_cell<ND>(d, t, color, i, j, k);
-  if (!in_box<ND>(i, j, k, L.P.mx, L.P.my, L.P.mz)) return;
-  relax_cell<ND, T, DEF>(d.prm, L, i, j, k);
+  // every thread reaches the barrier below (p0, p1 are the block's)
+  if (in_box<ND>(i, j, k, L.P.mx, L.P.my, L.P.mz)) relax_cell<ND, T, DEF>(d.prm, L, i, j, k);
+  if (p1 > p0) {
+    __syncthreads();
+    T *q = fields<T>(d).q;
+    for (int e = p0 + (int)threadIdx.x; e < p1; e += blockDim.x)
+      ghost_op<T, std::is_same<T, float>::value>(d.nrvars, q, pops[e]);
+  }
 }
 
 // One SetGhost(level, ipass) program: every ghost-cell update is independent.
@@ -1657,11 +1670,12 @@ hipError_t launch_relax(int nd, int prec, const Dev &d, const Tile *tiles, int n
 }
 
 hipError_t launch_relax_fused(int nd, int prec, const Dev &d, const Tile *tiles, int nt,
-                              int color, const int *start, const GhostOp *ops, hipStream_t s) {
+                              int color, const int *start, const GhostOp *ops,
+                              const int *pstart, const GhostOp *pops, hipStream_t s) {
   if (nt <= 0) return hipSuccess;
   if (d.tile_lx < 1) return hipErrorInvalidValue;
   BSAM_LAUNCH_ND_PREC(k_relax_fused, nd, prec, xcd_grid(nt), RELAX_THREADS, d, tiles, nt, color,
-                      start, ops);
+                      start, ops, pstart, pops);
   return hipGetLastError();
 }
 

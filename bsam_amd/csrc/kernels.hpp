@@ -9,9 +9,12 @@ namespace bsam {
 hipError_t launch_relax(int nd, int prec, const Dev &d, const Tile *tiles, int nt, int ipass,
                         hipStream_t s);
 // A SetGhost program's updates, grouped per tile (start has nt+1 entries),
-// then the relaxation of one colour, in one launch.
+// then the relaxation of one colour, then (pstart non-null) the next
+// program's updates grouped per tile the same way, in one launch. start or
+// pstart may be null (no updates).
 hipError_t launch_relax_fused(int nd, int prec, const Dev &d, const Tile *tiles, int nt,
-                              int color, const int *start, const GhostOp *ops, hipStream_t s);
+                              int color, const int *start, const GhostOp *ops,
+                              const int *pstart, const GhostOp *pops, hipStream_t s);
 hipError_t launch_ghost(int prec, const Dev &d, const GhostOp *ops, int n, hipStream_t s);
 // Restriction / fill-down of one level; gops[0..ng) is a SetGhost program of
 // that level applied in the same launch (its reads must be disjoint from the

@@ -184,6 +184,11 @@ def lib():
         L.bsam_ghost_rides_down.argtypes = [vp, C.c_int, C.c_int]
         L.bsam_fused_ghost_apply_host.argtypes = [vp, C.c_int, C.c_int, C.c_int,
                                                   C.POINTER(C.POINTER(C.c_double))]
+        L.bsam_fused_ghost_part_apply_host.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int,
+                                                       C.POINTER(C.POINTER(C.c_double))]
+        L.bsam_post_ghost_size.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.bsam_post_ghost_apply_host.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int,
+                                                 C.POINTER(C.POINTER(C.c_double))]
         _lib = L
     return _lib
 
@@ -454,6 +459,31 @@ class Engine:
             assert a.flags.f_contiguous and a.dtype == np.float64
             ptrs[n] = _dptr(a)
         self._ck(lib().bsam_fused_ghost_apply_host(self.ctx, level, ipass, color, ptrs))
+
+    def fused_ghost_part_apply_host(self, level: int, ipass: int, color: int, part: int,
+                                    qs: List[np.ndarray]):
+        """Part 0 (inside the launch) or 1 (launched before it) of a fused
+        transition, applied to host q arrays."""
+        ptrs = (C.POINTER(C.c_double) * len(qs))()
+        for n, a in enumerate(qs):
+            assert a.flags.f_contiguous and a.dtype == np.float64
+            ptrs[n] = _dptr(a)
+        self._ck(lib().bsam_fused_ghost_part_apply_host(self.ctx, level, ipass, color, part,
+                                                        ptrs))
+
+    def post_ghost_size(self, level: int, pending: int, color: int, part: int) -> int:
+        """Updates the relaxation launch of `color` (SetGhost(pending) before
+        it, -1 none) applies at its end (part 0) or leaves to a launch before
+        the next relaxation (part 1); -1 when it has no such plan."""
+        return int(lib().bsam_post_ghost_size(self.ctx, level, pending, color, part))
+
+    def post_ghost_apply_host(self, level: int, pending: int, color: int, part: int,
+                              qs: List[np.ndarray]):
+        ptrs = (C.POINTER(C.c_double) * len(qs))()
+        for n, a in enumerate(qs):
+            assert a.flags.f_contiguous and a.dtype == np.float64
+            ptrs[n] = _dptr(a)
+        self._ck(lib().bsam_post_ghost_apply_host(self.ctx, level, pending, color, part, ptrs))
 
     def ghost_apply_host(self, level: int, ipass: int, qs: List[np.ndarray]):
         """Apply SetGhost(level, ipass)'s program to host q arrays in place."""

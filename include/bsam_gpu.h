@@ -350,13 +350,17 @@ int bsam_tune_get(const bsam_ctx *ctx, const char *key, double *value);
  * so 24 B per cell of the level for one colour's half-sweep, which updates
  * half its cells; residual norm 32 B, restriction 24 B, coarse loading 40 B,
  * correction 40 B, fill-down 20 B, source 32 B per cell of the level
- * operated on; FP32 relaxation half of its figure). */
+ * operated on; ghost update 32 B, 16 B read and 16 B written; FP32
+ * relaxation and ghost updates half of their figures). */
 enum {
   BSAM_PROF_RELAX = 0, BSAM_PROF_GHOST, BSAM_PROF_RESTRICT, BSAM_PROF_COARSE_GHOST,
   BSAM_PROF_COARSE_LOAD, BSAM_PROF_CORRECT, BSAM_PROF_FILLDOWN, BSAM_PROF_SOURCE,
   BSAM_PROF_RESNORM, BSAM_PROF_TAIL,
-  BSAM_PROF_RELAX_FUSED,  /* relaxation launches that also apply the preceding
-                             SetGhost program (same bytes as RELAX) */
+  BSAM_PROF_RELAX_FUSED,  /* relaxation launches that also apply ghost
+                             updates (the preceding SetGhost program's before
+                             the sweep, the next one's moved to the end):
+                             RELAX's bytes plus 32 B per update, as GHOST
+                             (16 B in FP32) */
   BSAM_PROF_COPY,         /* whole-pool copies (source restore, qold = q):
                              16 B per element */
   BSAM_PROF_DEFECT,       /* mixed precision: defect set-up (q, f read, qb, e, fe
@@ -406,6 +410,20 @@ int bsam_fused_ghost_pre_size(const bsam_ctx *ctx, int level, int ipass, int col
 int bsam_ghost_rides_down(const bsam_ctx *ctx, int level, int ipass);
 int bsam_fused_ghost_apply_host(const bsam_ctx *ctx, int level, int ipass, int color,
                                 double *const *q_patches);
+/* Applies on the host one part of a fused transition: part 0 the updates
+ * applied inside the relaxation launch, part 1 those launched before it. */
+int bsam_fused_ghost_part_apply_host(const bsam_ctx *ctx, int level, int ipass, int color,
+                                     int part, double *const *q_patches);
+/* The relaxation launch of `color` with SetGhost(`pending`) before it (-1:
+ * none) can end with the next launch's pre updates (those of the transition
+ * SetGhost(color) -> the other colour), each tile applying its share after
+ * its relaxation. Part 0: the updates moved there; part 1: those that still
+ * run in a launch of their own before the next relaxation. Sizes (-1 when
+ * the launch has no such plan) and host application, for the CPU tests. No
+ * counterpart in the reference. */
+int bsam_post_ghost_size(const bsam_ctx *ctx, int level, int pending, int color, int part);
+int bsam_post_ghost_apply_host(const bsam_ctx *ctx, int level, int pending, int color, int part,
+                               double *const *q_patches);
 /* Apply a ghost-fill program on the host to per-patch q arrays (reference
  * layout, one pointer per patch): used to check the integer maps on a CPU. */
 int bsam_ghost_program_apply_host(const bsam_ctx *ctx, int level, int ipass,

@@ -42,6 +42,60 @@ def run(lay, eng, level, ghost0_first, fused, seed, nsmooth=2):
     return [ora.field(n, "q").copy() for n in range(len(ora.patches))], nfused
 
 
+def run_post(lay, eng, level, ghost0_first, seed, nsmooth=2):
+    """The engine's smoothing schedule with post plans: a launch with a post
+    plan ends with the next launch's moved updates, and the next launch runs
+    only the rest before its relaxation (the engine's level_relax)."""
+    ora = orc.Oracle(lay, nsmoothingpasses=nsmooth, **KW)
+    rng = np.random.default_rng(seed)
+    for n, p in enumerate(ora.patches):
+        for name in ("q", "f"):
+            ora.field(n, name)[...] = rng.uniform(-1, 1, lay.field_shape(p, name))
+    qs = [ora.field(n, "q") for n in range(len(ora.patches))]
+    pending = 0 if ghost0_first else -1
+    moved, nposts = None, 0
+    for s in range(nsmooth):
+        for color in (1, 2):
+            last = s == nsmooth - 1 and color == 2
+            fused = pending >= 0 and eng.fused_ghost_size(level, pending, color) >= 0
+            post = not last and eng.post_ghost_size(level, pending, color, 0) >= 0
+            if fused:
+                if moved is not None:
+                    eng.post_ghost_apply_host(level, moved[0], moved[1], 1, qs)
+                else:
+                    eng.fused_ghost_part_apply_host(level, pending, color, 1, qs)
+                eng.fused_ghost_part_apply_host(level, pending, color, 0, qs)
+            elif pending >= 0:
+                ora.set_ghost(level, pending)
+            ora.relax(level, color)
+            if post:
+                eng.post_ghost_apply_host(level, pending, color, 0, qs)
+                nposts += 1
+            moved = (pending, color) if post else None
+            pending = color
+    ora.set_ghost(level, pending)
+    return [ora.field(n, "q").copy() for n in range(len(ora.patches))], nposts
+
+
+@pytest.mark.parametrize("name", ["amr3d", "amr3d_user_bc", "amr3d_mixed_bc", "periodic3d_xz"])
+def test_post_plans_match_reference_sequence(name):
+    """Moved and remaining updates together are exactly the next launch's
+    pre updates: the smoothing with post plans leaves every array
+    bit-identical to the reference sequence."""
+    lay = LAYOUTS[name]()
+    eng = Engine(lay, device=-1, **KW)
+    total = 0
+    for level in range(lay.minlevel, lay.finestlevel + 1):
+        for g0 in (False, True):
+            seed = 11 * (level - lay.minlevel) + g0
+            want, _ = run(lay, eng, level, g0, False, seed)
+            got, nposts = run_post(lay, eng, level, g0, seed)
+            total += nposts
+            for n, (a, b) in enumerate(zip(got, want)):
+                assert np.array_equal(a, b), (level, g0, n)
+    assert total > 0
+
+
 @pytest.mark.parametrize("name", sorted(LAYOUTS))
 def test_fused_smoothing_matches_reference_sequence(name):
     lay = LAYOUTS[name]()
@@ -94,6 +148,17 @@ def test_fused_plans_pass_the_race_check_in_3d():
                 assert n >= 0, (name, level, ipass, color)
                 pre = E.lib().bsam_fused_ghost_pre_size(eng.ctx, level, ipass, color)
                 assert 0 <= pre <= n // 2 + 1, (name, level, ipass, color, pre, n)
+            # the launches before a transition with pre updates take them
+            # over at their end, each plan race-checked the same way
+            for pending, color in ((-1, 1), (0, 1), (2, 1), (1, 2)):
+                nxt = (color, 3 - color)
+                pre = E.lib().bsam_fused_ghost_pre_size(eng.ctx, level, *nxt)
+                if pre <= 0:
+                    continue
+                moved = eng.post_ghost_size(level, pending, color, 0)
+                rest = eng.post_ghost_size(level, pending, color, 1)
+                assert moved + rest == pre and moved >= pre // 2, \
+                    (name, level, pending, color, moved, rest, pre)
 
 
 @pytest.mark.parametrize("name", sorted(LAYOUTS))
