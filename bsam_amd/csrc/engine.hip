@@ -191,6 +191,7 @@ struct bsam_ctx {
   std::vector<int> group_state;            // per group after the last solve: 4 ints
   int last_status = -1;      // of the last solve: BSAM_SOLVE_* (bsam_solve_status)
   int last_iters = 0;        // iterations the last solve took
+  std::vector<std::pair<long long, long long>> f_restore;   // f runs [a, b) an iteration rewrites
   // kernel timing with HIP events on the context's stream (bsam_profile_*)
   unsigned prof_mask = 0;
   std::vector<hipEvent_t> ev_pool;
@@ -794,6 +795,18 @@ static void commit(bsam_ctx *c) {
     c->nf += D.fvs * nv;
   }
   if (c->nq >= (1LL << 31)) throw Error("q pool exceeds 2^31 elements");
+  // f ranges a multigrid iteration rewrites (the coarse loading writes f on
+  // every level below the finest, the tail the whole coarse chain's): the f
+  // of every patch below the finest level, merged into contiguous runs. The
+  // finest level's f is only read.
+  c->f_restore.clear();
+  for (int p = 0; p < n; p++) {
+    const DPatch &D = c->dp[p];
+    if (D.level >= c->finest) continue;
+    const long long a = D.f, b = D.f + D.fvs * nv;
+    if (!c->f_restore.empty() && c->f_restore.back().second == a) c->f_restore.back().second = b;
+    else c->f_restore.push_back({a, b});
+  }
   if (!c->host_only && c->tile_auto && !c->tile_user) choose_tiles(c);
   Periodic per;
   if (c->finest >= 0) {
@@ -1320,10 +1333,11 @@ static void multigrid_iterations(bsam_ctx *c, int *ncycles, double *hist, int ma
   // f on every level from ftmp alone: UpdateSourceFunction adds 0 for this
   // problem (f = ftmp + 0 on levels >= 0) and FillDown(f) then overwrites
   // every covered cell by restriction. Their result is the same each
-  // iteration, so it is computed once here and restored by a copy.
-  {
-    ProfScope ps(c, BSAM_PROF_COPY, 16.0 * c->nf);
-    HIPCK(launch_copy(c->f_saved, c->f, c->nf, c->stream));
+  // iteration, so it is computed once here and restored by a copy of the f
+  // an iteration rewrites (the levels below the finest, f_restore).
+  for (const auto &r : c->f_restore) {
+    ProfScope ps(c, BSAM_PROF_COPY, 16.0 * (r.second - r.first));
+    HIPCK(launch_copy(c->f_saved + r.first, c->f + r.first, r.second - r.first, c->stream));
   }
   reset_state(c);
   // one iteration of the vcycleloop (afasroutines.f90:69-104): the V-cycle,
@@ -1338,9 +1352,9 @@ static void multigrid_iterations(bsam_ctx *c, int *ncycles, double *hist, int ma
   int n = iterate(c, &c->graph, &c->graph_exec, [&]() {
     int pend = op_vcycle(c, c->finest, PREC_SOLUTION_FP64, true);
     op_fill_down(c, BSAM_FIELD_Q, pend, INT_MAX, 0);
-    {
-      ProfScope ps(c, BSAM_PROF_COPY, 16.0 * c->nf);
-      HIPCK(launch_copy(c->f, c->f_saved, c->nf, c->stream));
+    for (const auto &r : c->f_restore) {
+      ProfScope ps(c, BSAM_PROF_COPY, 16.0 * (r.second - r.first));
+      HIPCK(launch_copy(c->f + r.first, c->f_saved + r.first, r.second - r.first, c->stream));
     }
     queue_error(c, c->finest, c->cfg.qerrortol);
   }, &done);
