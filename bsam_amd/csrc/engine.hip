@@ -1333,15 +1333,24 @@ static void multigrid_iterations(bsam_ctx *c, int *ncycles, double *hist, int ma
   // f on every level from ftmp alone: UpdateSourceFunction adds 0 for this
   // problem (f = ftmp + 0 on levels >= 0) and FillDown(f) then overwrites
   // every covered cell by restriction. Their result is the same each
-  // iteration, so it is computed once here and restored by a copy of the f
-  // an iteration rewrites (the levels below the finest, f_restore).
+  // iteration, so it is computed once here and saved (the f an iteration
+  // rewrites: the levels below the finest, f_restore). Inside the loop the
+  // restore is dead: an iteration writes f only at parent cells under a
+  // child (CoarseLoadingFunction) and on the chain below the root (the tail),
+  // and the next iteration's descent rewrites each of those cells before
+  // reading it (the coarse loading into level l runs before level l's
+  // smoothing and before level l's own coarse loading reads f_l), while
+  // ErrorAFAS reads the finest level's f, which nothing writes. The restore
+  // therefore runs once, after the loop, so f ends as the reference's loop
+  // leaves it.
   for (const auto &r : c->f_restore) {
     ProfScope ps(c, BSAM_PROF_COPY, 16.0 * (r.second - r.first));
     HIPCK(launch_copy(c->f_saved + r.first, c->f + r.first, r.second - r.first, c->stream));
   }
   reset_state(c);
   // one iteration of the vcycleloop (afasroutines.f90:69-104): the V-cycle,
-  // FillDown(q), source restore + FillDown(f) (the copy), the residual norm
+  // FillDown(q), (source restore + FillDown(f): after the loop), the
+  // residual norm
   int done = 0;
   //
   // Inside the loop FillDown(q) stops at level 0: every level below the root
@@ -1352,13 +1361,13 @@ static void multigrid_iterations(bsam_ctx *c, int *ncycles, double *hist, int ma
   int n = iterate(c, &c->graph, &c->graph_exec, [&]() {
     int pend = op_vcycle(c, c->finest, PREC_SOLUTION_FP64, true);
     op_fill_down(c, BSAM_FIELD_Q, pend, INT_MAX, 0);
-    for (const auto &r : c->f_restore) {
-      ProfScope ps(c, BSAM_PROF_COPY, 16.0 * (r.second - r.first));
-      HIPCK(launch_copy(c->f + r.first, c->f_saved + r.first, r.second - r.first, c->stream));
-    }
     queue_error(c, c->finest, c->cfg.qerrortol);
   }, &done);
   op_fill_down(c, BSAM_FIELD_Q, -1, 0, INT_MIN);
+  for (const auto &r : c->f_restore) {
+    ProfScope ps(c, BSAM_PROF_COPY, 16.0 * (r.second - r.first));
+    HIPCK(launch_copy(c->f + r.first, c->f_saved + r.first, r.second - r.first, c->stream));
+  }
   report(c, n, done, ncycles, hist, maxhist, converged);
 }
 
