@@ -223,7 +223,7 @@ def main():
     ap.add_argument("--check", action="store_true",
                     help="mixed precision: also run the FP64 path and report the "
                          "difference (max |phi|, |mu| and mass) after the steps")
-    ap.add_argument("--inner-cycles", type=int, default=2,
+    ap.add_argument("--inner-cycles", type=int, default=3,
                     help="mixed precision: FP32 V-cycles per FP64 defect-correction step")
     ap.add_argument("--serving", type=int, default=256,
                     help="requests of the serving configuration to serve after the solve "

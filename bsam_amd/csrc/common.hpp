@@ -61,10 +61,12 @@ inline void from_device(const double *in, double *out, long long n, long long ny
         out[p * n * ny + i + n * j] = in[p * s.pl + scell(s, (int)i, (int)j, 0)];
 }
 
-// Device record of one patch. Field storage keeps the reference layout
-// (column-major, variable slowest, ghost-extended q with lower bound 0 for
-// mbc = 1), so uploads and downloads are plain copies; offsets index the
-// per-field pools.
+// Device record of one patch; offsets index the per-field pools. The q pool
+// (and qold, and the FP32 e and qb, which share its offsets) holds every
+// patch's first variable, then every patch's second at one pool-wide stride
+// (qvs), so a cell's variables sit at p and p + qvs for every patch; the
+// other pools keep a patch's variables one after the other (fvs, cvs, rvs:
+// the patch's size per variable).
 struct DPatch {
   int mx, my, mz;          // interior extents (mz = 1 in 2D)
   int px, py;              // q row pitch (mx+2) and rows per plane (my+2)
@@ -75,7 +77,8 @@ struct DPatch {
   long long q;             // offset of q(0,0,k0,1) in the q and qold pools
   long long f;             // offset of f(1,1,1,1) in the f/rf/ftmp pools
   long long qc;            // offset of qc(0,0,k0,1) in the qc pool
-  long long qvs, fvs, cvs; // variable strides of q, f and qc
+  long long qvs, fvs, cvs; // variable strides of q (the pool's), f and qc
+  long long qn;            // cells per variable of the patch's q
   long long qr, rvs;       // offset and variable stride of qrte(1,1,1,1) (qrte pool)
   long long fl;            // offset of errorflags(1,1,1) (error-flag pool)
   double dx;
@@ -90,15 +93,17 @@ struct DPatch {
 // interpolation with a single coarse value or a 3-point coarse interpolant,
 // 3 the 3D face form (two fine + bilinear coarse, four values). nflip counts
 // trailing "2 - x" boundary maps; group is the destination patch's group.
-// 48 bytes.
+// Offsets are of the first variable in the q pool; the second variable of
+// every cell is at the pool's variable stride, so the record carries none.
+// 32 bytes.
 struct GhostOp {
-  int dst, dvs;            // q-pool offset (variable 1) and variable stride
-  int kind;
-  short nflip, group;
-  int f0, f1, fvs;         // fine (or copy) sources and their variable stride
-  int cvs;                 // parent variable stride
+  int dst;                 // q-pool offset (variable 1)
+  unsigned char kind, nflip;
+  short group;
+  int f0, f1;              // fine (or copy) sources
   int c[4];                // parent sources
 };
+static_assert(sizeof(GhostOp) == 32, "ghost update record: 32 bytes");
 
 // A tile of cells of one patch: origin (i0, j0, k0) in the patch's index
 // space; the tile extent is fixed per kernel.
@@ -107,17 +112,19 @@ struct Tile {
 };
 
 // One level of the single-patch coarse chain, for the tail kernels: its patch
-// and a copy of the patch's record, its three SetGhost programs (ipass 0, 1,
-// 2; offsets relative to the patch's q origin) and where its arrays live:
-// lds[set][a] is a byte offset into the tail kernel's dynamic LDS or -1 for
-// the global pool, for a = q, qc, f, qb and set 0 (FP64 solution) or 1 (FP32
-// correction).
+// and, per field set, a copy of the patch's record, its three SetGhost
+// programs (ipass 0, 1, 2; offsets relative to the patch's q origin) and
+// where its arrays live: lds[set][a] is a byte offset into the tail kernel's
+// dynamic LDS or -1 for the global pool, for a = q, qc, f, qb and set 0 (FP64
+// solution) or 1 (FP32 correction). P[set].qvs is the variable stride of
+// that set's q where it lives: the patch's size per variable in LDS, the
+// pool's stride in the pool.
 struct TailLevel {
   int patch;
   int ngp[3];
   const GhostOp *gp[3];
   int lds[2][4];
-  DPatch P;
+  DPatch P[2];
 };
 
 // One chunk of an array the tail kernel stages between the global pools and
@@ -148,6 +155,7 @@ struct Dev {
   // a solve skip the patches and ghost updates of a group that has stopped
   const int *gstate;
   Problem prm;
+  long long qvs;          // the q pool's variable stride (also e and qb)
   int nrvars;
   int tile_lx, tile_ly;   // level-kernel tiles: 2^tile_lx x 2^tile_ly (x the rest of 256)
 };

@@ -128,13 +128,13 @@ const int kPostPending[4] = {-1, 0, 2, 1}, kPostColor[4] = {1, 1, 1, 2};
 // covers the domain).
 bool local_programs(const HPatch &h, const std::vector<GhostOp> (&gp)[3], int nv,
                     std::vector<GhostOp> (&out)[3]) {
-  long long n = h.qvs;
+  long long n = h.qn;
   for (int ip = 0; ip < 3; ip++) {
     out[ip].clear();
     for (GhostOp g : gp[ip]) {
       long long dst = g.dst - h.qoff, src = g.f0 - h.qoff;
-      if (g.kind != 0 || dst < 0 || dst >= n || src < 0 || src >= n || g.dvs != n ||
-          g.fvs != n || src >= (1 << 28) || g.nflip < 0 || g.nflip > 15)
+      if (g.kind != 0 || dst < 0 || dst >= n || src < 0 || src >= n || src >= (1 << 28) ||
+          g.nflip > 15)
         return false;
       (void)nv;
       g.dst = (int)dst;
@@ -191,6 +191,7 @@ struct bsam_ctx {
   std::vector<int> group_state;            // per group after the last solve: 4 ints
   int last_status = -1;      // of the last solve: BSAM_SOLVE_* (bsam_solve_status)
   int last_iters = 0;        // iterations the last solve took
+  long long qstride = 0;     // the q pool's variable stride
   std::vector<std::pair<long long, long long>> f_restore;   // f runs [a, b) an iteration rewrites
   // kernel timing with HIP events on the context's stream (bsam_profile_*)
   unsigned prof_mask = 0;
@@ -258,6 +259,7 @@ struct bsam_ctx {
     d.gstate = d_state;
     d.prm = Problem{cfg.dt, cfg.eps2, cfg.fc_alpha, cfg.fc_shift, cfg.fe_num, cfg.fe_den};
     d.nrvars = cfg.nrvars;
+    d.qvs = qstride;
     d.tile_lx = tile_lx;
     d.tile_ly = tile_ly;
     return d;
@@ -430,15 +432,15 @@ static bool plan_tail(bsam_ctx *c, long long budget, hipStream_t s) {
     const DPatch &D = c->dp[L.patches[0]];
     TailLevel t;
     t.patch = L.patches[0];
-    t.P = D;
+    t.P[0] = t.P[1] = D;
     for (int ip = 0; ip < 3; ip++) {
       t.ngp[ip] = (int)L.gp[ip].size();
       t.gp[ip] = L.d_tgp[ip];
     }
     for (int set = 0; set < 2; set++) {
       long long sz = set == 0 ? 8 : 4;
-      long long bytes[4] = {D.qvs * nv * sz, D.cvs * nv * sz, D.fvs * nv * sz,
-                            set == 1 ? D.qvs * sz : 0};
+      long long bytes[4] = {D.qn * nv * sz, D.cvs * nv * sz, D.fvs * nv * sz,
+                            set == 1 ? D.qn * sz : 0};
       for (int a = 0; a < 4; a++) t.lds[set][a] = -1;
       bool usable = set == 0 || c->cfg.precision == BSAM_PREC_MIXED;
       // a level is in LDS when q, qc (and the FP32 base point) all fit, its f
@@ -464,14 +466,20 @@ static bool plan_tail(bsam_ctx *c, long long budget, hipStream_t s) {
                                                                           : (char *)c->qb;
         }
         off = a == 0 || a == 3 ? D.q : a == 1 ? D.qc : D.f;
-        // staging chunks of one wave's width
-        const int n = (int)(bytes[a] / sz);
-        for (int e = 0; e < n; e += 64) {
-          TailCopy tc{pool + (off + e) * sz, t.lds[set][a] + (int)(e * sz), std::min(64, n - e)};
-          cin[set].push_back(tc);
-          if (a != 3) cout[set].push_back(tc);
+        // staging chunks of one wave's width; q's variables lie the pool's
+        // stride apart and are packed one after the other in LDS
+        const int nvar = a == 0 ? nv : 1;
+        const int n = (int)(bytes[a] / sz / nvar);
+        for (int v = 0; v < nvar; v++) {
+          for (int e = 0; e < n; e += 64) {
+            TailCopy tc{pool + (off + v * D.qvs + e) * sz,
+                        t.lds[set][a] + (int)((v * n + e) * sz), std::min(64, n - e)};
+            cin[set].push_back(tc);
+            if (a != 3) cout[set].push_back(tc);
+          }
         }
       }
+      if (t.lds[set][0] >= 0) t.P[set].qvs = D.qn;   // q packed in LDS
       nin[set].push_back((int)cin[set].size());
       nout[set].push_back((int)cout[set].size());
     }
@@ -768,8 +776,10 @@ static void commit(bsam_ctx *c) {
       D.mb[d][0] = h.mb[d][0];
       D.mb[d][1] = h.mb[d][1];
     }
-    // device planes of rows of n cells take 2 * ceil(n / 2) cells per row
-    D.q = c->nq; D.qvs = split_of(ex, ey).pl * ez;
+    // device planes of rows of n cells take 2 * ceil(n / 2) cells per row;
+    // q holds every patch's first variable, then every patch's second (the
+    // stride is set below, once the first variables are counted)
+    D.q = c->nq; D.qn = split_of(ex, ey).pl * ez;
     D.qc = c->nc; D.cvs = split_of(cx, cy).pl * cz;
     D.f = c->nf; D.fvs = split_of(h.mx[0], h.mx[1]).pl * (nd == 3 ? h.mx[2] : 1);
     {
@@ -789,11 +799,14 @@ static void commit(bsam_ctx *c) {
       D.eps2_h2 = eps2 / h2;
       D.lap_h2 = (nd == 2 ? 4.0 : 6.0) * eps2 / h2;
     }
-    h.qoff = D.q; h.qvs = D.qvs; h.px = ex; h.py = ey;
-    c->nq += D.qvs * nv;
+    h.qoff = D.q; h.qn = D.qn; h.px = ex; h.py = ey;
+    c->nq += D.qn;
     c->nc += D.cvs * nv;
     c->nf += D.fvs * nv;
   }
+  c->qstride = c->nq;   // cells per variable over all patches
+  for (DPatch &D : c->dp) D.qvs = c->qstride;
+  c->nq *= nv;
   if (c->nq >= (1LL << 31)) throw Error("q pool exceeds 2^31 elements");
   // f ranges a multigrid iteration rewrites (the coarse loading writes f on
   // every level below the finest, the tail the whole coarse chain's): the f
@@ -1600,10 +1613,11 @@ long long bsam_footprint_bytes(const bsam_ctx *c) {
 
 // A patch's field: its pool and offset, its length in the reference layout,
 // and its plane shape (rows of rx cells, ry rows per plane, `planes` planes
-// over all variables; rx = 0 for arrays the device keeps in the reference
-// layout). Device planes take split_of(rx, ry).pl cells.
+// per variable, nvar variables `vstride` apart on the device; rx = 0 for
+// arrays the device keeps in the reference layout). Device planes take
+// split_of(rx, ry).pl cells.
 struct FieldShape {
-  long long rx = 0, ry = 0, planes = 0;
+  long long rx = 0, ry = 0, planes = 0, nvar = 1, vstride = 0;
 };
 static void field_ptr(bsam_ctx *c, int patch, int field, double **base, long long *off,
                       long long *n, FieldShape *shape = nullptr) {
@@ -1616,24 +1630,24 @@ static void field_ptr(bsam_ctx *c, int patch, int field, double **base, long lon
     case BSAM_FIELD_QOLD:
       *base = field == BSAM_FIELD_Q ? c->q : c->qold;
       *off = D.q;
-      fs = FieldShape{D.px, D.py, (nd == 3 ? D.mz + 2 : 1) * (long long)nv};
+      fs = FieldShape{D.px, D.py, nd == 3 ? D.mz + 2 : 1, nv, D.qvs};
       break;
     case BSAM_FIELD_QC:
       *base = c->qc;
       *off = D.qc;
-      fs = FieldShape{D.cmx + 2, D.cmy + 2, (nd == 3 ? D.cmz + 2 : 1) * (long long)nv};
+      fs = FieldShape{D.cmx + 2, D.cmy + 2, nd == 3 ? D.cmz + 2 : 1, nv, D.cvs};
       break;
     case BSAM_FIELD_F:
     case BSAM_FIELD_FTMP:
     case BSAM_FIELD_RF:
       *base = field == BSAM_FIELD_F ? c->f : field == BSAM_FIELD_FTMP ? c->ftmp : c->rf;
       *off = D.f;
-      fs = FieldShape{D.mx, D.my, (long long)D.mz * nv};
+      fs = FieldShape{D.mx, D.my, D.mz, nv, D.fvs};
       break;
     case BSAM_FIELD_QRTE: *base = c->qrte; *off = D.qr; *n = D.rvs * nv; break;
     default: throw Error("unknown field");
   }
-  if (fs.rx > 0) *n = fs.rx * fs.ry * fs.planes;
+  if (fs.rx > 0) *n = fs.rx * fs.ry * fs.planes * fs.nvar;
   if (shape) *shape = fs;
 }
 
@@ -1659,12 +1673,14 @@ int bsam_upload(bsam_ctx *c, int patch, int field, const double *host, long n) {
   field_ptr(c, patch, field, &b, &off, &len, &fs);
   if (n != len) throw Error("upload: size mismatch");
   if (!host && n > 0) throw Error("upload: null host array");
-  if (fs.rx > 0) {   // reference planes into the device's split planes
-    const long long dn = split_of(fs.rx, fs.ry).pl * fs.planes;
-    std::vector<double> tmp((size_t)dn, 0.0);
-    to_device(host, tmp.data(), fs.rx, fs.ry, fs.planes);
-    HIPCK(hipMemcpyAsync(b + off, tmp.data(), sizeof(double) * dn, hipMemcpyHostToDevice,
-                         c->stream));
+  if (fs.rx > 0) {   // reference planes into the device's split planes, per variable
+    const long long dn = split_of(fs.rx, fs.ry).pl * fs.planes, hn = fs.rx * fs.ry * fs.planes;
+    std::vector<double> tmp((size_t)(dn * fs.nvar), 0.0);
+    for (long long v = 0; v < fs.nvar; v++) {
+      to_device(host + v * hn, tmp.data() + v * dn, fs.rx, fs.ry, fs.planes);
+      HIPCK(hipMemcpyAsync(b + off + v * fs.vstride, tmp.data() + v * dn, sizeof(double) * dn,
+                           hipMemcpyHostToDevice, c->stream));
+    }
     HIPCK(hipStreamSynchronize(c->stream));
   } else {
     HIPCK(hipMemcpyAsync(b + off, host, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
@@ -1683,12 +1699,14 @@ int bsam_download(bsam_ctx *c, int patch, int field, double *host, long n) {
   if (n != len) throw Error("download: size mismatch");
   if (!host && n > 0) throw Error("download: null host array");
   if (fs.rx > 0) {   // the device's split planes back into reference planes
-    const long long dn = split_of(fs.rx, fs.ry).pl * fs.planes;
-    std::vector<double> tmp((size_t)dn);
-    HIPCK(hipMemcpyAsync(tmp.data(), b + off, sizeof(double) * dn, hipMemcpyDeviceToHost,
-                         c->stream));
+    const long long dn = split_of(fs.rx, fs.ry).pl * fs.planes, hn = fs.rx * fs.ry * fs.planes;
+    std::vector<double> tmp((size_t)(dn * fs.nvar));
+    for (long long v = 0; v < fs.nvar; v++)
+      HIPCK(hipMemcpyAsync(tmp.data() + v * dn, b + off + v * fs.vstride, sizeof(double) * dn,
+                           hipMemcpyDeviceToHost, c->stream));
     HIPCK(hipStreamSynchronize(c->stream));
-    from_device(tmp.data(), host, fs.rx, fs.ry, fs.planes);
+    for (long long v = 0; v < fs.nvar; v++)
+      from_device(tmp.data() + v * dn, host + v * hn, fs.rx, fs.ry, fs.planes);
   } else {
     HIPCK(hipMemcpyAsync(host, b + off, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
     HIPCK(hipStreamSynchronize(c->stream));
@@ -1712,9 +1730,11 @@ int bsam_truncation_error(bsam_ctx *c, int level) {
   for (int l = level - 1; l <= level; l++)
     for (int p : c->L(l).patches) {
       beg = std::min(beg, c->dp[p].q);
-      end = std::max(end, c->dp[p].q + c->dp[p].qvs * c->cfg.nrvars);
+      end = std::max(end, c->dp[p].q + c->dp[p].qn);
     }
-  HIPCK(launch_copy(c->qold + beg, c->q + beg, end - beg, c->stream));
+  for (int v = 0; v < c->cfg.nrvars; v++)   // each variable's range of the pool
+    HIPCK(launch_copy(c->qold + beg + v * c->qstride, c->q + beg + v * c->qstride, end - beg,
+                      c->stream));
   HIPCK(launch_trunc_error(c->nd(), c->dev(), L.d_coarse, (int)L.t_coarse.size(), c->stream));
   API_END(c)
 }
@@ -1960,14 +1980,15 @@ int bsam_copy_levels(bsam_ctx *dst, const bsam_ctx *src, int top) {
   if (top < dst->cfg.minlevel || top > dst->finest || top > src->finest)
     throw Error("copy_levels: bad level");
   if (!same_levels(dst, src, top)) throw Error("copy_levels: hierarchies differ below the level");
-  // levels minlevel..top occupy the same leading range of both q pools
+  // levels minlevel..top occupy the same leading range of each variable's
+  // part of both q pools
   long long end = 0;
   for (size_t p = 0; p < dst->hp.size(); p++)
-    if (dst->hp[p].level <= top)
-      end = std::max(end, dst->dp[p].q + dst->dp[p].qvs * dst->cfg.nrvars);
+    if (dst->hp[p].level <= top) end = std::max(end, dst->dp[p].q + dst->dp[p].qn);
   HIPCK(hipStreamSynchronize(src->stream));
-  HIPCK(hipMemcpyAsync(dst->q, src->q, sizeof(double) * end, hipMemcpyDeviceToDevice,
-                       dst->stream));
+  for (int v = 0; v < dst->cfg.nrvars; v++)
+    HIPCK(hipMemcpyAsync(dst->q + v * dst->qstride, src->q + v * src->qstride,
+                         sizeof(double) * end, hipMemcpyDeviceToDevice, dst->stream));
   // complete before returning: the source may be re-committed or freed next
   HIPCK(hipStreamSynchronize(dst->stream));
   API_END(dst)
@@ -2082,7 +2103,8 @@ int bsam_set_dt(bsam_ctx *c, double dt) {
     c->dp[p].dt_h2 = dt / h2;
   }
   for (TailLevel &t : c->h_tail)
-    if (t.patch >= 0 && (size_t)t.patch < c->dp.size()) t.P.dt_h2 = c->dp[t.patch].dt_h2;
+    if (t.patch >= 0 && (size_t)t.patch < c->dp.size())
+      t.P[0].dt_h2 = t.P[1].dt_h2 = c->dp[t.patch].dt_h2;
   if (!c->host_only && c->committed) {
     // on the context's own stream (no legacy-stream work that another
     // thread's capture would forbid); the host tables stay alive until the

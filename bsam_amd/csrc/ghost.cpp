@@ -700,16 +700,16 @@ int build_ghost_program(const std::vector<HPatch> &hp, const std::vector<int> &p
       }
       GhostOp g;
       memset(&g, 0, sizeof g);
+      if (e.nflip < 0 || e.nflip > 255) {
+        msg = "ghost program: too many boundary maps on one ghost";
+        return -1;
+      }
       g.dst = (int)qoff(hp, dc);
-      g.dvs = (int)h.qvs;
-      g.kind = e.kind;
-      g.nflip = (short)e.nflip;
+      g.kind = (unsigned char)e.kind;
+      g.nflip = (unsigned char)e.nflip;
       g.f0 = (int)qoff(hp, e.f0);
-      g.fvs = (int)hp[e.f0.p].qvs;
       if (e.kind > 0) {
         g.f1 = (int)qoff(hp, e.f1);
-        const HPatch &pa = hp[h.parent];
-        g.cvs = (int)pa.qvs;
         int nc = e.kind == 1 ? 1 : (e.kind == 2 ? 3 : 4);
         for (int q = 0; q < nc; q++) g.c[q] = (int)qoff(hp, e.c[q]);
       }
@@ -761,7 +761,7 @@ struct LaunchGeometry {
   void locate(long long off, int &p, int &i, int &j, int &k) const {
     auto it = std::upper_bound(qoffs.begin(), qoffs.end(), off);
     p = (int)(it - qoffs.begin()) - 1;
-    if (p < 0 || off >= hp[p].qoff + hp[p].qvs) {
+    if (p < 0 || off >= hp[p].qoff + hp[p].qn) {
       p = -1;
       return;
     }
@@ -914,7 +914,7 @@ LaunchCells launch_cells(const std::vector<HPatch> &hp, const std::vector<Tile> 
   for (const Tile &t : tiles) {
     const HPatch &h = hp[t.patch];
     lc.lo = std::min(lc.lo, h.qoff);
-    lc.hi = std::max(lc.hi, h.qoff + h.qvs);
+    lc.hi = std::max(lc.hi, h.qoff + h.qn);
   }
   if (tiles.empty()) lc.lo = 0;
   lc.relaxed_by.assign((size_t)(lc.hi - lc.lo), -1);
@@ -1277,7 +1277,7 @@ void locate_q_cell(const std::vector<HPatch> &hp, long long off, int &p, int &i,
                    int &k) {
   p = -1;
   for (size_t n = 0; n < hp.size(); n++)
-    if (off >= hp[n].qoff && off < hp[n].qoff + hp[n].qvs) {
+    if (off >= hp[n].qoff && off < hp[n].qoff + hp[n].qn) {
       p = (int)n;
       break;
     }
@@ -1301,9 +1301,9 @@ void apply_ghost_program_host(const std::vector<HPatch> &hp, const std::vector<G
   };
   auto host_vs = [&](int p) {
     const HPatch &h = hp[p];
-    return h.px * h.py * (h.qvs / split_of(h.px, h.py).pl);
+    return h.px * h.py * (h.qn / split_of(h.px, h.py).pl);
   };
-  auto val = [&](long long off, int v, long long) {
+  auto val = [&](long long off, int v) {
     int p;
     long long at = host_index(off, p);
     return q_patches[p][at + (long long)v * host_vs(p)];
@@ -1314,20 +1314,20 @@ void apply_ghost_program_host(const std::vector<HPatch> &hp, const std::vector<G
     for (int v = 0; v < nrvars; v++) {
       double x;
       switch (g.kind) {
-        case 0: x = val(g.f0, v, g.fvs); break;
+        case 0: x = val(g.f0, v); break;
         case 1:
-          x = BSAM_A1 * val(g.f0, v, g.fvs) + BSAM_A2 * val(g.f1, v, g.fvs) +
-              BSAM_A3 * val(g.c[0], v, g.cvs);
+          x = BSAM_A1 * val(g.f0, v) + BSAM_A2 * val(g.f1, v) +
+              BSAM_A3 * val(g.c[0], v);
           break;
         case 2:
-          x = BSAM_A1 * val(g.f0, v, g.fvs) + BSAM_A2 * val(g.f1, v, g.fvs) +
-              BSAM_A3 * (BSAM_B1 * val(g.c[0], v, g.cvs) + BSAM_B2 * val(g.c[1], v, g.cvs) +
-                         BSAM_B3 * val(g.c[2], v, g.cvs));
+          x = BSAM_A1 * val(g.f0, v) + BSAM_A2 * val(g.f1, v) +
+              BSAM_A3 * (BSAM_B1 * val(g.c[0], v) + BSAM_B2 * val(g.c[1], v) +
+                         BSAM_B3 * val(g.c[2], v));
           break;
         default:
-          x = BSAM_A1 * val(g.f0, v, g.fvs) + BSAM_A2 * val(g.f1, v, g.fvs) +
-              BSAM_A3 * (BSAM_C1 * val(g.c[0], v, g.cvs) + BSAM_C2 * val(g.c[1], v, g.cvs) +
-                         BSAM_C3 * val(g.c[2], v, g.cvs) + BSAM_C4 * val(g.c[3], v, g.cvs));
+          x = BSAM_A1 * val(g.f0, v) + BSAM_A2 * val(g.f1, v) +
+              BSAM_A3 * (BSAM_C1 * val(g.c[0], v) + BSAM_C2 * val(g.c[1], v) +
+                         BSAM_C3 * val(g.c[2], v) + BSAM_C4 * val(g.c[3], v));
           break;
       }
       for (int r = 0; r < g.nflip; r++) x = 2.0 - x;

@@ -17,7 +17,9 @@ struct HPatch {
   int mg[3][2] = {{1, 1}, {1, 1}, {1, 1}};
   int mthbc[6] = {1, 1, 1, 1, 1, 1};
   double dx = 0.0, xlower[3] = {0.0, 0.0, 0.0};
-  long long qoff = 0, qvs = 0, px = 0, py = 0;
+  // q-pool offset of the first variable, cells per variable, row pitch and
+  // rows per plane of the ghost-extended q
+  long long qoff = 0, qn = 0, px = 0, py = 0;
   int group = 0;   // independent sub-problem the patch belongs to
 };
 

@@ -245,14 +245,15 @@ __device__ __forceinline__ T restrict_f(const T *x, const DPatch &P, int ci, int
 // kernels below map one tile of cells to a workgroup, and the tail kernels
 // loop the same functions over whole single-patch levels in one workgroup.
 
-// One ghost-cell update of a SetGhost program, offsets relative to q. A
+// One ghost-cell update of a SetGhost program, offsets relative to q and
+// variables vs apart. A
 // boundary "2 - x" map is affine; in the defect form only its linear part,
 // -x, applies.
 template <class T, bool DEF>
-__device__ __forceinline__ void ghost_op(int nrvars, T *q, const GhostOp &g) {
+__device__ __forceinline__ void ghost_op(int nrvars, T *q, long long vs, const GhostOp &g) {
   for (int v = 0; v < nrvars; v++) {
-    const T *f = q + (long long)v * g.fvs;
-    const T *c = q + (long long)v * g.cvs;
+    const T *f = q + (long long)v * vs;
+    const T *c = f;
     T val;
     switch (g.kind) {
       case 0:
@@ -273,7 +274,7 @@ __device__ __forceinline__ void ghost_op(int nrvars, T *q, const GhostOp &g) {
         break;
     }
     for (int r = 0; r < g.nflip; r++) val = DEF ? -val : (T)2 - val;
-    q[g.dst + (long long)v * g.dvs] = val;
+    q[g.dst + (long long)v * vs] = val;
   }
 }
 
@@ -715,7 +716,7 @@ __device__ __forceinline__ void ghost_tile(const Dev &d, int s0, int s1, const G
   if (s1 > s0) {
     T *q = fields<T>(d).q;
     for (int e = s0 + (int)threadIdx.x; e < s1; e += blockDim.x)
-      ghost_op<T, std::is_same<T, float>::value>(d.nrvars, q, ops[e]);
+      ghost_op<T, std::is_same<T, float>::value>(d.nrvars, q, d.qvs, ops[e]);
     __syncthreads();
   }
 }
@@ -750,7 +751,7 @@ __global__ __launch_bounds__(RELAX_THREADS) void k_relax_fused(Dev d, const Tile
     __syncthreads();
     T *q = fields<T>(d).q;
     for (int e = p0 + (int)threadIdx.x; e < p1; e += blockDim.x)
-      ghost_op<T, std::is_same<T, float>::value>(d.nrvars, q, pops[e]);
+      ghost_op<T, std::is_same<T, float>::value>(d.nrvars, q, d.qvs, pops[e]);
   }
 }
 
@@ -761,7 +762,7 @@ __global__ __launch_bounds__(256) void k_ghost(Dev d, const GhostOp *ops, int n)
   if (e >= n) return;
   const GhostOp &g = ops[e];
   if (stopped(d, g.group)) return;
-  ghost_op<T, DEF>(d.nrvars, fields<T>(d).q, g);
+  ghost_op<T, DEF>(d.nrvars, fields<T>(d).q, d.qvs, g);
 }
 
 // A SetGhost program whose reads are disjoint from what the host kernel
@@ -772,7 +773,7 @@ __device__ __forceinline__ void ghost_alongside(const Dev &d, const GhostOp *ops
   T *q = fields<T>(d).q;
   for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x)
     if (!stopped(d, ops[e].group))
-      ghost_op<T, std::is_same<T, float>::value>(d.nrvars, q, ops[e]);
+      ghost_op<T, std::is_same<T, float>::value>(d.nrvars, q, d.qvs, ops[e]);
 }
 
 template <int ND, class T>
@@ -1224,7 +1225,7 @@ __device__ __forceinline__ void for_cells(int ex, int ey, int ez, int base, F f)
 template <class T, bool IN_LDS>
 __device__ __forceinline__ Lvl<T> tail_ref(const Dev &d, const TailLevel &t, int set) {
   Fld<T> F = fields<T>(d);
-  const DPatch &P = t.P;
+  const DPatch &P = t.P[set];
   Lvl<T> L{F.q + P.q, F.qc + P.qc, F.f + P.f, F.qb ? F.qb + P.q : nullptr, P};
   if constexpr (IN_LDS) {
     const int *o = t.lds[set];
@@ -1342,7 +1343,7 @@ __device__ __forceinline__ void tail_ghost(int nrvars, const Lvl<T> &L, const Ta
     }
   }
   for (int e = threadIdx.x + TAIL_GR * blockDim.x; e < g.n; e += blockDim.x)
-    ghost_op<T, DEF>(nrvars, L.q, g.ops[e]);
+    ghost_op<T, DEF>(nrvars, L.q, vs, g.ops[e]);
   __syncthreads();
 }
 
