@@ -77,10 +77,12 @@ struct LevelData {
   // the next launch's pre updates applied at the end of a relaxation launch,
   // per launch kind (post_key): the updates and, for the next launch, what
   // still runs before it
-  PostGhost post[4];
-  int *d_pstart[4] = {nullptr, nullptr, nullptr, nullptr};
-  GhostOp *d_pops[4] = {nullptr, nullptr, nullptr, nullptr};
-  GhostOp *d_prest[4] = {nullptr, nullptr, nullptr, nullptr};
+  // key 4 (kPostFinal): the last launch of a smoothing (SetGhost(1) -> 2)
+  // ending with most of SetGhost(2), the program that follows the smoothing
+  PostGhost post[5];
+  int *d_pstart[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  GhostOp *d_pops[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  GhostOp *d_prest[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   // the parent's SetGhost(0) inside this level's coarse ghost + loading
   // launch, and its SetGhost(2) (the last of post-smoothing) inside this
   // level's correction launch
@@ -120,6 +122,11 @@ int post_key(int pending, int color) {
   return -1;
 }
 const int kPostPending[4] = {-1, 0, 2, 1}, kPostColor[4] = {1, 1, 1, 2};
+constexpr int kPostFinal = 4;
+// A SetGhost returned as pending may be only partly pending: kRestOnly + p
+// means that the last relaxation launch applied SetGhost(p) except its final
+// plan's rest, which the caller still launches.
+constexpr int kRestOnly = 8;
 
 // The SetGhost programs of a single-patch level with offsets relative to the
 // patch's q origin, for the tail kernels; f1 carries the source packed with
@@ -324,7 +331,7 @@ void bsam_ctx::free_device() {
       F(l.d_fops[p]);
       F(l.d_fpre[p]);
     }
-    for (int k = 0; k < 4; k++) {
+    for (int k = 0; k < 5; k++) {
       F(l.d_pstart[k]);
       F(l.d_pops[k]);
       F(l.d_prest[k]);
@@ -888,6 +895,15 @@ static void commit(bsam_ctx *c) {
                                   fb >= 0 && L.fused[fb].ok ? &L.fused[fb] : nullptr,
                                   L.fused[fn].pre, why);
     }
+    L.post[kPostFinal] = PostGhost();
+    if (c->fuse_ghost && !L.gp[2].empty()) {
+      // the last launch of a smoothing relaxes colour 2 after SetGhost(1)
+      const int fb = fused_index(1, 2);
+      std::string why;
+      L.post[kPostFinal] = plan_post_ghost(c->hp, L.t_int, nd, c->tx(), c->ty(), c->tz(), 2,
+                                           L.fused[fb].ok ? &L.fused[fb] : nullptr, L.gp[2],
+                                           why);
+    }
     if (l > minl) {
       // parent cells under this level's patches: what its restriction and
       // fill-down write
@@ -947,7 +963,7 @@ static void commit(bsam_ctx *c) {
         L.d_fops[f] = dev_upload(L.fused[f].ops, s);
         L.d_fpre[f] = dev_upload(L.fused[f].pre, s);
       }
-    for (int k = 0; k < 4; k++)
+    for (int k = 0; k < 5; k++)
       if (L.post[k].ok) {
         L.d_pstart[k] = dev_upload(L.post[k].start, s);
         L.d_pops[k] = dev_upload(L.post[k].ops, s);
@@ -1038,6 +1054,20 @@ static void op_set_ghost(bsam_ctx *c, int level, int ipass, int prec = PREC_SOLU
                (prec == PREC_SOLUTION_FP64 ? 32.0 : 16.0) * L.gp[ipass].size());
   HIPCK(launch_ghost(prec, c->dev(), L.d_gp[ipass], (int)L.gp[ipass].size(), c->stream));
 }
+// What remains of SetGhost(2) on `level` after the last relaxation launch of
+// a smoothing applied its final plan (a pending kRestOnly + 2).
+static void op_set_ghost_rest(bsam_ctx *c, int level, int prec) {
+  LevelData &L = c->L(level);
+  const std::vector<GhostOp> &r = L.post[kPostFinal].rest;
+  if (r.empty()) return;
+  ProfScope ps(c, BSAM_PROF_GHOST, (prec == PREC_SOLUTION_FP64 ? 32.0 : 16.0) * r.size());
+  HIPCK(launch_ghost(prec, c->dev(), L.d_prest[kPostFinal], (int)r.size(), c->stream));
+}
+// Applies a pending SetGhost of `level` now: whole, or its rest.
+static void op_apply_pending(bsam_ctx *c, int level, int pending, int prec) {
+  if (pending >= kRestOnly) op_set_ghost_rest(c, level, prec);
+  else if (pending >= 0) op_set_ghost(c, level, pending, prec);
+}
 static void op_relax(bsam_ctx *c, int level, int color, int prec = PREC_SOLUTION_FP64) {
   LevelData &L = c->L(level);
   ProfScope ps(c, BSAM_PROF_RELAX, (prec == PREC_SOLUTION_FP64 ? 24.0 : 12.0) * L.cells);
@@ -1053,6 +1083,10 @@ static void op_restrict(bsam_ctx *c, int level, int prec = PREC_SOLUTION_FP64, i
   LevelData &L = child_level(c, level);
   const GhostOp *gops = nullptr;
   int ng = 0;
+  if (ghost >= kRestOnly) {
+    op_set_ghost_rest(c, level, prec);
+    ghost = -1;
+  }
   if (ghost >= 0) {
     if (c->fuse_ghost && L.down_ok[ghost]) {
       gops = L.d_gp[ghost];
@@ -1082,6 +1116,10 @@ static void op_coarse_load(bsam_ctx *c, int level, int prec = PREC_SOLUTION_FP64
 static const LevelData::DRide *parent_ride(bsam_ctx *c, LevelData &L, int parent_ghost,
                                            int level, int prec, bool correct) {
   if (parent_ghost < 0) return nullptr;
+  if (parent_ghost >= kRestOnly) {
+    op_set_ghost_rest(c, level - 1, prec);
+    return nullptr;
+  }
   const RideGhost &r = correct ? L.ride_corr : L.ride_cgl;
   if (c->fuse_ghost && r.ok && parent_ghost == (correct ? 2 : 0))
     return correct ? &L.d_ride_corr : &L.d_ride_cgl;
@@ -1123,6 +1161,10 @@ static void op_fill_down(bsam_ctx *c, int field, int ghost = -1, int top = INT_M
     LevelData &L = c->L(level);
     const GhostOp *gops = nullptr;
     int ng = 0;
+    if (carry >= kRestOnly) {
+      op_set_ghost_rest(c, level, PREC_SOLUTION_FP64);
+      carry = -1;
+    }
     if (carry >= 0) {
       if (c->fuse_ghost && L.down_ok[carry]) {
         gops = L.d_gp[carry];
@@ -1138,7 +1180,7 @@ static void op_fill_down(bsam_ctx *c, int field, int ghost = -1, int top = INT_M
     }
     carry = field == BSAM_FIELD_Q ? 0 : -1;
   }
-  if (carry >= 0) op_set_ghost(c, level, carry);
+  op_apply_pending(c, level, carry, PREC_SOLUTION_FP64);
   if (c->ntail() > 1 && bottom < c->tail_top && top >= c->tail_top) {
     ProfScope ps(c, BSAM_PROF_TAIL, 0.0);
     HIPCK(launch_tail_filldown(c->nd(), c->dev(), c->d_tail, c->ntail(),
@@ -1158,9 +1200,12 @@ static void op_source(bsam_ctx *c, int full) {
 // inside the following relaxation launch where the level's fused plan allows
 // (its updates that read cells other tiles relax in a launch just before).
 // With defer_last the final SetGhost is not launched but returned (its ipass)
-// for the caller to apply.
+// for the caller to apply. With final_post (a caller that would launch it on
+// its own rather than ride it in another launch), the last launch applies the
+// final SetGhost(2) where its plan allows and kRestOnly + 2 is returned.
 static int level_relax(bsam_ctx *c, int level, int prec = PREC_SOLUTION_FP64,
-                       bool ghost0_first = false, bool defer_last = false) {
+                       bool ghost0_first = false, bool defer_last = false,
+                       bool final_post = false) {
   LevelData &L = c->L(level);
   int pending = ghost0_first ? 0 : -1;   // SetGhost program not yet applied
   int moved = -1;   // post plan the previous launch ran: its rest replaces this pre
@@ -1173,6 +1218,8 @@ static int level_relax(bsam_ctx *c, int level, int prec = PREC_SOLUTION_FP64,
       // this launch ends with the next launch's pre updates where a plan
       // exists (the next launch is SetGhost(color) -> the other colour)
       int k = (!last && c->fuse_ghost && c->post_ghost) ? post_key(pending, color) : -1;
+      if (last && final_post && c->fuse_ghost && c->post_ghost && pending == 1 && color == 2)
+        k = kPostFinal;
       if (k >= 0 && !L.post[k].ok) k = -1;
       const int *pst = k >= 0 ? L.d_pstart[k] : nullptr;
       const GhostOp *pop = k >= 0 ? L.d_pops[k] : nullptr;
@@ -1206,15 +1253,17 @@ static int level_relax(bsam_ctx *c, int level, int prec = PREC_SOLUTION_FP64,
       moved = k;
       pending = color;
     }
+  if (moved == kPostFinal) pending = kRestOnly + 2;   // SetGhost(2) all but its rest
   if (defer_last) return pending;
-  if (pending >= 0) op_set_ghost(c, level, pending, prec);
+  op_apply_pending(c, level, pending, prec);
   return -1;
 }
 // AFASVcycle, afasroutines.f90:488-521. With defer_final, the last
-// SetGhost of the level's post-smoothing is returned (its ipass, or -1)
+// SetGhost of the level's post-smoothing is returned (its ipass, kRestOnly +
+// its ipass when final_post let the last launch apply most of it, or -1)
 // instead of launched.
 static int op_vcycle(bsam_ctx *c, int level, int prec = PREC_SOLUTION_FP64,
-                     bool defer_final = false) {
+                     bool defer_final = false, bool final_post = false) {
   if (level <= c->tail_top) {
     ProfScope ps(c, BSAM_PROF_TAIL, 0.0);
     int nlev = level - c->cfg.minlevel + 1, set = prec == PREC_SOLUTION_FP64 ? 0 : 1;
@@ -1224,16 +1273,20 @@ static int op_vcycle(bsam_ctx *c, int level, int prec = PREC_SOLUTION_FP64,
                              c->tcout_n[set][nlev - 1], c->stream));
     return -1;
   }
-  if (level == c->cfg.minlevel) return level_relax(c, level, prec, false, defer_final);
-  int pend = level_relax(c, level, prec, false, true);
+  if (level == c->cfg.minlevel)
+    return level_relax(c, level, prec, false, defer_final, defer_final && final_post);
+  // the restriction runs the pending SetGhost(2) inside its launch when its
+  // reads allow; otherwise the last relaxation launch takes most of it
+  LevelData &L = c->L(level);
+  int pend = level_relax(c, level, prec, false, true, !(c->fuse_ghost && L.down_ok[2]));
   op_restrict(c, level, prec, pend);
   // SetGhost(level - 1, 0) rides inside the coarse ghost + loading launch and
   // the parent's last SetGhost of post-smoothing inside the correction launch
   // (or each is launched just before, without a plan)
   op_coarse_ghost_load(c, level, prec, 0);
-  int pp = op_vcycle(c, level - 1, prec, true);
+  int pp = op_vcycle(c, level - 1, prec, true, !(c->fuse_ghost && L.ride_corr.ok));
   op_correct(c, level, prec, pp);
-  return level_relax(c, level, prec, true, defer_final);
+  return level_relax(c, level, prec, true, defer_final, defer_final && final_post);
 }
 static void reset_state(bsam_ctx *c) {
   HIPCK(hipMemsetAsync(c->d_state, 0, sizeof(int) * 4 * c->ngroups, c->stream));
@@ -1372,7 +1425,10 @@ static void multigrid_iterations(bsam_ctx *c, int *ncycles, double *hist, int ma
   // residual is taken on the finest level. The part below the root runs once
   // after the loop, so the fields end as the reference's do.
   int n = iterate(c, &c->graph, &c->graph_exec, [&]() {
-    int pend = op_vcycle(c, c->finest, PREC_SOLUTION_FP64, true);
+    // the fill-down runs the pending SetGhost(2) inside its launch when its
+    // reads allow; otherwise the last relaxation launch takes most of it
+    int pend = op_vcycle(c, c->finest, PREC_SOLUTION_FP64, true,
+                         !(c->fuse_ghost && c->L(c->finest).down_ok[2]));
     op_fill_down(c, BSAM_FIELD_Q, pend, INT_MAX, 0);
     queue_error(c, c->finest, c->cfg.qerrortol);
   }, &done);
@@ -1408,9 +1464,9 @@ static void mixed_iterations(bsam_ctx *c, int *ncycles, double *hist, int maxhis
     // the last inner cycle's final ghost fill of the correction is never read
     // (the correction is applied on interior cells, then restarts at zero)
     for (int k = 0; k < c->inner_cycles; k++) {
-      int pend = op_vcycle(c, c->finest, PREC_DEFECT_FP32, true);
-      if (pend >= 0 && k + 1 < c->inner_cycles)
-        op_set_ghost(c, c->finest, pend, PREC_DEFECT_FP32);
+      const bool more = k + 1 < c->inner_cycles;
+      int pend = op_vcycle(c, c->finest, PREC_DEFECT_FP32, true, more);
+      if (more) op_apply_pending(c, c->finest, pend, PREC_DEFECT_FP32);
     }
     {
       ProfScope ps(c, BSAM_PROF_DEFECT, 40.0 * c->all_int_cells);
@@ -2558,12 +2614,13 @@ int bsam_fused_ghost_pre_size(const bsam_ctx *cc, int level, int ipass, int colo
 
 int bsam_post_ghost_size(const bsam_ctx *cc, int level, int pending, int color, int part) {
   bsam_ctx *c = const_cast<bsam_ctx *>(cc);
-  if (!c || !c->committed || part < 0 || part > 1) return -1;
+  if (!c || !c->committed || part < 0 || part > 3) return -1;
   if (level < c->cfg.minlevel || level > c->finest) return -1;
-  const int k = post_key(pending, color);
+  const int k = part >= 2 ? (pending == 1 && color == 2 ? kPostFinal : -1)
+                          : post_key(pending, color);
   if (k < 0 || !c->L(level).post[k].ok) return -1;
   const PostGhost &g = c->L(level).post[k];
-  return (int)(part == 0 ? g.ops.size() : g.rest.size());
+  return (int)(part % 2 == 0 ? g.ops.size() : g.rest.size());
 }
 
 int bsam_ghost_rides_down(const bsam_ctx *cc, int level, int ipass) {
@@ -2594,11 +2651,12 @@ int bsam_post_ghost_apply_host(const bsam_ctx *cc, int level, int pending, int c
   API_BEGIN(c)
   if (!c->committed) throw Error("hierarchy not committed");
   if (level < c->cfg.minlevel || level > c->finest) throw Error("level out of range");
-  const int k = post_key(pending, color);
+  if (part < 0 || part > 3) throw Error("part must be 0..3");
+  const int k = part >= 2 ? (pending == 1 && color == 2 ? kPostFinal : -1)
+                          : post_key(pending, color);
   if (k < 0 || !c->L(level).post[k].ok) throw Error("this launch has no post plan");
-  if (part != 0 && part != 1) throw Error("part must be 0 (moved) or 1 (still before)");
   const PostGhost &g = c->L(level).post[k];
-  apply_ghost_program_host(c->hp, part == 0 ? g.ops : g.rest, c->cfg.nrvars, q_patches);
+  apply_ghost_program_host(c->hp, part % 2 == 0 ? g.ops : g.rest, c->cfg.nrvars, q_patches);
   API_END(c)
 }
 

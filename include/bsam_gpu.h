@@ -418,9 +418,11 @@ int bsam_fused_ghost_part_apply_host(const bsam_ctx *ctx, int level, int ipass, 
  * none) can end with the next launch's pre updates (those of the transition
  * SetGhost(color) -> the other colour), each tile applying its share after
  * its relaxation. Part 0: the updates moved there; part 1: those that still
- * run in a launch of their own before the next relaxation. Sizes (-1 when
- * the launch has no such plan) and host application, for the CPU tests. No
- * counterpart in the reference. */
+ * run in a launch of their own before the next relaxation. Parts 2 and 3
+ * (pending 1, colour 2 only): the same split of SetGhost(2), the program after
+ * the last launch of a smoothing, when that launch takes it over. Sizes (-1
+ * when the launch has no such plan) and host application, for the CPU tests.
+ * No counterpart in the reference. */
 int bsam_post_ghost_size(const bsam_ctx *ctx, int level, int pending, int color, int part);
 int bsam_post_ghost_apply_host(const bsam_ctx *ctx, int level, int pending, int color, int part,
                                double *const *q_patches);

@@ -42,7 +42,7 @@ def run(lay, eng, level, ghost0_first, fused, seed, nsmooth=2):
     return [ora.field(n, "q").copy() for n in range(len(ora.patches))], nfused
 
 
-def run_post(lay, eng, level, ghost0_first, seed, nsmooth=2):
+def run_post(lay, eng, level, ghost0_first, seed, nsmooth=2, final=False):
     """The engine's smoothing schedule with post plans: a launch with a post
     plan ends with the next launch's moved updates, and the next launch runs
     only the rest before its relaxation (the engine's level_relax)."""
@@ -73,7 +73,14 @@ def run_post(lay, eng, level, ghost0_first, seed, nsmooth=2):
                 nposts += 1
             moved = (pending, color) if post else None
             pending = color
-    ora.set_ghost(level, pending)
+    # the program after the smoothing: its final plan's moved updates at the
+    # end of the last launch, then the rest (the engine's op_apply_pending)
+    if final and eng.post_ghost_size(level, 1, 2, 2) >= 0:
+        eng.post_ghost_apply_host(level, 1, 2, 2, qs)
+        eng.post_ghost_apply_host(level, 1, 2, 3, qs)
+        nposts += 1
+    else:
+        ora.set_ghost(level, pending)
     return [ora.field(n, "q").copy() for n in range(len(ora.patches))], nposts
 
 
@@ -89,10 +96,11 @@ def test_post_plans_match_reference_sequence(name):
         for g0 in (False, True):
             seed = 11 * (level - lay.minlevel) + g0
             want, _ = run(lay, eng, level, g0, False, seed)
-            got, nposts = run_post(lay, eng, level, g0, seed)
-            total += nposts
-            for n, (a, b) in enumerate(zip(got, want)):
-                assert np.array_equal(a, b), (level, g0, n)
+            for final in (False, True):
+                got, nposts = run_post(lay, eng, level, g0, seed, final=final)
+                total += nposts
+                for n, (a, b) in enumerate(zip(got, want)):
+                    assert np.array_equal(a, b), (level, g0, final, n)
     assert total > 0
 
 
@@ -159,6 +167,13 @@ def test_fused_plans_pass_the_race_check_in_3d():
                 rest = eng.post_ghost_size(level, pending, color, 1)
                 assert moved + rest == pre and moved >= pre // 2, \
                     (name, level, pending, color, moved, rest, pre)
+            # the program after a smoothing, taken over by its last launch
+            full = len(eng.ghost_program_dst(level, 2))
+            if full:
+                moved = eng.post_ghost_size(level, 1, 2, 2)
+                rest = eng.post_ghost_size(level, 1, 2, 3)
+                assert moved + rest == full and moved >= full // 2, \
+                    (name, level, moved, rest, full)
 
 
 @pytest.mark.parametrize("name", sorted(LAYOUTS))
