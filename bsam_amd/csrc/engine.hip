@@ -234,6 +234,7 @@ struct bsam_ctx {
   int use_graph = 1;
   int fuse_ghost = 1;               // apply SetGhost inside the next relaxation
   int post_ghost = 1;               // and the next launch's pre updates at a launch's end
+  int relax_lds = 0;                // relaxation stencils read through an LDS copy of the tile
   int tail_top = 0;                 // highest level run by the tail kernels
   TailLevel *d_tail = nullptr;      // levels minlevel..tail_top
   hipGraph_t graph = nullptr, graph_mixed = nullptr;
@@ -1071,7 +1072,12 @@ static void op_apply_pending(bsam_ctx *c, int level, int pending, int prec) {
 static void op_relax(bsam_ctx *c, int level, int color, int prec = PREC_SOLUTION_FP64) {
   LevelData &L = c->L(level);
   ProfScope ps(c, BSAM_PROF_RELAX, (prec == PREC_SOLUTION_FP64 ? 24.0 : 12.0) * L.cells);
-  HIPCK(launch_relax(c->nd(), prec, c->dev(), L.d_int, (int)L.t_int.size(), color, c->stream));
+  if (c->relax_lds)   // the staged form of the same sweep, with no ghost updates
+    HIPCK(launch_relax_fused(c->nd(), prec, c->dev(), L.d_int, (int)L.t_int.size(), color,
+                             nullptr, nullptr, nullptr, nullptr, true, c->stream));
+  else
+    HIPCK(launch_relax(c->nd(), prec, c->dev(), L.d_int, (int)L.t_int.size(), color,
+                       c->stream));
 }
 static LevelData &child_level(bsam_ctx *c, int level) {
   if (level <= c->cfg.minlevel) throw Error("level has no parent level");
@@ -1237,7 +1243,8 @@ static int level_relax(bsam_ctx *c, int level, int prec = PREC_SOLUTION_FP64,
         ProfScope ps(c, BSAM_PROF_RELAX_FUSED,
                      relax_bytes + (prec == PREC_SOLUTION_FP64 ? 32.0 : 16.0) * nops);
         HIPCK(launch_relax_fused(c->nd(), prec, c->dev(), L.d_int, (int)L.t_int.size(), color,
-                                 L.d_fstart[f], L.d_fops[f], pst, pop, c->stream));
+                                 L.d_fstart[f], L.d_fops[f], pst, pop, c->relax_lds != 0,
+                                 c->stream));
       } else {
         if (pending >= 0) op_set_ghost(c, level, pending, prec);
         if (k >= 0) {
@@ -1245,7 +1252,7 @@ static int level_relax(bsam_ctx *c, int level, int prec = PREC_SOLUTION_FP64,
                        relax_bytes +
                            (prec == PREC_SOLUTION_FP64 ? 32.0 : 16.0) * L.post[k].ops.size());
           HIPCK(launch_relax_fused(c->nd(), prec, c->dev(), L.d_int, (int)L.t_int.size(), color,
-                                   nullptr, nullptr, pst, pop, c->stream));
+                                   nullptr, nullptr, pst, pop, c->relax_lds != 0, c->stream));
         } else {
           op_relax(c, level, color, prec);
         }
@@ -2400,6 +2407,9 @@ int bsam_tune(bsam_ctx *c, const char *key, double value) {
   } else if (k == "post_ghost") {
     c->post_ghost = value != 0.0;
     c->drop_graphs();
+  } else if (k == "relax_lds") {
+    c->relax_lds = value != 0.0;
+    c->drop_graphs();
   } else {
     throw Error("unknown tuning key " + k);
   }
@@ -2419,6 +2429,7 @@ int bsam_tune_get(const bsam_ctx *cc, const char *key, double *value) {
   else if (k == "commits_reused") *value = (double)c->commits_reused;
   else if (k == "fuse_ghost") *value = c->fuse_ghost;
   else if (k == "post_ghost") *value = c->post_ghost;
+  else if (k == "relax_lds") *value = c->relax_lds;
   else if (k == "tile_x") *value = c->tx();
   else if (k == "tile_y") *value = c->ty();
   else if (k == "tile_z") *value = c->tz();

@@ -161,19 +161,20 @@ __device__ __forceinline__ void op_at(const Problem &prm, const View<T> &a, T ba
 // 2x2 (phi, mu) system, Cramer's rule (RelaxGrid2D/3D, problem.f90:164-215,
 // :453-518; Mob = 1). Defect form: the Newton step linearises
 // Dfc(base + e) about the current e.
+// The update of cell (i,j,k) reading q through view `a` (the patch's arrays,
+// or a staged copy of its neighbourhood) and writing the new values to *o1,
+// *o2; `base` is the FP32 defect form's base point at the cell.
 template <int ND, class T, bool DEF>
-__device__ __forceinline__ void relax_cell_f(const Problem &prm, const Lvl<T> &L, int i, int j,
-                                             int k, T fv1, T fv2) {
-  const DPatch &P = L.P;
-  View<T> a = qview(L);
+__device__ __forceinline__ void relax_core(const Problem &prm, const DPatch &P, const View<T> &a,
+                                           int i, int j, int k, T *o1, T *o2, T base, T fv1,
+                                           T fv2) {
   const long long c = a.at(i, j, k);
   const long long cxp = a.at(i + 1, j, k), cxm = a.at(i - 1, j, k);   // the other half-row
-  T *q1 = a.p, *q2 = a.p + a.vs;
+  const T *q1 = a.p, *q2 = a.p + a.vs;
   T tmp1 = (T)P.dt_h2, tmp2 = (T)P.eps2_h2, tmp3 = (T)P.lap_h2;
   T phi = q1[c];
   T dc, d2;
   if (DEF) {
-    T base = L.qb[c];
     dc = dfc_delta<T>(prm, base, phi);
     d2 = d2fc_at<T>(prm, base, phi);
   } else {
@@ -198,8 +199,16 @@ __device__ __forceinline__ void relax_cell_f(const Problem &prm, const Lvl<T> &L
                  q1[c - sz]);
   }
   T det = (T)1 - a12 * a21;
-  q1[c] = (b1 - a12 * b2) / det;
-  q2[c] = (b2 - a21 * b1) / det;
+  *o1 = (b1 - a12 * b2) / det;
+  *o2 = (b2 - a21 * b1) / det;
+}
+template <int ND, class T, bool DEF>
+__device__ __forceinline__ void relax_cell_f(const Problem &prm, const Lvl<T> &L, int i, int j,
+                                             int k, T fv1, T fv2) {
+  View<T> a = qview(L);
+  const long long c = a.at(i, j, k);
+  relax_core<ND, T, DEF>(prm, L.P, a, i, j, k, a.p + c, a.p + a.vs + c, DEF ? L.qb[c] : (T)0,
+                         fv1, fv2);
 }
 template <int ND, class T, bool DEF>
 __device__ __forceinline__ void relax_cell(const Problem &prm, const Lvl<T> &L, int i, int j,
@@ -729,7 +738,15 @@ __device__ __forceinline__ void ghost_tile(const Dev &d, int s0, int s1, const G
 // updates of the next SetGhost(level, color) that read cells this tile has
 // just relaxed (chosen on the host so that no other tile touches their
 // cells in this launch). start or pstart null: no list.
-template <int ND, class T, bool DEF>
+//
+// With STAGE, the tile's cells and their one-cell halo (both variables) are
+// first copied into LDS in the same parity-split row layout (a box of
+// (tx+2) x (ty+2) (x (tz+2)) cells whose first cell i0 - 1 is even, so a
+// cell's parity is the same in the box), with coalesced loads of whole
+// half-rows; the relaxation then reads its stencil from LDS and writes the
+// new values to the global pool. The arithmetic is relax_core's either way.
+extern __shared__ __align__(16) unsigned char relax_lds[];
+template <int ND, class T, bool DEF, bool STAGE>
 __global__ __launch_bounds__(RELAX_THREADS) void k_relax_fused(Dev d, const Tile *tiles, int nt,
                                                                 int color, const int *start,
                                                                 const GhostOp *ops,
@@ -745,8 +762,37 @@ __global__ __launch_bounds__(RELAX_THREADS) void k_relax_fused(Dev d, const Tile
   Lvl<T> L = level_ref<T>(d, t.patch);
   int i, j, k;
   tile_color_cell<ND>(d, t, color, i, j, k);
-  // every thread reaches the barrier below (p0, p1 are the block's)
-  if (in_box<ND>(i, j, k, L.P.mx, L.P.my, L.P.mz)) relax_cell<ND, T, DEF>(d.prm, L, i, j, k);
+  // every thread reaches the barriers below (p0, p1 are the block's)
+  const bool mine = in_box<ND>(i, j, k, L.P.mx, L.P.my, L.P.mz);
+  if constexpr (STAGE) {
+    const int tx = 1 << d.tile_lx, ty = 1 << d.tile_ly;
+    const int tz = ND == 3 ? 256 >> (d.tile_lx + d.tile_ly) : 1;
+    const int bx = tx + 2, by = ty + 2, bz = ND == 3 ? tz + 2 : 1;
+    const int hb = bx >> 1, ncell = bx * by * bz;
+    const View<T> g = qview(L);
+    T *box = (T *)relax_lds;
+    // box cell (li, lj, lk) is patch cell (i0 - 1 + li, j0 - 1 + lj, k0 - 1 + lk);
+    // cells past the patch's ghost ring are never read and stay unset
+    const int o0 = t.i0 - 1, o1 = t.j0 - 1, o2 = ND == 3 ? t.k0 - 1 : 0;
+    for (int e = threadIdx.x; e < 2 * ncell; e += blockDim.x) {
+      const int v = e >= ncell, r = e - v * ncell;
+      const int lk = r / (bx * by), m = r - lk * (bx * by), lj = m / bx, x = m - lj * bx;
+      const int li = x < hb ? 2 * x : 2 * (x - hb) + 1;
+      const int gi = o0 + li, gj = o1 + lj, gk = o2 + lk;
+      if (gi <= L.P.mx + 1 && gj <= L.P.my + 1 && (ND == 2 || gk <= L.P.mz + 1))
+        box[e] = g.p[g.at(gi, gj, gk) + v * g.vs];
+    }
+    __syncthreads();
+    if (mine) {
+      const View<T> a{box, hb, bx, (long long)bx * by, (long long)ncell};
+      const long long c = g.at(i, j, k);
+      const long long fo = fat(L.P, i, j, ND == 3 ? k : 1);
+      relax_core<ND, T, DEF>(d.prm, L.P, a, i - o0, j - o1, ND == 3 ? k - o2 : 0, g.p + c,
+                             g.p + g.vs + c, DEF ? L.qb[c] : (T)0, L.f[fo], L.f[fo + L.P.fvs]);
+    }
+  } else {
+    if (mine) relax_cell<ND, T, DEF>(d.prm, L, i, j, k);
+  }
   if (p1 > p0) {
     __syncthreads();
     T *q = fields<T>(d).q;
@@ -1672,11 +1718,48 @@ hipError_t launch_relax(int nd, int prec, const Dev &d, const Tile *tiles, int n
 
 hipError_t launch_relax_fused(int nd, int prec, const Dev &d, const Tile *tiles, int nt,
                               int color, const int *start, const GhostOp *ops,
-                              const int *pstart, const GhostOp *pops, hipStream_t s) {
+                              const int *pstart, const GhostOp *pops, bool stage,
+                              hipStream_t s) {
   if (nt <= 0) return hipSuccess;
   if (d.tile_lx < 1) return hipErrorInvalidValue;
-  BSAM_LAUNCH_ND_PREC(k_relax_fused, nd, prec, xcd_grid(nt), RELAX_THREADS, d, tiles, nt, color,
-                      start, ops, pstart, pops);
+  const dim3 grid(xcd_grid(nt));
+  const int tx = 1 << d.tile_lx, ty = 1 << d.tile_ly;
+  const int tz = nd == 3 ? 256 >> (d.tile_lx + d.tile_ly) : 1;
+  const size_t cells = (size_t)(tx + 2) * (ty + 2) * (nd == 3 ? tz + 2 : 1);
+  const size_t lds = 2 * cells * (prec == PREC_SOLUTION_FP64 ? 8 : 4);
+  if (prec == PREC_SOLUTION_FP64) {
+    if (stage) {
+      if (nd == 2)
+        k_relax_fused<2, double, false, true><<<grid, RELAX_THREADS, lds, s>>>(
+            d, tiles, nt, color, start, ops, pstart, pops);
+      else
+        k_relax_fused<3, double, false, true><<<grid, RELAX_THREADS, lds, s>>>(
+            d, tiles, nt, color, start, ops, pstart, pops);
+    } else {
+      if (nd == 2)
+        k_relax_fused<2, double, false, false><<<grid, RELAX_THREADS, 0, s>>>(
+            d, tiles, nt, color, start, ops, pstart, pops);
+      else
+        k_relax_fused<3, double, false, false><<<grid, RELAX_THREADS, 0, s>>>(
+            d, tiles, nt, color, start, ops, pstart, pops);
+    }
+  } else {
+    if (stage) {
+      if (nd == 2)
+        k_relax_fused<2, float, true, true><<<grid, RELAX_THREADS, lds, s>>>(
+            d, tiles, nt, color, start, ops, pstart, pops);
+      else
+        k_relax_fused<3, float, true, true><<<grid, RELAX_THREADS, lds, s>>>(
+            d, tiles, nt, color, start, ops, pstart, pops);
+    } else {
+      if (nd == 2)
+        k_relax_fused<2, float, true, false><<<grid, RELAX_THREADS, 0, s>>>(
+            d, tiles, nt, color, start, ops, pstart, pops);
+      else
+        k_relax_fused<3, float, true, false><<<grid, RELAX_THREADS, 0, s>>>(
+            d, tiles, nt, color, start, ops, pstart, pops);
+    }
+  }
   return hipGetLastError();
 }
 

@@ -11,10 +11,12 @@ hipError_t launch_relax(int nd, int prec, const Dev &d, const Tile *tiles, int n
 // A SetGhost program's updates, grouped per tile (start has nt+1 entries),
 // then the relaxation of one colour, then (pstart non-null) the next
 // program's updates grouped per tile the same way, in one launch. start or
-// pstart may be null (no updates).
+// pstart may be null (no updates). With stage, the tile and its halo are
+// read through LDS (2 x (tx+2)(ty+2)(tz+2) values of dynamic LDS).
 hipError_t launch_relax_fused(int nd, int prec, const Dev &d, const Tile *tiles, int nt,
                               int color, const int *start, const GhostOp *ops,
-                              const int *pstart, const GhostOp *pops, hipStream_t s);
+                              const int *pstart, const GhostOp *pops, bool stage,
+                              hipStream_t s);
 hipError_t launch_ghost(int prec, const Dev &d, const GhostOp *ops, int n, hipStream_t s);
 // Restriction / fill-down of one level; gops[0..ng) is a SetGhost program of
 // that level applied in the same launch (its reads must be disjoint from the

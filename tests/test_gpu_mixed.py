@@ -44,14 +44,16 @@ def root_mass(lay, q_root):
     return float(inner.sum()) * p.dx ** lay.ndims
 
 
-@pytest.mark.parametrize("inner", [1, 2])
+# inner FP32 cycles per defect-correction step, and the LDS-staged relaxation
+@pytest.mark.parametrize("inner,staged", [(1, 0), (2, 0), (3, 0), (2, 1)])
 @pytest.mark.parametrize("name", sorted(LAYOUTS))
-def test_mixed_matches_fp64(name, inner):
+def test_mixed_matches_fp64(name, inner, staged):
     lay = LAYOUTS[name]()
     kw = dict(dt=1e-3, eps=0.05 if lay.ndims == 3 else 0.03, well="plusminus",
               maxvcycles=40, qerrortol=QERRORTOL)
     q0 = noise_q(lay, seed=21, mean=0.1, amp=0.05)
-    eng = Engine(lay, device=0, precision=PREC_MIXED, tune={"inner_cycles": inner}, **kw)
+    eng = Engine(lay, device=0, precision=PREC_MIXED,
+                 tune={"inner_cycles": inner, "relax_lds": staged}, **kw)
     ora = orc.Oracle(lay, **kw)
     for n, p in enumerate(ora.patches):
         eng.upload(n, "q", q0[p.id])

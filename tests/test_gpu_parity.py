@@ -37,7 +37,9 @@ LAYOUTS = {
 # next relaxation launch, hipGraph replay); every single-patch level in the
 # tail, in LDS as far as it fits or all in global memory; SetGhost and
 # relaxation as separate launches; other tile shapes (in 3D the third extent
-# fills the 256 threads)
+# fills the 256 threads); relaxation stencils read through an LDS copy of the
+# tile and its halo (fused, unfused, and with another tile shape); no ghost
+# updates moved to the end of the previous launch
 SCHEDULES = {
     "launch": {"tail_cells": 0, "graph": 0},
     "default": {},
@@ -46,6 +48,10 @@ SCHEDULES = {
     "unfused": {"fuse_ghost": 0, "graph": 0},
     "tiles_64x4": {"tile_x": 64, "tile_y": 4},
     "tiles_16x16": {"tile_x": 16, "tile_y": 16, "fuse_ghost": 1},
+    "staged": {"relax_lds": 1},
+    "staged_unfused": {"relax_lds": 1, "fuse_ghost": 0, "graph": 0},
+    "staged_64x4": {"relax_lds": 1, "tile_x": 64, "tile_y": 4},
+    "no_post": {"post_ghost": 0},
 }
 
 
