@@ -29,6 +29,7 @@ hierarchies of a deeper regrid.
 
 from __future__ import annotations
 
+import json
 import os
 import re
 import time
@@ -118,8 +119,12 @@ def uniform_mesh(u: Engine, eng: Engine, top: int) -> np.ndarray:
 class Driver:
     """TakeTimeSteps on the GPU engine, regridding every step."""
 
-    def __init__(self, workdir: str, device: int = 0):
+    def __init__(self, workdir: str, device: int = 0, metrics: Optional[str] = None):
         self.workdir = workdir
+        # one JSON line per solve (MultigridIterations call), appended to
+        # `metrics` when given: step, time, equation, V-cycles, last residual,
+        # status, wall milliseconds, patches and cells of the hierarchy
+        self.metrics = metrics
         rd = read_namelist(os.path.join(workdir, "rundata.dat"))
         gd = read_namelist(os.path.join(workdir, "griddata.dat"))
         pd = read_namelist(os.path.join(workdir, "problemdata.dat"))
@@ -385,6 +390,15 @@ class Driver:
                 eng.upload(pos[ids[i]], "qold", q)   # CopyQToQold on the root
         return t
 
+    def _log_solve(self, eng, step, t, eqn, hist, conv, ms):
+        lay = eng.layout
+        rec = {"step": step, "time": t, "equation": eqn, "vcycles": len(hist),
+               "residual": float(hist[-1]) if len(hist) else None, "converged": bool(conv),
+               "status": eng.solve_status(), "ms": ms, "patches": len(lay.ordered()),
+               "cells": int(sum(lay.cells_on_level(l) for l in range(0, lay.finestlevel + 1)))}
+        with open(self.metrics, "a") as fh:
+            fh.write(json.dumps(rec) + "\n")
+
     def run(self):
         os.makedirs(os.path.join(self.workdir, "out"), exist_ok=True)
         cur = 0
@@ -444,12 +458,18 @@ class Driver:
                     frame0 = False
                 first = initial = False
                 t0 = time.perf_counter()
-                for _ in range(self.eqfn):
-                    eng.multigrid_iterations()
+                step = self.timing["steps"] + 1
+                step_time = starttime + float((n - firstframe) * itperprint + it) * self.dt
+                for eqn in range(self.eqfn):
+                    ts = time.perf_counter()
+                    hist, conv = eng.multigrid_iterations()
+                    if self.metrics:
+                        self._log_solve(eng, step, step_time, eqn + 1, hist, conv,
+                                        1e3 * (time.perf_counter() - ts))
                 eng.sync()
                 self.timing["solve_s"] += time.perf_counter() - t0
                 self.timing["steps"] += 1
-                currenttime = real_time = starttime + float((n - firstframe) * itperprint + it) * self.dt
+                currenttime = real_time = step_time
                 if self.stats:
                     mass, energy = eng.after_step_stats()
                     with open(os.path.join(self.workdir, "out", "stats.dat"), "a") as fh:
@@ -470,8 +490,10 @@ def main(argv=None):
     ap = argparse.ArgumentParser(description="BSAM time steps with regrid on the GPU engine")
     ap.add_argument("workdir")
     ap.add_argument("--device", type=int, default=0)
+    ap.add_argument("--metrics", default=None,
+                    help="append one JSON line per solve (step, V-cycles, residual, status, ms)")
     a = ap.parse_args(argv)
-    drv = Driver(a.workdir, device=a.device)
+    drv = Driver(a.workdir, device=a.device, metrics=a.metrics)
     try:
         print(json.dumps(drv.run()))
     finally:

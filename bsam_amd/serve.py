@@ -303,6 +303,20 @@ def serial(requests: List[Request], precision: int = E.PREC_FP64, device: int = 
             "vcycles": int(sum(sum(r.cycles) for r in results)), "results": results}
 
 
+def write_log(path: str, reqs: List[Request], results: List[Result]):
+    """One JSON line per served request, in completion order."""
+    by_id = {r.rid: r for r in reqs}
+    with open(path, "w") as fh:
+        for res in sorted(results, key=lambda x: x.finished):
+            q = by_id[res.rid]
+            fh.write(json.dumps({
+                "rid": res.rid, "ndims": q.ndims, "side": q.side, "eps": q.eps,
+                "steps": len(res.cycles), "arrival_s": res.arrival, "admitted_s": res.admitted,
+                "finished_s": res.finished, "latency_s": res.latency,
+                "vcycles_per_step": res.cycles, "converged": res.converged,
+                "footprint_bytes": res.footprint}) + "\n")
+
+
 def main():
     ap = argparse.ArgumentParser(description="serve a stream of CH solve requests")
     ap.add_argument("--requests", type=int, default=256)
@@ -316,6 +330,9 @@ def main():
                     help="all requests waiting at t = 0 (capacity rather than arrival-bound)")
     ap.add_argument("--batch", type=int, default=1,
                     help="requests of one shape class admitted together into one context")
+    ap.add_argument("--log", default=None,
+                    help="write one JSON line per served request (arrival, admission, "
+                         "completion, latency, V-cycles per step, converged)")
     args = ap.parse_args()
     reqs = make_requests(args.requests, args.rate, steps=args.steps)
     prec = E.PREC_MIXED if args.precision == "mixed" else E.PREC_FP64
@@ -323,6 +340,8 @@ def main():
     out = srv.run(reqs, realtime=not args.burst)
     out["peak_active"] = srv.peak_active
     out["peak_bytes"] = srv.peak_used
+    if args.log:
+        write_log(args.log, reqs, out["results"])
     out.pop("results")
     line = {"served": out}
     if args.baseline:

@@ -10,6 +10,7 @@ byte-identical: the hierarchy every step (patch lists, mglobal, mx) and q
 with its ghost layer, as printed.
 """
 
+import json
 import os
 import time
 
@@ -113,9 +114,16 @@ def test_native_driver_statistics(tmp_path):
     bp.run(bp.REF_PROGRAM, deck, str(tmp_path / "ref"))
     work = str(tmp_path / "native")
     bp.write_deck(work, deck)
-    drv = Driver(work, device=0)
+    metrics = str(tmp_path / "solves.jsonl")
+    drv = Driver(work, device=0, metrics=metrics)
     drv.run()
     drv.close()
+    # the per-solve log: one line per MultigridIterations call, all converged
+    with open(metrics) as fh:
+        log = [json.loads(l) for l in fh]
+    assert [r["step"] for r in log] == [1, 2, 3, 4]
+    assert all(r["converged"] and r["status"] == "converged" and r["vcycles"] >= 1 and
+               r["residual"] < 1e-8 and r["cells"] > 0 for r in log), log
     rows = []
     for d in ("ref", "native"):
         with open(str(tmp_path / d / "out" / "stats.dat")) as fh:

@@ -21,6 +21,25 @@ def test_requests_follow_the_configuration():
     assert 0.2 < np.mean([r.ndims == 3 for r in reqs]) < 0.4
 
 
+def test_request_log_lines(tmp_path):
+    """The server's request log: one JSON line per served request, in
+    completion order, with its latency and V-cycles per step."""
+    import json
+    reqs = serve.make_requests(3, 50.0, steps=2)
+    results = [serve.Result(rid=r.rid, arrival=r.arrival, admitted=r.arrival + 0.1,
+                            finished=r.arrival + 1.0 - 0.1 * n, cycles=[5, 4])
+               for n, r in enumerate(reqs)]
+    path = str(tmp_path / "served.jsonl")
+    serve.write_log(path, reqs, results)
+    with open(path) as fh:
+        lines = [json.loads(l) for l in fh]
+    assert [l["finished_s"] for l in lines] == sorted(r.finished for r in results)
+    for l in lines:
+        q = next(r for r in reqs if r.rid == l["rid"])
+        assert (l["ndims"], l["side"], l["steps"], l["vcycles_per_step"]) == (q.ndims, q.side, 2, [5, 4])
+        assert abs(l["latency_s"] - (l["finished_s"] - l["arrival_s"])) < 1e-12
+
+
 def test_footprint_grows_with_size_and_precision():
     small = serve.Request(0, 2, 64, 0.03, 1)
     big = serve.Request(1, 2, 256, 0.03, 1)
