@@ -236,6 +236,9 @@ def main():
                     help="level-kernel tile as X,Y (powers of two; the z extent fills 256 "
                          "cells in 3D) instead of the shape timed at commit, e.g. for "
                          "counter runs whose serialised kernels would distort that timing")
+    ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
+                    help="engine tuning key set before the hierarchy (repeatable), "
+                         "e.g. short_ghost=0")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the solver path here is single-GPU "
@@ -253,6 +256,9 @@ def main():
         if args.tile:
             tx, ty = (int(v) for v in args.tile.split(","))
             tune.update(tile_x=tx, tile_y=ty)
+        for kv in args.tune:
+            key, val = kv.split("=", 1)
+            tune[key] = float(val)
         eng = E.Engine(layout, dt=params["dt"], eps=params["eps"], well=params["well"],
                        nsmoothingpasses=params["nsmoothingpasses"], maxvcycles=50,
                        qerrortol=params["qerrortol"], precision=precision, device=0,
@@ -391,6 +397,7 @@ def main():
             "tile_auto": {"on": bool(eng.tune_get("tile_auto")),
                           "candidates_timed": int(eng.tune_get("tile_auto_candidates")),
                           "relax_rounds_us": eng.tune_get("tile_auto_us")},
+            "tune": dict(kv.split("=", 1) for kv in args.tune) or None,
         },
         "roofline": roof,
         "kernels": kinds,

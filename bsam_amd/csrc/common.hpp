@@ -105,6 +105,29 @@ struct GhostOp {
 };
 static_assert(sizeof(GhostOp) == 32, "ghost update record: 32 bytes");
 
+// The short device form of a copy (kind 0: src is the source) or of a kind-1
+// interpolation whose fine sources lie within 16-bit offsets of the
+// destination (src is the coarse source, the fine ones are dst + d0 and
+// dst + d1). 16 bytes.
+struct GhostS {
+  int dst, src;
+  short d0, d1;
+  unsigned char kind, nflip;
+  short group;
+};
+static_assert(sizeof(GhostS) == 16, "short ghost update record: 16 bytes");
+
+// A list of ghost updates on the device: the short records, then the others
+// in the 32-byte form. A list applied per tile has per-tile starts into each
+// (ss, sl: nt + 1 entries); a list applied grid-stride has none (ss null) and
+// ns, nl updates. An empty list has ns = nl = 0 and null starts.
+struct GhostList {
+  const GhostS *s;
+  const GhostOp *l;
+  const int *ss, *sl;
+  int ns, nl;
+};
+
 // A tile of cells of one patch: origin (i0, j0, k0) in the patch's index
 // space; the tile extent is fixed per kernel.
 struct Tile {

@@ -51,6 +51,28 @@ template <class T> T *dev_upload(const std::vector<T> &v, hipStream_t s) {
   return p;
 }
 
+// A ghost-update list in its device records (ghost_records), viewed by the
+// kernels as a GhostList.
+struct DGhost {
+  GhostS *s = nullptr;
+  GhostOp *l = nullptr;
+  int *ss = nullptr, *sl = nullptr;
+  int ns = 0, nl = 0;
+  GhostList view() const { return GhostList{s, l, ss, sl, ns, nl}; }
+};
+static DGhost upload_ghost(const std::vector<GhostOp> &ops, const std::vector<int> *start,
+                           bool use_short, hipStream_t st) {
+  const GhostRecords r = ghost_records(ops, start, use_short);
+  DGhost d;
+  d.s = dev_upload(r.s, st);
+  d.l = dev_upload(r.l, st);
+  d.ss = dev_upload(r.ss, st);
+  d.sl = dev_upload(r.sl, st);
+  d.ns = (int)r.s.size();
+  d.nl = (int)r.l.size();
+  return d;
+}
+
 struct LevelData {
   std::vector<int> patches;  // patch indices in level-list order
   int *d_patch_list = nullptr;  // the same on the device (regrid kernels)
@@ -63,7 +85,7 @@ struct LevelData {
   // linear tiles in device order over each patch's ghost-extended q array
   std::vector<Tile> t_qlin;
   std::vector<GhostOp> gp[3];
-  GhostOp *d_gp[3] = {nullptr, nullptr, nullptr};
+  DGhost d_gp[3];
   GhostOp *d_tgp[3] = {nullptr, nullptr, nullptr};  // patch-local copies (tail levels)
   // SetGhost(ipass) fused into the next relaxation, per transition
   // (ipass -> colour): 0: 0 -> 1, 1: 1 -> 2, 2: 2 -> 1
@@ -71,26 +93,22 @@ struct LevelData {
   // SetGhost(ipass) of this level may run inside this level's restriction or
   // fill-down launch (its reads avoid the parent cells those launches write)
   bool down_ok[3] = {false, false, false};
-  int *d_fstart[3] = {nullptr, nullptr, nullptr};
-  GhostOp *d_fops[3] = {nullptr, nullptr, nullptr};
-  GhostOp *d_fpre[3] = {nullptr, nullptr, nullptr};   // launched before the fused relaxation
+  DGhost d_fops[3];   // per tile
+  DGhost d_fpre[3];   // launched before the fused relaxation
   // the next launch's pre updates applied at the end of a relaxation launch,
   // per launch kind (post_key): the updates and, for the next launch, what
   // still runs before it
   // key 4 (kPostFinal): the last launch of a smoothing (SetGhost(1) -> 2)
   // ending with most of SetGhost(2), the program that follows the smoothing
   PostGhost post[5];
-  int *d_pstart[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
-  GhostOp *d_pops[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
-  GhostOp *d_prest[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  DGhost d_pops[5];   // per tile
+  DGhost d_prest[5];
   // the parent's SetGhost(0) inside this level's coarse ghost + loading
   // launch, and its SetGhost(2) (the last of post-smoothing) inside this
   // level's correction launch
   RideGhost ride_cgl, ride_corr;
   struct DRide {
-    GhostOp *rest = nullptr, *ops = nullptr;
-    int *start = nullptr;
-    int nrest = 0;
+    DGhost rest, ops;   // grid-stride, per tile
   } d_ride_cgl, d_ride_corr;
   long long cells = 0;
   // per group: its interior tiles of this level (first, count; a group's
@@ -235,6 +253,7 @@ struct bsam_ctx {
   int fuse_ghost = 1;               // apply SetGhost inside the next relaxation
   int post_ghost = 1;               // and the next launch's pre updates at a launch's end
   int relax_lds = 0;                // relaxation stencils read through an LDS copy of the tile
+  int short_ghost = 1;              // 16-byte device records for the updates that have them
   int tail_top = 0;                 // highest level run by the tail kernels
   TailLevel *d_tail = nullptr;      // levels minlevel..tail_top
   hipGraph_t graph = nullptr, graph_mixed = nullptr;
@@ -298,6 +317,13 @@ void bsam_ctx::free_device() {
     if (p) (void)hipFree((void *)p);
     p = nullptr;
   };
+  auto FG = [&](DGhost &g) {
+    F(g.s);
+    F(g.l);
+    F(g.ss);
+    F(g.sl);
+    g.ns = g.nl = 0;
+  };
   F(q); F(qold); F(qc); F(f); F(rf); F(ftmp); F(d_P); F(d_partial); F(d_hist);
   F(e); F(qce); F(fe); F(qb);
   e = qce = fe = qb = nullptr;
@@ -326,22 +352,18 @@ void bsam_ctx::free_device() {
     F(l.d_cgl); F(l.d_blk);
     F(l.d_granges); F(l.d_gcells);
     for (int p = 0; p < 3; p++) {
-      F(l.d_gp[p]);
+      FG(l.d_gp[p]);
       F(l.d_tgp[p]);
-      F(l.d_fstart[p]);
-      F(l.d_fops[p]);
-      F(l.d_fpre[p]);
+      FG(l.d_fops[p]);
+      FG(l.d_fpre[p]);
     }
     for (int k = 0; k < 5; k++) {
-      F(l.d_pstart[k]);
-      F(l.d_pops[k]);
-      F(l.d_prest[k]);
+      FG(l.d_pops[k]);
+      FG(l.d_prest[k]);
     }
     for (LevelData::DRide *r : {&l.d_ride_cgl, &l.d_ride_corr}) {
-      F(r->rest);
-      F(r->start);
-      F(r->ops);
-      r->nrest = 0;
+      FG(r->rest);
+      FG(r->ops);
     }
   }
   if (h_state) (void)hipHostFree(h_state);
@@ -957,25 +979,22 @@ static void commit(bsam_ctx *c) {
     L.d_cext = dev_upload(L.t_cext, s);
     L.d_cgl = dev_upload(L.t_cgl, s);
     L.d_blk = dev_upload(L.t_blk, s);
-    for (int ip = 0; ip < 3; ip++) L.d_gp[ip] = dev_upload(L.gp[ip], s);
+    const bool sh = c->short_ghost != 0;
+    for (int ip = 0; ip < 3; ip++) L.d_gp[ip] = upload_ghost(L.gp[ip], nullptr, sh, s);
     for (int f = 0; f < 3; f++)
       if (L.fused[f].ok) {
-        L.d_fstart[f] = dev_upload(L.fused[f].start, s);
-        L.d_fops[f] = dev_upload(L.fused[f].ops, s);
-        L.d_fpre[f] = dev_upload(L.fused[f].pre, s);
+        L.d_fops[f] = upload_ghost(L.fused[f].ops, &L.fused[f].start, sh, s);
+        L.d_fpre[f] = upload_ghost(L.fused[f].pre, nullptr, sh, s);
       }
     for (int k = 0; k < 5; k++)
       if (L.post[k].ok) {
-        L.d_pstart[k] = dev_upload(L.post[k].start, s);
-        L.d_pops[k] = dev_upload(L.post[k].ops, s);
-        L.d_prest[k] = dev_upload(L.post[k].rest, s);
+        L.d_pops[k] = upload_ghost(L.post[k].ops, &L.post[k].start, sh, s);
+        L.d_prest[k] = upload_ghost(L.post[k].rest, nullptr, sh, s);
       }
     auto up_ride = [&](const RideGhost &r, LevelData::DRide &d) {
       if (!r.ok) return;
-      d.rest = dev_upload(r.rest, s);
-      d.nrest = (int)r.rest.size();
-      d.start = dev_upload(r.start, s);
-      d.ops = dev_upload(r.ops, s);
+      d.rest = upload_ghost(r.rest, nullptr, sh, s);
+      d.ops = upload_ghost(r.ops, &r.start, sh, s);
     };
     up_ride(L.ride_cgl, L.d_ride_cgl);
     up_ride(L.ride_corr, L.d_ride_corr);
@@ -1053,7 +1072,7 @@ static void op_set_ghost(bsam_ctx *c, int level, int ipass, int prec = PREC_SOLU
   if (ipass < 0 || ipass > 2) throw Error("ipass must be 0, 1 or 2");
   ProfScope ps(c, BSAM_PROF_GHOST,
                (prec == PREC_SOLUTION_FP64 ? 32.0 : 16.0) * L.gp[ipass].size());
-  HIPCK(launch_ghost(prec, c->dev(), L.d_gp[ipass], (int)L.gp[ipass].size(), c->stream));
+  HIPCK(launch_ghost(prec, c->dev(), L.d_gp[ipass].view(), c->stream));
 }
 // What remains of SetGhost(2) on `level` after the last relaxation launch of
 // a smoothing applied its final plan (a pending kRestOnly + 2).
@@ -1062,7 +1081,7 @@ static void op_set_ghost_rest(bsam_ctx *c, int level, int prec) {
   const std::vector<GhostOp> &r = L.post[kPostFinal].rest;
   if (r.empty()) return;
   ProfScope ps(c, BSAM_PROF_GHOST, (prec == PREC_SOLUTION_FP64 ? 32.0 : 16.0) * r.size());
-  HIPCK(launch_ghost(prec, c->dev(), L.d_prest[kPostFinal], (int)r.size(), c->stream));
+  HIPCK(launch_ghost(prec, c->dev(), L.d_prest[kPostFinal].view(), c->stream));
 }
 // Applies a pending SetGhost of `level` now: whole, or its rest.
 static void op_apply_pending(bsam_ctx *c, int level, int pending, int prec) {
@@ -1074,7 +1093,7 @@ static void op_relax(bsam_ctx *c, int level, int color, int prec = PREC_SOLUTION
   ProfScope ps(c, BSAM_PROF_RELAX, (prec == PREC_SOLUTION_FP64 ? 24.0 : 12.0) * L.cells);
   if (c->relax_lds)   // the staged form of the same sweep, with no ghost updates
     HIPCK(launch_relax_fused(c->nd(), prec, c->dev(), L.d_int, (int)L.t_int.size(), color,
-                             nullptr, nullptr, nullptr, nullptr, true, c->stream));
+                             GhostList{}, GhostList{}, true, c->stream));
   else
     HIPCK(launch_relax(c->nd(), prec, c->dev(), L.d_int, (int)L.t_int.size(), color,
                        c->stream));
@@ -1087,22 +1106,20 @@ static LevelData &child_level(bsam_ctx *c, int level) {
 // runs in the same launch when its reads allow, else just before.
 static void op_restrict(bsam_ctx *c, int level, int prec = PREC_SOLUTION_FP64, int ghost = -1) {
   LevelData &L = child_level(c, level);
-  const GhostOp *gops = nullptr;
-  int ng = 0;
+  GhostList gops{};
   if (ghost >= kRestOnly) {
     op_set_ghost_rest(c, level, prec);
     ghost = -1;
   }
   if (ghost >= 0) {
     if (c->fuse_ghost && L.down_ok[ghost]) {
-      gops = L.d_gp[ghost];
-      ng = (int)L.gp[ghost].size();
+      gops = L.d_gp[ghost].view();
     } else {
       op_set_ghost(c, level, ghost, prec);
     }
   }
   ProfScope ps(c, BSAM_PROF_RESTRICT, 24.0 * L.cells);
-  HIPCK(launch_restrict(c->nd(), prec, c->dev(), L.d_coarse, (int)L.t_coarse.size(), gops, ng,
+  HIPCK(launch_restrict(c->nd(), prec, c->dev(), L.d_coarse, (int)L.t_coarse.size(), gops,
                         c->stream));
 }
 static void op_coarse_ghost(bsam_ctx *c, int level, int prec = PREC_SOLUTION_FP64) {
@@ -1138,8 +1155,8 @@ static void op_coarse_ghost_load(bsam_ctx *c, int level, int prec = PREC_SOLUTIO
   const LevelData::DRide *r = parent_ride(c, L, parent_ghost, level, prec, false);
   ProfScope ps(c, BSAM_PROF_COARSE_LOAD, 40.0 * L.cells);
   HIPCK(launch_coarse_ghost_load(c->nd(), prec, c->dev(), L.d_cgl, (int)L.t_cgl.size(),
-                                 r ? r->rest : nullptr, r ? r->nrest : 0,
-                                 r ? r->start : nullptr, r ? r->ops : nullptr, c->stream));
+                                 r ? r->rest.view() : GhostList{},
+                                 r ? r->ops.view() : GhostList{}, c->stream));
 }
 static void op_correct(bsam_ctx *c, int level, int prec = PREC_SOLUTION_FP64,
                        int parent_ghost = -1) {
@@ -1147,8 +1164,8 @@ static void op_correct(bsam_ctx *c, int level, int prec = PREC_SOLUTION_FP64,
   const LevelData::DRide *r = parent_ride(c, L, parent_ghost, level, prec, true);
   ProfScope ps(c, BSAM_PROF_CORRECT, 40.0 * L.cells);
   HIPCK(launch_correct(c->nd(), prec, c->dev(), L.d_blk, (int)L.t_blk.size(),
-                       r ? r->rest : nullptr, r ? r->nrest : 0, r ? r->start : nullptr,
-                       r ? r->ops : nullptr, c->stream));
+                       r ? r->rest.view() : GhostList{}, r ? r->ops.view() : GhostList{},
+                       c->stream));
 }
 // FillDown, afasroutines.f90:223-239. For q, each SetGhost(level - 1, 0) runs
 // inside the next level's fill-down launch when its reads allow; `ghost` >= 0
@@ -1165,16 +1182,14 @@ static void op_fill_down(bsam_ctx *c, int field, int ghost = -1, int top = INT_M
   int level = std::min(top, c->finest);
   for (; level >= low; level--) {
     LevelData &L = c->L(level);
-    const GhostOp *gops = nullptr;
-    int ng = 0;
+    GhostList gops{};
     if (carry >= kRestOnly) {
       op_set_ghost_rest(c, level, PREC_SOLUTION_FP64);
       carry = -1;
     }
     if (carry >= 0) {
       if (c->fuse_ghost && L.down_ok[carry]) {
-        gops = L.d_gp[carry];
-        ng = (int)L.gp[carry].size();
+        gops = L.d_gp[carry].view();
       } else {
         op_set_ghost(c, level, carry);
       }
@@ -1182,7 +1197,7 @@ static void op_fill_down(bsam_ctx *c, int field, int ghost = -1, int top = INT_M
     {
       ProfScope ps(c, BSAM_PROF_FILLDOWN, 20.0 * L.cells);
       HIPCK(launch_filldown(c->nd(), c->dev(), L.d_coarse, (int)L.t_coarse.size(),
-                            field == BSAM_FIELD_F ? 1 : 2, gops, ng, c->stream));
+                            field == BSAM_FIELD_F ? 1 : 2, gops, c->stream));
     }
     carry = field == BSAM_FIELD_Q ? 0 : -1;
   }
@@ -1227,15 +1242,14 @@ static int level_relax(bsam_ctx *c, int level, int prec = PREC_SOLUTION_FP64,
       if (last && final_post && c->fuse_ghost && c->post_ghost && pending == 1 && color == 2)
         k = kPostFinal;
       if (k >= 0 && !L.post[k].ok) k = -1;
-      const int *pst = k >= 0 ? L.d_pstart[k] : nullptr;
-      const GhostOp *pop = k >= 0 ? L.d_pops[k] : nullptr;
+      const GhostList pop = k >= 0 ? L.d_pops[k].view() : GhostList{};
       if (fused) {
         const std::vector<GhostOp> &pre = moved >= 0 ? L.post[moved].rest : L.fused[f].pre;
         if (!pre.empty()) {
           ProfScope pg(c, BSAM_PROF_GHOST,
                        (prec == PREC_SOLUTION_FP64 ? 32.0 : 16.0) * pre.size());
-          HIPCK(launch_ghost(prec, c->dev(), moved >= 0 ? L.d_prest[moved] : L.d_fpre[f],
-                             (int)pre.size(), c->stream));
+          HIPCK(launch_ghost(prec, c->dev(),
+                             (moved >= 0 ? L.d_prest[moved] : L.d_fpre[f]).view(), c->stream));
         }
         // the sweep's bytes plus 32 B per ghost update applied in the launch
         // (16 B read, 16 B written: the stand-alone ghost launch's figure)
@@ -1243,8 +1257,7 @@ static int level_relax(bsam_ctx *c, int level, int prec = PREC_SOLUTION_FP64,
         ProfScope ps(c, BSAM_PROF_RELAX_FUSED,
                      relax_bytes + (prec == PREC_SOLUTION_FP64 ? 32.0 : 16.0) * nops);
         HIPCK(launch_relax_fused(c->nd(), prec, c->dev(), L.d_int, (int)L.t_int.size(), color,
-                                 L.d_fstart[f], L.d_fops[f], pst, pop, c->relax_lds != 0,
-                                 c->stream));
+                                 L.d_fops[f].view(), pop, c->relax_lds != 0, c->stream));
       } else {
         if (pending >= 0) op_set_ghost(c, level, pending, prec);
         if (k >= 0) {
@@ -1252,7 +1265,7 @@ static int level_relax(bsam_ctx *c, int level, int prec = PREC_SOLUTION_FP64,
                        relax_bytes +
                            (prec == PREC_SOLUTION_FP64 ? 32.0 : 16.0) * L.post[k].ops.size());
           HIPCK(launch_relax_fused(c->nd(), prec, c->dev(), L.d_int, (int)L.t_int.size(), color,
-                                   nullptr, nullptr, pst, pop, c->relax_lds != 0, c->stream));
+                                   GhostList{}, pop, c->relax_lds != 0, c->stream));
         } else {
           op_relax(c, level, color, prec);
         }
@@ -2410,6 +2423,9 @@ int bsam_tune(bsam_ctx *c, const char *key, double value) {
   } else if (k == "relax_lds") {
     c->relax_lds = value != 0.0;
     c->drop_graphs();
+  } else if (k == "short_ghost") {
+    if (c->committed || c->building) throw Error("short_ghost must be set before the hierarchy");
+    c->short_ghost = value != 0.0;
   } else {
     throw Error("unknown tuning key " + k);
   }
@@ -2430,6 +2446,7 @@ int bsam_tune_get(const bsam_ctx *cc, const char *key, double *value) {
   else if (k == "fuse_ghost") *value = c->fuse_ghost;
   else if (k == "post_ghost") *value = c->post_ghost;
   else if (k == "relax_lds") *value = c->relax_lds;
+  else if (k == "short_ghost") *value = c->short_ghost;
   else if (k == "tile_x") *value = c->tx();
   else if (k == "tile_y") *value = c->ty();
   else if (k == "tile_z") *value = c->tz();

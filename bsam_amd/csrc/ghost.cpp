@@ -1263,6 +1263,48 @@ bool program_reads_avoid(const std::vector<GhostOp> &prog,
   return true;
 }
 
+bool short_ghost_record(const GhostOp &g, GhostS &out) {
+  auto near = [&](int src) {
+    const long long d = (long long)src - g.dst;
+    return d >= -32768 && d <= 32767;
+  };
+  if (g.kind == 0) {
+    out = GhostS{g.dst, g.f0, 0, 0, g.kind, g.nflip, g.group};
+    return true;
+  }
+  if (g.kind == 1 && near(g.f0) && near(g.f1)) {
+    out = GhostS{g.dst, g.c[0], (short)(g.f0 - g.dst), (short)(g.f1 - g.dst), g.kind, g.nflip,
+                 g.group};
+    return true;
+  }
+  return false;
+}
+
+GhostRecords ghost_records(const std::vector<GhostOp> &ops, const std::vector<int> *start,
+                           bool use_short) {
+  GhostRecords r;
+  auto add = [&](const GhostOp &g) {
+    GhostS t;
+    if (use_short && short_ghost_record(g, t)) r.s.push_back(t);
+    else r.l.push_back(g);
+  };
+  if (!start) {
+    for (const GhostOp &g : ops) add(g);
+    return r;
+  }
+  const size_t nt = start->empty() ? 0 : start->size() - 1;
+  r.ss.assign(nt + 1, 0);
+  r.sl.assign(nt + 1, 0);
+  for (size_t b = 0; b < nt; b++) {
+    r.ss[b] = (int)r.s.size();
+    r.sl[b] = (int)r.l.size();
+    for (int e = (*start)[b]; e < (*start)[b + 1]; e++) add(ops[e]);
+  }
+  r.ss[nt] = (int)r.s.size();
+  r.sl[nt] = (int)r.l.size();
+  return r;
+}
+
 void set_op_groups(const std::vector<HPatch> &hp, std::vector<GhostOp> &ops) {
   std::vector<long long> qoffs(hp.size());
   for (size_t n = 0; n < hp.size(); n++) qoffs[n] = hp[n].qoff;

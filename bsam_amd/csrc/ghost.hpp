@@ -146,6 +146,21 @@ bool fused_plan_race_free(const std::vector<HPatch> &hp, const std::vector<Tile>
 bool program_reads_avoid(const std::vector<GhostOp> &prog,
                          const std::vector<unsigned char> &written);
 
+// The device records of a list of updates: the 16-byte form (GhostS) of each
+// update that has one (use_short) and the 32-byte form of the others, each in
+// the list's order. With per-tile groups (start: nt + 1 entries), ss and sl
+// are the per-tile starts into s and l, every update staying with its tile.
+struct GhostRecords {
+  std::vector<GhostS> s;
+  std::vector<GhostOp> l;
+  std::vector<int> ss, sl;
+};
+// The 16-byte form of g, if it has one: a copy, or a kind-1 interpolation
+// whose fine sources lie within 16-bit offsets of the destination.
+bool short_ghost_record(const GhostOp &g, GhostS &out);
+GhostRecords ghost_records(const std::vector<GhostOp> &ops, const std::vector<int> *start,
+                           bool use_short);
+
 // Sets every update's group to its destination patch's group.
 void set_op_groups(const std::vector<HPatch> &hp, std::vector<GhostOp> &ops);
 

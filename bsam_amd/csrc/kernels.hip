@@ -286,6 +286,21 @@ __device__ __forceinline__ void ghost_op(int nrvars, T *q, long long vs, const G
     q[g.dst + (long long)v * vs] = val;
   }
 }
+// The same update from its short record (kinds 0 and 1, ghost_op's
+// expressions).
+template <class T, bool DEF>
+__device__ __forceinline__ void ghost_op_s(int nrvars, T *q, long long vs, const GhostS &g) {
+  for (int v = 0; v < nrvars; v++) {
+    const T *f = q + (long long)v * vs;
+    T val;
+    if (g.kind == 0)
+      val = f[g.src];
+    else
+      val = (T)BSAM_A1 * f[g.dst + g.d0] + (T)BSAM_A2 * f[g.dst + g.d1] + (T)BSAM_A3 * f[g.src];
+    for (int r = 0; r < g.nflip; r++) val = DEF ? -val : (T)2 - val;
+    q[g.dst + (long long)v * vs] = val;
+  }
+}
 
 // RestrictSolution (afasroutines.f90:600-644) for coarse cell (i,j,k): qc =
 // R(q) and the parent's q under the child takes the same value.
@@ -716,28 +731,44 @@ __global__ __launch_bounds__(RELAX_THREADS) void k_relax(Dev d, const Tile *tile
   relax_cell<ND, T, DEF>(d.prm, L, i, j, k);
 }
 
-// Ghost updates a workgroup applies for its own tile before its work: the
-// updates ops[s0 .. s1) whose destinations the tile reads (the same update
-// may be applied by several tiles, each writing the same value). The caller
-// loads the bounds together with its tile record.
+// Ghost updates a workgroup applies for its own tile: list g's updates of tile
+// b (the same update may be applied by several tiles, each writing the same
+// value), then a workgroup barrier if there were any.
 template <class T>
-__device__ __forceinline__ void ghost_tile(const Dev &d, int s0, int s1, const GhostOp *ops) {
-  if (s1 > s0) {
+__device__ __forceinline__ void ghost_tile(const Dev &d, const GhostList &g, int b) {
+  if (!g.ss) return;
+  const int s0 = g.ss[b], s1 = g.ss[b + 1], l0 = g.sl[b], l1 = g.sl[b + 1];
+  if (s1 > s0 || l1 > l0) {
     T *q = fields<T>(d).q;
+    constexpr bool DEF = std::is_same<T, float>::value;
     for (int e = s0 + (int)threadIdx.x; e < s1; e += blockDim.x)
-      ghost_op<T, std::is_same<T, float>::value>(d.nrvars, q, d.qvs, ops[e]);
+      ghost_op_s<T, DEF>(d.nrvars, q, d.qvs, g.s[e]);
+    for (int e = l0 + (int)threadIdx.x; e < l1; e += blockDim.x)
+      ghost_op<T, DEF>(d.nrvars, q, d.qvs, g.l[e]);
     __syncthreads();
   }
+}
+// A list's updates spread over the whole grid (no ordering among them),
+// skipping those of stopped groups.
+template <class T>
+__device__ __forceinline__ void ghost_alongside(const Dev &d, const GhostList &g) {
+  T *q = fields<T>(d).q;
+  constexpr bool DEF = std::is_same<T, float>::value;
+  const int stride = gridDim.x * blockDim.x, e0 = blockIdx.x * blockDim.x + threadIdx.x;
+  for (int e = e0; e < g.ns; e += stride)
+    if (!stopped(d, g.s[e].group)) ghost_op_s<T, DEF>(d.nrvars, q, d.qvs, g.s[e]);
+  for (int e = e0; e < g.nl; e += stride)
+    if (!stopped(d, g.l[e].group)) ghost_op<T, DEF>(d.nrvars, q, d.qvs, g.l[e]);
 }
 
 // SetGhost(level, ipass) followed by the relaxation of colour `color`, in one
 // launch: each workgroup first applies the ghost updates its tile's cells read
-// (ops[start[b] .. start[b+1]), chosen on the host so that no update reads a
-// cell another tile relaxes), then relaxes its tile. With post lists, the
-// workgroup then synchronises and applies pops[pstart[b] .. pstart[b+1]):
+// (its share of list `pre`, chosen on the host so that no update reads a cell
+// another tile relaxes), then relaxes its tile. With a `post` list, the
+// workgroup then synchronises and applies its share of it:
 // updates of the next SetGhost(level, color) that read cells this tile has
 // just relaxed (chosen on the host so that no other tile touches their
-// cells in this launch). start or pstart null: no list.
+// cells in this launch). A list with null starts: no updates.
 //
 // With STAGE, the tile's cells and their one-cell halo (both variables) are
 // first copied into LDS in the same parity-split row layout (a box of
@@ -748,21 +779,17 @@ __device__ __forceinline__ void ghost_tile(const Dev &d, int s0, int s1, const G
 extern __shared__ __align__(16) unsigned char relax_lds[];
 template <int ND, class T, bool DEF, bool STAGE>
 __global__ __launch_bounds__(RELAX_THREADS) void k_relax_fused(Dev d, const Tile *tiles, int nt,
-                                                                int color, const int *start,
-                                                                const GhostOp *ops,
-                                                                const int *pstart,
-                                                                const GhostOp *pops) {
+                                                                int color, GhostList pre,
+                                                                GhostList post) {
   const int b = xcd_tile(nt);
   if (b >= nt) return;
-  const Tile t = tiles[b];   // loaded together with the update bounds
-  const int s0 = start ? start[b] : 0, s1 = start ? start[b + 1] : 0;
-  const int p0 = pstart ? pstart[b] : 0, p1 = pstart ? pstart[b + 1] : 0;
+  const Tile t = tiles[b];
   if (stopped_patch(d, t.patch)) return;   // the tile's updates are its patch's group's
-  ghost_tile<T>(d, s0, s1, ops);
+  ghost_tile<T>(d, pre, b);
   Lvl<T> L = level_ref<T>(d, t.patch);
   int i, j, k;
   tile_color_cell<ND>(d, t, color, i, j, k);
-  // every thread reaches the barriers below (p0, p1 are the block's)
+  // every thread reaches the barriers below (they depend on the block only)
   const bool mine = in_box<ND>(i, j, k, L.P.mx, L.P.my, L.P.mz);
   if constexpr (STAGE) {
     const int tx = 1 << d.tile_lx, ty = 1 << d.tile_ly;
@@ -793,38 +820,28 @@ __global__ __launch_bounds__(RELAX_THREADS) void k_relax_fused(Dev d, const Tile
   } else {
     if (mine) relax_cell<ND, T, DEF>(d.prm, L, i, j, k);
   }
-  if (p1 > p0) {
+  if (post.ss && (post.ss[b + 1] > post.ss[b] || post.sl[b + 1] > post.sl[b])) {
     __syncthreads();
-    T *q = fields<T>(d).q;
-    for (int e = p0 + (int)threadIdx.x; e < p1; e += blockDim.x)
-      ghost_op<T, std::is_same<T, float>::value>(d.nrvars, q, d.qvs, pops[e]);
+    ghost_tile<T>(d, post, b);
   }
 }
 
 // One SetGhost(level, ipass) program: every ghost-cell update is independent.
+// (one thread per update, short records first)
 template <class T, bool DEF>
-__global__ __launch_bounds__(256) void k_ghost(Dev d, const GhostOp *ops, int n) {
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= n) return;
-  const GhostOp &g = ops[e];
-  if (stopped(d, g.group)) return;
-  ghost_op<T, DEF>(d.nrvars, fields<T>(d).q, d.qvs, g);
-}
-
-// A SetGhost program whose reads are disjoint from what the host kernel
-// writes, applied by the host kernel's threads (grid-stride) before their own
-// work; no thread of the host kernel reads what it writes.
-template <class T>
-__device__ __forceinline__ void ghost_alongside(const Dev &d, const GhostOp *ops, int n) {
-  T *q = fields<T>(d).q;
-  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x)
-    if (!stopped(d, ops[e].group))
-      ghost_op<T, std::is_same<T, float>::value>(d.nrvars, q, d.qvs, ops[e]);
+__global__ __launch_bounds__(256) void k_ghost(Dev d, GhostList g) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < g.ns) {
+    if (!stopped(d, g.s[e].group)) ghost_op_s<T, DEF>(d.nrvars, fields<T>(d).q, d.qvs, g.s[e]);
+  } else if (e - g.ns < g.nl) {
+    const GhostOp &u = g.l[e - g.ns];
+    if (!stopped(d, u.group)) ghost_op<T, DEF>(d.nrvars, fields<T>(d).q, d.qvs, u);
+  }
 }
 
 template <int ND, class T>
 __global__ __launch_bounds__(256) void k_restrict(Dev d, const Tile *tiles, int nt,
-                                                  const GhostOp *gops, int ng) {
+                                                  GhostList gops) {
   const int b = xcd_tile(nt);
   if (b < nt && !stopped_patch(d, tiles[b].patch)) {
     const Tile t = tiles[b];
@@ -836,12 +853,12 @@ __global__ __launch_bounds__(256) void k_restrict(Dev d, const Tile *tiles, int 
   }
   // the riding program reads no parent cell this launch writes and writes
   // fine ghost cells it does not read, so it runs after the work
-  ghost_alongside<T>(d, gops, ng);
+  ghost_alongside<T>(d, gops);
 }
 
 template <int ND, class T>
 __global__ __launch_bounds__(256) void k_filldown(Dev d, const Tile *tiles, int nt, int field,
-                                                  const GhostOp *gops, int ng) {
+                                                  GhostList gops) {
   const int b = xcd_tile(nt);
   if (b < nt && !stopped_patch(d, tiles[b].patch)) {
     const Tile t = tiles[b];
@@ -851,7 +868,7 @@ __global__ __launch_bounds__(256) void k_filldown(Dev d, const Tile *tiles, int 
     if (in_box<ND>(i, j, k, C.P.cmx, C.P.cmy, C.P.cmz))
       filldown_cell<ND, T>(d.nrvars, C, level_ref<T>(d, C.P.parent), i, j, k, field);
   }
-  ghost_alongside<T>(d, gops, ng);   // as in k_restrict
+  ghost_alongside<T>(d, gops);   // as in k_restrict
 }
 
 // Tiles cover the ghost-extended coarse box 0..cmx+1.
@@ -882,20 +899,16 @@ __global__ __launch_bounds__(256) void k_coarse_load(Dev d, const Tile *tiles, i
 }
 
 // Linear tiles over the ghost-extended coarse box 0..cmx+1 (LIN_TILE cells
-// each). A SetGhost program of the parent may ride along: gops[0..ng)
-// grid-stride, and per tile the updates it reads (tstart/tops; tstart null
-// for none).
+// each). A SetGhost program of the parent may ride along: `rest` grid-stride,
+// and per tile the updates it reads (`tops`).
 template <int ND, class T, bool DEF>
 __global__ __launch_bounds__(256) void k_coarse_ghost_load(Dev d, const Tile *tiles, int nt,
-                                                           const GhostOp *gops, int ng,
-                                                           const int *tstart,
-                                                           const GhostOp *tops) {
+                                                           GhostList rest, GhostList tops) {
   const int b = xcd_tile(nt);
   // the tile's updates are of its patch's parent, in the same group
   if (b < nt && !stopped_patch(d, tiles[b].patch)) {
     const Tile t = tiles[b];
-    const int s0 = tstart ? tstart[b] : 0, s1 = tstart ? tstart[b + 1] : 0;
-    ghost_tile<T>(d, s0, s1, tops);
+    ghost_tile<T>(d, tops, b);
     Lvl<T> C = level_ref<T>(d, t.patch);
     int i, j, k;
     if (lin_cell<ND>(t, C.P.cmx + 2, C.P.cmy + 2, C.P.cmz + 2, i, j, k))
@@ -903,7 +916,7 @@ __global__ __launch_bounds__(256) void k_coarse_ghost_load(Dev d, const Tile *ti
                                          k);
   }
   // the updates no tile reads, after the work (they touch nothing it uses)
-  ghost_alongside<T>(d, gops, ng);
+  ghost_alongside<T>(d, rest);
 }
 
 // Linear tiles over the 2x2(x2) fine blocks (m, n(, o)) = 0..cmx of the
@@ -912,20 +925,18 @@ __global__ __launch_bounds__(256) void k_coarse_ghost_load(Dev d, const Tile *ti
 // program of the parent may ride along as in k_coarse_ghost_load.
 template <int ND, class T>
 __global__ __launch_bounds__(256) void k_correct(Dev d, const Tile *tiles, int nt,
-                                                 const GhostOp *gops, int ng, const int *tstart,
-                                                 const GhostOp *tops) {
+                                                 GhostList rest, GhostList tops) {
   const int b = xcd_tile(nt);
   if (b < nt && !stopped_patch(d, tiles[b].patch)) {
     const Tile t = tiles[b];
-    const int s0 = tstart ? tstart[b] : 0, s1 = tstart ? tstart[b + 1] : 0;
-    ghost_tile<T>(d, s0, s1, tops);
+    ghost_tile<T>(d, tops, b);
     Lvl<T> C = level_ref<T>(d, t.patch);
     int m, n, o;
     if (lin_cell<ND>(t, C.P.cmx + 1, C.P.cmy + 1, C.P.cmz + 1, m, n, o))
       correct_block<ND, T>(d.nrvars, C, level_ref<T>(d, C.P.parent), m, n, o);
   }
   // the updates no tile reads, after the work (they touch nothing it uses)
-  ghost_alongside<T>(d, gops, ng);
+  ghost_alongside<T>(d, rest);
 }
 
 // RelativeTruncationError over the coarse interior tiles of a level.
@@ -1717,9 +1728,8 @@ hipError_t launch_relax(int nd, int prec, const Dev &d, const Tile *tiles, int n
 }
 
 hipError_t launch_relax_fused(int nd, int prec, const Dev &d, const Tile *tiles, int nt,
-                              int color, const int *start, const GhostOp *ops,
-                              const int *pstart, const GhostOp *pops, bool stage,
-                              hipStream_t s) {
+                              int color, const GhostList &pre, const GhostList &post,
+                              bool stage, hipStream_t s) {
   if (nt <= 0) return hipSuccess;
   if (d.tile_lx < 1) return hipErrorInvalidValue;
   const dim3 grid(xcd_grid(nt));
@@ -1731,58 +1741,59 @@ hipError_t launch_relax_fused(int nd, int prec, const Dev &d, const Tile *tiles,
     if (stage) {
       if (nd == 2)
         k_relax_fused<2, double, false, true><<<grid, RELAX_THREADS, lds, s>>>(
-            d, tiles, nt, color, start, ops, pstart, pops);
+            d, tiles, nt, color, pre, post);
       else
         k_relax_fused<3, double, false, true><<<grid, RELAX_THREADS, lds, s>>>(
-            d, tiles, nt, color, start, ops, pstart, pops);
+            d, tiles, nt, color, pre, post);
     } else {
       if (nd == 2)
         k_relax_fused<2, double, false, false><<<grid, RELAX_THREADS, 0, s>>>(
-            d, tiles, nt, color, start, ops, pstart, pops);
+            d, tiles, nt, color, pre, post);
       else
         k_relax_fused<3, double, false, false><<<grid, RELAX_THREADS, 0, s>>>(
-            d, tiles, nt, color, start, ops, pstart, pops);
+            d, tiles, nt, color, pre, post);
     }
   } else {
     if (stage) {
       if (nd == 2)
         k_relax_fused<2, float, true, true><<<grid, RELAX_THREADS, lds, s>>>(
-            d, tiles, nt, color, start, ops, pstart, pops);
+            d, tiles, nt, color, pre, post);
       else
         k_relax_fused<3, float, true, true><<<grid, RELAX_THREADS, lds, s>>>(
-            d, tiles, nt, color, start, ops, pstart, pops);
+            d, tiles, nt, color, pre, post);
     } else {
       if (nd == 2)
         k_relax_fused<2, float, true, false><<<grid, RELAX_THREADS, 0, s>>>(
-            d, tiles, nt, color, start, ops, pstart, pops);
+            d, tiles, nt, color, pre, post);
       else
         k_relax_fused<3, float, true, false><<<grid, RELAX_THREADS, 0, s>>>(
-            d, tiles, nt, color, start, ops, pstart, pops);
+            d, tiles, nt, color, pre, post);
     }
   }
   return hipGetLastError();
 }
 
-hipError_t launch_ghost(int prec, const Dev &d, const GhostOp *ops, int n, hipStream_t s) {
+hipError_t launch_ghost(int prec, const Dev &d, const GhostList &ops, hipStream_t s) {
+  const int n = ops.ns + ops.nl;
   if (n <= 0) return hipSuccess;
   dim3 g((n + 255) / 256);
-  if (prec == PREC_SOLUTION_FP64) k_ghost<double, false><<<g, 256, 0, s>>>(d, ops, n);
-  else k_ghost<float, true><<<g, 256, 0, s>>>(d, ops, n);
+  if (prec == PREC_SOLUTION_FP64) k_ghost<double, false><<<g, 256, 0, s>>>(d, ops);
+  else k_ghost<float, true><<<g, 256, 0, s>>>(d, ops);
   return hipGetLastError();
 }
 
 hipError_t launch_restrict(int nd, int prec, const Dev &d, const Tile *tiles, int nt,
-                           const GhostOp *gops, int ng, hipStream_t s) {
+                           const GhostList &gops, hipStream_t s) {
   if (nt <= 0) return hipSuccess;
-  BSAM_LAUNCH_ND_T(k_restrict, nd, prec, xcd_grid(nt), 256, d, tiles, nt, gops, ng);
+  BSAM_LAUNCH_ND_T(k_restrict, nd, prec, xcd_grid(nt), 256, d, tiles, nt, gops);
   return hipGetLastError();
 }
 
 hipError_t launch_filldown(int nd, const Dev &d, const Tile *tiles, int nt, int field,
-                           const GhostOp *gops, int ng, hipStream_t s) {
+                           const GhostList &gops, hipStream_t s) {
   if (nt <= 0) return hipSuccess;
-  if (nd == 2) k_filldown<2, double><<<xcd_grid(nt), 256, 0, s>>>(d, tiles, nt, field, gops, ng);
-  else k_filldown<3, double><<<xcd_grid(nt), 256, 0, s>>>(d, tiles, nt, field, gops, ng);
+  if (nd == 2) k_filldown<2, double><<<xcd_grid(nt), 256, 0, s>>>(d, tiles, nt, field, gops);
+  else k_filldown<3, double><<<xcd_grid(nt), 256, 0, s>>>(d, tiles, nt, field, gops);
   return hipGetLastError();
 }
 
@@ -1801,19 +1812,18 @@ hipError_t launch_coarse_load(int nd, int prec, const Dev &d, const Tile *tiles,
 }
 
 hipError_t launch_coarse_ghost_load(int nd, int prec, const Dev &d, const Tile *tiles, int nt,
-                                   const GhostOp *gops, int ng, const int *tstart,
-                                   const GhostOp *tops, hipStream_t s) {
+                                   const GhostList &rest, const GhostList &tops,
+                                   hipStream_t s) {
   if (nt <= 0) return hipSuccess;
-  BSAM_LAUNCH_ND_PREC(k_coarse_ghost_load, nd, prec, xcd_grid(nt), 256, d, tiles, nt, gops, ng,
-                      tstart, tops);
+  BSAM_LAUNCH_ND_PREC(k_coarse_ghost_load, nd, prec, xcd_grid(nt), 256, d, tiles, nt, rest,
+                      tops);
   return hipGetLastError();
 }
 
 hipError_t launch_correct(int nd, int prec, const Dev &d, const Tile *tiles, int nt,
-                          const GhostOp *gops, int ng, const int *tstart, const GhostOp *tops,
-                          hipStream_t s) {
+                          const GhostList &rest, const GhostList &tops, hipStream_t s) {
   if (nt <= 0) return hipSuccess;
-  BSAM_LAUNCH_ND_T(k_correct, nd, prec, xcd_grid(nt), 256, d, tiles, nt, gops, ng, tstart, tops);
+  BSAM_LAUNCH_ND_T(k_correct, nd, prec, xcd_grid(nt), 256, d, tiles, nt, rest, tops);
   return hipGetLastError();
 }
 

@@ -8,23 +8,22 @@ namespace bsam {
 
 hipError_t launch_relax(int nd, int prec, const Dev &d, const Tile *tiles, int nt, int ipass,
                         hipStream_t s);
-// A SetGhost program's updates, grouped per tile (start has nt+1 entries),
-// then the relaxation of one colour, then (pstart non-null) the next
-// program's updates grouped per tile the same way, in one launch. start or
-// pstart may be null (no updates). With stage, the tile and its halo are
-// read through LDS (2 x (tx+2)(ty+2)(tz+2) values of dynamic LDS).
+// A SetGhost program's updates grouped per tile (list `pre`, with per-tile
+// starts), then the relaxation of one colour, then the next program's updates
+// grouped per tile the same way (list `post`), in one launch. Either list may
+// be empty. With stage, the tile and its halo are read through LDS
+// (2 x (tx+2)(ty+2)(tz+2) values of dynamic LDS).
 hipError_t launch_relax_fused(int nd, int prec, const Dev &d, const Tile *tiles, int nt,
-                              int color, const int *start, const GhostOp *ops,
-                              const int *pstart, const GhostOp *pops, bool stage,
-                              hipStream_t s);
-hipError_t launch_ghost(int prec, const Dev &d, const GhostOp *ops, int n, hipStream_t s);
-// Restriction / fill-down of one level; gops[0..ng) is a SetGhost program of
-// that level applied in the same launch (its reads must be disjoint from the
-// cells the launch writes; ng = 0 for none).
+                              int color, const GhostList &pre, const GhostList &post,
+                              bool stage, hipStream_t s);
+hipError_t launch_ghost(int prec, const Dev &d, const GhostList &ops, hipStream_t s);
+// Restriction / fill-down of one level; gops is a SetGhost program of that
+// level applied in the same launch (grid-stride; its reads must be disjoint
+// from the cells the launch writes; empty for none).
 hipError_t launch_restrict(int nd, int prec, const Dev &d, const Tile *tiles, int nt,
-                           const GhostOp *gops, int ng, hipStream_t s);
+                           const GhostList &gops, hipStream_t s);
 hipError_t launch_filldown(int nd, const Dev &d, const Tile *tiles, int nt, int field,
-                           const GhostOp *gops, int ng, hipStream_t s);
+                           const GhostList &gops, hipStream_t s);
 hipError_t launch_coarse_ghost(int nd, int prec, const Dev &d, const Tile *tiles, int nt,
                                hipStream_t s);
 hipError_t launch_coarse_load(int nd, int prec, const Dev &d, const Tile *tiles, int nt,
@@ -32,15 +31,14 @@ hipError_t launch_coarse_load(int nd, int prec, const Dev &d, const Tile *tiles,
 // GetCoarseGhostPoints + CoarseLoadingFunction in one launch over the
 // ghost-extended coarse tiles, and CorrectFine over the ghost-extended fine
 // tiles. Either may apply a SetGhost program of the parent level in the same
-// launch: gops[0..ng) grid-stride (destinations the launch does not read) and,
-// per tile b, tops[tstart[b] .. tstart[b+1]) before its work (tstart null
-// for none); the launch writes none of the program's sources.
+// launch: `rest` grid-stride (destinations the launch does not read) and, per
+// tile, its share of `tops` before its work; the launch writes none of the
+// program's sources.
 hipError_t launch_coarse_ghost_load(int nd, int prec, const Dev &d, const Tile *tiles, int nt,
-                                   const GhostOp *gops, int ng, const int *tstart,
-                                   const GhostOp *tops, hipStream_t s);
+                                   const GhostList &rest, const GhostList &tops,
+                                   hipStream_t s);
 hipError_t launch_correct(int nd, int prec, const Dev &d, const Tile *tiles, int nt,
-                          const GhostOp *gops, int ng, const int *tstart, const GhostOp *tops,
-                          hipStream_t s);
+                          const GhostList &rest, const GhostList &tops, hipStream_t s);
 hipError_t launch_source(int nd, const Dev &d, const Tile *tiles, int nt, int full,
                          hipStream_t s);
 // RelativeTruncationError over a level's coarse interior tiles (qrte) and

@@ -39,7 +39,8 @@ LAYOUTS = {
 # relaxation as separate launches; other tile shapes (in 3D the third extent
 # fills the 256 threads); relaxation stencils read through an LDS copy of the
 # tile and its halo (fused, unfused, and with another tile shape); no ghost
-# updates moved to the end of the previous launch
+# updates moved to the end of the previous launch; every ghost update in the
+# 32-byte device record
 SCHEDULES = {
     "launch": {"tail_cells": 0, "graph": 0},
     "default": {},
@@ -52,6 +53,7 @@ SCHEDULES = {
     "staged_unfused": {"relax_lds": 1, "fuse_ghost": 0, "graph": 0},
     "staged_64x4": {"relax_lds": 1, "tile_x": 64, "tile_y": 4},
     "no_post": {"post_ghost": 0},
+    "long_records": {"short_ghost": 0},
 }
 
 
