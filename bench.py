@@ -64,7 +64,7 @@ KERNELS = {
     "restrict": "k_restrict", "coarse_ghost": "k_coarse_ghost",
     "coarse_load": "k_coarse_ghost_load", "correct": "k_correct", "filldown": "k_filldown",
     "source": "k_source", "resnorm": "k_resnorm", "tail": "k_tail_vcycle", "copy": "k_copy",
-    "defect": "k_defect_setup+k_apply_correction",
+    "defect": "k_defect_setup+k_apply_correction", "smooth_iso": "k_smooth_iso",
 }
 
 WORKLOADS = {
@@ -397,6 +397,9 @@ def main():
             "tile_auto": {"on": bool(eng.tune_get("tile_auto")),
                           "candidates_timed": int(eng.tune_get("tile_auto_candidates")),
                           "relax_rounds_us": eng.tune_get("tile_auto_us")},
+            # patches whose whole smoothing runs in one workgroup out of LDS
+            "iso": {"patches": int(eng.tune_get("iso_patches")),
+                    "cells": int(eng.tune_get("iso_cells"))},
             "tune": dict(kv.split("=", 1) for kv in args.tune) or None,
         },
         "roofline": roof,

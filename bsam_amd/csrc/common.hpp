@@ -128,6 +128,31 @@ struct GhostList {
   int ns, nl;
 };
 
+// A patch of a level whose SetGhost programs read no other patch of the
+// level and whose cells no other patch's ghost update reads ("isolated"): its
+// whole smoothing (every half-sweep and the SetGhost programs between them)
+// runs in one workgroup out of LDS. The workgroup holds, per variable, the
+// patch's ghost-extended q array (qn cells, the pool layout) followed by the
+// ns parent-level cells its programs read (`stage` entries from `stage0`,
+// q-pool offsets of variable 1); program p's updates are records
+// rec[p] .. rec[p + 1] - 1.
+struct IsoPatch {
+  int patch;
+  int ns, stage0;
+  int rec[4];
+};
+// One update of an isolated patch's program in LDS indices of one variable
+// (x < qn: the patch's cell x; x >= qn: staged parent cell x - qn): the value
+// of ghost_op's kind `kind` over sources s[0], s[1] (fine) and s[2]..s[5]
+// (coarse, as many as the kind takes), flipped nflip times, into cell dst.
+// 16 bytes.
+struct IsoRec {
+  unsigned short dst;
+  unsigned char kind, nflip;
+  unsigned short s[6];
+};
+static_assert(sizeof(IsoRec) == 16, "isolated-patch update record: 16 bytes");
+
 // A tile of cells of one patch: origin (i0, j0, k0) in the patch's index
 // space; the tile extent is fixed per kernel.
 struct Tile {

@@ -111,6 +111,22 @@ struct LevelData {
     DGhost rest, ops;   // grid-stride, per tile
   } d_ride_cgl, d_ride_corr;
   long long cells = 0;
+  // Isolated patches (IsoPatch): their whole smoothing runs in one launch out
+  // of LDS. The level's other patches keep the per-colour launches over
+  // t_relax, whose fused and post plans cover only those patches' updates.
+  // Without isolated patches t_relax is the interior tile list.
+  std::vector<IsoPatch> iso;
+  std::vector<int> iso_stage;
+  std::vector<IsoRec> iso_rec;
+  IsoPatch *d_iso = nullptr;
+  int *d_iso_stage = nullptr;
+  IsoRec *d_iso_rec = nullptr;
+  size_t iso_lds = 0;           // dynamic LDS of the FP64 smoothing (FP32: half)
+  long long iso_cells = 0;      // interior cells of the isolated patches
+  long long iso_updates[3] = {0, 0, 0};   // their updates per SetGhost program
+  std::vector<Tile> t_relax;
+  Tile *d_relax = nullptr;
+  long long relax_cells = 0;
   // per group: its interior tiles of this level (first, count; a group's
   // patches are contiguous in the level list) and its cells
   std::vector<int> granges;
@@ -178,6 +194,12 @@ struct bsam_ctx {
   std::string err;
   bool host_only = true;
   hipStream_t stream = nullptr;
+  // second stream on which a level's isolated-patch smoothing runs beside the
+  // launches of its other patches (forked from and joined back into `stream`
+  // by the two events; both streams at the default priority)
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  int iso_side = 1;
   bool building = false, committed = false;
   bool had_commit = false;    // hp/dp/lev/device state describe a committed hierarchy
   bool planned_fuse = false;  // that commit built the fusion and riding plans
@@ -254,6 +276,9 @@ struct bsam_ctx {
   int post_ghost = 1;               // and the next launch's pre updates at a launch's end
   int relax_lds = 0;                // relaxation stencils read through an LDS copy of the tile
   int short_ghost = 1;              // 16-byte device records for the updates that have them
+  // smoothing of isolated patches in one LDS launch on levels where they hold
+  // at least this many cells (0: never)
+  long long iso_min_cells = 65536;
   int tail_top = 0;                 // highest level run by the tail kernels
   TailLevel *d_tail = nullptr;      // levels minlevel..tail_top
   hipGraph_t graph = nullptr, graph_mixed = nullptr;
@@ -351,6 +376,7 @@ void bsam_ctx::free_device() {
     F(l.d_int); F(l.d_ext); F(l.d_coarse); F(l.d_cext); F(l.d_patch_list);
     F(l.d_cgl); F(l.d_blk);
     F(l.d_granges); F(l.d_gcells);
+    F(l.d_iso); F(l.d_iso_stage); F(l.d_iso_rec); F(l.d_relax);
     for (int p = 0; p < 3; p++) {
       FG(l.d_gp[p]);
       F(l.d_tgp[p]);
@@ -715,6 +741,139 @@ static void choose_tiles(bsam_ctx *c) {
   g_tile_cache[key] = {bl, bly};
 }
 
+// The sources an update reads (q-pool offsets of variable 1): fine f0 (the
+// copy source of kind 0), f1, and one, three or four coarse values.
+static int ghost_sources(const GhostOp &g, long long *s) {
+  s[0] = g.f0;
+  if (g.kind == 0) return 1;
+  s[1] = g.f1;
+  const int nc = g.kind == 1 ? 1 : g.kind == 2 ? 3 : 4;
+  for (int k = 0; k < nc; k++) s[2 + k] = g.c[k];
+  return 2 + nc;
+}
+
+// LDS of one workgroup on gfx950, the bound of an isolated patch's arrays.
+constexpr long long kIsoLdsBytes = 160 * 1024;
+
+// Finds the level's isolated patches: no update of the level's three
+// SetGhost programs with its destination in the patch reads another patch of
+// the level, and no update with its destination in another patch reads the
+// patch (sources on the parent level are read, not written, while the level
+// is smoothed). Those whose ghost-extended q and read parent cells fit the
+// LDS of one workgroup in FP64 (both variables) become IsoPatch records with
+// their programs in LDS indices, if together they hold at least
+// iso_min_cells cells. Sets t_relax to the other patches' interior tiles and
+// prog[ip] to the updates of program ip with their destination in those
+// patches (gp[ip] itself when no patch is isolated).
+static void plan_iso(bsam_ctx *c, LevelData &L, std::vector<GhostOp> (&prog)[3]) {
+  L.iso.clear();
+  L.iso_stage.clear();
+  L.iso_rec.clear();
+  L.iso_lds = 0;
+  L.iso_cells = 0;
+  for (long long &u : L.iso_updates) u = 0;
+  L.t_relax = L.t_int;
+  L.relax_cells = L.cells;
+  for (int ip = 0; ip < 3; ip++) prog[ip] = L.gp[ip];
+  const std::vector<int> &P = L.patches;
+  if (c->iso_min_cells <= 0 || !c->fuse_ghost || c->cfg.nrvars != 2 || P.empty()) return;
+  // the level's patches take ascending, contiguous ranges of the q pool
+  auto owner = [&](long long off) -> int {
+    size_t lo = 0, hi = P.size();
+    while (hi - lo > 1) {
+      const size_t mid = (lo + hi) / 2;
+      if (c->hp[P[mid]].qoff <= off) lo = mid;
+      else hi = mid;
+    }
+    const HPatch &h = c->hp[P[lo]];
+    return off >= h.qoff && off < h.qoff + h.qn ? (int)lo : -1;
+  };
+  std::vector<char> coupled(P.size(), 0);
+  long long src[6];
+  for (int ip = 0; ip < 3; ip++)
+    for (const GhostOp &g : L.gp[ip]) {
+      const int a = owner(g.dst);
+      if (a < 0) return;   // not a program of this level: no isolated patches
+      const int n = ghost_sources(g, src);
+      for (int k = 0; k < n; k++) {
+        const int b = owner(src[k]);
+        if (b >= 0 && b != a) coupled[(size_t)a] = coupled[(size_t)b] = 1;
+      }
+    }
+  // per candidate: its updates per program and its staged (parent) sources
+  struct Cand {
+    std::vector<const GhostOp *> ops[3];
+    std::unordered_map<long long, int> stage;
+    std::vector<long long> order;
+  };
+  std::vector<Cand> cand(P.size());
+  for (int ip = 0; ip < 3; ip++)
+    for (const GhostOp &g : L.gp[ip]) {
+      const int a = owner(g.dst);
+      if (coupled[(size_t)a]) continue;
+      Cand &C = cand[(size_t)a];
+      C.ops[ip].push_back(&g);
+      const HPatch &h = c->hp[P[(size_t)a]];
+      const int n = ghost_sources(g, src);
+      for (int k = 0; k < n; k++)
+        if ((src[k] < h.qoff || src[k] >= h.qoff + h.qn) && !C.stage.count(src[k])) {
+          C.stage[src[k]] = (int)C.order.size();
+          C.order.push_back(src[k]);
+        }
+    }
+  std::vector<char> iso(P.size(), 0);
+  long long cells = 0;
+  for (size_t a = 0; a < P.size(); a++) {
+    if (coupled[a]) continue;
+    const HPatch &h = c->hp[P[a]];
+    const long long w = h.qn + (long long)cand[a].order.size();
+    if (w > 65535 || 2 * w * (long long)sizeof(double) > kIsoLdsBytes) continue;
+    iso[a] = 1;
+    cells += (long long)h.mx[0] * h.mx[1] * (c->nd() == 3 ? h.mx[2] : 1);
+  }
+  if (cells < c->iso_min_cells) return;
+  for (size_t a = 0; a < P.size(); a++) {
+    if (!iso[a]) continue;
+    const HPatch &h = c->hp[P[a]];
+    const Cand &C = cand[a];
+    IsoPatch I{};
+    I.patch = P[a];
+    I.ns = (int)C.order.size();
+    I.stage0 = (int)L.iso_stage.size();
+    for (long long o : C.order) L.iso_stage.push_back((int)o);
+    auto lds_index = [&](long long off) -> unsigned short {
+      if (off >= h.qoff && off < h.qoff + h.qn) return (unsigned short)(off - h.qoff);
+      return (unsigned short)(h.qn + C.stage.at(off));
+    };
+    for (int ip = 0; ip < 3; ip++) {
+      I.rec[ip] = (int)L.iso_rec.size();
+      for (const GhostOp *g : C.ops[ip]) {
+        IsoRec r{};
+        r.dst = lds_index(g->dst);
+        r.kind = g->kind;
+        r.nflip = g->nflip;
+        const int n = ghost_sources(*g, src);
+        for (int k = 0; k < n; k++) r.s[k] = lds_index(src[k]);
+        L.iso_rec.push_back(r);
+      }
+      L.iso_updates[ip] += (long long)C.ops[ip].size();
+    }
+    I.rec[3] = (int)L.iso_rec.size();
+    L.iso.push_back(I);
+    L.iso_lds = std::max(L.iso_lds, (size_t)(2 * (h.qn + I.ns) * (long long)sizeof(double)));
+  }
+  L.iso_cells = cells;
+  L.relax_cells = L.cells - cells;
+  L.t_relax.clear();
+  for (const Tile &t : L.t_int)
+    if (!iso[(size_t)owner(c->hp[t.patch].qoff)]) L.t_relax.push_back(t);
+  for (int ip = 0; ip < 3; ip++) {
+    prog[ip].clear();
+    for (const GhostOp &g : L.gp[ip])
+      if (!iso[(size_t)owner(g.dst)]) prog[ip].push_back(g);
+  }
+}
+
 static void commit(bsam_ctx *c) {
   const int nd = c->nd(), nv = c->cfg.nrvars;
   int n = (int)c->hp.size();
@@ -900,12 +1059,30 @@ static void commit(bsam_ctx *c) {
       }
     }
     // the fusion and riding plans serve the V-cycle only; a context that
-    // does not fuse (fuse_ghost 0) goes without them
-    for (int f = 0; f < 3 && c->fuse_ghost; f++) {
-      std::string why;
-      L.fused[f] = plan_fused_ghost(c->hp, L.t_int, nd, c->tx(), c->ty(), c->tz(),
-                                    L.gp[kFusedPass[f]], L.gp[kFusedColor[f]], kFusedColor[f],
-                                    why);
+    // does not fuse (fuse_ghost 0) goes without them. They cover the patches
+    // the smoothing relaxes per colour: all but the isolated ones.
+    std::vector<GhostOp> prog[3];
+    plan_iso(c, L, prog);
+    auto plan_fused = [&]() {
+      for (int f = 0; f < 3; f++) {
+        std::string why;
+        L.fused[f] = c->fuse_ghost && !L.t_relax.empty()
+                         ? plan_fused_ghost(c->hp, L.t_relax, nd, c->tx(), c->ty(), c->tz(),
+                                            prog[kFusedPass[f]], prog[kFusedColor[f]],
+                                            kFusedColor[f], why)
+                         : FusedGhost();
+      }
+    };
+    plan_fused();
+    if (!L.iso.empty() && !L.t_relax.empty() &&
+        !(L.fused[0].ok && L.fused[1].ok && L.fused[2].ok)) {
+      // the smoothing of the other patches must fuse every transition; else
+      // no patch is isolated
+      const long long keep = c->iso_min_cells;
+      c->iso_min_cells = 0;
+      plan_iso(c, L, prog);
+      c->iso_min_cells = keep;
+      plan_fused();
     }
     for (int k = 0; k < 4 && c->fuse_ghost; k++) {
       L.post[k] = PostGhost();
@@ -914,17 +1091,17 @@ static void commit(bsam_ctx *c) {
       const int fn = fused_index(color, 3 - color);
       if (!L.fused[fn].ok || L.fused[fn].pre.empty()) continue;
       std::string why;
-      L.post[k] = plan_post_ghost(c->hp, L.t_int, nd, c->tx(), c->ty(), c->tz(), color,
+      L.post[k] = plan_post_ghost(c->hp, L.t_relax, nd, c->tx(), c->ty(), c->tz(), color,
                                   fb >= 0 && L.fused[fb].ok ? &L.fused[fb] : nullptr,
                                   L.fused[fn].pre, why);
     }
     L.post[kPostFinal] = PostGhost();
-    if (c->fuse_ghost && !L.gp[2].empty()) {
+    if (c->fuse_ghost && !prog[2].empty() && !L.t_relax.empty()) {
       // the last launch of a smoothing relaxes colour 2 after SetGhost(1)
       const int fb = fused_index(1, 2);
       std::string why;
-      L.post[kPostFinal] = plan_post_ghost(c->hp, L.t_int, nd, c->tx(), c->ty(), c->tz(), 2,
-                                           L.fused[fb].ok ? &L.fused[fb] : nullptr, L.gp[2],
+      L.post[kPostFinal] = plan_post_ghost(c->hp, L.t_relax, nd, c->tx(), c->ty(), c->tz(), 2,
+                                           L.fused[fb].ok ? &L.fused[fb] : nullptr, prog[2],
                                            why);
     }
     if (l > minl) {
@@ -971,7 +1148,7 @@ static void commit(bsam_ctx *c) {
   }
   zero_pools(c);
   c->d_P = dev_upload(c->dp, s);
-  size_t maxt = 1;
+  size_t maxt = 1, iso_lds = 0;
   for (auto &L : c->lev) {
     L.d_int = dev_upload(L.t_int, s);
     L.d_ext = dev_upload(L.t_ext, s);
@@ -998,7 +1175,22 @@ static void commit(bsam_ctx *c) {
     };
     up_ride(L.ride_cgl, L.d_ride_cgl);
     up_ride(L.ride_corr, L.d_ride_corr);
+    L.d_relax = dev_upload(L.t_relax, s);
+    if (!L.iso.empty()) {
+      L.d_iso = dev_upload(L.iso, s);
+      L.d_iso_stage = dev_upload(L.iso_stage, s);
+      L.d_iso_rec = dev_upload(L.iso_rec, s);
+      iso_lds = std::max(iso_lds, L.iso_lds);
+    }
     maxt = std::max(maxt, L.t_int.size());
+  }
+  if (iso_lds > 0) {
+    HIPCK(iso_allow_lds(iso_lds));
+    if (!c->side) {
+      HIPCK(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+      HIPCK(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+      HIPCK(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
+    }
   }
   // tail: the longest run of single-patch levels from minlevel whose cell
   // counts stay within tail_cells (levels <= 0 hold one patch each) and whose
@@ -1088,15 +1280,20 @@ static void op_apply_pending(bsam_ctx *c, int level, int pending, int prec) {
   if (pending >= kRestOnly) op_set_ghost_rest(c, level, prec);
   else if (pending >= 0) op_set_ghost(c, level, pending, prec);
 }
-static void op_relax(bsam_ctx *c, int level, int color, int prec = PREC_SOLUTION_FP64) {
+// RelaxPatch of colour `color` over the level (with coupled_only, over the
+// patches that are not isolated).
+static void op_relax(bsam_ctx *c, int level, int color, int prec = PREC_SOLUTION_FP64,
+                     bool coupled_only = false) {
   LevelData &L = c->L(level);
-  ProfScope ps(c, BSAM_PROF_RELAX, (prec == PREC_SOLUTION_FP64 ? 24.0 : 12.0) * L.cells);
+  const Tile *tiles = coupled_only ? L.d_relax : L.d_int;
+  const int nt = (int)(coupled_only ? L.t_relax.size() : L.t_int.size());
+  ProfScope ps(c, BSAM_PROF_RELAX, (prec == PREC_SOLUTION_FP64 ? 24.0 : 12.0) *
+                                       (double)(coupled_only ? L.relax_cells : L.cells));
   if (c->relax_lds)   // the staged form of the same sweep, with no ghost updates
-    HIPCK(launch_relax_fused(c->nd(), prec, c->dev(), L.d_int, (int)L.t_int.size(), color,
-                             GhostList{}, GhostList{}, true, c->stream));
+    HIPCK(launch_relax_fused(c->nd(), prec, c->dev(), tiles, nt, color, GhostList{},
+                             GhostList{}, true, c->stream));
   else
-    HIPCK(launch_relax(c->nd(), prec, c->dev(), L.d_int, (int)L.t_int.size(), color,
-                       c->stream));
+    HIPCK(launch_relax(c->nd(), prec, c->dev(), tiles, nt, color, c->stream));
 }
 static LevelData &child_level(bsam_ctx *c, int level) {
   if (level <= c->cfg.minlevel) throw Error("level has no parent level");
@@ -1230,7 +1427,34 @@ static int level_relax(bsam_ctx *c, int level, int prec = PREC_SOLUTION_FP64,
   LevelData &L = c->L(level);
   int pending = ghost0_first ? 0 : -1;   // SetGhost program not yet applied
   int moved = -1;   // post plan the previous launch ran: its rest replaces this pre
-  const double relax_bytes = (prec == PREC_SOLUTION_FP64 ? 24.0 : 12.0) * L.cells;
+  const double sz = prec == PREC_SOLUTION_FP64 ? 1.0 : 0.5;
+  // The isolated patches' whole smoothing, SetGhost(2) after it included, in
+  // one launch; the launches below relax the other patches, whose plans
+  // leave the isolated patches' updates out (they need the fused plans, so
+  // without fusion every patch takes the per-colour launches).
+  const bool iso = !L.iso.empty() && c->fuse_ghost;
+  // with other patches to relax, the isolated smoothing runs on the side
+  // stream beside their launches (not while it is being timed)
+  const bool side = iso && c->iso_side && c->side && !L.t_relax.empty() &&
+                    !(c->prof_mask & (1u << BSAM_PROF_SMOOTH_ISO));
+  if (iso) {
+    const int np = c->cfg.nsmoothingpasses;
+    const long long ups = (ghost0_first ? L.iso_updates[0] : 0) + (np - 1) * L.iso_updates[2] +
+                          np * L.iso_updates[1] + L.iso_updates[2];
+    if (side) {
+      HIPCK(hipEventRecord(c->ev_fork, c->stream));
+      HIPCK(hipStreamWaitEvent(c->side, c->ev_fork, 0));
+    }
+    ProfScope ps(c, BSAM_PROF_SMOOTH_ISO, sz * (48.0 * np * L.iso_cells + 32.0 * ups));
+    HIPCK(launch_smooth_iso(c->nd(), prec, c->dev(), L.d_iso, (int)L.iso.size(),
+                            L.d_iso_stage, L.d_iso_rec, np, ghost0_first ? 1 : 0, 1,
+                            prec == PREC_SOLUTION_FP64 ? L.iso_lds : L.iso_lds / 2,
+                            side ? c->side : c->stream));
+    if (L.t_relax.empty()) return -1;
+  }
+  const Tile *tiles = iso ? L.d_relax : L.d_int;
+  const int nt = (int)(iso ? L.t_relax.size() : L.t_int.size());
+  const double relax_bytes = 24.0 * sz * (double)(iso ? L.relax_cells : L.cells);
   for (int pass = 1; pass <= c->cfg.nsmoothingpasses; pass++)
     for (int color = 1; color <= 2; color++) {
       const bool last = pass == c->cfg.nsmoothingpasses && color == 2;
@@ -1256,23 +1480,32 @@ static int level_relax(bsam_ctx *c, int level, int prec = PREC_SOLUTION_FP64,
         const double nops = (double)L.fused[f].ops.size() + (k >= 0 ? L.post[k].ops.size() : 0);
         ProfScope ps(c, BSAM_PROF_RELAX_FUSED,
                      relax_bytes + (prec == PREC_SOLUTION_FP64 ? 32.0 : 16.0) * nops);
-        HIPCK(launch_relax_fused(c->nd(), prec, c->dev(), L.d_int, (int)L.t_int.size(), color,
+        HIPCK(launch_relax_fused(c->nd(), prec, c->dev(), tiles, nt, color,
                                  L.d_fops[f].view(), pop, c->relax_lds != 0, c->stream));
       } else {
+        // with isolated patches only the first launch of pre-smoothing comes
+        // here (no SetGhost pending): every transition after it is fused
+        if (iso && pending >= 0) throw Error("isolated patches need every transition fused");
         if (pending >= 0) op_set_ghost(c, level, pending, prec);
         if (k >= 0) {
           ProfScope ps(c, BSAM_PROF_RELAX_FUSED,
                        relax_bytes +
                            (prec == PREC_SOLUTION_FP64 ? 32.0 : 16.0) * L.post[k].ops.size());
-          HIPCK(launch_relax_fused(c->nd(), prec, c->dev(), L.d_int, (int)L.t_int.size(), color,
-                                   GhostList{}, pop, c->relax_lds != 0, c->stream));
+          HIPCK(launch_relax_fused(c->nd(), prec, c->dev(), tiles, nt, color, GhostList{}, pop,
+                                   c->relax_lds != 0, c->stream));
         } else {
-          op_relax(c, level, color, prec);
+          op_relax(c, level, color, prec, iso);
         }
       }
       moved = k;
       pending = color;
     }
+  if (side) {
+    // the side stream joins back before anything after the smoothing (the
+    // SetGhost(2) below or the caller's may rewrite isolated patches' ghosts)
+    HIPCK(hipEventRecord(c->ev_join, c->side));
+    HIPCK(hipStreamWaitEvent(c->stream, c->ev_join, 0));
+  }
   if (moved == kPostFinal) pending = kRestOnly + 2;   // SetGhost(2) all but its rest
   if (defer_last) return pending;
   op_apply_pending(c, level, pending, prec);
@@ -1572,8 +1805,12 @@ int bsam_ctx_destroy(bsam_ctx *c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     DeviceLock lock(g_device_lock);
     c->free_device();
+    if (c->side) (void)hipStreamSynchronize(c->side);
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+    if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+    if (c->side) (void)hipStreamDestroy(c->side);
     if (c->stream) (void)hipStreamDestroy(c->stream);
-    c->stream = nullptr;
+    c->stream = c->side = nullptr;
   }
   delete c;
   return 0;
@@ -2426,6 +2663,14 @@ int bsam_tune(bsam_ctx *c, const char *key, double value) {
   } else if (k == "short_ghost") {
     if (c->committed || c->building) throw Error("short_ghost must be set before the hierarchy");
     c->short_ghost = value != 0.0;
+  } else if (k == "iso_side") {
+    c->iso_side = value != 0.0;
+    c->drop_graphs();
+  } else if (k == "iso_min_cells") {
+    if (c->committed || c->building)
+      throw Error("iso_min_cells must be set before the hierarchy");
+    if (value < 0) throw Error("iso_min_cells must be >= 0");
+    c->iso_min_cells = (long long)value;
   } else {
     throw Error("unknown tuning key " + k);
   }
@@ -2447,6 +2692,14 @@ int bsam_tune_get(const bsam_ctx *cc, const char *key, double *value) {
   else if (k == "post_ghost") *value = c->post_ghost;
   else if (k == "relax_lds") *value = c->relax_lds;
   else if (k == "short_ghost") *value = c->short_ghost;
+  else if (k == "iso_min_cells") *value = (double)c->iso_min_cells;
+  else if (k == "iso_side") *value = c->iso_side;
+  else if (k == "iso_patches" || k == "iso_cells") {
+    double n = 0;
+    for (const LevelData &L : c->lev)
+      n += k == "iso_patches" ? (double)L.iso.size() : (double)L.iso_cells;
+    *value = n;
+  }
   else if (k == "tile_x") *value = c->tx();
   else if (k == "tile_y") *value = c->ty();
   else if (k == "tile_z") *value = c->tz();
@@ -2685,6 +2938,52 @@ int bsam_post_ghost_apply_host(const bsam_ctx *cc, int level, int pending, int c
   if (k < 0 || !c->L(level).post[k].ok) throw Error("this launch has no post plan");
   const PostGhost &g = c->L(level).post[k];
   apply_ghost_program_host(c->hp, part % 2 == 0 ? g.ops : g.rest, c->cfg.nrvars, q_patches);
+  API_END(c)
+}
+
+int bsam_iso_patches(const bsam_ctx *cc, int level, int *out, int maxn) {
+  bsam_ctx *c = const_cast<bsam_ctx *>(cc);
+  if (!c || !c->committed) return -1;
+  if (level < c->cfg.minlevel || level > c->finest) return -1;
+  const LevelData &L = c->L(level);
+  for (int n = 0; n < (int)L.iso.size() && n < maxn && out; n++) out[n] = L.iso[(size_t)n].patch;
+  return (int)L.iso.size();
+}
+
+int bsam_iso_program_apply_host(const bsam_ctx *cc, int level, int ipass,
+                                double *const *q_patches) {
+  bsam_ctx *c = const_cast<bsam_ctx *>(cc);
+  API_BEGIN(c)
+  if (!c->committed) throw Error("hierarchy not committed");
+  if (level < c->cfg.minlevel || level > c->finest) throw Error("level out of range");
+  if (ipass < 0 || ipass > 2) throw Error("bad ipass");
+  // the records read back into q-pool updates through the same LDS index
+  // map the smoothing kernel uses (x < qn: the patch's cell x, else staged
+  // parent cell x - qn)
+  const LevelData &L = c->L(level);
+  std::vector<GhostOp> ops;
+  for (const IsoPatch &I : L.iso) {
+    const HPatch &h = c->hp[(size_t)I.patch];
+    auto off = [&](unsigned short x) -> int {
+      return x < h.qn ? (int)(h.qoff + x) : L.iso_stage[(size_t)(I.stage0 + (x - h.qn))];
+    };
+    for (int r = I.rec[ipass]; r < I.rec[ipass + 1]; r++) {
+      const IsoRec &e = L.iso_rec[(size_t)r];
+      GhostOp g{};
+      g.dst = off(e.dst);
+      g.kind = e.kind;
+      g.nflip = e.nflip;
+      g.group = (short)h.group;
+      g.f0 = off(e.s[0]);
+      if (e.kind != 0) {
+        g.f1 = off(e.s[1]);
+        const int nc = e.kind == 1 ? 1 : e.kind == 2 ? 3 : 4;
+        for (int k = 0; k < nc; k++) g.c[k] = off(e.s[2 + k]);
+      }
+      ops.push_back(g);
+    }
+  }
+  apply_ghost_program_host(c->hp, ops, c->cfg.nrvars, q_patches);
   API_END(c)
 }
 

@@ -254,50 +254,57 @@ __device__ __forceinline__ T restrict_f(const T *x, const DPatch &P, int ci, int
 // kernels below map one tile of cells to a workgroup, and the tail kernels
 // loop the same functions over whole single-patch levels in one workgroup.
 
-// One ghost-cell update of a SetGhost program, offsets relative to q and
-// variables vs apart. A
+// The value of one ghost-cell update of a SetGhost program: kind 0 copies
+// source 0; kinds 1-3 are the quadratic coarse-fine interpolations of
+// Interpolate2D/3D over two fine sources (0, 1) and one, three or four
+// coarse sources (2..5), with the weights of lib/boundary.f90:45-54. A
 // boundary "2 - x" map is affine; in the defect form only its linear part,
-// -x, applies.
+// -x, applies (nflip times). src(n) loads source n; every caller (global
+// pools, short records, an isolated patch's LDS) evaluates the same
+// expression tree.
+template <class T, bool DEF, class Src>
+__device__ __forceinline__ T ghost_value(int kind, int nflip, Src src) {
+  T val;
+  switch (kind) {
+    case 0:
+      val = src(0);
+      break;
+    case 1:
+      val = (T)BSAM_A1 * src(0) + (T)BSAM_A2 * src(1) + (T)BSAM_A3 * src(2);
+      break;
+    case 2:
+      val = (T)BSAM_A1 * src(0) + (T)BSAM_A2 * src(1) +
+            (T)BSAM_A3 * ((T)BSAM_B1 * src(2) + (T)BSAM_B2 * src(3) + (T)BSAM_B3 * src(4));
+      break;
+    default:
+      val = (T)BSAM_A1 * src(0) + (T)BSAM_A2 * src(1) +
+            (T)BSAM_A3 * ((T)BSAM_C1 * src(2) + (T)BSAM_C2 * src(3) + (T)BSAM_C3 * src(4) +
+                          (T)BSAM_C4 * src(5));
+      break;
+  }
+  for (int r = 0; r < nflip; r++) val = DEF ? -val : (T)2 - val;
+  return val;
+}
+// One ghost-cell update of a SetGhost program, offsets relative to q and
+// variables vs apart.
 template <class T, bool DEF>
 __device__ __forceinline__ void ghost_op(int nrvars, T *q, long long vs, const GhostOp &g) {
   for (int v = 0; v < nrvars; v++) {
     const T *f = q + (long long)v * vs;
-    const T *c = f;
-    T val;
-    switch (g.kind) {
-      case 0:
-        val = f[g.f0];
-        break;
-      case 1:
-        val = (T)BSAM_A1 * f[g.f0] + (T)BSAM_A2 * f[g.f1] + (T)BSAM_A3 * c[g.c[0]];
-        break;
-      case 2:
-        val = (T)BSAM_A1 * f[g.f0] + (T)BSAM_A2 * f[g.f1] +
-              (T)BSAM_A3 * ((T)BSAM_B1 * c[g.c[0]] + (T)BSAM_B2 * c[g.c[1]] +
-                            (T)BSAM_B3 * c[g.c[2]]);
-        break;
-      default:
-        val = (T)BSAM_A1 * f[g.f0] + (T)BSAM_A2 * f[g.f1] +
-              (T)BSAM_A3 * ((T)BSAM_C1 * c[g.c[0]] + (T)BSAM_C2 * c[g.c[1]] +
-                            (T)BSAM_C3 * c[g.c[2]] + (T)BSAM_C4 * c[g.c[3]]);
-        break;
-    }
-    for (int r = 0; r < g.nflip; r++) val = DEF ? -val : (T)2 - val;
+    const T val = ghost_value<T, DEF>(g.kind, g.nflip, [&](int n) {
+      return f[n == 0 ? g.f0 : n == 1 ? g.f1 : g.c[n - 2]];
+    });
     q[g.dst + (long long)v * vs] = val;
   }
 }
-// The same update from its short record (kinds 0 and 1, ghost_op's
-// expressions).
+// The same update from its short record (kinds 0 and 1).
 template <class T, bool DEF>
 __device__ __forceinline__ void ghost_op_s(int nrvars, T *q, long long vs, const GhostS &g) {
   for (int v = 0; v < nrvars; v++) {
     const T *f = q + (long long)v * vs;
-    T val;
-    if (g.kind == 0)
-      val = f[g.src];
-    else
-      val = (T)BSAM_A1 * f[g.dst + g.d0] + (T)BSAM_A2 * f[g.dst + g.d1] + (T)BSAM_A3 * f[g.src];
-    for (int r = 0; r < g.nflip; r++) val = DEF ? -val : (T)2 - val;
+    const T val = ghost_value<T, DEF>(g.kind, g.nflip, [&](int n) {
+      return f[g.kind == 0 || n == 2 ? g.src : g.dst + (n == 0 ? g.d0 : g.d1)];
+    });
     q[g.dst + (long long)v * vs] = val;
   }
 }
@@ -824,6 +831,159 @@ __global__ __launch_bounds__(RELAX_THREADS) void k_relax_fused(Dev d, const Tile
     __syncthreads();
     ghost_tile<T>(d, post, b);
   }
+}
+
+// The whole smoothing of an isolated patch (IsoPatch) in one workgroup: the
+// patch's ghost-extended q and the parent cells its SetGhost programs read
+// are copied into LDS; then, as level_relax orders them, per pass colour 1
+// and colour 2 are relaxed, each after the pending program (SetGhost(0)
+// first when ghost0_first, then the colour just relaxed), and SetGhost(2)
+// after the last sweep when final2; the array goes back to the pool. No
+// other patch of the level reads or writes a cell of this patch, and the
+// parent level does not change while the level is smoothed, so the result
+// equals the level-wide sequence bit for bit. One workgroup per patch.
+extern __shared__ __align__(16) unsigned char iso_lds[];
+constexpr int ISO_THREADS = 1024;
+constexpr int ISO_LOAD = 8;   // array elements each thread has in flight when staging
+constexpr int ISO_FREG = 2;   // cells per colour whose f each thread keeps in registers
+constexpr int ISO_RREG = 2;   // updates of the next program each thread prefetches
+template <int ND, class T, bool DEF>
+__global__ __launch_bounds__(ISO_THREADS) void k_smooth_iso(Dev d, const IsoPatch *iso,
+                                                             const int *stage, const IsoRec *rec,
+                                                             int npasses, int ghost0_first,
+                                                             int final2) {
+  const IsoPatch &I = iso[blockIdx.x];   // read in place (scalar loads)
+  if (stopped_patch(d, I.patch)) return;
+  const DPatch &P = d.P[I.patch];
+  const Fld<T> F = fields<T>(d);
+  T *box = (T *)iso_lds;
+  const int qn = (int)P.qn, W = qn + I.ns, tid = threadIdx.x, nth = blockDim.x;
+  T *gq = F.q + P.q;
+  const T *f = F.f + P.f;
+  // the colour's cells: slot sl of row r = sl / hx is the t-th cell of the
+  // colour's parity in that row (i + j + k + color odd, tile_color_cell)
+  const int hx = (P.mx + 1) >> 1, rows = P.my * (ND == 3 ? P.mz : 1), slots = hx * rows;
+  auto cell = [&](int sl, int color, int &i, int &j, int &k) __attribute__((always_inline)) {
+    const int r = sl / hx, t = sl - r * hx;
+    j = 1 + (ND == 3 ? r % P.my : r);
+    k = ND == 3 ? 1 + r / P.my : 0;
+    i = 2 * t + 2 - ((j + k + color + 1) & 1);
+    return i <= P.mx;
+  };
+  // every load of the staging and of the first cells' f in flight at once
+  T fr[2][ISO_FREG][2];
+#pragma unroll
+  for (int c = 0; c < 2; c++)
+#pragma unroll
+    for (int u = 0; u < ISO_FREG; u++) {
+      int i, j, k;
+      const int sl = tid + u * nth;
+      fr[c][u][0] = fr[c][u][1] = (T)0;
+      if (sl < slots && cell(sl, c + 1, i, j, k)) {
+        const long long fo = fat(P, i, j, ND == 3 ? k : 1);
+        fr[c][u][0] = f[fo];
+        fr[c][u][1] = f[fo + P.fvs];
+      }
+    }
+  for (int e0 = 0; e0 < W; e0 += ISO_LOAD * nth) {
+    T v[ISO_LOAD][2];
+#pragma unroll
+    for (int u = 0; u < ISO_LOAD; u++) {
+      const int e = e0 + u * nth + tid;
+      if (e < qn) {
+        v[u][0] = gq[e];
+        v[u][1] = gq[e + d.qvs];
+      } else if (e < W) {
+        const long long o = stage[I.stage0 + e - qn];
+        v[u][0] = F.q[o];
+        v[u][1] = F.q[o + d.qvs];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < ISO_LOAD; u++) {
+      const int e = e0 + u * nth + tid;
+      if (e < W) {
+        box[e] = v[u][0];
+        box[W + e] = v[u][1];
+      }
+    }
+  }
+  __syncthreads();
+  // a program's first ISO_RREG updates per thread are fetched before the
+  // relaxation that precedes it
+  // (records as four dwords: dst | kind << 16 | nflip << 24, then the
+  // sources two to a dword, low half first)
+  const uint4 *rw = reinterpret_cast<const uint4 *>(rec);
+  uint4 pre[ISO_RREG];
+  auto fetch = [&](int p) __attribute__((always_inline)) {
+#pragma unroll
+    for (int u = 0; u < ISO_RREG; u++) {
+      const int r = I.rec[p] + u * nth + tid;
+      if (r < I.rec[p + 1]) pre[u] = rw[r];
+    }
+  };
+  auto apply = [&](const uint4 g) __attribute__((always_inline)) {
+    const unsigned dst = g.x & 0xffffu;
+    const int kind = (g.x >> 16) & 0xff, nflip = g.x >> 24;
+    const unsigned w[3] = {g.y, g.z, g.w};
+#pragma unroll
+    for (int v = 0; v < 2; v++) {
+      const T *b = box + v * W;
+      box[v * W + dst] = ghost_value<T, DEF>(
+          kind, nflip, [&](int n) { return b[(w[n >> 1] >> (16 * (n & 1))) & 0xffffu]; });
+    }
+  };
+  auto program = [&](int p) __attribute__((always_inline)) {
+#pragma unroll
+    for (int u = 0; u < ISO_RREG; u++)
+      if (I.rec[p] + u * nth + tid < I.rec[p + 1]) apply(pre[u]);
+    for (int r = I.rec[p] + ISO_RREG * nth + tid; r < I.rec[p + 1]; r += nth) apply(rw[r]);
+    __syncthreads();
+  };
+  const View<T> a = view_of<T>(box, P.px, P.py, W);
+  auto relax_color = [&](const T (&fc)[ISO_FREG][2], int color) {
+    auto one = [&](int sl, T f1, T f2) __attribute__((always_inline)) {
+      int i, j, k;
+      if (!cell(sl, color, i, j, k)) return;
+      const long long c = a.at(i, j, k);
+      relax_core<ND, T, DEF>(d.prm, P, a, i, j, k, box + c, box + W + c,
+                             DEF ? F.qb[P.q + c] : (T)0, f1, f2);
+    };
+#pragma unroll
+    for (int u = 0; u < ISO_FREG; u++)
+      if (tid + u * nth < slots) one(tid + u * nth, fc[u][0], fc[u][1]);
+    for (int sl = tid + ISO_FREG * nth; sl < slots; sl += nth) {
+      int i, j, k;
+      if (!cell(sl, color, i, j, k)) continue;
+      const long long fo = fat(P, i, j, ND == 3 ? k : 1);
+      one(sl, f[fo], f[fo + P.fvs]);
+    }
+    __syncthreads();
+  };
+  // level_relax's order: per pass colour 1 then 2, each after the pending
+  // program; SetGhost(2) after the last sweep
+  int pending = ghost0_first ? 0 : -1;
+  if (pending >= 0) fetch(pending);
+  for (int pass = 0; pass < npasses; pass++) {
+    if (pending >= 0) program(pending);
+    fetch(1);
+    relax_color(fr[0], 1);
+    program(1);
+    if (pass + 1 < npasses || final2) fetch(2);
+    relax_color(fr[1], 2);
+    pending = 2;
+    if (pass + 1 < npasses) continue;
+    if (final2) program(2);
+  }
+  for (int e0 = 0; e0 < qn; e0 += ISO_LOAD * nth)
+#pragma unroll
+    for (int u = 0; u < ISO_LOAD; u++) {
+      const int e = e0 + u * nth + tid;
+      if (e < qn) {
+        gq[e] = box[e];
+        gq[e + d.qvs] = box[W + e];
+      }
+    }
 }
 
 // One SetGhost(level, ipass) program: every ghost-cell update is independent.
@@ -1946,6 +2106,40 @@ hipError_t launch_tail_filldown(int nd, const Dev &d, const TailLevel *lv, int n
   if (nd == 2) k_tail_filldown<2><<<1, threads, 0, s>>>(d, lv, nlev, field);
   else k_tail_filldown<3><<<1, threads, 0, s>>>(d, lv, nlev, field);
   return hipGetLastError();
+}
+
+hipError_t launch_smooth_iso(int nd, int prec, const Dev &d, const IsoPatch *iso, int n,
+                             const int *stage, const IsoRec *rec, int npasses, int ghost0_first,
+                             int final2, size_t lds, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const dim3 grid((unsigned)n);
+  if (prec == PREC_SOLUTION_FP64) {
+    if (nd == 2)
+      k_smooth_iso<2, double, false><<<grid, ISO_THREADS, lds, s>>>(d, iso, stage, rec, npasses,
+                                                                    ghost0_first, final2);
+    else
+      k_smooth_iso<3, double, false><<<grid, ISO_THREADS, lds, s>>>(d, iso, stage, rec, npasses,
+                                                                    ghost0_first, final2);
+  } else {
+    if (nd == 2)
+      k_smooth_iso<2, float, true><<<grid, ISO_THREADS, lds, s>>>(d, iso, stage, rec, npasses,
+                                                                  ghost0_first, final2);
+    else
+      k_smooth_iso<3, float, true><<<grid, ISO_THREADS, lds, s>>>(d, iso, stage, rec, npasses,
+                                                                  ghost0_first, final2);
+  }
+  return hipGetLastError();
+}
+
+hipError_t iso_allow_lds(size_t lds) {
+  const void *fns[] = {
+      (const void *)k_smooth_iso<2, double, false>, (const void *)k_smooth_iso<3, double, false>,
+      (const void *)k_smooth_iso<2, float, true>, (const void *)k_smooth_iso<3, float, true>};
+  for (const void *f : fns) {
+    hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 hipError_t tail_allow_lds(size_t lds) {

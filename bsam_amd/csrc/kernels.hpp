@@ -16,6 +16,16 @@ hipError_t launch_relax(int nd, int prec, const Dev &d, const Tile *tiles, int n
 hipError_t launch_relax_fused(int nd, int prec, const Dev &d, const Tile *tiles, int nt,
                               int color, const GhostList &pre, const GhostList &post,
                               bool stage, hipStream_t s);
+// The whole smoothing of each isolated patch iso[0..n) (IsoPatch), one
+// workgroup per patch with `lds` bytes of dynamic LDS (2 x (qn + ns) values
+// of the largest): npasses passes of colour 1 then 2, each colour after the
+// pending SetGhost program (SetGhost(0) before the first when ghost0_first),
+// and SetGhost(2) after the last when final2.
+hipError_t launch_smooth_iso(int nd, int prec, const Dev &d, const IsoPatch *iso, int n,
+                             const int *stage, const IsoRec *rec, int npasses, int ghost0_first,
+                             int final2, size_t lds, hipStream_t s);
+// Raises the dynamic-LDS limit of the isolated-patch smoothing to `lds` bytes.
+hipError_t iso_allow_lds(size_t lds);
 hipError_t launch_ghost(int prec, const Dev &d, const GhostList &ops, hipStream_t s);
 // Restriction / fill-down of one level; gops is a SetGhost program of that
 // level applied in the same launch (grid-stride; its reads must be disjoint

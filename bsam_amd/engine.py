@@ -189,6 +189,9 @@ def lib():
         L.bsam_post_ghost_size.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int]
         L.bsam_post_ghost_apply_host.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int,
                                                  C.POINTER(C.POINTER(C.c_double))]
+        L.bsam_iso_patches.argtypes = [vp, C.c_int, ip, C.c_int]
+        L.bsam_iso_program_apply_host.argtypes = [vp, C.c_int, C.c_int,
+                                                  C.POINTER(C.POINTER(C.c_double))]
         _lib = L
     return _lib
 
@@ -411,7 +414,8 @@ class Engine:
 
     # -- kernel timing ----------------------------------------------------------
     PROF_KINDS = ("relax", "ghost", "restrict", "coarse_ghost", "coarse_load", "correct",
-                  "filldown", "source", "resnorm", "tail", "relax_fused", "copy", "defect")
+                  "filldown", "source", "resnorm", "tail", "relax_fused", "copy", "defect",
+                  "smooth_iso")
 
     def profile_begin(self, kinds=("relax",)):
         mask = 0
@@ -484,6 +488,25 @@ class Engine:
             assert a.flags.f_contiguous and a.dtype == np.float64
             ptrs[n] = _dptr(a)
         self._ck(lib().bsam_post_ghost_apply_host(self.ctx, level, pending, color, part, ptrs))
+
+    def iso_patches(self, level: int) -> List[int]:
+        """Patch indices of the level's isolated patches (their smoothing runs
+        in one launch out of LDS)."""
+        n = int(lib().bsam_iso_patches(self.ctx, level, None, 0))
+        if n < 0:
+            raise BsamError("bad level")
+        out = (C.c_int * max(n, 1))()
+        lib().bsam_iso_patches(self.ctx, level, out, n)
+        return list(out[:n])
+
+    def iso_program_apply_host(self, level: int, ipass: int, qs: List[np.ndarray]):
+        """SetGhost(level, ipass) of the isolated patches, from the records the
+        smoothing kernel reads, applied to host q arrays."""
+        ptrs = (C.POINTER(C.c_double) * len(qs))()
+        for n, a in enumerate(qs):
+            assert a.flags.f_contiguous and a.dtype == np.float64
+            ptrs[n] = _dptr(a)
+        self._ck(lib().bsam_iso_program_apply_host(self.ctx, level, ipass, ptrs))
 
     def ghost_apply_host(self, level: int, ipass: int, qs: List[np.ndarray]):
         """Apply SetGhost(level, ipass)'s program to host q arrays in place."""

@@ -329,7 +329,13 @@ int bsam_regrid_level(const bsam_regrid_params *rp, int npatch, const int *mx,
  *                  plans (contexts that never run a V-cycle).
  *   "tail_lds"     bytes of LDS the tail kernel may hold its levels' arrays
  *                  in (-1, default: the device limit; 0: none); before the
- *                  hierarchy. */
+ *                  hierarchy.
+ *   "iso_min_cells" a level smooths its isolated patches (whose SetGhost
+ *                  programs read no other patch of the level and which no
+ *                  other patch's programs read) in one launch per smoothing,
+ *                  each patch in one workgroup out of LDS, when those that
+ *                  fit hold at least this many cells (default 65536; 0:
+ *                  never). Needs fuse_ghost. Before the hierarchy. */
 int bsam_tune(bsam_ctx *ctx, const char *key, double value);
 /* Reads a tuning value back, or a property of the committed schedule:
  * "tail_top" (highest level run by the tail kernel; below minlevel when
@@ -337,8 +343,9 @@ int bsam_tune(bsam_ctx *ctx, const char *key, double value);
  * kernels), "commits_reused" (commits that found the hierarchy already
  * committed), "tile_z", and of the last commit's tile choice
  * "tile_auto_candidates" (shapes timed; 0 when taken from the cache) and
- * "tile_auto_us" (the chosen shape's relaxation rounds). No counterpart in
- * the reference. */
+ * "tile_auto_us" (the chosen shape's relaxation rounds), "iso_patches" and
+ * "iso_cells" (isolated patches smoothed out of LDS, over all levels). No
+ * counterpart in the reference. */
 int bsam_tune_get(const bsam_ctx *ctx, const char *key, double *value);
 
 /* Kernel timing. Between bsam_profile_begin and bsam_profile_end, every
@@ -366,6 +373,10 @@ enum {
   BSAM_PROF_DEFECT,       /* mixed precision: defect set-up (q, f read, qb, e, fe
                              written: 52 B per ghost-extended cell) and q += e
                              (40 B per interior cell) */
+  BSAM_PROF_SMOOTH_ISO,   /* whole smoothings of isolated patches (one launch
+                             per smoothing of a level): RELAX's 24 B per cell
+                             for each half-sweep and GHOST's 32 B per update
+                             applied, for every sweep and program it runs */
   BSAM_PROF_NKINDS
 };
 int bsam_profile_begin(bsam_ctx *ctx, unsigned kind_mask);
@@ -426,6 +437,15 @@ int bsam_fused_ghost_part_apply_host(const bsam_ctx *ctx, int level, int ipass, 
 int bsam_post_ghost_size(const bsam_ctx *ctx, int level, int pending, int color, int part);
 int bsam_post_ghost_apply_host(const bsam_ctx *ctx, int level, int pending, int color, int part,
                                double *const *q_patches);
+/* Isolated patches of a level (see "iso_min_cells"): their number (-1 for
+ * bad arguments) and up to maxn of their patch indices; and the application
+ * of SetGhost(level, ipass) to them on the host, decoded from the records the
+ * smoothing kernel reads, for the CPU tests. The level's other patches'
+ * updates are those of the fused and post plans. No counterpart in the
+ * reference. */
+int bsam_iso_patches(const bsam_ctx *ctx, int level, int *out, int maxn);
+int bsam_iso_program_apply_host(const bsam_ctx *ctx, int level, int ipass,
+                                double *const *q_patches);
 /* Apply a ghost-fill program on the host to per-patch q arrays (reference
  * layout, one pointer per patch): used to check the integer maps on a CPU. */
 int bsam_ghost_program_apply_host(const bsam_ctx *ctx, int level, int ipass,

@@ -12,6 +12,7 @@ import pytest
 import orc
 from bsam_amd.engine import Engine
 from bsam_amd.layout import Layout
+from bsam_amd.workloads import nested_block_layout_3d
 from helpers import (layout_amr2d, layout_amr2d_tiles, layout_amr3d, layout_periodic2d,
                      layout_periodic3d, layout_single, noise_q, random_fields)
 
@@ -29,6 +30,8 @@ LAYOUTS = {
     "periodic2d_xy": lambda: layout_periodic2d((2, 2, 2, 2)),
     "periodic2d_x_user_y": lambda: layout_periodic2d((2, 2, 10, 10)),
     "periodic3d_xyz": layout_periodic3d,
+    # 8^3 and 16^3 blocks at random places: some touch a sibling, some not
+    "blocks3d": lambda: nested_block_layout_3d(32, 2, (0.25, 0.25), seed=5, sides=(8, 16)),
 }
 
 
@@ -40,7 +43,10 @@ LAYOUTS = {
 # fills the 256 threads); relaxation stencils read through an LDS copy of the
 # tile and its halo (fused, unfused, and with another tile shape); no ghost
 # updates moved to the end of the previous launch; every ghost update in the
-# 32-byte device record
+# 32-byte device record; isolated patches smoothed out of LDS on every level
+# that has one (with the launches of the other patches fused, moved or
+# staged, with every single-patch coarse level run by level launches, and on
+# the engine's stream instead of beside the other patches' launches)
 SCHEDULES = {
     "launch": {"tail_cells": 0, "graph": 0},
     "default": {},
@@ -54,6 +60,11 @@ SCHEDULES = {
     "staged_64x4": {"relax_lds": 1, "tile_x": 64, "tile_y": 4},
     "no_post": {"post_ghost": 0},
     "long_records": {"short_ghost": 0},
+    "iso": {"iso_min_cells": 1},
+    "iso_launch": {"iso_min_cells": 1, "tail_cells": 0, "graph": 0},
+    "iso_no_post": {"iso_min_cells": 1, "post_ghost": 0},
+    "iso_staged": {"iso_min_cells": 1, "relax_lds": 1},
+    "iso_one_stream": {"iso_min_cells": 1, "iso_side": 0},
 }
 
 
