@@ -139,7 +139,8 @@ struct GhostList {
 struct IsoPatch {
   int patch;
   int ns, stage0;
-  int rec[4];
+  int rec[4];    // 16-byte records (IsoRec)
+  int srec[4];   // 8-byte records (iso_short); program p: srec[p] .. srec[p + 1] - 1
 };
 // One update of an isolated patch's program in LDS indices of one variable
 // (x < qn: the patch's cell x; x >= qn: staged parent cell x - qn): the value
@@ -152,6 +153,31 @@ struct IsoRec {
   unsigned short s[6];
 };
 static_assert(sizeof(IsoRec) == 16, "isolated-patch update record: 16 bytes");
+// The 8-byte form of a copy or a kind-1 interpolation whose indices are below
+// 2^13: dst, s[0], s[1], s[2] in 13 bits each from bit 0, the kind (0 or 1)
+// at bit 52, nflip in bits 53-56.
+constexpr int ISO_SHORT_BITS = 13;
+__host__ __device__ __forceinline__ bool iso_short(const IsoRec &r, unsigned long long &out) {
+  if (r.kind > 1 || r.nflip > 15) return false;
+  const int n = r.kind == 0 ? 1 : 3;
+  if (r.dst >= (1u << ISO_SHORT_BITS)) return false;
+  out = r.dst;
+  for (int k = 0; k < n; k++) {
+    if (r.s[k] >= (1u << ISO_SHORT_BITS)) return false;
+    out |= (unsigned long long)r.s[k] << (ISO_SHORT_BITS * (k + 1));
+  }
+  out |= (unsigned long long)r.kind << 52 | (unsigned long long)r.nflip << 53;
+  return true;
+}
+__host__ __device__ __forceinline__ IsoRec iso_unshort(unsigned long long w) {
+  IsoRec r{};
+  const unsigned m = (1u << ISO_SHORT_BITS) - 1;
+  r.dst = (unsigned short)(w & m);
+  r.kind = (unsigned char)((w >> 52) & 1);
+  r.nflip = (unsigned char)((w >> 53) & 15);
+  for (int k = 0; k < 3; k++) r.s[k] = (unsigned short)((w >> (ISO_SHORT_BITS * (k + 1))) & m);
+  return r;
+}
 
 // A tile of cells of one patch: origin (i0, j0, k0) in the patch's index
 // space; the tile extent is fixed per kernel.

@@ -118,9 +118,11 @@ struct LevelData {
   std::vector<IsoPatch> iso;
   std::vector<int> iso_stage;
   std::vector<IsoRec> iso_rec;
+  std::vector<unsigned long long> iso_srec;
   IsoPatch *d_iso = nullptr;
   int *d_iso_stage = nullptr;
   IsoRec *d_iso_rec = nullptr;
+  unsigned long long *d_iso_srec = nullptr;
   size_t iso_lds = 0;           // dynamic LDS of the FP64 smoothing (FP32: half)
   long long iso_cells = 0;      // interior cells of the isolated patches
   long long iso_updates[3] = {0, 0, 0};   // their updates per SetGhost program
@@ -376,7 +378,7 @@ void bsam_ctx::free_device() {
     F(l.d_int); F(l.d_ext); F(l.d_coarse); F(l.d_cext); F(l.d_patch_list);
     F(l.d_cgl); F(l.d_blk);
     F(l.d_granges); F(l.d_gcells);
-    F(l.d_iso); F(l.d_iso_stage); F(l.d_iso_rec); F(l.d_relax);
+    F(l.d_iso); F(l.d_iso_stage); F(l.d_iso_rec); F(l.d_iso_srec); F(l.d_relax);
     for (int p = 0; p < 3; p++) {
       FG(l.d_gp[p]);
       F(l.d_tgp[p]);
@@ -769,6 +771,7 @@ static void plan_iso(bsam_ctx *c, LevelData &L, std::vector<GhostOp> (&prog)[3])
   L.iso.clear();
   L.iso_stage.clear();
   L.iso_rec.clear();
+  L.iso_srec.clear();
   L.iso_lds = 0;
   L.iso_cells = 0;
   for (long long &u : L.iso_updates) u = 0;
@@ -847,6 +850,7 @@ static void plan_iso(bsam_ctx *c, LevelData &L, std::vector<GhostOp> (&prog)[3])
     };
     for (int ip = 0; ip < 3; ip++) {
       I.rec[ip] = (int)L.iso_rec.size();
+      I.srec[ip] = (int)L.iso_srec.size();
       for (const GhostOp *g : C.ops[ip]) {
         IsoRec r{};
         r.dst = lds_index(g->dst);
@@ -854,11 +858,14 @@ static void plan_iso(bsam_ctx *c, LevelData &L, std::vector<GhostOp> (&prog)[3])
         r.nflip = g->nflip;
         const int n = ghost_sources(*g, src);
         for (int k = 0; k < n; k++) r.s[k] = lds_index(src[k]);
-        L.iso_rec.push_back(r);
+        unsigned long long w;
+        if (iso_short(r, w)) L.iso_srec.push_back(w);
+        else L.iso_rec.push_back(r);
       }
       L.iso_updates[ip] += (long long)C.ops[ip].size();
     }
     I.rec[3] = (int)L.iso_rec.size();
+    I.srec[3] = (int)L.iso_srec.size();
     L.iso.push_back(I);
     L.iso_lds = std::max(L.iso_lds, (size_t)(2 * (h.qn + I.ns) * (long long)sizeof(double)));
   }
@@ -1180,6 +1187,7 @@ static void commit(bsam_ctx *c) {
       L.d_iso = dev_upload(L.iso, s);
       L.d_iso_stage = dev_upload(L.iso_stage, s);
       L.d_iso_rec = dev_upload(L.iso_rec, s);
+      L.d_iso_srec = dev_upload(L.iso_srec, s);
       iso_lds = std::max(iso_lds, L.iso_lds);
     }
     maxt = std::max(maxt, L.t_int.size());
@@ -1447,7 +1455,7 @@ static int level_relax(bsam_ctx *c, int level, int prec = PREC_SOLUTION_FP64,
     }
     ProfScope ps(c, BSAM_PROF_SMOOTH_ISO, sz * (48.0 * np * L.iso_cells + 32.0 * ups));
     HIPCK(launch_smooth_iso(c->nd(), prec, c->dev(), L.d_iso, (int)L.iso.size(),
-                            L.d_iso_stage, L.d_iso_rec, np, ghost0_first ? 1 : 0, 1,
+                            L.d_iso_stage, L.d_iso_rec, L.d_iso_srec, np, ghost0_first ? 1 : 0, 1,
                             prec == PREC_SOLUTION_FP64 ? L.iso_lds : L.iso_lds / 2,
                             side ? c->side : c->stream));
     if (L.t_relax.empty()) return -1;
@@ -2904,6 +2912,14 @@ int bsam_post_ghost_size(const bsam_ctx *cc, int level, int pending, int color, 
   return (int)(part % 2 == 0 ? g.ops.size() : g.rest.size());
 }
 
+int bsam_parent_ghost_rides(const bsam_ctx *cc, int level, int which) {
+  bsam_ctx *c = const_cast<bsam_ctx *>(cc);
+  if (!c || !c->committed || which < 0 || which > 1) return -1;
+  if (level <= c->cfg.minlevel || level > c->finest) return -1;
+  const LevelData &L = c->L(level);
+  return (which == 0 ? L.ride_cgl.ok : L.ride_corr.ok) ? 1 : 0;
+}
+
 int bsam_ghost_rides_down(const bsam_ctx *cc, int level, int ipass) {
   bsam_ctx *c = const_cast<bsam_ctx *>(cc);
   if (!c || !c->committed || ipass < 0 || ipass > 2) return -1;
@@ -2967,8 +2983,11 @@ int bsam_iso_program_apply_host(const bsam_ctx *cc, int level, int ipass,
     auto off = [&](unsigned short x) -> int {
       return x < h.qn ? (int)(h.qoff + x) : L.iso_stage[(size_t)(I.stage0 + (x - h.qn))];
     };
-    for (int r = I.rec[ipass]; r < I.rec[ipass + 1]; r++) {
-      const IsoRec &e = L.iso_rec[(size_t)r];
+    std::vector<IsoRec> recs;
+    for (int r = I.srec[ipass]; r < I.srec[ipass + 1]; r++)
+      recs.push_back(iso_unshort(L.iso_srec[(size_t)r]));
+    for (int r = I.rec[ipass]; r < I.rec[ipass + 1]; r++) recs.push_back(L.iso_rec[(size_t)r]);
+    for (const IsoRec &e : recs) {
       GhostOp g{};
       g.dst = off(e.dst);
       g.kind = e.kind;

@@ -850,6 +850,7 @@ constexpr int ISO_RREG = 2;   // updates of the next program each thread prefetc
 template <int ND, class T, bool DEF>
 __global__ __launch_bounds__(ISO_THREADS) void k_smooth_iso(Dev d, const IsoPatch *iso,
                                                              const int *stage, const IsoRec *rec,
+                                                             const unsigned long long *srec,
                                                              int npasses, int ghost0_first,
                                                              int final2) {
   const IsoPatch &I = iso[blockIdx.x];   // read in place (scalar loads)
@@ -909,35 +910,42 @@ __global__ __launch_bounds__(ISO_THREADS) void k_smooth_iso(Dev d, const IsoPatc
     }
   }
   __syncthreads();
-  // a program's first ISO_RREG updates per thread are fetched before the
-  // relaxation that precedes it
-  // (records as four dwords: dst | kind << 16 | nflip << 24, then the
-  // sources two to a dword, low half first)
+  // a program's first ISO_RREG short records per thread are fetched before
+  // the relaxation that precedes it; the 16-byte records (dwords: dst |
+  // kind << 16 | nflip << 24, then the sources two to a dword, low half
+  // first) are read as they are applied
   const uint4 *rw = reinterpret_cast<const uint4 *>(rec);
-  uint4 pre[ISO_RREG];
+  unsigned long long pre[ISO_RREG];
   auto fetch = [&](int p) __attribute__((always_inline)) {
 #pragma unroll
     for (int u = 0; u < ISO_RREG; u++) {
-      const int r = I.rec[p] + u * nth + tid;
-      if (r < I.rec[p + 1]) pre[u] = rw[r];
+      const int r = I.srec[p] + u * nth + tid;
+      if (r < I.srec[p + 1]) pre[u] = srec[r];
     }
   };
-  auto apply = [&](const uint4 g) __attribute__((always_inline)) {
-    const unsigned dst = g.x & 0xffffu;
-    const int kind = (g.x >> 16) & 0xff, nflip = g.x >> 24;
-    const unsigned w[3] = {g.y, g.z, g.w};
+  auto put = [&](unsigned dst, int kind, int nflip, auto src) __attribute__((always_inline)) {
 #pragma unroll
     for (int v = 0; v < 2; v++) {
       const T *b = box + v * W;
-      box[v * W + dst] = ghost_value<T, DEF>(
-          kind, nflip, [&](int n) { return b[(w[n >> 1] >> (16 * (n & 1))) & 0xffffu]; });
+      box[v * W + dst] = ghost_value<T, DEF>(kind, nflip, [&](int n) { return b[src(n)]; });
     }
+  };
+  auto apply_s = [&](const unsigned long long w) __attribute__((always_inline)) {
+    constexpr unsigned m = (1u << ISO_SHORT_BITS) - 1;
+    put((unsigned)w & m, (int)(w >> 52) & 1, (int)(w >> 53) & 15,
+        [&](int n) { return (unsigned)(w >> (ISO_SHORT_BITS * (n + 1))) & m; });
+  };
+  auto apply_l = [&](const uint4 g) __attribute__((always_inline)) {
+    const unsigned wv[3] = {g.y, g.z, g.w};
+    put(g.x & 0xffffu, (g.x >> 16) & 0xff, g.x >> 24,
+        [&](int n) { return (wv[n >> 1] >> (16 * (n & 1))) & 0xffffu; });
   };
   auto program = [&](int p) __attribute__((always_inline)) {
 #pragma unroll
     for (int u = 0; u < ISO_RREG; u++)
-      if (I.rec[p] + u * nth + tid < I.rec[p + 1]) apply(pre[u]);
-    for (int r = I.rec[p] + ISO_RREG * nth + tid; r < I.rec[p + 1]; r += nth) apply(rw[r]);
+      if (I.srec[p] + u * nth + tid < I.srec[p + 1]) apply_s(pre[u]);
+    for (int r = I.srec[p] + ISO_RREG * nth + tid; r < I.srec[p + 1]; r += nth) apply_s(srec[r]);
+    for (int r = I.rec[p] + tid; r < I.rec[p + 1]; r += nth) apply_l(rw[r]);
     __syncthreads();
   };
   const View<T> a = view_of<T>(box, P.px, P.py, W);
@@ -2109,23 +2117,24 @@ hipError_t launch_tail_filldown(int nd, const Dev &d, const TailLevel *lv, int n
 }
 
 hipError_t launch_smooth_iso(int nd, int prec, const Dev &d, const IsoPatch *iso, int n,
-                             const int *stage, const IsoRec *rec, int npasses, int ghost0_first,
+                             const int *stage, const IsoRec *rec,
+                             const unsigned long long *srec, int npasses, int ghost0_first,
                              int final2, size_t lds, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   const dim3 grid((unsigned)n);
   if (prec == PREC_SOLUTION_FP64) {
     if (nd == 2)
-      k_smooth_iso<2, double, false><<<grid, ISO_THREADS, lds, s>>>(d, iso, stage, rec, npasses,
+      k_smooth_iso<2, double, false><<<grid, ISO_THREADS, lds, s>>>(d, iso, stage, rec, srec, npasses,
                                                                     ghost0_first, final2);
     else
-      k_smooth_iso<3, double, false><<<grid, ISO_THREADS, lds, s>>>(d, iso, stage, rec, npasses,
+      k_smooth_iso<3, double, false><<<grid, ISO_THREADS, lds, s>>>(d, iso, stage, rec, srec, npasses,
                                                                     ghost0_first, final2);
   } else {
     if (nd == 2)
-      k_smooth_iso<2, float, true><<<grid, ISO_THREADS, lds, s>>>(d, iso, stage, rec, npasses,
+      k_smooth_iso<2, float, true><<<grid, ISO_THREADS, lds, s>>>(d, iso, stage, rec, srec, npasses,
                                                                   ghost0_first, final2);
     else
-      k_smooth_iso<3, float, true><<<grid, ISO_THREADS, lds, s>>>(d, iso, stage, rec, npasses,
+      k_smooth_iso<3, float, true><<<grid, ISO_THREADS, lds, s>>>(d, iso, stage, rec, srec, npasses,
                                                                   ghost0_first, final2);
   }
   return hipGetLastError();

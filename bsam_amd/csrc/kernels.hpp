@@ -22,7 +22,8 @@ hipError_t launch_relax_fused(int nd, int prec, const Dev &d, const Tile *tiles,
 // pending SetGhost program (SetGhost(0) before the first when ghost0_first),
 // and SetGhost(2) after the last when final2.
 hipError_t launch_smooth_iso(int nd, int prec, const Dev &d, const IsoPatch *iso, int n,
-                             const int *stage, const IsoRec *rec, int npasses, int ghost0_first,
+                             const int *stage, const IsoRec *rec,
+                             const unsigned long long *srec, int npasses, int ghost0_first,
                              int final2, size_t lds, hipStream_t s);
 // Raises the dynamic-LDS limit of the isolated-patch smoothing to `lds` bytes.
 hipError_t iso_allow_lds(size_t lds);

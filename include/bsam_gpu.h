@@ -419,6 +419,11 @@ int bsam_fused_ghost_pre_size(const bsam_ctx *ctx, int level, int ipass, int col
  * level's restriction and fill-down write, so the engine runs it in the same
  * launch as those; 0 if not; -1 for bad arguments. */
 int bsam_ghost_rides_down(const bsam_ctx *ctx, int level, int ipass);
+/* 1 if the parent level's SetGhost(0) runs inside this level's coarse ghost +
+ * loading launch (which 0), or the parent's SetGhost(2) inside its
+ * correction launch (which 1); 0 if it is launched on its own; -1 for bad
+ * arguments. No counterpart in the reference. */
+int bsam_parent_ghost_rides(const bsam_ctx *ctx, int level, int which);
 int bsam_fused_ghost_apply_host(const bsam_ctx *ctx, int level, int ipass, int color,
                                 double *const *q_patches);
 /* Applies on the host one part of a fused transition: part 0 the updates
