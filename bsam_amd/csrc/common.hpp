@@ -179,6 +179,48 @@ __host__ __device__ __forceinline__ IsoRec iso_unshort(unsigned long long w) {
   return r;
 }
 
+// Tiles of the coarse loading launch. Fine tiles of 2^lx x 2^ly (x 2^lz)
+// interior cells (origins 1 + multiples of the extents, so a tile holds
+// whole 2^d blocks of children): 32 x 8 in 2D, 16 x 8 x 2 in 3D. The qc ring
+// (GetCoarseGhostPoints) is copied by linear tiles over the ring cells of each
+// patch's ghost-extended coarse box, numbered by ring_cell.
+__host__ __device__ __forceinline__ int cl_lx(int nd) { return nd == 3 ? 4 : 5; }
+__host__ __device__ __forceinline__ int cl_ly(int) { return 3; }
+__host__ __device__ __forceinline__ int cl_lz(int nd) { return nd == 3 ? 1 : 0; }
+// Ring cells of a ghost-extended coarse box of interior cx x cy (x cz): in 3D
+// the planes k = 0 and k = cz + 1, then per plane k = 1..cz the rows j = 0 and
+// j = cy + 1 and the cells i = 0 and i = cx + 1 of the rows between; in 2D
+// the rows and end cells alone (k = 0).
+__host__ __device__ __forceinline__ long long ring_count(int nd, int cx, int cy, int cz) {
+  const long long X = cx + 2, Y = cy + 2, per = 2 * X + 2LL * cy;
+  return nd == 3 ? 2 * X * Y + (long long)cz * per : per;
+}
+__host__ __device__ __forceinline__ void ring_cell(int nd, int cx, int cy, int cz, long long e,
+                                                   int &i, int &j, int &k) {
+  const long long X = cx + 2, Y = cy + 2, per = 2 * X + 2LL * cy;
+  k = 0;
+  if (nd == 3) {
+    if (e < 2 * X * Y) {
+      k = e < X * Y ? 0 : cz + 1;
+      const long long r = e % (X * Y);
+      i = (int)(r % X);
+      j = (int)(r / X);
+      return;
+    }
+    e -= 2 * X * Y;
+    k = (int)(e / per) + 1;
+    e %= per;
+  }
+  if (e < 2 * X) {
+    j = e < X ? 0 : cy + 1;
+    i = (int)(e % X);
+  } else {
+    e -= 2 * X;
+    j = (int)(e >> 1) + 1;
+    i = (e & 1) ? cx + 1 : 0;
+  }
+}
+
 // A tile of cells of one patch: origin (i0, j0, k0) in the patch's index
 // space; the tile extent is fixed per kernel.
 struct Tile {

@@ -78,9 +78,11 @@ struct LevelData {
   int *d_patch_list = nullptr;  // the same on the device (regrid kernels)
   std::vector<Tile> t_int, t_ext, t_coarse, t_cext;
   Tile *d_int = nullptr, *d_ext = nullptr, *d_coarse = nullptr, *d_cext = nullptr;
-  // linear tiles (LIN_TILE cells each): the ghost-extended coarse box of the
-  // coarse ghost + loading launch, the 2x2(x2) fine blocks of the correction
+  // the coarse ghost + loading launch: fine tiles (cl_lx/cl_ly/cl_lz), then
+  // linear tiles over the qc ring cells (n_cgl_res: the fine tiles); linear
+  // tiles (LIN_TILE cells each) over the 2x2(x2) fine blocks of the correction
   std::vector<Tile> t_cgl, t_blk;
+  int n_cgl_res = 0;
   Tile *d_cgl = nullptr, *d_blk = nullptr;
   // linear tiles in device order over each patch's ghost-extended q array
   std::vector<Tile> t_qlin;
@@ -281,6 +283,9 @@ struct bsam_ctx {
   // smoothing of isolated patches in one LDS launch on levels where they hold
   // at least this many cells (0: never)
   long long iso_min_cells = 65536;
+  // coarse loading over fine tiles on levels of at least this many cells
+  // (-1: never)
+  long long cgl_fine_cells = 524288;
   int tail_top = 0;                 // highest level run by the tail kernels
   TailLevel *d_tail = nullptr;      // levels minlevel..tail_top
   hipGraph_t graph = nullptr, graph_mixed = nullptr;
@@ -459,6 +464,14 @@ static void build_tiles(bsam_ctx *c, LevelData &L, int level) {
   auto lin = [&](std::vector<Tile> &out, int p, long long n) {
     for (long long e = 0; e < n; e += LIN_TILE) out.push_back(Tile{p, (int)e, 0, 0});
   };
+  // the coarse loading runs over fine tiles on levels of at least
+  // cgl_fine_cells cells, else over linear tiles of the coarse box
+  long long level_cells = 0;
+  for (int p : L.patches) {
+    const HPatch &h = c->hp[p];
+    level_cells += (long long)h.mx[0] * h.mx[1] * (nd == 3 ? h.mx[2] : 1);
+  }
+  const bool fine_cgl = c->cgl_fine_cells >= 0 && level_cells >= c->cgl_fine_cells;
   for (int p : L.patches) {
     const HPatch &h = c->hp[p];
     cover(L.t_int, p, 1, h.mx[0], h.mx[1], h.mx[2]);
@@ -467,11 +480,26 @@ static void build_tiles(bsam_ctx *c, LevelData &L, int level) {
     if (level > c->cfg.minlevel) {
       cover(L.t_coarse, p, 1, h.cmx[0], h.cmx[1], h.cmx[2]);
       cover(L.t_cext, p, 0, h.cmx[0] + 2, h.cmx[1] + 2, h.cmx[2] + 2);
-      lin(L.t_cgl, p, (long long)(h.cmx[0] + 2) * (h.cmx[1] + 2) * (nd == 3 ? h.cmx[2] + 2 : 1));
+      if (fine_cgl) {
+        const int rx = 1 << cl_lx(nd), ry = 1 << cl_ly(nd), rz = 1 << cl_lz(nd);
+        for (int k = (nd == 3 ? 1 : 0); k < (nd == 3 ? 1 + h.mx[2] : 1); k += rz)
+          for (int j = 1; j < 1 + h.mx[1]; j += ry)
+            for (int i = 1; i < 1 + h.mx[0]; i += rx) L.t_cgl.push_back(Tile{p, i, j, k});
+      } else {
+        lin(L.t_cgl, p,
+            (long long)(h.cmx[0] + 2) * (h.cmx[1] + 2) * (nd == 3 ? h.cmx[2] + 2 : 1));
+      }
       lin(L.t_blk, p, (long long)(h.cmx[0] + 1) * (h.cmx[1] + 1) * (nd == 3 ? h.cmx[2] + 1 : 1));
     }
     L.cells += (long long)h.mx[0] * h.mx[1] * (nd == 3 ? h.mx[2] : 1);
   }
+  // the coarse loading's ring tiles follow its fine tiles
+  L.n_cgl_res = fine_cgl ? (int)L.t_cgl.size() : -1;
+  if (fine_cgl && level > c->cfg.minlevel)
+    for (int p : L.patches) {
+      const HPatch &h = c->hp[p];
+      lin(L.t_cgl, p, ring_count(nd, h.cmx[0], h.cmx[1], nd == 3 ? h.cmx[2] : 1));
+    }
 }
 
 // Places the tail levels' arrays in LDS within `budget` bytes (coarsest level
@@ -1128,7 +1156,8 @@ static void commit(bsam_ctx *c) {
       // only the reads of cells the parent's program writes matter to a ride
       std::vector<int> at;
       if (c->fuse_ghost && destination_index(PL.gp[0], at))
-        L.ride_cgl = plan_ride(PL.gp[0], coarse_ghost_load_reads(c->hp, L.t_cgl, nd, &at));
+        L.ride_cgl = plan_ride(PL.gp[0],
+                               coarse_ghost_load_reads(c->hp, L.t_cgl, L.n_cgl_res, nd, &at));
       if (c->fuse_ghost && destination_index(PL.gp[2], at))
         L.ride_corr = plan_ride(PL.gp[2], correct_reads(c->hp, L.t_blk, nd, &at));
     }
@@ -1360,6 +1389,7 @@ static void op_coarse_ghost_load(bsam_ctx *c, int level, int prec = PREC_SOLUTIO
   const LevelData::DRide *r = parent_ride(c, L, parent_ghost, level, prec, false);
   ProfScope ps(c, BSAM_PROF_COARSE_LOAD, 40.0 * L.cells);
   HIPCK(launch_coarse_ghost_load(c->nd(), prec, c->dev(), L.d_cgl, (int)L.t_cgl.size(),
+                                 L.n_cgl_res,
                                  r ? r->rest.view() : GhostList{},
                                  r ? r->ops.view() : GhostList{}, c->stream));
 }
@@ -2674,6 +2704,10 @@ int bsam_tune(bsam_ctx *c, const char *key, double value) {
   } else if (k == "iso_side") {
     c->iso_side = value != 0.0;
     c->drop_graphs();
+  } else if (k == "cgl_fine_cells") {
+    if (c->committed || c->building)
+      throw Error("cgl_fine_cells must be set before the hierarchy");
+    c->cgl_fine_cells = (long long)value;
   } else if (k == "iso_min_cells") {
     if (c->committed || c->building)
       throw Error("iso_min_cells must be set before the hierarchy");
@@ -2702,6 +2736,7 @@ int bsam_tune_get(const bsam_ctx *cc, const char *key, double *value) {
   else if (k == "short_ghost") *value = c->short_ghost;
   else if (k == "iso_min_cells") *value = (double)c->iso_min_cells;
   else if (k == "iso_side") *value = c->iso_side;
+  else if (k == "cgl_fine_cells") *value = (double)c->cgl_fine_cells;
   else if (k == "iso_patches" || k == "iso_cells") {
     double n = 0;
     for (const LevelData &L : c->lev)

@@ -1146,14 +1146,47 @@ static void for_lin_cells(const Tile &tl, const int e[3], F f) {
     f((int)(x % e[0]), (int)((x / e[0]) % e[1]), (int)(x / ((long long)e[0] * e[1])));
 }
 
+// The coarse cells tile `tl` of the coarse ghost + loading launch works on:
+// a fine tile (res) the interior coarse cells whose children it holds, a ring
+// tile its ring cells (ring_cell).
+template <class F>
+static void for_cgl_cells(const HPatch &h, const Tile &tl, int kind, int nd, F f) {
+  const int cz = nd == 3 ? h.cmx[2] : 1;
+  if (kind < 0) {   // a linear tile over the ghost-extended coarse box
+    const int box[3] = {h.cmx[0] + 2, h.cmx[1] + 2, nd == 3 ? cz + 2 : 1};
+    for_lin_cells(tl, box, f);
+    return;
+  }
+  if (kind == 1) {
+    const int hx = (1 << cl_lx(nd)) / 2, hy = (1 << cl_ly(nd)) / 2,
+              hz = nd == 3 ? (1 << cl_lz(nd)) / 2 : 1;
+    for (int c = 0; c < hz; c++)
+      for (int b = 0; b < hy; b++)
+        for (int a = 0; a < hx; a++) {
+          const int i = (tl.i0 + 1) / 2 + a, j = (tl.j0 + 1) / 2 + b,
+                    k = nd == 3 ? (tl.k0 + 1) / 2 + c : 0;
+          if (i <= h.cmx[0] && j <= h.cmx[1] && (nd == 2 || k <= cz)) f(i, j, k);
+        }
+    return;
+  }
+  const long long n = ring_count(nd, h.cmx[0], h.cmx[1], cz);
+  for (long long e = tl.i0; e < std::min<long long>(tl.i0 + LIN_TILE, n); e++) {
+    int i, j, k;
+    ring_cell(nd, h.cmx[0], h.cmx[1], cz, e, i, j, k);
+    f(i, j, k);
+  }
+}
+
 std::vector<std::vector<long long>> coarse_ghost_load_reads(const std::vector<HPatch> &hp,
                                                             const std::vector<Tile> &tiles,
-                                                            int nd, const std::vector<int> *only) {
+                                                            int nres, int nd,
+                                                            const std::vector<int> *only) {
   std::vector<std::vector<long long>> out(tiles.size());
   for (size_t t = 0; t < tiles.size(); t++) {
     const Tile &tl = tiles[t];
     const HPatch &h = hp[tl.patch];
-    const int box[3] = {h.cmx[0] + 2, h.cmx[1] + 2, nd == 3 ? h.cmx[2] + 2 : 1};
+    // -1: linear tile over the coarse box, 1: fine tile, 0: ring tile
+    const int kind = nres < 0 ? -1 : (int)t < nres ? 1 : 0;
     const int e[3] = {h.cmx[0] + 1, h.cmx[1] + 1, nd == 3 ? h.cmx[2] + 1 : 0};
     auto ring = [&](int a, int b, int c) {
       return a == 0 || a == e[0] || b == 0 || b == e[1] || (nd == 3 && (c == 0 || c == e[2]));
@@ -1162,19 +1195,20 @@ std::vector<std::vector<long long>> coarse_ghost_load_reads(const std::vector<HP
     if (only) {
       // every cell read lies within one cell of the tile's cells
       int lo[3] = {INT_MAX, INT_MAX, nd == 3 ? INT_MAX : 0}, hi[3] = {INT_MIN, INT_MIN, 0};
-      for_lin_cells(tl, box, [&](int i, int j, int k) {
+      for_cgl_cells(h, tl, kind, nd, [&](int i, int j, int k) {
         const int v[3] = {i, j, k};
         for (int d = 0; d < nd; d++) {
-          lo[d] = std::min(lo[d], v[d] - 1);
+          lo[d] = std::min(lo[d], std::max(v[d] - 1, 0));
           hi[d] = std::max(hi[d], std::min(v[d] + 1, e[d]));
         }
       });
-      if (parent_interior(hp, h, nd, lo, hi)) continue;
+      if (lo[0] == INT_MAX || parent_interior(hp, h, nd, lo, hi)) continue;
     }
-    for_lin_cells(tl, box, [&](int i, int j, int k) {
-      // the cell itself (a ring cell copies the parent) and its stencil
-      // neighbours on the ring (read from the parent by interior cells)
-      for (int s = 0; s < 2 * nd + 1; s++) {
+    for_cgl_cells(h, tl, kind, nd, [&](int i, int j, int k) {
+      // a ring cell copies the parent; an interior cell reads its stencil
+      // neighbours on the ring from the parent
+      const bool inner = !ring(i, j, k);
+      for (int s = 0; s < (inner ? 2 * nd + 1 : 1); s++) {
         const int d[7][3] = {{0, 0, 0}, {1, 0, 0}, {-1, 0, 0}, {0, 1, 0},
                              {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
         int a = i + d[s][0], b = j + d[s][1], c = k + d[s][2];

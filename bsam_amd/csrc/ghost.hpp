@@ -57,13 +57,14 @@ struct RideGhost {
 // patch, the first at cell Tile::i0 of the box.
 constexpr int LIN_TILE = 256;
 // Parent q-pool offsets (variable 1) that each tile of the child level's
-// coarse ghost + loading launch (linear tiles over the ghost-extended coarse
-// box) or correction launch (linear tiles over the 2x2(x2) fine blocks
-// 0..cmx) may read, ascending; with `only`, just the offsets off with
-// only[off] >= 0.
+// coarse ghost + loading launch (fine tiles tiles[0..nres), then linear tiles
+// over the qc ring cells; with nres < 0, linear tiles over the ghost-extended
+// coarse box) or correction launch (linear tiles over the
+// 2x2(x2) fine blocks 0..cmx) may read, ascending; with `only`, just the
+// offsets off with only[off] >= 0.
 std::vector<std::vector<long long>> coarse_ghost_load_reads(const std::vector<HPatch> &hp,
                                                             const std::vector<Tile> &tiles,
-                                                            int ndims,
+                                                            int nres, int ndims,
                                                             const std::vector<int> *only = nullptr);
 std::vector<std::vector<long long>> correct_reads(const std::vector<HPatch> &hp,
                                                   const std::vector<Tile> &tiles, int ndims,

@@ -445,31 +445,24 @@ __device__ __forceinline__ void trunc_error_cell(const Dev &d, const Lvl<double>
   d.qrte[o + P.rvs] = R2 - c2;
 }
 
-// GetCoarseGhostPoints and CoarseLoadingFunction in one pass over the
-// ghost-extended coarse box (i,j,k in 0..cmx+1): ring cells copy the parent
-// into qc; interior cells evaluate the coarse operator reading the ring
-// values straight from the parent (the same values the copy stores).
+// The coarse part of CoarseLoadingFunction at interior coarse cell (i,j,k)
+// once the restricted fine residual (R1, R2) is known: parent f(mbounds) =
+// R + A_2h(qc), the coarse stencil reading qc inside the box and, on the
+// ring, the parent cells GetCoarseGhostPoints copies there (the same values).
 template <int ND, class T, bool DEF>
-__device__ __forceinline__ void coarse_ghost_load_cell(const Problem &prm, int nrvars,
-                                                       const Lvl<T> &C, const Lvl<T> &PA, int i,
-                                                       int j, int k) {
+__device__ __forceinline__ void coarse_load_from(const Problem &prm, const Lvl<T> &C,
+                                                 const Lvl<T> &PA, int i, int j, int k, T R1,
+                                                 T R2) {
   const DPatch &P = C.P;
   auto on_ring = [&](int a, int b, int cc) {
     return a == 0 || a == P.cmx + 1 || b == 0 || b == P.cmy + 1 ||
            (ND == 3 && (cc == 0 || cc == P.cmz + 1));
   };
-  if (on_ring(i, j, k)) {
-    coarse_ghost_cell<ND, T>(nrvars, C, PA, i, j, k);
-    return;
-  }
-  View<T> a = qview(C), cq = qcview(C), pq = qview(PA);
-  T R1, R2;
-  restricted_residual<ND, T, DEF>(prm, C, a, i, j, k, R1, R2);
+  View<T> cq = qcview(C), pq = qview(PA);
   int pi = P.mb[0][0] + i - 1, pj = P.mb[1][0] + j - 1,
       pk = ND == 3 ? P.mb[2][0] + k - 1 : 1;
   T base = (T)0;
   if (DEF) base = PA.qb[qat(PA.P, pi, pj, ND == 3 ? pk : 0)];
-  // the coarse stencil: qc inside the box, the parent on the ring
   const int di[7] = {0, 1, -1, 0, 0, 0, 0}, dj[7] = {0, 0, 0, 1, -1, 0, 0},
             dk[7] = {0, 0, 0, 0, 0, 1, -1};
   T a1[2 * ND + 1], a2[2 * ND + 1];
@@ -492,6 +485,25 @@ __device__ __forceinline__ void coarse_ghost_load_cell(const Problem &prm, int n
   long long po = fat(PA.P, pi, pj, pk);
   PA.f[po] = R1 + c1;
   PA.f[po + PA.P.fvs] = R2 + c2;
+}
+
+// GetCoarseGhostPoints and CoarseLoadingFunction for one cell of the
+// ghost-extended coarse box (i,j,k in 0..cmx+1): ring cells copy the parent
+// into qc; interior cells form their children's restricted residual and the
+// coarse part (coarse_load_from).
+template <int ND, class T, bool DEF>
+__device__ __forceinline__ void coarse_ghost_load_cell(const Problem &prm, int nrvars,
+                                                       const Lvl<T> &C, const Lvl<T> &PA, int i,
+                                                       int j, int k) {
+  const DPatch &P = C.P;
+  if (i == 0 || i == P.cmx + 1 || j == 0 || j == P.cmy + 1 ||
+      (ND == 3 && (k == 0 || k == P.cmz + 1))) {
+    coarse_ghost_cell<ND, T>(nrvars, C, PA, i, j, k);
+    return;
+  }
+  T R1, R2;
+  restricted_residual<ND, T, DEF>(prm, C, qview(C), i, j, k, R1, R2);
+  coarse_load_from<ND, T, DEF>(prm, C, PA, i, j, k, R1, R2);
 }
 
 // CorrectFine (afasroutines.f90:842-963, mbc = 1): q += P1(parent q - qc) at
@@ -1066,11 +1078,14 @@ __global__ __launch_bounds__(256) void k_coarse_load(Dev d, const Tile *tiles, i
   coarse_load_cell<ND, T, DEF>(d.prm, C, level_ref<T>(d, C.P.parent), i, j, k);
 }
 
-// Linear tiles over the ghost-extended coarse box 0..cmx+1 (LIN_TILE cells
-// each). A SetGhost program of the parent may ride along: `rest` grid-stride,
-// and per tile the updates it reads (`tops`).
+// The same launch over linear tiles of the ghost-extended coarse box
+// 0..cmx+1 (LIN_TILE cells each), one thread per coarse cell, which forms the
+// residuals of its children one after another (restricted_residual): fewer,
+// fuller workgroups on levels of small patches. A SetGhost program of the
+// parent may ride along: `rest` grid-stride, and per tile the updates it
+// reads (`tops`).
 template <int ND, class T, bool DEF>
-__global__ __launch_bounds__(256) void k_coarse_ghost_load(Dev d, const Tile *tiles, int nt,
+__global__ __launch_bounds__(256) void k_coarse_ghost_load_lin(Dev d, const Tile *tiles, int nt,
                                                            GhostList rest, GhostList tops) {
   const int b = xcd_tile(nt);
   // the tile's updates are of its patch's parent, in the same group
@@ -1082,6 +1097,83 @@ __global__ __launch_bounds__(256) void k_coarse_ghost_load(Dev d, const Tile *ti
     if (lin_cell<ND>(t, C.P.cmx + 2, C.P.cmy + 2, C.P.cmz + 2, i, j, k))
       coarse_ghost_load_cell<ND, T, DEF>(d.prm, d.nrvars, C, level_ref<T>(d, C.P.parent), i, j,
                                          k);
+  }
+  // the updates no tile reads, after the work (they touch nothing it uses)
+  ghost_alongside<T>(d, rest);
+}
+
+// GetCoarseGhostPoints + CoarseLoadingFunction (afasroutines.f90:646-714)
+// of a level in one launch. Tiles 0..nres-1 are fine tiles (cl_lx, cl_ly,
+// cl_lz): each thread forms the fine residual f - A_h(q) of one cell into
+// LDS; after a barrier one thread per coarse cell of the tile sums its 2^d
+// children in the reference's child order (restricted_residual's sum),
+// evaluates the coarse operator on qc (the parent on the ring, the values the
+// ring copy stores) and writes the parent's f. Tiles nres.. are linear tiles
+// over the ring cells of the ghost-extended coarse box (ring_cell), which
+// copy the parent into qc. A SetGhost program of the parent may ride along:
+// `rest` grid-stride, and per tile the updates it reads (`tops`).
+template <int ND, class T, bool DEF>
+__global__ __launch_bounds__(256) void k_coarse_ghost_load(Dev d, const Tile *tiles, int nt,
+                                                           int nres, GhostList rest,
+                                                           GhostList tops) {
+  constexpr int LX = ND == 3 ? 4 : 5, LY = 3, LZ = ND == 3 ? 1 : 0;
+  constexpr int TX = 1 << LX, TY = 1 << LY, HX = TX / 2, HY = TY / 2;
+  constexpr int HZ = ND == 3 ? (1 << LZ) / 2 : 1;
+  static_assert(TX * TY * (1 << LZ) == 256, "256 fine cells per tile");
+  __shared__ T res[2][256];
+  const int b = xcd_tile(nt);
+  // the tile's updates are of its patch's parent, in the same group
+  if (b < nt && !stopped_patch(d, tiles[b].patch)) {
+    const Tile t = tiles[b];
+    ghost_tile<T>(d, tops, b);
+    const Lvl<T> C = level_ref<T>(d, t.patch);
+    const Lvl<T> PA = level_ref<T>(d, C.P.parent);
+    const DPatch &P = C.P;
+    if (b < nres) {
+      // this thread's fine cell, the lanes of a row ordered by the parity of i
+      const int tid = threadIdx.x, l = tid & (TX - 1);
+      const int li = 2 * (l & (HX - 1)) + (l >> (LX - 1)), lj = (tid >> LX) & (TY - 1);
+      const int lk = ND == 3 ? tid >> (LX + LY) : 0;
+      const int i = t.i0 + li, j = t.j0 + lj, k = ND == 3 ? t.k0 + lk : 0;
+      const int at = li + TX * (lj + TY * lk);
+      if (i <= P.mx && j <= P.my && (ND == 2 || k <= P.mz)) {
+        const View<T> a = qview(C);
+        const T base = DEF ? C.qb[a.at(i, j, k)] : (T)0;
+        T o1, o2;
+        op_at<ND, T, DEF>(d.prm, a, base, P, i, j, k, o1, o2);
+        const long long fo = fat(P, i, j, ND == 3 ? k : 1);
+        res[0][at] = C.f[fo] - o1;
+        res[1][at] = C.f[fo + P.fvs] - o2;
+      }
+      __syncthreads();
+      if (tid < HX * HY * HZ) {
+        const int ca = tid % HX, cb = (tid / HX) % HY, cc = tid / (HX * HY);
+        const int ci = (t.i0 + 1) / 2 + ca, cj = (t.j0 + 1) / 2 + cb;
+        const int ck = ND == 3 ? (t.k0 + 1) / 2 + cc : 0;
+        if (ci <= P.cmx && cj <= P.cmy && (ND == 2 || ck <= P.cmz)) {
+          constexpr int NC = ND == 2 ? 4 : 8;
+          T s1 = (T)0, s2 = (T)0;
+#pragma unroll
+          for (int s = 0; s < NC; s++) {
+            const int x = 2 * ca + 1 - (s & 1), y = 2 * cb + 1 - ((s >> 1) & 1);
+            const int z = ND == 3 ? 2 * cc + 1 - ((s >> 2) & 1) : 0;
+            const int e = x + TX * (y + TY * z);
+            s1 = s == 0 ? res[0][e] : s1 + res[0][e];
+            s2 = s == 0 ? res[1][e] : s2 + res[1][e];
+          }
+          const T R1 = (T)(ND == 2 ? 0.25 : 0.125) * s1, R2 = (T)(ND == 2 ? 0.25 : 0.125) * s2;
+          coarse_load_from<ND, T, DEF>(d.prm, C, PA, ci, cj, ck, R1, R2);
+        }
+      }
+    } else {
+      const long long e = t.i0 + (long long)threadIdx.x;
+      const int cz = ND == 3 ? P.cmz : 1;
+      if (e < ring_count(ND, P.cmx, P.cmy, cz)) {
+        int i, j, k;
+        ring_cell(ND, P.cmx, P.cmy, cz, e, i, j, k);
+        coarse_ghost_cell<ND, T>(d.nrvars, C, PA, i, j, k);
+      }
+    }
   }
   // the updates no tile reads, after the work (they touch nothing it uses)
   ghost_alongside<T>(d, rest);
@@ -1980,11 +2072,15 @@ hipError_t launch_coarse_load(int nd, int prec, const Dev &d, const Tile *tiles,
 }
 
 hipError_t launch_coarse_ghost_load(int nd, int prec, const Dev &d, const Tile *tiles, int nt,
-                                   const GhostList &rest, const GhostList &tops,
+                                   int nres, const GhostList &rest, const GhostList &tops,
                                    hipStream_t s) {
   if (nt <= 0) return hipSuccess;
-  BSAM_LAUNCH_ND_PREC(k_coarse_ghost_load, nd, prec, xcd_grid(nt), 256, d, tiles, nt, rest,
-                      tops);
+  if (nres < 0)
+    BSAM_LAUNCH_ND_PREC(k_coarse_ghost_load_lin, nd, prec, xcd_grid(nt), 256, d, tiles, nt, rest,
+                        tops);
+  else
+    BSAM_LAUNCH_ND_PREC(k_coarse_ghost_load, nd, prec, xcd_grid(nt), 256, d, tiles, nt, nres,
+                        rest, tops);
   return hipGetLastError();
 }
 

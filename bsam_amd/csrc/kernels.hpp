@@ -39,14 +39,18 @@ hipError_t launch_coarse_ghost(int nd, int prec, const Dev &d, const Tile *tiles
                                hipStream_t s);
 hipError_t launch_coarse_load(int nd, int prec, const Dev &d, const Tile *tiles, int nt,
                               hipStream_t s);
-// GetCoarseGhostPoints + CoarseLoadingFunction in one launch over the
-// ghost-extended coarse tiles, and CorrectFine over the ghost-extended fine
-// tiles. Either may apply a SetGhost program of the parent level in the same
-// launch: `rest` grid-stride (destinations the launch does not read) and, per
-// tile, its share of `tops` before its work; the launch writes none of the
-// program's sources.
+// GetCoarseGhostPoints + CoarseLoadingFunction in one launch: tiles[0..nres)
+// are fine tiles (cl_lx, cl_ly, cl_lz extents) forming the coarse right-hand
+// side, the rest linear tiles over the qc ring cells (ring_cell); with nres
+// < 0, linear tiles over the ghost-extended coarse box, one coarse cell per
+// thread. And
+// CorrectFine over linear tiles of the ghost-extended fine blocks. Either may
+// apply a SetGhost program of the parent level in the same launch: `rest`
+// grid-stride (destinations the launch does not read) and, per tile, its
+// share of `tops` before its work; the launch writes none of the program's
+// sources.
 hipError_t launch_coarse_ghost_load(int nd, int prec, const Dev &d, const Tile *tiles, int nt,
-                                   const GhostList &rest, const GhostList &tops,
+                                   int nres, const GhostList &rest, const GhostList &tops,
                                    hipStream_t s);
 hipError_t launch_correct(int nd, int prec, const Dev &d, const Tile *tiles, int nt,
                           const GhostList &rest, const GhostList &tops, hipStream_t s);

@@ -335,7 +335,13 @@ int bsam_regrid_level(const bsam_regrid_params *rp, int npatch, const int *mx,
  *                  other patch's programs read) in one launch per smoothing,
  *                  each patch in one workgroup out of LDS, when those that
  *                  fit hold at least this many cells (default 65536; 0:
- *                  never). Needs fuse_ghost. Before the hierarchy. */
+ *                  never). Needs fuse_ghost. Before the hierarchy.
+ *   "cgl_fine_cells" levels of at least this many cells (default 524288;
+ *                  0: all, -1: none) run the coarse ghost + loading launch
+ *                  over fine tiles, each thread one fine residual summed per
+ *                  coarse cell through LDS; the others over linear tiles of
+ *                  the coarse box, one coarse cell per thread. Before the
+ *                  hierarchy. */
 int bsam_tune(bsam_ctx *ctx, const char *key, double value);
 /* Reads a tuning value back, or a property of the committed schedule:
  * "tail_top" (highest level run by the tail kernel; below minlevel when

@@ -81,11 +81,12 @@ def test_mixed_matches_fp64(name, inner, staged):
 @pytest.mark.parametrize("inner", [1, 3])
 @pytest.mark.parametrize("name", ["amr2d_tiles", "amr3d", "blocks3d"])
 def test_mixed_iso_matches_mixed(name, inner):
-    """The FP32 smoothing of isolated patches out of LDS evaluates the
-    per-colour launches' arithmetic in the same order: the mixed solve with it
-    (on every level that has such patches, coarse single-patch levels run by
-    level launches included) leaves q, cycle counts and residuals
-    bit-identical to the mixed solve without it."""
+    """The FP32 smoothing of isolated patches out of LDS, and the coarse
+    loading over fine tiles, evaluate the per-colour launches' and the
+    per-coarse-cell loading's arithmetic in the same order: the mixed solve
+    with both (on every level, coarse single-patch levels run by level
+    launches included) leaves q, cycle counts and residuals bit-identical to
+    the mixed solve without them."""
     from bsam_amd.workloads import nested_block_layout_3d
     lay = (nested_block_layout_3d(32, 2, (0.25, 0.25), seed=5, sides=(8, 16))
            if name == "blocks3d" else LAYOUTS[name]())
@@ -93,7 +94,8 @@ def test_mixed_iso_matches_mixed(name, inner):
               maxvcycles=40, qerrortol=QERRORTOL)
     q0 = noise_q(lay, seed=23, mean=0.1, amp=0.05)
     runs = []
-    for tune in ({"iso_min_cells": 0, "tail_cells": 0}, {"iso_min_cells": 1, "tail_cells": 0}):
+    for tune in ({"iso_min_cells": 0, "tail_cells": 0, "cgl_fine_cells": -1},
+                 {"iso_min_cells": 1, "tail_cells": 0, "cgl_fine_cells": 0}):
         eng = Engine(lay, device=0, precision=PREC_MIXED,
                      tune=dict(tune, inner_cycles=inner), **kw)
         if tune["iso_min_cells"]:

@@ -46,7 +46,9 @@ LAYOUTS = {
 # 32-byte device record; isolated patches smoothed out of LDS on every level
 # that has one (with the launches of the other patches fused, moved or
 # staged, with every single-patch coarse level run by level launches, and on
-# the engine's stream instead of beside the other patches' launches)
+# the engine's stream instead of beside the other patches' launches); the
+# coarse loading over fine tiles on every level, or over linear coarse tiles
+# on every level
 SCHEDULES = {
     "launch": {"tail_cells": 0, "graph": 0},
     "default": {},
@@ -65,6 +67,9 @@ SCHEDULES = {
     "iso_no_post": {"iso_min_cells": 1, "post_ghost": 0},
     "iso_staged": {"iso_min_cells": 1, "relax_lds": 1},
     "iso_one_stream": {"iso_min_cells": 1, "iso_side": 0},
+    "cgl_fine": {"cgl_fine_cells": 0},
+    "cgl_fine_iso_launch": {"cgl_fine_cells": 0, "iso_min_cells": 1, "tail_cells": 0},
+    "cgl_lin": {"cgl_fine_cells": -1},
 }
 
 
