@@ -109,9 +109,17 @@ struct LevelData {
   // launch, and its SetGhost(2) (the last of post-smoothing) inside this
   // level's correction launch
   RideGhost ride_cgl, ride_corr;
+  // Inside the multigrid loop (FP64, levels >= 1): the parent's SetGhost(0)
+  // that the loop's FillDown applied in full, since when only the parent
+  // cells under this level (this V-cycle's restriction), the grandparent
+  // cells under the parent and the grandparent's ghost cells (FillDown's
+  // next level and SetGhost) have changed; the ride carries only the updates
+  // that read one of those, the others would write the values their
+  // destinations hold.
+  RideGhost ride_cgl_loop;
   struct DRide {
     DGhost rest, ops;   // grid-stride, per tile
-  } d_ride_cgl, d_ride_corr;
+  } d_ride_cgl, d_ride_corr, d_ride_cgl_loop;
   long long cells = 0;
   // Isolated patches (IsoPatch): their whole smoothing runs in one launch out
   // of LDS. The level's other patches keep the per-colour launches over
@@ -394,7 +402,7 @@ void bsam_ctx::free_device() {
       FG(l.d_pops[k]);
       FG(l.d_prest[k]);
     }
-    for (LevelData::DRide *r : {&l.d_ride_cgl, &l.d_ride_corr}) {
+    for (LevelData::DRide *r : {&l.d_ride_cgl, &l.d_ride_corr, &l.d_ride_cgl_loop}) {
       FG(r->rest);
       FG(r->ops);
     }
@@ -1058,6 +1066,9 @@ static void commit(bsam_ctx *c) {
           if (c->hp[p].mthbc[fc] == 2)
             throw Error("groups: periodic boundaries would join the groups' roots");
   }
+  // per level l: the parent cells under level l's patches (and the same of
+  // level l - 1), marked over the q pool
+  std::vector<unsigned char> under, under_prev;
   for (int l = minl; l <= c->finest; l++) {
     LevelData &L = c->lev[l - minl];
     build_tiles(c, L, l);
@@ -1142,7 +1153,9 @@ static void commit(bsam_ctx *c) {
     if (l > minl) {
       // parent cells under this level's patches: what its restriction and
       // fill-down write
-      std::vector<unsigned char> written((size_t)c->nq, 0);
+      under_prev.swap(under);
+      std::vector<unsigned char> &written = under;
+      written.assign((size_t)c->nq, 0);
       for (int p : L.patches) {
         const HPatch &h = c->hp[p];
         const HPatch &pa = c->hp[h.parent];
@@ -1158,6 +1171,36 @@ static void commit(bsam_ctx *c) {
       if (c->fuse_ghost && destination_index(PL.gp[0], at))
         L.ride_cgl = plan_ride(PL.gp[0],
                                coarse_ghost_load_reads(c->hp, L.t_cgl, L.n_cgl_res, nd, &at));
+      L.ride_cgl_loop = RideGhost();
+      if (c->fuse_ghost && l - 1 >= 0) {
+        // cells changed since the loop's FillDown applied SetGhost(l - 1, 0):
+        // level l - 1 under level l (this V-cycle's restriction: `written`),
+        // and what FillDown wrote after that program, level l - 2 under
+        // l - 1 (the previous level's mask) and, when l - 2 >= 0, level l - 2's
+        // ghost cells (its SetGhost(l - 2, 0))
+        std::vector<unsigned char> later;
+        if (l - 2 >= minl) later = under_prev;
+        if (l - 2 >= 0)
+          for (const GhostOp &g : c->lev[l - 2 - minl].gp[0]) {
+            if ((size_t)g.dst >= later.size()) later.resize((size_t)c->nq, 0);
+            later[(size_t)g.dst] = 1;
+          }
+        std::vector<GhostOp> sub;
+        long long src[6];
+        auto hit = [&](const std::vector<unsigned char> &m, long long o) {
+          return o >= 0 && o < (long long)m.size() && m[(size_t)o];
+        };
+        for (const GhostOp &g : PL.gp[0]) {
+          const int n = ghost_sources(g, src);
+          bool changed = false;
+          for (int k = 0; k < n && !changed; k++)
+            changed = hit(written, src[k]) || hit(later, src[k]);
+          if (changed) sub.push_back(g);
+        }
+        if (destination_index(sub, at))
+          L.ride_cgl_loop = plan_ride(sub, coarse_ghost_load_reads(c->hp, L.t_cgl, L.n_cgl_res,
+                                                                   nd, &at));
+      }
       if (c->fuse_ghost && destination_index(PL.gp[2], at))
         L.ride_corr = plan_ride(PL.gp[2], correct_reads(c->hp, L.t_blk, nd, &at));
     }
@@ -1211,6 +1254,7 @@ static void commit(bsam_ctx *c) {
     };
     up_ride(L.ride_cgl, L.d_ride_cgl);
     up_ride(L.ride_corr, L.d_ride_corr);
+    up_ride(L.ride_cgl_loop, L.d_ride_cgl_loop);
     L.d_relax = dev_upload(L.t_relax, s);
     if (!L.iso.empty()) {
       L.d_iso = dev_upload(L.iso, s);
@@ -1371,22 +1415,29 @@ static void op_coarse_load(bsam_ctx *c, int level, int prec = PREC_SOLUTION_FP64
 // launch, its SetGhost(2) in the correction launch); otherwise it is launched
 // first.
 static const LevelData::DRide *parent_ride(bsam_ctx *c, LevelData &L, int parent_ghost,
-                                           int level, int prec, bool correct) {
+                                           int level, int prec, bool correct,
+                                           bool loop = false) {
   if (parent_ghost < 0) return nullptr;
   if (parent_ghost >= kRestOnly) {
     op_set_ghost_rest(c, level - 1, prec);
     return nullptr;
   }
+  if (loop && !correct && parent_ghost == 0 && prec == PREC_SOLUTION_FP64 && c->fuse_ghost &&
+      L.ride_cgl_loop.ok)
+    return &L.d_ride_cgl_loop;
   const RideGhost &r = correct ? L.ride_corr : L.ride_cgl;
   if (c->fuse_ghost && r.ok && parent_ghost == (correct ? 2 : 0))
     return correct ? &L.d_ride_corr : &L.d_ride_cgl;
   op_set_ghost(c, level - 1, parent_ghost, prec);
   return nullptr;
 }
+// With `loop`, the call comes from the FP64 multigrid loop's V-cycle, whose
+// FillDown has applied the parent's SetGhost(0) since the parent last changed
+// outside this descent (ride_cgl_loop).
 static void op_coarse_ghost_load(bsam_ctx *c, int level, int prec = PREC_SOLUTION_FP64,
-                                 int parent_ghost = -1) {
+                                 int parent_ghost = -1, bool loop = false) {
   LevelData &L = child_level(c, level);
-  const LevelData::DRide *r = parent_ride(c, L, parent_ghost, level, prec, false);
+  const LevelData::DRide *r = parent_ride(c, L, parent_ghost, level, prec, false, loop);
   ProfScope ps(c, BSAM_PROF_COARSE_LOAD, 40.0 * L.cells);
   HIPCK(launch_coarse_ghost_load(c->nd(), prec, c->dev(), L.d_cgl, (int)L.t_cgl.size(),
                                  L.n_cgl_res,
@@ -1554,7 +1605,7 @@ static int level_relax(bsam_ctx *c, int level, int prec = PREC_SOLUTION_FP64,
 // its ipass when final_post let the last launch apply most of it, or -1)
 // instead of launched.
 static int op_vcycle(bsam_ctx *c, int level, int prec = PREC_SOLUTION_FP64,
-                     bool defer_final = false, bool final_post = false) {
+                     bool defer_final = false, bool final_post = false, bool loop = false) {
   if (level <= c->tail_top) {
     ProfScope ps(c, BSAM_PROF_TAIL, 0.0);
     int nlev = level - c->cfg.minlevel + 1, set = prec == PREC_SOLUTION_FP64 ? 0 : 1;
@@ -1574,8 +1625,8 @@ static int op_vcycle(bsam_ctx *c, int level, int prec = PREC_SOLUTION_FP64,
   // SetGhost(level - 1, 0) rides inside the coarse ghost + loading launch and
   // the parent's last SetGhost of post-smoothing inside the correction launch
   // (or each is launched just before, without a plan)
-  op_coarse_ghost_load(c, level, prec, 0);
-  int pp = op_vcycle(c, level - 1, prec, true, !(c->fuse_ghost && L.ride_corr.ok));
+  op_coarse_ghost_load(c, level, prec, 0, loop);
+  int pp = op_vcycle(c, level - 1, prec, true, !(c->fuse_ghost && L.ride_corr.ok), loop);
   op_correct(c, level, prec, pp);
   return level_relax(c, level, prec, true, defer_final, defer_final && final_post);
 }
@@ -1719,7 +1770,7 @@ static void multigrid_iterations(bsam_ctx *c, int *ncycles, double *hist, int ma
     // the fill-down runs the pending SetGhost(2) inside its launch when its
     // reads allow; otherwise the last relaxation launch takes most of it
     int pend = op_vcycle(c, c->finest, PREC_SOLUTION_FP64, true,
-                         !(c->fuse_ghost && c->L(c->finest).down_ok[2]));
+                         !(c->fuse_ghost && c->L(c->finest).down_ok[2]), true);
     op_fill_down(c, BSAM_FIELD_Q, pend, INT_MAX, 0);
     queue_error(c, c->finest, c->cfg.qerrortol);
   }, &done);
@@ -2953,6 +3004,36 @@ int bsam_parent_ghost_rides(const bsam_ctx *cc, int level, int which) {
   if (level <= c->cfg.minlevel || level > c->finest) return -1;
   const LevelData &L = c->L(level);
   return (which == 0 ? L.ride_cgl.ok : L.ride_corr.ok) ? 1 : 0;
+}
+
+int bsam_ride_loop_size(const bsam_ctx *cc, int level) {
+  bsam_ctx *c = const_cast<bsam_ctx *>(cc);
+  if (!c || !c->committed) return -1;
+  if (level <= c->cfg.minlevel || level > c->finest) return -1;
+  const RideGhost &r = c->L(level).ride_cgl_loop;
+  if (!r.ok) return -1;
+  std::vector<int> at;
+  std::vector<GhostOp> all(r.rest);
+  all.insert(all.end(), r.ops.begin(), r.ops.end());
+  std::sort(all.begin(), all.end(), [](const GhostOp &a, const GhostOp &b) { return a.dst < b.dst; });
+  int n = 0;
+  for (size_t k = 0; k < all.size(); k++) n += k == 0 || all[k].dst != all[k - 1].dst;
+  return n;
+}
+
+int bsam_ride_loop_apply_host(const bsam_ctx *cc, int level, double *const *q_patches) {
+  bsam_ctx *c = const_cast<bsam_ctx *>(cc);
+  API_BEGIN(c)
+  if (!c->committed) throw Error("hierarchy not committed");
+  if (level <= c->cfg.minlevel || level > c->finest) throw Error("level out of range");
+  const RideGhost &r = c->L(level).ride_cgl_loop;
+  if (!r.ok) throw Error("no loop ride on this level");
+  // the updates applied per tile and grid-stride (a per-tile update may be
+  // listed for several tiles; each copy writes the same value)
+  std::vector<GhostOp> all(r.rest);
+  all.insert(all.end(), r.ops.begin(), r.ops.end());
+  apply_ghost_program_host(c->hp, all, c->cfg.nrvars, q_patches);
+  API_END(c)
 }
 
 int bsam_ghost_rides_down(const bsam_ctx *cc, int level, int ipass) {

@@ -190,6 +190,8 @@ def lib():
         L.bsam_post_ghost_apply_host.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int,
                                                  C.POINTER(C.POINTER(C.c_double))]
         L.bsam_iso_patches.argtypes = [vp, C.c_int, ip, C.c_int]
+        L.bsam_ride_loop_size.argtypes = [vp, C.c_int]
+        L.bsam_ride_loop_apply_host.argtypes = [vp, C.c_int, C.POINTER(C.POINTER(C.c_double))]
         L.bsam_iso_program_apply_host.argtypes = [vp, C.c_int, C.c_int,
                                                   C.POINTER(C.POINTER(C.c_double))]
         _lib = L
@@ -507,6 +509,18 @@ class Engine:
             assert a.flags.f_contiguous and a.dtype == np.float64
             ptrs[n] = _dptr(a)
         self._ck(lib().bsam_iso_program_apply_host(self.ctx, level, ipass, ptrs))
+
+    def ride_loop_size(self, level: int) -> int:
+        """Updates of SetGhost(level - 1, 0) the loop's coarse loading of
+        `level` carries (-1: no such plan)."""
+        return int(lib().bsam_ride_loop_size(self.ctx, level))
+
+    def ride_loop_apply_host(self, level: int, qs: List[np.ndarray]):
+        ptrs = (C.POINTER(C.c_double) * len(qs))()
+        for n, a in enumerate(qs):
+            assert a.flags.f_contiguous and a.dtype == np.float64
+            ptrs[n] = _dptr(a)
+        self._ck(lib().bsam_ride_loop_apply_host(self.ctx, level, ptrs))
 
     def ghost_apply_host(self, level: int, ipass: int, qs: List[np.ndarray]):
         """Apply SetGhost(level, ipass)'s program to host q arrays in place."""

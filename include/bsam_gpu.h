@@ -430,6 +430,14 @@ int bsam_ghost_rides_down(const bsam_ctx *ctx, int level, int ipass);
  * correction launch (which 1); 0 if it is launched on its own; -1 for bad
  * arguments. No counterpart in the reference. */
 int bsam_parent_ghost_rides(const bsam_ctx *ctx, int level, int which);
+/* Inside the FP64 multigrid loop the coarse ghost + loading launch of level
+ * `level` (>= 1) carries only the updates of SetGhost(level - 1, 0) that read
+ * a cell changed since the loop's FillDown applied that program: level-1
+ * cells under the level (the restriction), level-2 cells under level - 1 or
+ * level-2 ghost cells (FillDown's next level and its SetGhost). Their number (-1 when there is no such plan) and their host
+ * application, for the CPU tests. No counterpart in the reference. */
+int bsam_ride_loop_size(const bsam_ctx *ctx, int level);
+int bsam_ride_loop_apply_host(const bsam_ctx *ctx, int level, double *const *q_patches);
 int bsam_fused_ghost_apply_host(const bsam_ctx *ctx, int level, int ipass, int color,
                                 double *const *q_patches);
 /* Applies on the host one part of a fused transition: part 0 the updates
