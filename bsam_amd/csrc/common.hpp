@@ -96,6 +96,13 @@ struct DPatch {
 // Offsets are of the first variable in the q pool; the second variable of
 // every cell is at the pool's variable stride, so the record carries none.
 // 32 bytes.
+//
+// The nflip byte also carries the offset a periodic copy adds to the value
+// it copies (PeriodicBC, PeriodicGhostOverlap, boundary.f90:417-596): bits
+// 0-3 count the "2 - x" maps, bits 4-5 hold the offset added before them:
+// 1 +qpo(n), 2 -qpo(n) (across the periodic x boundary), 3 +0.0 (the zero
+// qoffset PeriodicGhostOverlap adds for images shifted in y or z only, which
+// turns a -0.0 into +0.0), 0 none.
 struct GhostOp {
   int dst;                 // q-pool offset (variable 1)
   unsigned char kind, nflip;
@@ -155,10 +162,10 @@ struct IsoRec {
 static_assert(sizeof(IsoRec) == 16, "isolated-patch update record: 16 bytes");
 // The 8-byte form of a copy or a kind-1 interpolation whose indices are below
 // 2^13: dst, s[0], s[1], s[2] in 13 bits each from bit 0, the kind (0 or 1)
-// at bit 52, nflip in bits 53-56.
+// at bit 52, the nflip byte's six bits in bits 53-58.
 constexpr int ISO_SHORT_BITS = 13;
 __host__ __device__ __forceinline__ bool iso_short(const IsoRec &r, unsigned long long &out) {
-  if (r.kind > 1 || r.nflip > 15) return false;
+  if (r.kind > 1 || r.nflip > 63) return false;
   const int n = r.kind == 0 ? 1 : 3;
   if (r.dst >= (1u << ISO_SHORT_BITS)) return false;
   out = r.dst;
@@ -174,7 +181,7 @@ __host__ __device__ __forceinline__ IsoRec iso_unshort(unsigned long long w) {
   const unsigned m = (1u << ISO_SHORT_BITS) - 1;
   r.dst = (unsigned short)(w & m);
   r.kind = (unsigned char)((w >> 52) & 1);
-  r.nflip = (unsigned char)((w >> 53) & 15);
+  r.nflip = (unsigned char)((w >> 53) & 63);
   for (int k = 0; k < 3; k++) r.s[k] = (unsigned short)((w >> (ISO_SHORT_BITS * (k + 1))) & m);
   return r;
 }
@@ -254,7 +261,27 @@ struct TailCopy {
 struct Problem {
   double dt, eps2;
   double fc_alpha, fc_shift, fe_num, fe_den;
+  double qpo[2];   // quasi-periodic offsets per variable (0: periodic)
 };
+
+// The parts of a ghost update's nflip byte (GhostOp): the number of "2 - x"
+// maps and the periodic offset code.
+__host__ __device__ __forceinline__ int flip_count(int b) { return b & 15; }
+__host__ __device__ __forceinline__ int qpo_code(int b) { return (b >> 4) & 3; }
+__host__ __device__ __forceinline__ int flip_byte(int nflip, int code) {
+  return nflip | code << 4;
+}
+// A value with its periodic offset code applied: the reference's
+// `q + qpo(n)`, `q - qpo(n)` or `q + 0.0`.
+template <class T>
+__host__ __device__ __forceinline__ T add_qpo(T val, int code, const double *qpo, int v) {
+  switch (code) {
+    case 1: return val + (T)qpo[v];
+    case 2: return val - (T)qpo[v];
+    case 3: return val + (T)0;
+    default: return val;
+  }
+}
 
 // Kernel argument bundle: pool base pointers plus the patch table. The FP64
 // pools hold the solution; the FP32 pools (mixed precision only) hold the

@@ -176,7 +176,7 @@ constexpr int kRestOnly = 8;
 
 // The SetGhost programs of a single-patch level with offsets relative to the
 // patch's q origin, for the tail kernels; f1 carries the source packed with
-// the flip count (f0 | nflip << 28). Returns false unless every update is a
+// the nflip byte (f0 | nflip << 26). Returns false unless every update is a
 // copy within the patch (physical and periodic boundaries of a level that
 // covers the domain).
 bool local_programs(const HPatch &h, const std::vector<GhostOp> (&gp)[3], int nv,
@@ -186,13 +186,13 @@ bool local_programs(const HPatch &h, const std::vector<GhostOp> (&gp)[3], int nv
     out[ip].clear();
     for (GhostOp g : gp[ip]) {
       long long dst = g.dst - h.qoff, src = g.f0 - h.qoff;
-      if (g.kind != 0 || dst < 0 || dst >= n || src < 0 || src >= n || src >= (1 << 28) ||
-          g.nflip > 15)
+      if (g.kind != 0 || dst < 0 || dst >= n || src < 0 || src >= n || src >= (1 << 26) ||
+          g.nflip > 63)
         return false;
       (void)nv;
       g.dst = (int)dst;
       g.f0 = (int)src;
-      g.f1 = (int)src | (g.nflip << 28);
+      g.f1 = (int)((unsigned)src | (unsigned)g.nflip << 26);
       out[ip].push_back(g);
     }
   }
@@ -212,6 +212,7 @@ struct bsam_ctx {
   hipStream_t side = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   int iso_side = 1;
+  double qpo[2] = {0.0, 0.0};   // quasi-periodic offsets (bsam_set_qpo)
   bool building = false, committed = false;
   bool had_commit = false;    // hp/dp/lev/device state describe a committed hierarchy
   bool planned_fuse = false;  // that commit built the fusion and riding plans
@@ -324,7 +325,8 @@ struct bsam_ctx {
     d.qrte = qrte; d.eflags = eflags;
     d.P = d_P;
     d.gstate = d_state;
-    d.prm = Problem{cfg.dt, cfg.eps2, cfg.fc_alpha, cfg.fc_shift, cfg.fe_num, cfg.fe_den};
+    d.prm = Problem{cfg.dt, cfg.eps2, cfg.fc_alpha, cfg.fc_shift, cfg.fe_num, cfg.fe_den,
+                    {qpo[0], qpo[1]}};
     d.nrvars = cfg.nrvars;
     d.qvs = qstride;
     d.tile_lx = tile_lx;
@@ -2522,6 +2524,14 @@ int bsam_set_dt(bsam_ctx *c, double dt) {
   API_END(c)
 }
 
+int bsam_set_qpo(bsam_ctx *c, const double *qpo, int n) {
+  API_BEGIN(c)
+  if (n != c->cfg.nrvars || n > 2 || !qpo) throw Error("set_qpo: one offset per variable");
+  for (int v = 0; v < n; v++) c->qpo[v] = qpo[v];
+  c->drop_graphs();   // the captured launches hold the problem constants
+  API_END(c)
+}
+
 int bsam_transfer_values(bsam_ctx *dst, const bsam_ctx *src, int level) {
   API_BEGIN(dst)
   copy_level_boxes(dst, src, level, false, "transfer");
@@ -2965,7 +2975,7 @@ int bsam_ghost_program_apply_host(const bsam_ctx *cc, int level, int ipass,
   API_BEGIN(c)
   if (!c->committed) throw Error("hierarchy not committed");
   if (ipass < 0 || ipass > 2) throw Error("bad ipass");
-  apply_ghost_program_host(c->hp, c->L(level).gp[ipass], c->cfg.nrvars, q_patches);
+  apply_ghost_program_host(c->hp, c->L(level).gp[ipass], c->cfg.nrvars, q_patches, c->qpo);
   API_END(c)
 }
 
@@ -3032,7 +3042,7 @@ int bsam_ride_loop_apply_host(const bsam_ctx *cc, int level, double *const *q_pa
   // listed for several tiles; each copy writes the same value)
   std::vector<GhostOp> all(r.rest);
   all.insert(all.end(), r.ops.begin(), r.ops.end());
-  apply_ghost_program_host(c->hp, all, c->cfg.nrvars, q_patches);
+  apply_ghost_program_host(c->hp, all, c->cfg.nrvars, q_patches, c->qpo);
   API_END(c)
 }
 
@@ -3054,7 +3064,7 @@ int bsam_fused_ghost_part_apply_host(const bsam_ctx *cc, int level, int ipass, i
   const FusedGhost &g = c->L(level).fused[f];
   if (!g.ok) throw Error("this transition is not fused on this level");
   if (part != 0 && part != 1) throw Error("part must be 0 (in the launch) or 1 (before it)");
-  apply_ghost_program_host(c->hp, part == 0 ? g.ops : g.pre, c->cfg.nrvars, q_patches);
+  apply_ghost_program_host(c->hp, part == 0 ? g.ops : g.pre, c->cfg.nrvars, q_patches, c->qpo);
   API_END(c)
 }
 
@@ -3069,7 +3079,7 @@ int bsam_post_ghost_apply_host(const bsam_ctx *cc, int level, int pending, int c
                           : post_key(pending, color);
   if (k < 0 || !c->L(level).post[k].ok) throw Error("this launch has no post plan");
   const PostGhost &g = c->L(level).post[k];
-  apply_ghost_program_host(c->hp, part % 2 == 0 ? g.ops : g.rest, c->cfg.nrvars, q_patches);
+  apply_ghost_program_host(c->hp, part % 2 == 0 ? g.ops : g.rest, c->cfg.nrvars, q_patches, c->qpo);
   API_END(c)
 }
 
@@ -3118,7 +3128,7 @@ int bsam_iso_program_apply_host(const bsam_ctx *cc, int level, int ipass,
       ops.push_back(g);
     }
   }
-  apply_ghost_program_host(c->hp, ops, c->cfg.nrvars, q_patches);
+  apply_ghost_program_host(c->hp, ops, c->cfg.nrvars, q_patches, c->qpo);
   API_END(c)
 }
 
@@ -3135,7 +3145,7 @@ int bsam_fused_ghost_apply_host(const bsam_ctx *cc, int level, int ipass, int co
   // (no update reads what another writes, so one combined application)
   std::vector<GhostOp> all(g.pre);
   all.insert(all.end(), g.ops.begin(), g.ops.end());
-  apply_ghost_program_host(c->hp, all, c->cfg.nrvars, q_patches);
+  apply_ghost_program_host(c->hp, all, c->cfg.nrvars, q_patches, c->qpo);
   API_END(c)
 }
 

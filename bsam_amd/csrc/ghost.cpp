@@ -36,6 +36,7 @@ struct Expr {
   Cell f0, f1;
   Cell c[4];
   int nflip = 0;
+  int qcode = 0;  // periodic offset added before the maps (flip_byte's code)
 };
 
 // A Fortran array-section subscript: start, stop, stride (scalars have
@@ -518,9 +519,10 @@ struct Builder {
     return e;
   }
 
-  // PeriodicBC (boundary.f90:417-486, qpo = 0): a patch periodic on both
-  // sides of a dimension wraps itself, dimensions in order, over the full
-  // ghost-extended tangential range.
+  // PeriodicBC (boundary.f90:417-486): a patch periodic on both sides of a
+  // dimension wraps itself, dimensions in order, over the full
+  // ghost-extended tangential range; the wrap in x adds +qpo to the high
+  // ghosts and -qpo to the low ones.
   void periodic_self(PatchMap &pm) {
     const HPatch &h = hp[pm.p];
     for (int d = 0; d < nd; d++) {
@@ -535,8 +537,20 @@ struct Builder {
           lo[d] = 0; slo[d] = h.mx[d];
           hi[ta] = shi[ta] = lo[ta] = slo[ta] = a;
           hi[tb] = shi[tb] = lo[tb] = slo[tb] = b;
-          upd.push_back({key(h, hi[0], hi[1], hi[2]), source(pm, h, shi[0], shi[1], shi[2])});
-          upd.push_back({key(h, lo[0], lo[1], lo[2]), source(pm, h, slo[0], slo[1], slo[2])});
+          Expr ehi = source(pm, h, shi[0], shi[1], shi[2]);
+          Expr elo = source(pm, h, slo[0], slo[1], slo[2]);
+          if (d == 0) {
+            // the sources are interior cells or ghosts set by the
+            // interpolation phase: no offset or map yet
+            if (ehi.qcode || elo.qcode || ehi.nflip || elo.nflip) {
+              ok = false;
+              msg = "ghost program: a periodic offset over another offset or map";
+            }
+            ehi.qcode = 1;   // + qpo(n)
+            elo.qcode = 2;   // - qpo(n)
+          }
+          upd.push_back({key(h, hi[0], hi[1], hi[2]), ehi});
+          upd.push_back({key(h, lo[0], lo[1], lo[2]), elo});
         }
       for (auto &u : upd) pm.m[u.first] = u.second;
     }
@@ -544,7 +558,9 @@ struct Builder {
 
   // PeriodicGhostOverlap (boundary.f90:510-596) for the ordered pair
   // (g1, g2): every periodic image of g2 exchanges its ghost-extended overlap
-  // with g1 both ways; all sources are interior cells.
+  // with g1 both ways; all sources are interior cells. The copies into g1
+  // add qoffset (GetPeriodicOffsets, :965-1044: +qpo for an image shifted by
+  // +x, -qpo for -x, none otherwise), those into g2 subtract it.
   void periodic_pair(PatchMap &m1, PatchMap &m2, int level, const Periodic &per) {
     const HPatch &g1 = hp[m1.p], &g2 = hp[m2.p];
     static const int tab[13][3] = {{1, 0, 0},  {0, 1, 0},  {0, 0, 1},  {1, 1, 0},  {-1, 1, 0},
@@ -564,23 +580,29 @@ struct Builder {
         lo2[n] = std::max(o0, a1[n]); hi2[n] = std::min(o1, a2[n]);
       }
       if (empty) continue;
+      // into g1: + qoffset (+qpo, -qpo, or +0.0 for an image not shifted in
+      // x); into g2: - qoffset (-qpo, +qpo, or - 0.0, which leaves every
+      // value as it is)
+      const int sx = tab[per.idx[o] - 1][0];
+      const int code1 = sx > 0 ? 1 : sx < 0 ? 2 : 3, code2 = sx > 0 ? 2 : sx < 0 ? 1 : 0;
       auto put_copy = [&](PatchMap &dm, const HPatch &dh, int di, int dj, int dk, int sp,
-                          int si, int sj, int sk) {
+                          int si, int sj, int sk, int code) {
         Expr e;
         e.kind = 0;
         e.f0 = Cell{sp, si, sj, sk};
+        e.qcode = code;
         dm.m[key(dh, di, dj, dk)] = e;
       };
       for (int k = lo1[2]; k <= hi1[2]; k++)
         for (int j = lo1[1]; j <= hi1[1]; j++)
           for (int i = lo1[0]; i <= hi1[0]; i++)
             put_copy(m1, g1, i - a1[0] + 1, j - a1[1] + 1, nd == 3 ? k - a1[2] + 1 : 0, m2.p,
-                     i - b1[0] + 1, j - b1[1] + 1, nd == 3 ? k - b1[2] + 1 : 0);
+                     i - b1[0] + 1, j - b1[1] + 1, nd == 3 ? k - b1[2] + 1 : 0, code1);
       for (int k = lo2[2]; k <= hi2[2]; k++)
         for (int j = lo2[1]; j <= hi2[1]; j++)
           for (int i = lo2[0]; i <= hi2[0]; i++)
             put_copy(m2, g2, i - b1[0] + 1, j - b1[1] + 1, nd == 3 ? k - b1[2] + 1 : 0, m1.p,
-                     i - a1[0] + 1, j - a1[1] + 1, nd == 3 ? k - a1[2] + 1 : 0);
+                     i - a1[0] + 1, j - a1[1] + 1, nd == 3 ? k - a1[2] + 1 : 0, code2);
     }
   }
 
@@ -680,6 +702,7 @@ int build_ghost_program(const std::vector<HPatch> &hp, const std::vector<int> &p
   }
   if (per.noff > 0) {
     for (auto &pm : maps) B.periodic_self(pm);
+    if (!B.ok) return -1;
     if (level > 0)
       for (size_t a = 0; a < maps.size(); a++)
         for (size_t b = a + 1; b < maps.size(); b++) {
@@ -700,13 +723,13 @@ int build_ghost_program(const std::vector<HPatch> &hp, const std::vector<int> &p
       }
       GhostOp g;
       memset(&g, 0, sizeof g);
-      if (e.nflip < 0 || e.nflip > 255) {
+      if (e.nflip < 0 || e.nflip > 15) {
         msg = "ghost program: too many boundary maps on one ghost";
         return -1;
       }
       g.dst = (int)qoff(hp, dc);
       g.kind = (unsigned char)e.kind;
-      g.nflip = (unsigned char)e.nflip;
+      g.nflip = (unsigned char)flip_byte(e.nflip, e.qcode);
       g.f0 = (int)qoff(hp, e.f0);
       if (e.kind > 0) {
         g.f1 = (int)qoff(hp, e.f1);
@@ -1366,7 +1389,7 @@ void locate_q_cell(const std::vector<HPatch> &hp, long long off, int &p, int &i,
 }
 
 void apply_ghost_program_host(const std::vector<HPatch> &hp, const std::vector<GhostOp> &ops,
-                              int nrvars, double *const *q_patches) {
+                              int nrvars, double *const *q_patches, const double *qpo) {
   // a device offset's element in the patch's reference-layout host array,
   // and that array's variable stride (the device's may hold padding)
   auto host_index = [&](long long off, int &p) {
@@ -1406,7 +1429,8 @@ void apply_ghost_program_host(const std::vector<HPatch> &hp, const std::vector<G
                          BSAM_C3 * val(g.c[2], v) + BSAM_C4 * val(g.c[3], v));
           break;
       }
-      for (int r = 0; r < g.nflip; r++) x = 2.0 - x;
+      x = add_qpo<double>(x, qpo_code(g.nflip), qpo, v);
+      for (int r = 0; r < flip_count(g.nflip); r++) x = 2.0 - x;
       res[e * nrvars + v] = x;
     }
   }

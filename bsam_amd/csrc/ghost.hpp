@@ -172,6 +172,6 @@ void locate_q_cell(const std::vector<HPatch> &hp, long long off, int &p, int &i,
 // Evaluates a program on host arrays (one reference-layout q array per patch),
 // reading the state before the program as the device kernel does.
 void apply_ghost_program_host(const std::vector<HPatch> &hp, const std::vector<GhostOp> &ops,
-                              int nrvars, double *const *q_patches);
+                              int nrvars, double *const *q_patches, const double *qpo);
 
 }  // namespace bsam

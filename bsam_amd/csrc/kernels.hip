@@ -259,11 +259,12 @@ __device__ __forceinline__ T restrict_f(const T *x, const DPatch &P, int ci, int
 // Interpolate2D/3D over two fine sources (0, 1) and one, three or four
 // coarse sources (2..5), with the weights of lib/boundary.f90:45-54. A
 // boundary "2 - x" map is affine; in the defect form only its linear part,
-// -x, applies (nflip times). src(n) loads source n; every caller (global
-// pools, short records, an isolated patch's LDS) evaluates the same
+// -x, applies (flip_count(fb) times), after the periodic offset of code
+// qpo_code(fb) for variable v (add_qpo). src(n) loads source n; every caller
+// (global pools, short records, an isolated patch's LDS) evaluates the same
 // expression tree.
 template <class T, bool DEF, class Src>
-__device__ __forceinline__ T ghost_value(int kind, int nflip, Src src) {
+__device__ __forceinline__ T ghost_value(int kind, int fb, Src src, const double *qpo, int v) {
   T val;
   switch (kind) {
     case 0:
@@ -282,29 +283,33 @@ __device__ __forceinline__ T ghost_value(int kind, int nflip, Src src) {
                           (T)BSAM_C4 * src(5));
       break;
   }
-  for (int r = 0; r < nflip; r++) val = DEF ? -val : (T)2 - val;
+  // the periodic offset, before the maps (a constant: none on a correction)
+  if (!DEF) val = add_qpo<T>(val, qpo_code(fb), qpo, v);
+  for (int r = 0; r < flip_count(fb); r++) val = DEF ? -val : (T)2 - val;
   return val;
 }
 // One ghost-cell update of a SetGhost program, offsets relative to q and
 // variables vs apart.
 template <class T, bool DEF>
-__device__ __forceinline__ void ghost_op(int nrvars, T *q, long long vs, const GhostOp &g) {
+__device__ __forceinline__ void ghost_op(int nrvars, T *q, long long vs, const GhostOp &g,
+                                         const double *qpo) {
   for (int v = 0; v < nrvars; v++) {
     const T *f = q + (long long)v * vs;
     const T val = ghost_value<T, DEF>(g.kind, g.nflip, [&](int n) {
       return f[n == 0 ? g.f0 : n == 1 ? g.f1 : g.c[n - 2]];
-    });
+    }, qpo, v);
     q[g.dst + (long long)v * vs] = val;
   }
 }
 // The same update from its short record (kinds 0 and 1).
 template <class T, bool DEF>
-__device__ __forceinline__ void ghost_op_s(int nrvars, T *q, long long vs, const GhostS &g) {
+__device__ __forceinline__ void ghost_op_s(int nrvars, T *q, long long vs, const GhostS &g,
+                                           const double *qpo) {
   for (int v = 0; v < nrvars; v++) {
     const T *f = q + (long long)v * vs;
     const T val = ghost_value<T, DEF>(g.kind, g.nflip, [&](int n) {
       return f[g.kind == 0 || n == 2 ? g.src : g.dst + (n == 0 ? g.d0 : g.d1)];
-    });
+    }, qpo, v);
     q[g.dst + (long long)v * vs] = val;
   }
 }
@@ -761,9 +766,9 @@ __device__ __forceinline__ void ghost_tile(const Dev &d, const GhostList &g, int
     T *q = fields<T>(d).q;
     constexpr bool DEF = std::is_same<T, float>::value;
     for (int e = s0 + (int)threadIdx.x; e < s1; e += blockDim.x)
-      ghost_op_s<T, DEF>(d.nrvars, q, d.qvs, g.s[e]);
+      ghost_op_s<T, DEF>(d.nrvars, q, d.qvs, g.s[e], d.prm.qpo);
     for (int e = l0 + (int)threadIdx.x; e < l1; e += blockDim.x)
-      ghost_op<T, DEF>(d.nrvars, q, d.qvs, g.l[e]);
+      ghost_op<T, DEF>(d.nrvars, q, d.qvs, g.l[e], d.prm.qpo);
     __syncthreads();
   }
 }
@@ -775,9 +780,9 @@ __device__ __forceinline__ void ghost_alongside(const Dev &d, const GhostList &g
   constexpr bool DEF = std::is_same<T, float>::value;
   const int stride = gridDim.x * blockDim.x, e0 = blockIdx.x * blockDim.x + threadIdx.x;
   for (int e = e0; e < g.ns; e += stride)
-    if (!stopped(d, g.s[e].group)) ghost_op_s<T, DEF>(d.nrvars, q, d.qvs, g.s[e]);
+    if (!stopped(d, g.s[e].group)) ghost_op_s<T, DEF>(d.nrvars, q, d.qvs, g.s[e], d.prm.qpo);
   for (int e = e0; e < g.nl; e += stride)
-    if (!stopped(d, g.l[e].group)) ghost_op<T, DEF>(d.nrvars, q, d.qvs, g.l[e]);
+    if (!stopped(d, g.l[e].group)) ghost_op<T, DEF>(d.nrvars, q, d.qvs, g.l[e], d.prm.qpo);
 }
 
 // SetGhost(level, ipass) followed by the relaxation of colour `color`, in one
@@ -939,12 +944,13 @@ __global__ __launch_bounds__(ISO_THREADS) void k_smooth_iso(Dev d, const IsoPatc
 #pragma unroll
     for (int v = 0; v < 2; v++) {
       const T *b = box + v * W;
-      box[v * W + dst] = ghost_value<T, DEF>(kind, nflip, [&](int n) { return b[src(n)]; });
+      box[v * W + dst] =
+          ghost_value<T, DEF>(kind, nflip, [&](int n) { return b[src(n)]; }, d.prm.qpo, v);
     }
   };
   auto apply_s = [&](const unsigned long long w) __attribute__((always_inline)) {
     constexpr unsigned m = (1u << ISO_SHORT_BITS) - 1;
-    put((unsigned)w & m, (int)(w >> 52) & 1, (int)(w >> 53) & 15,
+    put((unsigned)w & m, (int)(w >> 52) & 1, (int)(w >> 53) & 63,
         [&](int n) { return (unsigned)(w >> (ISO_SHORT_BITS * (n + 1))) & m; });
   };
   auto apply_l = [&](const uint4 g) __attribute__((always_inline)) {
@@ -1012,10 +1018,10 @@ template <class T, bool DEF>
 __global__ __launch_bounds__(256) void k_ghost(Dev d, GhostList g) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e < g.ns) {
-    if (!stopped(d, g.s[e].group)) ghost_op_s<T, DEF>(d.nrvars, fields<T>(d).q, d.qvs, g.s[e]);
+    if (!stopped(d, g.s[e].group)) ghost_op_s<T, DEF>(d.nrvars, fields<T>(d).q, d.qvs, g.s[e], d.prm.qpo);
   } else if (e - g.ns < g.nl) {
     const GhostOp &u = g.l[e - g.ns];
-    if (!stopped(d, u.group)) ghost_op<T, DEF>(d.nrvars, fields<T>(d).q, d.qvs, u);
+    if (!stopped(d, u.group)) ghost_op<T, DEF>(d.nrvars, fields<T>(d).q, d.qvs, u, d.prm.qpo);
   }
 }
 
@@ -1610,7 +1616,7 @@ __device__ void tail_stage(const TailCopy *tab, int nchunks) {
 }
 
 // One SetGhost program of a tail level, this thread's share of its updates
-// held in registers as loaded (destination, and source | nflip << 28, which
+// held in registers as loaded (destination, and source | nflip byte << 26, which
 // the host stores in f1), so that the loads stay in flight until the program
 // is applied.
 struct TailPass {
@@ -1643,24 +1649,24 @@ __device__ __forceinline__ void tail_ghost_load(const TailLevel &t, TailGhost &g
   tail_pass_load(t.ngp[1], t.gp[1], g.p1);
   tail_pass_load(t.ngp[2], t.gp[2], g.p2);
 }
-// SetGhost on a tail level: patch-local copies with "2 - x" maps (the
-// ghost_op kind 0 update).
+// SetGhost on a tail level: patch-local copies with "2 - x" maps and
+// quasi-periodic offsets (the ghost_op kind 0 update).
 template <class T, bool DEF>
-__device__ __forceinline__ void tail_ghost(int nrvars, const Lvl<T> &L, const TailPass &g) {
+__device__ __forceinline__ void tail_ghost(int nrvars, const Lvl<T> &L, const TailPass &g,
+                                           const double *qpo) {
   const long long vs = L.P.qvs;
 #pragma unroll
   for (int r = 0; r < TAIL_GR; r++) {
     if ((int)(threadIdx.x + r * blockDim.x) < g.n) {
-      int src = g.srcf[r] & 0x0fffffff, nflip = (unsigned)g.srcf[r] >> 28;
+      const int src = g.srcf[r] & 0x03ffffff, fb = (int)((unsigned)g.srcf[r] >> 26);
       for (int v = 0; v < nrvars; v++) {
-        T val = L.q[src + v * vs];
-        for (int x = 0; x < nflip; x++) val = DEF ? -val : (T)2 - val;
+        const T val = ghost_value<T, DEF>(0, fb, [&](int) { return L.q[src + v * vs]; }, qpo, v);
         L.q[g.dst[r] + v * vs] = val;
       }
     }
   }
   for (int e = threadIdx.x + TAIL_GR * blockDim.x; e < g.n; e += blockDim.x)
-    ghost_op<T, DEF>(nrvars, L.q, vs, g.ops[e]);
+    ghost_op<T, DEF>(nrvars, L.q, vs, g.ops[e], qpo);
   __syncthreads();
 }
 
@@ -1737,9 +1743,9 @@ __device__ void tail_relax(const Problem &prm, int nrvars, const Lvl<T> &L, bool
   };
   for (int pass = 0; pass < nsmooth; pass++) {
     half(0);
-    tail_ghost<T, DEF>(nrvars, L, g.p1);
+    tail_ghost<T, DEF>(nrvars, L, g.p1, prm.qpo);
     half(1);
-    tail_ghost<T, DEF>(nrvars, L, g.p2);
+    tail_ghost<T, DEF>(nrvars, L, g.p2, prm.qpo);
   }
 }
 
@@ -1758,7 +1764,7 @@ __device__ void tail_down(const Dev &d, const TailLevel *__restrict__ lv, int l,
   for_cells<ND>(P.cmx, P.cmy, cz, 1,
                 [&](int i, int j, int k) { restrict_cell<ND, T>(nv, C, PA, i, j, k); });
   __syncthreads();
-  tail_ghost<T, DEF>(nv, PA, gp);
+  tail_ghost<T, DEF>(nv, PA, gp, d.prm.qpo);
   for_cells<ND>(P.cmx + 2, P.cmy + 2, ND == 3 ? P.cmz + 2 : 1, 0,
                 [&](int i, int j, int k) { coarse_ghost_cell<ND, T>(nv, C, PA, i, j, k); });
   __syncthreads();
@@ -1777,7 +1783,7 @@ __device__ void tail_up(const Dev &d, const TailLevel *__restrict__ lv, int l, i
   for_cells<ND>(P.cmx + 1, P.cmy + 1, ND == 3 ? P.cmz + 1 : 1, 0,
                 [&](int m, int n, int o) { correct_block<ND, T>(nv, C, PA, m, n, o); });
   __syncthreads();
-  tail_ghost<T, DEF>(nv, C, gc.p0);
+  tail_ghost<T, DEF>(nv, C, gc.p0, d.prm.qpo);
   tail_relax<ND, T, DEF>(d.prm, nv, C, LC && lv[l].lds[set][2] >= 0, gc, nsmooth);
 }
 
@@ -1843,7 +1849,7 @@ __global__ __launch_bounds__(1024) void k_tail_filldown(Dev d, const TailLevel *
       filldown_cell<ND, double>(nv, C, PA, i, j, k, field);
     });
     __syncthreads();
-    if (field == 2) tail_ghost<double, false>(nv, PA, gp);
+    if (field == 2) tail_ghost<double, false>(nv, PA, gp, d.prm.qpo);
   }
 }
 

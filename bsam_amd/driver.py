@@ -161,8 +161,8 @@ class Driver:
         self.xlower = [float(v) for v in _arr(gd.get("xlower"), nd, 0.0)]
         self.xupper = [float(v) for v in _arr(gd.get("xupper"), nd, 1.0)]
         self.mthbc = [int(v) for v in _arr(gd.get("mthbc"), 2 * nd, 1)]
-        if any(float(v) != 0.0 for v in _arr(gd.get("qpo"), 2, 0.0)):
-            raise NotImplementedError("quasi-periodic offsets (qpo) other than 0")
+        # the quasi-periodic offsets of the periodic x copies (RootInit: 0)
+        self.qpo = [float(v) for v in _arr(gd.get("qpo"), 2, 0.0)]
         if int(gd.get("mbc", 1)) != 1 or int(gd.get("maux", 0)) != 0 or int(gd.get("nrvars", 1)) != 2:
             raise NotImplementedError("mbc = 1, maux = 0 and nrvars = 2 only")
         self.alpha = float(pd.get("alpha", 1.0))
@@ -197,7 +197,7 @@ class Driver:
                                       well="reference", nsmoothingpasses=self.nsmooth,
                                       maxvcycles=self.maxvcycles, qerrortol=self.qerrortol,
                                       device=self.device, tune=tune,
-                                      errortype=self.errortype)
+                                      errortype=self.errortype, qpo=self.qpo)
         else:
             self._engines[k].describe(layout)
             if self._engines[k].dt != self._dt_now:

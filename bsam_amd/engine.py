@@ -12,7 +12,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
-from typing import Dict, List, Optional
+from typing import Dict, List, Optional, Sequence
 
 import numpy as np
 
@@ -156,6 +156,7 @@ def lib():
         L.bsam_transfer_values.argtypes = [vp, vp, C.c_int]
         L.bsam_overlay_level.argtypes = [vp, vp, C.c_int]
         L.bsam_set_dt.argtypes = [vp, C.c_double]
+        L.bsam_set_qpo.argtypes = [vp, dp, C.c_int]
         L.bsam_regrid_level.argtypes = [C.POINTER(RegridParams), C.c_int, ip, ip,
                                         C.POINTER(ip), C.c_int, ip, ip, ip]
         L.bsam_after_step_stats.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
@@ -217,7 +218,8 @@ class Engine:
                  well: str = "reference", nsmoothingpasses: int = 2,
                  maxvcycles: int = 20, qerrortol: float = 1e-8,
                  precision: int = PREC_FP64, device: int = 0,
-                 tune: Optional[Dict[str, float]] = None, errortype: int = 1):
+                 tune: Optional[Dict[str, float]] = None, errortype: int = 1,
+                 qpo: Optional[Sequence[float]] = None):
         L = lib()
         fa, fs, fn, fd = WELLS[well](alpha)
         cfg = Cfg(layout.ndims, layout.nrvars, layout.mbc, layout.minlevel, dt, eps * eps,
@@ -231,6 +233,8 @@ class Engine:
             raise BsamError(L.bsam_last_error(None).decode())
         for key, val in (tune or {}).items():
             self._ck(L.bsam_tune(self.ctx, key.encode(), float(val)))
+        if qpo is not None:
+            self.set_qpo(qpo)
         self.describe(layout)
 
     def describe(self, layout: Layout):
@@ -328,6 +332,14 @@ class Engine:
         """The time step of the problem constants from now on."""
         self._ck(lib().bsam_set_dt(self.ctx, float(dt)))
         self.dt = float(dt)
+
+    def set_qpo(self, qpo: Sequence[float]):
+        """The quasi-periodic offsets qpo(1:nrvars) (nodeinfodef.f90:101):
+        added to values copied across the periodic x boundary, +qpo(n) to
+        the high side, -qpo(n) to the low side (boundary.f90:441-461,
+        :578-591)."""
+        vals = np.ascontiguousarray(qpo, dtype=np.float64)
+        self._ck(lib().bsam_set_qpo(self.ctx, _dptr(vals), int(vals.size)))
 
     def overlay_level_from(self, src: "Engine", level: int):
         """CopyPatchToUniformGrid over `level`: src's patches, ghosts

@@ -63,12 +63,7 @@ contains
     use ProblemDef
     integer, intent(in) :: device, precision
     type(bsam_cfg) :: cfg
-    if (any(qpo(1:2) /= 0.0)) then
-      ! PeriodicBC's quasi-periodic offset (boundary.f90:432-461) is not
-      ! part of the engine's ghost programs
-      print *, 'AFASGPU: quasi-periodic offsets (qpo) other than 0 are not supported'
-      stop 1
-    end if
+    real(c_double), dimension(2) :: qpo2
     cfg%ndims = ndims
     cfg%nrvars = 2
     cfg%mbc = 1
@@ -91,6 +86,10 @@ contains
                trim(bsam_error_message(c_null_ptr))
       stop 1
     end if
+    ! PeriodicBC's and PeriodicGhostOverlap's quasi-periodic offsets
+    ! (boundary.f90:432-461, :578-591)
+    qpo2 = real(qpo(1:2), c_double)
+    call check(bsam_set_qpo(gpu_ctx, qpo2, 2_c_int), 'bsam_set_qpo')
   end subroutine GpuOpen
 
   ! The module's dt as it is now: TakeTimeSteps sets it to 0 for the first

@@ -157,6 +157,13 @@ module bsam_gpu
       real(c_double), value :: dt
       integer(c_int) :: rc
     end function
+    function bsam_set_qpo(ctx, qpo, n) bind(C, name='bsam_set_qpo') result(rc)
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), dimension(*), intent(in) :: qpo
+      integer(c_int), value :: n
+      integer(c_int) :: rc
+    end function
     function bsam_sync(ctx) bind(C, name='bsam_sync') result(rc)
       import :: c_ptr, c_int
       type(c_ptr), value :: ctx

@@ -274,6 +274,13 @@ int bsam_solve_status(const bsam_ctx *ctx);
  * changes under MultigridIterations, so the Fortran binding passes it on
  * each call. dt must be finite and >= 0. Waits for the context's stream. */
 int bsam_set_dt(bsam_ctx *ctx, double dt);
+/* The quasi-periodic offsets qpo(1:nrvars) (lib/nodeinfodef.f90:101; 0 by
+ * default, as RootInit sets them, lib/bsamroutines.f90:241): a value copied
+ * across the periodic x boundary gains +qpo(n) going to the high side and
+ * -qpo(n) going to the low side (PeriodicBC, PeriodicGhostOverlap with
+ * GetPeriodicOffsets' qoffset, lib/boundary.f90:417-596, :965-1044). n must be
+ * nrvars. Takes effect for the following SetGhost programs. */
+int bsam_set_qpo(bsam_ctx *ctx, const double *qpo, int n);
 int bsam_copy_levels(bsam_ctx *dst, const bsam_ctx *src, int top);
 int bsam_init_fields(bsam_ctx *ctx, int level);
 int bsam_transfer_values(bsam_ctx *dst, const bsam_ctx *src, int level);

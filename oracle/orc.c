@@ -187,6 +187,12 @@ void orc_set_problem(orc_hier *h, double dt, double eps2, double fc_alpha,
   h->fe_num = fe_num; h->fe_den = fe_den;
 }
 
+/* qpo(1:nrvars), lib/nodeinfodef.f90:101 (0 unless set) */
+void orc_set_qpo(orc_hier *h, double qpo1, double qpo2) {
+  h->qpo[0] = qpo1;
+  h->qpo[1] = qpo2;
+}
+
 void orc_set_solver(orc_hier *h, int nsmoothingpasses, int maxvcycles,
                     double qerrortol) {
   h->nsmoothingpasses = nsmoothingpasses; h->maxvcycles = maxvcycles;
@@ -885,9 +891,10 @@ static int set_bc_patch(orc_hier *h, orc_patch *p) {
   return 0;
 }
 
-/* PeriodicBC, lib/boundary.f90:417-486 (quasi-periodic offset qpo = 0): a
- * patch periodic on both sides of a dimension wraps its own interior into its
- * ghosts, dimensions in order, over the full ghost-extended range. */
+/* PeriodicBC, lib/boundary.f90:417-486: a patch periodic on both sides of a
+ * dimension wraps its own interior into its ghosts, dimensions in order, over
+ * the full ghost-extended range; in dimension 1 the high ghosts take
+ * +qpo(n), the low ones -qpo(n) (:441-442, :460-461). */
 static void periodic_bc(orc_hier *h, orc_patch *p) {
   int nd = h->ndims;
   for (int d = 0; d < nd; d++) {
@@ -904,13 +911,19 @@ static void periodic_bc(orc_hier *h, orc_patch *p) {
             lo[d] = 1 - ibc; slo[d] = p->mx[d] - ibc + 1;
             hi[ta] = lo[ta] = shi[ta] = slo[ta] = a;
             hi[tb] = lo[tb] = shi[tb] = slo[tb] = b;
-            Q(p, hi[0], hi[1], hi[2], v) = Q(p, shi[0], shi[1], shi[2], v);
-            Q(p, lo[0], lo[1], lo[2], v) = Q(p, slo[0], slo[1], slo[2], v);
+            if (d == 0) {
+              Q(p, hi[0], hi[1], hi[2], v) = Q(p, shi[0], shi[1], shi[2], v) + h->qpo[v - 1];
+              Q(p, lo[0], lo[1], lo[2], v) = Q(p, slo[0], slo[1], slo[2], v) - h->qpo[v - 1];
+            } else {
+              Q(p, hi[0], hi[1], hi[2], v) = Q(p, shi[0], shi[1], shi[2], v);
+              Q(p, lo[0], lo[1], lo[2], v) = Q(p, slo[0], slo[1], slo[2], v);
+            }
           }
   }
 }
 
-/* GetPeriodicOffsets, lib/boundary.f90:965-1044 (qoffset = 0 when qpo = 0). */
+/* GetPeriodicOffsets, lib/boundary.f90:965-1044: the offset vector of index
+ * idx; its qoffset is qpo times the sign of the offset's x component. */
 static void periodic_offset(const orc_hier *h, int level, int idx, int off[3]) {
   int W = h->rootmx[0] << level, H = h->rootmx[1] << level, D = h->rootmx[2] << level;
   static const int tab[13][3] = {{1, 0, 0},  {0, 1, 0},  {0, 0, 1},  {1, 1, 0},  {-1, 1, 0},
@@ -943,18 +956,24 @@ static void periodic_overlap(orc_hier *h, int level, orc_patch *g1, orc_patch *g
       o2[n][1] = ov[n][1] < a2[n] ? ov[n][1] : a2[n];
     }
     if (empty) continue;
-    for (int v = 1; v <= h->nrvars; v++)
+    /* qoffset(offset, n): +qpo(n) for an x offset > 0, -qpo(n) for < 0, else 0
+       (as a value: 0.0 when the x offset is 0, whatever qpo) */
+    for (int v = 1; v <= h->nrvars; v++) {
+      const double qo = off[0] > 0 ? h->qpo[v - 1] : off[0] < 0 ? -h->qpo[v - 1] : 0.0;
       for (int k = o1[2][0]; k <= o1[2][1]; k++)
         for (int j = o1[1][0]; j <= o1[1][1]; j++)
           for (int i = o1[0][0]; i <= o1[0][1]; i++)
             Q(g1, i - a1[0] + 1, j - a1[1] + 1, k - a1[2] + 1, v) =
-                Q(g2, i - b1[0] + 1, j - b1[1] + 1, k - b1[2] + 1, v);
-    for (int v = 1; v <= h->nrvars; v++)
+                Q(g2, i - b1[0] + 1, j - b1[1] + 1, k - b1[2] + 1, v) + qo;
+    }
+    for (int v = 1; v <= h->nrvars; v++) {
+      const double qo = off[0] > 0 ? h->qpo[v - 1] : off[0] < 0 ? -h->qpo[v - 1] : 0.0;
       for (int k = o2[2][0]; k <= o2[2][1]; k++)
         for (int j = o2[1][0]; j <= o2[1][1]; j++)
           for (int i = o2[0][0]; i <= o2[0][1]; i++)
             Q(g2, i - b1[0] + 1, j - b1[1] + 1, k - b1[2] + 1, v) =
-                Q(g1, i - a1[0] + 1, j - a1[1] + 1, k - a1[2] + 1, v);
+                Q(g1, i - a1[0] + 1, j - a1[1] + 1, k - a1[2] + 1, v) - qo;
+    }
   }
 }
 

@@ -61,6 +61,7 @@ typedef struct orc_hier {
   int nsmoothingpasses, maxvcycles;
   double qerrortol;
   int periodic;                /* root has periodic faces (set at commit) */
+  double qpo[2];               /* quasi-periodic offsets qpo(1:nrvars) */
   int noff, offidx[13];        /* PeriodicSetup's offset list */
   int rootmx[3];
   char err[256];
@@ -76,6 +77,7 @@ int orc_add_patch(orc_hier *h, int id, int level, int parent, const int *mx,
 int orc_commit(orc_hier *h);
 void orc_set_problem(orc_hier *h, double dt, double eps2, double fc_alpha,
                      double fc_shift, double fe_num, double fe_den);
+void orc_set_qpo(orc_hier *h, double qpo1, double qpo2);
 void orc_set_solver(orc_hier *h, int nsmoothingpasses, int maxvcycles,
                     double qerrortol);
 double *orc_field(orc_hier *h, int patch, int field);

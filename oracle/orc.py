@@ -41,6 +41,7 @@ def lib():
         L.orc_commit.argtypes = [vp]
         L.orc_set_problem.argtypes = [vp] + [C.c_double] * 6
         L.orc_set_solver.argtypes = [vp, C.c_int, C.c_int, C.c_double]
+        L.orc_set_qpo.argtypes = [vp, C.c_double, C.c_double]
         L.orc_field.restype = dp
         L.orc_field.argtypes = [vp, C.c_int, C.c_int]
         L.orc_field_len.restype = C.c_long
@@ -78,7 +79,7 @@ class Oracle:
     """An oracle hierarchy mirroring a :class:`bsam_amd.layout.Layout`."""
 
     def __init__(self, layout, dt, eps, alpha=None, well="reference",
-                 nsmoothingpasses=2, maxvcycles=20, qerrortol=1e-8):
+                 nsmoothingpasses=2, maxvcycles=20, qerrortol=1e-8, qpo=None):
         L = lib()
         self.layout = layout
         self.h = L.orc_create(layout.ndims, layout.nrvars, layout.mbc, layout.minlevel)
@@ -102,6 +103,9 @@ class Oracle:
         else:
             raise ValueError(well)
         L.orc_set_solver(self.h, nsmoothingpasses, maxvcycles, qerrortol)
+        if qpo is not None:
+            q = [float(v) for v in qpo] + [0.0, 0.0]
+            L.orc_set_qpo(self.h, q[0], q[1])
         self.maxvcycles = maxvcycles
 
     def __del__(self):

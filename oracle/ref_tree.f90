@@ -264,6 +264,8 @@ contains
     real, dimension(1:3)    :: rxl
     real    :: rdx
     integer, allocatable :: cid(:), clev(:), cpar(:), cmb(:,:,:)
+    character(len=128) :: qstr
+    integer :: qlen, qstat
   read(unit,*) ndims, nrv_in, mbc_in
   read(unit,*) dt, alpha, eps
   eps2 = eps*eps
@@ -283,7 +285,12 @@ contains
   nrootgrids = 1
   maxlevel = finest_in
   finestlevel = 0
+  ! qpo(1:nrvars), the griddata namelist's quasi-periodic offsets
+  ! (bsamroutines.f90:221): from BSAM_HARNESS_QPO when set, else 0 as
+  ! RootInit leaves them
   qpo = 0.0
+  call get_environment_variable('BSAM_HARNESS_QPO', qstr, qlen, qstat)
+  if (qstat == 0 .and. qlen > 0) read(qstr,*) qpo(1:nrv_in)
   lconverged = .false.
 
   call InitForest

@@ -57,9 +57,10 @@ def write_spec(path, layout, dt, alpha, eps, nsmooth, maxv, tol, ops):
 
 
 def run(layout, arrays, ops, dt=1e-3, alpha=1.0, eps=0.03, nsmooth=2, maxv=20,
-        tol=1e-8, keep=None):
+        tol=1e-8, keep=None, qpo=None):
     """Run ``ops`` on the reference. ``arrays`` maps patch id -> dict of
-    'q', 'f', 'qc' Fortran-order arrays. Returns (outputs, residuals, listing)
+    'q', 'f', 'qc' Fortran-order arrays; ``qpo`` the quasi-periodic offsets
+    (one per variable, 0 when None). Returns (outputs, residuals, listing)
     with outputs: id -> {'q','f','qc','rf','qold','qrte','errorflags'}
     (errorflags as int32)."""
     if not available():
@@ -80,7 +81,11 @@ def run(layout, arrays, ops, dt=1e-3, alpha=1.0, eps=0.03, nsmooth=2, maxv=20,
                 arr = np.asarray(a[name], dtype=np.float64)
                 assert arr.shape == layout.field_shape(p, name), (p.id, name, arr.shape)
                 fh.write(arr.tobytes(order="F"))
-    proc = subprocess.run([HARNESS, spec, fin, fout, fres], cwd=tmp,
+    env = dict(os.environ)
+    env.pop("BSAM_HARNESS_QPO", None)
+    if qpo is not None:
+        env["BSAM_HARNESS_QPO"] = " ".join(repr(float(v)) for v in qpo)
+    proc = subprocess.run([HARNESS, spec, fin, fout, fres], cwd=tmp, env=env,
                           capture_output=True, text=True, preexec_fn=_big_stack)
     if proc.returncode != 0:
         raise RuntimeError(f"harness failed: {proc.stdout}\n{proc.stderr}")

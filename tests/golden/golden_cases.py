@@ -119,6 +119,23 @@ OPS_CASES = {
     "amr3d_userflags": ("amr3d", 20, USERFLAG_OPS),
     "periodic2d_userflags": ("periodic2d", 21, USERFLAG_OPS),
     "periodic3d_userflags": ("periodic3d", 22, USERFLAG_OPS),
+    # quasi-periodic offsets (QPO below) on the periodic layouts
+    "periodic2d_qpo_seq": ("periodic2d", 23, OPS),
+    "periodic2d_x_userbc_qpo_vcycle": ("periodic2d_x_userbc", 24, [("vcycle", 2)]),
+    "periodic3d_qpo_seq": ("periodic3d", 25, OPS),
+    "periodic3d_xz_qpo_vcycle": ("periodic3d_xz", 26, [("vcycle", 2)]),
+}
+
+# qpo(1:nrvars) of the cases that set it (0 for the others): the values
+# copied across the periodic x boundary gain +qpo(n) on the high side and
+# -qpo(n) on the low side
+QPO = {
+    "periodic2d_qpo_seq": (0.375, -1.25),
+    "periodic2d_x_userbc_qpo_vcycle": (0.125, 0.5),
+    "periodic3d_qpo_seq": (-0.625, 0.25),
+    "periodic3d_xz_qpo_vcycle": (0.25, -0.75),
+    "periodic2d_qpo": (0.02, 0.0),
+    "periodic3d_qpo": (-0.015, 0.001),
 }
 
 
@@ -161,6 +178,10 @@ STEP_CASES = {
                        alpha=1.0, eps=0.03, maxv=40, tol=1e-8, steps=3),
     "periodic3d": dict(layout="periodic3d", init=noise_init(0.5, 0.05, 14), dt=5e-4,
                        alpha=1.0, eps=0.16, maxv=40, tol=1e-8, steps=2),
+    "periodic2d_qpo": dict(layout="periodic2d", init=noise_init(0.5, 0.05, 15), dt=1e-3,
+                           alpha=1.0, eps=0.03, maxv=40, tol=1e-8, steps=3),
+    "periodic3d_qpo": dict(layout="periodic3d", init=noise_init(0.5, 0.05, 16), dt=5e-4,
+                           alpha=1.0, eps=0.16, maxv=40, tol=1e-8, steps=2),
     # fixed 6 V-cycles (tol 0) of the mapped +-1 problem, phi ~ 0.1 + U(-.05,.05)
     "plusminus_map": dict(layout="amr2d", init=noise_init(0.55, 0.025, 6), dt=PM_DT / 2,
                           alpha=8.0, eps=math.sqrt(2.0) * PM_EPS, maxv=6, tol=0.0, steps=1),

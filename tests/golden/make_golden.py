@@ -4,7 +4,10 @@ Runs the reference BSAM modules (compiled unmodified by `make -C oracle ref`
 into oracle/_ref/harness) on small seeded inputs and stores inputs and
 outputs as .npz (numpy arrays only, no pickles). Run in the build container:
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [case ...]
+
+With case names, writes only those fixtures (the layout listing is always
+rewritten).
 """
 
 from __future__ import annotations
@@ -22,7 +25,7 @@ sys.path.insert(0, HERE)
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
 import refrun  # noqa: E402
-from golden_cases import LAYOUTS, OPS_CASES, STEP_CASES  # noqa: E402
+from golden_cases import LAYOUTS, OPS_CASES, QPO, STEP_CASES  # noqa: E402
 
 
 def pack_fields(prefix, layout, fields, names, out):
@@ -31,7 +34,7 @@ def pack_fields(prefix, layout, fields, names, out):
             out[f"{prefix}/{n}/{name}"] = np.asarray(fields[p.id][name])
 
 
-def main():
+def main(only=()):
     if not refrun.available():
         raise SystemExit("build the reference harness first: make -C oracle ref")
     listing = {}
@@ -45,11 +48,13 @@ def main():
         json.dump(listing, fh, indent=0)
 
     for case, (lname, seed, ops) in OPS_CASES.items():
+        if only and case not in only:
+            continue
         lay = LAYOUTS[lname]()
         rng = np.random.default_rng(seed)
         fields = {p.id: {k: np.asfortranarray(rng.uniform(-1, 1, lay.field_shape(p, k)))
                          for k in ("q", "f", "qc")} for p in lay.ordered()}
-        out, res, _ = refrun.run(lay, fields, ops)
+        out, res, _ = refrun.run(lay, fields, ops, qpo=QPO.get(case))
         arrs = {}
         pack_fields("in", lay, fields, ("q", "f", "qc"), arrs)
         names = ("q", "f", "qc")
@@ -64,10 +69,12 @@ def main():
         np.savez_compressed(os.path.join(HERE, f"ops_{case}.npz"), **arrs)
 
     for case, spec in STEP_CASES.items():
+        if only and case not in only:
+            continue
         lay = LAYOUTS[spec["layout"]]()
         fields = spec["init"](lay)
         kw = dict(dt=spec["dt"], alpha=spec["alpha"], eps=spec["eps"], maxv=spec["maxv"],
-                  tol=spec["tol"])
+                  tol=spec["tol"], qpo=QPO.get(case))
         out, res, _ = refrun.run(lay, fields, [("step", spec["steps"], 0)], **kw)
         arrs = {"residuals": np.array([[s, it, r] for s, it, r in res])}
         pack_fields("in", lay, fields, ("q",), arrs)
@@ -80,4 +87,4 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    main(tuple(sys.argv[1:]))

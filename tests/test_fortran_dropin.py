@@ -47,9 +47,15 @@ def write_case(tmp_path, lname, seed, dt=1e-3, alpha=1.0, eps=0.03, maxv=40, tol
     return spec, inp
 
 
-def run(mode, spec, inp, *extra):
+def run(mode, spec, inp, *extra, qpo=None):
+    """Runs dropin_check; `qpo` sets the module's quasi-periodic offsets
+    (read by the tree set-up from BSAM_HARNESS_QPO)."""
+    env = dict(os.environ)
+    env.pop("BSAM_HARNESS_QPO", None)
+    if qpo is not None:
+        env["BSAM_HARNESS_QPO"] = " ".join(repr(float(v)) for v in qpo)
     out = subprocess.run([BIN, mode, spec, inp, *extra], capture_output=True, text=True,
-                         timeout=300)
+                         timeout=300, env=env)
     assert out.returncode == 0, out.stdout + out.stderr
     diff = [l for l in out.stdout.splitlines() if l.startswith("max_abs_diff")]
     assert diff, out.stdout
@@ -64,6 +70,28 @@ def test_ghost_programs_on_reference_arrays(tmp_path, lname):
     spec, inp = write_case(str(tmp_path), lname, seed=1)
     diff, _ = run("host", spec, inp)
     assert diff == 0.0
+
+
+@pytest.mark.parametrize("lname", ["periodic2d", "periodic2d_x_userbc", "periodic3d",
+                                   "periodic3d_xz"])
+def test_ghost_programs_with_qpo_on_reference_arrays(tmp_path, lname):
+    """GpuOpen hands the module's qpo to the engine: its host programs match
+    the reference SetGhost with quasi-periodic offsets."""
+    ensure_built()
+    spec, inp = write_case(str(tmp_path), lname, seed=4)
+    diff, _ = run("host", spec, inp, qpo=(0.375, -1.25))
+    assert diff == 0.0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("lname", ["periodic2d", "periodic3d_xz"])
+def test_gpu_multigrid_iterations_with_qpo_replace_reference(tmp_path, lname):
+    if not os.path.exists(BIN):
+        pytest.skip("dropin_check not built (needs the reference build)")
+    spec, inp = write_case(str(tmp_path), lname, seed=5)
+    diff, out = run("gpu", spec, inp, "2", qpo=(0.02, -0.001))
+    assert "converged 1" in out, out
+    assert diff == 0.0, out
 
 
 @pytest.mark.gpu

@@ -79,3 +79,58 @@ def test_red_and_black_programs_partition_interpolated_faces():
     assert not red & black
     assert red | black == full
     assert len(full) == 4 * 16 + 4
+
+
+def _bits(a):
+    return np.ascontiguousarray(a).view(np.uint64)
+
+
+@pytest.mark.parametrize("name", sorted(k for k in LAYOUTS if k.startswith("periodic")))
+@pytest.mark.parametrize("qpo", [(0.375, -1.25), (0.0, 0.0)])
+def test_ghost_program_with_qpo_matches_oracle(name, qpo):
+    """Quasi-periodic offsets: copies across the periodic x boundary gain
+    +qpo(n) on the high side and -qpo(n) on the low side (boundary.f90:441-461,
+    :578-591), and the +0.0 offset of images shifted in y or z only turns a
+    -0.0 into +0.0 as the reference's `+ qoffset` does. Compared bit for bit
+    on inputs holding signed zeros."""
+    lay = LAYOUTS[name]()
+    eng = Engine(lay, dt=1e-3, eps=0.03, device=-1, qpo=qpo)
+    rng = np.random.default_rng(1)
+    for level in range(lay.minlevel, lay.finestlevel + 1):
+        for ipass in (0, 1, 2):
+            ora = orc.Oracle(lay, dt=1e-3, eps=0.03, qpo=qpo)
+            qs = []
+            for n, p in enumerate(ora.patches):
+                shape = lay.field_shape(p, "q")
+                a = np.where(rng.uniform(size=shape) < 0.5, rng.choice([-0.0, 0.0], size=shape),
+                             rng.uniform(-1, 1, shape))
+                ora.field(n, "q")[...] = a
+                qs.append(np.asfortranarray(a.copy()))
+            ora.set_ghost(level, ipass)
+            eng.ghost_apply_host(level, ipass, qs)
+            for n, p in enumerate(ora.patches):
+                assert np.array_equal(_bits(ora.field(n, "q")), _bits(qs[n])), (level, ipass, p.id)
+
+
+def test_qpo_offsets_change_the_periodic_ghosts():
+    """The oracle's offsets have teeth: with qpo != 0 the x-periodic ghosts
+    differ from those without by exactly +-qpo(n), and the y-periodic ones
+    not at all (a planted offset of the wrong sign or dimension fails)."""
+    lay = LAYOUTS["periodic_xy"]()
+    qpo = (0.375, -1.25)
+    outs = []
+    for off in (None, qpo):
+        ora = orc.Oracle(lay, dt=1e-3, eps=0.03, qpo=off)
+        rng = np.random.default_rng(2)
+        for n, p in enumerate(ora.patches):
+            ora.field(n, "q")[...] = rng.uniform(-1, 1, lay.field_shape(p, "q"))
+        ora.set_ghost(0, 0)
+        root = next(n for n, p in enumerate(ora.patches) if p.level == 0)
+        outs.append(ora.field(root, "q").copy())
+    d = outs[1] - outs[0]
+    mx, my = ora.patches[root].mx[:2]
+    for v in range(2):
+        np.testing.assert_allclose(d[mx + 1, 1:my + 1, 0, v], qpo[v], rtol=0, atol=1e-15)
+        np.testing.assert_allclose(d[0, 1:my + 1, 0, v], -qpo[v], rtol=0, atol=1e-15)
+        assert not d[1:mx + 1, 0, 0, v].any() and not d[1:mx + 1, my + 1, 0, v].any()
+        assert not d[1:mx + 1, 1:my + 1, 0, v].any()

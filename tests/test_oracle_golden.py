@@ -13,7 +13,7 @@ import numpy as np
 import pytest
 
 import orc
-from golden_cases import LAYOUTS, OPS_CASES, PM_DT, PM_EPS, STEP_CASES
+from golden_cases import LAYOUTS, OPS_CASES, PM_DT, PM_EPS, QPO, STEP_CASES
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
@@ -59,7 +59,7 @@ def test_ops_match_reference(case):
     lname, _, ops = OPS_CASES[case]
     lay = LAYOUTS[lname]()
     z = load(f"ops_{case}.npz")
-    o = orc.Oracle(lay, dt=1e-3, eps=0.03, alpha=1.0)
+    o = orc.Oracle(lay, dt=1e-3, eps=0.03, alpha=1.0, qpo=QPO.get(case))
     for n, _ in enumerate(o.patches):
         for f in ("q", "f", "qc"):
             o.field(n, f)[...] = z[f"in/{n}/{f}"]
@@ -82,7 +82,7 @@ def test_time_steps_match_reference(case):
     lay = LAYOUTS[spec["layout"]]()
     z = load(f"steps_{case}.npz")
     o = orc.Oracle(lay, dt=spec["dt"], eps=spec["eps"], alpha=spec["alpha"],
-                   maxvcycles=spec["maxv"], qerrortol=spec["tol"])
+                   maxvcycles=spec["maxv"], qerrortol=spec["tol"], qpo=QPO.get(case))
     for n, _ in enumerate(o.patches):
         o.field(n, "q")[...] = z[f"in/{n}/q"]
     res = z["residuals"]

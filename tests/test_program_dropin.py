@@ -35,6 +35,10 @@ def first_difference(a: bytes, b: bytes) -> str:
 def variant(name: str, steps: int):
     deck = bp.shipped_deck(steps)
     g = deck["griddata"]
+    if name.endswith("_qpo"):
+        # quasi-periodic offsets on the periodic x boundary
+        g["qpo"] = {1: 0.05, 2: -0.002}
+        name = name[:-len("_qpo")]
     if name.endswith("_sync"):
         # syncelliptic: a first regrid and solve with dt = 0 before the steps
         deck["rundata"]["syncelliptic"] = True
@@ -52,7 +56,8 @@ def variant(name: str, steps: int):
 
 
 @pytest.mark.parametrize("name,steps", [("shipped", 6), ("neumann_l3", 3), ("3d_l1", 3),
-                                        ("shipped_sync", 3), ("3d_l1_sync", 2)])
+                                        ("shipped_sync", 3), ("3d_l1_sync", 2),
+                                        ("shipped_qpo", 3), ("3d_l1_qpo", 2)])
 def test_program_frames_identical(tmp_path, name, steps):
     for prog in (bp.REF_PROGRAM, bp.GPU_PROGRAM):
         if not os.path.exists(prog):
@@ -69,7 +74,8 @@ def test_program_frames_identical(tmp_path, name, steps):
 
 
 @pytest.mark.parametrize("name,steps", [("shipped", 6), ("neumann_l3", 3), ("3d_l1", 3),
-                                        ("shipped_sync", 3), ("3d_l1_sync", 2)])
+                                        ("shipped_sync", 3), ("3d_l1_sync", 2),
+                                        ("shipped_qpo", 3), ("3d_l1_qpo", 2)])
 def test_native_driver_frames_identical(tmp_path, name, steps):
     """The engine's own time-step driver (bsam_amd/driver.py: the regrid's
     clustering on the host, its tagging, InitFields and TransferValues on
