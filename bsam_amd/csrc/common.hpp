@@ -301,6 +301,7 @@ struct Dev {
   long long qvs;          // the q pool's variable stride (also e and qb)
   int nrvars;
   int tile_lx, tile_ly;   // level-kernel tiles: 2^tile_lx x 2^tile_ly (x the rest of 256)
+  int one_group;          // 1: every patch is in group 0
 };
 
 // One workgroup's share of the after-step statistics: rows (2D) or planes

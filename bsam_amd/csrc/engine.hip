@@ -331,6 +331,7 @@ struct bsam_ctx {
     d.qvs = qstride;
     d.tile_lx = tile_lx;
     d.tile_ly = tile_ly;
+    d.one_group = ngroups == 1;
     return d;
   }
   void need_device() const {

@@ -66,7 +66,8 @@ __device__ __forceinline__ bool stopped(const Dev &d, int group) {
   return d.gstate && d.gstate[4 * group + 3];
 }
 __device__ __forceinline__ bool stopped_patch(const Dev &d, int patch) {
-  return d.gstate && d.gstate[4 * d.P[patch].group + 3];
+  // with one group the flag's address needs no patch record
+  return d.gstate && d.gstate[4 * (d.one_group ? 0 : d.P[patch].group) + 3];
 }
 
 // One patch's arrays, each addressed from its own origin: q(0,0,0,1),
@@ -161,46 +162,76 @@ __device__ __forceinline__ void op_at(const Problem &prm, const View<T> &a, T ba
 // 2x2 (phi, mu) system, Cramer's rule (RelaxGrid2D/3D, problem.f90:164-215,
 // :453-518; Mob = 1). Defect form: the Newton step linearises
 // Dfc(base + e) about the current e.
-// The update of cell (i,j,k) reading q through view `a` (the patch's arrays,
-// or a staged copy of its neighbourhood) and writing the new values to *o1,
-// *o2; `base` is the FP32 defect form's base point at the cell.
+// relax_gather loads what the update of cell (i,j,k) reads through view `a`
+// (the patch's arrays, or a staged copy of its neighbourhood): its phi, the
+// 2*ND neighbours of both variables (slots i-1, i+1, j-1, j+1, k-1, k+1), its
+// f and `base`, the FP32 defect form's base point at the cell; relax_apply
+// forms the new values from them; relax_core does both and writes the new
+// values to *o1, *o2.
+template <int ND, class T>
+struct RelaxIn {
+  T phi, n1[2 * ND], n2[2 * ND], f1, f2, base;
+};
 template <int ND, class T, bool DEF>
-__device__ __forceinline__ void relax_core(const Problem &prm, const DPatch &P, const View<T> &a,
-                                           int i, int j, int k, T *o1, T *o2, T base, T fv1,
-                                           T fv2) {
+__device__ __forceinline__ void relax_gather(const View<T> &a, int i, int j, int k, T base,
+                                             T fv1, T fv2, RelaxIn<ND, T> &in) {
   const long long c = a.at(i, j, k);
   const long long cxp = a.at(i + 1, j, k), cxm = a.at(i - 1, j, k);   // the other half-row
   const T *q1 = a.p, *q2 = a.p + a.vs;
+  const long long sy = a.rs, sz = a.pl;
+  in.phi = q1[c];
+  in.n1[0] = q1[cxm]; in.n1[1] = q1[cxp]; in.n1[2] = q1[c - sy]; in.n1[3] = q1[c + sy];
+  in.n2[0] = q2[cxm]; in.n2[1] = q2[cxp]; in.n2[2] = q2[c - sy]; in.n2[3] = q2[c + sy];
+  if (ND == 3) {
+    in.n1[2 * ND - 2] = q1[c - sz]; in.n1[2 * ND - 1] = q1[c + sz];
+    in.n2[2 * ND - 2] = q2[c - sz]; in.n2[2 * ND - 1] = q2[c + sz];
+  }
+  in.f1 = fv1;
+  in.f2 = fv2;
+  in.base = base;
+}
+template <int ND, class T, bool DEF>
+__device__ __forceinline__ void relax_apply(const Problem &prm, const DPatch &P,
+                                            const RelaxIn<ND, T> &in, T &o1, T &o2) {
   T tmp1 = (T)P.dt_h2, tmp2 = (T)P.eps2_h2, tmp3 = (T)P.lap_h2;
-  T phi = q1[c];
+  const T phi = in.phi;
   T dc, d2;
   if (DEF) {
-    dc = dfc_delta<T>(prm, base, phi);
-    d2 = d2fc_at<T>(prm, base, phi);
+    dc = dfc_delta<T>(prm, in.base, phi);
+    d2 = d2fc_at<T>(prm, in.base, phi);
   } else {
     dc = (T)dfc(prm, (double)phi);
     d2 = (T)d2fc(prm, (double)phi);
   }
   T a12, a21, b1, b2;
-  long long sy = a.rs, sz = a.pl;
+  const T *m1 = in.n1, *m2 = in.n2;
   if (ND == 2) {
     a12 = tmp1 * (T)4;
     a21 = -d2 - tmp3;
-    b1 = fv1 + tmp1 * (((q2[cxm] + q2[cxp]) + q2[c - sy]) + q2[c + sy]);
-    b2 = ((fv2 + dc) - d2 * phi) -
-         tmp2 * (((q1[cxp] + q1[cxm]) + q1[c + sy]) + q1[c - sy]);
+    b1 = in.f1 + tmp1 * (((m2[0] + m2[1]) + m2[2]) + m2[3]);
+    b2 = ((in.f2 + dc) - d2 * phi) - tmp2 * (((m1[1] + m1[0]) + m1[3]) + m1[2]);
   } else {
     a12 = tmp1 * (T)6;
     a21 = -d2 - tmp3;
-    b1 = fv1 + tmp1 * (((((q2[cxm] + q2[cxp]) + q2[c - sy]) + q2[c + sy]) +
-                         q2[c - sz]) + q2[c + sz]);
-    b2 = ((fv2 + dc) - d2 * phi) -
-         tmp2 * (((((q1[cxp] + q1[cxm]) + q1[c + sy]) + q1[c - sy]) + q1[c + sz]) +
-                 q1[c - sz]);
+    b1 = in.f1 + tmp1 * (((((m2[0] + m2[1]) + m2[2]) + m2[3]) + m2[2 * ND - 2]) +
+                         m2[2 * ND - 1]);
+    b2 = ((in.f2 + dc) - d2 * phi) -
+         tmp2 * (((((m1[1] + m1[0]) + m1[3]) + m1[2]) + m1[2 * ND - 1]) + m1[2 * ND - 2]);
   }
   T det = (T)1 - a12 * a21;
-  *o1 = (b1 - a12 * b2) / det;
-  *o2 = (b2 - a21 * b1) / det;
+  o1 = (b1 - a12 * b2) / det;
+  o2 = (b2 - a21 * b1) / det;
+}
+template <int ND, class T, bool DEF>
+__device__ __forceinline__ void relax_core(const Problem &prm, const DPatch &P, const View<T> &a,
+                                           int i, int j, int k, T *o1, T *o2, T base, T fv1,
+                                           T fv2) {
+  RelaxIn<ND, T> in;
+  relax_gather<ND, T, DEF>(a, i, j, k, base, fv1, fv2, in);
+  T r1, r2;
+  relax_apply<ND, T, DEF>(prm, P, in, r1, r2);
+  *o1 = r1;
+  *o2 = r2;
 }
 template <int ND, class T, bool DEF>
 __device__ __forceinline__ void relax_cell_f(const Problem &prm, const Lvl<T> &L, int i, int j,
@@ -792,7 +823,10 @@ __device__ __forceinline__ void ghost_alongside(const Dev &d, const GhostList &g
 // workgroup then synchronises and applies its share of it:
 // updates of the next SetGhost(level, color) that read cells this tile has
 // just relaxed (chosen on the host so that no other tile touches their
-// cells in this launch). A list with null starts: no updates.
+// cells in this launch). A list with null starts: no updates. The tile, both
+// lists' per-tile starts and each thread's first record of both lists are
+// loaded at once at the start, so the chain of dependent loads ahead of the
+// post updates is short.
 //
 // With STAGE, the tile's cells and their one-cell halo (both variables) are
 // first copied into LDS in the same parity-split row layout (a box of
@@ -807,15 +841,38 @@ __global__ __launch_bounds__(RELAX_THREADS) void k_relax_fused(Dev d, const Tile
                                                                 GhostList post) {
   const int b = xcd_tile(nt);
   if (b >= nt) return;
+  // the tile and both lists' shares of it, loads independent of one another
   const Tile t = tiles[b];
+  int s0 = 0, s1 = 0, l0 = 0, l1 = 0, ps0 = 0, ps1 = 0, pl0 = 0, pl1 = 0;
+  if (pre.ss) {
+    s0 = pre.ss[b]; s1 = pre.ss[b + 1]; l0 = pre.sl[b]; l1 = pre.sl[b + 1];
+  }
+  if (post.ss) {
+    ps0 = post.ss[b]; ps1 = post.ss[b + 1]; pl0 = post.sl[b]; pl1 = post.sl[b + 1];
+  }
+  // this thread's first update of each list, loaded now (records only)
+  const int se = s0 + (int)threadIdx.x, pe = ps0 + (int)threadIdx.x;
+  GhostS sr{}, pr{};
+  if (se < s1) sr = pre.s[se];
+  if (pe < ps1) pr = post.s[pe];
   if (stopped_patch(d, t.patch)) return;   // the tile's updates are its patch's group's
-  ghost_tile<T>(d, pre, b);
+  T *q = fields<T>(d).q;
   Lvl<T> L = level_ref<T>(d, t.patch);
   int i, j, k;
   tile_color_cell<ND>(d, t, color, i, j, k);
   // every thread reaches the barriers below (they depend on the block only)
   const bool mine = in_box<ND>(i, j, k, L.P.mx, L.P.my, L.P.mz);
-  if constexpr (STAGE) {
+  if (s1 > s0 || l1 > l0) {
+    if (se < s1) ghost_op_s<T, DEF>(d.nrvars, q, d.qvs, sr, d.prm.qpo);
+    for (int e = se + blockDim.x; e < s1; e += blockDim.x)
+      ghost_op_s<T, DEF>(d.nrvars, q, d.qvs, pre.s[e], d.prm.qpo);
+    for (int e = l0 + (int)threadIdx.x; e < l1; e += blockDim.x)
+      ghost_op<T, DEF>(d.nrvars, q, d.qvs, pre.l[e], d.prm.qpo);
+    __syncthreads();
+  }
+  if constexpr (!STAGE) {
+    if (mine) relax_cell<ND, T, DEF>(d.prm, L, i, j, k);
+  } else {
     const int tx = 1 << d.tile_lx, ty = 1 << d.tile_ly;
     const int tz = ND == 3 ? 256 >> (d.tile_lx + d.tile_ly) : 1;
     const int bx = tx + 2, by = ty + 2, bz = ND == 3 ? tz + 2 : 1;
@@ -841,12 +898,14 @@ __global__ __launch_bounds__(RELAX_THREADS) void k_relax_fused(Dev d, const Tile
       relax_core<ND, T, DEF>(d.prm, L.P, a, i - o0, j - o1, ND == 3 ? k - o2 : 0, g.p + c,
                              g.p + g.vs + c, DEF ? L.qb[c] : (T)0, L.f[fo], L.f[fo + L.P.fvs]);
     }
-  } else {
-    if (mine) relax_cell<ND, T, DEF>(d.prm, L, i, j, k);
   }
-  if (post.ss && (post.ss[b + 1] > post.ss[b] || post.sl[b + 1] > post.sl[b])) {
+  if (ps1 > ps0 || pl1 > pl0) {
     __syncthreads();
-    ghost_tile<T>(d, post, b);
+    if (pe < ps1) ghost_op_s<T, DEF>(d.nrvars, q, d.qvs, pr, d.prm.qpo);
+    for (int e = pe + blockDim.x; e < ps1; e += blockDim.x)
+      ghost_op_s<T, DEF>(d.nrvars, q, d.qvs, post.s[e], d.prm.qpo);
+    for (int e = pl0 + (int)threadIdx.x; e < pl1; e += blockDim.x)
+      ghost_op<T, DEF>(d.nrvars, q, d.qvs, post.l[e], d.prm.qpo);
   }
 }
 
