@@ -271,16 +271,19 @@ __host__ __device__ __forceinline__ int qpo_code(int b) { return (b >> 4) & 3; }
 __host__ __device__ __forceinline__ int flip_byte(int nflip, int code) {
   return nflip | code << 4;
 }
+// The two variables' periodic offsets, passed by value: kernels read them
+// from their argument block, and no address of it is taken (an address would
+// copy the argument block to scratch memory).
+struct Qpo {
+  double a, b;
+};
 // A value with its periodic offset code applied: the reference's
 // `q + qpo(n)`, `q - qpo(n)` or `q + 0.0`.
 template <class T>
-__host__ __device__ __forceinline__ T add_qpo(T val, int code, const double *qpo, int v) {
-  switch (code) {
-    case 1: return val + (T)qpo[v];
-    case 2: return val - (T)qpo[v];
-    case 3: return val + (T)0;
-    default: return val;
-  }
+__host__ __device__ __forceinline__ T add_qpo(T val, int code, Qpo qpo, int v) {
+  if (code == 0) return val;
+  const T q = code == 3 ? (T)0 : (T)(v == 0 ? qpo.a : qpo.b);
+  return code == 2 ? val - q : val + q;
 }
 
 // Kernel argument bundle: pool base pointers plus the patch table. The FP64

@@ -212,6 +212,7 @@ struct bsam_ctx {
   hipStream_t side = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   int iso_side = 1;
+  bool tail_qpo = false;   // a tail level's SetGhost programs add periodic offsets
   double qpo[2] = {0.0, 0.0};   // quasi-periodic offsets (bsam_set_qpo)
   bool building = false, committed = false;
   bool had_commit = false;    // hp/dp/lev/device state describe a committed hierarchy
@@ -519,6 +520,7 @@ static void build_tiles(bsam_ctx *c, LevelData &L, int level) {
 // resulting dynamic LDS size.
 static bool plan_tail(bsam_ctx *c, long long budget, hipStream_t s) {
   const int minl = c->cfg.minlevel, nv = c->cfg.nrvars;
+  c->tail_qpo = false;
   std::vector<TailLevel> tl;
   long long used[2] = {0, 0};
   bool open[2] = {true, true};   // no level has failed to fit yet
@@ -533,6 +535,8 @@ static bool plan_tail(bsam_ctx *c, long long budget, hipStream_t s) {
     for (int ip = 0; ip < 3; ip++) {
       t.ngp[ip] = (int)L.gp[ip].size();
       t.gp[ip] = L.d_tgp[ip];
+      for (const GhostOp &g : L.gp[ip])
+        if (qpo_code(g.nflip)) c->tail_qpo = true;
     }
     for (int set = 0; set < 2; set++) {
       long long sz = set == 0 ? 8 : 4;
@@ -1615,7 +1619,7 @@ static int op_vcycle(bsam_ctx *c, int level, int prec = PREC_SOLUTION_FP64,
     HIPCK(launch_tail_vcycle(c->nd(), prec, c->dev(), c->d_tail, nlev, c->cfg.nsmoothingpasses,
                              c->tail_threads, c->tail_lds[set], c->d_tcin[set],
                              c->tcin_n[set][nlev - 1], c->d_tcout[set],
-                             c->tcout_n[set][nlev - 1], c->stream));
+                             c->tcout_n[set][nlev - 1], c->tail_qpo, c->stream));
     return -1;
   }
   if (level == c->cfg.minlevel)

@@ -1429,7 +1429,7 @@ void apply_ghost_program_host(const std::vector<HPatch> &hp, const std::vector<G
                          BSAM_C3 * val(g.c[2], v) + BSAM_C4 * val(g.c[3], v));
           break;
       }
-      x = add_qpo<double>(x, qpo_code(g.nflip), qpo, v);
+      x = add_qpo<double>(x, qpo_code(g.nflip), Qpo{qpo[0], qpo[1]}, v);
       for (int r = 0; r < flip_count(g.nflip); r++) x = 2.0 - x;
       res[e * nrvars + v] = x;
     }

@@ -162,76 +162,46 @@ __device__ __forceinline__ void op_at(const Problem &prm, const View<T> &a, T ba
 // 2x2 (phi, mu) system, Cramer's rule (RelaxGrid2D/3D, problem.f90:164-215,
 // :453-518; Mob = 1). Defect form: the Newton step linearises
 // Dfc(base + e) about the current e.
-// relax_gather loads what the update of cell (i,j,k) reads through view `a`
-// (the patch's arrays, or a staged copy of its neighbourhood): its phi, the
-// 2*ND neighbours of both variables (slots i-1, i+1, j-1, j+1, k-1, k+1), its
-// f and `base`, the FP32 defect form's base point at the cell; relax_apply
-// forms the new values from them; relax_core does both and writes the new
-// values to *o1, *o2.
-template <int ND, class T>
-struct RelaxIn {
-  T phi, n1[2 * ND], n2[2 * ND], f1, f2, base;
-};
+// The update of cell (i,j,k) reading q through view `a` (the patch's arrays,
+// or a staged copy of its neighbourhood) and writing the new values to *o1,
+// *o2; `base` is the FP32 defect form's base point at the cell.
 template <int ND, class T, bool DEF>
-__device__ __forceinline__ void relax_gather(const View<T> &a, int i, int j, int k, T base,
-                                             T fv1, T fv2, RelaxIn<ND, T> &in) {
+__device__ __forceinline__ void relax_core(const Problem &prm, const DPatch &P, const View<T> &a,
+                                           int i, int j, int k, T *o1, T *o2, T base, T fv1,
+                                           T fv2) {
   const long long c = a.at(i, j, k);
   const long long cxp = a.at(i + 1, j, k), cxm = a.at(i - 1, j, k);   // the other half-row
   const T *q1 = a.p, *q2 = a.p + a.vs;
-  const long long sy = a.rs, sz = a.pl;
-  in.phi = q1[c];
-  in.n1[0] = q1[cxm]; in.n1[1] = q1[cxp]; in.n1[2] = q1[c - sy]; in.n1[3] = q1[c + sy];
-  in.n2[0] = q2[cxm]; in.n2[1] = q2[cxp]; in.n2[2] = q2[c - sy]; in.n2[3] = q2[c + sy];
-  if (ND == 3) {
-    in.n1[2 * ND - 2] = q1[c - sz]; in.n1[2 * ND - 1] = q1[c + sz];
-    in.n2[2 * ND - 2] = q2[c - sz]; in.n2[2 * ND - 1] = q2[c + sz];
-  }
-  in.f1 = fv1;
-  in.f2 = fv2;
-  in.base = base;
-}
-template <int ND, class T, bool DEF>
-__device__ __forceinline__ void relax_apply(const Problem &prm, const DPatch &P,
-                                            const RelaxIn<ND, T> &in, T &o1, T &o2) {
   T tmp1 = (T)P.dt_h2, tmp2 = (T)P.eps2_h2, tmp3 = (T)P.lap_h2;
-  const T phi = in.phi;
+  T phi = q1[c];
   T dc, d2;
   if (DEF) {
-    dc = dfc_delta<T>(prm, in.base, phi);
-    d2 = d2fc_at<T>(prm, in.base, phi);
+    dc = dfc_delta<T>(prm, base, phi);
+    d2 = d2fc_at<T>(prm, base, phi);
   } else {
     dc = (T)dfc(prm, (double)phi);
     d2 = (T)d2fc(prm, (double)phi);
   }
   T a12, a21, b1, b2;
-  const T *m1 = in.n1, *m2 = in.n2;
+  long long sy = a.rs, sz = a.pl;
   if (ND == 2) {
     a12 = tmp1 * (T)4;
     a21 = -d2 - tmp3;
-    b1 = in.f1 + tmp1 * (((m2[0] + m2[1]) + m2[2]) + m2[3]);
-    b2 = ((in.f2 + dc) - d2 * phi) - tmp2 * (((m1[1] + m1[0]) + m1[3]) + m1[2]);
+    b1 = fv1 + tmp1 * (((q2[cxm] + q2[cxp]) + q2[c - sy]) + q2[c + sy]);
+    b2 = ((fv2 + dc) - d2 * phi) -
+         tmp2 * (((q1[cxp] + q1[cxm]) + q1[c + sy]) + q1[c - sy]);
   } else {
     a12 = tmp1 * (T)6;
     a21 = -d2 - tmp3;
-    b1 = in.f1 + tmp1 * (((((m2[0] + m2[1]) + m2[2]) + m2[3]) + m2[2 * ND - 2]) +
-                         m2[2 * ND - 1]);
-    b2 = ((in.f2 + dc) - d2 * phi) -
-         tmp2 * (((((m1[1] + m1[0]) + m1[3]) + m1[2]) + m1[2 * ND - 1]) + m1[2 * ND - 2]);
+    b1 = fv1 + tmp1 * (((((q2[cxm] + q2[cxp]) + q2[c - sy]) + q2[c + sy]) +
+                         q2[c - sz]) + q2[c + sz]);
+    b2 = ((fv2 + dc) - d2 * phi) -
+         tmp2 * (((((q1[cxp] + q1[cxm]) + q1[c + sy]) + q1[c - sy]) + q1[c + sz]) +
+                 q1[c - sz]);
   }
   T det = (T)1 - a12 * a21;
-  o1 = (b1 - a12 * b2) / det;
-  o2 = (b2 - a21 * b1) / det;
-}
-template <int ND, class T, bool DEF>
-__device__ __forceinline__ void relax_core(const Problem &prm, const DPatch &P, const View<T> &a,
-                                           int i, int j, int k, T *o1, T *o2, T base, T fv1,
-                                           T fv2) {
-  RelaxIn<ND, T> in;
-  relax_gather<ND, T, DEF>(a, i, j, k, base, fv1, fv2, in);
-  T r1, r2;
-  relax_apply<ND, T, DEF>(prm, P, in, r1, r2);
-  *o1 = r1;
-  *o2 = r2;
+  *o1 = (b1 - a12 * b2) / det;
+  *o2 = (b2 - a21 * b1) / det;
 }
 template <int ND, class T, bool DEF>
 __device__ __forceinline__ void relax_cell_f(const Problem &prm, const Lvl<T> &L, int i, int j,
@@ -285,6 +255,8 @@ __device__ __forceinline__ T restrict_f(const T *x, const DPatch &P, int ci, int
 // kernels below map one tile of cells to a workgroup, and the tail kernels
 // loop the same functions over whole single-patch levels in one workgroup.
 
+__device__ __forceinline__ Qpo qpo_of(const Problem &prm) { return Qpo{prm.qpo[0], prm.qpo[1]}; }
+
 // The value of one ghost-cell update of a SetGhost program: kind 0 copies
 // source 0; kinds 1-3 are the quadratic coarse-fine interpolations of
 // Interpolate2D/3D over two fine sources (0, 1) and one, three or four
@@ -294,8 +266,8 @@ __device__ __forceinline__ T restrict_f(const T *x, const DPatch &P, int ci, int
 // qpo_code(fb) for variable v (add_qpo). src(n) loads source n; every caller
 // (global pools, short records, an isolated patch's LDS) evaluates the same
 // expression tree.
-template <class T, bool DEF, class Src>
-__device__ __forceinline__ T ghost_value(int kind, int fb, Src src, const double *qpo, int v) {
+template <class T, bool DEF, class Src, bool QPO = true>
+__device__ __forceinline__ T ghost_value(int kind, int fb, Src src, Qpo qpo, int v) {
   T val;
   switch (kind) {
     case 0:
@@ -315,27 +287,26 @@ __device__ __forceinline__ T ghost_value(int kind, int fb, Src src, const double
       break;
   }
   // the periodic offset, before the maps (a constant: none on a correction)
-  if (!DEF) val = add_qpo<T>(val, qpo_code(fb), qpo, v);
+  if (!DEF && QPO) val = add_qpo<T>(val, qpo_code(fb), qpo, v);
   for (int r = 0; r < flip_count(fb); r++) val = DEF ? -val : (T)2 - val;
   return val;
 }
 // One ghost-cell update of a SetGhost program, offsets relative to q and
 // variables vs apart.
-template <class T, bool DEF>
+template <class T, bool DEF, bool QPO = true>
 __device__ __forceinline__ void ghost_op(int nrvars, T *q, long long vs, const GhostOp &g,
-                                         const double *qpo) {
+                                         Qpo qpo) {
   for (int v = 0; v < nrvars; v++) {
     const T *f = q + (long long)v * vs;
-    const T val = ghost_value<T, DEF>(g.kind, g.nflip, [&](int n) {
-      return f[n == 0 ? g.f0 : n == 1 ? g.f1 : g.c[n - 2]];
-    }, qpo, v);
+    auto src = [&](int n) { return f[n == 0 ? g.f0 : n == 1 ? g.f1 : g.c[n - 2]]; };
+    const T val = ghost_value<T, DEF, decltype(src), QPO>(g.kind, g.nflip, src, qpo, v);
     q[g.dst + (long long)v * vs] = val;
   }
 }
 // The same update from its short record (kinds 0 and 1).
 template <class T, bool DEF>
 __device__ __forceinline__ void ghost_op_s(int nrvars, T *q, long long vs, const GhostS &g,
-                                           const double *qpo) {
+                                           Qpo qpo) {
   for (int v = 0; v < nrvars; v++) {
     const T *f = q + (long long)v * vs;
     const T val = ghost_value<T, DEF>(g.kind, g.nflip, [&](int n) {
@@ -797,9 +768,9 @@ __device__ __forceinline__ void ghost_tile(const Dev &d, const GhostList &g, int
     T *q = fields<T>(d).q;
     constexpr bool DEF = std::is_same<T, float>::value;
     for (int e = s0 + (int)threadIdx.x; e < s1; e += blockDim.x)
-      ghost_op_s<T, DEF>(d.nrvars, q, d.qvs, g.s[e], d.prm.qpo);
+      ghost_op_s<T, DEF>(d.nrvars, q, d.qvs, g.s[e], qpo_of(d.prm));
     for (int e = l0 + (int)threadIdx.x; e < l1; e += blockDim.x)
-      ghost_op<T, DEF>(d.nrvars, q, d.qvs, g.l[e], d.prm.qpo);
+      ghost_op<T, DEF>(d.nrvars, q, d.qvs, g.l[e], qpo_of(d.prm));
     __syncthreads();
   }
 }
@@ -811,9 +782,9 @@ __device__ __forceinline__ void ghost_alongside(const Dev &d, const GhostList &g
   constexpr bool DEF = std::is_same<T, float>::value;
   const int stride = gridDim.x * blockDim.x, e0 = blockIdx.x * blockDim.x + threadIdx.x;
   for (int e = e0; e < g.ns; e += stride)
-    if (!stopped(d, g.s[e].group)) ghost_op_s<T, DEF>(d.nrvars, q, d.qvs, g.s[e], d.prm.qpo);
+    if (!stopped(d, g.s[e].group)) ghost_op_s<T, DEF>(d.nrvars, q, d.qvs, g.s[e], qpo_of(d.prm));
   for (int e = e0; e < g.nl; e += stride)
-    if (!stopped(d, g.l[e].group)) ghost_op<T, DEF>(d.nrvars, q, d.qvs, g.l[e], d.prm.qpo);
+    if (!stopped(d, g.l[e].group)) ghost_op<T, DEF>(d.nrvars, q, d.qvs, g.l[e], qpo_of(d.prm));
 }
 
 // SetGhost(level, ipass) followed by the relaxation of colour `color`, in one
@@ -863,11 +834,11 @@ __global__ __launch_bounds__(RELAX_THREADS) void k_relax_fused(Dev d, const Tile
   // every thread reaches the barriers below (they depend on the block only)
   const bool mine = in_box<ND>(i, j, k, L.P.mx, L.P.my, L.P.mz);
   if (s1 > s0 || l1 > l0) {
-    if (se < s1) ghost_op_s<T, DEF>(d.nrvars, q, d.qvs, sr, d.prm.qpo);
+    if (se < s1) ghost_op_s<T, DEF>(d.nrvars, q, d.qvs, sr, qpo_of(d.prm));
     for (int e = se + blockDim.x; e < s1; e += blockDim.x)
-      ghost_op_s<T, DEF>(d.nrvars, q, d.qvs, pre.s[e], d.prm.qpo);
+      ghost_op_s<T, DEF>(d.nrvars, q, d.qvs, pre.s[e], qpo_of(d.prm));
     for (int e = l0 + (int)threadIdx.x; e < l1; e += blockDim.x)
-      ghost_op<T, DEF>(d.nrvars, q, d.qvs, pre.l[e], d.prm.qpo);
+      ghost_op<T, DEF>(d.nrvars, q, d.qvs, pre.l[e], qpo_of(d.prm));
     __syncthreads();
   }
   if constexpr (!STAGE) {
@@ -901,11 +872,11 @@ __global__ __launch_bounds__(RELAX_THREADS) void k_relax_fused(Dev d, const Tile
   }
   if (ps1 > ps0 || pl1 > pl0) {
     __syncthreads();
-    if (pe < ps1) ghost_op_s<T, DEF>(d.nrvars, q, d.qvs, pr, d.prm.qpo);
+    if (pe < ps1) ghost_op_s<T, DEF>(d.nrvars, q, d.qvs, pr, qpo_of(d.prm));
     for (int e = pe + blockDim.x; e < ps1; e += blockDim.x)
-      ghost_op_s<T, DEF>(d.nrvars, q, d.qvs, post.s[e], d.prm.qpo);
+      ghost_op_s<T, DEF>(d.nrvars, q, d.qvs, post.s[e], qpo_of(d.prm));
     for (int e = pl0 + (int)threadIdx.x; e < pl1; e += blockDim.x)
-      ghost_op<T, DEF>(d.nrvars, q, d.qvs, post.l[e], d.prm.qpo);
+      ghost_op<T, DEF>(d.nrvars, q, d.qvs, post.l[e], qpo_of(d.prm));
   }
 }
 
@@ -1004,7 +975,7 @@ __global__ __launch_bounds__(ISO_THREADS) void k_smooth_iso(Dev d, const IsoPatc
     for (int v = 0; v < 2; v++) {
       const T *b = box + v * W;
       box[v * W + dst] =
-          ghost_value<T, DEF>(kind, nflip, [&](int n) { return b[src(n)]; }, d.prm.qpo, v);
+          ghost_value<T, DEF>(kind, nflip, [&](int n) { return b[src(n)]; }, qpo_of(d.prm), v);
     }
   };
   auto apply_s = [&](const unsigned long long w) __attribute__((always_inline)) {
@@ -1077,10 +1048,10 @@ template <class T, bool DEF>
 __global__ __launch_bounds__(256) void k_ghost(Dev d, GhostList g) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e < g.ns) {
-    if (!stopped(d, g.s[e].group)) ghost_op_s<T, DEF>(d.nrvars, fields<T>(d).q, d.qvs, g.s[e], d.prm.qpo);
+    if (!stopped(d, g.s[e].group)) ghost_op_s<T, DEF>(d.nrvars, fields<T>(d).q, d.qvs, g.s[e], qpo_of(d.prm));
   } else if (e - g.ns < g.nl) {
     const GhostOp &u = g.l[e - g.ns];
-    if (!stopped(d, u.group)) ghost_op<T, DEF>(d.nrvars, fields<T>(d).q, d.qvs, u, d.prm.qpo);
+    if (!stopped(d, u.group)) ghost_op<T, DEF>(d.nrvars, fields<T>(d).q, d.qvs, u, qpo_of(d.prm));
   }
 }
 
@@ -1710,22 +1681,23 @@ __device__ __forceinline__ void tail_ghost_load(const TailLevel &t, TailGhost &g
 }
 // SetGhost on a tail level: patch-local copies with "2 - x" maps and
 // quasi-periodic offsets (the ghost_op kind 0 update).
-template <class T, bool DEF>
+template <class T, bool DEF, bool QPO>
 __device__ __forceinline__ void tail_ghost(int nrvars, const Lvl<T> &L, const TailPass &g,
-                                           const double *qpo) {
+                                           Qpo qpo) {
   const long long vs = L.P.qvs;
 #pragma unroll
   for (int r = 0; r < TAIL_GR; r++) {
     if ((int)(threadIdx.x + r * blockDim.x) < g.n) {
       const int src = g.srcf[r] & 0x03ffffff, fb = (int)((unsigned)g.srcf[r] >> 26);
       for (int v = 0; v < nrvars; v++) {
-        const T val = ghost_value<T, DEF>(0, fb, [&](int) { return L.q[src + v * vs]; }, qpo, v);
+        auto sv = [&](int) { return L.q[src + v * vs]; };
+        const T val = ghost_value<T, DEF, decltype(sv), QPO>(0, fb, sv, qpo, v);
         L.q[g.dst[r] + v * vs] = val;
       }
     }
   }
   for (int e = threadIdx.x + TAIL_GR * blockDim.x; e < g.n; e += blockDim.x)
-    ghost_op<T, DEF>(nrvars, L.q, vs, g.ops[e], qpo);
+    ghost_op<T, DEF, QPO>(nrvars, L.q, vs, g.ops[e], qpo);
   __syncthreads();
 }
 
@@ -1767,7 +1739,7 @@ __device__ __forceinline__ T tail_load(const T *p, long long i, bool lds) {
 // nsmooth red-black passes on a tail level. Threads walk the cells of each
 // colour; the first TAIL_FR cells per thread and colour keep their f (in LDS
 // when f_lds) in registers for the whole visit.
-template <int ND, class T, bool DEF>
+template <int ND, class T, bool DEF, bool QPO>
 __device__ void tail_relax(const Problem &prm, int nrvars, const Lvl<T> &L, bool f_lds,
                            const TailGhost &g, int nsmooth) {
   const DPatch &P = L.P;
@@ -1802,16 +1774,16 @@ __device__ void tail_relax(const Problem &prm, int nrvars, const Lvl<T> &L, bool
   };
   for (int pass = 0; pass < nsmooth; pass++) {
     half(0);
-    tail_ghost<T, DEF>(nrvars, L, g.p1, prm.qpo);
+    tail_ghost<T, DEF, QPO>(nrvars, L, g.p1, qpo_of(prm));
     half(1);
-    tail_ghost<T, DEF>(nrvars, L, g.p2, prm.qpo);
+    tail_ghost<T, DEF, QPO>(nrvars, L, g.p2, qpo_of(prm));
   }
 }
 
 // The tail's visit of level C = lv[l] on the way down: smoothing, then
 // RestrictSolution, SetGhost(parent, 0), GetCoarseGhostPoints and
 // CoarseLoadingFunction into the parent PA = lv[l - 1].
-template <int ND, class T, bool DEF, bool LC, bool LP>
+template <int ND, class T, bool DEF, bool QPO, bool LC, bool LP>
 __device__ void tail_down(const Dev &d, const TailLevel *__restrict__ lv, int l, int set,
                           int nsmooth, const TailGhost &gc, const TailPass &gp) {
   const int nv = d.nrvars;
@@ -1819,11 +1791,11 @@ __device__ void tail_down(const Dev &d, const TailLevel *__restrict__ lv, int l,
   Lvl<T> C = tail_ref<T, LC>(d, lv[l], set), PA = tail_ref<T, LP>(d, lv[l - 1], set);
   const DPatch &P = C.P;
   int cz = ND == 3 ? P.cmz : 1;
-  tail_relax<ND, T, DEF>(prm, nv, C, LC && lv[l].lds[set][2] >= 0, gc, nsmooth);
+  tail_relax<ND, T, DEF, QPO>(prm, nv, C, LC && lv[l].lds[set][2] >= 0, gc, nsmooth);
   for_cells<ND>(P.cmx, P.cmy, cz, 1,
                 [&](int i, int j, int k) { restrict_cell<ND, T>(nv, C, PA, i, j, k); });
   __syncthreads();
-  tail_ghost<T, DEF>(nv, PA, gp, d.prm.qpo);
+  tail_ghost<T, DEF, QPO>(nv, PA, gp, qpo_of(d.prm));
   for_cells<ND>(P.cmx + 2, P.cmy + 2, ND == 3 ? P.cmz + 2 : 1, 0,
                 [&](int i, int j, int k) { coarse_ghost_cell<ND, T>(nv, C, PA, i, j, k); });
   __syncthreads();
@@ -1833,7 +1805,7 @@ __device__ void tail_down(const Dev &d, const TailLevel *__restrict__ lv, int l,
 }
 
 // On the way up: CorrectFine from the parent, SetGhost(C, 0), smoothing.
-template <int ND, class T, bool DEF, bool LC, bool LP>
+template <int ND, class T, bool DEF, bool QPO, bool LC, bool LP>
 __device__ void tail_up(const Dev &d, const TailLevel *__restrict__ lv, int l, int set,
                         int nsmooth, const TailGhost &gc) {
   const int nv = d.nrvars;
@@ -1842,22 +1814,24 @@ __device__ void tail_up(const Dev &d, const TailLevel *__restrict__ lv, int l, i
   for_cells<ND>(P.cmx + 1, P.cmy + 1, ND == 3 ? P.cmz + 1 : 1, 0,
                 [&](int m, int n, int o) { correct_block<ND, T>(nv, C, PA, m, n, o); });
   __syncthreads();
-  tail_ghost<T, DEF>(nv, C, gc.p0, d.prm.qpo);
-  tail_relax<ND, T, DEF>(d.prm, nv, C, LC && lv[l].lds[set][2] >= 0, gc, nsmooth);
+  tail_ghost<T, DEF, QPO>(nv, C, gc.p0, qpo_of(d.prm));
+  tail_relax<ND, T, DEF, QPO>(d.prm, nv, C, LC && lv[l].lds[set][2] >= 0, gc, nsmooth);
 }
 
-template <int ND, class T, bool DEF, bool LB>
+template <int ND, class T, bool DEF, bool QPO, bool LB>
 __device__ void tail_bottom(const Dev &d, const TailLevel *__restrict__ lv, int set, int nsmooth,
                             const TailGhost &gb) {
   Lvl<T> B = tail_ref<T, LB>(d, lv[0], set);
-  tail_relax<ND, T, DEF>(d.prm, d.nrvars, B, LB && lv[0].lds[set][2] >= 0, gb, nsmooth);
+  tail_relax<ND, T, DEF, QPO>(d.prm, d.nrvars, B, LB && lv[0].lds[set][2] >= 0, gb, nsmooth);
 }
 
 // AFASVcycle(top) (afasroutines.f90:488-521) on levels 0..nlev-1 of the
 // single-patch chain. Levels holding their q in LDS form a prefix of the
 // chain (coarsest first), so a visit pairs LDS/LDS, global/LDS or
 // global/global child and parent.
-template <int ND, class T, bool DEF>
+// QPO: the chain's SetGhost programs hold periodic offsets (without them
+// the offset code is left out, which keeps the kernel within its registers).
+template <int ND, class T, bool DEF, bool QPO>
 __global__ __launch_bounds__(1024) void k_tail_vcycle(Dev d, const TailLevel *__restrict__ lv,
                                                        int nlev, int nsmooth, int lds,
                                                        const TailCopy *cin, int nin,
@@ -1872,21 +1846,21 @@ __global__ __launch_bounds__(1024) void k_tail_vcycle(Dev d, const TailLevel *__
   if (lds) tail_stage<T, true>(cin, nin);
   for (int l = nlev - 1; l >= 1; l--) {
     tail_ghost_load(lv[l - 1], gnext);
-    if (in_lds(l)) tail_down<ND, T, DEF, true, true>(d, lv, l, set, nsmooth, gcur, gnext.p0);
+    if (in_lds(l)) tail_down<ND, T, DEF, QPO, true, true>(d, lv, l, set, nsmooth, gcur, gnext.p0);
     else if (in_lds(l - 1))
-      tail_down<ND, T, DEF, false, true>(d, lv, l, set, nsmooth, gcur, gnext.p0);
-    else tail_down<ND, T, DEF, false, false>(d, lv, l, set, nsmooth, gcur, gnext.p0);
+      tail_down<ND, T, DEF, QPO, false, true>(d, lv, l, set, nsmooth, gcur, gnext.p0);
+    else tail_down<ND, T, DEF, QPO, false, false>(d, lv, l, set, nsmooth, gcur, gnext.p0);
     gcur = gnext;
   }
   if (nlev > 1) tail_ghost_load(lv[1], gnext);
-  if (in_lds(0)) tail_bottom<ND, T, DEF, true>(d, lv, set, nsmooth, gcur);
-  else tail_bottom<ND, T, DEF, false>(d, lv, set, nsmooth, gcur);
+  if (in_lds(0)) tail_bottom<ND, T, DEF, QPO, true>(d, lv, set, nsmooth, gcur);
+  else tail_bottom<ND, T, DEF, QPO, false>(d, lv, set, nsmooth, gcur);
   for (int l = 1; l < nlev; l++) {
     gcur = gnext;
     if (l + 1 < nlev) tail_ghost_load(lv[l + 1], gnext);
-    if (in_lds(l)) tail_up<ND, T, DEF, true, true>(d, lv, l, set, nsmooth, gcur);
-    else if (in_lds(l - 1)) tail_up<ND, T, DEF, false, true>(d, lv, l, set, nsmooth, gcur);
-    else tail_up<ND, T, DEF, false, false>(d, lv, l, set, nsmooth, gcur);
+    if (in_lds(l)) tail_up<ND, T, DEF, QPO, true, true>(d, lv, l, set, nsmooth, gcur);
+    else if (in_lds(l - 1)) tail_up<ND, T, DEF, QPO, false, true>(d, lv, l, set, nsmooth, gcur);
+    else tail_up<ND, T, DEF, QPO, false, false>(d, lv, l, set, nsmooth, gcur);
   }
   if (lds) tail_stage<T, false>(cout, nout);
 }
@@ -1908,7 +1882,7 @@ __global__ __launch_bounds__(1024) void k_tail_filldown(Dev d, const TailLevel *
       filldown_cell<ND, double>(nv, C, PA, i, j, k, field);
     });
     __syncthreads();
-    if (field == 2) tail_ghost<double, false>(nv, PA, gp, d.prm.qpo);
+    if (field == 2) tail_ghost<double, false, true>(nv, PA, gp, qpo_of(d.prm));
   }
 }
 
@@ -2244,24 +2218,33 @@ hipError_t launch_copy(double *dst, const double *src, long long n, hipStream_t 
 
 hipError_t launch_tail_vcycle(int nd, int prec, const Dev &d, const TailLevel *lv, int nlev,
                               int nsmooth, int threads, size_t lds, const TailCopy *cin,
-                              int nin, const TailCopy *cout, int nout, hipStream_t s) {
+                              int nin, const TailCopy *cout, int nout, bool qpo, hipStream_t s) {
   if (nlev <= 0) return hipSuccess;
   if (nlev > TAIL_MAXLEV) return hipErrorInvalidValue;
   int use = lds > 0;
   if (prec == PREC_SOLUTION_FP64) {
-    if (nd == 2)
-      k_tail_vcycle<2, double, false><<<1, threads, lds, s>>>(d, lv, nlev, nsmooth, use, cin, nin,
-                                                             cout, nout);
-    else
-      k_tail_vcycle<3, double, false><<<1, threads, lds, s>>>(d, lv, nlev, nsmooth, use, cin, nin,
-                                                             cout, nout);
+    if (nd == 2) {
+      if (qpo)
+        k_tail_vcycle<2, double, false, true><<<1, threads, lds, s>>>(d, lv, nlev, nsmooth, use,
+                                                                     cin, nin, cout, nout);
+      else
+        k_tail_vcycle<2, double, false, false><<<1, threads, lds, s>>>(d, lv, nlev, nsmooth, use,
+                                                                      cin, nin, cout, nout);
+    } else {
+      if (qpo)
+        k_tail_vcycle<3, double, false, true><<<1, threads, lds, s>>>(d, lv, nlev, nsmooth, use,
+                                                                     cin, nin, cout, nout);
+      else
+        k_tail_vcycle<3, double, false, false><<<1, threads, lds, s>>>(d, lv, nlev, nsmooth, use,
+                                                                      cin, nin, cout, nout);
+    }
   } else {
     if (nd == 2)
-      k_tail_vcycle<2, float, true><<<1, threads, lds, s>>>(d, lv, nlev, nsmooth, use, cin, nin,
-                                                           cout, nout);
+      k_tail_vcycle<2, float, true, false><<<1, threads, lds, s>>>(d, lv, nlev, nsmooth, use, cin,
+                                                                  nin, cout, nout);
     else
-      k_tail_vcycle<3, float, true><<<1, threads, lds, s>>>(d, lv, nlev, nsmooth, use, cin, nin,
-                                                           cout, nout);
+      k_tail_vcycle<3, float, true, false><<<1, threads, lds, s>>>(d, lv, nlev, nsmooth, use, cin,
+                                                                  nin, cout, nout);
   }
   return hipGetLastError();
 }
@@ -2314,8 +2297,12 @@ hipError_t iso_allow_lds(size_t lds) {
 
 hipError_t tail_allow_lds(size_t lds) {
   const void *fns[] = {
-      (const void *)k_tail_vcycle<2, double, false>, (const void *)k_tail_vcycle<3, double, false>,
-      (const void *)k_tail_vcycle<2, float, true>, (const void *)k_tail_vcycle<3, float, true>};
+      (const void *)k_tail_vcycle<2, double, false, true>,
+      (const void *)k_tail_vcycle<3, double, false, true>,
+      (const void *)k_tail_vcycle<2, double, false, false>,
+      (const void *)k_tail_vcycle<3, double, false, false>,
+      (const void *)k_tail_vcycle<2, float, true, false>,
+      (const void *)k_tail_vcycle<3, float, true, false>};
   for (const void *f : fns) {
     hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;

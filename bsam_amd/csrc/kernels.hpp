@@ -84,10 +84,11 @@ hipError_t launch_finalize(const double *partial, int n, double count, double to
 hipError_t launch_copy(double *dst, const double *src, long long n, hipStream_t s);
 // Tail kernels: one workgroup of `threads` over levels 0..nlev-1 of the
 // single-patch chain; the V-cycle with `lds` bytes of dynamic LDS
-// (TailLevel::lds; 0 keeps every array in the global pools).
+// (TailLevel::lds; 0 keeps every array in the global pools); `qpo`: some
+// level's SetGhost programs add periodic offsets.
 hipError_t launch_tail_vcycle(int nd, int prec, const Dev &d, const TailLevel *lv, int nlev,
                               int nsmooth, int threads, size_t lds, const TailCopy *cin,
-                              int nin, const TailCopy *cout, int nout, hipStream_t s);
+                              int nin, const TailCopy *cout, int nout, bool qpo, hipStream_t s);
 // Capacity of the tail's level table, and the static LDS the tail kernel takes.
 int tail_max_levels();
 size_t tail_static_lds();
