@@ -320,12 +320,21 @@ __device__ __forceinline__ void ghost_op_s(int nrvars, T *q, long long vs, const
 // R(q) and the parent's q under the child takes the same value.
 template <int ND, class T>
 __device__ __forceinline__ void restrict_cell(int nrvars, const Lvl<T> &C, const Lvl<T> &PA,
-                                              int i, int j, int k) {
+                                              int i, int j, int k, bool both_first = false) {
   const DPatch &P = C.P;
   View<T> a = qview(C), cq = qcview(C), pq = qview(PA);
   long long pc = pq.at(P.mb[0][0] + i - 1, P.mb[1][0] + j - 1,
                        ND == 3 ? P.mb[2][0] + k - 1 : 0);
   long long cc = cq.at(i, j, k);
+  if (both_first && nrvars == 2) {
+    // both variables' children loaded before either store
+    const T r0 = restrict_view<ND, T>(a, 0, i, j, k), r1 = restrict_view<ND, T>(a, 1, i, j, k);
+    cq.p[cc] = r0;
+    pq.p[pc] = r0;
+    cq.p[cc + cq.vs] = r1;
+    pq.p[pc + pq.vs] = r1;
+    return;
+  }
   for (int v = 0; v < nrvars; v++) {
     T r = restrict_view<ND, T>(a, v, i, j, k);
     cq.p[cc + v * cq.vs] = r;
@@ -337,10 +346,27 @@ __device__ __forceinline__ void restrict_cell(int nrvars, const Lvl<T> &C, const
 // for X = f (field 1) or q (field 2).
 template <int ND, class T>
 __device__ __forceinline__ void filldown_cell(int nrvars, const Lvl<T> &C, const Lvl<T> &PA,
-                                              int i, int j, int k, int field) {
+                                              int i, int j, int k, int field,
+                                              bool both_first = false) {
   const DPatch &P = C.P;
   int pi = P.mb[0][0] + i - 1, pj = P.mb[1][0] + j - 1,
       pk = ND == 3 ? P.mb[2][0] + k - 1 : 1;
+  if (both_first && nrvars == 2) {
+    // both variables' children loaded before either store
+    if (field == 1) {
+      const long long po = fat(PA.P, pi, pj, pk);
+      const T r0 = restrict_f<ND, T>(C.f, P, i, j, k), r1 = restrict_f<ND, T>(C.f + P.fvs, P, i, j, k);
+      PA.f[po] = r0;
+      PA.f[po + PA.P.fvs] = r1;
+    } else {
+      View<T> a = qview(C), pq = qview(PA);
+      const long long pc = pq.at(pi, pj, ND == 3 ? pk : 0);
+      const T r0 = restrict_view<ND, T>(a, 0, i, j, k), r1 = restrict_view<ND, T>(a, 1, i, j, k);
+      pq.p[pc] = r0;
+      pq.p[pc + pq.vs] = r1;
+    }
+    return;
+  }
   if (field == 1) {
     for (int v = 0; v < nrvars; v++)
       PA.f[fat(PA.P, pi, pj, pk) + v * PA.P.fvs] =
@@ -604,6 +630,89 @@ __device__ __forceinline__ void correct_block(int nrvars, const Lvl<T> &C, const
           long long q0 = a.at(i, j, k);
           qv[q0] = qv[q0] + p;
         }
+  }
+}
+
+// correct_block with every variable's loads issued before any store (one
+// round of loads per thread instead of one per variable), for the level
+// kernel; the tail keeps correct_block, which needs fewer registers.
+template <int ND, class T>
+__device__ __forceinline__ void correct_block_all(int nrvars, const Lvl<T> &C, const Lvl<T> &PA,
+                                              int m, int n, int o) {
+  const DPatch &P = C.P;
+  View<T> a = qview(C), cq = qcview(C), pq = qview(PA);
+  int o0 = P.mb[0][0] - 1, o1 = P.mb[1][0] - 1, o2 = ND == 3 ? P.mb[2][0] - 1 : 0;
+  constexpr int NZ = ND == 3 ? 2 : 1, NF = 4 * NZ;
+  // the fine cells' offsets (-1 past the box) and the coarse corrections of
+  // every variable, all loaded before any store
+  long long fo[NF];
+#pragma unroll
+  for (int fz = 0; fz < NZ; fz++)
+#pragma unroll
+    for (int fy = 0; fy < 2; fy++)
+#pragma unroll
+      for (int fx = 0; fx < 2; fx++) {
+        int i = 2 * m + fx, j = 2 * n + fy, k = ND == 3 ? 2 * o + fz : 0;
+        const bool in = !(i > P.mx + 1 || j > P.my + 1 || (ND == 3 && k > P.mz + 1));
+        fo[fx + 2 * (fy + 2 * fz)] = in ? a.at(i, j, k) : -1;
+      }
+  auto one = [&](int v, T (&cr)[2][2][NZ], T (&qf)[NF]) {
+    const T *cp = cq.p + v * cq.vs, *pp = pq.p + v * pq.vs;
+    const T *qv = a.p + v * a.vs;
+#pragma unroll
+    for (int dz = 0; dz < NZ; dz++)
+#pragma unroll
+      for (int dy = 0; dy < 2; dy++)
+#pragma unroll
+        for (int dx = 0; dx < 2; dx++) {
+          int ci = m + dx, cj = n + dy, ck = ND == 3 ? o + dz : 0;
+          cr[dx][dy][dz] = pp[pq.at(o0 + ci, o1 + cj, ND == 3 ? o2 + ck : 0)] - cp[cq.at(ci, cj, ck)];
+        }
+#pragma unroll
+    for (int e = 0; e < NF; e++) qf[e] = fo[e] >= 0 ? qv[fo[e]] : (T)0;
+  };
+  auto put = [&](int v, const T (&cr)[2][2][NZ], const T (&qf)[NF]) {
+    T *qv = a.p + v * a.vs;
+#pragma unroll
+    for (int fz = 0; fz < NZ; fz++)
+#pragma unroll
+      for (int fy = 0; fy < 2; fy++)
+#pragma unroll
+        for (int fx = 0; fx < 2; fx++) {
+          const int e = fx + 2 * (fy + 2 * fz);
+          if (fo[e] < 0) continue;
+          bool ex = fx == 0, ey = fy == 0, ez = fz == 0;
+          auto xpass = [&](int dy, int dz) {
+            T a0 = cr[0][dy][dz], a1 = cr[1][dy][dz];
+            return ex ? (T)3 * a0 + a1 : a0 + (T)3 * a1;
+          };
+          T p;
+          if constexpr (ND == 2) {
+            T b0 = xpass(0, 0), b1 = xpass(1, 0);
+            p = ey ? ((T)3 * b0 + b1) / (T)16 : (b0 + (T)3 * b1) / (T)16;
+          } else {
+            auto ypass = [&](int dz) {
+              T b0 = xpass(0, dz), b1 = xpass(1, dz);
+              return ey ? (T)3 * b0 + b1 : b0 + (T)3 * b1;
+            };
+            T c0 = ypass(0), c1 = ypass(1);
+            p = ez ? ((T)3 * c0 + c1) / (T)64 : (c0 + (T)3 * c1) / (T)64;
+          }
+          qv[fo[e]] = qf[e] + p;
+        }
+  };
+  if (nrvars == 2) {
+    T cr0[2][2][NZ], cr1[2][2][NZ], q0[NF], q1[NF];
+    one(0, cr0, q0);
+    one(1, cr1, q1);
+    put(0, cr0, q0);
+    put(1, cr1, q1);
+    return;
+  }
+  for (int v = 0; v < nrvars; v++) {
+    T cr[2][2][NZ], qf[NF];
+    one(v, cr, qf);
+    put(v, cr, qf);
   }
 }
 
@@ -1065,7 +1174,7 @@ __global__ __launch_bounds__(256) void k_restrict(Dev d, const Tile *tiles, int 
     int i, j, k;
     tile_cell<ND>(d, t, i, j, k);
     if (in_box<ND>(i, j, k, C.P.cmx, C.P.cmy, C.P.cmz))
-      restrict_cell<ND, T>(d.nrvars, C, level_ref<T>(d, C.P.parent), i, j, k);
+      restrict_cell<ND, T>(d.nrvars, C, level_ref<T>(d, C.P.parent), i, j, k, true);
   }
   // the riding program reads no parent cell this launch writes and writes
   // fine ghost cells it does not read, so it runs after the work
@@ -1082,7 +1191,7 @@ __global__ __launch_bounds__(256) void k_filldown(Dev d, const Tile *tiles, int 
     int i, j, k;
     tile_cell<ND>(d, t, i, j, k);
     if (in_box<ND>(i, j, k, C.P.cmx, C.P.cmy, C.P.cmz))
-      filldown_cell<ND, T>(d.nrvars, C, level_ref<T>(d, C.P.parent), i, j, k, field);
+      filldown_cell<ND, T>(d.nrvars, C, level_ref<T>(d, C.P.parent), i, j, k, field, true);
   }
   ghost_alongside<T>(d, gops);   // as in k_restrict
 }
@@ -1229,7 +1338,7 @@ __global__ __launch_bounds__(256) void k_correct(Dev d, const Tile *tiles, int n
     Lvl<T> C = level_ref<T>(d, t.patch);
     int m, n, o;
     if (lin_cell<ND>(t, C.P.cmx + 1, C.P.cmy + 1, C.P.cmz + 1, m, n, o))
-      correct_block<ND, T>(d.nrvars, C, level_ref<T>(d, C.P.parent), m, n, o);
+      correct_block_all<ND, T>(d.nrvars, C, level_ref<T>(d, C.P.parent), m, n, o);
   }
   // the updates no tile reads, after the work (they touch nothing it uses)
   ghost_alongside<T>(d, rest);
